@@ -1,0 +1,184 @@
+/*
+ * dcpse_hip.h - C ABI of the MI355X (gfx950) DC-PSE operator library.
+ *
+ * One shared library (libdcpse_hip.so) implements the DC-PSE hot path of
+ * openfpm_numerics: neighbour-support construction, FP64 kernel assembly and
+ * CSR operator application (forward and transposed). Host code in any
+ * language binds these entry points; no C++ or torch types cross the ABI.
+ *
+ * Conventions
+ *  - Positions are structure-of-arrays doubles: component d of particle i is
+ *    pos[d * n + i] (n = that set's particle count).
+ *  - Every pointer documented "device" is caller-allocated device memory; the
+ *    library never frees caller memory. Pointers documented "host" are read
+ *    before the call returns.
+ *  - All device work is ordered on the context's stream and asynchronous
+ *    unless stated otherwise; dcpse_sync() waits for it.
+ *  - Errors: every entry point returns a dcpse_status; dcpse_last_error()
+ *    gives the message. Per-particle assembly failures go to info[].
+ *  - One context per host thread.
+ *
+ * Reference interfaces replaced (paths relative to the openfpm_numerics root):
+ *  dcpse_support_count/fill  src/DCPSE/SupportBuilder.cuh:65-107
+ *                            (SupportBuilderGPU::getSupport) and the
+ *                            VL_NON_SYMMETRIC|VL_SKIP_REF_PART VerletList
+ *                            used by every Dcpse test, e.g.
+ *                            src/DCPSE/tests/Dcpse_unit_tests.cpp:97
+ *  dcpse_assemble            src/DCPSE/Dcpse.hpp:690-809
+ *                            (Dcpse::initializeStaticSize + computeKernel)
+ *                            src/DCPSE/Dcpse.cuh:563-717 (Dcpse_gpu assembly)
+ *  dcpse_apply_f32/f64       src/DCPSE/Dcpse.hpp:467-499 and :560-659
+ *                            (Dcpse::computeDifferentialOperator) as driven by
+ *                            src/DCPSE/DCPSE_op/DCPSE_op.hpp:64-67 (VECT_DCPSE)
+ *                            and :389-401 (VECT_DCPSE_V_SUM, Laplacian)
+ *  dcpse_transpose_*         the transpose of the operator whose rows
+ *  dcpse_apply_transpose_*   src/DCPSE/DCPSE_op/DCPSE_op.hpp:69-83 (value_nz)
+ *                            exports to src/DCPSE/DCPSE_op/DCPSE_Solver.hpp:978-1036
+ *  dcpse_basis, dcpse_rhs    src/DCPSE/MonomialBasis.hpp:115-156,
+ *                            src/DCPSE/DcpseRhs.hpp:27-60
+ */
+#ifndef DCPSE_HIP_H
+#define DCPSE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dcpse_ctx dcpse_ctx;
+typedef int32_t dcpse_status;
+
+#define DCPSE_OK 0
+#define DCPSE_ERR_INVALID 1     /* bad argument */
+#define DCPSE_ERR_HIP 2         /* HIP runtime error */
+#define DCPSE_ERR_UNSUPPORTED 3 /* configuration outside the compiled kernels */
+#define DCPSE_ERR_RANGE 4       /* a size exceeds a documented limit */
+
+/* per-particle assembly status written to info[] */
+#define DCPSE_INFO_OK 0
+#define DCPSE_INFO_TOO_FEW 1   /* fewer neighbours than monomials (Vandermonde.hpp:109-112) */
+#define DCPSE_INFO_NOT_SPD 2   /* non-positive pivot in the FP64 Cholesky of A */
+#define DCPSE_INFO_EPS_ZERO 3  /* all neighbours coincide with p (Vandermonde.hpp:80) */
+#define DCPSE_INFO_NONFINITE 4 /* a kernel weight is not finite */
+
+/* dcpse_assemble flags */
+#define DCPSE_ASM_REFINE 1u /* one step of FP64 iterative refinement of the solve */
+
+/* maximum neighbours per row handled by the support builder */
+#define DCPSE_MAX_ROW 1024
+
+/* ---- context ---------------------------------------------------------- */
+int32_t dcpse_version(void);
+dcpse_status dcpse_ctx_create(dcpse_ctx** ctx, int32_t device);
+dcpse_status dcpse_ctx_destroy(dcpse_ctx* ctx);
+/* Orders all further work of ctx on hip_stream, a caller-owned hipStream_t
+ * (NULL = the device's default stream, which is also the initial binding). */
+dcpse_status dcpse_ctx_set_stream(dcpse_ctx* ctx, void* hip_stream);
+void* dcpse_ctx_get_stream(dcpse_ctx* ctx);
+const char* dcpse_last_error(const dcpse_ctx* ctx);
+dcpse_status dcpse_sync(dcpse_ctx* ctx);
+/* Apply-kernel launch shape: lanes cooperating on one row (4..64, power of
+ * two; 0 = automatic) and threads per block (64..1024). */
+dcpse_status dcpse_set_apply_config(dcpse_ctx* ctx, int32_t group_lanes, int32_t block_threads);
+/* Assembly launch shape: wavefronts per block (must be 1) and particles each
+ * wavefront assembles in sequence (0 = automatic). */
+dcpse_status dcpse_set_assembly_config(dcpse_ctx* ctx, int32_t waves_per_block, int32_t particles_per_wave);
+
+/* ---- host-only helpers (no device work) -------------------------------- */
+/* Number of monomials m for signature beta[dim] and convergence order. */
+int32_t dcpse_basis_size(int32_t dim, const uint32_t* beta, uint32_t order);
+/* Writes m*dim exponents (host) in the reference enumeration order; returns
+ * m, or -m if cap < m. */
+int32_t dcpse_basis(int32_t dim, const uint32_t* beta, uint32_t order, uint32_t* exps, int32_t cap);
+/* Writes the RHS b (host, m doubles); returns m, or -m if cap < m. */
+int32_t dcpse_rhs(int32_t dim, const uint32_t* beta, uint32_t order, double* b, int32_t cap);
+
+/* ---- neighbour support (Verlet CSR) ------------------------------------ */
+/* Row p lists, in ascending index, every support particle q with
+ * |x_p - x_q| < rcut_p (q != p when pos_dom == NULL, i.e. the domain set is
+ * the support set). rcut is a device array of 1 value (uniform) or n_dom
+ * values (rcut_per_particle != 0). box_lo/box_hi: host, dim doubles; a
+ * periodic dimension (bit d of periodic_mask) uses minimum-image
+ * displacements and needs at least 3 cells of edge >= max rcut.
+ * count: writes row_ptr[0..n_dom] (device int64) and *nnz (host; this call
+ * synchronises the stream). fill: writes col[nnz] (device int32). */
+dcpse_status dcpse_support_count(dcpse_ctx* ctx, int32_t dim,
+                                 const double* pos_sup, int64_t n_sup,
+                                 const double* pos_dom, int64_t n_dom,
+                                 const double* rcut, int32_t rcut_per_particle,
+                                 const double* box_lo, const double* box_hi, uint32_t periodic_mask,
+                                 int64_t* row_ptr, int64_t* nnz);
+dcpse_status dcpse_support_fill(dcpse_ctx* ctx, int32_t dim,
+                                const double* pos_sup, int64_t n_sup,
+                                const double* pos_dom, int64_t n_dom,
+                                const double* rcut, int32_t rcut_per_particle,
+                                const double* box_lo, const double* box_hi, uint32_t periodic_mask,
+                                const int64_t* row_ptr, int32_t* col);
+
+/* ---- kernel assembly (FP64) ------------------------------------------- */
+/* Assembles sum_k coeffs[k] * D^{beta_k} for n_sig signatures that share one
+ * monomial basis (same |beta|; e.g. the dim terms of the Laplacian), order
+ * `order`, eps_p = mean_q |x_pq|_1 / h_over_eps (the reference uses 0.9).
+ * betas/coeffs: host arrays (n_sig*dim, n_sig). Outputs (device, any may be
+ * NULL): eps[n_dom], eps_inv_pow[n_dom] = eps^-|beta|, K64[nnz] raw kernel
+ * (the reference's calcKernels), W32[nnz] and W64[nnz] = eps^-|beta| * K,
+ * info[n_dom]. pos_dom == NULL means the domain set is the support set. */
+dcpse_status dcpse_assemble(dcpse_ctx* ctx, int32_t dim,
+                            const double* pos_sup, int64_t n_sup,
+                            const double* pos_dom, int64_t n_dom,
+                            const double* box_lo, const double* box_hi, uint32_t periodic_mask,
+                            const int64_t* row_ptr, const int32_t* col,
+                            int32_t n_sig, const uint32_t* betas, const double* coeffs,
+                            uint32_t order, double h_over_eps, uint32_t flags,
+                            double* eps, double* eps_inv_pow, double* K64,
+                            float* W32, double* W64, int32_t* info);
+
+/* ---- operator application --------------------------------------------- */
+/* For n_ops operators sharing one support, each with pre-scaled weights
+ * W[o] (host array of device pointers):
+ *   out[o][b*ld_out + p] = alpha * sum_j W[o][j] * (f[b*ld_f + col_j] - f_self[b*ld_self + p])
+ *                          + beta * out[o][b*ld_out + p]
+ * for rows p < n_rows and fields b < n_fields. f_self == NULL means f (the
+ * domain set is the support set). beta == 0 never reads out. */
+dcpse_status dcpse_apply_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int32_t* col,
+                             int32_t n_ops, const float* const* W,
+                             const float* f, int64_t ld_f, int32_t n_fields,
+                             const float* f_self, int64_t ld_self,
+                             float* const* out, int64_t ld_out, float alpha, float beta);
+dcpse_status dcpse_apply_f64(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int32_t* col,
+                             int32_t n_ops, const double* const* W,
+                             const double* f, int64_t ld_f, int32_t n_fields,
+                             const double* f_self, int64_t ld_self,
+                             double* const* out, int64_t ld_out, double alpha, double beta);
+
+/* ---- transposed operator (adjoint), gather only, no float atomics ------ */
+/* The forward operator is A = W - diag(rowsum W) (difference form). Its
+ * transpose is stored as a CSR over the n_cols support particles:
+ * rowT_ptr[n_cols+1], colT[nnz] (ascending), WT[nnz], plus diag[n_rows] =
+ * -rowsum(W) (accumulated in FP64). Needs n_rows == n_cols (same set).
+ * count writes rowT_ptr (device); fill writes colT, WT, diag. */
+dcpse_status dcpse_transpose_count(dcpse_ctx* ctx, int64_t n_rows, int64_t n_cols,
+                                   const int64_t* row_ptr, const int32_t* col, int64_t* rowT_ptr);
+dcpse_status dcpse_transpose_fill_f32(dcpse_ctx* ctx, int64_t n_rows, int64_t n_cols,
+                                      const int64_t* row_ptr, const int32_t* col, const float* W,
+                                      const int64_t* rowT_ptr, int32_t* colT, float* WT, float* diag);
+dcpse_status dcpse_transpose_fill_f64(dcpse_ctx* ctx, int64_t n_rows, int64_t n_cols,
+                                      const int64_t* row_ptr, const int32_t* col, const double* W,
+                                      const int64_t* rowT_ptr, int32_t* colT, double* WT, double* diag);
+/* out[b*ld_out + q] = alpha * (sum_j WT[j] g[b*ld_g + colT_j] + diag[q] g[b*ld_g + q])
+ *                     + beta * out[b*ld_out + q] */
+dcpse_status dcpse_apply_transpose_f32(dcpse_ctx* ctx, int64_t n_cols, const int64_t* rowT_ptr,
+                                       const int32_t* colT, const float* WT, const float* diag,
+                                       const float* g, int64_t ld_g, int32_t n_fields,
+                                       float* out, int64_t ld_out, float alpha, float beta);
+dcpse_status dcpse_apply_transpose_f64(dcpse_ctx* ctx, int64_t n_cols, const int64_t* rowT_ptr,
+                                       const int32_t* colT, const double* WT, const double* diag,
+                                       const double* g, int64_t ld_g, int32_t n_fields,
+                                       double* out, int64_t ld_out, double alpha, double beta);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DCPSE_HIP_H */

@@ -1,0 +1,485 @@
+// DC-PSE kernel assembly in FP64 for gfx950: one wavefront per particle.
+//
+// Restates src/DCPSE/Dcpse.hpp:690-809 (Dcpse::initializeStaticSize +
+// computeKernel) with an MI355X-first structure:
+//   * eps_p = mean_q |x_pq|_1 / h_over_eps       (Vandermonde.hpp:63-81)
+//   * the moment matrix A = B^T B with B = E V (Dcpse.hpp:755-757) is built
+//     from its distinct entries, the weighted moments
+//        M(gamma) = sum_q w_q z_q^gamma,  z = x_pq / eps,  w = exp(-|z|^2) = E_qq^2,
+//     so A_ij = M(alpha_i + alpha_j). Neighbours are staged 32 at a time as
+//     per-dimension power tables in LDS; each lane owns one or more moments.
+//   * A is factored by an in-register FP64 Cholesky (lane i holds row i,
+//     broadcasts by v_readlane) instead of the reference's
+//     colPivHouseholderQr (Dcpse.hpp:766); an optional FP64 iterative
+//     refinement step recomputes the residual against the moments.
+//   * epilogue: lanes own neighbours again and write
+//        K_pq = w_q * sum_i a_i z_q^alpha_i              (Dcpse.hpp:798-809)
+//     coalesced, together with the eps^-|beta| pre-scaled FP64/FP32 weights.
+// There is no dense n x n E matrix and no per-particle heap traffic.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "dcpse_common.hpp"
+#include "dcpse_internal.hpp"
+
+namespace dcpse {
+
+struct AsmArgs {
+    const double* pos_sup;
+    int64_t n_sup;
+    const double* pos_dom;
+    int64_t n_dom;
+    const int64_t* row_ptr;
+    const int32_t* col;
+    double len[3];
+    unsigned periodic;
+    unsigned border;
+    double h_over_eps;
+    int refine;
+    int ppb;
+    double b[64];
+    double* eps;
+    double* eps_inv_pow;
+    double* K64;
+    float* W32;
+    double* W64;
+    int32_t* info;
+};
+
+constexpr int kChunk = 32;  // neighbours staged per LDS round
+
+template <int DIM, int L, int AMIN>
+__constant__ BasisTables<DIM, L, AMIN> kTab = BasisTables<DIM, L, AMIN>();
+
+__device__ __forceinline__ double ipow_rt(double x, unsigned e) {
+    double r = 1.0;
+    while (e) {
+        if (e & 1u) r *= x;
+        x *= x;
+        e >>= 1u;
+    }
+    return r;
+}
+
+template <int DIM, int L, int AMIN>
+__global__ __launch_bounds__(64) void k_assemble(const AsmArgs a) {
+    using S = BasisShape<DIM, L, AMIN>;
+    constexpr int M = S::M, P = S::P, GW = S::GW, MG = S::MG, G = S::G;
+    constexpr int ROUNDS = (G + kWave - 1) / kWave;
+    constexpr int PW = P + 1;
+    constexpr int TROW = DIM * PW;
+    constexpr int TAB = (kChunk * TROW > M * M) ? kChunk * TROW : M * M;
+    static_assert(M <= kWave, "basis larger than a wavefront");
+    __shared__ double tab[TAB];
+    __shared__ double mom[MG];
+    constexpr BasisTables<DIM, L, AMIN> E{};
+    const int lane = threadIdx.x;
+
+    // moments owned by this lane: table offsets and moment-grid index
+    int o0[ROUNDS], o1[ROUNDS], o2[ROUNDS], gi[ROUNDS];
+    bool gv[ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        const int g = lane + kWave * r;
+        gv[r] = g < G;
+        const int gg = gv[r] ? g : 0;
+        const int e0 = kTab<DIM, L, AMIN>.g[gg][0];
+        const int e1 = kTab<DIM, L, AMIN>.g[gg][1];
+        const int e2 = DIM == 3 ? kTab<DIM, L, AMIN>.g[gg][2] : 0;
+        o0[r] = e0;
+        o1[r] = PW + e1;
+        o2[r] = 2 * PW + e2;
+        gi[r] = e0 + GW * e1 + GW * GW * e2;
+    }
+    // basis row owned by this lane in the Cholesky phase, and its RHS entry
+    const int li = lane < M ? lane : 0;
+    const int ai0 = kTab<DIM, L, AMIN>.e[li][0];
+    const int ai1 = kTab<DIM, L, AMIN>.e[li][1];
+    const int ai2 = DIM == 3 ? kTab<DIM, L, AMIN>.e[li][2] : 0;
+    double b_lane = 0.0;
+#pragma unroll
+    for (int k = 0; k < M; ++k)
+        if (lane == k) b_lane = a.b[k];
+
+    const int64_t pbeg = (int64_t)blockIdx.x * a.ppb;
+    const int64_t pend = min(pbeg + (int64_t)a.ppb, a.n_dom);
+    for (int64_t p = pbeg; p < pend; ++p) {
+        const int64_t rs = a.row_ptr[p];
+        const int n = (int)(a.row_ptr[p + 1] - rs);
+        double xp[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) xp[d] = a.pos_dom[d * a.n_dom + p];
+        auto disp = [&](int j, double* dv) {
+            const int q = a.col[rs + j];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+                dv[d] = min_image(xp[d] - a.pos_sup[d * a.n_sup + q], a.len[d], (a.periodic >> d) & 1u);
+        };
+
+        // ---- eps -------------------------------------------------------------
+        double l1 = 0.0;
+        for (int j = lane; j < n; j += kWave) {
+            double dv[3];
+            disp(j, dv);
+            double s = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) s += fabs(dv[d]);
+            l1 += s;
+        }
+        l1 = wave_sum(l1);
+        const double eps = n > 0 ? (l1 / n) / a.h_over_eps : 0.0;
+        int status = DCPSE_INFO_OK;
+        if (n < M) status = DCPSE_INFO_TOO_FEW;
+        else if (!(eps > 0.0) || !isfinite(eps)) status = DCPSE_INFO_EPS_ZERO;
+        if (status != DCPSE_INFO_OK) {
+            for (int j = lane; j < n; j += kWave) {
+                if (a.K64) a.K64[rs + j] = 0.0;
+                if (a.W64) a.W64[rs + j] = 0.0;
+                if (a.W32) a.W32[rs + j] = 0.0f;
+            }
+            if (lane == 0) {
+                if (a.eps) a.eps[p] = 0.0;
+                if (a.eps_inv_pow) a.eps_inv_pow[p] = 0.0;
+                if (a.info) a.info[p] = status;
+            }
+            continue;
+        }
+
+        // ---- moments -----------------------------------------------------------
+        double acc[ROUNDS];
+#pragma unroll
+        for (int r = 0; r < ROUNDS; ++r) acc[r] = 0.0;
+        for (int c0 = 0; c0 < n; c0 += kChunk) {
+            const int cnt = min(kChunk, n - c0);
+            const int jl = lane & (kChunk - 1);
+            if (jl < cnt) {
+                double dv[3];
+                disp(c0 + jl, dv);
+                double z[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) z[d] = dv[d] / eps;
+                double* row = tab + jl * TROW;
+                if (lane < kChunk) {
+#pragma unroll
+                    for (int d = 0; d < DIM - 1; ++d) {
+                        double pw = 1.0;
+#pragma unroll
+                        for (int k = 0; k <= P; ++k) {
+                            row[d * PW + k] = pw;
+                            pw *= z[d];
+                        }
+                    }
+                } else {
+                    double z2 = 0.0;
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) z2 += z[d] * z[d];
+                    double pw = exp(-z2);
+#pragma unroll
+                    for (int k = 0; k <= P; ++k) {
+                        row[(DIM - 1) * PW + k] = pw;
+                        pw *= z[DIM - 1];
+                    }
+                }
+            }
+            wave_lds_sync();
+#pragma unroll 4
+            for (int j = 0; j < cnt; ++j) {
+                const double* row = tab + j * TROW;
+#pragma unroll
+                for (int r = 0; r < ROUNDS; ++r) {
+                    if (gv[r]) {
+                        if (DIM == 3) acc[r] = fma(row[o0[r]] * row[o1[r]], row[o2[r]], acc[r]);
+                        else acc[r] = fma(row[o0[r]], row[o1[r]], acc[r]);
+                    }
+                }
+            }
+            wave_lds_sync();
+        }
+#pragma unroll
+        for (int r = 0; r < ROUNDS; ++r)
+            if (gv[r]) mom[gi[r]] = acc[r];
+        wave_lds_sync();
+
+        // ---- A rows, Cholesky (lane i = row i) ------------------------------------
+        double ar[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const int idx = (ai0 + E.e[k][0]) + GW * (ai1 + E.e[k][1]) + GW * GW * (ai2 + E.e[k][2]);
+            const double v = mom[idx];
+            ar[k] = lane < M ? v : 0.0;
+        }
+        bool ok = true;
+        double idg[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            double dkk = readlane_d(ar[k], k);
+            if (!(dkk > 0.0) || !isfinite(dkk)) {
+                ok = false;
+                dkk = 1.0;
+            }
+            const double sk = sqrt(dkk);
+            const double inv = 1.0 / sk;
+            idg[k] = inv;
+            const double lik = ar[k] * inv;
+            ar[k] = (lane == k) ? sk : lik;
+#pragma unroll
+            for (int j = k + 1; j < M; ++j) {
+                const double ljk = readlane_d(lik, j);
+                ar[j] = fma(-lik, ljk, ar[j]);
+            }
+        }
+        // L rows to LDS (for column access in the backward solve)
+        double* lmat = tab;
+#pragma unroll
+        for (int k = 0; k < M; ++k)
+            if (k <= lane && lane < M) lmat[lane * M + k] = ar[k];
+        wave_lds_sync();
+
+        auto chol_solve = [&](double rhs, double* x) {
+            double r = rhs, t = 0.0;
+#pragma unroll
+            for (int k = 0; k < M; ++k) {
+                const double yk = readlane_d(r, k) * idg[k];
+                if (lane == k) t = yk;
+                r = fma(-ar[k], yk, r);
+            }
+#pragma unroll
+            for (int j = M - 1; j >= 0; --j) {
+                const double xj = readlane_d(t, j) * idg[j];
+                x[j] = xj;
+                const double ljk = lane < j ? lmat[j * M + lane] : 0.0;
+                t = fma(-ljk, xj, t);
+            }
+        };
+        double av[M];
+        chol_solve(b_lane, av);
+        if (a.refine) {
+            double rr = b_lane;
+#pragma unroll
+            for (int k = 0; k < M; ++k) {
+                const int idx = (ai0 + E.e[k][0]) + GW * (ai1 + E.e[k][1]) + GW * GW * (ai2 + E.e[k][2]);
+                const double aik = lane < M ? mom[idx] : 0.0;
+                rr = fma(-aik, av[k], rr);
+            }
+            double dx[M];
+            chol_solve(rr, dx);
+#pragma unroll
+            for (int k = 0; k < M; ++k) av[k] += dx[k];
+        }
+
+        // ---- kernel weights ----------------------------------------------------
+        const double eip = 1.0 / ipow_rt(eps, a.border);
+        bool bad = false;
+        for (int j = lane; j < n; j += kWave) {
+            double dv[3];
+            disp(j, dv);
+            double z[3] = {0.0, 0.0, 0.0};
+            double z2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                z[d] = dv[d] / eps;
+                z2 += z[d] * z[d];
+            }
+            const double w = exp(-z2);
+            double px[L], py[L], pz[L];
+            px[0] = py[0] = pz[0] = 1.0;
+#pragma unroll
+            for (int k = 1; k < L; ++k) {
+                px[k] = px[k - 1] * z[0];
+                py[k] = py[k - 1] * z[1];
+                pz[k] = pz[k - 1] * z[2];
+            }
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < M; ++k) {
+                double v = px[E.e[k][0]] * py[E.e[k][1]];
+                if (DIM == 3) v *= pz[E.e[k][2]];
+                s = fma(av[k], v, s);
+            }
+            const double K = ok ? s * w : 0.0;
+            bad |= !isfinite(K);
+            if (a.K64) a.K64[rs + j] = K;
+            if (a.W64) a.W64[rs + j] = K * eip;
+            if (a.W32) a.W32[rs + j] = (float)(K * eip);
+        }
+        const bool anybad = __ballot(bad) != 0ull;
+        if (lane == 0) {
+            if (a.eps) a.eps[p] = eps;
+            if (a.eps_inv_pow) a.eps_inv_pow[p] = eip;
+            if (a.info) a.info[p] = !ok ? DCPSE_INFO_NOT_SPD : (anybad ? DCPSE_INFO_NONFINITE : DCPSE_INFO_OK);
+        }
+        wave_lds_sync();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host-side basis rules (the device templates enumerate the same way)
+// ---------------------------------------------------------------------------
+struct HostBasis {
+    int dim = 0, L = 0, amin = 0, m = 0;
+    std::vector<uint32_t> e;
+};
+
+static HostBasis host_basis(int dim, const uint32_t* beta, uint32_t order) {
+    HostBasis B;
+    B.dim = dim;
+    uint32_t s = 0;
+    for (int d = 0; d < dim; ++d) s += beta[d];
+    B.L = (int)(s + order);
+    B.amin = (s % 2 == 0 && s != 0) ? 1 : 0;
+    long total = 1;
+    for (int d = 0; d < dim; ++d) total *= B.L;
+    for (long lin = 0; lin < total; ++lin) {
+        uint32_t k[3] = {0, 0, 0};
+        long r = lin;
+        int sum = 0;
+        for (int d = 0; d < dim; ++d) {
+            k[d] = (uint32_t)(r % B.L);
+            r /= B.L;
+            sum += (int)k[d];
+        }
+        if (sum >= B.amin && sum < B.L) {
+            for (int d = 0; d < dim; ++d) B.e.push_back(k[d]);
+            ++B.m;
+        }
+    }
+    return B;
+}
+
+// b_i = sign * (d^beta m_i)(0), sign = +1 for even |beta| (DcpseRhs.hpp:27-60):
+// nonzero only for alpha_i == beta, where it is sign * beta!.
+static void host_rhs(const HostBasis& B, const uint32_t* beta, double* b) {
+    uint32_t s = 0;
+    for (int d = 0; d < B.dim; ++d) s += beta[d];
+    const double sign = (s % 2 == 0) ? 1.0 : -1.0;
+    for (int i = 0; i < B.m; ++i) {
+        bool eq = true;
+        double fact = 1.0;
+        for (int d = 0; d < B.dim; ++d) {
+            if (B.e[i * B.dim + d] != beta[d]) eq = false;
+            for (uint32_t k = 2; k <= beta[d]; ++k) fact *= k;
+        }
+        b[i] = eq ? sign * fact : 0.0;
+    }
+}
+
+typedef void (*AsmKernel)(const AsmArgs);
+
+template <int DIM, int L, int AMIN>
+static void launch_asm(const AsmArgs& args, int64_t blocks, hipStream_t s) {
+    k_assemble<DIM, L, AMIN><<<(unsigned)blocks, kWave, 0, s>>>(args);
+}
+
+typedef void (*AsmLauncher)(const AsmArgs&, int64_t, hipStream_t);
+
+static AsmLauncher pick_launcher(int dim, int L, int amin) {
+#define DCPSE_ASM_CASE(D, LL, A) \
+    if (dim == D && L == LL && amin == A) return &launch_asm<D, LL, A>;
+    DCPSE_ASM_CASE(2, 2, 0) DCPSE_ASM_CASE(2, 2, 1)
+    DCPSE_ASM_CASE(2, 3, 0) DCPSE_ASM_CASE(2, 3, 1)
+    DCPSE_ASM_CASE(2, 4, 0) DCPSE_ASM_CASE(2, 4, 1)
+    DCPSE_ASM_CASE(2, 5, 0) DCPSE_ASM_CASE(2, 5, 1)
+    DCPSE_ASM_CASE(2, 6, 0) DCPSE_ASM_CASE(2, 6, 1)
+    DCPSE_ASM_CASE(3, 2, 0) DCPSE_ASM_CASE(3, 2, 1)
+    DCPSE_ASM_CASE(3, 3, 0) DCPSE_ASM_CASE(3, 3, 1)
+    DCPSE_ASM_CASE(3, 4, 0) DCPSE_ASM_CASE(3, 4, 1)
+    DCPSE_ASM_CASE(3, 5, 0) DCPSE_ASM_CASE(3, 5, 1)
+#undef DCPSE_ASM_CASE
+    return nullptr;
+}
+
+}  // namespace dcpse
+
+using namespace dcpse;
+
+extern "C" {
+
+int32_t dcpse_basis_size(int32_t dim, const uint32_t* beta, uint32_t order) {
+    if ((dim != 2 && dim != 3) || !beta) return -1;
+    return host_basis(dim, beta, order).m;
+}
+
+int32_t dcpse_basis(int32_t dim, const uint32_t* beta, uint32_t order, uint32_t* exps, int32_t cap) {
+    if ((dim != 2 && dim != 3) || !beta) return 0;
+    HostBasis B = host_basis(dim, beta, order);
+    if (B.m > cap || !exps) return -B.m;
+    std::copy(B.e.begin(), B.e.end(), exps);
+    return B.m;
+}
+
+int32_t dcpse_rhs(int32_t dim, const uint32_t* beta, uint32_t order, double* b, int32_t cap) {
+    if ((dim != 2 && dim != 3) || !beta) return 0;
+    HostBasis B = host_basis(dim, beta, order);
+    if (B.m > cap || !b) return -B.m;
+    host_rhs(B, beta, b);
+    return B.m;
+}
+
+dcpse_status dcpse_assemble(dcpse_ctx* ctx, int32_t dim, const double* pos_sup, int64_t n_sup,
+                            const double* pos_dom, int64_t n_dom, const double* box_lo, const double* box_hi,
+                            uint32_t periodic_mask, const int64_t* row_ptr, const int32_t* col, int32_t n_sig,
+                            const uint32_t* betas, const double* coeffs, uint32_t order, double h_over_eps,
+                            uint32_t flags, double* eps, double* eps_inv_pow, double* K64, float* W32,
+                            double* W64, int32_t* info) {
+    if (!ctx) return DCPSE_ERR_INVALID;
+    if (dim != 2 && dim != 3) return fail(ctx, DCPSE_ERR_UNSUPPORTED, "dim must be 2 or 3");
+    if (!pos_sup || !row_ptr || n_sup < 0 || n_dom < 0 || n_sig < 1 || !betas)
+        return fail(ctx, DCPSE_ERR_INVALID, "null pointer or bad size");
+    if (!(h_over_eps > 0.0)) return fail(ctx, DCPSE_ERR_INVALID, "h_over_eps must be > 0");
+    if (pos_dom == nullptr) {
+        pos_dom = pos_sup;
+        n_dom = n_sup;
+    }
+    if (periodic_mask && (!box_lo || !box_hi))
+        return fail(ctx, DCPSE_ERR_INVALID, "periodic boundaries need box_lo/box_hi");
+    DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+
+    HostBasis B0 = host_basis(dim, betas, order);
+    uint32_t border = 0;
+    for (int d = 0; d < dim; ++d) border += betas[d];
+    AsmArgs args;
+    memset(&args, 0, sizeof(args));
+    std::vector<double> bk(B0.m);
+    for (int k = 0; k < n_sig; ++k) {
+        HostBasis Bk = host_basis(dim, betas + k * dim, order);
+        uint32_t bo = 0;
+        for (int d = 0; d < dim; ++d) bo += betas[k * dim + d];
+        if (Bk.L != B0.L || Bk.amin != B0.amin || bo != border)
+            return fail(ctx, DCPSE_ERR_INVALID, "combined signatures must share |beta| (one basis)");
+        host_rhs(Bk, betas + k * dim, bk.data());
+        const double c = coeffs ? coeffs[k] : 1.0;
+        for (int i = 0; i < B0.m; ++i) args.b[i] += c * bk[i];
+    }
+    if (B0.m > 64) return fail(ctx, DCPSE_ERR_UNSUPPORTED, "basis of %d monomials exceeds 64", B0.m);
+    AsmLauncher launch = pick_launcher(dim, B0.L, B0.amin);
+    if (!launch)
+        return fail(ctx, DCPSE_ERR_UNSUPPORTED, "no compiled assembly kernel for dim=%d |beta|+order=%d", dim,
+                    B0.L);
+    args.pos_sup = pos_sup;
+    args.n_sup = n_sup;
+    args.pos_dom = pos_dom;
+    args.n_dom = n_dom;
+    args.row_ptr = row_ptr;
+    args.col = col;
+    for (int d = 0; d < 3; ++d) args.len[d] = (d < dim && box_lo && box_hi) ? box_hi[d] - box_lo[d] : 1.0;
+    args.periodic = periodic_mask;
+    args.border = border;
+    args.h_over_eps = h_over_eps;
+    args.refine = (flags & DCPSE_ASM_REFINE) ? 1 : 0;
+    args.ppb = std::max(1, ctx->asm_ppb);
+    args.eps = eps;
+    args.eps_inv_pow = eps_inv_pow;
+    args.K64 = K64;
+    args.W32 = W32;
+    args.W64 = W64;
+    args.info = info;
+    if (n_dom == 0) return DCPSE_OK;
+    const int64_t blocks = (n_dom + args.ppb - 1) / args.ppb;
+    launch(args, blocks, ctx->stream);
+    DCPSE_LAUNCH_CHECK(ctx);
+    return DCPSE_OK;
+}
+
+}  // extern "C"

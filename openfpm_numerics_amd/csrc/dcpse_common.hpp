@@ -1,0 +1,113 @@
+// Shared device helpers and compile-time DC-PSE basis tables for gfx950.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dcpse {
+
+constexpr int kWave = 64;
+
+// ---- wave-level primitives --------------------------------------------------
+
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+    const long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)x, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(x >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+__device__ __forceinline__ int lane_prefix(unsigned long long mask) {
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
+// Orders LDS traffic of the lanes of one wavefront (all lanes that share the
+// wave's private LDS slice) without a workgroup barrier.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Minimum-image displacement component (same arithmetic as the oracle).
+__device__ __forceinline__ double min_image(double v, double len, bool periodic) {
+    return periodic ? v - len * rint(v / len) : v;
+}
+
+// ---- compile-time basis / moment tables ---------------------------------------
+//
+// The monomial basis of a DC-PSE operator with signature beta and convergence
+// order r is every alpha in [0, L)^DIM with AMIN <= |alpha| < L, L = |beta| + r,
+// AMIN = 1 for even nonzero |beta| and 0 otherwise, enumerated with dimension 0
+// varying fastest (openfpm grid iteration order). The moment matrix
+// A_ij = sum_q w_q z_q^(alpha_i + alpha_j) is a function of the moments
+// M(gamma) = sum_q w_q z_q^gamma for 2*AMIN <= |gamma| <= 2(L-1).
+
+constexpr int ipow_c(int b, int e) {
+    int r = 1;
+    for (int i = 0; i < e; ++i) r *= b;
+    return r;
+}
+
+template <int DIM, int L, int AMIN>
+struct BasisShape {
+    static constexpr int count_m() {
+        int c = 0;
+        for (int lin = 0; lin < ipow_c(L, DIM); ++lin) {
+            int s = 0, r = lin;
+            for (int d = 0; d < DIM; ++d) { s += r % L; r /= L; }
+            if (s >= AMIN && s < L) ++c;
+        }
+        return c;
+    }
+    static constexpr int P = 2 * (L - 1);  // largest moment exponent
+    static constexpr int GW = P + 1;        // moment grid width per dimension
+    static constexpr int count_g() {
+        int c = 0;
+        for (int lin = 0; lin < ipow_c(GW, DIM); ++lin) {
+            int s = 0, r = lin;
+            for (int d = 0; d < DIM; ++d) { s += r % GW; r /= GW; }
+            if (s >= 2 * AMIN && s <= P) ++c;
+        }
+        return c;
+    }
+    static constexpr int M = count_m();
+    static constexpr int G = count_g();
+    static constexpr int MG = ipow_c(GW, DIM);
+};
+
+template <int DIM, int L, int AMIN>
+struct BasisTables {
+    using S = BasisShape<DIM, L, AMIN>;
+    int e[S::M][3];  // basis exponents
+    int g[S::G][3];  // moment exponents
+    constexpr BasisTables() : e{}, g{} {
+        int c = 0;
+        for (int lin = 0; lin < ipow_c(L, DIM); ++lin) {
+            int k[3] = {0, 0, 0}, s = 0, r = lin;
+            for (int d = 0; d < DIM; ++d) { k[d] = r % L; r /= L; s += k[d]; }
+            if (s >= AMIN && s < L) {
+                e[c][0] = k[0]; e[c][1] = k[1]; e[c][2] = k[2];
+                ++c;
+            }
+        }
+        c = 0;
+        for (int lin = 0; lin < ipow_c(S::GW, DIM); ++lin) {
+            int k[3] = {0, 0, 0}, s = 0, r = lin;
+            for (int d = 0; d < DIM; ++d) { k[d] = r % S::GW; r /= S::GW; s += k[d]; }
+            if (s >= 2 * AMIN && s <= S::P) {
+                g[c][0] = k[0]; g[c][1] = k[1]; g[c][2] = k[2];
+                ++c;
+            }
+        }
+    }
+};
+
+}  // namespace dcpse

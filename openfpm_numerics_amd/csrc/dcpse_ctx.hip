@@ -1,0 +1,79 @@
+// Context lifecycle, stream binding and launch configuration of the C ABI.
+#include <cstring>
+
+#include "dcpse_internal.hpp"
+
+using namespace dcpse;
+
+extern "C" {
+
+int32_t dcpse_version(void) { return 100; }
+
+dcpse_status dcpse_ctx_create(dcpse_ctx** out, int32_t device) {
+    if (!out) return DCPSE_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return DCPSE_ERR_HIP;
+    if (device < 0 || device >= count) return DCPSE_ERR_INVALID;
+    if (hipSetDevice(device) != hipSuccess) return DCPSE_ERR_HIP;
+    dcpse_ctx* ctx = new dcpse_ctx();
+    ctx->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        ctx->num_cus = prop.multiProcessorCount;
+    // the device's default (null) stream until the caller binds another one
+    ctx->stream = nullptr;
+    *out = ctx;
+    return DCPSE_OK;
+}
+
+dcpse_status dcpse_ctx_destroy(dcpse_ctx* ctx) {
+    if (!ctx) return DCPSE_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->ws) (void)hipFree(ctx->ws);
+    delete ctx;
+    return DCPSE_OK;
+}
+
+dcpse_status dcpse_ctx_set_stream(dcpse_ctx* ctx, void* hip_stream) {
+    if (!ctx) return DCPSE_ERR_INVALID;
+    ctx->stream = static_cast<hipStream_t>(hip_stream);
+    return DCPSE_OK;
+}
+
+void* dcpse_ctx_get_stream(dcpse_ctx* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
+const char* dcpse_last_error(const dcpse_ctx* ctx) { return ctx ? ctx->err : "null context"; }
+
+dcpse_status dcpse_sync(dcpse_ctx* ctx) {
+    if (!ctx) return DCPSE_ERR_INVALID;
+    DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    DCPSE_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return DCPSE_OK;
+}
+
+dcpse_status dcpse_set_apply_config(dcpse_ctx* ctx, int32_t group_lanes, int32_t block_threads) {
+    if (!ctx) return DCPSE_ERR_INVALID;
+    if (group_lanes != 0 && group_lanes != 4 && group_lanes != 8 && group_lanes != 16 && group_lanes != 32 &&
+        group_lanes != 64)
+        return fail(ctx, DCPSE_ERR_INVALID, "group_lanes must be 0 or a power of two in [4, 64]");
+    if (block_threads < 64 || block_threads > 1024 || block_threads % 64)
+        return fail(ctx, DCPSE_ERR_INVALID, "block_threads must be a multiple of 64 in [64, 1024]");
+    ctx->apply_group = group_lanes;
+    ctx->apply_block = block_threads;
+    return DCPSE_OK;
+}
+
+dcpse_status dcpse_set_assembly_config(dcpse_ctx* ctx, int32_t waves_per_block, int32_t particles_per_wave) {
+    if (!ctx) return DCPSE_ERR_INVALID;
+    if (waves_per_block != 1)
+        return fail(ctx, DCPSE_ERR_UNSUPPORTED, "the assembly kernel runs one wavefront per block");
+    if (particles_per_wave < 0 || particles_per_wave > 4096)
+        return fail(ctx, DCPSE_ERR_INVALID, "particles_per_wave must be in [0, 4096]");
+    ctx->asm_wpb = waves_per_block;
+    ctx->asm_ppb = particles_per_wave > 0 ? particles_per_wave : 8;
+    return DCPSE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,72 @@
+// Host-side internals shared by the translation units of libdcpse_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdarg>
+#include <cstdio>
+
+#include "dcpse_hip.h"
+
+struct dcpse_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;  // nullptr = the device's default stream
+    int num_cus = 256;
+    int apply_group = 0;   // 0 = automatic
+    int apply_block = 256;
+    const void* grp_rows = nullptr;  // cache of the automatic apply group
+    int64_t grp_n = -1;
+    int grp_g = 16;
+    int asm_wpb = 1;       // waves per block in the assembly kernel
+    int asm_ppb = 8;       // particles per block (per wave) batch
+    void* ws = nullptr;    // grow-only device workspace
+    size_t ws_size = 0;
+    char err[1024] = {0};
+};
+
+namespace dcpse {
+
+inline dcpse_status fail(dcpse_ctx* ctx, dcpse_status code, const char* fmt, ...) {
+    if (ctx) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(ctx->err, sizeof(ctx->err), fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define DCPSE_HIP_CHECK(ctx, expr)                                                        \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return ::dcpse::fail(ctx, DCPSE_ERR_HIP, "%s: %s (%s:%d)", #expr,            \
+                                 hipGetErrorString(e_), __FILE__, __LINE__);              \
+    } while (0)
+
+#define DCPSE_LAUNCH_CHECK(ctx) DCPSE_HIP_CHECK(ctx, hipGetLastError())
+
+// Bump allocator over the context workspace, valid for one entry-point call.
+struct Workspace {
+    char* base = nullptr;
+    size_t used = 0, cap = 0;
+    template <typename T>
+    T* take(size_t n) {
+        size_t off = (used + 255) & ~size_t(255);
+        used = off + n * sizeof(T);
+        return used <= cap ? reinterpret_cast<T*>(base + off) : nullptr;
+    }
+};
+
+// Ensures ctx->ws holds at least `bytes`; waits for queued work before growing.
+dcpse_status reserve_workspace(dcpse_ctx* ctx, size_t bytes, Workspace* w);
+
+// Exclusive scan of n int32 counts into out[0..n] (int64), on ctx->stream.
+dcpse_status scan_counts(dcpse_ctx* ctx, const int32_t* in, int64_t n, int64_t* out, Workspace* w);
+size_t scan_workspace_bytes(int64_t n);
+
+// Device fill helpers (kernels, so they can be captured in a graph).
+void fill_i32(hipStream_t s, int32_t* p, int64_t n, int32_t v);
+
+}  // namespace dcpse

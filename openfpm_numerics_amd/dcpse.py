@@ -1,0 +1,374 @@
+"""``Dcpse``: one DC-PSE operator, assembled and applied on the GPU.
+
+Mirrors the interface of the reference class
+``Dcpse<dim, VerletList, vector_type, vector_type2>`` (src/DCPSE/Dcpse.hpp:43-826):
+constructor arguments (particles, verletList, differentialSignature,
+convergenceOrder, rCut, opt) and the two-set form (:111-130), the bulk apply
+``computeDifferentialOperator`` (:467-499), the row accessors ``getNumNN``,
+``getCoeffNN``, ``getIndexNN``, ``getSign``, ``getEpsilonInvPrefactor``
+(:507-550), ``initializeUpdate`` (:661-687), ``save``/``load`` (:302-379),
+``checkMomenta`` (:381-457) and ``DrawKernel`` (:151-193).
+
+Assembly runs the FP64 HIP kernel (dcpse_assemble) and keeps in HBM the
+reference's four arrays (localEps, localEpsInvPow, kerOffsets, calcKernels) plus
+the eps^-|beta| pre-scaled weights in FP64 and FP32 used by the apply kernels.
+A ``Dcpse`` can carry a linear combination of signatures that share one basis
+(``signatures``/``coeffs``); the Laplacian uses that to assemble
+sum_d D^{2 e_d} as one operator.
+"""
+from __future__ import annotations
+
+import os
+import struct
+
+import numpy as np
+import torch
+
+from . import _lib
+from .particles import ParticleSet, VerletList
+
+# support_options (src/DCPSE/SupportBuilder.hpp:13-19)
+RADIUS = 0
+LOAD = 1
+ADAPTIVE = 2
+AT_LEAST_N_PARTICLES = 3
+
+_FILE_MAGIC = b"DCPSEHIP"
+_FILE_VERSION = 1
+
+
+def basis(beta, order):
+    """Monomial exponents (m, dim) of the operator's basis, reference order."""
+    lib = _lib.load()
+    b = np.ascontiguousarray(beta, dtype=np.uint32)
+    m = lib.dcpse_basis_size(b.size, b.ctypes.data, order)
+    if m < 0:
+        raise ValueError("dimension must be 2 or 3")
+    out = np.zeros(max(m, 1) * b.size, np.uint32)
+    lib.dcpse_basis(b.size, b.ctypes.data, order, out.ctypes.data, m)
+    return out[: m * b.size].reshape(m, b.size)
+
+
+def rhs(beta, order):
+    lib = _lib.load()
+    b = np.ascontiguousarray(beta, dtype=np.uint32)
+    m = lib.dcpse_basis_size(b.size, b.ctypes.data, order)
+    out = np.zeros(max(m, 1))
+    lib.dcpse_rhs(b.size, b.ctypes.data, order, out.ctypes.data, m)
+    return out[:m]
+
+
+class Dcpse:
+    #: c in eps = <h>/c (src/DCPSE/Dcpse.hpp:80)
+    HOverEpsilon = 0.9
+
+    def __init__(self, particles: ParticleSet, verletList: VerletList, differentialSignature,
+                 convergenceOrder: int, rCut, opt: int = RADIUS, *, particlesDomain: ParticleSet | None = None,
+                 signatures=None, coeffs=None, h_over_eps: float | None = None, refine: bool = True):
+        self.particlesSupport = particles
+        self.particlesDomain = particles if particlesDomain is None else particlesDomain
+        self.same_set = particlesDomain is None
+        self.verletList = verletList
+        sig = tuple(int(x) for x in differentialSignature)
+        self.signatures = [sig] if signatures is None else [tuple(int(x) for x in s) for s in signatures]
+        self.coeffs = [1.0] * len(self.signatures) if coeffs is None else [float(c) for c in coeffs]
+        if len(self.coeffs) != len(self.signatures):
+            raise ValueError("one coefficient per signature")
+        self.differentialSignature = self.signatures[0]
+        self.differentialOrder = sum(self.differentialSignature)
+        if any(sum(s) != self.differentialOrder for s in self.signatures):
+            raise ValueError("combined signatures must share |beta|")
+        self.convergenceOrder = int(convergenceOrder)
+        self.rCut = rCut
+        self.opt = opt
+        if h_over_eps is not None:
+            self.HOverEpsilon = float(h_over_eps)
+        self.refine = refine
+        self.monomialBasis = basis(self.differentialSignature, self.convergenceOrder)
+        self.localEps = None
+        self.localEpsInvPow = None
+        self.calcKernels = None
+        self.info = None
+        self._W = {}
+        self._T = {}
+        self.update_ctr = particles.map_ctr
+        if opt != LOAD:
+            self.initializeStaticSize(self.particlesSupport, self.particlesDomain)
+
+    # ---- assembly --------------------------------------------------------------
+    @property
+    def dims(self) -> int:
+        return self.particlesSupport.dims
+
+    @property
+    def kerOffsets(self) -> torch.Tensor:
+        return self.verletList.row_ptr
+
+    def initializeStaticSize(self, particlesSupport: ParticleSet, particlesDomain: ParticleSet):
+        vl = self.verletList
+        sup, dom = particlesSupport, particlesDomain
+        dev = sup.device
+        n_dom = dom.size_local()
+        if vl.n_rows != n_dom:
+            raise ValueError(f"Verlet list has {vl.n_rows} rows, domain has {n_dom} particles")
+        nnz = vl.nnz
+        eps = torch.empty(n_dom, dtype=torch.float64, device=dev)
+        eip = torch.empty(n_dom, dtype=torch.float64, device=dev)
+        K64 = torch.empty(max(nnz, 1), dtype=torch.float64, device=dev)
+        W64 = torch.empty(max(nnz, 1), dtype=torch.float64, device=dev)
+        W32 = torch.empty(max(nnz, 1), dtype=torch.float32, device=dev)
+        info = torch.empty(n_dom, dtype=torch.int32, device=dev)
+        betas = np.ascontiguousarray(np.array(self.signatures, dtype=np.uint32).ravel())
+        coeffs = np.ascontiguousarray(np.array(self.coeffs, dtype=np.float64))
+        lo = np.ascontiguousarray(sup.box_lo)
+        hi = np.ascontiguousarray(sup.box_hi)
+        ctx = _lib.context(dev.index or 0)
+        ctx.call("dcpse_assemble", sup.dims, _lib.ptr(sup.pos_soa), sup.size_local(),
+                 None if self.same_set else _lib.ptr(dom.pos_soa), n_dom,
+                 lo.ctypes.data, hi.ctypes.data, sup.periodic_mask,
+                 _lib.ptr(vl.row_ptr), _lib.ptr(vl.col) if nnz else None,
+                 len(self.signatures), betas.ctypes.data, coeffs.ctypes.data,
+                 self.convergenceOrder, float(self.HOverEpsilon),
+                 _lib.DCPSE_ASM_REFINE if self.refine else 0,
+                 _lib.ptr(eps), _lib.ptr(eip), _lib.ptr(K64), _lib.ptr(W32), _lib.ptr(W64), _lib.ptr(info))
+        self.localEps, self.localEpsInvPow = eps, eip
+        self.calcKernels = K64[:nnz]
+        self._W = {torch.float64: W64[:nnz], torch.float32: W32[:nnz]}
+        self._T = {}
+        self.info = info
+        self.update_ctr = sup.map_ctr
+        bad = info != 0
+        if bool(bad.any()):
+            first = int(torch.nonzero(bad)[0])
+            code = int(info[first])
+            msg = _lib.DCPSE_INFO.get(code, str(code))
+            if code == 1:
+                raise ValueError(f"Not enough neighbour points passed for Vandermonde matrix "
+                                 f"construction! (particle {first}: {vl.getNNPart(first)} neighbours, "
+                                 f"{len(self.monomialBasis)} monomials)")
+            raise ArithmeticError(f"DC-PSE assembly failed at particle {first}: {msg}")
+
+    def initializeUpdate(self, particlesSupport: ParticleSet | None = None,
+                         particlesDomain: ParticleSet | None = None):
+        sup = particlesSupport or self.particlesSupport
+        dom = particlesDomain or (sup if self.same_set else self.particlesDomain)
+        self.initializeStaticSize(sup, dom)
+
+    # ---- apply -------------------------------------------------------------------
+    def weights(self, dtype) -> torch.Tensor:
+        if dtype not in self._W:
+            raise TypeError(f"no weights for {dtype}; use float32 or float64")
+        return self._W[dtype]
+
+    def computeDifferentialOperator(self, f: torch.Tensor, out: torch.Tensor | None = None,
+                                    f_domain: torch.Tensor | None = None, alpha=1.0, beta=0.0):
+        """D f for a field (N,) or a batch of fields (B, N) over the support set."""
+        return apply_ops([self], f, [out] if out is not None else None, f_domain, alpha, beta)[0]
+
+    def __call__(self, f):
+        return self.computeDifferentialOperator(f)
+
+    # ---- row accessors (value_nz contract, DCPSE_op.hpp:69-83) ------------------------
+    def _host_rows(self):
+        if getattr(self, "_hk", None) is None or self._hk[0] is not self.calcKernels:
+            self._hk = (self.calcKernels, self.calcKernels.cpu().numpy(), self.localEpsInvPow.cpu().numpy())
+        return self._hk
+
+    def getNumNN(self, p: int) -> int:
+        return self.verletList.getNNPart(p)
+
+    def getCoeffNN(self, p: int, j: int) -> float:
+        _, K, _ = self._host_rows()
+        rp, _ = self.verletList._mirror()
+        return float(K[rp[p] + j])
+
+    def getIndexNN(self, p: int, j: int) -> int:
+        return self.verletList.get(p, j)
+
+    def getSign(self) -> float:
+        return -1.0 if (self.differentialOrder % 2 == 0 and self.differentialOrder != 0) else 1.0
+
+    def getEpsilonInvPrefactor(self, p: int) -> float:
+        return float(self._host_rows()[2][p])
+
+    # ---- adjoint -----------------------------------------------------------------
+    def transposed(self, dtype):
+        """(rowT_ptr, colT, WT, diag) of the transpose of the difference-form operator."""
+        if dtype in self._T:
+            return self._T[dtype]
+        if not self.same_set:
+            raise NotImplementedError("the transposed operator needs the domain set to be the support set")
+        vl = self.verletList
+        dev = vl.row_ptr.device
+        n = vl.n_rows
+        ctx = _lib.context(dev.index or 0)
+        rowT = torch.empty(n + 1, dtype=torch.int64, device=dev)
+        ctx.call("dcpse_transpose_count", n, vl.n_cols, _lib.ptr(vl.row_ptr), _lib.ptr(vl.col), _lib.ptr(rowT))
+        colT = torch.empty(max(vl.nnz, 1), dtype=torch.int32, device=dev)
+        WT = torch.empty(max(vl.nnz, 1), dtype=dtype, device=dev)
+        diag = torch.empty(n, dtype=dtype, device=dev)
+        name = "dcpse_transpose_fill_f32" if dtype == torch.float32 else "dcpse_transpose_fill_f64"
+        ctx.call(name, n, vl.n_cols, _lib.ptr(vl.row_ptr), _lib.ptr(vl.col), _lib.ptr(self.weights(dtype)),
+                 _lib.ptr(rowT), _lib.ptr(colT), _lib.ptr(WT), _lib.ptr(diag))
+        self._T[dtype] = (rowT, colT[: vl.nnz], WT[: vl.nnz], diag)
+        return self._T[dtype]
+
+    def applyTranspose(self, g: torch.Tensor, out: torch.Tensor | None = None, alpha=1.0, beta=0.0):
+        """A^T g for a field (N,) or a batch (B, N)."""
+        if g.dtype not in (torch.float32, torch.float64):
+            raise TypeError("fields must be float32 or float64")
+        rowT, colT, WT, diag = self.transposed(g.dtype)
+        g2 = g.reshape(-1, g.shape[-1]).contiguous()
+        nb, n = g2.shape
+        if n != self.verletList.n_cols:
+            raise ValueError("field length differs from the support set")
+        o = torch.empty_like(g2) if out is None else out.reshape(nb, n)
+        ctx = _lib.context(g.device.index or 0)
+        name = "dcpse_apply_transpose_f32" if g.dtype == torch.float32 else "dcpse_apply_transpose_f64"
+        ctx.call(name, n, _lib.ptr(rowT), _lib.ptr(colT), _lib.ptr(WT), _lib.ptr(diag), _lib.ptr(g2), n, nb,
+                 _lib.ptr(o), n, alpha, beta)
+        return o.reshape(g.shape)
+
+    # ---- persistence (Dcpse.hpp:302-379) -------------------------------------------
+    def save(self, file: str, rank: int = 0):
+        """Writes localEps, localEpsInvPow, calcKernels, kerOffsets and the support."""
+        path = f"{file}_{rank}"
+        eps = self.localEps.cpu().numpy()
+        eip = self.localEpsInvPow.cpu().numpy()
+        K = self.calcKernels.cpu().numpy()
+        rp = self.verletList.row_ptr.cpu().numpy()
+        col = self.verletList.col.cpu().numpy()
+        sig = np.array(self.signatures, dtype=np.uint32)
+        with open(path, "wb") as fh:
+            fh.write(_FILE_MAGIC)
+            fh.write(struct.pack("<IIIId", _FILE_VERSION, self.dims, len(self.signatures),
+                                 self.convergenceOrder, float(self.HOverEpsilon)))
+            fh.write(struct.pack("<qq", eps.size, K.size))
+            fh.write(sig.astype("<u4").tobytes())
+            fh.write(np.array(self.coeffs, dtype="<f8").tobytes())
+            for arr, dt in ((eps, "<f8"), (eip, "<f8"), (K, "<f8"), (rp, "<i8"), (col, "<i4")):
+                fh.write(np.ascontiguousarray(arr).astype(dt).tobytes())
+
+    def load(self, file: str, rank: int = 0):
+        path = f"{file}_{rank}"
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        with open(path, "rb") as fh:
+            if fh.read(8) != _FILE_MAGIC:
+                raise ValueError(f"{path} is not a DC-PSE kernel file")
+            ver, dim, nsig, order, hoe = struct.unpack("<IIIId", fh.read(24))
+            if ver != _FILE_VERSION:
+                raise ValueError(f"unsupported kernel file version {ver}")
+            n, nnz = struct.unpack("<qq", fh.read(16))
+            sig = np.frombuffer(fh.read(4 * nsig * dim), dtype="<u4").reshape(nsig, dim)
+            coeffs = np.frombuffer(fh.read(8 * nsig), dtype="<f8")
+            eps = np.frombuffer(fh.read(8 * n), dtype="<f8")
+            eip = np.frombuffer(fh.read(8 * n), dtype="<f8")
+            K = np.frombuffer(fh.read(8 * nnz), dtype="<f8")
+            rp = np.frombuffer(fh.read(8 * (n + 1)), dtype="<i8")
+            col = np.frombuffer(fh.read(4 * nnz), dtype="<i4")
+        if [tuple(int(v) for v in s) for s in sig] != self.signatures:
+            raise ValueError("kernel file holds a different differential signature")
+        dev = self.particlesSupport.device
+        vl = self.verletList
+        if vl.n_rows != n or vl.nnz != nnz or not np.array_equal(vl.row_ptr.cpu().numpy(), rp) \
+                or not np.array_equal(vl.col.cpu().numpy(), col):
+            # the file's support replaces the current Verlet rows
+            vl.row_ptr = torch.as_tensor(rp.copy(), device=dev)
+            vl.col = torch.as_tensor(col.copy(), device=dev)
+            vl.n_rows = n
+            vl._host = None
+        self.convergenceOrder, self.HOverEpsilon, self.coeffs = int(order), float(hoe), list(coeffs)
+        self.localEps = torch.as_tensor(eps.copy(), device=dev)
+        self.localEpsInvPow = torch.as_tensor(eip.copy(), device=dev)
+        self.calcKernels = torch.as_tensor(K.copy(), device=dev)
+        rows = torch.repeat_interleave(torch.arange(n, device=dev), vl.row_lengths())
+        W64 = self.calcKernels * self.localEpsInvPow[rows]
+        self._W = {torch.float64: W64, torch.float32: W64.to(torch.float32)}
+        self._T = {}
+        self.info = torch.zeros(n, dtype=torch.int32, device=dev)
+        self.update_ctr = self.particlesSupport.map_ctr
+
+    # ---- diagnostics -------------------------------------------------------------
+    def checkMomenta(self, particles: ParticleSet | None = None):
+        """Per-monomial min/max over particles of sum_q K_pq m_alpha(x_pq/eps).
+
+        For a valid operator every row reproduces b_alpha (DcpseRhs.hpp:27-60).
+        """
+        sup = particles or self.particlesSupport
+        dom = self.particlesDomain if not self.same_set else sup
+        vl = self.verletList
+        dev = sup.device
+        rows = torch.repeat_interleave(torch.arange(vl.n_rows, device=dev), vl.row_lengths())
+        cols = vl.col.long()
+        d = dom.pos_soa[:, rows] - sup.pos_soa[:, cols]
+        for k in range(sup.dims):
+            if sup.periodic_mask >> k & 1:
+                L = float(sup.box_hi[k] - sup.box_lo[k])
+                d[k] -= L * torch.round(d[k] / L)
+        z = d / self.localEps[rows]
+        out = []
+        for e in self.monomialBasis:
+            mval = torch.ones_like(z[0])
+            for k in range(sup.dims):
+                mval = mval * z[k] ** int(e[k])
+            mom = torch.zeros(vl.n_rows, dtype=torch.float64, device=dev).index_add_(0, rows, mval * self.calcKernels)
+            out.append((float(mom.min()), float(mom.max())))
+        return out
+
+    def DrawKernel(self, prop: torch.Tensor, p: int):
+        """prop[q] += K_pq for every neighbour q of p."""
+        vl = self.verletList
+        rp, col = vl._mirror()
+        s, e = int(rp[p]), int(rp[p + 1])
+        prop.index_add_(0, vl.col[s:e].long(), self.calcKernels[s:e].to(prop.dtype))
+        return prop
+
+    def DrawKernelNN(self, prop: torch.Tensor, p: int):
+        vl = self.verletList
+        rp, _ = vl._mirror()
+        prop[vl.col[int(rp[p]):int(rp[p + 1])].long()] = 1.0
+        return prop
+
+
+def apply_ops(ops, f: torch.Tensor, outs=None, f_domain: torch.Tensor | None = None, alpha=1.0, beta=0.0):
+    """Applies several operators that share one Verlet list in a single pass.
+
+    ``f``: (N,) or (B, N) float32/float64 over the support set. Returns one
+    output per operator, shaped like ``f`` but over the domain set.
+    """
+    if not ops:
+        return []
+    vl = ops[0].verletList
+    for op in ops[1:]:
+        if op.verletList is not vl:
+            raise ValueError("fused operators must share one Verlet list")
+    if f.dtype not in (torch.float32, torch.float64):
+        raise TypeError("fields must be float32 or float64")
+    for op in ops:
+        if op.update_ctr != op.particlesSupport.map_ctr:
+            raise RuntimeError("particles moved since assembly: call update()/initializeUpdate() first")
+    f2 = f.reshape(-1, f.shape[-1]).contiguous()
+    nb, n = f2.shape
+    if n != vl.n_cols:
+        raise ValueError(f"field has {n} values, support set has {vl.n_cols} particles")
+    fd = None
+    if f_domain is not None:
+        fd = f_domain.reshape(nb, -1).contiguous().to(f.dtype)
+    elif not ops[0].same_set:
+        raise ValueError("two-set operators need f_domain (the field on the domain particles)")
+    n_rows = vl.n_rows
+    if outs is None:
+        outs = [torch.empty((nb, n_rows), dtype=f.dtype, device=f.device) for _ in ops]
+    else:
+        outs = [o.reshape(nb, n_rows) for o in outs]
+    W = [op.weights(f.dtype) for op in ops]
+    ctx = _lib.context(f.device.index or 0)
+    name = "dcpse_apply_f32" if f.dtype == torch.float32 else "dcpse_apply_f64"
+    if len(ops) > 1 and nb > 1:
+        res = [apply_ops([op], f, [o], f_domain, alpha, beta)[0] for op, o in zip(ops, outs)]
+        return [r.reshape(*f.shape[:-1], n_rows) for r in res]
+    ctx.call(name, n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(vl.col), len(ops), _lib.ptr_array(W),
+             _lib.ptr(f2), n, nb, _lib.ptr(fd), n_rows if fd is not None else n, _lib.ptr_array(outs), n_rows,
+             alpha, beta)
+    return [o.reshape(*f.shape[:-1], n_rows) for o in outs]

@@ -1,0 +1,137 @@
+"""Particle sets and Verlet (support) lists on the device.
+
+``ParticleSet`` stands in for openfpm's ``vector_dist`` as far as the DC-PSE
+path needs it (positions, box, boundary conditions, named properties), and
+``VerletList`` for the ``VL_NON_SYMMETRIC | VL_SKIP_REF_PART`` Verlet list the
+reference builds with ``vector_dist::getVerlet`` (e.g.
+src/DCPSE/tests/Dcpse_unit_tests.cpp:97): row p lists every q != p with
+|x_p - x_q| < rCut, here in ascending particle index. Both live in HBM; the
+neighbour search runs on the GPU (dcpse_support_count / dcpse_support_fill).
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+
+PERIODIC = 1
+NON_PERIODIC = 0
+
+
+class ParticleSet:
+    """Positions (structure of arrays, float64) plus named properties."""
+
+    def __init__(self, pos, box_lo, box_hi, bc=None, device=None):
+        dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        p = torch.as_tensor(pos, dtype=torch.float64)
+        if p.dim() != 2 or p.shape[1] not in (2, 3):
+            raise ValueError("pos must be (N, 2) or (N, 3)")
+        self.dims = int(p.shape[1])
+        self.pos_soa = p.t().contiguous().to(dev)
+        self.box_lo = np.asarray(box_lo, dtype=np.float64).reshape(self.dims).copy()
+        self.box_hi = np.asarray(box_hi, dtype=np.float64).reshape(self.dims).copy()
+        bc = [NON_PERIODIC] * self.dims if bc is None else list(bc)
+        self.periodic_mask = sum(1 << d for d in range(self.dims) if bc[d] == PERIODIC)
+        self.props: dict[str, torch.Tensor] = {}
+        self.map_ctr = 0  # bumped whenever positions change (SE_CLASS1-style staleness check)
+
+    @property
+    def device(self):
+        return self.pos_soa.device
+
+    def size_local(self) -> int:
+        return int(self.pos_soa.shape[1])
+
+    def getPos(self, p: int) -> np.ndarray:
+        return self.pos_soa[:, p].cpu().numpy()
+
+    def positions(self) -> torch.Tensor:
+        """(N, dim) view of the positions."""
+        return self.pos_soa.t()
+
+    def set_positions(self, pos):
+        p = torch.as_tensor(pos, dtype=torch.float64, device=self.device)
+        self.pos_soa.copy_(p.t())
+        self.map_ctr += 1
+
+    def getProp(self, name: str) -> torch.Tensor:
+        return self.props[name]
+
+    def setProp(self, name: str, value) -> torch.Tensor:
+        t = torch.as_tensor(value, device=self.device)
+        self.props[name] = t
+        return t
+
+    def getVerlet(self, rCut) -> "VerletList":
+        return VerletList.build(self, rCut)
+
+    def updateVerlet(self, vl: "VerletList", rCut) -> "VerletList":
+        fresh = VerletList.build(self, rCut, domain=vl.domain)
+        vl.__dict__.update(fresh.__dict__)
+        return vl
+
+
+class VerletList:
+    """Neighbour CSR in HBM: row_ptr int64[n+1], col int32[nnz]."""
+
+    def __init__(self, row_ptr: torch.Tensor, col: torch.Tensor, n_cols: int, rcut, domain=None):
+        self.row_ptr = row_ptr
+        self.col = col
+        self.n_rows = int(row_ptr.numel() - 1)
+        self.n_cols = n_cols
+        self.rcut = rcut
+        self.domain = domain
+        self._host = None
+        self.map_ctr = -1
+
+    @property
+    def nnz(self) -> int:
+        return int(self.col.numel())
+
+    @classmethod
+    def build(cls, support: ParticleSet, rCut, domain: ParticleSet | None = None) -> "VerletList":
+        ctx = _lib.context(support.device.index or 0)
+        dev = support.device
+        dom = support if domain is None else domain
+        if dom.dims != support.dims:
+            raise ValueError("support and domain sets differ in dimension")
+        rc = torch.as_tensor(rCut, dtype=torch.float64, device=dev).reshape(-1).contiguous()
+        per = int(rc.numel() > 1)
+        if per and rc.numel() != dom.size_local():
+            raise ValueError("per-particle rCut needs one value per domain particle")
+        n_dom = dom.size_local()
+        row_ptr = torch.empty(n_dom + 1, dtype=torch.int64, device=dev)
+        nnz = np.zeros(1, np.int64)
+        pdom = None if domain is None else _lib.ptr(dom.pos_soa)
+        lo = np.ascontiguousarray(support.box_lo)
+        hi = np.ascontiguousarray(support.box_hi)
+        ctx.call("dcpse_support_count", support.dims, _lib.ptr(support.pos_soa), support.size_local(),
+                 pdom, n_dom, _lib.ptr(rc), per, lo.ctypes.data, hi.ctypes.data,
+                 support.periodic_mask, _lib.ptr(row_ptr), nnz.ctypes.data)
+        col = torch.empty(max(int(nnz[0]), 1), dtype=torch.int32, device=dev)
+        if int(nnz[0]) > 0:
+            ctx.call("dcpse_support_fill", support.dims, _lib.ptr(support.pos_soa), support.size_local(),
+                     pdom, n_dom, _lib.ptr(rc), per, lo.ctypes.data, hi.ctypes.data,
+                     support.periodic_mask, _lib.ptr(row_ptr), _lib.ptr(col))
+        col = col[: int(nnz[0])]
+        vl = cls(row_ptr, col, support.size_local(), rc, domain)
+        vl.map_ctr = support.map_ctr
+        return vl
+
+    # host mirror for per-key accessors (getNNPart / get), built on first use
+    def _mirror(self):
+        if self._host is None:
+            self._host = (self.row_ptr.cpu().numpy(), self.col.cpu().numpy())
+        return self._host
+
+    def getNNPart(self, p: int) -> int:
+        rp, _ = self._mirror()
+        return int(rp[p + 1] - rp[p])
+
+    def get(self, p: int, j: int) -> int:
+        rp, col = self._mirror()
+        return int(col[rp[p] + j])
+
+    def row_lengths(self) -> torch.Tensor:
+        return self.row_ptr[1:] - self.row_ptr[:-1]

@@ -1,0 +1,543 @@
+// DC-PSE CPU oracle: a plain C++17 restatement of the reference's CPU Dcpse path.
+//
+// TEST INFRASTRUCTURE ONLY. Only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py may load this library, and only as the checker
+// (or the timed CPU baseline). The product path (openfpm_numerics_amd) never
+// calls it.
+//
+// What it restates (reference file:line, paths relative to the reference root):
+//   basis enumeration        src/DCPSE/MonomialBasis.hpp:115-156
+//   monomial eval/derivative src/DCPSE/Monomial.hpp:146-176
+//   RHS vector               src/DCPSE/DcpseRhs.hpp:27-60
+//   x_pq list + eps          src/DCPSE/Vandermonde.hpp:63-116
+//   Vandermonde rows         src/DCPSE/VandermondeRowBuilder.hpp:26-35
+//   diagonal scaling E       src/DCPSE/DcpseDiagonalScalingMatrix.hpp:22-41
+//   assembly driver          src/DCPSE/Dcpse.hpp:690-796 (B=E*V dense, A=B^T B,
+//                            colPivHouseholderQr solve, kernel :798-809)
+//   apply (bulk)             src/DCPSE/Dcpse.hpp:467-499
+//   value_nz row export      src/DCPSE/DCPSE_op/DCPSE_op.hpp:69-83
+//   support (Verlet, self excluded, dist < rCut)
+//                            src/DCPSE/SupportBuilder.cuh:15-61 (semantics)
+//
+// Deliberate, documented differences (rounding-level only):
+//   * Eigen's ColPivHouseholderQR is replaced by a textbook Householder QR with
+//     column pivoting (Golub & Van Loan, Matrix Computations 4th ed., Alg. 5.4.1)
+//     whose column norms are recomputed exactly at every step instead of being
+//     downdated; the Householder reflector follows the same convention
+//     (beta = -sign(c0)*||x||, tau = (beta-c0)/beta, v = [1; tail/(c0-beta)]).
+//   * Each support row lists its neighbours in ascending particle index
+//     (openfpm's VerletList order is cell-iteration order; it only changes
+//     summation order).
+//   * The distance test is d2 < rCut^2 with d2 built from explicit fma() so the
+//     GPU builder reproduces it bit for bit; the reference tests sqrt(d2) < rCut.
+//   * Periodic boundaries use the minimum-image displacement instead of ghost
+//     copies of the particles (same x_pq when rCut < L/2).
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr int kMaxDim = 3;
+
+// Binary exponentiation, the rule openfpm::math::intpowlog names.
+inline double intpow(double x, unsigned e) {
+    double r = 1.0;
+    while (e) {
+        if (e & 1u) r *= x;
+        x *= x;
+        e >>= 1u;
+    }
+    return r;
+}
+
+struct Basis {
+    int dim = 0;
+    int m = 0;
+    std::vector<unsigned> e;  // m*dim exponents, row-major
+    unsigned order_limit = 0, alpha_min = 0;
+};
+
+// MonomialBasis.hpp:115-156. The candidate grid is [0, |beta|+r)^dim, walked
+// with dimension 0 varying fastest (openfpm grid_key_dx iteration order).
+Basis make_basis(int dim, const unsigned* beta, unsigned order) {
+    Basis B;
+    B.dim = dim;
+    unsigned msum = 0;
+    for (int d = 0; d < dim; ++d) msum += beta[d];
+    B.order_limit = msum + order;
+    B.alpha_min = (msum % 2 == 0) ? 1u : 0u;
+    if (msum == 0) B.alpha_min = 0;
+    unsigned L = B.order_limit;
+    unsigned total = 1;
+    for (int d = 0; d < dim; ++d) total *= L;
+    for (unsigned lin = 0; lin < total; ++lin) {
+        unsigned k[kMaxDim] = {0, 0, 0};
+        unsigned r = lin, s = 0;
+        for (int d = 0; d < dim; ++d) {
+            k[d] = r % L;
+            r /= L;
+            s += k[d];
+        }
+        if (s < L && s >= B.alpha_min) {
+            for (int d = 0; d < dim; ++d) B.e.push_back(k[d]);
+            ++B.m;
+        }
+    }
+    return B;
+}
+
+// Monomial.hpp:146-156
+inline double mono_eval(int dim, const unsigned* e, double scalar, const double* x) {
+    double r = scalar;
+    for (int d = 0; d < dim; ++d) r *= intpow(x[d], e[d]);
+    return r;
+}
+
+// Monomial.hpp:158-174: falling-factorial scalar, exponents clamped at 0.
+inline unsigned mono_derivative(int dim, const unsigned* e, unsigned scalar,
+                                const unsigned* beta, unsigned* e_out) {
+    unsigned s = scalar;
+    for (int d = 0; d < dim; ++d) {
+        int orig = (int)e[d];
+        int target = orig - (int)beta[d];
+        for (int k = orig; k > target && k >= 0; --k) s *= (unsigned)k;
+        e_out[d] = (unsigned)std::max(target, 0);
+    }
+    return s;
+}
+
+// DcpseRhs.hpp:27-60: b_i = sign * (d^beta m_i)(0), sign = +1 for even |beta|.
+void make_rhs(const Basis& B, const unsigned* beta, double* b) {
+    unsigned ord = 0;
+    for (int d = 0; d < B.dim; ++d) ord += beta[d];
+    double sign = (ord % 2 == 0) ? 1.0 : -1.0;
+    const double zero[kMaxDim] = {0, 0, 0};
+    for (int i = 0; i < B.m; ++i) {
+        unsigned eo[kMaxDim];
+        unsigned s = mono_derivative(B.dim, &B.e[i * B.dim], 1u, beta, eo);
+        b[i] = sign * mono_eval(B.dim, eo, (double)s, zero);
+    }
+}
+
+inline void displacement(int dim, const double* pos_dom, int64_t ndom, int64_t p,
+                         const double* pos_sup, int64_t nsup, int64_t q,
+                         const double* box_lo, const double* box_hi, unsigned periodic,
+                         double* d) {
+    for (int k = 0; k < dim; ++k) {
+        double v = pos_dom[k * ndom + p] - pos_sup[k * nsup + q];
+        if (periodic & (1u << k)) {
+            double L = box_hi[k] - box_lo[k];
+            v -= L * std::nearbyint(v / L);
+        }
+        d[k] = v;
+    }
+}
+
+inline double dist2_fma(int dim, const double* d) {
+    double s = d[dim - 1] * d[dim - 1];
+    for (int k = dim - 2; k >= 0; --k) s = std::fma(d[k], d[k], s);
+    return s;
+}
+
+// Householder QR with column pivoting of the m x m matrix A (column-major),
+// then solve A x = b (Golub & Van Loan Alg. 5.4.1). Returns 0 on success.
+// Rank is taken as the number of |R_kk| > eps * m * |R_00|; the remaining
+// unknowns are set to zero, the way Eigen's solve treats a rank-deficient A.
+int colpiv_qr_solve(int m, std::vector<double>& A, const double* b, double* x) {
+    std::vector<int> perm(m);
+    for (int j = 0; j < m; ++j) perm[j] = j;
+    std::vector<double> tau(m, 0.0), c(b, b + m);
+    auto at = [&](int i, int j) -> double& { return A[(size_t)j * m + i]; };
+    for (int k = 0; k < m; ++k) {
+        // pivot: column of largest remaining norm (recomputed exactly)
+        int piv = k;
+        double best = -1.0;
+        for (int j = k; j < m; ++j) {
+            double s = 0.0;
+            for (int i = k; i < m; ++i) s += at(i, j) * at(i, j);
+            if (s > best) { best = s; piv = j; }
+        }
+        if (piv != k) {
+            for (int i = 0; i < m; ++i) std::swap(at(i, k), at(i, piv));
+            std::swap(perm[k], perm[piv]);
+        }
+        // Householder for column k, rows k..m-1
+        double c0 = at(k, k);
+        double tail = 0.0;
+        for (int i = k + 1; i < m; ++i) tail += at(i, k) * at(i, k);
+        double beta, t;
+        if (tail <= std::numeric_limits<double>::min()) {
+            t = 0.0;
+            beta = c0;
+            for (int i = k + 1; i < m; ++i) at(i, k) = 0.0;
+        } else {
+            beta = std::sqrt(c0 * c0 + tail);
+            if (c0 >= 0.0) beta = -beta;
+            double inv = 1.0 / (c0 - beta);
+            for (int i = k + 1; i < m; ++i) at(i, k) *= inv;  // essential part
+            t = (beta - c0) / beta;
+        }
+        at(k, k) = beta;
+        tau[k] = t;
+        if (t != 0.0) {
+            for (int j = k + 1; j < m; ++j) {
+                double s = at(k, j);
+                for (int i = k + 1; i < m; ++i) s += at(i, k) * at(i, j);
+                s *= t;
+                at(k, j) -= s;
+                for (int i = k + 1; i < m; ++i) at(i, j) -= s * at(i, k);
+            }
+            // c <- H c
+            double s = c[k];
+            for (int i = k + 1; i < m; ++i) s += at(i, k) * c[i];
+            s *= t;
+            c[k] -= s;
+            for (int i = k + 1; i < m; ++i) c[i] -= s * at(i, k);
+        }
+    }
+    double thr = std::numeric_limits<double>::epsilon() * m * std::fabs(at(0, 0));
+    int rank = 0;
+    while (rank < m && std::fabs(at(rank, rank)) > thr) ++rank;
+    std::vector<double> z(m, 0.0);
+    for (int i = rank - 1; i >= 0; --i) {
+        double s = c[i];
+        for (int j = i + 1; j < rank; ++j) s -= at(i, j) * z[j];
+        z[i] = s / at(i, i);
+    }
+    for (int j = 0; j < m; ++j) x[perm[j]] = z[j];
+    return rank == m ? 0 : 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_basis_size(int dim, const unsigned* beta, unsigned order) {
+    return make_basis(dim, beta, order).m;
+}
+
+// Writes m*dim exponents (row-major) and returns m.
+int orc_basis(int dim, const unsigned* beta, unsigned order, unsigned* exps, int cap) {
+    Basis B = make_basis(dim, beta, order);
+    if (B.m > cap) return -B.m;
+    std::copy(B.e.begin(), B.e.end(), exps);
+    return B.m;
+}
+
+double orc_monomial_eval(int dim, const unsigned* e, unsigned scalar, const double* x) {
+    return mono_eval(dim, e, (double)scalar, x);
+}
+
+unsigned orc_monomial_derivative(int dim, const unsigned* e, unsigned scalar,
+                                 const unsigned* beta, unsigned* e_out) {
+    return mono_derivative(dim, e, scalar, beta, e_out);
+}
+
+int orc_rhs(int dim, const unsigned* beta, unsigned order, double* b) {
+    Basis B = make_basis(dim, beta, order);
+    make_rhs(B, beta, b);
+    return B.m;
+}
+
+// VandermondeRowBuilder.hpp:26-35: row[j] = m_j(x) for x already divided by eps.
+void orc_vandermonde_row(int dim, int m, const unsigned* exps, const double* x, double* row) {
+    for (int j = 0; j < m; ++j) row[j] = mono_eval(dim, exps + j * dim, 1.0, x);
+}
+
+// Neighbour support: q != p (same-set) and |x_p - x_q| < rcut_p. A uniform
+// cell grid of edge >= max rcut; rows are sorted by ascending q.
+// rcut has 1 entry (uniform) or ndom entries (per_particle != 0).
+// Pass 1 (col == nullptr) writes row_ptr[0..ndom] and returns nnz;
+// pass 2 (col != nullptr) fills col using the row_ptr from pass 1.
+int64_t orc_support(int dim, const double* pos_sup, int64_t nsup,
+                    const double* pos_dom, int64_t ndom, int same_set,
+                    const double* rcut, int per_particle,
+                    const double* box_lo, const double* box_hi, unsigned periodic,
+                    int64_t* row_ptr, int32_t* col, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+    double rmax = rcut[0];
+    if (per_particle)
+        for (int64_t p = 0; p < ndom; ++p) rmax = std::max(rmax, rcut[p]);
+    int nc[kMaxDim] = {1, 1, 1};
+    double cs[kMaxDim] = {1, 1, 1};
+    for (int d = 0; d < dim; ++d) {
+        double L = box_hi[d] - box_lo[d];
+        int n = (int)std::floor(L / rmax);
+        if (n < 1) n = 1;
+        if ((periodic & (1u << d)) && n < 3) return -1;  // image duplicates
+        nc[d] = n;
+        cs[d] = L / n;
+    }
+    int64_t ncell = (int64_t)nc[0] * nc[1] * nc[2];
+    auto cell_coord = [&](const double* pos, int64_t np, int64_t i, int* c) {
+        for (int d = 0; d < dim; ++d) {
+            int v = (int)std::floor((pos[d * np + i] - box_lo[d]) / cs[d]);
+            if (periodic & (1u << d)) { v %= nc[d]; if (v < 0) v += nc[d]; }
+            else v = std::min(std::max(v, 0), nc[d] - 1);
+            c[d] = v;
+        }
+        for (int d = dim; d < kMaxDim; ++d) c[d] = 0;
+    };
+    std::vector<int64_t> cstart(ncell + 1, 0);
+    std::vector<int32_t> cpart(nsup);
+    std::vector<int64_t> cidx(nsup);
+    for (int64_t q = 0; q < nsup; ++q) {
+        int c[kMaxDim];
+        cell_coord(pos_sup, nsup, q, c);
+        cidx[q] = c[0] + (int64_t)nc[0] * (c[1] + (int64_t)nc[1] * c[2]);
+        cstart[cidx[q] + 1]++;
+    }
+    for (int64_t c = 0; c < ncell; ++c) cstart[c + 1] += cstart[c];
+    {
+        std::vector<int64_t> cur(cstart.begin(), cstart.end() - 1);
+        for (int64_t q = 0; q < nsup; ++q) cpart[cur[cidx[q]]++] = (int32_t)q;
+    }
+    const bool fill = (col != nullptr);
+    std::vector<int64_t> counts(fill ? 0 : ndom);
+#pragma omp parallel
+    {
+        std::vector<int32_t> row;
+#pragma omp for schedule(dynamic, 512)
+        for (int64_t p = 0; p < ndom; ++p) {
+            int c[kMaxDim];
+            cell_coord(pos_dom, ndom, p, c);
+            double r = per_particle ? rcut[p] : rcut[0];
+            double r2 = r * r;
+            row.clear();
+            int lo[kMaxDim], hi[kMaxDim];
+            for (int d = 0; d < kMaxDim; ++d) {
+                lo[d] = d < dim ? -1 : 0;
+                hi[d] = d < dim ? 1 : 0;
+            }
+            for (int oz = lo[2]; oz <= hi[2]; ++oz)
+                for (int oy = lo[1]; oy <= hi[1]; ++oy)
+                    for (int ox = lo[0]; ox <= hi[0]; ++ox) {
+                        int cc[kMaxDim] = {c[0] + ox, c[1] + oy, c[2] + oz};
+                        bool ok = true;
+                        for (int d = 0; d < dim; ++d) {
+                            if (periodic & (1u << d)) {
+                                cc[d] = (cc[d] + nc[d]) % nc[d];
+                            } else if (cc[d] < 0 || cc[d] >= nc[d]) {
+                                ok = false;
+                            }
+                        }
+                        if (!ok) continue;
+                        int64_t cl = cc[0] + (int64_t)nc[0] * (cc[1] + (int64_t)nc[1] * cc[2]);
+                        for (int64_t k = cstart[cl]; k < cstart[cl + 1]; ++k) {
+                            int32_t q = cpart[k];
+                            if (same_set && q == p) continue;
+                            double dv[kMaxDim];
+                            displacement(dim, pos_dom, ndom, p, pos_sup, nsup, q, box_lo, box_hi,
+                                         periodic, dv);
+                            if (dist2_fma(dim, dv) < r2) row.push_back(q);
+                        }
+                    }
+            if (fill) {
+                std::sort(row.begin(), row.end());
+                std::copy(row.begin(), row.end(), col + row_ptr[p]);
+            } else {
+                counts[p] = (int64_t)row.size();
+            }
+        }
+    }
+    if (!fill) {
+        row_ptr[0] = 0;
+        for (int64_t p = 0; p < ndom; ++p) row_ptr[p + 1] = row_ptr[p] + counts[p];
+        return row_ptr[ndom];
+    }
+    return row_ptr[ndom];
+}
+
+// Dcpse.hpp:690-809 for one differential signature beta. Outputs (all sized by
+// the caller): eps[ndom], eps_inv_pow[ndom], K[nnz] (raw kernel, Verlet order),
+// info[ndom] (0 ok, 1 fewer neighbours than monomials, 2 rank-deficient A,
+// 3 eps == 0). dense_ev != 0 forms B = E*V with E stored as a dense n x n
+// matrix exactly as the reference does; 0 uses the diagonal (same values,
+// fewer flops) for a stronger CPU baseline.
+int orc_assemble(int dim, const double* pos_sup, int64_t nsup,
+                 const double* pos_dom, int64_t ndom,
+                 const double* box_lo, const double* box_hi, unsigned periodic,
+                 const int64_t* row_ptr, const int32_t* col,
+                 const unsigned* beta, unsigned order, double h_over_eps, int dense_ev,
+                 int64_t p_begin, int64_t p_end,
+                 double* eps_out, double* eps_inv_pow, double* K, int32_t* info,
+                 int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+    Basis B = make_basis(dim, beta, order);
+    const int m = B.m;
+    std::vector<double> b(m);
+    make_rhs(B, beta, b.data());
+    unsigned border = 0;
+    for (int d = 0; d < dim; ++d) border += beta[d];
+    if (p_end < 0 || p_end > ndom) p_end = ndom;
+    int failures = 0;
+#pragma omp parallel reduction(+ : failures)
+    {
+        std::vector<double> xpq, V, E, Bm, A, a(m);
+#pragma omp for schedule(dynamic, 256)
+        for (int64_t p = p_begin; p < p_end; ++p) {
+            const int64_t rs = row_ptr[p];
+            const int n = (int)(row_ptr[p + 1] - rs);
+            info[p] = 0;
+            // Vandermonde.hpp:93-116: x_pq list, n >= m check, eps
+            xpq.resize((size_t)n * dim);
+            double avg = 0.0;
+            for (int j = 0; j < n; ++j) {
+                displacement(dim, pos_dom, ndom, p, pos_sup, nsup, col[rs + j], box_lo, box_hi,
+                             periodic, &xpq[(size_t)j * dim]);
+                double s = 0.0;
+                for (int d = 0; d < dim; ++d) s += std::fabs(xpq[(size_t)j * dim + d]);
+                avg += s;
+            }
+            if (n < m) {
+                info[p] = 1;
+                eps_out[p] = 0.0;
+                eps_inv_pow[p] = 0.0;
+                for (int j = 0; j < n; ++j) K[rs + j] = 0.0;
+                ++failures;
+                continue;
+            }
+            avg /= n;
+            double eps = avg / h_over_eps;
+            if (eps == 0.0) {
+                info[p] = 3;
+                eps_out[p] = 0.0;
+                eps_inv_pow[p] = 0.0;
+                for (int j = 0; j < n; ++j) K[rs + j] = 0.0;
+                ++failures;
+                continue;
+            }
+            eps_out[p] = eps;
+            eps_inv_pow[p] = 1.0 / intpow(eps, border);
+            // V (n x m, row-major): rows of monomials at x_pq / eps
+            V.resize((size_t)n * m);
+            for (int j = 0; j < n; ++j) {
+                double z[kMaxDim];
+                for (int d = 0; d < dim; ++d) z[d] = xpq[(size_t)j * dim + d] / eps;
+                for (int i = 0; i < m; ++i) V[(size_t)j * m + i] = mono_eval(dim, &B.e[i * dim], 1.0, z);
+            }
+            // E diagonal: exp(-|x_pq|^2 / (2 eps^2))
+            Bm.assign((size_t)n * m, 0.0);
+            if (dense_ev) {
+                E.assign((size_t)n * n, 0.0);
+                for (int j = 0; j < n; ++j) {
+                    const double* d = &xpq[(size_t)j * dim];
+                    double nn = 0.0;
+                    for (int k = 0; k < dim; ++k) nn += d[k] * d[k];
+                    E[(size_t)j * n + j] = std::exp(-nn / (2.0 * eps * eps));
+                }
+                for (int i = 0; i < n; ++i)
+                    for (int k = 0; k < n; ++k) {
+                        double e = E[(size_t)i * n + k];
+                        for (int c = 0; c < m; ++c) Bm[(size_t)i * m + c] += e * V[(size_t)k * m + c];
+                    }
+            } else {
+                for (int j = 0; j < n; ++j) {
+                    const double* d = &xpq[(size_t)j * dim];
+                    double nn = 0.0;
+                    for (int k = 0; k < dim; ++k) nn += d[k] * d[k];
+                    double e = std::exp(-nn / (2.0 * eps * eps));
+                    for (int c = 0; c < m; ++c) Bm[(size_t)j * m + c] = e * V[(size_t)j * m + c];
+                }
+            }
+            // A = B^T B (column-major m x m)
+            A.assign((size_t)m * m, 0.0);
+            for (int j = 0; j < n; ++j) {
+                const double* r = &Bm[(size_t)j * m];
+                for (int c = 0; c < m; ++c)
+                    for (int rr = 0; rr < m; ++rr) A[(size_t)c * m + rr] += r[rr] * r[c];
+            }
+            if (colpiv_qr_solve(m, A, b.data(), a.data()) != 0) {
+                info[p] = 2;
+                ++failures;
+            }
+            // Dcpse.hpp:798-809: K = exp(-|x/eps|^2) * sum_i a_i V(row, i)
+            for (int j = 0; j < n; ++j) {
+                double z2 = 0.0;
+                for (int d = 0; d < dim; ++d) {
+                    double z = xpq[(size_t)j * dim + d] / eps;
+                    z2 += z * z;
+                }
+                double ef = std::exp(-z2);
+                double res = 0.0;
+                for (int i = 0; i < m; ++i) res += a[i] * V[(size_t)j * m + i] * ef;
+                K[rs + j] = res;
+            }
+        }
+    }
+    return failures;
+}
+
+// Dcpse.hpp:467-499: Df_p = epsInvPow_p * sum_q (f_q + s f_p) K_pq with
+// s = -1 for even differential order (including 0), +1 for odd.
+void orc_apply(int64_t ndom, const int64_t* row_ptr, const int32_t* col, const double* K,
+               const double* eps_inv_pow, unsigned diff_order,
+               const double* f_sup, const double* f_dom, double* out, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+    const double sign = (diff_order % 2 == 0) ? -1.0 : 1.0;
+#pragma omp parallel for schedule(static, 1024)
+    for (int64_t p = 0; p < ndom; ++p) {
+        double fxp = sign * f_dom[p];
+        double acc = 0.0;
+        for (int64_t j = row_ptr[p]; j < row_ptr[p + 1]; ++j) acc += (f_sup[col[j]] + fxp) * K[j];
+        out[p] = acc * eps_inv_pow[p];
+    }
+}
+
+// Same gather in single precision over pre-scaled FP32 weights W = eps^-|b| K,
+// difference form sum_q W_pq (f_q - f_p); the CPU counterpart of the device
+// FP32 apply, used only as a timed baseline.
+void orc_apply_f32(int64_t ndom, const int64_t* row_ptr, const int32_t* col, const float* W,
+                   const float* f, float* out, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel for schedule(static, 1024)
+    for (int64_t p = 0; p < ndom; ++p) {
+        float fp = f[p];
+        float acc = 0.0f;
+        for (int64_t j = row_ptr[p]; j < row_ptr[p + 1]; ++j) acc += W[j] * (f[col[j]] - fp);
+        out[p] = acc;
+    }
+}
+
+// DCPSE_op.hpp:69-83 (value_nz): row p of the assembled operator as triplets.
+// For every neighbour j: (p, q_j, K*epsInvPow) and (p, p, getSign()*K*epsInvPow)
+// with getSign() = -1 for even nonzero order, +1 otherwise (Dcpse.hpp:537-545).
+// Writes 2*nnz triplets; duplicates are meant to be summed by the consumer.
+void orc_value_nz(int64_t ndom, const int64_t* row_ptr, const int32_t* col, const double* K,
+                  const double* eps_inv_pow, unsigned diff_order,
+                  int64_t* rows, int64_t* cols, double* vals) {
+    const double sign = (diff_order % 2 == 0 && diff_order != 0) ? -1.0 : 1.0;
+    int64_t t = 0;
+    for (int64_t p = 0; p < ndom; ++p)
+        for (int64_t j = row_ptr[p]; j < row_ptr[p + 1]; ++j) {
+            double c = K[j] * eps_inv_pow[p];
+            rows[t] = p; cols[t] = col[j]; vals[t] = c; ++t;
+            rows[t] = p; cols[t] = p; vals[t] = sign * c; ++t;
+        }
+}
+
+int orc_max_threads() {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+}  // extern "C"

@@ -1,0 +1,155 @@
+"""Parity of the HIP DC-PSE path against the CPU oracle, on seeded inputs.
+
+Tolerances (stated per test):
+  * support (neighbour CSR): bit-exact (integer work).
+  * FP64 kernel weights: per row max|K_gpu - K_ref| / max|K_ref| <= 1e-10.
+  * FP64 apply: max|D_gpu - D_ref| / max|D_ref| <= 1e-10.
+  * FP32 apply vs the FP64 oracle applied to the FP32-rounded input:
+    relative L2 <= 1e-5 and max-abs / max|ref| <= 1e-5.
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _ps(w, dev):
+    from openfpm_numerics_amd import ParticleSet
+
+    bc = [(w["periodic"] >> d) & 1 for d in range(len(w["box_lo"]))]
+    return ParticleSet(w["pos"], w["box_lo"], w["box_hi"], bc=bc, device=dev)
+
+
+def _row_rel(Kg, Kr, row_ptr):
+    n = row_ptr.size - 1
+    rows = np.repeat(np.arange(n), np.diff(row_ptr))
+    num = np.zeros(n)
+    den = np.zeros(n)
+    np.maximum.at(num, rows, np.abs(Kg - Kr))
+    np.maximum.at(den, rows, np.abs(Kr))
+    return float(np.max(num / np.maximum(den, 1e-300)))
+
+
+WORKLOADS = {
+    "2d_lattice_64": lambda W: W.lattice_2d_periodic(64),
+    "2d_jitter_96": lambda W: W.jittered_2d(96, seed=7),
+    "3d_holes_20": lambda W: W.jittered_3d_holes(20, seed=11),
+}
+
+
+@pytest.mark.parametrize("name", sorted(WORKLOADS))
+def test_support_matches_oracle(name, oracle, cuda):
+    from openfpm_numerics_amd import workloads as W
+
+    print(f"[support] {name}", flush=True)
+    w = WORKLOADS[name](W)
+    ps = _ps(w, cuda)
+    vl = ps.getVerlet(w["rcut"])
+    rp, col = oracle.support(w["pos"], w["rcut"], w["box_lo"], w["box_hi"], w["periodic"])
+    assert np.array_equal(vl.row_ptr.cpu().numpy(), rp)
+    assert np.array_equal(vl.col.cpu().numpy(), col)
+
+
+SIGS = {
+    2: [(1, 0), (0, 1), (2, 0), (0, 2), (1, 1)],
+    3: [(1, 0, 0), (0, 0, 1), (2, 0, 0), (0, 1, 1)],
+}
+
+
+@pytest.mark.parametrize("name", sorted(WORKLOADS))
+def test_assembly_matches_oracle(name, oracle, cuda):
+    from openfpm_numerics_amd import Dcpse, workloads as W
+
+    w = WORKLOADS[name](W)
+    ps = _ps(w, cuda)
+    vl = ps.getVerlet(w["rcut"])
+    rp, col = vl.row_ptr.cpu().numpy(), vl.col.cpu().numpy()
+    dim = ps.dims
+    for sig in SIGS[dim]:
+        print(f"[assembly] {name} beta={sig}", flush=True)
+        d = Dcpse(ps, vl, sig, 2, w["rcut"])
+        ref = oracle.assemble(w["pos"], rp, col, sig, 2, w["box_lo"], w["box_hi"], w["periodic"])
+        assert ref.info.max() == 0
+        np.testing.assert_allclose(d.localEps.cpu().numpy(), ref.eps, rtol=1e-13)
+        np.testing.assert_allclose(d.localEpsInvPow.cpu().numpy(), ref.eps_inv_pow, rtol=1e-12)
+        err = _row_rel(d.calcKernels.cpu().numpy(), ref.K, rp)
+        assert err <= 1e-10, f"beta={sig}: row-relative kernel error {err:.3e}"
+
+
+@pytest.mark.parametrize("name", sorted(WORKLOADS))
+def test_laplacian_is_sum_of_directions(name, oracle, cuda):
+    """One fused Laplacian operator == sum over d of the reference's dim operators."""
+    from openfpm_numerics_amd import Laplacian, workloads as W
+
+    w = WORKLOADS[name](W)
+    ps = _ps(w, cuda)
+    vl = ps.getVerlet(w["rcut"])
+    rp, col = vl.row_ptr.cpu().numpy(), vl.col.cpu().numpy()
+    lap = Laplacian(ps, vl, 2, w["rcut"])
+    dim = ps.dims
+    refs = [oracle.assemble(w["pos"], rp, col, tuple(2 if k == d else 0 for k in range(dim)), 2,
+                            w["box_lo"], w["box_hi"], w["periodic"]) for d in range(dim)]
+    Kref = sum(r.K for r in refs)
+    err = _row_rel(lap.dcpse.calcKernels.cpu().numpy(), Kref, rp)
+    assert err <= 1e-10, f"Laplacian kernel error {err:.3e}"
+    f, _, _, lap_exact = W.sin_field(w["pos"])
+    ref_apply = sum(oracle.apply(r, rp, col, f) for r in refs)
+    got = lap(torch.as_tensor(f, device=cuda)).cpu().numpy()
+    rel = np.max(np.abs(got - ref_apply)) / np.max(np.abs(ref_apply))
+    assert rel <= 1e-10, f"FP64 Laplacian apply error {rel:.3e}"
+
+
+@pytest.mark.parametrize("name", sorted(WORKLOADS))
+def test_apply_fp64_fp32(name, oracle, cuda):
+    from openfpm_numerics_amd import Derivative_x, Derivative_y, workloads as W
+
+    w = WORKLOADS[name](W)
+    ps = _ps(w, cuda)
+    vl = ps.getVerlet(w["rcut"])
+    rp, col = vl.row_ptr.cpu().numpy(), vl.col.cpu().numpy()
+    f, dx_exact, dy_exact, _ = W.sin_field(w["pos"])
+    for Op, sig in ((Derivative_x, (1, 0, 0)), (Derivative_y, (0, 1, 0))):
+        sig = sig[: ps.dims]
+        print(f"[apply] {name} {Op.__name__}", flush=True)
+        op = Op(ps, vl, 2, w["rcut"])
+        ref = oracle.assemble(w["pos"], rp, col, sig, 2, w["box_lo"], w["box_hi"], w["periodic"])
+        want = oracle.apply(ref, rp, col, f)
+        got64 = op(torch.as_tensor(f, device=cuda)).cpu().numpy()
+        assert np.max(np.abs(got64 - want)) / np.max(np.abs(want)) <= 1e-10
+        f32 = f.astype(np.float32)
+        want32 = oracle.apply(ref, rp, col, f32.astype(np.float64))
+        got32 = op(torch.as_tensor(f32, device=cuda)).cpu().numpy().astype(np.float64)
+        l2 = np.linalg.norm(got32 - want32) / np.linalg.norm(want32)
+        mx = np.max(np.abs(got32 - want32)) / np.max(np.abs(want32))
+        assert l2 <= 1e-5 and mx <= 1e-5, f"FP32 apply rel L2 {l2:.2e}, max {mx:.2e}"
+
+
+def test_fused_apply_equals_separate(cuda):
+    from openfpm_numerics_amd import Derivative_x, Laplacian, fused, workloads as W
+
+    w = W.jittered_3d_holes(24, seed=5)
+    ps = _ps(w, cuda)
+    vl = ps.getVerlet(w["rcut"])
+    lap = Laplacian(ps, vl, 2, w["rcut"])
+    dx = Derivative_x(ps, vl, 2, w["rcut"])
+    f = torch.as_tensor(W.sin_field(w["pos"])[0], device=cuda)
+    for dt in (torch.float64, torch.float32):
+        a, b = fused(lap, dx, f=f.to(dt))
+        assert torch.equal(a, lap(f.to(dt)))
+        assert torch.equal(b, dx(f.to(dt)))
+
+
+def test_multi_field_apply_equals_single(cuda):
+    from openfpm_numerics_amd import Laplacian, workloads as W
+
+    w = W.jittered_2d(64, seed=3)
+    ps = _ps(w, cuda)
+    vl = ps.getVerlet(w["rcut"])
+    lap = Laplacian(ps, vl, 2, w["rcut"])
+    g = torch.Generator(device="cpu").manual_seed(1)
+    F = torch.randn(13, ps.size_local(), generator=g, dtype=torch.float64).to(cuda)
+    for dt in (torch.float64, torch.float32):
+        batch = lap(F.to(dt))
+        for b in range(F.shape[0]):
+            assert torch.equal(batch[b], lap(F[b].to(dt)))
