@@ -1,0 +1,287 @@
+"""DC-PSE hot-path benchmark on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): 3D, 128^3
+lattice of spacing h = 1/128 jittered by U(-0.1h, 0.1h), 10% of the particles
+removed at random (seed 3), periodic unit cube, rCut = 2.4h -> about 1.89M
+particles with 35..60 neighbours. Convergence order 2.
+
+  setup     neighbour CSR on the GPU (support_ms), FP64 assembly of the
+            Laplacian (one operator, 19 monomials) and Dx (10 monomials)
+            (assembly_ms, HIP events).
+  step      one fused FP32 pass applying Laplacian and Dx to
+            f = sin(2 pi x) sin(2 pi y) sin(2 pi z) (inputs resident in HBM).
+  value     M particle-op evals/s = particles * 2 operators / step time.
+  roofline  the apply kernel against HBM (8 TB/s): algorithmic bytes per
+            launch / mean launch duration (HIP events on the launch stream).
+  cpu       the CPU oracle (faithful restatement of the reference's CPU Dcpse:
+            reference-form FP64 apply, Laplacian as 3 separate operators) timed
+            on the host cores on a bounded sample, same process.
+
+Prints one JSON line. --gpus above 1 exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+METRIC = "M particle-op evals/s + assembly ms, 3D 2.1M pts; FP32 vs FP64 at 1e-5 rel err"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def events_ms(fn, reps, torch):
+    """Mean duration of fn() over reps, each bracketed by its own event pair."""
+    s = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
+    for i in range(reps):
+        s[i].record()
+        fn()
+        e[i].record()
+    torch.cuda.synchronize()
+    d = [s[i].elapsed_time(e[i]) for i in range(reps)]
+    return sum(d) / reps, d
+
+
+def read_traffic():
+    """HBM bytes per apply launch from the committed rocprofv3 --pmc summary, if any."""
+    path = os.path.join(ROOT, "profiles", "apply_fused_lap_dx_3d128.pmc.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as fh:
+        for line in fh:
+            if "=" in line and line.strip().startswith("traffic_"):
+                k, v = line.strip().split("=", 1)
+                try:
+                    vals[k.strip()] = float(v.split()[0])
+                except ValueError:
+                    pass
+    return vals.get("traffic_bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=128, help="lattice edge (128 -> ~1.89M particles)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU baseline work")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
+    ap.add_argument("--no-tune", action="store_true", help="skip launch-shape autotuning")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this DC-PSE build runs on exactly one GPU; "
+                                   f"--gpus {args.gpus} (WORLD_SIZE={world}) is not supported"}))
+        sys.exit(2)
+
+    sys.path.insert(0, ROOT)
+    import numpy as np
+    import torch
+
+    from openfpm_numerics_amd import Derivative_x, Laplacian, ParticleSet, workloads as W
+    from openfpm_numerics_amd.autotune import tune_apply, tune_assembly
+    from openfpm_numerics_amd.dcpse import Dcpse, PreparedApply
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU visible"}))
+        sys.exit(1)
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+
+    log(f"[bench] generating workload n={args.n}")
+    w = W.jittered_3d_holes(args.n, seed=3)
+    N = w["pos"].shape[0]
+    ps = ParticleSet(w["pos"], w["box_lo"], w["box_hi"], bc=[1, 1, 1], device=dev)
+    rcut = w["rcut"]
+
+    # ---- support -------------------------------------------------------------
+    ps.getVerlet(rcut)  # warm (code objects, workspace)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    vl = ps.getVerlet(rcut)
+    torch.cuda.synchronize()
+    support_ms = (time.perf_counter() - t0) * 1e3
+    nnz = vl.nnz
+    lens = vl.row_lengths()
+    log(f"[bench] N={N} nnz={nnz} neighbours min/mean/max = {int(lens.min())}/{nnz / N:.1f}/{int(lens.max())} "
+        f"support {support_ms:.2f} ms")
+
+    # ---- assembly ---------------------------------------------------------------
+    lap = Laplacian(ps, vl, 2, rcut)
+    dx = Derivative_x(ps, vl, 2, rcut)
+    torch.cuda.synchronize()
+
+    def assemble_both():
+        lap.dcpse.initializeStaticSize(ps, ps, check=False)
+        dx.dcpse.initializeStaticSize(ps, ps, check=False)
+
+    if not args.no_tune:
+        (ppw, _), asm_tune = tune_assembly(assemble_both)
+        log(f"[bench] assembly autotune particles/wave -> {ppw}: "
+            + ", ".join(f"{k}:{v:.2f}ms" for k, v in asm_tune.items()))
+    asm_lap_ms, _ = events_ms(lambda: lap.dcpse.initializeStaticSize(ps, ps, check=False), 3, torch)
+    asm_dx_ms, _ = events_ms(lambda: dx.dcpse.initializeStaticSize(ps, ps, check=False), 3, torch)
+    assembly_ms, _ = events_ms(assemble_both, 3, torch)
+    lap.dcpse.check_info()
+    dx.dcpse.check_info()
+    log(f"[bench] assembly Lap {asm_lap_ms:.3f} ms, Dx {asm_dx_ms:.3f} ms, both {assembly_ms:.3f} ms")
+
+    # ---- apply --------------------------------------------------------------------
+    f64, dx_exact, _, lap_exact = W.sin_field(w["pos"])
+    f64_t = torch.as_tensor(f64, device=dev)
+    f32_t = f64_t.to(torch.float32).contiguous()
+    out_lap = torch.empty(N, dtype=torch.float32, device=dev)
+    out_dx = torch.empty(N, dtype=torch.float32, device=dev)
+    step = PreparedApply([lap.dcpse, dx.dcpse], f32_t, [out_lap, out_dx])
+    tuned = None
+    if not args.no_tune:
+        tuned, app_tune = tune_apply(step)
+        step = PreparedApply([lap.dcpse, dx.dcpse], f32_t, [out_lap, out_dx])
+        log("[bench] apply autotune (G,epl,block) -> %s: %s" % (
+            tuned[0], ", ".join(f"{k}:{v * 1e3:.1f}us" for k, v in app_tune.items())))
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    launch_ms, per = events_ms(step, args.steps, torch)
+    wall_s = time.perf_counter() - t0
+    # whole-region timing (back-to-back launches, no per-launch events)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(args.steps):
+        step()
+    b.record()
+    torch.cuda.synchronize()
+    ms_per_step = a.elapsed_time(b) / args.steps
+    evals = 2.0 * N
+    value = evals / (ms_per_step * 1e-3) / 1e6
+
+    # algorithmic bytes of one fused launch: col + 2 weight streams per nonzero,
+    # row_ptr (int64), f once, 2 outputs per particle
+    alg_bytes = nnz * (4 + 4 + 4) + (N + 1) * 8 + N * 4 + 2 * N * 4
+    achieved = alg_bytes / (launch_ms * 1e-3) / 1e9
+    traffic = read_traffic()
+
+    # ---- accuracy -----------------------------------------------------------------
+    lap64, dx64 = [t for t in __import__("openfpm_numerics_amd").apply_ops(
+        [lap.dcpse, dx.dcpse], f32_t.to(torch.float64))]
+    err = {}
+    for name, got, ref in (("laplacian", out_lap, lap64), ("dx", out_dx, dx64)):
+        d = got.double() - ref
+        err[name] = {"rel_l2": float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(ref)),
+                     "max_rel": float(d.abs().max() / ref.abs().max())}
+    ana = {
+        "laplacian_max_rel": float((lap64.cpu().numpy() - lap_exact).__abs__().max() / np.abs(lap_exact).max()),
+        "dx_max_rel": float((dx64.cpu().numpy() - dx_exact).__abs__().max() / np.abs(dx_exact).max()),
+    }
+    log(f"[bench] step {ms_per_step * 1e3:.1f} us ({launch_ms * 1e3:.1f} us per launch) -> {value:.1f} "
+        f"M evals/s, {achieved:.0f} GB/s ({achieved / HBM_PEAK_GBS:.1%} of HBM); fp32 err {err}")
+
+    # assembly roofline (FP64 flops of the reference's algorithm per particle)
+    def asm_flops(n, m):
+        return n * m * (m + 1) + m ** 3 / 3.0 + 2 * m * m + 3 * n * m
+    nmean = nnz / N
+    flops_lap = N * 3 * asm_flops(nmean, 19)
+    flops_dx = N * asm_flops(nmean, 10)
+    asm_tflops = (flops_lap + flops_dx) / (assembly_ms * 1e-3) / 1e12
+
+    # ---- CPU baseline -----------------------------------------------------------------
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        cores = min(16, os.cpu_count() or 1)
+        rp = vl.row_ptr.cpu().numpy()
+        col = vl.col.cpu().numpy()
+        sigs = [(2, 0, 0), (0, 2, 0), (0, 0, 2), (1, 0, 0)]
+        asms = []
+        for sgn in sigs:
+            d = Dcpse(ps, vl, sgn, 2, rcut)
+            asms.append(O.Assembly(d.localEps.cpu().numpy(), d.localEpsInvPow.cpu().numpy(),
+                                   d.calcKernels.cpu().numpy(), d.info.cpu().numpy(), sgn, 2))
+            del d
+        f32h = f32_t.cpu().numpy().astype(np.float64)
+        O.apply(asms[0], rp, col, f32h, nthreads=cores)  # warm
+        reps = 0
+        t0 = time.perf_counter()
+        while True:
+            for a_ in asms:
+                O.apply(a_, rp, col, f32h, nthreads=cores)
+            reps += 1
+            el = time.perf_counter() - t0
+            if el > 0.4 * args.cpu_seconds or reps >= 20:
+                break
+        cpu_step_s = el / reps
+        # assembly on a particle sample, extrapolated to N
+        S = 2000
+        t0 = time.perf_counter()
+        for sgn in sigs:
+            O.assemble(w["pos"], rp, col, sgn, 2, w["box_lo"], w["box_hi"], w["periodic"],
+                       p_range=(0, S), nthreads=cores)
+        t_small = time.perf_counter() - t0
+        S = int(min(N, max(S, S * 0.5 * args.cpu_seconds / max(t_small, 1e-3))))
+        t0 = time.perf_counter()
+        for sgn in sigs:
+            O.assemble(w["pos"], rp, col, sgn, 2, w["box_lo"], w["box_hi"], w["periodic"],
+                       p_range=(0, S), nthreads=cores)
+        t_asm = time.perf_counter() - t0
+        cpu_asm_ms = t_asm / S * N * 1e3
+        cpu = {"value": evals / cpu_step_s / 1e6, "unit": "M particle-op evals/s", "cores": cores,
+               "kind": "port",
+               "sample": f"apply: full N={N}, Laplacian as 3 FP64 operators + Dx, reference form, {reps} reps; "
+                         f"assembly: first {S} particles x 4 operators (dense E*V, pivoted QR), "
+                         f"extrapolated to N",
+               "assembly_ms_extrapolated": cpu_asm_ms,
+               "step_ms": cpu_step_s * 1e3}
+        log(f"[bench] CPU ({cores} threads): step {cpu_step_s * 1e3:.1f} ms -> {cpu['value']:.1f} M evals/s; "
+            f"assembly {cpu_asm_ms:.0f} ms (extrapolated from {S})")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "M particle-op evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "3d_lattice128_jitter0.1h_minus10pct_periodic_rcut2.4h_order2_lap+dx",
+                   "particles": N, "nnz": nnz, "operators": ["Laplacian", "Derivative_x"],
+                   "apply_precision": "fp32", "assembly_precision": "fp64",
+                   "apply_launch": {"group_lanes": tuned[0][0], "elems_per_lane": tuned[0][1],
+                                    "block": tuned[0][2]} if tuned else None},
+        "assembly_ms": assembly_ms,
+        "assembly_ms_by_operator": {"laplacian": asm_lap_ms, "dx": asm_dx_ms},
+        "support_ms": support_ms,
+        "fp32_error_vs_fp64": err,
+        "fp64_error_vs_analytic": ana,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_apply<float,G,EPL,2,1,diff>", "alg_bytes_per_launch": alg_bytes,
+                     "launch_ms": launch_ms},
+        "roofline_assembly": {"bound": "fp64", "achieved": asm_tflops, "peak": FP64_PEAK_TFLOPS,
+                              "unit": "TFLOP/s", "frac": asm_tflops / FP64_PEAK_TFLOPS,
+                              "flops": flops_lap + flops_dx},
+        "cpu_baseline": cpu,
+        "wall_s_timed_region": wall_s,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,60 @@
+"""Launch-shape autotuner for the apply and assembly kernels.
+
+Times each candidate with HIP events on the current stream (the stream the
+library launches on) and keeps the fastest in the library context:
+  apply:    (lanes per row, nonzeros per lane) shapes x threads per block {256, 512}
+  assembly: particles each wavefront assembles in sequence {1, 2, 4, 8, 16}
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+
+
+def _time(fn, reps):
+    fn()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(reps):
+        fn()
+    stop.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / reps
+
+
+APPLY_SHAPES = ((4, 16), (8, 8), (16, 4), (32, 2), (64, 1), (8, 16), (16, 8))
+
+
+def tune_apply(fn, shapes=APPLY_SHAPES, blocks=(256, 512), reps=10, device=0):
+    """fn() runs one apply; returns ((G, epl, block), ms) of the fastest shape."""
+    ctx = _lib.context(device)
+    best = None
+    results = {}
+    for g, e in shapes:
+        for b in blocks:
+            if b % g:
+                continue
+            ctx.check(ctx.lib.dcpse_set_apply_config(ctx.h, g, e, b), "dcpse_set_apply_config")
+            ms = _time(fn, reps)
+            results[(g, e, b)] = ms
+            if best is None or ms < best[1]:
+                best = ((g, e, b), ms)
+    ctx.check(ctx.lib.dcpse_set_apply_config(ctx.h, *best[0]), "dcpse_set_apply_config")
+    return best, results
+
+
+def tune_assembly(fn, ppws=(1, 2, 4, 8, 16), reps=2, device=0):
+    """fn() runs one assembly; returns (particles_per_wave, ms) of the fastest."""
+    ctx = _lib.context(device)
+    best = None
+    results = {}
+    for p in ppws:
+        ctx.check(ctx.lib.dcpse_set_assembly_config(ctx.h, 1, p), "dcpse_set_assembly_config")
+        ms = _time(fn, reps)
+        results[p] = ms
+        if best is None or ms < best[1]:
+            best = (p, ms)
+    ctx.check(ctx.lib.dcpse_set_assembly_config(ctx.h, 1, best[0]), "dcpse_set_assembly_config")
+    return best, results

@@ -44,8 +44,15 @@ struct ApplyArgs {
 
 enum ApplyMode { kDiff = 0, kExplicitDiag = 1 };
 
-template <typename T, int G, int NOPS, int NF, int MODE>
-__global__ __launch_bounds__(256) void k_apply(const ApplyArgs<T> a) {
+constexpr int kApplyMaxBlock = 512;  // launch bound of k_apply; larger blocks are rejected
+
+// G lanes cooperate on one row. The first G*EPL nonzeros of the row are
+// handled with every load issued before any is consumed: EPL col and
+// NOPS*EPL weight loads per lane (indices clamped into the row, so no branch
+// separates them), then EPL gathers of f, then the FMAs. Longer rows finish
+// in a strided loop. Invalid (clamped) slots are masked out by a select.
+template <typename T, int G, int EPL, int NOPS, int NF, int MODE>
+__global__ __launch_bounds__(kApplyMaxBlock) void k_apply(const ApplyArgs<T> a) {
     // XCD-aware remap: hardware block b runs on XCD b % 8; give each XCD a
     // contiguous range of logical blocks.
     const int64_t hb = blockIdx.x;
@@ -65,18 +72,43 @@ __global__ __launch_bounds__(256) void k_apply(const ApplyArgs<T> a) {
     for (int o = 0; o < NOPS; ++o)
 #pragma unroll
         for (int b = 0; b < NF; ++b) acc[o][b] = T(0);
-#pragma unroll 4
-    for (int64_t j = rs + gl; j < re; j += G) {
-        const int32_t q = __builtin_nontemporal_load(a.col + j);
-        T w[NOPS];
+    if (re > rs) {
+        int32_t q[EPL];
+        T w[NOPS][EPL];
+        bool v[EPL];
 #pragma unroll
-        for (int o = 0; o < NOPS; ++o) w[o] = __builtin_nontemporal_load(a.W[o] + j);
+        for (int k = 0; k < EPL; ++k) {
+            const int64_t j = rs + gl + k * G;
+            v[k] = j < re;
+            const int64_t jj = v[k] ? j : re - 1;
+            q[k] = __builtin_nontemporal_load(a.col + jj);
+#pragma unroll
+            for (int o = 0; o < NOPS; ++o) w[o][k] = __builtin_nontemporal_load(a.W[o] + jj);
+        }
 #pragma unroll
         for (int b = 0; b < NF; ++b) {
-            const T fq = a.f[b * a.ld_f + q];
-            const T v = MODE == kDiff ? fq - fp[b] : fq;
+            T fq[EPL];
 #pragma unroll
-            for (int o = 0; o < NOPS; ++o) acc[o][b] = fma(w[o], v, acc[o][b]);
+            for (int k = 0; k < EPL; ++k) fq[k] = a.f[b * a.ld_f + q[k]];
+#pragma unroll
+            for (int k = 0; k < EPL; ++k) {
+                const T d = MODE == kDiff ? fq[k] - fp[b] : fq[k];
+#pragma unroll
+                for (int o = 0; o < NOPS; ++o) acc[o][b] = v[k] ? fma(w[o][k], d, acc[o][b]) : acc[o][b];
+            }
+        }
+        for (int64_t j = rs + gl + G * EPL; j < re; j += G) {
+            const int32_t qq = __builtin_nontemporal_load(a.col + j);
+            T ww[NOPS];
+#pragma unroll
+            for (int o = 0; o < NOPS; ++o) ww[o] = __builtin_nontemporal_load(a.W[o] + j);
+#pragma unroll
+            for (int b = 0; b < NF; ++b) {
+                const T fq = a.f[b * a.ld_f + qq];
+                const T d = MODE == kDiff ? fq - fp[b] : fq;
+#pragma unroll
+                for (int o = 0; o < NOPS; ++o) acc[o][b] = fma(ww[o], d, acc[o][b]);
+            }
         }
     }
 #pragma unroll
@@ -90,85 +122,98 @@ __global__ __launch_bounds__(256) void k_apply(const ApplyArgs<T> a) {
         for (int o = 0; o < NOPS; ++o)
 #pragma unroll
             for (int b = 0; b < NF; ++b) {
-                T v = acc[o][b];
-                if (MODE == kExplicitDiag) v = fma(a.diag[row], fp[b], v);
+                T val = acc[o][b];
+                if (MODE == kExplicitDiag) val = fma(a.diag[row], fp[b], val);
                 T* dst = a.out[o] + b * a.ld_out + row;
-                v *= a.alpha;
-                if (a.beta != T(0)) v = fma(a.beta, *dst, v);
-                *dst = v;
+                val *= a.alpha;
+                if (a.beta != T(0)) val = fma(a.beta, *dst, val);
+                *dst = val;
             }
     }
 }
 
-template <typename T, int G, int NOPS, int NF, int MODE>
+template <typename T, int G, int EPL, int NOPS, int NF, int MODE>
 static void launch_apply(ApplyArgs<T> a, int block, hipStream_t s) {
     const int rows_per_block = block / G;
     a.blocks_needed = (a.n_rows + rows_per_block - 1) / rows_per_block;
     a.per_xcd = (a.blocks_needed + 7) / 8;
     const int64_t grid = a.per_xcd * 8;
-    k_apply<T, G, NOPS, NF, MODE><<<(unsigned)grid, block, 0, s>>>(a);
+    k_apply<T, G, EPL, NOPS, NF, MODE><<<(unsigned)grid, block, 0, s>>>(a);
 }
 
-template <typename T, int G, int MODE>
+template <typename T, int G, int EPL, int MODE>
 static bool dispatch_nops_nf(const ApplyArgs<T>& a, int nops, int nf, int block, hipStream_t s) {
     if (nf == 1) {
         switch (nops) {
-            case 1: launch_apply<T, G, 1, 1, MODE>(a, block, s); return true;
-            case 2: launch_apply<T, G, 2, 1, MODE>(a, block, s); return true;
-            case 3: launch_apply<T, G, 3, 1, MODE>(a, block, s); return true;
-            case 4: launch_apply<T, G, 4, 1, MODE>(a, block, s); return true;
+            case 1: launch_apply<T, G, EPL, 1, 1, MODE>(a, block, s); return true;
+            case 2: launch_apply<T, G, EPL, 2, 1, MODE>(a, block, s); return true;
+            case 3: launch_apply<T, G, EPL, 3, 1, MODE>(a, block, s); return true;
+            case 4: launch_apply<T, G, EPL, 4, 1, MODE>(a, block, s); return true;
         }
         return false;
     }
     if (nops != 1) return false;
+    // fields per pass are capped at 32 / EPL (see max_fields) to bound registers
     switch (nf) {
-        case 2: launch_apply<T, G, 1, 2, MODE>(a, block, s); return true;
-        case 4: launch_apply<T, G, 1, 4, MODE>(a, block, s); return true;
-        case 8: launch_apply<T, G, 1, 8, MODE>(a, block, s); return true;
-        case 16: launch_apply<T, G, 1, 16, MODE>(a, block, s); return true;
+        case 2: if constexpr (2 * EPL <= 32) { launch_apply<T, G, EPL, 1, 2, MODE>(a, block, s); return true; }
+                break;
+        case 4: if constexpr (4 * EPL <= 32) { launch_apply<T, G, EPL, 1, 4, MODE>(a, block, s); return true; }
+                break;
+        case 8: if constexpr (8 * EPL <= 32) { launch_apply<T, G, EPL, 1, 8, MODE>(a, block, s); return true; }
+                break;
+        case 16: if constexpr (16 * EPL <= 32) { launch_apply<T, G, EPL, 1, 16, MODE>(a, block, s); return true; }
+                break;
     }
     return false;
 }
 
+// (lanes per row, elements per lane issued up front): every shape covers 64
+// nonzeros per pass except the last two, which cover 128.
 template <typename T, int MODE>
-static bool dispatch_group(const ApplyArgs<T>& a, int G, int nops, int nf, int block, hipStream_t s) {
-    switch (G) {
-        case 4: return dispatch_nops_nf<T, 4, MODE>(a, nops, nf, block, s);
-        case 8: return dispatch_nops_nf<T, 8, MODE>(a, nops, nf, block, s);
-        case 16: return dispatch_nops_nf<T, 16, MODE>(a, nops, nf, block, s);
-        case 32: return dispatch_nops_nf<T, 32, MODE>(a, nops, nf, block, s);
-        case 64: return dispatch_nops_nf<T, 64, MODE>(a, nops, nf, block, s);
-    }
+static bool dispatch_group(const ApplyArgs<T>& a, int G, int epl, int nops, int nf, int block, hipStream_t s) {
+    if (G == 4 && epl == 16) return dispatch_nops_nf<T, 4, 16, MODE>(a, nops, nf, block, s);
+    if (G == 8 && epl == 8) return dispatch_nops_nf<T, 8, 8, MODE>(a, nops, nf, block, s);
+    if (G == 16 && epl == 4) return dispatch_nops_nf<T, 16, 4, MODE>(a, nops, nf, block, s);
+    if (G == 32 && epl == 2) return dispatch_nops_nf<T, 32, 2, MODE>(a, nops, nf, block, s);
+    if (G == 64 && epl == 1) return dispatch_nops_nf<T, 64, 1, MODE>(a, nops, nf, block, s);
+    if (G == 8 && epl == 16) return dispatch_nops_nf<T, 8, 16, MODE>(a, nops, nf, block, s);
+    if (G == 16 && epl == 8) return dispatch_nops_nf<T, 16, 8, MODE>(a, nops, nf, block, s);
     return false;
 }
 
-// Lanes per row from the mean row length: enough lanes to cover a row in
-// about two strides. The mean is read once per (row_ptr, n_rows) and cached in the context, so
-// repeated applies of one operator never synchronise the stream.
-static int auto_group(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr) {
-    if (ctx->apply_group > 0) return ctx->apply_group;
-    if (ctx->grp_rows == row_ptr && ctx->grp_n == n_rows) return ctx->grp_g;
+struct ApplyShape {
+    int G, epl;
+};
+
+// Launch shape from the mean row length: rows of up to 64 nonzeros take 8
+// lanes x 8 elements, longer rows 16 x 8. The mean is read once per
+// (row_ptr, n_rows) and cached in the context, so repeated applies of one
+// operator never synchronise the stream. dcpse_set_apply_config overrides it.
+static ApplyShape auto_shape(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr) {
+    if (ctx->apply_group > 0) return {ctx->apply_group, ctx->apply_epl};
+    if (ctx->grp_rows == row_ptr && ctx->grp_n == n_rows) return {ctx->grp_g, ctx->grp_epl};
     int64_t nnz = 0;
-    int g = 16;
+    ApplyShape sh{8, 8};
     if (n_rows > 0 &&
         hipMemcpyAsync(&nnz, row_ptr + n_rows, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
         hipStreamSynchronize(ctx->stream) == hipSuccess) {
         double mean = (double)nnz / (double)n_rows;
-        g = mean <= 6 ? 4 : mean <= 12 ? 8 : mean <= 40 ? 16 : 32;
+        if (mean > 64) sh = {16, 8};
     }
     ctx->grp_rows = row_ptr;
     ctx->grp_n = n_rows;
-    ctx->grp_g = g;
-    return g;
+    ctx->grp_g = sh.G;
+    ctx->grp_epl = sh.epl;
+    return sh;
 }
 
-// Field chunk for n_fields: the largest compiled NF dividing the remaining count.
-static int field_chunk(int remaining, int nops) {
+// Fields per pass: the largest compiled NF <= the remaining count and
+// <= 32 / elems_per_lane (every field keeps EPL gathered values live).
+static int field_chunk(int remaining, int nops, int epl) {
     if (nops != 1) return 1;
-    if (remaining >= 16) return 16;
-    if (remaining >= 8) return 8;
-    if (remaining >= 4) return 4;
-    if (remaining >= 2) return 2;
+    const int cap = 32 / epl;
+    for (int nf = 16; nf >= 2; nf >>= 1)
+        if (remaining >= nf && nf <= cap) return nf;
     return 1;
 }
 
@@ -188,10 +233,10 @@ static dcpse_status apply_impl(dcpse_ctx* ctx, int64_t n_rows, const int64_t* ro
         if (!W[o] || !out[o]) return fail(ctx, DCPSE_ERR_INVALID, "null operator pointer");
     if (n_rows == 0) return DCPSE_OK;
     DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
-    const int G = auto_group(ctx, n_rows, row_ptr);
+    const ApplyShape sh = auto_shape(ctx, n_rows, row_ptr);
     const int block = ctx->apply_block;
-    if (block % G != 0 || block < 64 || block > 1024)
-        return fail(ctx, DCPSE_ERR_INVALID, "block %d incompatible with group %d", block, G);
+    if (block % sh.G != 0 || block < 64 || block > kApplyMaxBlock)
+        return fail(ctx, DCPSE_ERR_INVALID, "block %d incompatible with group %d", block, sh.G);
     ApplyArgs<T> a;
     a.n_rows = n_rows;
     a.row_ptr = row_ptr;
@@ -208,12 +253,13 @@ static dcpse_status apply_impl(dcpse_ctx* ctx, int64_t n_rows, const int64_t* ro
     }
     int b0 = 0;
     while (b0 < nfields) {
-        const int nf = field_chunk(nfields - b0, nops);
+        const int nf = field_chunk(nfields - b0, nops, sh.epl);
         a.f = f + (int64_t)b0 * ld_f;
         a.f_self = (f_self ? f_self + (int64_t)b0 * ld_self : a.f);
         for (int o = 0; o < nops; ++o) a.out[o] = out[o] + (int64_t)b0 * ld_out;
-        if (!dispatch_group<T, MODE>(a, G, nops, nf, block, ctx->stream))
-            return fail(ctx, DCPSE_ERR_UNSUPPORTED, "no apply kernel for group=%d ops=%d fields=%d", G, nops, nf);
+        if (!dispatch_group<T, MODE>(a, sh.G, sh.epl, nops, nf, block, ctx->stream))
+            return fail(ctx, DCPSE_ERR_UNSUPPORTED, "no apply kernel for group=%d x %d ops=%d fields=%d", sh.G,
+                        sh.epl, nops, nf);
         DCPSE_LAUNCH_CHECK(ctx);
         b0 += nf;
     }

@@ -53,14 +53,21 @@ dcpse_status dcpse_sync(dcpse_ctx* ctx) {
     return DCPSE_OK;
 }
 
-dcpse_status dcpse_set_apply_config(dcpse_ctx* ctx, int32_t group_lanes, int32_t block_threads) {
+dcpse_status dcpse_set_apply_config(dcpse_ctx* ctx, int32_t group_lanes, int32_t elems_per_lane,
+                                    int32_t block_threads) {
     if (!ctx) return DCPSE_ERR_INVALID;
-    if (group_lanes != 0 && group_lanes != 4 && group_lanes != 8 && group_lanes != 16 && group_lanes != 32 &&
-        group_lanes != 64)
-        return fail(ctx, DCPSE_ERR_INVALID, "group_lanes must be 0 or a power of two in [4, 64]");
-    if (block_threads < 64 || block_threads > 1024 || block_threads % 64)
-        return fail(ctx, DCPSE_ERR_INVALID, "block_threads must be a multiple of 64 in [64, 1024]");
+    const bool ok = group_lanes == 0 || (group_lanes == 4 && elems_per_lane == 16) ||
+                    (group_lanes == 8 && (elems_per_lane == 8 || elems_per_lane == 16)) ||
+                    (group_lanes == 16 && (elems_per_lane == 4 || elems_per_lane == 8)) ||
+                    (group_lanes == 32 && elems_per_lane == 2) || (group_lanes == 64 && elems_per_lane == 1);
+    if (!ok)
+        return fail(ctx, DCPSE_ERR_INVALID,
+                    "unsupported apply shape %d x %d (compiled: 4x16, 8x8, 8x16, 16x4, 16x8, 32x2, 64x1)",
+                    group_lanes, elems_per_lane);
+    if (block_threads < 64 || block_threads > 512 || block_threads % 64)
+        return fail(ctx, DCPSE_ERR_INVALID, "block_threads must be a multiple of 64 in [64, 512]");
     ctx->apply_group = group_lanes;
+    ctx->apply_epl = elems_per_lane;
     ctx->apply_block = block_threads;
     return DCPSE_OK;
 }

@@ -13,11 +13,13 @@ struct dcpse_ctx {
     int device = 0;
     hipStream_t stream = nullptr;  // nullptr = the device's default stream
     int num_cus = 256;
-    int apply_group = 0;   // 0 = automatic
+    int apply_group = 0;   // lanes per row, 0 = automatic
+    int apply_epl = 8;     // nonzeros per lane issued up front
     int apply_block = 256;
-    const void* grp_rows = nullptr;  // cache of the automatic apply group
+    const void* grp_rows = nullptr;  // cache of the automatic apply shape
     int64_t grp_n = -1;
-    int grp_g = 16;
+    int grp_g = 8;
+    int grp_epl = 8;
     int asm_wpb = 1;       // waves per block in the assembly kernel
     int asm_ppb = 8;       // particles per block (per wave) batch
     void* ws = nullptr;    // grow-only device workspace

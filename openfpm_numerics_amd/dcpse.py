@@ -104,7 +104,8 @@ class Dcpse:
     def kerOffsets(self) -> torch.Tensor:
         return self.verletList.row_ptr
 
-    def initializeStaticSize(self, particlesSupport: ParticleSet, particlesDomain: ParticleSet):
+    def initializeStaticSize(self, particlesSupport: ParticleSet, particlesDomain: ParticleSet,
+                             check: bool = True):
         vl = self.verletList
         sup, dom = particlesSupport, particlesDomain
         dev = sup.device
@@ -137,6 +138,12 @@ class Dcpse:
         self._T = {}
         self.info = info
         self.update_ctr = sup.map_ctr
+        if check:
+            self.check_info()
+
+    def check_info(self):
+        """Raises like the reference on the first particle whose assembly failed."""
+        info, vl = self.info, self.verletList
         bad = info != 0
         if bool(bad.any()):
             first = int(torch.nonzero(bad)[0])
@@ -372,3 +379,33 @@ def apply_ops(ops, f: torch.Tensor, outs=None, f_domain: torch.Tensor | None = N
              _lib.ptr(f2), n, nb, _lib.ptr(fd), n_rows if fd is not None else n, _lib.ptr_array(outs), n_rows,
              alpha, beta)
     return [o.reshape(*f.shape[:-1], n_rows) for o in outs]
+
+
+class PreparedApply:
+    """A fused apply with every pointer bound once: calling it costs one C call.
+
+    For timing loops and graph capture; the tensors passed in must stay alive
+    and keep their storage. ``f`` is (N,) or (B, N); ``outs`` one tensor per op.
+    """
+
+    def __init__(self, ops, f: torch.Tensor, outs, alpha=1.0, beta=0.0):
+        vl = ops[0].verletList
+        if any(op.verletList is not vl for op in ops):
+            raise ValueError("fused operators must share one Verlet list")
+        if not f.is_contiguous() or any(not o.is_contiguous() for o in outs):
+            raise ValueError("contiguous tensors only")
+        f2 = f.reshape(-1, f.shape[-1])
+        nb, n = f2.shape
+        self._keep = (ops, f, outs, [op.weights(f.dtype) for op in ops])
+        self.ctx = _lib.context(f.device.index or 0)
+        self.fn = self.ctx.lib.dcpse_apply_f32 if f.dtype == torch.float32 else self.ctx.lib.dcpse_apply_f64
+        self.W = _lib.ptr_array(self._keep[3])
+        self.O = _lib.ptr_array(outs)
+        self.args = (self.ctx.h, vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(vl.col), len(ops), self.W,
+                     _lib.ptr(f2), n, nb, None, n, self.O, vl.n_rows, alpha, beta)
+        self.ctx.bind_current_stream()
+
+    def __call__(self):
+        st = self.fn(*self.args)
+        if st:
+            self.ctx.check(st, "dcpse_apply")
