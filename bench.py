@@ -76,6 +76,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
     ap.add_argument("--no-tune", action="store_true", help="skip launch-shape autotuning")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run torch's copy kernel on 1 GiB (HBM counter calibration)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -96,6 +98,15 @@ def main():
         sys.exit(1)
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.calibrate_copy:
+        n = 1 << 28  # 2^28 floats = 1 GiB read + 1 GiB written per copy
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n}))
+        return
 
     log(f"[bench] generating workload n={args.n}")
     w = W.jittered_3d_holes(args.n, seed=3)
@@ -146,7 +157,7 @@ def main():
     if not args.no_tune:
         tuned, app_tune = tune_apply(step)
         step = PreparedApply([lap.dcpse, dx.dcpse], f32_t, [out_lap, out_dx])
-        log("[bench] apply autotune (G,epl,block) -> %s: %s" % (
+        log("[bench] apply autotune (G,epl,vec,block) -> %s: %s" % (
             tuned[0], ", ".join(f"{k}:{v * 1e3:.1f}us" for k, v in app_tune.items())))
     for _ in range(args.warmup):
         step()
@@ -264,7 +275,7 @@ def main():
                    "particles": N, "nnz": nnz, "operators": ["Laplacian", "Derivative_x"],
                    "apply_precision": "fp32", "assembly_precision": "fp64",
                    "apply_launch": {"group_lanes": tuned[0][0], "elems_per_lane": tuned[0][1],
-                                    "block": tuned[0][2]} if tuned else None},
+                                    "vector_width": tuned[0][2], "block": tuned[0][3]} if tuned else None},
         "assembly_ms": assembly_ms,
         "assembly_ms_by_operator": {"laplacian": asm_lap_ms, "dx": asm_dx_ms},
         "support_ms": support_ms,

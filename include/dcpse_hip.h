@@ -78,12 +78,15 @@ dcpse_status dcpse_ctx_set_stream(dcpse_ctx* ctx, void* hip_stream);
 void* dcpse_ctx_get_stream(dcpse_ctx* ctx);
 const char* dcpse_last_error(const dcpse_ctx* ctx);
 dcpse_status dcpse_sync(dcpse_ctx* ctx);
-/* Apply-kernel launch shape: lanes cooperating on one row x nonzeros each
- * lane loads before consuming (compiled: 4x16, 8x8, 8x16, 16x4, 16x8, 32x2,
- * 64x1; group_lanes 0 = automatic) and threads per block (a multiple of 64
- * in 64..512). */
+/* Apply-kernel launch shape: lanes cooperating on one row, nonzeros each
+ * lane loads before consuming, and load width in elements (1: 4-byte loads;
+ * 4: 16-byte loads, FP32 only). Compiled shapes, width 1: 4x16, 8x8, 8x16,
+ * 16x4, 16x8, 32x2, 64x1; width 4: 4x8, 8x4, 8x8, 16x4, 16x8, 32x4. group_lanes 0 =
+ * automatic. FP64 applies ignore width 4 and use the width-1 shape with the
+ * same lanes and elements when compiled (automatic otherwise).
+ * block_threads: a multiple of 64 in 64..512. */
 dcpse_status dcpse_set_apply_config(dcpse_ctx* ctx, int32_t group_lanes, int32_t elems_per_lane,
-                                    int32_t block_threads);
+                                    int32_t vector_width, int32_t block_threads);
 /* Assembly launch shape: wavefronts per block (must be 1) and particles each
  * wavefront assembles in sequence (0 = automatic). */
 dcpse_status dcpse_set_assembly_config(dcpse_ctx* ctx, int32_t waves_per_block, int32_t particles_per_wave);

@@ -34,7 +34,7 @@ SIGNATURES = {
     "dcpse_ctx_get_stream": (_vp, [_vp]),
     "dcpse_last_error": (C.c_char_p, [_vp]),
     "dcpse_sync": (_i32, [_vp]),
-    "dcpse_set_apply_config": (_i32, [_vp, _i32, _i32, _i32]),
+    "dcpse_set_apply_config": (_i32, [_vp, _i32, _i32, _i32, _i32]),
     "dcpse_set_assembly_config": (_i32, [_vp, _i32, _i32]),
     "dcpse_basis_size": (_i32, [_i32, _vp, _u32]),
     "dcpse_basis": (_i32, [_i32, _vp, _u32, _vp, _i32]),

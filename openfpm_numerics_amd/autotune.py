@@ -24,23 +24,25 @@ def _time(fn, reps):
     return start.elapsed_time(stop) / reps
 
 
-APPLY_SHAPES = ((4, 16), (8, 8), (16, 4), (32, 2), (64, 1), (8, 16), (16, 8))
+# (lanes per row, elements per lane, load width)
+APPLY_SHAPES = ((4, 8, 4), (8, 4, 4), (8, 8, 4), (16, 4, 4), (16, 8, 4), (32, 4, 4),
+                (4, 16, 1), (8, 8, 1), (16, 4, 1), (32, 2, 1), (64, 1, 1), (8, 16, 1), (16, 8, 1))
 
 
 def tune_apply(fn, shapes=APPLY_SHAPES, blocks=(256, 512), reps=10, device=0):
-    """fn() runs one apply; returns ((G, epl, block), ms) of the fastest shape."""
+    """fn() runs one apply; returns ((G, epl, vec, block), ms) of the fastest shape."""
     ctx = _lib.context(device)
     best = None
     results = {}
-    for g, e in shapes:
+    for g, e, v in shapes:
         for b in blocks:
             if b % g:
                 continue
-            ctx.check(ctx.lib.dcpse_set_apply_config(ctx.h, g, e, b), "dcpse_set_apply_config")
+            ctx.check(ctx.lib.dcpse_set_apply_config(ctx.h, g, e, v, b), "dcpse_set_apply_config")
             ms = _time(fn, reps)
-            results[(g, e, b)] = ms
+            results[(g, e, v, b)] = ms
             if best is None or ms < best[1]:
-                best = ((g, e, b), ms)
+                best = ((g, e, v, b), ms)
     ctx.check(ctx.lib.dcpse_set_apply_config(ctx.h, *best[0]), "dcpse_set_apply_config")
     return best, results
 

@@ -16,11 +16,21 @@
 // remapped so that each XCD walks one contiguous slice of rows, keeping the
 // f halo of a slice in that XCD's L2.
 #include <algorithm>
+#include <type_traits>
 
 #include "dcpse_common.hpp"
 #include "dcpse_internal.hpp"
 
 namespace dcpse {
+
+// Loads of the col/W streams. Every byte of them is used once per apply, but
+// rows are not aligned to cache lines, so a line can be split between two load
+// instructions; the policy decides whether the second one finds it in L2.
+#ifdef DCPSE_APPLY_NONTEMPORAL
+#define DCPSE_STREAM_LOAD(p) __builtin_nontemporal_load(p)
+#else
+#define DCPSE_STREAM_LOAD(p) (*(p))
+#endif
 
 constexpr int kMaxOps = 4;
 
@@ -81,10 +91,11 @@ __global__ __launch_bounds__(kApplyMaxBlock) void k_apply(const ApplyArgs<T> a) 
             const int64_t j = rs + gl + k * G;
             v[k] = j < re;
             const int64_t jj = v[k] ? j : re - 1;
-            q[k] = __builtin_nontemporal_load(a.col + jj);
+            q[k] = DCPSE_STREAM_LOAD(a.col + jj);
 #pragma unroll
-            for (int o = 0; o < NOPS; ++o) w[o][k] = __builtin_nontemporal_load(a.W[o] + jj);
+            for (int o = 0; o < NOPS; ++o) w[o][k] = DCPSE_STREAM_LOAD(a.W[o] + jj);
         }
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int b = 0; b < NF; ++b) {
             T fq[EPL];
@@ -98,10 +109,10 @@ __global__ __launch_bounds__(kApplyMaxBlock) void k_apply(const ApplyArgs<T> a) 
             }
         }
         for (int64_t j = rs + gl + G * EPL; j < re; j += G) {
-            const int32_t qq = __builtin_nontemporal_load(a.col + j);
+            const int32_t qq = DCPSE_STREAM_LOAD(a.col + j);
             T ww[NOPS];
 #pragma unroll
-            for (int o = 0; o < NOPS; ++o) ww[o] = __builtin_nontemporal_load(a.W[o] + j);
+            for (int o = 0; o < NOPS; ++o) ww[o] = DCPSE_STREAM_LOAD(a.W[o] + j);
 #pragma unroll
             for (int b = 0; b < NF; ++b) {
                 const T fq = a.f[b * a.ld_f + qq];
@@ -132,13 +143,121 @@ __global__ __launch_bounds__(kApplyMaxBlock) void k_apply(const ApplyArgs<T> a) 
     }
 }
 
+// 4-byte-aligned 4-vectors: one 16-byte load per lane at any element offset.
+typedef int32_t vi4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef float vf4u __attribute__((ext_vector_type(4), aligned(4)));
+
+// FP32 variant: each lane loads 4 consecutive nonzeros of its row with one
+// 16-byte load per stream (col and every weight array), so a wavefront
+// instruction moves a contiguous 64*G-byte segment per row. NCH such chunks per
+// lane are issued before the gathers; longer rows finish in a strided loop.
+// The last rows of the CSR, whose 16-byte loads could pass the end of the
+// arrays, take the scalar path.
+template <int G, int NCH, int NOPS, int NF, int MODE>
+__global__ __launch_bounds__(kApplyMaxBlock) void k_apply_v4(const ApplyArgs<float> a) {
+    const int64_t hb = blockIdx.x;
+    const int64_t lb = (hb & 7) * a.per_xcd + (hb >> 3);
+    if (lb >= a.blocks_needed) return;
+    const int rows_per_block = blockDim.x / G;
+    const int gl = threadIdx.x % G;
+    const int64_t row = lb * rows_per_block + threadIdx.x / G;
+    if (row >= a.n_rows) return;
+    const int64_t rs = a.row_ptr[row];
+    const int64_t re = a.row_ptr[row + 1];
+    const int64_t nnz_total = a.row_ptr[a.n_rows];
+    float fp[NF];
+#pragma unroll
+    for (int b = 0; b < NF; ++b) fp[b] = a.f_self[b * a.ld_self + row];
+    float acc[NOPS][NF];
+#pragma unroll
+    for (int o = 0; o < NOPS; ++o)
+#pragma unroll
+        for (int b = 0; b < NF; ++b) acc[o][b] = 0.0f;
+    int64_t jtail = rs + gl;  // first element of the strided scalar loop
+    // 16-byte loads read elements e..e+3 with e <= re-1, so re + 3 <= nnz_total
+    // keeps them inside the arrays; empty rows skip the vector section.
+    if (re > rs && re + 3 <= nnz_total) {
+        vi4u q[NCH];
+        vf4u w[NOPS][NCH];
+        int64_t e0[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            e0[c] = rs + 4 * (gl + c * G);
+            const int64_t ee = e0[c] < re ? e0[c] : re - 1;  // clamped chunks are masked below
+            q[c] = DCPSE_STREAM_LOAD((const vi4u*)(a.col + ee));
+#pragma unroll
+            for (int o = 0; o < NOPS; ++o) w[o][c] = DCPSE_STREAM_LOAD((const vf4u*)(a.W[o] + ee));
+        }
+        // keep every stream load ahead of the gathers that depend on col
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int b = 0; b < NF; ++b) {
+            float fq[NCH][4];
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                fq[c][0] = a.f[b * a.ld_f + q[c].x];
+                fq[c][1] = a.f[b * a.ld_f + q[c].y];
+                fq[c][2] = a.f[b * a.ld_f + q[c].z];
+                fq[c][3] = a.f[b * a.ld_f + q[c].w];
+            }
+#pragma unroll
+            for (int c = 0; c < NCH; ++c)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const bool v = e0[c] + i < re;
+                    const float d = MODE == kDiff ? fq[c][i] - fp[b] : fq[c][i];
+#pragma unroll
+                    for (int o = 0; o < NOPS; ++o) acc[o][b] = v ? fmaf(w[o][c][i], d, acc[o][b]) : acc[o][b];
+                }
+        }
+        jtail = rs + 4 * G * NCH + gl;
+    }
+    for (int64_t j = jtail; j < re; j += G) {
+        const int32_t qq = DCPSE_STREAM_LOAD(a.col + j);
+        float ww[NOPS];
+#pragma unroll
+        for (int o = 0; o < NOPS; ++o) ww[o] = DCPSE_STREAM_LOAD(a.W[o] + j);
+#pragma unroll
+        for (int b = 0; b < NF; ++b) {
+            const float fq = a.f[b * a.ld_f + qq];
+            const float d = MODE == kDiff ? fq - fp[b] : fq;
+#pragma unroll
+            for (int o = 0; o < NOPS; ++o) acc[o][b] = fmaf(ww[o], d, acc[o][b]);
+        }
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int o = 0; o < NOPS; ++o)
+#pragma unroll
+            for (int b = 0; b < NF; ++b) acc[o][b] += __shfl_xor(acc[o][b], off, kWave);
+    if (gl == 0) {
+#pragma unroll
+        for (int o = 0; o < NOPS; ++o)
+#pragma unroll
+            for (int b = 0; b < NF; ++b) {
+                float val = acc[o][b];
+                if (MODE == kExplicitDiag) val = fmaf(a.diag[row], fp[b], val);
+                float* dst = a.out[o] + b * a.ld_out + row;
+                val *= a.alpha;
+                if (a.beta != 0.0f) val = fmaf(a.beta, *dst, val);
+                *dst = val;
+            }
+    }
+}
+
 template <typename T, int G, int EPL, int NOPS, int NF, int MODE>
 static void launch_apply(ApplyArgs<T> a, int block, hipStream_t s) {
     const int rows_per_block = block / G;
     a.blocks_needed = (a.n_rows + rows_per_block - 1) / rows_per_block;
     a.per_xcd = (a.blocks_needed + 7) / 8;
     const int64_t grid = a.per_xcd * 8;
-    k_apply<T, G, EPL, NOPS, NF, MODE><<<(unsigned)grid, block, 0, s>>>(a);
+    if constexpr (EPL >= 100) {
+        if constexpr (std::is_same<T, float>::value)
+            k_apply_v4<G, EPL - 100, NOPS, NF, MODE><<<(unsigned)grid, block, 0, s>>>(a);
+    } else {
+        k_apply<T, G, EPL, NOPS, NF, MODE><<<(unsigned)grid, block, 0, s>>>(a);
+    }
 }
 
 template <typename T, int G, int EPL, int MODE>
@@ -155,20 +274,21 @@ static bool dispatch_nops_nf(const ApplyArgs<T>& a, int nops, int nf, int block,
     if (nops != 1) return false;
     // fields per pass are capped at 32 / EPL (see max_fields) to bound registers
     switch (nf) {
-        case 2: if constexpr (2 * EPL <= 32) { launch_apply<T, G, EPL, 1, 2, MODE>(a, block, s); return true; }
+        case 2: if constexpr (2 * (EPL >= 100 ? 4 * (EPL - 100) : EPL) <= 32) { launch_apply<T, G, EPL, 1, 2, MODE>(a, block, s); return true; }
                 break;
-        case 4: if constexpr (4 * EPL <= 32) { launch_apply<T, G, EPL, 1, 4, MODE>(a, block, s); return true; }
+        case 4: if constexpr (4 * (EPL >= 100 ? 4 * (EPL - 100) : EPL) <= 32) { launch_apply<T, G, EPL, 1, 4, MODE>(a, block, s); return true; }
                 break;
-        case 8: if constexpr (8 * EPL <= 32) { launch_apply<T, G, EPL, 1, 8, MODE>(a, block, s); return true; }
+        case 8: if constexpr (8 * (EPL >= 100 ? 4 * (EPL - 100) : EPL) <= 32) { launch_apply<T, G, EPL, 1, 8, MODE>(a, block, s); return true; }
                 break;
-        case 16: if constexpr (16 * EPL <= 32) { launch_apply<T, G, EPL, 1, 16, MODE>(a, block, s); return true; }
+        case 16: if constexpr (16 * (EPL >= 100 ? 4 * (EPL - 100) : EPL) <= 32) { launch_apply<T, G, EPL, 1, 16, MODE>(a, block, s); return true; }
                 break;
     }
     return false;
 }
 
 // (lanes per row, elements per lane issued up front): every shape covers 64
-// nonzeros per pass except the last two, which cover 128.
+// nonzeros per pass except the last two, which cover 128. Template EPL >= 100
+// selects the 16-byte-load kernel with EPL - 100 chunks of 4 per lane.
 template <typename T, int MODE>
 static bool dispatch_group(const ApplyArgs<T>& a, int G, int epl, int nops, int nf, int block, hipStream_t s) {
     if (G == 4 && epl == 16) return dispatch_nops_nf<T, 4, 16, MODE>(a, nops, nf, block, s);
@@ -181,29 +301,44 @@ static bool dispatch_group(const ApplyArgs<T>& a, int G, int epl, int nops, int 
     return false;
 }
 
+// 16-byte-load shapes (FP32 only): epl = 4 * chunks per lane.
+template <int MODE>
+static bool dispatch_group_v4(const ApplyArgs<float>& a, int G, int epl, int nops, int nf, int block,
+                              hipStream_t s) {
+    if (G == 4 && epl == 8) return dispatch_nops_nf<float, 4, 102, MODE>(a, nops, nf, block, s);
+    if (G == 8 && epl == 4) return dispatch_nops_nf<float, 8, 101, MODE>(a, nops, nf, block, s);
+    if (G == 8 && epl == 8) return dispatch_nops_nf<float, 8, 102, MODE>(a, nops, nf, block, s);
+    if (G == 16 && epl == 4) return dispatch_nops_nf<float, 16, 101, MODE>(a, nops, nf, block, s);
+    if (G == 16 && epl == 8) return dispatch_nops_nf<float, 16, 102, MODE>(a, nops, nf, block, s);
+    if (G == 32 && epl == 4) return dispatch_nops_nf<float, 32, 101, MODE>(a, nops, nf, block, s);
+    return false;
+}
+
 struct ApplyShape {
-    int G, epl;
+    int G, epl, vec;
 };
 
-// Launch shape from the mean row length: rows of up to 64 nonzeros take 8
-// lanes x 8 elements, longer rows 16 x 8. The mean is read once per
+// Launch shape from the mean row length (16-byte loads): up to 32 nonzeros
+// 8 lanes x 4, up to 64 8 x 8, longer rows 16 x 8. The mean is read once per
 // (row_ptr, n_rows) and cached in the context, so repeated applies of one
 // operator never synchronise the stream. dcpse_set_apply_config overrides it.
 static ApplyShape auto_shape(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr) {
-    if (ctx->apply_group > 0) return {ctx->apply_group, ctx->apply_epl};
-    if (ctx->grp_rows == row_ptr && ctx->grp_n == n_rows) return {ctx->grp_g, ctx->grp_epl};
+    if (ctx->apply_group > 0) return {ctx->apply_group, ctx->apply_epl, ctx->apply_vec};
+    if (ctx->grp_rows == row_ptr && ctx->grp_n == n_rows) return {ctx->grp_g, ctx->grp_epl, ctx->grp_vec};
     int64_t nnz = 0;
-    ApplyShape sh{8, 8};
+    ApplyShape sh{8, 8, 4};
     if (n_rows > 0 &&
         hipMemcpyAsync(&nnz, row_ptr + n_rows, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
         hipStreamSynchronize(ctx->stream) == hipSuccess) {
         double mean = (double)nnz / (double)n_rows;
-        if (mean > 64) sh = {16, 8};
+        if (mean <= 32) sh = {8, 4, 4};
+        else if (mean > 64) sh = {16, 8, 4};
     }
     ctx->grp_rows = row_ptr;
     ctx->grp_n = n_rows;
     ctx->grp_g = sh.G;
     ctx->grp_epl = sh.epl;
+    ctx->grp_vec = sh.vec;
     return sh;
 }
 
@@ -257,7 +392,14 @@ static dcpse_status apply_impl(dcpse_ctx* ctx, int64_t n_rows, const int64_t* ro
         a.f = f + (int64_t)b0 * ld_f;
         a.f_self = (f_self ? f_self + (int64_t)b0 * ld_self : a.f);
         for (int o = 0; o < nops; ++o) a.out[o] = out[o] + (int64_t)b0 * ld_out;
-        if (!dispatch_group<T, MODE>(a, sh.G, sh.epl, nops, nf, block, ctx->stream))
+        bool ok;
+        if constexpr (std::is_same<T, float>::value)
+            ok = sh.vec == 4 ? dispatch_group_v4<MODE>(a, sh.G, sh.epl, nops, nf, block, ctx->stream)
+                             : dispatch_group<T, MODE>(a, sh.G, sh.epl, nops, nf, block, ctx->stream);
+        else
+            ok = dispatch_group<T, MODE>(a, sh.G, sh.epl, nops, nf, block, ctx->stream) ||
+                 (sh.vec == 4 && dispatch_group<T, MODE>(a, 16, 4, nops, nf, block, ctx->stream));
+        if (!ok)
             return fail(ctx, DCPSE_ERR_UNSUPPORTED, "no apply kernel for group=%d x %d ops=%d fields=%d", sh.G,
                         sh.epl, nops, nf);
         DCPSE_LAUNCH_CHECK(ctx);

@@ -54,20 +54,28 @@ dcpse_status dcpse_sync(dcpse_ctx* ctx) {
 }
 
 dcpse_status dcpse_set_apply_config(dcpse_ctx* ctx, int32_t group_lanes, int32_t elems_per_lane,
-                                    int32_t block_threads) {
+                                    int32_t vector_width, int32_t block_threads) {
     if (!ctx) return DCPSE_ERR_INVALID;
-    const bool ok = group_lanes == 0 || (group_lanes == 4 && elems_per_lane == 16) ||
-                    (group_lanes == 8 && (elems_per_lane == 8 || elems_per_lane == 16)) ||
-                    (group_lanes == 16 && (elems_per_lane == 4 || elems_per_lane == 8)) ||
-                    (group_lanes == 32 && elems_per_lane == 2) || (group_lanes == 64 && elems_per_lane == 1);
+    const int g = group_lanes, e = elems_per_lane;
+    bool ok = g == 0;
+    if (vector_width == 1)
+        ok = ok || (g == 4 && e == 16) || (g == 8 && (e == 8 || e == 16)) || (g == 16 && (e == 4 || e == 8)) ||
+             (g == 32 && e == 2) || (g == 64 && e == 1);
+    else if (vector_width == 4)
+        ok = ok || (g == 4 && e == 8) || (g == 8 && (e == 4 || e == 8)) || (g == 16 && (e == 4 || e == 8)) ||
+             (g == 32 && e == 4);
+    else
+        ok = false;
     if (!ok)
         return fail(ctx, DCPSE_ERR_INVALID,
-                    "unsupported apply shape %d x %d (compiled: 4x16, 8x8, 8x16, 16x4, 16x8, 32x2, 64x1)",
-                    group_lanes, elems_per_lane);
+                    "unsupported apply shape %d lanes x %d elements, vector width %d (compiled: width 1: 4x16, "
+                    "8x8, 8x16, 16x4, 16x8, 32x2, 64x1; width 4: 4x8, 8x4, 8x8, 16x4, 16x8, 32x4)",
+                    g, e, vector_width);
     if (block_threads < 64 || block_threads > 512 || block_threads % 64)
         return fail(ctx, DCPSE_ERR_INVALID, "block_threads must be a multiple of 64 in [64, 512]");
-    ctx->apply_group = group_lanes;
-    ctx->apply_epl = elems_per_lane;
+    ctx->apply_group = g;
+    ctx->apply_epl = e;
+    ctx->apply_vec = vector_width;
     ctx->apply_block = block_threads;
     return DCPSE_OK;
 }

@@ -15,11 +15,13 @@ struct dcpse_ctx {
     int num_cus = 256;
     int apply_group = 0;   // lanes per row, 0 = automatic
     int apply_epl = 8;     // nonzeros per lane issued up front
+    int apply_vec = 1;     // 1: 4-byte loads, 4: 16-byte loads (FP32 only)
     int apply_block = 256;
     const void* grp_rows = nullptr;  // cache of the automatic apply shape
     int64_t grp_n = -1;
     int grp_g = 8;
     int grp_epl = 8;
+    int grp_vec = 4;
     int asm_wpb = 1;       // waves per block in the assembly kernel
     int asm_ppb = 8;       // particles per block (per wave) batch
     void* ws = nullptr;    // grow-only device workspace

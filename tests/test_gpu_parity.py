@@ -176,12 +176,12 @@ def test_every_apply_shape(oracle, cuda):
     want32 = np.stack([sum(oracle.apply(r, rp, col, F32[b].astype(np.float64)) for r in refs) for b in range(5)])
     ctx = _lib.context(0)
     try:
-        for g, e in APPLY_SHAPES:
+        for g, e, vw in APPLY_SHAPES:
             for block in (64, 256, 512):
                 if block % g:
                     continue
-                print(f"[shape] G={g} epl={e} block={block}", flush=True)
-                ctx.check(ctx.lib.dcpse_set_apply_config(ctx.h, g, e, block), "config")
+                print(f"[shape] G={g} epl={e} vec={vw} block={block}", flush=True)
+                ctx.check(ctx.lib.dcpse_set_apply_config(ctx.h, g, e, vw, block), "config")
                 got = lap(torch.as_tensor(F, device=cuda)).cpu().numpy()
                 assert np.max(np.abs(got - want)) / np.max(np.abs(want)) <= 1e-12
                 got32 = lap(torch.as_tensor(F32, device=cuda)).cpu().numpy().astype(np.float64)
@@ -189,7 +189,8 @@ def test_every_apply_shape(oracle, cuda):
                 gt = lap.applyTranspose(torch.as_tensor(F, device=cuda)).cpu().numpy()
                 one = lap.applyTranspose(torch.as_tensor(F[2], device=cuda)).cpu().numpy()
                 assert np.max(np.abs(gt[2] - one)) <= 1e-12 * np.max(np.abs(one))
-        assert ctx.lib.dcpse_set_apply_config(ctx.h, 8, 8, 1024) != 0
-        assert ctx.lib.dcpse_set_apply_config(ctx.h, 8, 3, 256) != 0
+        assert ctx.lib.dcpse_set_apply_config(ctx.h, 8, 8, 1, 1024) != 0
+        assert ctx.lib.dcpse_set_apply_config(ctx.h, 8, 3, 1, 256) != 0
+        assert ctx.lib.dcpse_set_apply_config(ctx.h, 64, 1, 4, 256) != 0
     finally:
-        ctx.check(ctx.lib.dcpse_set_apply_config(ctx.h, 0, 8, 256), "config reset")
+        ctx.check(ctx.lib.dcpse_set_apply_config(ctx.h, 0, 4, 4, 256), "config reset")
