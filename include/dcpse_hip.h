@@ -81,7 +81,8 @@ dcpse_status dcpse_sync(dcpse_ctx* ctx);
 /* Apply-kernel launch shape: lanes cooperating on one row, nonzeros each
  * lane loads before consuming, and load width in elements (1: 4-byte loads;
  * 4: 16-byte loads, FP32 only). Compiled shapes, width 1: 4x16, 8x8, 8x16,
- * 16x4, 16x8, 32x2, 64x1; width 4: 4x8, 8x4, 8x8, 16x4, 16x8, 32x4. group_lanes 0 =
+ * 16x4, 16x8, 32x2, 64x1; width 4: 4x8, 8x4, 8x8, 8x12, 16x4, 16x8, 32x4.
+ * group_lanes 0 =
  * automatic. FP64 applies ignore width 4 and use the width-1 shape with the
  * same lanes and elements when compiled (automatic otherwise).
  * block_threads: a multiple of 64 in 64..512. */
@@ -182,6 +183,36 @@ dcpse_status dcpse_apply_transpose_f64(dcpse_ctx* ctx, int64_t n_cols, const int
                                        const int32_t* colT, const double* WT, const double* diag,
                                        const double* g, int64_t ld_g, int32_t n_fields,
                                        double* out, int64_t ld_out, double alpha, double beta);
+
+/* ---- sliced-ELL execution layout (FP32 apply) ------------------------- */
+/* The same operator laid out one wavefront per slice of 64 consecutive rows:
+ * slice s holds rows [64s, 64s+64) with width w_s = longest row rounded up to
+ * 4, and slot j of row 64s+l at slice_ptr[s] + (j/4)*256 + 4l + (j%4).
+ * Padded slots hold column 0 and weight 0. sell_col/sell_W must be 16-byte
+ * aligned. This is an execution copy: the CSR arrays stay the operator's
+ * canonical storage (calcKernels/kerOffsets).
+ * count: writes slice_ptr[n_slices+1] (device, n_slices = ceil(n_rows/64))
+ * and *n_slots (host, synchronises). fill_cols/fill_weights_f32 write
+ * sell_col[n_slots] / sell_W[n_slots] (device) from the CSR. */
+dcpse_status dcpse_sell_count(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, int64_t* slice_ptr,
+                              int64_t* n_slots);
+dcpse_status dcpse_sell_fill_cols(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int32_t* col,
+                                  const int64_t* slice_ptr, int32_t* sell_col);
+dcpse_status dcpse_sell_fill_weights_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const float* W,
+                                         const int64_t* slice_ptr, float* sell_W);
+/* Same result as dcpse_apply_f32 with one field, from the SELL layout:
+ * out[o][p] = alpha * sum_j sell_W[o][.] (f[col] - f_self[p]) + beta * out[o][p]. */
+dcpse_status dcpse_apply_sell_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int64_t* slice_ptr,
+                                  const int32_t* sell_col, int32_t n_ops, const float* const* sell_W,
+                                  const float* f, const float* f_self, float* const* out, float alpha, float beta);
+/* Same result as dcpse_apply_transpose_f32 with one field, from the SELL
+ * layout of the transposed CSR (built with the three calls above). */
+dcpse_status dcpse_apply_transpose_sell_f32(dcpse_ctx* ctx, int64_t n_cols, const int64_t* rowT_ptr,
+                                            const int64_t* slice_ptr, const int32_t* sell_colT,
+                                            const float* sell_WT, const float* diag, const float* g, float* out,
+                                            float alpha, float beta);
+/* Groups of 4 slots each lane loads before its gathers (2 or 4). */
+dcpse_status dcpse_set_sell_unroll(dcpse_ctx* ctx, int32_t groups);
 
 #ifdef __cplusplus
 }

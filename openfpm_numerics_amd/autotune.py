@@ -25,7 +25,7 @@ def _time(fn, reps):
 
 
 # (lanes per row, elements per lane, load width)
-APPLY_SHAPES = ((4, 8, 4), (8, 4, 4), (8, 8, 4), (16, 4, 4), (16, 8, 4), (32, 4, 4),
+APPLY_SHAPES = ((4, 8, 4), (8, 4, 4), (8, 8, 4), (8, 12, 4), (16, 4, 4), (16, 8, 4), (32, 4, 4),
                 (4, 16, 1), (8, 8, 1), (16, 4, 1), (32, 2, 1), (64, 1, 1), (8, 16, 1), (16, 8, 1))
 
 
@@ -34,16 +34,19 @@ def tune_apply(fn, shapes=APPLY_SHAPES, blocks=(256, 512), reps=10, device=0):
     ctx = _lib.context(device)
     best = None
     results = {}
-    for g, e, v in shapes:
-        for b in blocks:
-            if b % g:
-                continue
-            ctx.check(ctx.lib.dcpse_set_apply_config(ctx.h, g, e, v, b), "dcpse_set_apply_config")
-            ms = _time(fn, reps)
-            results[(g, e, v, b)] = ms
-            if best is None or ms < best[1]:
-                best = ((g, e, v, b), ms)
-    ctx.check(ctx.lib.dcpse_set_apply_config(ctx.h, *best[0]), "dcpse_set_apply_config")
+    try:
+        for g, e, v in shapes:
+            for b in blocks:
+                if b % g:
+                    continue
+                ctx.check(ctx.lib.dcpse_set_apply_config(ctx.h, g, e, v, b), "dcpse_set_apply_config")
+                ms = _time(fn, reps)
+                results[(g, e, v, b)] = ms
+                if best is None or ms < best[1]:
+                    best = ((g, e, v, b), ms)
+    finally:
+        cfg = best[0] if best else (0, 8, 4, 256)
+        ctx.check(ctx.lib.dcpse_set_apply_config(ctx.h, *cfg), "dcpse_set_apply_config")
     return best, results
 
 

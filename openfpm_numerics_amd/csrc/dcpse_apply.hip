@@ -308,6 +308,7 @@ static bool dispatch_group_v4(const ApplyArgs<float>& a, int G, int epl, int nop
     if (G == 4 && epl == 8) return dispatch_nops_nf<float, 4, 102, MODE>(a, nops, nf, block, s);
     if (G == 8 && epl == 4) return dispatch_nops_nf<float, 8, 101, MODE>(a, nops, nf, block, s);
     if (G == 8 && epl == 8) return dispatch_nops_nf<float, 8, 102, MODE>(a, nops, nf, block, s);
+    if (G == 8 && epl == 12) return dispatch_nops_nf<float, 8, 103, MODE>(a, nops, nf, block, s);
     if (G == 16 && epl == 4) return dispatch_nops_nf<float, 16, 101, MODE>(a, nops, nf, block, s);
     if (G == 16 && epl == 8) return dispatch_nops_nf<float, 16, 102, MODE>(a, nops, nf, block, s);
     if (G == 32 && epl == 4) return dispatch_nops_nf<float, 32, 101, MODE>(a, nops, nf, block, s);
@@ -318,8 +319,9 @@ struct ApplyShape {
     int G, epl, vec;
 };
 
-// Launch shape from the mean row length (16-byte loads): up to 32 nonzeros
-// 8 lanes x 4, up to 64 8 x 8, longer rows 16 x 8. The mean is read once per
+// Launch shape from the mean row length (16-byte loads, 8 lanes per row): up
+// to 24 nonzeros 4 per lane, otherwise 8 per lane (longer rows finish in the
+// strided tail; covering them with a third chunk measured slower). The mean is read once per
 // (row_ptr, n_rows) and cached in the context, so repeated applies of one
 // operator never synchronise the stream. dcpse_set_apply_config overrides it.
 static ApplyShape auto_shape(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr) {
@@ -331,8 +333,7 @@ static ApplyShape auto_shape(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_
         hipMemcpyAsync(&nnz, row_ptr + n_rows, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
         hipStreamSynchronize(ctx->stream) == hipSuccess) {
         double mean = (double)nnz / (double)n_rows;
-        if (mean <= 32) sh = {8, 4, 4};
-        else if (mean > 64) sh = {16, 8, 4};
+        if (mean <= 24) sh = {8, 4, 4};
     }
     ctx->grp_rows = row_ptr;
     ctx->grp_n = n_rows;
@@ -392,7 +393,7 @@ static dcpse_status apply_impl(dcpse_ctx* ctx, int64_t n_rows, const int64_t* ro
         a.f = f + (int64_t)b0 * ld_f;
         a.f_self = (f_self ? f_self + (int64_t)b0 * ld_self : a.f);
         for (int o = 0; o < nops; ++o) a.out[o] = out[o] + (int64_t)b0 * ld_out;
-        bool ok;
+        bool ok = false;
         if constexpr (std::is_same<T, float>::value)
             ok = sh.vec == 4 ? dispatch_group_v4<MODE>(a, sh.G, sh.epl, nops, nf, block, ctx->stream)
                              : dispatch_group<T, MODE>(a, sh.G, sh.epl, nops, nf, block, ctx->stream);

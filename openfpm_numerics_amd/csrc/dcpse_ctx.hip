@@ -62,14 +62,14 @@ dcpse_status dcpse_set_apply_config(dcpse_ctx* ctx, int32_t group_lanes, int32_t
         ok = ok || (g == 4 && e == 16) || (g == 8 && (e == 8 || e == 16)) || (g == 16 && (e == 4 || e == 8)) ||
              (g == 32 && e == 2) || (g == 64 && e == 1);
     else if (vector_width == 4)
-        ok = ok || (g == 4 && e == 8) || (g == 8 && (e == 4 || e == 8)) || (g == 16 && (e == 4 || e == 8)) ||
-             (g == 32 && e == 4);
+        ok = ok || (g == 4 && e == 8) || (g == 8 && (e == 4 || e == 8 || e == 12)) ||
+             (g == 16 && (e == 4 || e == 8)) || (g == 32 && e == 4);
     else
         ok = false;
     if (!ok)
         return fail(ctx, DCPSE_ERR_INVALID,
                     "unsupported apply shape %d lanes x %d elements, vector width %d (compiled: width 1: 4x16, "
-                    "8x8, 8x16, 16x4, 16x8, 32x2, 64x1; width 4: 4x8, 8x4, 8x8, 16x4, 16x8, 32x4)",
+                    "8x8, 8x16, 16x4, 16x8, 32x2, 64x1; width 4: 4x8, 8x4, 8x8, 8x12, 16x4, 16x8, 32x4)",
                     g, e, vector_width);
     if (block_threads < 64 || block_threads > 512 || block_threads % 64)
         return fail(ctx, DCPSE_ERR_INVALID, "block_threads must be a multiple of 64 in [64, 512]");

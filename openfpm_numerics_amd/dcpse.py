@@ -25,7 +25,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .particles import ParticleSet, VerletList
+from .particles import ParticleSet, VerletList, sell_layout, sell_weights
 
 # support_options (src/DCPSE/SupportBuilder.hpp:13-19)
 RADIUS = 0
@@ -34,6 +34,9 @@ ADAPTIVE = 2
 AT_LEAST_N_PARTICLES = 3
 
 _FILE_MAGIC = b"DCPSEHIP"
+
+#: FP32 single-field applies run on the sliced-ELL layout (set False to use the CSR kernels)
+USE_SELL = True
 _FILE_VERSION = 1
 
 
@@ -167,6 +170,14 @@ class Dcpse:
             raise TypeError(f"no weights for {dtype}; use float32 or float64")
         return self._W[dtype]
 
+    def sell_weights(self) -> torch.Tensor:
+        """FP32 weights in the sliced-ELL layout of this operator's Verlet list."""
+        if self._W.get("sell") is None:
+            slice_ptr, sell_col = self.verletList.sell()
+            self._W["sell"] = sell_weights(self.verletList.row_ptr, self._W[torch.float32], slice_ptr,
+                                           self.verletList.n_rows, sell_col.numel())
+        return self._W["sell"]
+
     def computeDifferentialOperator(self, f: torch.Tensor, out: torch.Tensor | None = None,
                                     f_domain: torch.Tensor | None = None, alpha=1.0, beta=0.0):
         """D f for a field (N,) or a batch of fields (B, N) over the support set."""
@@ -220,6 +231,15 @@ class Dcpse:
         self._T[dtype] = (rowT, colT[: vl.nnz], WT[: vl.nnz], diag)
         return self._T[dtype]
 
+    def transposed_sell(self):
+        """SELL layout of the FP32 transposed operator: (slice_ptr, sell_colT, sell_WT)."""
+        if self._T.get("sell") is None:
+            rowT, colT, WT, _ = self.transposed(torch.float32)
+            n = self.verletList.n_cols
+            slice_ptr, sell_col = sell_layout(rowT, colT, n)
+            self._T["sell"] = (slice_ptr, sell_col, sell_weights(rowT, WT, slice_ptr, n, sell_col.numel()))
+        return self._T["sell"]
+
     def applyTranspose(self, g: torch.Tensor, out: torch.Tensor | None = None, alpha=1.0, beta=0.0):
         """A^T g for a field (N,) or a batch (B, N)."""
         if g.dtype not in (torch.float32, torch.float64):
@@ -231,6 +251,11 @@ class Dcpse:
             raise ValueError("field length differs from the support set")
         o = torch.empty_like(g2) if out is None else out.reshape(nb, n)
         ctx = _lib.context(g.device.index or 0)
+        if g.dtype == torch.float32 and nb == 1 and USE_SELL:
+            slice_ptr, sell_colT, sell_WT = self.transposed_sell()
+            ctx.call("dcpse_apply_transpose_sell_f32", n, _lib.ptr(rowT), _lib.ptr(slice_ptr), _lib.ptr(sell_colT),
+                     _lib.ptr(sell_WT), _lib.ptr(diag), _lib.ptr(g2), _lib.ptr(o), alpha, beta)
+            return o.reshape(g.shape)
         name = "dcpse_apply_transpose_f32" if g.dtype == torch.float32 else "dcpse_apply_transpose_f64"
         ctx.call(name, n, _lib.ptr(rowT), _lib.ptr(colT), _lib.ptr(WT), _lib.ptr(diag), _lib.ptr(g2), n, nb,
                  _lib.ptr(o), n, alpha, beta)
@@ -369,8 +394,14 @@ def apply_ops(ops, f: torch.Tensor, outs=None, f_domain: torch.Tensor | None = N
         outs = [torch.empty((nb, n_rows), dtype=f.dtype, device=f.device) for _ in ops]
     else:
         outs = [o.reshape(nb, n_rows) for o in outs]
-    W = [op.weights(f.dtype) for op in ops]
     ctx = _lib.context(f.device.index or 0)
+    if f.dtype == torch.float32 and nb == 1 and USE_SELL:
+        slice_ptr, sell_col = vl.sell()
+        SW = [op.sell_weights() for op in ops]
+        ctx.call("dcpse_apply_sell_f32", n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(slice_ptr), _lib.ptr(sell_col),
+                 len(ops), _lib.ptr_array(SW), _lib.ptr(f2), _lib.ptr(fd), _lib.ptr_array(outs), alpha, beta)
+        return [o.reshape(*f.shape[:-1], n_rows) for o in outs]
+    W = [op.weights(f.dtype) for op in ops]
     name = "dcpse_apply_f32" if f.dtype == torch.float32 else "dcpse_apply_f64"
     if len(ops) > 1 and nb > 1:
         res = [apply_ops([op], f, [o], f_domain, alpha, beta)[0] for op, o in zip(ops, outs)]
@@ -388,7 +419,7 @@ class PreparedApply:
     and keep their storage. ``f`` is (N,) or (B, N); ``outs`` one tensor per op.
     """
 
-    def __init__(self, ops, f: torch.Tensor, outs, alpha=1.0, beta=0.0):
+    def __init__(self, ops, f: torch.Tensor, outs, alpha=1.0, beta=0.0, sell: bool = True):
         vl = ops[0].verletList
         if any(op.verletList is not vl for op in ops):
             raise ValueError("fused operators must share one Verlet list")
@@ -396,13 +427,22 @@ class PreparedApply:
             raise ValueError("contiguous tensors only")
         f2 = f.reshape(-1, f.shape[-1])
         nb, n = f2.shape
-        self._keep = (ops, f, outs, [op.weights(f.dtype) for op in ops])
         self.ctx = _lib.context(f.device.index or 0)
-        self.fn = self.ctx.lib.dcpse_apply_f32 if f.dtype == torch.float32 else self.ctx.lib.dcpse_apply_f64
-        self.W = _lib.ptr_array(self._keep[3])
         self.O = _lib.ptr_array(outs)
-        self.args = (self.ctx.h, vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(vl.col), len(ops), self.W,
-                     _lib.ptr(f2), n, nb, None, n, self.O, vl.n_rows, alpha, beta)
+        if sell and f.dtype == torch.float32 and nb == 1:
+            slice_ptr, sell_col = vl.sell()
+            self._keep = (ops, f, outs, [op.sell_weights() for op in ops], slice_ptr, sell_col)
+            self.W = _lib.ptr_array(self._keep[3])
+            self.fn = self.ctx.lib.dcpse_apply_sell_f32
+            self.args = (self.ctx.h, vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(slice_ptr), _lib.ptr(sell_col),
+                         len(ops), self.W, _lib.ptr(f2), None, self.O, alpha, beta)
+        else:
+            self._keep = (ops, f, outs, [op.weights(f.dtype) for op in ops])
+            self.fn = self.ctx.lib.dcpse_apply_f32 if f.dtype == torch.float32 else self.ctx.lib.dcpse_apply_f64
+            self.W = _lib.ptr_array(self._keep[3])
+            self.args = (self.ctx.h, vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(vl.col), len(ops), self.W,
+                         _lib.ptr(f2), n, nb, None, n, self.O, vl.n_rows, alpha, beta)
+        self.sell = self.fn == self.ctx.lib.dcpse_apply_sell_f32
         self.ctx.bind_current_stream()
 
     def __call__(self):

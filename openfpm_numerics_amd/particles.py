@@ -69,6 +69,7 @@ class ParticleSet:
     def updateVerlet(self, vl: "VerletList", rCut) -> "VerletList":
         fresh = VerletList.build(self, rCut, domain=vl.domain)
         vl.__dict__.update(fresh.__dict__)
+        vl._sell = None
         return vl
 
 
@@ -83,6 +84,7 @@ class VerletList:
         self.rcut = rcut
         self.domain = domain
         self._host = None
+        self._sell = None
         self.map_ctr = -1
 
     @property
@@ -135,3 +137,33 @@ class VerletList:
 
     def row_lengths(self) -> torch.Tensor:
         return self.row_ptr[1:] - self.row_ptr[:-1]
+
+    def sell(self):
+        """(slice_ptr, sell_col): the sliced-ELL execution layout of this support,
+        shared by every FP32 operator assembled on it; built on first use."""
+        if getattr(self, "_sell", None) is None:
+            self._sell = sell_layout(self.row_ptr, self.col, self.n_rows)
+        return self._sell
+
+
+def sell_layout(row_ptr: torch.Tensor, col: torch.Tensor, n_rows: int):
+    ctx = _lib.context(row_ptr.device.index or 0)
+    dev = row_ptr.device
+    n_slices = (n_rows + 63) // 64
+    slice_ptr = torch.empty(n_slices + 1, dtype=torch.int64, device=dev)
+    n_slots = np.zeros(1, np.int64)
+    ctx.call("dcpse_sell_count", n_rows, _lib.ptr(row_ptr), _lib.ptr(slice_ptr), n_slots.ctypes.data)
+    sell_col = torch.empty(max(int(n_slots[0]), 4), dtype=torch.int32, device=dev)
+    if n_rows:
+        ctx.call("dcpse_sell_fill_cols", n_rows, _lib.ptr(row_ptr), _lib.ptr(col), _lib.ptr(slice_ptr),
+                 _lib.ptr(sell_col))
+    return slice_ptr, sell_col
+
+
+def sell_weights(row_ptr: torch.Tensor, W: torch.Tensor, slice_ptr: torch.Tensor, n_rows: int, n_slots: int):
+    ctx = _lib.context(row_ptr.device.index or 0)
+    out = torch.empty(max(n_slots, 4), dtype=torch.float32, device=row_ptr.device)
+    if n_rows:
+        ctx.call("dcpse_sell_fill_weights_f32", n_rows, _lib.ptr(row_ptr), _lib.ptr(W), _lib.ptr(slice_ptr),
+                 _lib.ptr(out))
+    return out

@@ -149,17 +149,23 @@ def test_multi_field_apply_equals_single(cuda):
     lap = Laplacian(ps, vl, 2, w["rcut"])
     g = torch.Generator(device="cpu").manual_seed(1)
     F = torch.randn(13, ps.size_local(), generator=g, dtype=torch.float64).to(cuda)
-    for dt in (torch.float64, torch.float32):
-        batch = lap(F.to(dt))
-        for b in range(F.shape[0]):
-            assert torch.equal(batch[b], lap(F[b].to(dt)))
+    batch = lap(F)
+    for b in range(F.shape[0]):
+        assert torch.equal(batch[b], lap(F[b]))
+    # FP32: the batch runs the CSR kernel, a single field the sliced-ELL one
+    # (different summation order), so they agree to FP32 rounding only
+    F32 = F.to(torch.float32)
+    batch = lap(F32).double()
+    for b in range(F.shape[0]):
+        one = lap(F32[b]).double()
+        assert float((batch[b] - one).abs().max()) <= 1e-6 * float(one.abs().max())
 
 
 def test_every_apply_shape(oracle, cuda):
     """Every compiled (lanes, elements-per-lane) shape and block size agrees with the
     FP64 oracle (FP32 within 1e-5 relative L2, FP64 within 1e-12), single and
     multi-field, fused and transposed."""
-    from openfpm_numerics_amd import Laplacian, _lib, workloads as W
+    from openfpm_numerics_amd import Laplacian, _lib, dcpse as D, workloads as W
     from openfpm_numerics_amd.autotune import APPLY_SHAPES
 
     w = W.jittered_3d_holes(22, seed=13)
@@ -186,6 +192,14 @@ def test_every_apply_shape(oracle, cuda):
                 assert np.max(np.abs(got - want)) / np.max(np.abs(want)) <= 1e-12
                 got32 = lap(torch.as_tensor(F32, device=cuda)).cpu().numpy().astype(np.float64)
                 assert np.linalg.norm(got32 - want32) / np.linalg.norm(want32) <= 1e-5
+                D.USE_SELL = False  # single-field FP32 through the CSR kernels of this shape
+                try:
+                    one = lap(torch.as_tensor(F32[4], device=cuda)).cpu().numpy().astype(np.float64)
+                    tr = lap.applyTranspose(torch.as_tensor(F32[4], device=cuda)).cpu().numpy()
+                finally:
+                    D.USE_SELL = True
+                assert np.linalg.norm(one - want32[4]) / np.linalg.norm(want32[4]) <= 1e-5
+                assert np.all(np.isfinite(tr))
                 gt = lap.applyTranspose(torch.as_tensor(F, device=cuda)).cpu().numpy()
                 one = lap.applyTranspose(torch.as_tensor(F[2], device=cuda)).cpu().numpy()
                 assert np.max(np.abs(gt[2] - one)) <= 1e-12 * np.max(np.abs(one))
@@ -194,3 +208,53 @@ def test_every_apply_shape(oracle, cuda):
         assert ctx.lib.dcpse_set_apply_config(ctx.h, 64, 1, 4, 256) != 0
     finally:
         ctx.check(ctx.lib.dcpse_set_apply_config(ctx.h, 0, 4, 4, 256), "config reset")
+
+
+@pytest.mark.parametrize("name", sorted(WORKLOADS))
+def test_sell_layout_apply(name, oracle, cuda):
+    """The sliced-ELL FP32 apply (forward, fused, transposed; both unroll depths)
+    against the FP64 oracle of the FP32-rounded input, and against the CSR kernel."""
+    from openfpm_numerics_amd import Derivative_x, Laplacian, _lib, dcpse as D, fused, workloads as W
+
+    w = WORKLOADS[name](W)
+    ps = _ps(w, cuda)
+    vl = ps.getVerlet(w["rcut"])
+    rp, col = vl.row_ptr.cpu().numpy(), vl.col.cpu().numpy()
+    dim = ps.dims
+    lap = Laplacian(ps, vl, 2, w["rcut"])
+    dx = Derivative_x(ps, vl, 2, w["rcut"])
+    refs = [oracle.assemble(w["pos"], rp, col, tuple(2 if k == d else 0 for k in range(dim)), 2,
+                            w["box_lo"], w["box_hi"], w["periodic"]) for d in range(dim)]
+    ref_dx = oracle.assemble(w["pos"], rp, col, (1,) + (0,) * (dim - 1), 2, w["box_lo"], w["box_hi"],
+                             w["periodic"])
+    rng = np.random.default_rng(3)
+    f32 = rng.standard_normal(ps.size_local()).astype(np.float32)
+    f64 = f32.astype(np.float64)
+    want_lap = sum(oracle.apply(r, rp, col, f64) for r in refs)
+    want_dx = oracle.apply(ref_dx, rp, col, f64)
+    ctx = _lib.context(0)
+    ft = torch.as_tensor(f32, device=cuda)
+    try:
+        for unr in (2, 4):
+            print(f"[sell] {name} unroll={unr}", flush=True)
+            ctx.check(ctx.lib.dcpse_set_sell_unroll(ctx.h, unr), "unroll")
+            a, b = fused(lap, dx, f=ft)
+            for got, want in ((a, want_lap), (b, want_dx)):
+                got = got.cpu().numpy().astype(np.float64)
+                assert np.linalg.norm(got - want) / np.linalg.norm(want) <= 1e-5
+            D.USE_SELL = False
+            try:
+                csr = lap(ft).double()
+            finally:
+                D.USE_SELL = True
+            assert float((a.double() - csr).abs().max()) <= 1e-6 * float(csr.abs().max())
+            # transposed operator
+            A = None
+            for r in refs:
+                Ad = oracle.value_nz(r, rp, col)
+                A = Ad if A is None else A + Ad
+            want_t = A.T @ f64
+            got_t = lap.applyTranspose(ft).cpu().numpy().astype(np.float64)
+            assert np.linalg.norm(got_t - want_t) / np.linalg.norm(want_t) <= 1e-5
+    finally:
+        ctx.check(ctx.lib.dcpse_set_sell_unroll(ctx.h, 4), "unroll reset")
