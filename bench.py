@@ -76,6 +76,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
     ap.add_argument("--no-tune", action="store_true", help="skip launch-shape autotuning")
+    ap.add_argument("--order", choices=["morton", "lattice"], default="lattice",
+                    help="particle storage order (morton: ParticleSet.reorder, as openfpm's reorder)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run torch's copy kernel on 1 GiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -90,7 +92,7 @@ def main():
     import torch
 
     from openfpm_numerics_amd import Derivative_x, Laplacian, ParticleSet, workloads as W
-    from openfpm_numerics_amd.autotune import tune_apply, tune_assembly
+    from openfpm_numerics_amd.autotune import tune_apply, tune_assembly, tune_sell
     from openfpm_numerics_amd.dcpse import Dcpse, PreparedApply
 
     if not torch.cuda.is_available():
@@ -113,6 +115,9 @@ def main():
     N = w["pos"].shape[0]
     ps = ParticleSet(w["pos"], w["box_lo"], w["box_hi"], bc=[1, 1, 1], device=dev)
     rcut = w["rcut"]
+    if args.order == "morton":
+        perm = ps.reorder("morton")
+        w["pos"] = np.ascontiguousarray(w["pos"][perm])
 
     # ---- support -------------------------------------------------------------
     ps.getVerlet(rcut)  # warm (code objects, workspace)
@@ -154,11 +159,18 @@ def main():
     out_dx = torch.empty(N, dtype=torch.float32, device=dev)
     step = PreparedApply([lap.dcpse, dx.dcpse], f32_t, [out_lap, out_dx])
     tuned = None
+    sell_unroll = None
     if not args.no_tune:
-        tuned, app_tune = tune_apply(step)
-        step = PreparedApply([lap.dcpse, dx.dcpse], f32_t, [out_lap, out_dx])
-        log("[bench] apply autotune (G,epl,vec,block) -> %s: %s" % (
+        # CSR kernels (several FP32 fields, FP64) and the sliced-ELL single-field kernel
+        csr_step = PreparedApply([lap.dcpse, dx.dcpse], f32_t, [out_lap, out_dx], sell=False)
+        tuned, app_tune = tune_apply(csr_step)
+        log("[bench] CSR apply autotune (G,epl,vec,block) -> %s: %s" % (
             tuned[0], ", ".join(f"{k}:{v * 1e3:.1f}us" for k, v in app_tune.items())))
+        (sell_unroll, sell_ms), sell_tune = tune_sell(step)
+        log("[bench] SELL apply unroll -> %d: %s" % (
+            sell_unroll, ", ".join(f"{k}:{v * 1e3:.1f}us" for k, v in sell_tune.items())))
+        if tuned[1] < sell_ms:
+            step = csr_step
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -272,10 +284,12 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "3d_lattice128_jitter0.1h_minus10pct_periodic_rcut2.4h_order2_lap+dx",
-                   "particles": N, "nnz": nnz, "operators": ["Laplacian", "Derivative_x"],
+                   "particles": N, "nnz": nnz, "particle_order": args.order, "operators": ["Laplacian", "Derivative_x"],
                    "apply_precision": "fp32", "assembly_precision": "fp64",
-                   "apply_launch": {"group_lanes": tuned[0][0], "elems_per_lane": tuned[0][1],
-                                    "vector_width": tuned[0][2], "block": tuned[0][3]} if tuned else None},
+                   "apply_layout": "sell64" if step.sell else "csr",
+                   "apply_launch": ({"sell_unroll": sell_unroll} if step.sell else
+                                    {"group_lanes": tuned[0][0], "elems_per_lane": tuned[0][1],
+                                     "vector_width": tuned[0][2], "block": tuned[0][3]}) if tuned else None},
         "assembly_ms": assembly_ms,
         "assembly_ms_by_operator": {"laplacian": asm_lap_ms, "dx": asm_dx_ms},
         "support_ms": support_ms,
@@ -283,7 +297,7 @@ def main():
         "fp64_error_vs_analytic": ana,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_apply<float,G,EPL,2,1,diff>", "alg_bytes_per_launch": alg_bytes,
+                     "kernel": "k_apply_sell<2,UNR,diff>" if step.sell else "k_apply_v4<G,NCH,2,1,diff>", "alg_bytes_per_launch": alg_bytes,
                      "launch_ms": launch_ms},
         "roofline_assembly": {"bound": "fp64", "achieved": asm_tflops, "peak": FP64_PEAK_TFLOPS,
                               "unit": "TFLOP/s", "frac": asm_tflops / FP64_PEAK_TFLOPS,

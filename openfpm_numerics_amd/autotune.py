@@ -2,7 +2,8 @@
 
 Times each candidate with HIP events on the current stream (the stream the
 library launches on) and keeps the fastest in the library context:
-  apply:    (lanes per row, nonzeros per lane) shapes x threads per block {256, 512}
+  apply:    CSR (lanes per row, nonzeros per lane, load width) x threads per block;
+            sliced-ELL groups of 4 slots loaded ahead {2, 4}
   assembly: particles each wavefront assembles in sequence {1, 2, 4, 8, 16}
 """
 from __future__ import annotations
@@ -47,6 +48,23 @@ def tune_apply(fn, shapes=APPLY_SHAPES, blocks=(256, 512), reps=10, device=0):
     finally:
         cfg = best[0] if best else (0, 8, 4, 256)
         ctx.check(ctx.lib.dcpse_set_apply_config(ctx.h, *cfg), "dcpse_set_apply_config")
+    return best, results
+
+
+def tune_sell(fn, unrolls=(2, 4), reps=10, device=0):
+    """fn() runs one sliced-ELL apply; returns (unroll, ms) of the fastest."""
+    ctx = _lib.context(device)
+    best = None
+    results = {}
+    try:
+        for u in unrolls:
+            ctx.check(ctx.lib.dcpse_set_sell_unroll(ctx.h, u), "dcpse_set_sell_unroll")
+            ms = _time(fn, reps)
+            results[u] = ms
+            if best is None or ms < best[1]:
+                best = (u, ms)
+    finally:
+        ctx.check(ctx.lib.dcpse_set_sell_unroll(ctx.h, best[0] if best else 4), "dcpse_set_sell_unroll")
     return best, results
 
 

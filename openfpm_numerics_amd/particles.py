@@ -63,6 +63,22 @@ class ParticleSet:
         self.props[name] = t
         return t
 
+    def reorder(self, order: str = "morton") -> np.ndarray:
+        """Sorts the particles along a space-filling curve (Morton / Z-order over
+        the box), the counterpart of openfpm's vector_dist::reorder. Positions and
+        properties are permuted in place; returns perm with new[i] = old[perm[i]].
+        Verlet lists and operators built before must be rebuilt (map counter)."""
+        if order != "morton":
+            raise ValueError("only 'morton' ordering is implemented")
+        pos = self.pos_soa.t().cpu().numpy()
+        perm = morton_order(pos, self.box_lo, self.box_hi)
+        idx = torch.as_tensor(perm, device=self.device)
+        self.pos_soa = self.pos_soa[:, idx].contiguous()
+        for k, v in self.props.items():
+            self.props[k] = v[..., idx].contiguous() if v.shape[-1] == perm.size else v
+        self.map_ctr += 1
+        return perm
+
     def getVerlet(self, rCut) -> "VerletList":
         return VerletList.build(self, rCut)
 
@@ -71,6 +87,22 @@ class ParticleSet:
         vl.__dict__.update(fresh.__dict__)
         vl._sell = None
         return vl
+
+
+def morton_order(pos: np.ndarray, box_lo, box_hi, bits: int | None = None) -> np.ndarray:
+    """Permutation sorting (N, dim) points by their Morton code over the box."""
+    pos = np.asarray(pos, dtype=np.float64)
+    n, dim = pos.shape
+    bits = bits or (21 if dim == 3 else 31)
+    lo = np.asarray(box_lo, dtype=np.float64)
+    hi = np.asarray(box_hi, dtype=np.float64)
+    cells = (1 << bits) - 1
+    q = np.clip(np.floor((pos - lo) / (hi - lo) * cells), 0, cells).astype(np.uint64)
+    code = np.zeros(n, np.uint64)
+    for b in range(bits):
+        for d in range(dim):
+            code |= ((q[:, d] >> np.uint64(b)) & np.uint64(1)) << np.uint64(b * dim + d)
+    return np.argsort(code, kind="stable")
 
 
 class VerletList:
