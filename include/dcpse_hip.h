@@ -184,6 +184,14 @@ dcpse_status dcpse_apply_transpose_f64(dcpse_ctx* ctx, int64_t n_cols, const int
                                        const double* g, int64_t ld_g, int32_t n_fields,
                                        double* out, int64_t ld_out, double alpha, double beta);
 
+/* ---- elementwise helper of the expression evaluation ------------------ */
+/* y[i] = alpha * x[i] + beta * y[i] for i < n (beta == 0 never reads y); the
+ * device counterpart of the per-particle SUM/MUL nodes the reference's host
+ * evaluation loop applies (src/Operators/Vector/vector_dist_operators.hpp,
+ * evaluated by src/Operators/Vector/cuda/vector_dist_operators_cuda.cuh:445-458). */
+dcpse_status dcpse_axpby_f32(dcpse_ctx* ctx, int64_t n, float alpha, const float* x, float beta, float* y);
+dcpse_status dcpse_axpby_f64(dcpse_ctx* ctx, int64_t n, double alpha, const double* x, double beta, double* y);
+
 /* ---- sliced-ELL execution layout (FP32 apply) ------------------------- */
 /* The same operator laid out one wavefront per slice of 64 consecutive rows:
  * slice s holds rows [64s, 64s+64) with width w_s = longest row rounded up to

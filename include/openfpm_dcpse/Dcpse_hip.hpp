@@ -1,0 +1,290 @@
+// Dcpse_hip<dim, VerletList, vector_type>: the Dcpse_type plug point of the
+// reference operator classes (template<unsigned, typename, typename...> class
+// Dcpse_type, src/DCPSE/DCPSE_op/DCPSE_op.hpp:649) implemented on the gfx950
+// kernels of libdcpse_hip.so. It mirrors the reference Dcpse
+// (src/DCPSE/Dcpse.hpp:43-826): the same constructors (:91-130), the bulk
+// computeDifferentialOperator<f, Df> (:467-499), the row accessors getNumNN /
+// getCoeffNN / getIndexNN / getSign / getEpsilonInvPrefactor (:507-550),
+// initializeUpdate (:661-687), save / load (:302-379) and DrawKernel
+// (:151-193). The per-particle computeDifferentialOperator(key, expr) used by
+// the reference's host expression loop is replaced by whole-field device
+// applies (see DCPSE_op.hpp in this directory).
+#pragma once
+
+#include <cstdint>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "dcpse_runtime.hpp"
+#include "vector_dist_hip.hpp"
+
+#ifndef OPENFPM_DCPSE_NO_VALUE_TYPES
+enum support_options { RADIUS, LOAD, ADAPTIVE, AT_LEAST_N_PARTICLES };
+#endif
+
+namespace openfpm_dcpse {
+
+// File layout shared with the Python package: "DCPSEHIP", u32 version, dim,
+// n_sig, order, f64 h_over_eps, i64 n, nnz, u32 signatures[n_sig*dim],
+// f64 coeffs[n_sig], f64 eps[n], eps_inv_pow[n], K[nnz], i64 row_ptr[n+1],
+// i32 col[nnz]; little endian, written to <file>_<rank>.
+constexpr char kFileMagic[8] = {'D', 'C', 'P', 'S', 'E', 'H', 'I', 'P'};
+constexpr uint32_t kFileVersion = 1;
+
+}  // namespace openfpm_dcpse
+
+// The trailing pack matches the reference plug point's
+// template<unsigned int, typename, typename...> signature; a second particle
+// set (support != domain) is not supported by this container.
+template <unsigned int dim, typename VerletList_type, typename vector_type, typename... vector_type2>
+class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::prop_type> {
+    static_assert(sizeof...(vector_type2) == 0, "separate support and domain particle sets are not supported");
+
+   public:
+    typedef typename vector_type::stype T;
+    typedef typename vector_type::prop_type prop_type;
+    typedef vector_type vtype;
+
+    T HOverEpsilon = 0.9;
+
+    template <typename PointT>
+    Dcpse_hip(vector_type& particles, VerletList_type& verletList, PointT differentialSignature,
+              unsigned int convergenceOrder, T rCut, support_options opt = support_options::RADIUS)
+        : particles_(particles), verletList_(verletList), convergenceOrder_(convergenceOrder), rCut_(rCut) {
+        std::vector<uint32_t> sig(dim);
+        for (unsigned d = 0; d < dim; ++d) sig[d] = (uint32_t)differentialSignature.value(d);
+        sigs_ = sig;
+        coeffs_ = {1.0};
+        finish_init(opt);
+    }
+
+    // Linear combination sum_k coeffs[k] D^{sig_k} of signatures sharing one basis
+    // (the Laplacian assembles sum_d D^{2 e_d} this way).
+    Dcpse_hip(vector_type& particles, VerletList_type& verletList, const std::vector<uint32_t>& signatures,
+              const std::vector<double>& coeffs, unsigned int convergenceOrder, T rCut,
+              support_options opt = support_options::RADIUS)
+        : particles_(particles), verletList_(verletList), convergenceOrder_(convergenceOrder), rCut_(rCut),
+          sigs_(signatures), coeffs_(coeffs) {
+        finish_init(opt);
+    }
+
+    // ---- apply ---------------------------------------------------------------------------
+    template <unsigned int fValuePos, unsigned int DfValuePos>
+    void computeDifferentialOperator(vector_type& particles) {
+        const prop_type* f = particles.devProp(fValuePos);
+        prop_type* out = particles.devProp(DfValuePos);
+        apply(f, out, prop_type(1), prop_type(0));
+        particles.devPropWritten(DfValuePos);
+    }
+
+    // out = alpha * D f + beta * out on device arrays of the particle count
+    void apply(const prop_type* f, prop_type* out, prop_type alpha, prop_type beta) override {
+        auto& c = openfpm_dcpse::context();
+        check_fresh();
+        const int64_t n = (int64_t)verletList_.size();
+        if constexpr (std::is_same<prop_type, float>::value) {
+            const float* W[1] = {W32_.data()};
+            float* O[1] = {out};
+            c.check(dcpse_apply_f32(c.get(), n, verletList_.devRowPtr(), verletList_.devCol(), 1, W, f, n, 1,
+                                    nullptr, n, O, n, alpha, beta),
+                    "dcpse_apply_f32");
+        } else {
+            const double* W[1] = {W64_.data()};
+            double* O[1] = {out};
+            c.check(dcpse_apply_f64(c.get(), n, verletList_.devRowPtr(), verletList_.devCol(), 1, W, f, n, 1,
+                                    nullptr, n, O, n, alpha, beta),
+                    "dcpse_apply_f64");
+        }
+    }
+
+    // ---- row accessors (value_nz contract, DCPSE_op.hpp:69-83) ------------------------------
+    int getNumNN(const vect_dist_key_dx& p) { return (int)verletList_.getNNPart(p); }
+    T getCoeffNN(const vect_dist_key_dx& p, int j) {
+        mirror();
+        return (T)K_host_[rp_host_[p.getKey()] + j];
+    }
+    size_t getIndexNN(const vect_dist_key_dx& p, int j) { return verletList_.get(p, j); }
+    T getSign() {
+        const unsigned o = order_of_derivative();
+        return (o % 2 == 0 && o != 0) ? T(-1) : T(1);
+    }
+    T getEpsilonInvPrefactor(const vect_dist_key_dx& p) {
+        mirror();
+        return (T)eip_host_[p.getKey()];
+    }
+
+    // ---- lifecycle ------------------------------------------------------------------------
+    void initializeUpdate(vector_type& particles) {
+        (void)particles;
+        assemble();
+    }
+
+    template <unsigned int prp>
+    void DrawKernel(vector_type& particles, int p) {
+        mirror();
+        for (int64_t j = rp_host_[p]; j < rp_host_[p + 1]; ++j)
+            particles.template getProp<prp>((size_t)col_host_[j]) += (prop_type)K_host_[j];
+    }
+    template <unsigned int prp>
+    void DrawKernelNN(vector_type& particles, int p) {
+        mirror();
+        for (int64_t j = rp_host_[p]; j < rp_host_[p + 1]; ++j)
+            particles.template getProp<prp>((size_t)col_host_[j]) = prop_type(1);
+    }
+
+    void save(const std::string& file) {
+        mirror();
+        std::ofstream out(file + "_0", std::ios::binary);
+        if (!out) throw openfpm_dcpse::dcpse_error("cannot write " + file + "_0");
+        const uint32_t hdr[4] = {openfpm_dcpse::kFileVersion, dim, (uint32_t)coeffs_.size(), convergenceOrder_};
+        const double hoe = (double)HOverEpsilon;
+        const int64_t n = (int64_t)eps_host_.size(), nnz = (int64_t)K_host_.size();
+        out.write(openfpm_dcpse::kFileMagic, 8);
+        out.write((const char*)hdr, sizeof(hdr));
+        out.write((const char*)&hoe, 8);
+        out.write((const char*)&n, 8);
+        out.write((const char*)&nnz, 8);
+        out.write((const char*)sigs_.data(), 4 * sigs_.size());
+        out.write((const char*)coeffs_.data(), 8 * coeffs_.size());
+        out.write((const char*)eps_host_.data(), 8 * n);
+        out.write((const char*)eip_host_.data(), 8 * n);
+        out.write((const char*)K_host_.data(), 8 * nnz);
+        out.write((const char*)rp_host_.data(), 8 * (n + 1));
+        out.write((const char*)col_host_.data(), 4 * nnz);
+    }
+
+    // Loads kernels saved by save() (or by the Python package) for the same
+    // signature and support; the Verlet list of this operator must match.
+    void load(const std::string& file) {
+        std::ifstream in(file + "_0", std::ios::binary);
+        if (!in) throw openfpm_dcpse::dcpse_error("cannot open " + file + "_0");
+        char magic[8];
+        uint32_t hdr[4];
+        double hoe;
+        int64_t n, nnz;
+        in.read(magic, 8);
+        in.read((char*)hdr, sizeof(hdr));
+        in.read((char*)&hoe, 8);
+        in.read((char*)&n, 8);
+        in.read((char*)&nnz, 8);
+        if (!in || std::memcmp(magic, openfpm_dcpse::kFileMagic, 8) != 0 || hdr[0] != openfpm_dcpse::kFileVersion ||
+            hdr[1] != dim)
+            throw openfpm_dcpse::dcpse_error(file + "_0 is not a DC-PSE kernel file of this dimension");
+        std::vector<uint32_t> sig(hdr[2] * dim);
+        std::vector<double> co(hdr[2]);
+        in.read((char*)sig.data(), 4 * sig.size());
+        in.read((char*)co.data(), 8 * co.size());
+        if (sig != sigs_) throw openfpm_dcpse::dcpse_error("kernel file holds a different differential signature");
+        if ((size_t)n != verletList_.size() || (size_t)nnz != verletList_.nnz())
+            throw openfpm_dcpse::dcpse_error("kernel file support differs from this operator's Verlet list");
+        eps_host_.resize(n);
+        eip_host_.resize(n);
+        K_host_.resize(nnz);
+        rp_host_.resize(n + 1);
+        col_host_.resize(nnz);
+        in.read((char*)eps_host_.data(), 8 * n);
+        in.read((char*)eip_host_.data(), 8 * n);
+        in.read((char*)K_host_.data(), 8 * nnz);
+        in.read((char*)rp_host_.data(), 8 * (n + 1));
+        in.read((char*)col_host_.data(), 4 * nnz);
+        if (!in) throw openfpm_dcpse::dcpse_error(file + "_0 is truncated");
+        convergenceOrder_ = hdr[3];
+        HOverEpsilon = (T)hoe;
+        coeffs_ = co;
+        // pre-scaled weights W = eps^-|beta| K for the apply kernels
+        std::vector<double> W64(nnz);
+        std::vector<float> W32(nnz);
+        for (int64_t p = 0; p < n; ++p)
+            for (int64_t j = rp_host_[p]; j < rp_host_[p + 1]; ++j) {
+                W64[j] = K_host_[j] * eip_host_[p];
+                W32[j] = (float)W64[j];
+            }
+        K64_.upload(K_host_);
+        W64_.upload(W64);
+        W32_.upload(W32);
+        mirrored_ = true;
+        ctr_ = particles_.getMapCtr();
+    }
+
+    const std::vector<uint32_t>& signatures() const { return sigs_; }
+
+   private:
+    unsigned order_of_derivative() const {
+        unsigned o = 0;
+        for (unsigned d = 0; d < dim; ++d) o += sigs_[d];
+        return o;
+    }
+    void finish_init(support_options opt) {
+        if (opt != support_options::LOAD) assemble();
+    }
+    void check_fresh() {
+        if (W64_.size() == 0 && verletList_.nnz() != 0)
+            throw openfpm_dcpse::dcpse_error("DC-PSE operator has no kernels: call load() or update()");
+        if (ctr_ != particles_.getMapCtr())
+            throw openfpm_dcpse::dcpse_error("particles moved since assembly: call update() first");
+    }
+    void assemble() {
+        auto& c = openfpm_dcpse::context();
+        const size_t n = verletList_.size(), nnz = verletList_.nnz();
+        eps_.resize(n);
+        eip_.resize(n);
+        info_.resize(n);
+        K64_.resize(nnz);
+        W64_.resize(nnz);
+        W32_.resize(nnz);
+        double lo[dim], hi[dim];
+        for (unsigned d = 0; d < dim; ++d) {
+            lo[d] = (double)particles_.boxLow()[d];
+            hi[d] = (double)particles_.boxHigh()[d];
+        }
+        c.check(dcpse_assemble(c.get(), dim, particles_.devPos(), (int64_t)n, nullptr, (int64_t)n, lo, hi,
+                               particles_.periodicMask(), verletList_.devRowPtr(), verletList_.devCol(),
+                               (int32_t)coeffs_.size(), sigs_.data(), coeffs_.data(), convergenceOrder_,
+                               (double)HOverEpsilon, DCPSE_ASM_REFINE, eps_.data(), eip_.data(), K64_.data(),
+                               W32_.data(), W64_.data(), info_.data()),
+                "dcpse_assemble");
+        const std::vector<int32_t> info = info_.to_host();
+        for (size_t p = 0; p < n; ++p)
+            if (info[p] == DCPSE_INFO_TOO_FEW)
+                throw std::length_error("Not enough neighbour points passed for Vandermonde matrix construction!");
+            else if (info[p] != DCPSE_INFO_OK)
+                throw openfpm_dcpse::dcpse_error("DC-PSE assembly failed at particle " + std::to_string(p) +
+                                                 " (info " + std::to_string(info[p]) + ")");
+        mirrored_ = false;
+        ctr_ = particles_.getMapCtr();
+    }
+    void mirror() {
+        if (mirrored_) return;
+        eps_host_ = eps_.to_host();
+        eip_host_ = eip_.to_host();
+        K_host_ = K64_.to_host();
+        std::vector<int64_t> rp(verletList_.size() + 1);
+        hip_copy(rp.data(), verletList_.devRowPtr(), rp.size());
+        rp_host_ = rp;
+        col_host_.resize(verletList_.nnz());
+        hip_copy(col_host_.data(), verletList_.devCol(), col_host_.size());
+        mirrored_ = true;
+    }
+    template <typename U>
+    static void hip_copy(U* dst, const U* src, size_t n) {
+        if (n) openfpm_dcpse::hip_check(hipMemcpy(dst, src, n * sizeof(U), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    }
+
+    vector_type& particles_;
+    VerletList_type& verletList_;
+    unsigned int convergenceOrder_;
+    T rCut_;
+    std::vector<uint32_t> sigs_;
+    std::vector<double> coeffs_;
+    openfpm_dcpse::device_array<double> eps_, eip_, K64_, W64_;
+    openfpm_dcpse::device_array<float> W32_;
+    openfpm_dcpse::device_array<int32_t> info_;
+    std::vector<double> eps_host_, eip_host_, K_host_;
+    std::vector<int64_t> rp_host_;
+    std::vector<int32_t> col_host_;
+    bool mirrored_ = false;
+    size_t ctr_ = 0;
+};

@@ -1,4 +1,5 @@
-"""Builds libdcpse_hip.so (gfx950) in-tree with hipcc."""
+"""Builds libdcpse_hip.so (gfx950) in-tree with hipcc, and the C++ drop-in
+test program that links it (tests/cpp/dropin_test)."""
 from __future__ import annotations
 
 import os
@@ -42,5 +43,26 @@ def build(verbose: bool = False, extra=None) -> str:
     return LIB
 
 
+DROPIN_SRC = os.path.join(ROOT, "tests", "cpp", "dropin_test.cpp")
+DROPIN_BIN = os.path.join(ROOT, "tests", "cpp", "dropin_test")
+
+
+def build_dropin_test(verbose: bool = False) -> str:
+    """Host-only C++ program over include/openfpm_dcpse (g++, HIP runtime
+    headers, linked against the in-tree libdcpse_hip.so by rpath)."""
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__",
+           "-I", os.path.join(ROOT, "include"), "-I", os.path.join(rocm, "include"),
+           DROPIN_SRC, "-o", DROPIN_BIN + ".tmp",
+           "-L", HERE, "-ldcpse_hip", "-Wl,-rpath,$ORIGIN/../../openfpm_numerics_amd",
+           "-L", os.path.join(rocm, "lib"), "-lamdhip64", "-Wl,-rpath," + os.path.join(rocm, "lib")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(DROPIN_BIN + ".tmp", DROPIN_BIN)
+    return DROPIN_BIN
+
+
 if __name__ == "__main__":
     build(verbose=True, extra=sys.argv[1:])
+    build_dropin_test(verbose=True)

@@ -409,6 +409,25 @@ static dcpse_status apply_impl(dcpse_ctx* ctx, int64_t n_rows, const int64_t* ro
     return DCPSE_OK;
 }
 
+// y = alpha * x + beta * y (beta == 0 never reads y)
+template <typename T>
+__global__ void k_axpby(int64_t n, T alpha, const T* __restrict__ x, T beta, T* __restrict__ y) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = beta != T(0) ? fma(alpha, x[i], beta * y[i]) : alpha * x[i];
+}
+
+template <typename T>
+static dcpse_status axpby(dcpse_ctx* ctx, int64_t n, T alpha, const T* x, T beta, T* y) {
+    if (!ctx) return DCPSE_ERR_INVALID;
+    if (n < 0 || (n > 0 && (!x || !y))) return fail(ctx, DCPSE_ERR_INVALID, "null pointer or negative size");
+    DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    if (n == 0) return DCPSE_OK;
+    const int64_t b = std::min<int64_t>((n + 255) / 256, (int64_t)ctx->num_cus * 8);
+    k_axpby<T><<<(unsigned)b, 256, 0, ctx->stream>>>(n, alpha, x, beta, y);
+    DCPSE_LAUNCH_CHECK(ctx);
+    return DCPSE_OK;
+}
+
 }  // namespace dcpse
 
 using namespace dcpse;
@@ -455,6 +474,14 @@ dcpse_status dcpse_apply_transpose_f64(dcpse_ctx* ctx, int64_t n_cols, const int
     double* Op[1] = {out};
     return apply_impl<double, kExplicitDiag>(ctx, n_cols, rowT_ptr, colT, 1, Wp, g, ld_g, n_fields, nullptr, ld_g,
                                              diag, Op, ld_out, alpha, beta);
+}
+
+dcpse_status dcpse_axpby_f32(dcpse_ctx* ctx, int64_t n, float alpha, const float* x, float beta, float* y) {
+    return axpby<float>(ctx, n, alpha, x, beta, y);
+}
+
+dcpse_status dcpse_axpby_f64(dcpse_ctx* ctx, int64_t n, double alpha, const double* x, double beta, double* y) {
+    return axpby<double>(ctx, n, alpha, x, beta, y);
 }
 
 }  // extern "C"

@@ -1,0 +1,348 @@
+// C++ drop-in checks: the operator classes and Dcpse_type of
+// include/openfpm_dcpse used exactly as the reference's own tests use theirs.
+//   Dcpse_2D_test, Dcpse_3D_test        src/DCPSE/tests/Dcpse_unit_tests.cpp:36-133, 243-337
+//   dcpse_op_tests, dcpse_op_save_load,
+//   dcpse_op_test_lap                   src/DCPSE/DCPSE_op/tests/DCPSE_op_test_base_tests.cpp:30-175, 394-468
+// plus FP32 fields, expression aliasing and the row accessors (value_nz
+// contract) checked against the apply. Prints one line per case and exits
+// non-zero if any fails.
+#include <cmath>
+#include <cstdio>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "openfpm_dcpse/DCPSE_op_hip.hpp"
+
+using openfpm_dcpse::vector_dist_hip;
+
+static int failures = 0;
+
+static void run(const char* name, const std::function<bool()>& f) {
+    std::printf("[ RUN  ] %s\n", name);
+    std::fflush(stdout);
+    bool ok = false;
+    try {
+        ok = f();
+    } catch (const std::exception& e) {
+        std::printf("         exception: %s\n", e.what());
+    }
+    std::printf("[ %s ] %s\n", ok ? " OK " : "FAIL", name);
+    std::fflush(stdout);
+    if (!ok) ++failures;
+}
+
+template <typename V>
+static double worst_diff(V& domain, unsigned a, unsigned b) {
+    double worst = 0.0;
+    auto it = domain.getDomainIterator();
+    while (it.isNext()) {
+        auto p = it.get();
+        double va = 0, vb = 0;
+        switch (a) {
+            case 1: va = domain.template getProp<1>(p); break;
+            case 3: va = domain.template getProp<3>(p); break;
+        }
+        switch (b) {
+            case 2: vb = domain.template getProp<2>(p); break;
+            case 3: vb = domain.template getProp<3>(p); break;
+            case 4: vb = domain.template getProp<4>(p); break;
+        }
+        worst = std::fmax(worst, std::fabs(va - vb));
+        ++it;
+    }
+    return worst;
+}
+
+// Reference Dcpse_2D_test: d/dx sin(x) on a 40x40 lattice of [0,1]^2, rCut 3.1h,
+// |error| < 2 (2h)^2 everywhere.
+static bool dcpse_2d_test() {
+    const size_t sz[2] = {40, 40};
+    Box<2, double> box({0, 0}, {1, 1});
+    size_t bc[2] = {NON_PERIODIC, NON_PERIODIC};
+    const double h[2] = {1.0 / (sz[0] - 1), 1.0 / (sz[1] - 1)};
+    const double rCut = 3.1 * h[0];
+    vector_dist_hip<2, double, 3> domain(0, box, bc, Ghost<2, double>(rCut));
+    auto it = domain.getGridIterator(sz);
+    while (it.isNext()) {
+        domain.add();
+        auto key = it.get();
+        const double x = key.get(0) * h[0], y = key.get(1) * h[1];
+        domain.getLastPos()[0] = x;
+        domain.getLastPos()[1] = y;
+        domain.getLastProp<0>() = std::sin(x);
+        domain.getLastProp<2>() = std::cos(x);
+        ++it;
+    }
+    domain.map();
+    domain.ghost_get<0>();
+    auto verletList = domain.getVerlet<VL_NON_SYMMETRIC | VL_SKIP_REF_PART>(rCut);
+    Dcpse_hip<2, decltype(verletList), decltype(domain)> dcpse(domain, verletList, Point<2, unsigned int>({1, 0}), 2,
+                                                             rCut, support_options::RADIUS);
+    dcpse.computeDifferentialOperator<0, 1>(domain);
+    const double avg = h[0] + h[1], TOL = 2 * avg * avg;
+    const double worst = worst_diff(domain, 1, 2);
+    std::printf("         worst %.3e tol %.3e\n", worst, TOL);
+    return worst < TOL;
+}
+
+// Reference Dcpse_3D_test: d/dz sin(z) on a 40^3 lattice, |error| < (3h)^2.
+static bool dcpse_3d_test() {
+    const size_t sz[3] = {40, 40, 40};
+    Box<3, double> box({0, 0, 0}, {1, 1, 1});
+    size_t bc[3] = {NON_PERIODIC, NON_PERIODIC, NON_PERIODIC};
+    const double h = 1.0 / (sz[0] - 1);
+    const double rCut = 3.1 * h;
+    vector_dist_hip<3, double, 3> domain(0, box, bc, Ghost<3, double>(rCut));
+    auto it = domain.getGridIterator(sz);
+    while (it.isNext()) {
+        domain.add();
+        auto key = it.get();
+        for (unsigned d = 0; d < 3; ++d) domain.getLastPos()[d] = key.get(d) * h;
+        const double z = key.get(2) * h;
+        domain.getLastProp<0>() = std::sin(z);
+        domain.getLastProp<2>() = std::cos(z);
+        ++it;
+    }
+    domain.map();
+    auto verletList = domain.getVerlet<VL_NON_SYMMETRIC | VL_SKIP_REF_PART>(rCut);
+    Dcpse_hip<3, decltype(verletList), decltype(domain)> dcpse(domain, verletList, Point<3, unsigned int>({0, 0, 1}),
+                                                             2, rCut, support_options::RADIUS);
+    dcpse.computeDifferentialOperator<0, 1>(domain);
+    const double avg = 3 * h, TOL = avg * avg;
+    const double worst = worst_diff(domain, 1, 2);
+    std::printf("         worst %.3e tol %.3e\n", worst, TOL);
+    return worst < TOL;
+}
+
+// 80x80 lattice of [0, 2pi]^2 (reference dcpse_op_tests set-up);
+// prop 0 = sin x + sin y, prop 2 = 2 cos x + cos y.
+template <typename V>
+static void fill_op_domain(V& domain, const size_t (&sz)[2], const double (&h)[2]) {
+    auto it = domain.getGridIterator(sz);
+    while (it.isNext()) {
+        domain.add();
+        auto key = it.get();
+        const double x = key.get(0) * h[0], y = key.get(1) * h[1];
+        domain.getLastPos()[0] = x;
+        domain.getLastPos()[1] = y;
+        domain.template getLastProp<0>() = std::sin(x) + std::sin(y);
+        domain.template getLastProp<2>() = 2 * std::cos(x) + std::cos(y);
+        ++it;
+    }
+    domain.map();
+    domain.template ghost_get<0>();
+}
+
+static bool dcpse_op_tests() {
+    const size_t sz[2] = {80, 80};
+    Box<2, double> box({0, 0}, {2 * M_PI, 2 * M_PI});
+    size_t bc[2] = {NON_PERIODIC, NON_PERIODIC};
+    const double h[2] = {2 * M_PI / (sz[0] - 1), 2 * M_PI / (sz[1] - 1)};
+    const double rCut = 3.9 * h[0];
+    vector_dist_hip<2, double, 5> domain(0, box, bc, Ghost<2, double>(rCut));
+    fill_op_domain(domain, sz, h);
+    auto verletList = domain.getVerlet<VL_NON_SYMMETRIC | VL_SKIP_REF_PART>(rCut);
+    Derivative_x<decltype(verletList)> Dx(domain, verletList, 2, rCut);
+    Derivative_y<decltype(verletList)> Dy(domain, verletList, 2, rCut);
+    Laplacian<decltype(verletList)> Lap(domain, verletList, 2, rCut);
+    auto v = getV<1>(domain);
+    auto P = getV<0>(domain);
+    v = 2 * Dx(P) + Dy(P);
+    const double worst = worst_diff(domain, 1, 2);
+    std::printf("         worst %.3e (reference bound 0.03)\n", worst);
+    return worst < 0.03;
+}
+
+static bool dcpse_op_save_load() {
+    const size_t sz[2] = {80, 80};
+    Box<2, double> box({0, 0}, {2 * M_PI, 2 * M_PI});
+    size_t bc[2] = {NON_PERIODIC, NON_PERIODIC};
+    const double h[2] = {2 * M_PI / (sz[0] - 1), 2 * M_PI / (sz[1] - 1)};
+    const double rCut = 3.9 * h[0];
+    vector_dist_hip<2, double, 5> domain(0, box, bc, Ghost<2, double>(rCut));
+    fill_op_domain(domain, sz, h);
+    auto verletList = domain.getVerlet<VL_NON_SYMMETRIC | VL_SKIP_REF_PART>(rCut);
+    Derivative_x<decltype(verletList)> Dx(domain, verletList, 2, rCut);
+    Derivative_y<decltype(verletList)> Dy(domain, verletList, 2, rCut);
+    auto v = getV<1>(domain);
+    auto v2 = getV<3>(domain);
+    auto P = getV<0>(domain);
+    v2 = 2 * Dx(P) + Dy(P);
+    Dx.save(domain, "DX_test_hip");
+    Dy.save(domain, "DY_test_hip");
+    Derivative_x<decltype(verletList)> DxLoaded(domain, verletList, 2, rCut, support_options::LOAD);
+    Derivative_y<decltype(verletList)> DyLoaded(domain, verletList, 2, rCut, support_options::LOAD);
+    DxLoaded.load(domain, "DX_test_hip");
+    DyLoaded.load(domain, "DY_test_hip");
+    v = 2 * DxLoaded(P) + DyLoaded(P);
+    std::remove("DX_test_hip_0");
+    std::remove("DY_test_hip_0");
+    const double worst = worst_diff(domain, 1, 2);
+    const double same = worst_diff(domain, 1, 3);  // loaded vs assembled: identical
+    std::printf("         worst %.3e (bound 0.03), loaded vs assembled max diff %.3e\n", worst, same);
+    return worst < 0.03 && same == 0.0;
+}
+
+static bool dcpse_op_test_lap() {
+    const size_t sz[2] = {163, 163};
+    Box<2, double> box({0, 0}, {2 * M_PI, 2 * M_PI});
+    size_t bc[2] = {NON_PERIODIC, NON_PERIODIC};
+    const double h[2] = {2 * M_PI / (sz[0] - 1), 2 * M_PI / (sz[1] - 1)};
+    const double rCut = 3.9 * h[0];
+    vector_dist_hip<2, double, 4> domain(0, box, bc, Ghost<2, double>(rCut));
+    auto it = domain.getGridIterator(sz);
+    while (it.isNext()) {
+        domain.add();
+        auto key = it.get();
+        const double x = key.get(0) * h[0], y = key.get(1) * h[1];
+        domain.getLastPos()[0] = x;
+        domain.getLastPos()[1] = y;
+        domain.getLastProp<0>() = std::sin(x) + std::sin(y);
+        domain.getLastProp<1>() = -std::sin(x) - std::sin(y);
+        ++it;
+    }
+    domain.map();
+    auto verletList = domain.getVerlet<VL_NON_SYMMETRIC | VL_SKIP_REF_PART>(rCut);
+    Laplacian<decltype(verletList)> Lap(domain, verletList, 2, rCut);
+    auto v = getV<1>(domain);
+    auto P = getV<0>(domain);
+    auto vv = getV<2>(domain);
+    auto errv = getV<3>(domain);
+    vv = Lap(P);
+    const double worst = worst_diff(domain, 1, 2);
+    errv = v - vv;
+    double maxerr = 0.0;
+    for (size_t p = 0; p < domain.size_local(); ++p) maxerr = std::fmax(maxerr, std::fabs(domain.getProp<3>(p)));
+    std::printf("         worst %.3e (reference bound 0.3), |v - vv| field max %.3e\n", worst, maxerr);
+    return worst < 0.3 && std::fabs(maxerr - worst) <= 1e-12 * (1 + worst);
+}
+
+// FP32 fields run the FP32 kernels; compared with the FP64 result of the same
+// operator (rel. L2 within 1e-5).
+static bool dcpse_fp32_fields() {
+    const size_t sz[2] = {80, 80};
+    Box<2, double> box({0, 0}, {2 * M_PI, 2 * M_PI});
+    size_t bc[2] = {PERIODIC, PERIODIC};
+    const double h[2] = {2 * M_PI / sz[0], 2 * M_PI / sz[1]};
+    const double rCut = 3.1 * h[0];
+    vector_dist_hip<2, double, 3> d64(0, box, bc, Ghost<2, double>(rCut));
+    vector_dist_hip<2, double, 3, float> d32(0, box, bc, Ghost<2, double>(rCut));
+    auto it = d64.getGridIterator(sz);
+    while (it.isNext()) {
+        auto key = it.get();
+        const double x = key.get(0) * h[0], y = key.get(1) * h[1];
+        d64.add();
+        d32.add();
+        d64.getLastPos()[0] = d32.getLastPos()[0] = x;
+        d64.getLastPos()[1] = d32.getLastPos()[1] = y;
+        const float f = (float)(std::sin(x) * std::cos(2 * y));
+        d32.getLastProp<0>() = f;
+        d64.getLastProp<0>() = (double)f;
+        ++it;
+    }
+    auto vl64 = d64.getVerlet<VL_NON_SYMMETRIC | VL_SKIP_REF_PART>(rCut);
+    auto vl32 = d32.getVerlet<VL_NON_SYMMETRIC | VL_SKIP_REF_PART>(rCut);
+    Laplacian<decltype(vl64)> L64(d64, vl64, 2, rCut);
+    Derivative_x<decltype(vl64)> X64(d64, vl64, 2, rCut);
+    Laplacian<decltype(vl32)> L32(d32, vl32, 2, rCut);
+    Derivative_x<decltype(vl32)> X32(d32, vl32, 2, rCut);
+    getV<1>(d64) = L64(getV<0>(d64)) + 0.5 * X64(getV<0>(d64));
+    getV<1>(d32) = L32(getV<0>(d32)) + 0.5 * X32(getV<0>(d32));
+    double num = 0, den = 0;
+    for (size_t p = 0; p < d64.size_local(); ++p) {
+        const double a = d64.getProp<1>(p), b = d32.getProp<1>(p);
+        num += (a - b) * (a - b);
+        den += a * a;
+    }
+    const double rel = std::sqrt(num / den);
+    std::printf("         FP32 vs FP64 rel L2 %.3e\n", rel);
+    return rel < 1e-5;
+}
+
+// An assignment whose target is also an operand reads the unmodified field;
+// plain field terms and scalars combine with operator terms.
+static bool dcpse_expression_aliasing() {
+    const size_t sz[2] = {64, 64};
+    Box<2, double> box({0, 0}, {1, 1});
+    size_t bc[2] = {PERIODIC, PERIODIC};
+    const double h[2] = {1.0 / sz[0], 1.0 / sz[1]};
+    const double rCut = 3.1 * h[0];
+    vector_dist_hip<2, double, 3> domain(0, box, bc, Ghost<2, double>(rCut));
+    auto it = domain.getGridIterator(sz);
+    while (it.isNext()) {
+        auto key = it.get();
+        domain.add();
+        domain.getLastPos()[0] = key.get(0) * h[0];
+        domain.getLastPos()[1] = key.get(1) * h[1];
+        domain.getLastProp<0>() = std::sin(2 * M_PI * key.get(0) * h[0]) + 0.25 * key.get(1) * h[1];
+        ++it;
+    }
+    auto vl = domain.getVerlet<VL_NON_SYMMETRIC | VL_SKIP_REF_PART>(rCut);
+    Derivative_y<decltype(vl)> Dy(domain, vl, 2, rCut);
+    auto f = getV<0>(domain);
+    auto a = getV<1>(domain);
+    auto b = getV<2>(domain);
+    a = Dy(f);                // 0.25 except where the periodic wrap of y breaks linearity
+    b = f - 3.0 * Dy(f);      // reference result
+    f = f - 3.0 * Dy(f);      // in place
+    double worst = 0;
+    for (size_t p = 0; p < domain.size_local(); ++p)
+        worst = std::fmax(worst, std::fabs(domain.getProp<0>(p) - domain.getProp<2>(p)));
+    std::printf("         in-place vs out-of-place max diff %.3e\n", worst);
+    return worst == 0.0;
+}
+
+// Row accessors (the value_nz contract of the reference operator expression,
+// DCPSE_op.hpp:69-83) reproduce the apply:
+// Df(p) = eps_p^-|b| * sum_j K_pj (f_q + sign f_p).
+static bool dcpse_row_accessors() {
+    const size_t sz[2] = {40, 40};
+    Box<2, double> box({0, 0}, {1, 1});
+    size_t bc[2] = {NON_PERIODIC, NON_PERIODIC};
+    const double h = 1.0 / (sz[0] - 1);
+    const double rCut = 3.1 * h;
+    vector_dist_hip<2, double, 3> domain(0, box, bc, Ghost<2, double>(rCut));
+    auto it = domain.getGridIterator(sz);
+    while (it.isNext()) {
+        auto key = it.get();
+        domain.add();
+        const double x = key.get(0) * h, y = key.get(1) * h;
+        domain.getLastPos()[0] = x;
+        domain.getLastPos()[1] = y;
+        domain.getLastProp<0>() = std::exp(x) * std::cos(3 * y);
+        ++it;
+    }
+    auto vl = domain.getVerlet<VL_NON_SYMMETRIC | VL_SKIP_REF_PART>(rCut);
+    Derivative_xy<decltype(vl)> Dxy(domain, vl, 2, rCut);
+    getV<1>(domain) = Dxy(getV<0>(domain));
+    auto& dc = Dxy.dcpse(domain);
+    double worst = 0, scale = 0;
+    auto it2 = domain.getDomainIterator();
+    while (it2.isNext()) {
+        auto p = it2.get();
+        double s = 0;
+        const double fp = domain.getProp<0>(p);
+        for (int j = 0; j < dc.getNumNN(p); ++j)
+            s += dc.getCoeffNN(p, j) * (domain.getProp<0>(dc.getIndexNN(p, j)) + dc.getSign() * fp);
+        s *= dc.getEpsilonInvPrefactor(p);
+        worst = std::fmax(worst, std::fabs(s - domain.getProp<1>(p)));
+        scale = std::fmax(scale, std::fabs(s));
+        ++it2;
+    }
+    std::printf("         accessor sum vs apply max diff %.3e (scale %.3e)\n", worst, scale);
+    return worst <= 1e-10 * scale;
+}
+
+int main() {
+    run("Dcpse_2D_test", dcpse_2d_test);
+    run("Dcpse_3D_test", dcpse_3d_test);
+    run("dcpse_op_tests", dcpse_op_tests);
+    run("dcpse_op_save_load", dcpse_op_save_load);
+    run("dcpse_op_test_lap", dcpse_op_test_lap);
+    run("dcpse_fp32_fields", dcpse_fp32_fields);
+    run("dcpse_expression_aliasing", dcpse_expression_aliasing);
+    run("dcpse_row_accessors", dcpse_row_accessors);
+    std::printf("%d failure(s)\n", failures);
+    return failures ? 1 : 0;
+}
