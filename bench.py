@@ -75,7 +75,9 @@ def main():
     ap.add_argument("--n", type=int, default=128, help="lattice edge (128 -> ~1.89M particles)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
-    ap.add_argument("--no-tune", action="store_true", help="skip launch-shape autotuning")
+    ap.add_argument("--tune", action="store_true",
+                    help="autotune launch shapes (CSR shapes, SELL unroll, assembly particles/wave) and use the "
+                         "fastest; the default is the fixed sliced-ELL kernel that the committed profiles cover")
     ap.add_argument("--order", choices=["morton", "lattice"], default="lattice",
                     help="particle storage order (morton: ParticleSet.reorder, as openfpm's reorder)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -140,7 +142,7 @@ def main():
         lap.dcpse.initializeStaticSize(ps, ps, check=False)
         dx.dcpse.initializeStaticSize(ps, ps, check=False)
 
-    if not args.no_tune:
+    if args.tune:
         (ppw, _), asm_tune = tune_assembly(assemble_both)
         log(f"[bench] assembly autotune particles/wave -> {ppw}: "
             + ", ".join(f"{k}:{v:.2f}ms" for k, v in asm_tune.items()))
@@ -159,16 +161,16 @@ def main():
     out_dx = torch.empty(N, dtype=torch.float32, device=dev)
     step = PreparedApply([lap.dcpse, dx.dcpse], f32_t, [out_lap, out_dx])
     tuned = None
-    sell_unroll = None
-    if not args.no_tune:
+    sell_pipe = None
+    if args.tune:
         # CSR kernels (several FP32 fields, FP64) and the sliced-ELL single-field kernel
         csr_step = PreparedApply([lap.dcpse, dx.dcpse], f32_t, [out_lap, out_dx], sell=False)
         tuned, app_tune = tune_apply(csr_step)
         log("[bench] CSR apply autotune (G,epl,vec,block) -> %s: %s" % (
             tuned[0], ", ".join(f"{k}:{v * 1e3:.1f}us" for k, v in app_tune.items())))
-        (sell_unroll, sell_ms), sell_tune = tune_sell(step)
-        log("[bench] SELL apply unroll -> %d: %s" % (
-            sell_unroll, ", ".join(f"{k}:{v * 1e3:.1f}us" for k, v in sell_tune.items())))
+        (sell_pipe, sell_ms), sell_tune = tune_sell(step)
+        log("[bench] SELL apply (groups, depth) -> %s: %s" % (
+            sell_pipe, ", ".join(f"{k}:{v * 1e3:.1f}us" for k, v in sell_tune.items())))
         if tuned[1] < sell_ms:
             step = csr_step
     for _ in range(args.warmup):
@@ -287,9 +289,10 @@ def main():
                    "particles": N, "nnz": nnz, "particle_order": args.order, "operators": ["Laplacian", "Derivative_x"],
                    "apply_precision": "fp32", "assembly_precision": "fp64",
                    "apply_layout": "sell64" if step.sell else "csr",
-                   "apply_launch": ({"sell_unroll": sell_unroll} if step.sell else
+                   "apply_launch": ({"sell_groups": (sell_pipe or (2, 2))[0], "sell_depth": (sell_pipe or (2, 2))[1]}
+                                    if step.sell else
                                     {"group_lanes": tuned[0][0], "elems_per_lane": tuned[0][1],
-                                     "vector_width": tuned[0][2], "block": tuned[0][3]}) if tuned else None},
+                                     "vector_width": tuned[0][2], "block": tuned[0][3]})},
         "assembly_ms": assembly_ms,
         "assembly_ms_by_operator": {"laplacian": asm_lap_ms, "dx": asm_dx_ms},
         "support_ms": support_ms,

@@ -195,32 +195,39 @@ dcpse_status dcpse_axpby_f64(dcpse_ctx* ctx, int64_t n, double alpha, const doub
 /* ---- sliced-ELL execution layout (FP32 apply) ------------------------- */
 /* The same operator laid out one wavefront per slice of 64 consecutive rows:
  * slice s holds rows [64s, 64s+64) with width w_s = longest row rounded up to
- * 4, and slot j of row 64s+l at slice_ptr[s] + (j/4)*256 + 4l + (j%4).
- * Padded slots hold column 0 and weight 0. sell_col/sell_W must be 16-byte
- * aligned. This is an execution copy: the CSR arrays stay the operator's
- * canonical storage (calcKernels/kerOffsets).
- * count: writes slice_ptr[n_slices+1] (device, n_slices = ceil(n_rows/64))
- * and *n_slots (host, synchronises). fill_cols/fill_weights_f32 write
- * sell_col[n_slots] / sell_W[n_slots] (device) from the CSR. */
+ * 4. Within a slice the rows are ordered by decreasing length (ties by
+ * index): lane l serves row sell_row[64s+l] (-1: no row), and slot j of lane
+ * l sits at slice_ptr[s] + (j/4)*256 + 4l + (j%4). Padded slots hold column
+ * 0 and weight 0. sell_col/sell_W must be 16-byte aligned. This is an
+ * execution copy: the CSR arrays stay the operator's canonical storage
+ * (calcKernels/kerOffsets).
+ * count: writes slice_ptr[n_slices+1] and sell_row[64*n_slices] (device,
+ * n_slices = ceil(n_rows/64)) and *n_slots (host, synchronises).
+ * fill_cols/fill_weights_f32 write sell_col[n_slots] / sell_W[n_slots]
+ * (device) from the CSR. */
 dcpse_status dcpse_sell_count(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, int64_t* slice_ptr,
-                              int64_t* n_slots);
+                              int32_t* sell_row, int64_t* n_slots);
 dcpse_status dcpse_sell_fill_cols(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int32_t* col,
-                                  const int64_t* slice_ptr, int32_t* sell_col);
+                                  const int64_t* slice_ptr, const int32_t* sell_row, int32_t* sell_col);
 dcpse_status dcpse_sell_fill_weights_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const float* W,
-                                         const int64_t* slice_ptr, float* sell_W);
+                                         const int64_t* slice_ptr, const int32_t* sell_row, float* sell_W);
 /* Same result as dcpse_apply_f32 with one field, from the SELL layout:
  * out[o][p] = alpha * sum_j sell_W[o][.] (f[col] - f_self[p]) + beta * out[o][p]. */
 dcpse_status dcpse_apply_sell_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int64_t* slice_ptr,
-                                  const int32_t* sell_col, int32_t n_ops, const float* const* sell_W,
-                                  const float* f, const float* f_self, float* const* out, float alpha, float beta);
+                                  const int32_t* sell_row, const int32_t* sell_col, int32_t n_ops,
+                                  const float* const* sell_W, const float* f, const float* f_self,
+                                  float* const* out, float alpha, float beta);
 /* Same result as dcpse_apply_transpose_f32 with one field, from the SELL
  * layout of the transposed CSR (built with the three calls above). */
 dcpse_status dcpse_apply_transpose_sell_f32(dcpse_ctx* ctx, int64_t n_cols, const int64_t* rowT_ptr,
-                                            const int64_t* slice_ptr, const int32_t* sell_colT,
-                                            const float* sell_WT, const float* diag, const float* g, float* out,
-                                            float alpha, float beta);
-/* Groups of 4 slots each lane loads before its gathers (2 or 4). */
-dcpse_status dcpse_set_sell_unroll(dcpse_ctx* ctx, int32_t groups);
+                                            const int64_t* slice_ptr, const int32_t* sell_rowT,
+                                            const int32_t* sell_colT, const float* sell_WT, const float* diag,
+                                            const float* g, float* out, float alpha, float beta);
+/* Software pipeline of the SELL apply: groups of 4 slots per batch and
+ * stages. depth 2: (groups 2 or 4) the stream loads of one batch overlap the
+ * gathers and FMAs of the other; depth 3: (groups 1 or 2) columns are loaded
+ * two batches ahead and weights one batch ahead. Default (2, 2). */
+dcpse_status dcpse_set_sell_pipeline(dcpse_ctx* ctx, int32_t groups, int32_t depth);
 
 #ifdef __cplusplus
 }

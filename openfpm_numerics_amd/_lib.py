@@ -56,12 +56,12 @@ SIGNATURES = {
                                          _f64, _f64]),
     "dcpse_axpby_f32": (_i32, [_vp, _i64, _f32, _vp, _f32, _vp]),
     "dcpse_axpby_f64": (_i32, [_vp, _i64, _f64, _vp, _f64, _vp]),
-    "dcpse_sell_count": (_i32, [_vp, _i64, _vp, _vp, _vp]),
-    "dcpse_sell_fill_cols": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp]),
-    "dcpse_sell_fill_weights_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp]),
-    "dcpse_apply_sell_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _f32, _f32]),
-    "dcpse_apply_transpose_sell_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f32, _f32]),
-    "dcpse_set_sell_unroll": (_i32, [_vp, _i32]),
+    "dcpse_sell_count": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp]),
+    "dcpse_sell_fill_cols": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp]),
+    "dcpse_sell_fill_weights_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp]),
+    "dcpse_apply_sell_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _f32, _f32]),
+    "dcpse_apply_transpose_sell_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f32, _f32]),
+    "dcpse_set_sell_pipeline": (_i32, [_vp, _i32, _i32]),
 }
 
 _LIB = None

@@ -51,20 +51,23 @@ def tune_apply(fn, shapes=APPLY_SHAPES, blocks=(256, 512), reps=10, device=0):
     return best, results
 
 
-def tune_sell(fn, unrolls=(2, 4), reps=10, device=0):
-    """fn() runs one sliced-ELL apply; returns (unroll, ms) of the fastest."""
+SELL_PIPELINES = ((2, 2), (4, 2), (1, 3), (2, 3))  # (groups per batch, stages)
+
+
+def tune_sell(fn, pipelines=SELL_PIPELINES, reps=10, device=0):
+    """fn() runs one sliced-ELL apply; returns ((groups, depth), ms) of the fastest."""
     ctx = _lib.context(device)
     best = None
     results = {}
     try:
-        for u in unrolls:
-            ctx.check(ctx.lib.dcpse_set_sell_unroll(ctx.h, u), "dcpse_set_sell_unroll")
+        for gd in pipelines:
+            ctx.check(ctx.lib.dcpse_set_sell_pipeline(ctx.h, *gd), "dcpse_set_sell_pipeline")
             ms = _time(fn, reps)
-            results[u] = ms
+            results[gd] = ms
             if best is None or ms < best[1]:
-                best = (u, ms)
+                best = (gd, ms)
     finally:
-        ctx.check(ctx.lib.dcpse_set_sell_unroll(ctx.h, best[0] if best else 4), "dcpse_set_sell_unroll")
+        ctx.check(ctx.lib.dcpse_set_sell_pipeline(ctx.h, *(best[0] if best else (2, 2))), "dcpse_set_sell_pipeline")
     return best, results
 
 

@@ -4,14 +4,20 @@
 // calcKernels, src/DCPSE/Dcpse.hpp:59-63). For the FP32 apply the same
 // nonzeros are also laid out one wavefront per slice of 64 consecutive rows:
 // slice s holds rows [64s, 64s+64), its width w_s is the longest row rounded up
-// to 4, and slot j of row 64s+l sits at
+// to 4. Inside a slice the rows are ordered by decreasing length (ties by
+// index): lane l of slice s serves row sell_row[64s+l] (-1 for the lanes of the
+// last slice that have no row), and slot j of that lane sits at
 //     slice_ptr[s] + (j/4)*256 + 4*l + (j%4).
 // A wavefront then reads each stream as 1 KB contiguous per load instruction
 // (lane l takes 4 consecutive slots of its own row with one 16-byte load), the
-// f gathers of neighbouring lanes hit the same cache lines (lanes are
-// consecutive particles, taking the same neighbour slot), and every lane owns
-// its row's sum outright (no cross-lane reduction). Padded slots hold column 0
-// and weight 0 and are masked out by the row length.
+// f gathers of neighbouring lanes hit the same cache lines (the lanes' rows are
+// the same 64 consecutive particles, taking the same neighbour slot), and every
+// lane owns its row's sum outright (no cross-lane reduction). Because lengths
+// decrease with the lane, the padding of a slot group sits in the highest
+// lanes: a 128-byte line whose 8 lanes are all past their rows is never
+// fetched (the kernel re-reads group 0 there), so the padding costs memory
+// footprint but hardly any HBM traffic. Padded slots hold column 0 and
+// weight 0 and are masked out by the row length.
 #include <algorithm>
 
 #include "dcpse_common.hpp"
@@ -24,39 +30,53 @@ typedef float vf4a __attribute__((ext_vector_type(4)));
 
 constexpr int kSellWaves = 4;  // slices (wavefronts) per block
 
-// counts[s] = 64 * (longest row of slice s, rounded up to 4)
+// counts[s] = 64 * (longest row of slice s, rounded up to 4); sell_row: the
+// rows of the slice by decreasing length (ties by index), -1 for lanes without
+// a row. One wavefront per slice; a lane's rank is found by comparing its key
+// with every other lane's.
 __global__ __launch_bounds__(256) void k_sell_width(int64_t n_rows, const int64_t* __restrict__ row_ptr,
-                                                   int32_t* __restrict__ counts, int32_t* __restrict__ overflow) {
+                                                   int32_t* __restrict__ counts, int32_t* __restrict__ sell_row,
+                                                   int32_t* __restrict__ overflow) {
     const int64_t s = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     const int64_t n_slices = (n_rows + 63) / 64;
     if (s >= n_slices) return;  // whole wavefront
     const int64_t row = s * 64 + lane;
-    int len = row < n_rows ? (int)(row_ptr[row + 1] - row_ptr[row]) : 0;
+    const int len = row < n_rows ? (int)(row_ptr[row + 1] - row_ptr[row]) : -1;
+    int rank = 0;
+    for (int m = 0; m < 64; ++m) {
+        const int lm = __shfl(len, m, kWave);
+        rank += (lm > len) || (lm == len && m < lane);
+    }
+    sell_row[s * 64 + rank] = row < n_rows ? (int32_t)row : -1;
+    int mx = len;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) len = max(len, __shfl_xor(len, off, kWave));
+    for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off, kWave));
+    mx = max(mx, 0);
     if (lane == 0) {
-        if (len > DCPSE_MAX_ROW) atomicOr(overflow, 1);
-        counts[s] = 64 * ((len + 3) & ~3);
+        if (mx > DCPSE_MAX_ROW) atomicOr(overflow, 1);
+        counts[s] = 64 * ((mx + 3) & ~3);
     }
 }
 
 // One thread per lane of every slice: writes every slot, padding included,
-// also for the lanes of the last slice that have no row (row >= n_rows), so no
-// slot of the layout is ever left uninitialised.
+// also for the lanes of the last slice that have no row, so no slot of the
+// layout is ever left uninitialised.
 template <bool COLS>
 __global__ void k_sell_fill(int64_t n_rows, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
                             const float* __restrict__ W, const int64_t* __restrict__ slice_ptr,
-                            int32_t* __restrict__ sell_col, float* __restrict__ sell_W) {
+                            const int32_t* __restrict__ sell_row, int32_t* __restrict__ sell_col,
+                            float* __restrict__ sell_W) {
     const int64_t n_lanes = ((n_rows + 63) / 64) * 64;
-    for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < n_lanes;
-         row += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t s = row >> 6;
-        const int l = (int)(row & 63);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_lanes;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = i >> 6;
+        const int l = (int)(i & 63);
         const int64_t base = slice_ptr[s];
         const int width = (int)((slice_ptr[s + 1] - base) / 64);
-        const int64_t rs = row < n_rows ? row_ptr[row] : 0;
-        const int len = row < n_rows ? (int)(row_ptr[row + 1] - rs) : 0;
+        const int32_t row = sell_row[i];
+        const int64_t rs = row >= 0 ? row_ptr[row] : 0;
+        const int len = row >= 0 ? (int)(row_ptr[row + 1] - rs) : 0;
         for (int j = 0; j < width; ++j) {
             const int64_t dst = base + (int64_t)(j >> 2) * 256 + 4 * l + (j & 3);
             if (COLS) sell_col[dst] = j < len ? col[rs + j] : 0;
@@ -69,6 +89,7 @@ struct SellArgs {
     int64_t n_rows;
     const int64_t* row_ptr;
     const int64_t* slice_ptr;
+    const int32_t* sell_row;
     const int32_t* sell_col;
     const float* W[4];
     const float* f;
@@ -87,48 +108,62 @@ struct SellBatch {
 
 // Groups [g0, g0+UNR) of the lane's row: the 16-byte col/W loads. Groups past
 // the row (or the slice) re-read group 0, so every load is issued
-// unconditionally and the wait counts stay exact; sell_use masks them out.
-// Every slot a lane can read is written by k_sell_fill (padding: column 0).
+// unconditionally and the wait counts stay exact; sell_fma masks them out.
+// Every slot a lane can read is written by k_sell_fill: padding and the lanes
+// without a row hold column 0, so every gather stays inside f. The loaded
+// columns are used as they are (no select on them), so nothing waits for a
+// batch before its gathers are due.
 template <int NOPS, int UNR>
 __device__ __forceinline__ void sell_load(const SellArgs& a, const int32_t* cp, int64_t woff, int g0, int len,
-                                          bool live, SellBatch<NOPS, UNR>& b) {
+                                          SellBatch<NOPS, UNR>& b) {
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
         const int g = g0 + u;
         const int64_t off = 4 * g < len ? (int64_t)g * 256 : 0;
-        const vi4a q = *(const vi4a*)(cp + off);
-        b.q[u] = live ? q : vi4a{0, 0, 0, 0};  // lanes without a row gather f[0]
+        b.q[u] = *(const vi4a*)(cp + off);
 #pragma unroll
         for (int o = 0; o < NOPS; ++o) b.w[o][u] = *(const vf4a*)(a.W[o] + woff + off);
     }
 }
 
-// The gathers and FMAs of groups [g0, g0+UNR); slots past the row are masked.
-template <int NOPS, int UNR, int MODE>
-__device__ __forceinline__ void sell_use(const SellArgs& a, int g0, int len, float fp, const SellBatch<NOPS, UNR>& b,
-                                         float* acc) {
+// The f gathers of groups [g0, g0+UNR) (columns of a loaded batch).
+template <int UNR>
+struct SellGathered {
     float fq[UNR][4];
+};
+
+template <int NOPS, int UNR>
+__device__ __forceinline__ void sell_gather(const SellArgs& a, const SellBatch<NOPS, UNR>& b, SellGathered<UNR>& g) {
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
-        fq[u][0] = a.f[b.q[u].x];
-        fq[u][1] = a.f[b.q[u].y];
-        fq[u][2] = a.f[b.q[u].z];
-        fq[u][3] = a.f[b.q[u].w];
+        g.fq[u][0] = a.f[b.q[u].x];
+        g.fq[u][1] = a.f[b.q[u].y];
+        g.fq[u][2] = a.f[b.q[u].z];
+        g.fq[u][3] = a.f[b.q[u].w];
     }
+}
+
+// The FMAs of groups [g0, g0+UNR); slots past the row are masked.
+template <int NOPS, int UNR, int MODE>
+__device__ __forceinline__ void sell_fma(int g0, int len, float fp, const SellBatch<NOPS, UNR>& b,
+                                         const SellGathered<UNR>& g, float* acc) {
 #pragma unroll
     for (int u = 0; u < UNR; ++u)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const bool v = 4 * (g0 + u) + i < len;
-            const float d = MODE == 0 ? fq[u][i] - fp : fq[u][i];
+            const float d = MODE == 0 ? g.fq[u][i] - fp : g.fq[u][i];
 #pragma unroll
             for (int o = 0; o < NOPS; ++o) acc[o] = v ? fmaf(b.w[o][u][i], d, acc[o]) : acc[o];
         }
 }
 
-// One wavefront per slice. Two register batches alternate: the loads of groups
-// [g0+UNR, g0+2UNR) are in flight while groups [g0, g0+UNR) gather and
-// accumulate, so stream and gather latencies overlap within the wavefront.
+// One wavefront per slice. Two register batches alternate. Issue order per
+// half step: the gathers of the loaded batch, then the stream loads of the
+// other batch, then the FMAs. Vector-memory loads complete in issue order, so
+// waiting for the gathers does not wait for the stream loads issued after
+// them: the next batch's col/W loads stay in flight across the FMAs, and its
+// gathers go out as soon as its columns arrive.
 template <int NOPS, int UNR, int MODE>
 __global__ __launch_bounds__(kSellWaves * kWave) void k_apply_sell(const SellArgs a) {
     const int64_t hb = blockIdx.x;
@@ -136,9 +171,9 @@ __global__ __launch_bounds__(kSellWaves * kWave) void k_apply_sell(const SellArg
     if (lb >= a.blocks_needed) return;
     const int lane = threadIdx.x & 63;
     const int64_t s = lb * kSellWaves + (threadIdx.x >> 6);
-    const int64_t row = s * 64 + lane;
     if (s * 64 >= a.n_rows) return;  // whole wavefront
-    const bool live = row < a.n_rows;
+    const int32_t row = a.sell_row[s * 64 + lane];
+    const bool live = row >= 0;
     const int len = live ? (int)(a.row_ptr[row + 1] - a.row_ptr[row]) : 0;
     const float fp = live ? a.f_self[row] : 0.0f;
     const int64_t base = a.slice_ptr[s];
@@ -150,14 +185,21 @@ __global__ __launch_bounds__(kSellWaves * kWave) void k_apply_sell(const SellArg
     for (int o = 0; o < NOPS; ++o) acc[o] = 0.0f;
     if (ng > 0) {  // a slice of empty rows has no slots to read
         SellBatch<NOPS, UNR> bA, bB;
-        sell_load<NOPS, UNR>(a, cp, woff, 0, len, live, bA);
+        SellGathered<UNR> gA, gB;
+        sell_load<NOPS, UNR>(a, cp, woff, 0, len, bA);
         for (int g0 = 0; g0 < ng; g0 += 2 * UNR) {
-            sell_load<NOPS, UNR>(a, cp, woff, g0 + UNR, len, live, bB);
+            sell_gather<NOPS, UNR>(a, bA, gA);
             __builtin_amdgcn_sched_barrier(0);
-            sell_use<NOPS, UNR, MODE>(a, g0, len, fp, bA, acc);
-            sell_load<NOPS, UNR>(a, cp, woff, g0 + 2 * UNR, len, live, bA);
+            sell_load<NOPS, UNR>(a, cp, woff, g0 + UNR, len, bB);
             __builtin_amdgcn_sched_barrier(0);
-            sell_use<NOPS, UNR, MODE>(a, g0 + UNR, len, fp, bB, acc);
+            sell_fma<NOPS, UNR, MODE>(g0, len, fp, bA, gA, acc);
+            __builtin_amdgcn_sched_barrier(0);
+            sell_gather<NOPS, UNR>(a, bB, gB);
+            __builtin_amdgcn_sched_barrier(0);
+            sell_load<NOPS, UNR>(a, cp, woff, g0 + 2 * UNR, len, bA);
+            __builtin_amdgcn_sched_barrier(0);
+            sell_fma<NOPS, UNR, MODE>(g0 + UNR, len, fp, bB, gB, acc);
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
     if (live) {
@@ -172,21 +214,148 @@ __global__ __launch_bounds__(kSellWaves * kWave) void k_apply_sell(const SellArg
     }
 }
 
+// Three-stage variant: columns are loaded two batches ahead, weights one batch
+// ahead, and the gathers of the next batch go out before the FMAs of the
+// current one. Per step a wavefront waits only for memory operations issued a
+// full step earlier (loads complete in issue order), at the cost of holding
+// two column, two weight and two gather batches in registers.
+template <int NOPS, int UNR>
+__device__ __forceinline__ void sell_load_cols(const int32_t* cp, int g0, int len, vi4a (&q)[UNR]) {
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+        const int g = g0 + u;
+        q[u] = *(const vi4a*)(cp + (4 * g < len ? (int64_t)g * 256 : 0));
+    }
+}
+
+template <int NOPS, int UNR>
+__device__ __forceinline__ void sell_load_w(const SellArgs& a, int64_t woff, int g0, int len, vf4a (&w)[NOPS][UNR]) {
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+        const int g = g0 + u;
+        const int64_t off = 4 * g < len ? (int64_t)g * 256 : 0;
+#pragma unroll
+        for (int o = 0; o < NOPS; ++o) w[o][u] = *(const vf4a*)(a.W[o] + woff + off);
+    }
+}
+
+template <int UNR>
+__device__ __forceinline__ void sell_gather_q(const SellArgs& a, const vi4a (&q)[UNR], SellGathered<UNR>& g) {
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+        g.fq[u][0] = a.f[q[u].x];
+        g.fq[u][1] = a.f[q[u].y];
+        g.fq[u][2] = a.f[q[u].z];
+        g.fq[u][3] = a.f[q[u].w];
+    }
+}
+
 template <int NOPS, int UNR, int MODE>
+__device__ __forceinline__ void sell_fma_w(int g0, int len, float fp, const vf4a (&w)[NOPS][UNR],
+                                           const SellGathered<UNR>& g, float* acc) {
+#pragma unroll
+    for (int u = 0; u < UNR; ++u)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const bool v = 4 * (g0 + u) + i < len;
+            const float d = MODE == 0 ? g.fq[u][i] - fp : g.fq[u][i];
+#pragma unroll
+            for (int o = 0; o < NOPS; ++o) acc[o] = v ? fmaf(w[o][u][i], d, acc[o]) : acc[o];
+        }
+}
+
+template <int NOPS, int UNR, int MODE>
+__global__ __launch_bounds__(kSellWaves * kWave) void k_apply_sell3(const SellArgs a) {
+    const int64_t hb = blockIdx.x;
+    const int64_t lb = (hb & 7) * a.per_xcd + (hb >> 3);
+    if (lb >= a.blocks_needed) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t s = lb * kSellWaves + (threadIdx.x >> 6);
+    if (s * 64 >= a.n_rows) return;  // whole wavefront
+    const int32_t row = a.sell_row[s * 64 + lane];
+    const bool live = row >= 0;
+    const int len = live ? (int)(a.row_ptr[row + 1] - a.row_ptr[row]) : 0;
+    const float fp = live ? a.f_self[row] : 0.0f;
+    const int64_t base = a.slice_ptr[s];
+    const int ng = (int)((a.slice_ptr[s + 1] - base) >> 8);  // groups of 4 slots
+    const int32_t* cp = a.sell_col + base + 4 * lane;
+    const int64_t woff = base + 4 * lane;
+    float acc[NOPS];
+#pragma unroll
+    for (int o = 0; o < NOPS; ++o) acc[o] = 0.0f;
+    if (ng > 0) {
+        vi4a cX[UNR], cY[UNR];
+        vf4a wX[NOPS][UNR], wY[NOPS][UNR];
+        SellGathered<UNR> gX, gY;
+        sell_load_cols<NOPS, UNR>(cp, 0, len, cX);
+        sell_load_cols<NOPS, UNR>(cp, UNR, len, cY);
+        sell_load_w<NOPS, UNR>(a, woff, 0, len, wX);
+        __builtin_amdgcn_sched_barrier(0);
+        sell_gather_q<UNR>(a, cX, gX);
+        __builtin_amdgcn_sched_barrier(0);
+        for (int g0 = 0; g0 < ng; g0 += 2 * UNR) {
+            // batch g0: W in wX, gathers in gX; batch g0+UNR: columns in cY
+            sell_load_cols<NOPS, UNR>(cp, g0 + 2 * UNR, len, cX);
+            sell_load_w<NOPS, UNR>(a, woff, g0 + UNR, len, wY);
+            __builtin_amdgcn_sched_barrier(0);
+            sell_gather_q<UNR>(a, cY, gY);
+            __builtin_amdgcn_sched_barrier(0);
+            sell_fma_w<NOPS, UNR, MODE>(g0, len, fp, wX, gX, acc);
+            __builtin_amdgcn_sched_barrier(0);
+            // batch g0+UNR: W in wY, gathers in gY; batch g0+2UNR: columns in cX
+            sell_load_cols<NOPS, UNR>(cp, g0 + 3 * UNR, len, cY);
+            sell_load_w<NOPS, UNR>(a, woff, g0 + 2 * UNR, len, wX);
+            __builtin_amdgcn_sched_barrier(0);
+            sell_gather_q<UNR>(a, cX, gX);
+            __builtin_amdgcn_sched_barrier(0);
+            sell_fma_w<NOPS, UNR, MODE>(g0 + UNR, len, fp, wY, gY, acc);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    if (live) {
+#pragma unroll
+        for (int o = 0; o < NOPS; ++o) {
+            float val = acc[o];
+            if (MODE == 1) val = fmaf(a.diag[row], fp, val);
+            val *= a.alpha;
+            if (a.beta != 0.0f) val = fmaf(a.beta, a.out[o][row], val);
+            a.out[o][row] = val;
+        }
+    }
+}
+
+template <int NOPS, int UNR, int DEPTH, int MODE>
 static void launch_sell(SellArgs a, hipStream_t st) {
     const int64_t n_slices = (a.n_rows + 63) / 64;
     a.blocks_needed = (n_slices + kSellWaves - 1) / kSellWaves;
     a.per_xcd = (a.blocks_needed + 7) / 8;
-    k_apply_sell<NOPS, UNR, MODE><<<(unsigned)(a.per_xcd * 8), kSellWaves * kWave, 0, st>>>(a);
+    if (DEPTH == 3) k_apply_sell3<NOPS, UNR, MODE><<<(unsigned)(a.per_xcd * 8), kSellWaves * kWave, 0, st>>>(a);
+    else k_apply_sell<NOPS, UNR, MODE><<<(unsigned)(a.per_xcd * 8), kSellWaves * kWave, 0, st>>>(a);
 }
 
-template <int UNR, int MODE>
+template <int UNR, int DEPTH, int MODE>
 static bool dispatch_sell_ops(const SellArgs& a, int nops, hipStream_t st) {
     switch (nops) {
-        case 1: launch_sell<1, UNR, MODE>(a, st); return true;
-        case 2: launch_sell<2, UNR, MODE>(a, st); return true;
-        case 3: launch_sell<3, UNR, MODE>(a, st); return true;
-        case 4: launch_sell<4, UNR, MODE>(a, st); return true;
+        case 1: launch_sell<1, UNR, DEPTH, MODE>(a, st); return true;
+        case 2: launch_sell<2, UNR, DEPTH, MODE>(a, st); return true;
+        case 3: launch_sell<3, UNR, DEPTH, MODE>(a, st); return true;
+        case 4: launch_sell<4, UNR, DEPTH, MODE>(a, st); return true;
+    }
+    return false;
+}
+
+template <int MODE>
+static bool dispatch_sell(const SellArgs& a, int nops, int unr, int depth, hipStream_t st) {
+    if (depth == 3) {
+        switch (unr) {
+            case 1: return dispatch_sell_ops<1, 3, MODE>(a, nops, st);
+            case 2: return dispatch_sell_ops<2, 3, MODE>(a, nops, st);
+        }
+    } else {
+        switch (unr) {
+            case 2: return dispatch_sell_ops<2, 2, MODE>(a, nops, st);
+            case 4: return dispatch_sell_ops<4, 2, MODE>(a, nops, st);
+        }
     }
     return false;
 }
@@ -198,9 +367,10 @@ using namespace dcpse;
 extern "C" {
 
 dcpse_status dcpse_sell_count(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, int64_t* slice_ptr,
-                              int64_t* n_slots) {
+                              int32_t* sell_row, int64_t* n_slots) {
     if (!ctx) return DCPSE_ERR_INVALID;
-    if (!row_ptr || !slice_ptr || n_rows < 0) return fail(ctx, DCPSE_ERR_INVALID, "null pointer or negative size");
+    if (!row_ptr || !slice_ptr || !sell_row || n_rows < 0)
+        return fail(ctx, DCPSE_ERR_INVALID, "null pointer or negative size");
     DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     const int64_t n_slices = (n_rows + 63) / 64;
     Workspace w;
@@ -210,7 +380,8 @@ dcpse_status dcpse_sell_count(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row
     int32_t* flags = w.take<int32_t>(16);
     fill_i32(ctx->stream, flags, 16, 0);
     if (n_slices > 0) {
-        k_sell_width<<<(unsigned)((n_slices * 64 + 255) / 256), 256, 0, ctx->stream>>>(n_rows, row_ptr, counts, flags);
+        k_sell_width<<<(unsigned)((n_slices * 64 + 255) / 256), 256, 0, ctx->stream>>>(n_rows, row_ptr, counts,
+                                                                                        sell_row, flags);
         DCPSE_LAUNCH_CHECK(ctx);
     }
     if ((st = scan_counts(ctx, counts, n_slices, slice_ptr, &w))) return st;
@@ -225,39 +396,39 @@ dcpse_status dcpse_sell_count(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row
 }
 
 dcpse_status dcpse_sell_fill_cols(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int32_t* col,
-                                  const int64_t* slice_ptr, int32_t* sell_col) {
+                                  const int64_t* slice_ptr, const int32_t* sell_row, int32_t* sell_col) {
     if (!ctx) return DCPSE_ERR_INVALID;
-    if (!row_ptr || !col || !slice_ptr || !sell_col || n_rows < 0)
+    if (!row_ptr || !col || !slice_ptr || !sell_row || !sell_col || n_rows < 0)
         return fail(ctx, DCPSE_ERR_INVALID, "null pointer or negative size");
     DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     if (n_rows == 0) return DCPSE_OK;
     const int64_t b = std::min<int64_t>((n_rows + 63 + 255) / 256, 65536);
-    k_sell_fill<true><<<(unsigned)b, 256, 0, ctx->stream>>>(n_rows, row_ptr, col, nullptr, slice_ptr, sell_col,
-                                                            nullptr);
+    k_sell_fill<true><<<(unsigned)b, 256, 0, ctx->stream>>>(n_rows, row_ptr, col, nullptr, slice_ptr, sell_row,
+                                                            sell_col, nullptr);
     DCPSE_LAUNCH_CHECK(ctx);
     return DCPSE_OK;
 }
 
 dcpse_status dcpse_sell_fill_weights_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const float* W,
-                                         const int64_t* slice_ptr, float* sell_W) {
+                                         const int64_t* slice_ptr, const int32_t* sell_row, float* sell_W) {
     if (!ctx) return DCPSE_ERR_INVALID;
-    if (!row_ptr || !W || !slice_ptr || !sell_W || n_rows < 0)
+    if (!row_ptr || !W || !slice_ptr || !sell_row || !sell_W || n_rows < 0)
         return fail(ctx, DCPSE_ERR_INVALID, "null pointer or negative size");
     DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     if (n_rows == 0) return DCPSE_OK;
     const int64_t b = std::min<int64_t>((n_rows + 63 + 255) / 256, 65536);
-    k_sell_fill<false><<<(unsigned)b, 256, 0, ctx->stream>>>(n_rows, row_ptr, nullptr, W, slice_ptr, nullptr,
-                                                             sell_W);
+    k_sell_fill<false><<<(unsigned)b, 256, 0, ctx->stream>>>(n_rows, row_ptr, nullptr, W, slice_ptr, sell_row,
+                                                             nullptr, sell_W);
     DCPSE_LAUNCH_CHECK(ctx);
     return DCPSE_OK;
 }
 
 static dcpse_status sell_apply(dcpse_ctx* ctx, int mode, int64_t n_rows, const int64_t* row_ptr,
-                               const int64_t* slice_ptr, const int32_t* sell_col, int32_t n_ops,
-                               const float* const* sell_W, const float* f, const float* f_self, const float* diag,
-                               float* const* out, float alpha, float beta) {
+                               const int64_t* slice_ptr, const int32_t* sell_row, const int32_t* sell_col,
+                               int32_t n_ops, const float* const* sell_W, const float* f, const float* f_self,
+                               const float* diag, float* const* out, float alpha, float beta) {
     if (!ctx) return DCPSE_ERR_INVALID;
-    if (!row_ptr || !slice_ptr || !sell_col || !sell_W || !f || !out || n_rows < 0 || n_ops < 1)
+    if (!row_ptr || !slice_ptr || !sell_row || !sell_col || !sell_W || !f || !out || n_rows < 0 || n_ops < 1)
         return fail(ctx, DCPSE_ERR_INVALID, "null pointer or bad size");
     if (n_ops > 4) return fail(ctx, DCPSE_ERR_UNSUPPORTED, "at most 4 operators per pass");
     for (int o = 0; o < n_ops; ++o)
@@ -268,6 +439,7 @@ static dcpse_status sell_apply(dcpse_ctx* ctx, int mode, int64_t n_rows, const i
     a.n_rows = n_rows;
     a.row_ptr = row_ptr;
     a.slice_ptr = slice_ptr;
+    a.sell_row = sell_row;
     a.sell_col = sell_col;
     for (int o = 0; o < 4; ++o) {
         a.W[o] = o < n_ops ? sell_W[o] : nullptr;
@@ -278,35 +450,39 @@ static dcpse_status sell_apply(dcpse_ctx* ctx, int mode, int64_t n_rows, const i
     a.diag = diag;
     a.alpha = alpha;
     a.beta = beta;
-    const int unr = ctx->sell_unroll;
-    bool ok;
-    if (mode == 0) ok = unr == 4 ? dispatch_sell_ops<4, 0>(a, n_ops, ctx->stream) : dispatch_sell_ops<2, 0>(a, n_ops, ctx->stream);
-    else ok = unr == 4 ? dispatch_sell_ops<4, 1>(a, n_ops, ctx->stream) : dispatch_sell_ops<2, 1>(a, n_ops, ctx->stream);
+    const bool ok = mode == 0 ? dispatch_sell<0>(a, n_ops, ctx->sell_groups, ctx->sell_depth, ctx->stream)
+                              : dispatch_sell<1>(a, n_ops, ctx->sell_groups, ctx->sell_depth, ctx->stream);
     if (!ok) return fail(ctx, DCPSE_ERR_UNSUPPORTED, "no SELL kernel for %d operators", n_ops);
     DCPSE_LAUNCH_CHECK(ctx);
     return DCPSE_OK;
 }
 
 dcpse_status dcpse_apply_sell_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int64_t* slice_ptr,
-                                  const int32_t* sell_col, int32_t n_ops, const float* const* sell_W, const float* f,
-                                  const float* f_self, float* const* out, float alpha, float beta) {
-    return sell_apply(ctx, 0, n_rows, row_ptr, slice_ptr, sell_col, n_ops, sell_W, f, f_self, nullptr, out, alpha,
-                      beta);
+                                  const int32_t* sell_row, const int32_t* sell_col, int32_t n_ops,
+                                  const float* const* sell_W, const float* f, const float* f_self, float* const* out,
+                                  float alpha, float beta) {
+    return sell_apply(ctx, 0, n_rows, row_ptr, slice_ptr, sell_row, sell_col, n_ops, sell_W, f, f_self, nullptr, out,
+                      alpha, beta);
 }
 
 dcpse_status dcpse_apply_transpose_sell_f32(dcpse_ctx* ctx, int64_t n_cols, const int64_t* rowT_ptr,
-                                            const int64_t* slice_ptr, const int32_t* sell_colT, const float* sell_WT,
-                                            const float* diag, const float* g, float* out, float alpha, float beta) {
+                                            const int64_t* slice_ptr, const int32_t* sell_rowT,
+                                            const int32_t* sell_colT, const float* sell_WT, const float* diag,
+                                            const float* g, float* out, float alpha, float beta) {
+    if (!ctx) return DCPSE_ERR_INVALID;
     if (!diag) return fail(ctx, DCPSE_ERR_INVALID, "diag is null");
     const float* Wp[1] = {sell_WT};
     float* Op[1] = {out};
-    return sell_apply(ctx, 1, n_cols, rowT_ptr, slice_ptr, sell_colT, 1, Wp, g, nullptr, diag, Op, alpha, beta);
+    return sell_apply(ctx, 1, n_cols, rowT_ptr, slice_ptr, sell_rowT, sell_colT, 1, Wp, g, nullptr, diag, Op, alpha,
+                      beta);
 }
 
-dcpse_status dcpse_set_sell_unroll(dcpse_ctx* ctx, int32_t groups) {
+dcpse_status dcpse_set_sell_pipeline(dcpse_ctx* ctx, int32_t groups, int32_t depth) {
     if (!ctx) return DCPSE_ERR_INVALID;
-    if (groups != 2 && groups != 4) return fail(ctx, DCPSE_ERR_INVALID, "groups must be 2 or 4");
-    ctx->sell_unroll = groups;
+    const bool ok = (depth == 2 && (groups == 2 || groups == 4)) || (depth == 3 && (groups == 1 || groups == 2));
+    if (!ok) return fail(ctx, DCPSE_ERR_INVALID, "(groups, depth) must be (2,2), (4,2), (1,3) or (2,3)");
+    ctx->sell_groups = groups;
+    ctx->sell_depth = depth;
     return DCPSE_OK;
 }
 

@@ -173,9 +173,8 @@ class Dcpse:
     def sell_weights(self) -> torch.Tensor:
         """FP32 weights in the sliced-ELL layout of this operator's Verlet list."""
         if self._W.get("sell") is None:
-            slice_ptr, sell_col = self.verletList.sell()
-            self._W["sell"] = sell_weights(self.verletList.row_ptr, self._W[torch.float32], slice_ptr,
-                                           self.verletList.n_rows, sell_col.numel())
+            self._W["sell"] = sell_weights(self.verletList.row_ptr, self._W[torch.float32], self.verletList.sell(),
+                                           self.verletList.n_rows)
         return self._W["sell"]
 
     def computeDifferentialOperator(self, f: torch.Tensor, out: torch.Tensor | None = None,
@@ -232,12 +231,12 @@ class Dcpse:
         return self._T[dtype]
 
     def transposed_sell(self):
-        """SELL layout of the FP32 transposed operator: (slice_ptr, sell_colT, sell_WT)."""
+        """SELL layout of the FP32 transposed operator: (slice_ptr, sell_rowT, sell_colT, sell_WT)."""
         if self._T.get("sell") is None:
             rowT, colT, WT, _ = self.transposed(torch.float32)
             n = self.verletList.n_cols
-            slice_ptr, sell_col = sell_layout(rowT, colT, n)
-            self._T["sell"] = (slice_ptr, sell_col, sell_weights(rowT, WT, slice_ptr, n, sell_col.numel()))
+            layout = sell_layout(rowT, colT, n)
+            self._T["sell"] = (*layout, sell_weights(rowT, WT, layout, n))
         return self._T["sell"]
 
     def applyTranspose(self, g: torch.Tensor, out: torch.Tensor | None = None, alpha=1.0, beta=0.0):
@@ -252,9 +251,9 @@ class Dcpse:
         o = torch.empty_like(g2) if out is None else out.reshape(nb, n)
         ctx = _lib.context(g.device.index or 0)
         if g.dtype == torch.float32 and nb == 1 and USE_SELL:
-            slice_ptr, sell_colT, sell_WT = self.transposed_sell()
-            ctx.call("dcpse_apply_transpose_sell_f32", n, _lib.ptr(rowT), _lib.ptr(slice_ptr), _lib.ptr(sell_colT),
-                     _lib.ptr(sell_WT), _lib.ptr(diag), _lib.ptr(g2), _lib.ptr(o), alpha, beta)
+            slice_ptr, sell_rowT, sell_colT, sell_WT = self.transposed_sell()
+            ctx.call("dcpse_apply_transpose_sell_f32", n, _lib.ptr(rowT), _lib.ptr(slice_ptr), _lib.ptr(sell_rowT),
+                     _lib.ptr(sell_colT), _lib.ptr(sell_WT), _lib.ptr(diag), _lib.ptr(g2), _lib.ptr(o), alpha, beta)
             return o.reshape(g.shape)
         name = "dcpse_apply_transpose_f32" if g.dtype == torch.float32 else "dcpse_apply_transpose_f64"
         ctx.call(name, n, _lib.ptr(rowT), _lib.ptr(colT), _lib.ptr(WT), _lib.ptr(diag), _lib.ptr(g2), n, nb,
@@ -396,10 +395,10 @@ def apply_ops(ops, f: torch.Tensor, outs=None, f_domain: torch.Tensor | None = N
         outs = [o.reshape(nb, n_rows) for o in outs]
     ctx = _lib.context(f.device.index or 0)
     if f.dtype == torch.float32 and nb == 1 and USE_SELL:
-        slice_ptr, sell_col = vl.sell()
+        slice_ptr, sell_row, sell_col = vl.sell()
         SW = [op.sell_weights() for op in ops]
-        ctx.call("dcpse_apply_sell_f32", n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(slice_ptr), _lib.ptr(sell_col),
-                 len(ops), _lib.ptr_array(SW), _lib.ptr(f2), _lib.ptr(fd), _lib.ptr_array(outs), alpha, beta)
+        ctx.call("dcpse_apply_sell_f32", n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(slice_ptr), _lib.ptr(sell_row),
+                 _lib.ptr(sell_col), len(ops), _lib.ptr_array(SW), _lib.ptr(f2), _lib.ptr(fd), _lib.ptr_array(outs), alpha, beta)
         return [o.reshape(*f.shape[:-1], n_rows) for o in outs]
     W = [op.weights(f.dtype) for op in ops]
     name = "dcpse_apply_f32" if f.dtype == torch.float32 else "dcpse_apply_f64"
@@ -430,12 +429,12 @@ class PreparedApply:
         self.ctx = _lib.context(f.device.index or 0)
         self.O = _lib.ptr_array(outs)
         if sell and f.dtype == torch.float32 and nb == 1:
-            slice_ptr, sell_col = vl.sell()
-            self._keep = (ops, f, outs, [op.sell_weights() for op in ops], slice_ptr, sell_col)
+            slice_ptr, sell_row, sell_col = vl.sell()
+            self._keep = (ops, f, outs, [op.sell_weights() for op in ops], slice_ptr, sell_row, sell_col)
             self.W = _lib.ptr_array(self._keep[3])
             self.fn = self.ctx.lib.dcpse_apply_sell_f32
-            self.args = (self.ctx.h, vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(slice_ptr), _lib.ptr(sell_col),
-                         len(ops), self.W, _lib.ptr(f2), None, self.O, alpha, beta)
+            self.args = (self.ctx.h, vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(slice_ptr), _lib.ptr(sell_row),
+                         _lib.ptr(sell_col), len(ops), self.W, _lib.ptr(f2), None, self.O, alpha, beta)
         else:
             self._keep = (ops, f, outs, [op.weights(f.dtype) for op in ops])
             self.fn = self.ctx.lib.dcpse_apply_f32 if f.dtype == torch.float32 else self.ctx.lib.dcpse_apply_f64

@@ -171,31 +171,38 @@ class VerletList:
         return self.row_ptr[1:] - self.row_ptr[:-1]
 
     def sell(self):
-        """(slice_ptr, sell_col): the sliced-ELL execution layout of this support,
-        shared by every FP32 operator assembled on it; built on first use."""
+        """(slice_ptr, sell_row, sell_col): the sliced-ELL execution layout of
+        this support, shared by every FP32 operator assembled on it; built on
+        first use."""
         if getattr(self, "_sell", None) is None:
             self._sell = sell_layout(self.row_ptr, self.col, self.n_rows)
         return self._sell
 
 
 def sell_layout(row_ptr: torch.Tensor, col: torch.Tensor, n_rows: int):
+    """Sliced-ELL layout of a CSR: slices of 64 rows, rows ordered by
+    decreasing length inside a slice (sell_row maps lane -> row, -1 = none)."""
     ctx = _lib.context(row_ptr.device.index or 0)
     dev = row_ptr.device
     n_slices = (n_rows + 63) // 64
     slice_ptr = torch.empty(n_slices + 1, dtype=torch.int64, device=dev)
+    sell_row = torch.empty(max(64 * n_slices, 64), dtype=torch.int32, device=dev)
     n_slots = np.zeros(1, np.int64)
-    ctx.call("dcpse_sell_count", n_rows, _lib.ptr(row_ptr), _lib.ptr(slice_ptr), n_slots.ctypes.data)
+    ctx.call("dcpse_sell_count", n_rows, _lib.ptr(row_ptr), _lib.ptr(slice_ptr), _lib.ptr(sell_row),
+             n_slots.ctypes.data)
     sell_col = torch.empty(max(int(n_slots[0]), 4), dtype=torch.int32, device=dev)
     if n_rows:
         ctx.call("dcpse_sell_fill_cols", n_rows, _lib.ptr(row_ptr), _lib.ptr(col), _lib.ptr(slice_ptr),
-                 _lib.ptr(sell_col))
-    return slice_ptr, sell_col
+                 _lib.ptr(sell_row), _lib.ptr(sell_col))
+    return slice_ptr, sell_row, sell_col
 
 
-def sell_weights(row_ptr: torch.Tensor, W: torch.Tensor, slice_ptr: torch.Tensor, n_rows: int, n_slots: int):
+def sell_weights(row_ptr: torch.Tensor, W: torch.Tensor, layout, n_rows: int):
+    """FP32 weights W (CSR order) in the sliced-ELL layout (slice_ptr, sell_row, sell_col)."""
+    slice_ptr, sell_row, sell_col = layout
     ctx = _lib.context(row_ptr.device.index or 0)
-    out = torch.empty(max(n_slots, 4), dtype=torch.float32, device=row_ptr.device)
+    out = torch.empty(sell_col.numel(), dtype=torch.float32, device=row_ptr.device)
     if n_rows:
         ctx.call("dcpse_sell_fill_weights_f32", n_rows, _lib.ptr(row_ptr), _lib.ptr(W), _lib.ptr(slice_ptr),
-                 _lib.ptr(out))
+                 _lib.ptr(sell_row), _lib.ptr(out))
     return out

@@ -235,9 +235,9 @@ def test_sell_layout_apply(name, oracle, cuda):
     ctx = _lib.context(0)
     ft = torch.as_tensor(f32, device=cuda)
     try:
-        for unr in (2, 4):
-            print(f"[sell] {name} unroll={unr}", flush=True)
-            ctx.check(ctx.lib.dcpse_set_sell_unroll(ctx.h, unr), "unroll")
+        for gd in ((2, 2), (4, 2), (1, 3), (2, 3)):
+            print(f"[sell] {name} groups,depth={gd}", flush=True)
+            ctx.check(ctx.lib.dcpse_set_sell_pipeline(ctx.h, *gd), "pipeline")
             a, b = fused(lap, dx, f=ft)
             for got, want in ((a, want_lap), (b, want_dx)):
                 got = got.cpu().numpy().astype(np.float64)
@@ -257,4 +257,4 @@ def test_sell_layout_apply(name, oracle, cuda):
             got_t = lap.applyTranspose(ft).cpu().numpy().astype(np.float64)
             assert np.linalg.norm(got_t - want_t) / np.linalg.norm(want_t) <= 1e-5
     finally:
-        ctx.check(ctx.lib.dcpse_set_sell_unroll(ctx.h, 4), "unroll reset")
+        ctx.check(ctx.lib.dcpse_set_sell_pipeline(ctx.h, 2, 2), "pipeline reset")

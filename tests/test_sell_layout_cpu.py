@@ -4,6 +4,8 @@ Restates, in numpy, the index computations of k_sell_width / k_sell_fill /
 k_apply_sell (openfpm_numerics_amd/csrc/dcpse_sell.hip) and checks on ragged
 shapes (particle counts that are not multiples of 64, empty rows, one long
 row) that
+  * the lane -> row order (rows by decreasing length inside a slice, ties by
+    index, -1 for lanes without a row) is a permutation of the slice's rows;
   * the fill writes every slot of the layout exactly once (dead lanes of the
     last slice included), so no slot the kernel can read is uninitialised;
   * every 16-byte load the apply kernel issues (including the clamped re-reads
@@ -15,25 +17,49 @@ import pytest
 
 
 def layout(lengths):
+    """(L, width, slice_ptr, sell_row) as k_sell_width computes them; L[i] is the
+    length of the row served by lane i (0 for lanes without a row)."""
     n = len(lengths)
     ns = (n + 63) // 64
-    L = np.zeros(ns * 64, np.int64)
-    L[:n] = lengths
-    width = ((L.reshape(ns, 64).max(axis=1) + 3) // 4) * 4
+    key = np.full(ns * 64, -1, np.int64)
+    key[:n] = lengths
+    width = ((np.maximum(key.reshape(ns, 64).max(axis=1), 0) + 3) // 4) * 4
     slice_ptr = np.concatenate([[0], np.cumsum(64 * width)])
-    return L, width, slice_ptr
+    sell_row = np.full(ns * 64, -1, np.int64)
+    for s in range(ns):
+        k = key[64 * s: 64 * s + 64]
+        for lane in range(64):
+            rank = int(np.sum((k > k[lane]) | ((k == k[lane]) & (np.arange(64) < lane))))
+            sell_row[64 * s + rank] = 64 * s + lane if 64 * s + lane < n else -1
+    L = np.where(sell_row >= 0, key[np.maximum(sell_row, 0)], 0)
+    return L, width, slice_ptr, sell_row
 
 
 def fill_targets(L, width, slice_ptr):
     hits = np.zeros(slice_ptr[-1], np.int64)
-    for row in range(L.size):
-        s, l = divmod(row, 64)
+    for i in range(L.size):
+        s, l = divmod(i, 64)
         for j in range(width[s]):
             hits[slice_ptr[s] + (j >> 2) * 256 + 4 * l + (j & 3)] += 1
     return hits
 
 
-def kernel_reads(L, width, slice_ptr, n_rows, unr):
+def groups_loaded(ng, unr, depth):
+    """Slot groups whose col/W loads k_apply_sell (depth 2) or k_apply_sell3
+    (depth 3) issue for a slice of ng groups, before clamping."""
+    b = lambda g0: [g0 + u for u in range(unr)]
+    if depth == 2:
+        gs = b(0)
+        for g0 in range(0, ng, 2 * unr):
+            gs += b(g0 + unr) + b(g0 + 2 * unr)
+        return gs
+    gs = b(0) + b(unr)
+    for g0 in range(0, ng, 2 * unr):
+        gs += b(g0 + 2 * unr) + b(g0 + unr) + b(g0 + 3 * unr) + b(g0 + 2 * unr)
+    return gs
+
+
+def kernel_reads(L, width, slice_ptr, unr, depth=2):
     """(lo, hi) of every 16-byte load, per slice."""
     out = []
     for s in range(width.size):
@@ -42,11 +68,8 @@ def kernel_reads(L, width, slice_ptr, n_rows, unr):
             continue
         base = slice_ptr[s]
         for lane in range(64):
-            row = 64 * s + lane
-            ln = L[row] if row < n_rows else 0
-            gs = list(range(0, unr))  # prologue
-            for g0 in range(0, ng, 2 * unr):
-                gs += [g0 + unr + u for u in range(unr)] + [g0 + 2 * unr + u for u in range(unr)]
+            ln = L[64 * s + lane]
+            gs = groups_loaded(ng, unr, depth)
             for g in gs:
                 off = g * 256 if 4 * g < ln else 0
                 lo = base + 4 * lane + off
@@ -64,12 +87,39 @@ SHAPES = {
 
 
 @pytest.mark.parametrize("name", sorted(SHAPES))
-@pytest.mark.parametrize("unr", [2, 4])
-def test_sell_fill_covers_and_reads_stay_in_slice(name, unr):
+@pytest.mark.parametrize("unr,depth", [(2, 2), (4, 2), (1, 3), (2, 3)])
+def test_sell_fill_covers_and_reads_stay_in_slice(name, unr, depth):
     rng = np.random.default_rng(0)
     lengths = SHAPES[name](rng)
-    L, width, slice_ptr = layout(lengths)
+    L, width, slice_ptr, sell_row = layout(lengths)
+    n = lengths.size
+    for s in range(width.size):
+        rows = sell_row[64 * s: 64 * s + 64]
+        live = rows[rows >= 0]
+        assert sorted(live.tolist()) == list(range(64 * s, min(64 * s + 64, n)))
+        assert np.all(rows[len(live):] == -1), "lanes without a row come last"
+        assert np.all(np.diff(L[64 * s: 64 * s + 64]) <= 0), "lengths decrease with the lane"
+        assert width[s] >= L[64 * s] and width[s] % 4 == 0
     hits = fill_targets(L, width, slice_ptr)
     assert np.all(hits == 1), "fill must write every slot exactly once"
-    for s, lo, hi in kernel_reads(L, width, slice_ptr, lengths.size, unr):
+    for s, lo, hi in kernel_reads(L, width, slice_ptr, unr, depth):
         assert slice_ptr[s] <= lo and hi < slice_ptr[s + 1], (s, lo, hi)
+
+
+def test_padding_lines_are_not_touched():
+    """With lanes ordered by decreasing length, a 128-byte line (8 lanes x 16 B)
+    of a slot group is read only if one of its lanes still has slots there: the
+    lines fetched per slice are within one line per group of the row data."""
+    rng = np.random.default_rng(1)
+    lengths = rng.integers(37, 71, 64 * 40)
+    L, width, slice_ptr, _ = layout(lengths)
+    fetched = stored = 0
+    for s in range(width.size):
+        Ls = L[64 * s: 64 * s + 64]
+        for g in range(width[s] // 4):
+            for line in range(8):
+                stored += 1
+                fetched += bool(np.any(Ls[8 * line: 8 * line + 8] > 4 * g))
+    need = lengths.sum() * 4 / 128
+    assert fetched <= need * 1.06 + width.size * 2, (fetched, need, stored)
+    assert stored > need * 1.08  # the layout itself is padded
