@@ -229,6 +229,36 @@ dcpse_status dcpse_apply_transpose_sell_f32(dcpse_ctx* ctx, int64_t n_cols, cons
  * two batches ahead and weights one batch ahead. Default (2, 2). */
 dcpse_status dcpse_set_sell_pipeline(dcpse_ctx* ctx, int32_t groups, int32_t depth);
 
+/* ---- explicit time integration (df/dt = alpha * D f) ----------------- */
+/* One stage of an explicit Runge-Kutta step as ONE launch: the gather
+ * k = alpha * D u (difference form, one operator) with two axpys fused into
+ * its epilogue: out_a = y_a + ca * k and out_b = y_b + cb * k (an output is
+ * skipped when null; outputs may alias y_a/y_b but not u). Replaces the
+ * per-stage vector kernels of src/OdeIntegrators/vector_algebra_ofp_gpu.hpp:13-362
+ * that odeint's runge_kutta4 drives through src/OdeIntegrators/OdeIntegrators.hpp:63-135.
+ * CSR (f32/f64) and sliced-ELL (f32) layouts as for the apply. */
+dcpse_status dcpse_stage_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int32_t* col,
+                             const float* W, const float* u, float alpha, const float* y_a, float ca, float* out_a,
+                             const float* y_b, float cb, float* out_b);
+dcpse_status dcpse_stage_f64(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int32_t* col,
+                             const double* W, const double* u, double alpha, const double* y_a, double ca,
+                             double* out_a, const double* y_b, double cb, double* out_b);
+dcpse_status dcpse_stage_sell_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int64_t* slice_ptr,
+                                  const int32_t* sell_row, const int32_t* sell_col, const float* sell_W,
+                                  const float* u, float alpha, const float* y_a, float ca, float* out_a,
+                                  const float* y_b, float cb, float* out_b);
+/* n_steps classic RK4 steps of df/dt = alpha * D f (the integration of
+ * src/OdeIntegrators/tests/OdeIntegratores_base_tests.cpp:478-577 with
+ * runge_kutta4), state f updated in place; 4 stage launches per step.
+ * work: 3 * n_rows device elements of scratch. */
+dcpse_status dcpse_rk4_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int32_t* col,
+                           const float* W, float alpha, double dt, int64_t n_steps, float* f, float* work);
+dcpse_status dcpse_rk4_f64(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int32_t* col,
+                           const double* W, double alpha, double dt, int64_t n_steps, double* f, double* work);
+dcpse_status dcpse_rk4_sell_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int64_t* slice_ptr,
+                                const int32_t* sell_row, const int32_t* sell_col, const float* sell_W, float alpha,
+                                double dt, int64_t n_steps, float* f, float* work);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,6 +9,7 @@ from .autograd import dcpse_apply
 from .dcpse import ADAPTIVE, AT_LEAST_N_PARTICLES, LOAD, RADIUS, Dcpse, apply_ops, basis, rhs
 from .operators import (Derivative_G, Derivative_x, Derivative_xx, Derivative_xy, Derivative_xz, Derivative_y,
                         Derivative_yy, Derivative_yz, Derivative_z, Derivative_zz, Gradient, Laplacian, fused)
+from .ode import rk4
 from .particles import NON_PERIODIC, PERIODIC, ParticleSet, VerletList
 
 __all__ = [
@@ -16,5 +17,5 @@ __all__ = [
     "Dcpse", "DcpseError", "ParticleSet", "VerletList", "apply_ops", "basis", "rhs", "dcpse_apply", "fused",
     "Derivative_G", "Derivative_x", "Derivative_xx", "Derivative_xy", "Derivative_xz", "Derivative_y",
     "Derivative_yy", "Derivative_yz", "Derivative_z", "Derivative_zz", "Gradient", "Laplacian",
-    "load_library",
+    "load_library", "rk4",
 ]

@@ -97,8 +97,37 @@ struct SellArgs {
     const float* diag;
     float* out[4];
     float alpha, beta;
+    // MODE 2 (explicit time-integration stage, one operator): k = alpha * D f,
+    // out_a = y_a + ca * k and out_b = y_b + cb * k (each skipped when null)
+    const float* y_a;
+    const float* y_b;
+    float* out_a;
+    float* out_b;
+    float ca, cb;
     int64_t blocks_needed, per_xcd;
 };
+
+// Row epilogue shared by the SELL kernels: MODE 0 forward, 1 transposed
+// (explicit diagonal), 2 time-integration stage.
+template <int NOPS, int MODE>
+__device__ __forceinline__ void sell_epilogue(const SellArgs& a, int32_t row, float fp, const float* acc) {
+    if (MODE == 2) {  // both inputs are read before either output is written
+        const float k = a.alpha * acc[0];
+        const float ya = a.out_a ? a.y_a[row] : 0.0f;
+        const float yb = a.out_b ? a.y_b[row] : 0.0f;
+        if (a.out_a) a.out_a[row] = fmaf(a.ca, k, ya);
+        if (a.out_b) a.out_b[row] = fmaf(a.cb, k, yb);
+        return;
+    }
+#pragma unroll
+    for (int o = 0; o < NOPS; ++o) {
+        float val = acc[o];
+        if (MODE == 1) val = fmaf(a.diag[row], fp, val);
+        val *= a.alpha;
+        if (a.beta != 0.0f) val = fmaf(a.beta, a.out[o][row], val);
+        a.out[o][row] = val;
+    }
+}
 
 template <int NOPS, int UNR>
 struct SellBatch {
@@ -152,7 +181,7 @@ __device__ __forceinline__ void sell_fma(int g0, int len, float fp, const SellBa
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const bool v = 4 * (g0 + u) + i < len;
-            const float d = MODE == 0 ? g.fq[u][i] - fp : g.fq[u][i];
+            const float d = MODE == 1 ? g.fq[u][i] : g.fq[u][i] - fp;
 #pragma unroll
             for (int o = 0; o < NOPS; ++o) acc[o] = v ? fmaf(b.w[o][u][i], d, acc[o]) : acc[o];
         }
@@ -202,16 +231,7 @@ __global__ __launch_bounds__(kSellWaves * kWave) void k_apply_sell(const SellArg
             __builtin_amdgcn_sched_barrier(0);
         }
     }
-    if (live) {
-#pragma unroll
-        for (int o = 0; o < NOPS; ++o) {
-            float val = acc[o];
-            if (MODE == 1) val = fmaf(a.diag[row], fp, val);
-            val *= a.alpha;
-            if (a.beta != 0.0f) val = fmaf(a.beta, a.out[o][row], val);
-            a.out[o][row] = val;
-        }
-    }
+    if (live) sell_epilogue<NOPS, MODE>(a, row, fp, acc);
 }
 
 // Three-stage variant: columns are loaded two batches ahead, weights one batch
@@ -258,7 +278,7 @@ __device__ __forceinline__ void sell_fma_w(int g0, int len, float fp, const vf4a
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const bool v = 4 * (g0 + u) + i < len;
-            const float d = MODE == 0 ? g.fq[u][i] - fp : g.fq[u][i];
+            const float d = MODE == 1 ? g.fq[u][i] : g.fq[u][i] - fp;
 #pragma unroll
             for (int o = 0; o < NOPS; ++o) acc[o] = v ? fmaf(w[o][u][i], d, acc[o]) : acc[o];
         }
@@ -312,16 +332,7 @@ __global__ __launch_bounds__(kSellWaves * kWave) void k_apply_sell3(const SellAr
             __builtin_amdgcn_sched_barrier(0);
         }
     }
-    if (live) {
-#pragma unroll
-        for (int o = 0; o < NOPS; ++o) {
-            float val = acc[o];
-            if (MODE == 1) val = fmaf(a.diag[row], fp, val);
-            val *= a.alpha;
-            if (a.beta != 0.0f) val = fmaf(a.beta, a.out[o][row], val);
-            a.out[o][row] = val;
-        }
-    }
+    if (live) sell_epilogue<NOPS, MODE>(a, row, fp, acc);
 }
 
 template <int NOPS, int UNR, int DEPTH, int MODE>
@@ -335,6 +346,11 @@ static void launch_sell(SellArgs a, hipStream_t st) {
 
 template <int UNR, int DEPTH, int MODE>
 static bool dispatch_sell_ops(const SellArgs& a, int nops, hipStream_t st) {
+    if constexpr (MODE == 2) {  // stages apply one operator
+        if (nops != 1) return false;
+        launch_sell<1, UNR, DEPTH, MODE>(a, st);
+        return true;
+    }
     switch (nops) {
         case 1: launch_sell<1, UNR, DEPTH, MODE>(a, st); return true;
         case 2: launch_sell<2, UNR, DEPTH, MODE>(a, st); return true;
@@ -450,9 +466,50 @@ static dcpse_status sell_apply(dcpse_ctx* ctx, int mode, int64_t n_rows, const i
     a.diag = diag;
     a.alpha = alpha;
     a.beta = beta;
+    a.y_a = a.y_b = nullptr;
+    a.out_a = a.out_b = nullptr;
+    a.ca = a.cb = 0.0f;
     const bool ok = mode == 0 ? dispatch_sell<0>(a, n_ops, ctx->sell_groups, ctx->sell_depth, ctx->stream)
                               : dispatch_sell<1>(a, n_ops, ctx->sell_groups, ctx->sell_depth, ctx->stream);
     if (!ok) return fail(ctx, DCPSE_ERR_UNSUPPORTED, "no SELL kernel for %d operators", n_ops);
+    DCPSE_LAUNCH_CHECK(ctx);
+    return DCPSE_OK;
+}
+
+dcpse_status dcpse_stage_sell_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int64_t* slice_ptr,
+                                  const int32_t* sell_row, const int32_t* sell_col, const float* sell_W,
+                                  const float* u, float alpha, const float* y_a, float ca, float* out_a,
+                                  const float* y_b, float cb, float* out_b) {
+    if (!ctx) return DCPSE_ERR_INVALID;
+    if (!row_ptr || !slice_ptr || !sell_row || !sell_col || !sell_W || !u || n_rows < 0)
+        return fail(ctx, DCPSE_ERR_INVALID, "null pointer or negative size");
+    if ((out_a && !y_a) || (out_b && !y_b)) return fail(ctx, DCPSE_ERR_INVALID, "an output without its input");
+    if (out_a == u || out_b == u) return fail(ctx, DCPSE_ERR_INVALID, "a stage output may not alias its input u");
+    DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    if (n_rows == 0 || (!out_a && !out_b)) return DCPSE_OK;
+    SellArgs a;
+    a.n_rows = n_rows;
+    a.row_ptr = row_ptr;
+    a.slice_ptr = slice_ptr;
+    a.sell_row = sell_row;
+    a.sell_col = sell_col;
+    for (int o = 0; o < 4; ++o) {
+        a.W[o] = o == 0 ? sell_W : nullptr;
+        a.out[o] = nullptr;
+    }
+    a.f = u;
+    a.f_self = u;
+    a.diag = nullptr;
+    a.alpha = alpha;
+    a.beta = 0.0f;
+    a.y_a = y_a;
+    a.y_b = y_b;
+    a.out_a = out_a;
+    a.out_b = out_b;
+    a.ca = ca;
+    a.cb = cb;
+    if (!dispatch_sell<2>(a, 1, ctx->sell_groups, ctx->sell_depth, ctx->stream))
+        return fail(ctx, DCPSE_ERR_UNSUPPORTED, "no SELL stage kernel for this pipeline");
     DCPSE_LAUNCH_CHECK(ctx);
     return DCPSE_OK;
 }
