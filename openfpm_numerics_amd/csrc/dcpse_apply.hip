@@ -77,11 +77,11 @@ __global__ __launch_bounds__(kApplyMaxBlock) void k_apply(const ApplyArgs<T> a) 
     T fp[NF];
 #pragma unroll
     for (int b = 0; b < NF; ++b) fp[b] = a.f_self[b * a.ld_self + row];
-    T acc[NOPS][NF];
+    double acc[NOPS][NF];  // FP64 accumulation for both value types
 #pragma unroll
     for (int o = 0; o < NOPS; ++o)
 #pragma unroll
-        for (int b = 0; b < NF; ++b) acc[o][b] = T(0);
+        for (int b = 0; b < NF; ++b) acc[o][b] = 0.0;
     if (re > rs) {
         int32_t q[EPL];
         T w[NOPS][EPL];
@@ -103,9 +103,9 @@ __global__ __launch_bounds__(kApplyMaxBlock) void k_apply(const ApplyArgs<T> a) 
             for (int k = 0; k < EPL; ++k) fq[k] = a.f[b * a.ld_f + q[k]];
 #pragma unroll
             for (int k = 0; k < EPL; ++k) {
-                const T d = MODE == kDiff ? fq[k] - fp[b] : fq[k];
+                const double d = MODE == kDiff ? (double)fq[k] - (double)fp[b] : (double)fq[k];
 #pragma unroll
-                for (int o = 0; o < NOPS; ++o) acc[o][b] = v[k] ? fma(w[o][k], d, acc[o][b]) : acc[o][b];
+                for (int o = 0; o < NOPS; ++o) acc[o][b] = v[k] ? fma((double)w[o][k], d, acc[o][b]) : acc[o][b];
             }
         }
         for (int64_t j = rs + gl + G * EPL; j < re; j += G) {
@@ -116,9 +116,9 @@ __global__ __launch_bounds__(kApplyMaxBlock) void k_apply(const ApplyArgs<T> a) 
 #pragma unroll
             for (int b = 0; b < NF; ++b) {
                 const T fq = a.f[b * a.ld_f + qq];
-                const T d = MODE == kDiff ? fq - fp[b] : fq;
+                const double d = MODE == kDiff ? (double)fq - (double)fp[b] : (double)fq;
 #pragma unroll
-                for (int o = 0; o < NOPS; ++o) acc[o][b] = fma(ww[o], d, acc[o][b]);
+                for (int o = 0; o < NOPS; ++o) acc[o][b] = fma((double)ww[o], d, acc[o][b]);
             }
         }
     }
@@ -133,12 +133,12 @@ __global__ __launch_bounds__(kApplyMaxBlock) void k_apply(const ApplyArgs<T> a) 
         for (int o = 0; o < NOPS; ++o)
 #pragma unroll
             for (int b = 0; b < NF; ++b) {
-                T val = acc[o][b];
-                if (MODE == kExplicitDiag) val = fma(a.diag[row], fp[b], val);
+                double val = acc[o][b];
+                if (MODE == kExplicitDiag) val = fma((double)a.diag[row], (double)fp[b], val);
                 T* dst = a.out[o] + b * a.ld_out + row;
-                val *= a.alpha;
-                if (a.beta != T(0)) val = fma(a.beta, *dst, val);
-                *dst = val;
+                val *= (double)a.alpha;
+                if (a.beta != T(0)) val = fma((double)a.beta, (double)*dst, val);
+                *dst = (T)val;
             }
     }
 }
@@ -168,11 +168,11 @@ __global__ __launch_bounds__(kApplyMaxBlock) void k_apply_v4(const ApplyArgs<flo
     float fp[NF];
 #pragma unroll
     for (int b = 0; b < NF; ++b) fp[b] = a.f_self[b * a.ld_self + row];
-    float acc[NOPS][NF];
+    double acc[NOPS][NF];  // FP64 accumulation of the FP32 products
 #pragma unroll
     for (int o = 0; o < NOPS; ++o)
 #pragma unroll
-        for (int b = 0; b < NF; ++b) acc[o][b] = 0.0f;
+        for (int b = 0; b < NF; ++b) acc[o][b] = 0.0;
     int64_t jtail = rs + gl;  // first element of the strided scalar loop
     // 16-byte loads read elements e..e+3 with e <= re-1, so re + 3 <= nnz_total
     // keeps them inside the arrays; empty rows skip the vector section.
@@ -205,9 +205,9 @@ __global__ __launch_bounds__(kApplyMaxBlock) void k_apply_v4(const ApplyArgs<flo
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const bool v = e0[c] + i < re;
-                    const float d = MODE == kDiff ? fq[c][i] - fp[b] : fq[c][i];
+                    const double d = MODE == kDiff ? (double)fq[c][i] - (double)fp[b] : (double)fq[c][i];
 #pragma unroll
-                    for (int o = 0; o < NOPS; ++o) acc[o][b] = v ? fmaf(w[o][c][i], d, acc[o][b]) : acc[o][b];
+                    for (int o = 0; o < NOPS; ++o) acc[o][b] = v ? fma((double)w[o][c][i], d, acc[o][b]) : acc[o][b];
                 }
         }
         jtail = rs + 4 * G * NCH + gl;
@@ -220,9 +220,9 @@ __global__ __launch_bounds__(kApplyMaxBlock) void k_apply_v4(const ApplyArgs<flo
 #pragma unroll
         for (int b = 0; b < NF; ++b) {
             const float fq = a.f[b * a.ld_f + qq];
-            const float d = MODE == kDiff ? fq - fp[b] : fq;
+            const double d = MODE == kDiff ? (double)fq - (double)fp[b] : (double)fq;
 #pragma unroll
-            for (int o = 0; o < NOPS; ++o) acc[o][b] = fmaf(ww[o], d, acc[o][b]);
+            for (int o = 0; o < NOPS; ++o) acc[o][b] = fma((double)ww[o], d, acc[o][b]);
         }
     }
 #pragma unroll
@@ -236,12 +236,12 @@ __global__ __launch_bounds__(kApplyMaxBlock) void k_apply_v4(const ApplyArgs<flo
         for (int o = 0; o < NOPS; ++o)
 #pragma unroll
             for (int b = 0; b < NF; ++b) {
-                float val = acc[o][b];
-                if (MODE == kExplicitDiag) val = fmaf(a.diag[row], fp[b], val);
+                double val = acc[o][b];
+                if (MODE == kExplicitDiag) val = fma((double)a.diag[row], (double)fp[b], val);
                 float* dst = a.out[o] + b * a.ld_out + row;
-                val *= a.alpha;
-                if (a.beta != 0.0f) val = fmaf(a.beta, *dst, val);
-                *dst = val;
+                val *= (double)a.alpha;
+                if (a.beta != 0.0f) val = fma((double)a.beta, (double)*dst, val);
+                *dst = (float)val;
             }
     }
 }

@@ -46,17 +46,17 @@ __global__ __launch_bounds__(256) void k_stage_csr(const StageArgs<T> a) {
     const int gl = threadIdx.x % G;
     if (row >= a.n_rows) return;  // whole G-lane groups leave together
     const int64_t rs = a.row_ptr[row], re = a.row_ptr[row + 1];
-    const T up = a.u[row];
-    T acc = T(0);
-    for (int64_t j = rs + gl; j < re; j += G) acc = fma(a.W[j], a.u[a.col[j]] - up, acc);
+    const double up = a.u[row];
+    double acc = 0.0;  // FP64 accumulation for both value types
+    for (int64_t j = rs + gl; j < re; j += G) acc = fma((double)a.W[j], (double)a.u[a.col[j]] - up, acc);
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kWave);
     if (gl == 0) {
-        const T k = a.alpha * acc;
-        const T ya = a.out_a ? a.y_a[row] : T(0);
-        const T yb = a.out_b ? a.y_b[row] : T(0);
-        if (a.out_a) a.out_a[row] = fma(a.ca, k, ya);
-        if (a.out_b) a.out_b[row] = fma(a.cb, k, yb);
+        const double k = (double)a.alpha * acc;
+        const double ya = a.out_a ? (double)a.y_a[row] : 0.0;
+        const double yb = a.out_b ? (double)a.y_b[row] : 0.0;
+        if (a.out_a) a.out_a[row] = (T)fma((double)a.ca, k, ya);
+        if (a.out_b) a.out_b[row] = (T)fma((double)a.cb, k, yb);
     }
 }
 

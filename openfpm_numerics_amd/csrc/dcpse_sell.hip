@@ -109,23 +109,26 @@ struct SellArgs {
 
 // Row epilogue shared by the SELL kernels: MODE 0 forward, 1 transposed
 // (explicit diagonal), 2 time-integration stage.
+// The row sums are accumulated in FP64 (the FP32 products are exact in FP64):
+// at high resolution the difference-form terms cancel to first order, and
+// FP32 accumulation rounding, not the FP32 weights, dominated the error.
 template <int NOPS, int MODE>
-__device__ __forceinline__ void sell_epilogue(const SellArgs& a, int32_t row, float fp, const float* acc) {
+__device__ __forceinline__ void sell_epilogue(const SellArgs& a, int32_t row, float fp, const double* acc) {
     if (MODE == 2) {  // both inputs are read before either output is written
-        const float k = a.alpha * acc[0];
-        const float ya = a.out_a ? a.y_a[row] : 0.0f;
-        const float yb = a.out_b ? a.y_b[row] : 0.0f;
-        if (a.out_a) a.out_a[row] = fmaf(a.ca, k, ya);
-        if (a.out_b) a.out_b[row] = fmaf(a.cb, k, yb);
+        const double k = (double)a.alpha * acc[0];
+        const double ya = a.out_a ? (double)a.y_a[row] : 0.0;
+        const double yb = a.out_b ? (double)a.y_b[row] : 0.0;
+        if (a.out_a) a.out_a[row] = (float)fma((double)a.ca, k, ya);
+        if (a.out_b) a.out_b[row] = (float)fma((double)a.cb, k, yb);
         return;
     }
 #pragma unroll
     for (int o = 0; o < NOPS; ++o) {
-        float val = acc[o];
-        if (MODE == 1) val = fmaf(a.diag[row], fp, val);
-        val *= a.alpha;
-        if (a.beta != 0.0f) val = fmaf(a.beta, a.out[o][row], val);
-        a.out[o][row] = val;
+        double val = acc[o];
+        if (MODE == 1) val = fma((double)a.diag[row], (double)fp, val);
+        val *= (double)a.alpha;
+        if (a.beta != 0.0f) val = fma((double)a.beta, (double)a.out[o][row], val);
+        a.out[o][row] = (float)val;
     }
 }
 
@@ -175,15 +178,15 @@ __device__ __forceinline__ void sell_gather(const SellArgs& a, const SellBatch<N
 // The FMAs of groups [g0, g0+UNR); slots past the row are masked.
 template <int NOPS, int UNR, int MODE>
 __device__ __forceinline__ void sell_fma(int g0, int len, float fp, const SellBatch<NOPS, UNR>& b,
-                                         const SellGathered<UNR>& g, float* acc) {
+                                         const SellGathered<UNR>& g, double* acc) {
 #pragma unroll
     for (int u = 0; u < UNR; ++u)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const bool v = 4 * (g0 + u) + i < len;
-            const float d = MODE == 1 ? g.fq[u][i] : g.fq[u][i] - fp;
+            const double d = MODE == 1 ? (double)g.fq[u][i] : (double)g.fq[u][i] - (double)fp;
 #pragma unroll
-            for (int o = 0; o < NOPS; ++o) acc[o] = v ? fmaf(b.w[o][u][i], d, acc[o]) : acc[o];
+            for (int o = 0; o < NOPS; ++o) acc[o] = v ? fma((double)b.w[o][u][i], d, acc[o]) : acc[o];
         }
 }
 
@@ -209,9 +212,9 @@ __global__ __launch_bounds__(kSellWaves * kWave) void k_apply_sell(const SellArg
     const int ng = (int)((a.slice_ptr[s + 1] - base) >> 8);  // groups of 4 slots
     const int32_t* cp = a.sell_col + base + 4 * lane;
     const int64_t woff = base + 4 * lane;
-    float acc[NOPS];
+    double acc[NOPS];
 #pragma unroll
-    for (int o = 0; o < NOPS; ++o) acc[o] = 0.0f;
+    for (int o = 0; o < NOPS; ++o) acc[o] = 0.0;
     if (ng > 0) {  // a slice of empty rows has no slots to read
         SellBatch<NOPS, UNR> bA, bB;
         SellGathered<UNR> gA, gB;
@@ -272,15 +275,15 @@ __device__ __forceinline__ void sell_gather_q(const SellArgs& a, const vi4a (&q)
 
 template <int NOPS, int UNR, int MODE>
 __device__ __forceinline__ void sell_fma_w(int g0, int len, float fp, const vf4a (&w)[NOPS][UNR],
-                                           const SellGathered<UNR>& g, float* acc) {
+                                           const SellGathered<UNR>& g, double* acc) {
 #pragma unroll
     for (int u = 0; u < UNR; ++u)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const bool v = 4 * (g0 + u) + i < len;
-            const float d = MODE == 1 ? g.fq[u][i] : g.fq[u][i] - fp;
+            const double d = MODE == 1 ? (double)g.fq[u][i] : (double)g.fq[u][i] - (double)fp;
 #pragma unroll
-            for (int o = 0; o < NOPS; ++o) acc[o] = v ? fmaf(w[o][u][i], d, acc[o]) : acc[o];
+            for (int o = 0; o < NOPS; ++o) acc[o] = v ? fma((double)w[o][u][i], d, acc[o]) : acc[o];
         }
 }
 
@@ -300,9 +303,9 @@ __global__ __launch_bounds__(kSellWaves * kWave) void k_apply_sell3(const SellAr
     const int ng = (int)((a.slice_ptr[s + 1] - base) >> 8);  // groups of 4 slots
     const int32_t* cp = a.sell_col + base + 4 * lane;
     const int64_t woff = base + 4 * lane;
-    float acc[NOPS];
+    double acc[NOPS];
 #pragma unroll
-    for (int o = 0; o < NOPS; ++o) acc[o] = 0.0f;
+    for (int o = 0; o < NOPS; ++o) acc[o] = 0.0;
     if (ng > 0) {
         vi4a cX[UNR], cY[UNR];
         vf4a wX[NOPS][UNR], wY[NOPS][UNR];
