@@ -229,6 +229,21 @@ dcpse_status dcpse_apply_transpose_sell_f32(dcpse_ctx* ctx, int64_t n_cols, cons
  * two batches ahead and weights one batch ahead. Default (2, 2). */
 dcpse_status dcpse_set_sell_pipeline(dcpse_ctx* ctx, int32_t groups, int32_t depth);
 
+/* ---- sparse-matrix export -------------------------------------------- */
+/* The assembled operator as a CSR matrix in the reference's value_nz
+ * semantics (src/DCPSE/DCPSE_op/DCPSE_op.hpp:69-83, consumed by
+ * src/DCPSE/DCPSE_op/DCPSE_Solver.hpp:978-1036): row p holds W_pq at every
+ * neighbour q and sign * sum_q W_pq at column p (sign = getSign(): -1 for
+ * even nonzero |beta|, +1 otherwise), columns ascending. Same-set operators
+ * (n_rows == n_cols). Writes out_row_ptr[n_rows+1] (= row_ptr[p] + p),
+ * out_col / out_val[nnz + n_rows] (device). The diagonal is summed in FP64. */
+dcpse_status dcpse_export_csr_f64(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int32_t* col,
+                                  const double* W, double sign, int64_t* out_row_ptr, int32_t* out_col,
+                                  double* out_val);
+dcpse_status dcpse_export_csr_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int32_t* col,
+                                  const float* W, double sign, int64_t* out_row_ptr, int32_t* out_col,
+                                  float* out_val);
+
 /* ---- explicit time integration (df/dt = alpha * D f) ----------------- */
 /* One stage of an explicit Runge-Kutta step as ONE launch: the gather
  * k = alpha * D u (difference form, one operator) with two axpys fused into

@@ -62,6 +62,8 @@ SIGNATURES = {
     "dcpse_apply_sell_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _f32, _f32]),
     "dcpse_apply_transpose_sell_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f32, _f32]),
     "dcpse_set_sell_pipeline": (_i32, [_vp, _i32, _i32]),
+    "dcpse_export_csr_f64": (_i32, [_vp, _i64, _vp, _vp, _vp, _f64, _vp, _vp, _vp]),
+    "dcpse_export_csr_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _f64, _vp, _vp, _vp]),
     "dcpse_stage_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _f32, _vp, _f32, _vp, _vp, _f32, _vp]),
     "dcpse_stage_f64": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _f64, _vp, _f64, _vp, _vp, _f64, _vp]),
     "dcpse_stage_sell_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _f32, _vp, _f32, _vp, _vp, _f32,

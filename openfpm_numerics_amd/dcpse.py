@@ -208,6 +208,30 @@ class Dcpse:
     def getEpsilonInvPrefactor(self, p: int) -> float:
         return float(self._host_rows()[2][p])
 
+    # ---- sparse export (value_nz semantics, DCPSE_op.hpp:69-83) ----------------------
+    def export_csr(self, dtype=torch.float64):
+        """The operator as a CSR matrix (row_ptr int64, col int32, val) on the device:
+        W_pq at every neighbour q and getSign() * sum_q W_pq on the diagonal,
+        columns ascending - the matrix the reference's implicit solvers assemble
+        from value_nz (DCPSE_Solver.hpp:978-1036)."""
+        if not self.same_set:
+            raise ValueError("export needs an operator whose support and domain sets are the same")
+        vl = self.verletList
+        n, dev = vl.n_rows, vl.row_ptr.device
+        rp = torch.empty(n + 1, dtype=torch.int64, device=dev)
+        col = torch.empty(vl.nnz + n, dtype=torch.int32, device=dev)
+        val = torch.empty(vl.nnz + n, dtype=dtype, device=dev)
+        name = "dcpse_export_csr_f64" if dtype == torch.float64 else "dcpse_export_csr_f32"
+        _lib.context(dev.index or 0).call(name, n, _lib.ptr(vl.row_ptr), _lib.ptr(vl.col), _lib.ptr(self.weights(dtype)),
+                                          float(self.getSign()), _lib.ptr(rp), _lib.ptr(col), _lib.ptr(val))
+        return rp, col, val
+
+    def to_sparse_csr(self, dtype=torch.float64) -> torch.Tensor:
+        """export_csr() as a torch sparse CSR tensor (N x N) on the device."""
+        rp, col, val = self.export_csr(dtype)
+        n = self.verletList.n_rows
+        return torch.sparse_csr_tensor(rp, col.to(torch.int64), val, size=(n, n))
+
     # ---- adjoint -----------------------------------------------------------------
     def transposed(self, dtype):
         """(rowT_ptr, colT, WT, diag) of the transpose of the difference-form operator."""

@@ -52,6 +52,12 @@ class _DcpseOperator:
     def applyTranspose(self, g, out=None, alpha=1.0, beta=0.0):
         return self.dcpse.applyTranspose(g, out=out, alpha=alpha, beta=beta)
 
+    def export_csr(self, dtype=torch.float64):
+        return self.dcpse.export_csr(dtype)
+
+    def to_sparse_csr(self, dtype=torch.float64):
+        return self.dcpse.to_sparse_csr(dtype)
+
     # forwarders (DCPSE_op.hpp:733-784)
     def update(self, parts: ParticleSet):
         self.dcpse.initializeUpdate(parts)

@@ -83,3 +83,35 @@ def test_autograd_against_central_differences(cuda):
             fd = float(((rp - rm) * (rp + rm)).sum()) / (2 * h)
             worst = max(worst, abs(float(grad[b, q]) - fd) / max(abs(fd), 1e-30))
     assert worst <= 1e-6, f"max relative gradient error vs central differences {worst:.3e}"
+
+
+@pytest.mark.parametrize("sig", [(1, 0), (2, 0), (1, 1), "lap"])
+def test_sparse_export_matches_value_nz(sig, oracle, cuda):
+    """export_csr == the oracle's value_nz triplets summed (scipy CSR), and
+    export_csr @ f == the apply."""
+    from openfpm_numerics_amd import Dcpse, workloads as W
+
+    w, ps, vl, lap = _setup(cuda, n=40, seed=2)
+    rp, col = vl.row_ptr.cpu().numpy(), vl.col.cpu().numpy()
+    sigs = [(2, 0), (0, 2)] if sig == "lap" else [sig]
+    A = None
+    for s in sigs:
+        Ad = oracle.value_nz(oracle.assemble(w["pos"], rp, col, s, 2, w["box_lo"], w["box_hi"], w["periodic"]),
+                             rp, col)
+        A = Ad if A is None else A + Ad
+    A = A.tocsr()
+    A.sort_indices()
+    op = lap.dcpse if sig == "lap" else Dcpse(ps, vl, sig, 2, w["rcut"])
+    erp, ecol, eval_ = (t.cpu().numpy() for t in op.export_csr(torch.float64))
+    assert np.array_equal(erp, A.indptr) and np.array_equal(ecol, A.indices)
+    scale = np.abs(A.data).max()
+    assert np.max(np.abs(eval_ - A.data)) <= 1e-10 * scale
+    f = torch.randn(ps.size_local(), dtype=torch.float64, generator=torch.Generator().manual_seed(0)).to(cuda)
+    T = op.to_sparse_csr()
+    assert T.shape == (ps.size_local(), ps.size_local())
+    assert torch.equal(T.crow_indices().cpu(), torch.as_tensor(A.indptr, dtype=torch.int64))
+    got = (A @ f.cpu().numpy())
+    want = op.computeDifferentialOperator(f).cpu().numpy()
+    assert np.max(np.abs(got - want)) <= 1e-10 * np.max(np.abs(want))
+    e32 = op.export_csr(torch.float32)[2].cpu().numpy().astype(np.float64)
+    assert np.max(np.abs(e32 - A.data)) <= 1e-6 * scale
