@@ -78,7 +78,9 @@ def main():
     ap.add_argument("--tune", action="store_true",
                     help="autotune launch shapes (CSR shapes, SELL unroll, assembly particles/wave) and use the "
                          "fastest; the default is the fixed sliced-ELL kernel that the committed profiles cover")
-    ap.add_argument("--order", choices=["morton", "lattice"], default="lattice",
+    ap.add_argument("--layout", choices=["sell", "sell_bundle", "sell_bundle16", "csr"], default="sell_bundle16",
+                    help="FP32 apply layout of the timed step")
+    ap.add_argument("--order", default="lattice",
                     help="particle storage order (morton: ParticleSet.reorder, as openfpm's reorder)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run torch's copy kernel on 1 GiB (HBM counter calibration)")
@@ -117,8 +119,8 @@ def main():
     N = w["pos"].shape[0]
     ps = ParticleSet(w["pos"], w["box_lo"], w["box_hi"], bc=[1, 1, 1], device=dev)
     rcut = w["rcut"]
-    if args.order == "morton":
-        perm = ps.reorder("morton")
+    if args.order != "lattice":
+        perm = ps.reorder(args.order)
         w["pos"] = np.ascontiguousarray(w["pos"][perm])
 
     # ---- support -------------------------------------------------------------
@@ -159,7 +161,9 @@ def main():
     f32_t = f64_t.to(torch.float32).contiguous()
     out_lap = torch.empty(N, dtype=torch.float32, device=dev)
     out_dx = torch.empty(N, dtype=torch.float32, device=dev)
-    step = PreparedApply([lap.dcpse, dx.dcpse], f32_t, [out_lap, out_dx])
+    step = PreparedApply([lap.dcpse, dx.dcpse], f32_t, [out_lap, out_dx],
+                         sell={"sell": True, "sell_bundle": "bundle", "sell_bundle16": "bundle16",
+                               "csr": False}[args.layout])
     tuned = None
     sell_pipe = None
     if args.tune:
@@ -214,9 +218,12 @@ def main():
     torch.cuda.synchronize()
     pcie_ms = a.elapsed_time(b) / reps_h
 
-    # algorithmic bytes of one fused launch: col + 2 weight streams per nonzero,
-    # row_ptr (int64), f once, 2 outputs per particle
+    # algorithmic bytes of one fused launch (the per-neighbour unit of the
+    # operator: a 4-byte column and one 4-byte weight per operator; row_ptr
+    # (int64), f once, 2 outputs per particle). The 16-bit-offset layout moves
+    # fewer bytes than this (2-byte columns): its measured traffic is below it.
     alg_bytes = nnz * (4 + 4 + 4) + (N + 1) * 8 + N * 4 + 2 * N * 4
+    layout_bytes = nnz * ((2 if step.layout == "sell_bundle16" else 4) + 4 + 4) + (N + 1) * 8 + N * 4 + 2 * N * 4
     achieved = alg_bytes / (launch_ms * 1e-3) / 1e9
     traffic = read_traffic()
 
@@ -311,7 +318,7 @@ def main():
         "config": {"workload": "3d_lattice128_jitter0.1h_minus10pct_periodic_rcut2.4h_order2_lap+dx",
                    "particles": N, "nnz": nnz, "particle_order": args.order, "operators": ["Laplacian", "Derivative_x"],
                    "apply_precision": "fp32", "assembly_precision": "fp64",
-                   "apply_layout": "sell64" if step.sell else "csr",
+                   "apply_layout": step.layout,
                    "apply_launch": ({"sell_groups": (sell_pipe or (2, 2))[0], "sell_depth": (sell_pipe or (2, 2))[1]}
                                     if step.sell else
                                     {"group_lanes": tuned[0][0], "elems_per_lane": tuned[0][1],
@@ -323,7 +330,11 @@ def main():
         "fp64_error_vs_analytic": ana,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": ("k_apply_sell%s<2,%d,0>" % ("3" if (sell_pipe or (2, 2))[1] == 3 else "", (sell_pipe or (2, 2))[0])) if step.sell else "k_apply_v4<G,NCH,2,1,diff>", "alg_bytes_per_launch": alg_bytes,
+                     "kernel": ("k_apply_sell%s<2,%d,0>%s" % ("3" if (sell_pipe or (2, 2))[1] == 3 else "", (sell_pipe or (2, 2))[0],
+                                                        {"sell_bundle": " (interleaved bundle)",
+                                                         "sell_bundle16": " (interleaved bundle, 16-bit columns)"}
+                                                        .get(step.layout, "")))
+                     if step.sell else "k_apply_v4<G,NCH,2,1,diff>", "alg_bytes_per_launch": alg_bytes, "layout_min_bytes_per_launch": layout_bytes,
                      "launch_ms": launch_ms},
         "roofline_assembly": {"bound": "fp64", "achieved": asm_tflops, "peak": FP64_PEAK_TFLOPS,
                               "unit": "TFLOP/s", "frac": asm_tflops / FP64_PEAK_TFLOPS,

@@ -217,6 +217,41 @@ dcpse_status dcpse_apply_sell_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t*
                                   const int32_t* sell_row, const int32_t* sell_col, int32_t n_ops,
                                   const float* const* sell_W, const float* f, const float* f_self,
                                   float* const* out, float alpha, float beta);
+/* Interleaved form for n_ops (1..4) operators sharing the support: slot
+ * group g of slice s is one block of (1 + n_ops) * 256 words - the 64 lanes'
+ * columns, then each operator's weights (float bits) - starting at
+ * (1 + n_ops) * slice_ptr[s]; bundle holds (1 + n_ops) * n_slots words
+ * (16-byte aligned). The apply reads one contiguous span per slot group. */
+dcpse_status dcpse_sell_fill_bundle_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int32_t* col,
+                                        int32_t n_ops, const float* const* W, const int64_t* slice_ptr,
+                                        const int32_t* sell_row, int32_t* bundle);
+dcpse_status dcpse_apply_sell_bundle_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr,
+                                         const int64_t* slice_ptr, const int32_t* sell_row, int32_t n_ops,
+                                         const int32_t* bundle, const float* f, const float* f_self,
+                                         float* const* out, float alpha, float beta);
+/* Bundle with 16-bit columns (same set of n < 2^30 particles on both sides):
+ * the column q of row p is stored as d = (q - p) taken modulo n into
+ * [-n/2, n/2); slot group g of slice s is one block of 128 + 256 * n_ops
+ * words (the 64 lanes' four 16-bit offsets, then each operator's weights)
+ * starting at (128 + 256 * n_ops) * slice_ptr[s] / 256; bundle holds
+ * (0.5 + n_ops) * n_slots words. Neighbours with |d| > 32767 are stored as
+ * d = 0 and listed per lane instead: count writes exc_ptr[64*n_slices+1]
+ * (device) and *n_exc (host, synchronises); fill writes the bundle,
+ * sell_row16 (sell_row with bit 30 set on lanes that have such neighbours),
+ * exc_col[n_exc] and exc_W[n_ops*n_exc] (weights of operator o at
+ * o*n_exc + k). The apply adds the listed neighbours after the slot loop. */
+dcpse_status dcpse_sell_bundle16_count(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int32_t* col,
+                                       const int32_t* sell_row, int64_t* exc_ptr, int64_t* n_exc);
+dcpse_status dcpse_sell_fill_bundle16_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr,
+                                          const int32_t* col, int32_t n_ops, const float* const* W,
+                                          const int64_t* slice_ptr, const int32_t* sell_row,
+                                          const int64_t* exc_ptr, int64_t n_exc, int32_t* bundle,
+                                          int32_t* sell_row16, int32_t* exc_col, float* exc_W);
+dcpse_status dcpse_apply_sell_bundle16_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr,
+                                           const int64_t* slice_ptr, const int32_t* sell_row16, int32_t n_ops,
+                                           const int32_t* bundle, const int64_t* exc_ptr, const int32_t* exc_col,
+                                           const float* exc_W, int64_t n_exc, const float* f, const float* f_self,
+                                           float* const* out, float alpha, float beta);
 /* Same result as dcpse_apply_transpose_f32 with one field, from the SELL
  * layout of the transposed CSR (built with the three calls above). */
 dcpse_status dcpse_apply_transpose_sell_f32(dcpse_ctx* ctx, int64_t n_cols, const int64_t* rowT_ptr,

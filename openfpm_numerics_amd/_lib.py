@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libdcpse_hip.so")
 
 DCPSE_OK = 0
+DCPSE_ERR_RANGE = 4
 DCPSE_INFO = {0: "ok", 1: "fewer neighbours than monomials", 2: "moment matrix not positive definite",
               3: "eps is zero (coincident support)", 4: "non-finite kernel weight"}
 DCPSE_ASM_REFINE = 1
@@ -62,6 +63,12 @@ SIGNATURES = {
     "dcpse_apply_sell_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _f32, _f32]),
     "dcpse_apply_transpose_sell_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f32, _f32]),
     "dcpse_set_sell_pipeline": (_i32, [_vp, _i32, _i32]),
+    "dcpse_sell_fill_bundle_f32": (_i32, [_vp, _i64, _vp, _vp, _i32, _vp, _vp, _vp, _vp]),
+    "dcpse_apply_sell_bundle_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _f32, _f32]),
+    "dcpse_sell_bundle16_count": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp]),
+    "dcpse_sell_fill_bundle16_f32": (_i32, [_vp, _i64, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp]),
+    "dcpse_apply_sell_bundle16_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp,
+                                             _f32, _f32]),
     "dcpse_export_csr_f64": (_i32, [_vp, _i64, _vp, _vp, _vp, _f64, _vp, _vp, _vp]),
     "dcpse_export_csr_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _f64, _vp, _vp, _vp]),
     "dcpse_stage_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _f32, _vp, _f32, _vp, _vp, _f32, _vp]),

@@ -19,6 +19,7 @@
 // footprint but hardly any HBM traffic. Padded slots hold column 0 and
 // weight 0 and are masked out by the row length.
 #include <algorithm>
+#include <type_traits>
 
 #include "dcpse_common.hpp"
 #include "dcpse_internal.hpp"
@@ -26,6 +27,7 @@
 namespace dcpse {
 
 typedef int32_t vi4a __attribute__((ext_vector_type(4)));
+typedef int16_t vs4a __attribute__((ext_vector_type(4)));
 typedef float vf4a __attribute__((ext_vector_type(4)));
 
 constexpr int kSellWaves = 4;  // slices (wavefronts) per block
@@ -85,12 +87,126 @@ __global__ void k_sell_fill(int64_t n_rows, const int64_t* __restrict__ row_ptr,
     }
 }
 
+// Interleaved ("bundle") form of the layout for n_ops operators sharing one
+// support: slot group g of slice s is one block of (1 + n_ops) * 256 words,
+// [columns of the 64 lanes][W_0 of the 64 lanes]...[W_{n_ops-1}], starting at
+// (1 + n_ops) * slice_ptr[s] (weights stored as their float bits). A
+// wavefront then reads one contiguous (1 + n_ops) KB span per slot group
+// instead of 1 KB from each of 1 + n_ops arrays. Every word is written.
+__global__ void k_sell_fill_bundle(int64_t n_rows, const int64_t* __restrict__ row_ptr,
+                                   const int32_t* __restrict__ col, int n_ops, const float* W0, const float* W1,
+                                   const float* W2, const float* W3, const int64_t* __restrict__ slice_ptr,
+                                   const int32_t* __restrict__ sell_row, int32_t* __restrict__ bundle) {
+    const int64_t n_lanes = ((n_rows + 63) / 64) * 64;
+    const float* W[4] = {W0, W1, W2, W3};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_lanes;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = i >> 6;
+        const int l = (int)(i & 63);
+        const int64_t base = slice_ptr[s];
+        const int width = (int)((slice_ptr[s + 1] - base) / 64);
+        const int32_t row = sell_row[i];
+        const int64_t rs = row >= 0 ? row_ptr[row] : 0;
+        const int len = row >= 0 ? (int)(row_ptr[row + 1] - rs) : 0;
+        for (int j = 0; j < width; ++j) {
+            const int64_t dst = (int64_t)(1 + n_ops) * (base + (int64_t)(j >> 2) * 256) + 4 * l + (j & 3);
+            bundle[dst] = j < len ? col[rs + j] : 0;
+            for (int o = 0; o < n_ops; ++o) bundle[dst + (1 + o) * 256] = j < len ? __float_as_int(W[o][rs + j]) : 0;
+        }
+    }
+}
+
+// The same interleaved layout with 16-bit columns, for operators whose support
+// and domain are one set of n particles: a column q of row p is stored as the
+// offset d = q - p taken modulo n into [-n/2, n/2). Slot group g of slice s is
+// one block of 128 + 256 * n_ops words - the 64 lanes' 4 offsets (8 bytes per
+// lane), then each operator's weights - starting at
+// (128 + 256 * n_ops) * slice_ptr[s] / 256. Padding and lanes without a row
+// hold offset 0.
+// A neighbour further than 32767 positions (in a lattice-ordered periodic set,
+// the rare pairs whose offset combines a wrap with plane steps) is stored as
+// offset 0 - so in the gather it contributes W * (f_p - f_p) = 0 - and kept in
+// a per-lane exception list (exc_ptr over lanes, exc_col, exc_W[o * n_exc + k])
+// that the apply adds after the slot loop; lanes with exceptions are marked
+// in sell_row16 by kExcFlag.
+constexpr int32_t kExcFlag = 1 << 30;
+
+__device__ __forceinline__ int64_t col_offset(int32_t q, int32_t p, int64_t n) {
+    int64_t d = (int64_t)q - p;
+    if (2 * d >= n) d -= n;
+    if (2 * d < -n) d += n;
+    return d;
+}
+
+// exc_count[i] = far neighbours of the row of lane i.
+__global__ void k_bundle16_count(int64_t n_rows, const int64_t* __restrict__ row_ptr,
+                                 const int32_t* __restrict__ col, const int32_t* __restrict__ sell_row,
+                                 int32_t* __restrict__ exc_count) {
+    const int64_t n_lanes = ((n_rows + 63) / 64) * 64;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_lanes;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t row = sell_row[i];
+        int c = 0;
+        if (row >= 0)
+            for (int64_t j = row_ptr[row]; j < row_ptr[row + 1]; ++j) {
+                const int64_t d = col_offset(col[j], row, n_rows);
+                c += d > 32767 || d < -32768;
+            }
+        exc_count[i] = c;
+    }
+}
+
+__global__ void k_sell_fill_bundle16(int64_t n_rows, const int64_t* __restrict__ row_ptr,
+                                     const int32_t* __restrict__ col, int n_ops, const float* W0, const float* W1,
+                                     const float* W2, const float* W3, const int64_t* __restrict__ slice_ptr,
+                                     const int32_t* __restrict__ sell_row, const int64_t* __restrict__ exc_ptr,
+                                     int64_t n_exc, int32_t* __restrict__ bundle, int32_t* __restrict__ sell_row16,
+                                     int32_t* __restrict__ exc_col, float* __restrict__ exc_W) {
+    const int64_t n_lanes = ((n_rows + 63) / 64) * 64;
+    const float* W[4] = {W0, W1, W2, W3};
+    const int64_t blk = 128 + 256 * (int64_t)n_ops;
+    int16_t* cols16 = reinterpret_cast<int16_t*>(bundle);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_lanes;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = i >> 6;
+        const int l = (int)(i & 63);
+        const int64_t base = slice_ptr[s];
+        const int width = (int)((slice_ptr[s + 1] - base) / 64);
+        const int32_t row = sell_row[i];
+        const int64_t rs = row >= 0 ? row_ptr[row] : 0;
+        const int len = row >= 0 ? (int)(row_ptr[row + 1] - rs) : 0;
+        int64_t e = exc_ptr[i];
+        sell_row16[i] = row < 0 ? row : (exc_ptr[i + 1] > e ? (row | kExcFlag) : row);
+        for (int j = 0; j < width; ++j) {
+            const int64_t gb = (base / 256 + (j >> 2)) * blk;  // word offset of the slot group's block
+            int64_t d = 0;
+            if (j < len) {
+                d = col_offset(col[rs + j], row, n_rows);
+                if (d > 32767 || d < -32768) {
+                    exc_col[e] = col[rs + j];
+                    for (int o = 0; o < n_ops; ++o) exc_W[o * n_exc + e] = W[o][rs + j];
+                    ++e;
+                    d = 0;
+                }
+            }
+            cols16[2 * gb + 4 * l + (j & 3)] = (int16_t)d;
+            for (int o = 0; o < n_ops; ++o)
+                bundle[gb + 128 + o * 256 + 4 * l + (j & 3)] = j < len ? __float_as_int(W[o][rs + j]) : 0;
+        }
+    }
+}
+
 struct SellArgs {
     int64_t n_rows;
     const int64_t* row_ptr;
     const int64_t* slice_ptr;
     const int32_t* sell_row;
     const int32_t* sell_col;
+    const int32_t* bundle;  // interleaved layout (see k_sell_fill_bundle), or null
+    const int64_t* exc_ptr;  // 16-bit bundle: far neighbours per lane
+    const int32_t* exc_col;
+    const float* exc_W;
+    int64_t n_exc;
     const float* W[4];
     const float* f;
     const float* f_self;
@@ -132,9 +248,9 @@ __device__ __forceinline__ void sell_epilogue(const SellArgs& a, int32_t row, fl
     }
 }
 
-template <int NOPS, int UNR>
+template <int NOPS, int UNR, bool C16 = false>
 struct SellBatch {
-    vi4a q[UNR];
+    typename std::conditional<C16, vs4a, vi4a>::type q[UNR];  // columns, or 16-bit offsets
     vf4a w[NOPS][UNR];
 };
 
@@ -145,17 +261,47 @@ struct SellBatch {
 // without a row hold column 0, so every gather stays inside f. The loaded
 // columns are used as they are (no select on them), so nothing waits for a
 // batch before its gathers are due.
-template <int NOPS, int UNR>
-__device__ __forceinline__ void sell_load(const SellArgs& a, const int32_t* cp, int64_t woff, int g0, int len,
-                                          SellBatch<NOPS, UNR>& b) {
+template <int NOPS, int UNR, bool C16 = false>
+__device__ __forceinline__ void sell_load(const int32_t* cp, const float* const* wp, int64_t gs, int g0, int len,
+                                          SellBatch<NOPS, UNR, C16>& b) {
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
         const int g = g0 + u;
-        const int64_t off = 4 * g < len ? (int64_t)g * 256 : 0;
-        b.q[u] = *(const vi4a*)(cp + off);
+        const int64_t off = 4 * g < len ? (int64_t)g * gs : 0;
+        if constexpr (C16)  // cp points at this lane's 8 bytes of offsets
+            b.q[u] = *(const vs4a*)(reinterpret_cast<const int16_t*>(cp) + 2 * off);
+        else
+            b.q[u] = *(const vi4a*)(cp + off);
 #pragma unroll
-        for (int o = 0; o < NOPS; ++o) b.w[o][u] = *(const vf4a*)(a.W[o] + woff + off);
+        for (int o = 0; o < NOPS; ++o) b.w[o][u] = *(const vf4a*)(wp[o] + off);
     }
+}
+
+// Where lane `lane` of slice s finds its columns and weights, and the element
+// stride between its consecutive slot groups: separate arrays (stride 256) or
+// the interleaved bundle (stride (1 + NOPS) * 256).
+template <int NOPS, bool C16 = false>
+__device__ __forceinline__ int64_t sell_streams(const SellArgs& a, int64_t base, int lane, const int32_t*& cp,
+                                                const float* (&wp)[NOPS]) {
+    if constexpr (C16) {
+        const int64_t blk = 128 + 256 * NOPS;
+        const int32_t* bp = a.bundle + (base / 256) * blk;
+        cp = bp + 2 * lane;  // 8 bytes of offsets per lane
+#pragma unroll
+        for (int o = 0; o < NOPS; ++o) wp[o] = reinterpret_cast<const float*>(bp + 128 + o * 256) + 4 * lane;
+        return blk;
+    }
+    if (a.bundle) {
+        const int32_t* bp = a.bundle + (int64_t)(1 + NOPS) * base + 4 * lane;
+        cp = bp;
+#pragma unroll
+        for (int o = 0; o < NOPS; ++o) wp[o] = reinterpret_cast<const float*>(bp + (1 + o) * 256);
+        return (int64_t)(1 + NOPS) * 256;
+    }
+    cp = a.sell_col + base + 4 * lane;
+#pragma unroll
+    for (int o = 0; o < NOPS; ++o) wp[o] = a.W[o] + base + 4 * lane;
+    return 256;
 }
 
 // The f gathers of groups [g0, g0+UNR) (columns of a loaded batch).
@@ -164,20 +310,37 @@ struct SellGathered {
     float fq[UNR][4];
 };
 
-template <int NOPS, int UNR>
-__device__ __forceinline__ void sell_gather(const SellArgs& a, const SellBatch<NOPS, UNR>& b, SellGathered<UNR>& g) {
+// Column of a 16-bit offset d of row rb in a set of n particles.
+__device__ __forceinline__ int32_t sell_col16(int32_t rb, int16_t d, int32_t n) {
+    int32_t q = rb + (int32_t)d;
+    q += q < 0 ? n : 0;
+    q -= q >= n ? n : 0;
+    return q;
+}
+
+template <int NOPS, int UNR, bool C16 = false>
+__device__ __forceinline__ void sell_gather(const SellArgs& a, const SellBatch<NOPS, UNR, C16>& b,
+                                            SellGathered<UNR>& g, int32_t rb = 0) {
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
-        g.fq[u][0] = a.f[b.q[u].x];
-        g.fq[u][1] = a.f[b.q[u].y];
-        g.fq[u][2] = a.f[b.q[u].z];
-        g.fq[u][3] = a.f[b.q[u].w];
+        if constexpr (C16) {
+            const int32_t n = (int32_t)a.n_rows;
+            g.fq[u][0] = a.f[sell_col16(rb, b.q[u].x, n)];
+            g.fq[u][1] = a.f[sell_col16(rb, b.q[u].y, n)];
+            g.fq[u][2] = a.f[sell_col16(rb, b.q[u].z, n)];
+            g.fq[u][3] = a.f[sell_col16(rb, b.q[u].w, n)];
+        } else {
+            g.fq[u][0] = a.f[b.q[u].x];
+            g.fq[u][1] = a.f[b.q[u].y];
+            g.fq[u][2] = a.f[b.q[u].z];
+            g.fq[u][3] = a.f[b.q[u].w];
+        }
     }
 }
 
 // The FMAs of groups [g0, g0+UNR); slots past the row are masked.
-template <int NOPS, int UNR, int MODE>
-__device__ __forceinline__ void sell_fma(int g0, int len, float fp, const SellBatch<NOPS, UNR>& b,
+template <int NOPS, int UNR, int MODE, bool C16 = false>
+__device__ __forceinline__ void sell_fma(int g0, int len, float fp, const SellBatch<NOPS, UNR, C16>& b,
                                          const SellGathered<UNR>& g, double* acc) {
 #pragma unroll
     for (int u = 0; u < UNR; ++u)
@@ -196,7 +359,7 @@ __device__ __forceinline__ void sell_fma(int g0, int len, float fp, const SellBa
 // waiting for the gathers does not wait for the stream loads issued after
 // them: the next batch's col/W loads stay in flight across the FMAs, and its
 // gathers go out as soon as its columns arrive.
-template <int NOPS, int UNR, int MODE>
+template <int NOPS, int UNR, int MODE, bool C16 = false>
 __global__ __launch_bounds__(kSellWaves * kWave) void k_apply_sell(const SellArgs a) {
     const int64_t hb = blockIdx.x;
     const int64_t lb = (hb & 7) * a.per_xcd + (hb >> 3);
@@ -204,34 +367,48 @@ __global__ __launch_bounds__(kSellWaves * kWave) void k_apply_sell(const SellArg
     const int lane = threadIdx.x & 63;
     const int64_t s = lb * kSellWaves + (threadIdx.x >> 6);
     if (s * 64 >= a.n_rows) return;  // whole wavefront
-    const int32_t row = a.sell_row[s * 64 + lane];
-    const bool live = row >= 0;
+    const int32_t row_raw = a.sell_row[s * 64 + lane];
+    const bool live = row_raw >= 0;
+    const bool has_exc = C16 && live && (row_raw & kExcFlag);
+    const int32_t row = C16 && live ? (row_raw & ~kExcFlag) : row_raw;
     const int len = live ? (int)(a.row_ptr[row + 1] - a.row_ptr[row]) : 0;
     const float fp = live ? a.f_self[row] : 0.0f;
     const int64_t base = a.slice_ptr[s];
     const int ng = (int)((a.slice_ptr[s + 1] - base) >> 8);  // groups of 4 slots
-    const int32_t* cp = a.sell_col + base + 4 * lane;
-    const int64_t woff = base + 4 * lane;
+    const int32_t* cp;
+    const float* wp[NOPS];
+    const int64_t gs = sell_streams<NOPS, C16>(a, base, lane, cp, wp);
+    const int32_t rb = live ? row : 0;  // 16-bit offsets are relative to the lane's row
     double acc[NOPS];
 #pragma unroll
     for (int o = 0; o < NOPS; ++o) acc[o] = 0.0;
     if (ng > 0) {  // a slice of empty rows has no slots to read
-        SellBatch<NOPS, UNR> bA, bB;
+        SellBatch<NOPS, UNR, C16> bA, bB;
         SellGathered<UNR> gA, gB;
-        sell_load<NOPS, UNR>(a, cp, woff, 0, len, bA);
+        sell_load<NOPS, UNR, C16>(cp, wp, gs, 0, len, bA);
         for (int g0 = 0; g0 < ng; g0 += 2 * UNR) {
-            sell_gather<NOPS, UNR>(a, bA, gA);
+            sell_gather<NOPS, UNR, C16>(a, bA, gA, rb);
             __builtin_amdgcn_sched_barrier(0);
-            sell_load<NOPS, UNR>(a, cp, woff, g0 + UNR, len, bB);
+            sell_load<NOPS, UNR, C16>(cp, wp, gs, g0 + UNR, len, bB);
             __builtin_amdgcn_sched_barrier(0);
-            sell_fma<NOPS, UNR, MODE>(g0, len, fp, bA, gA, acc);
+            sell_fma<NOPS, UNR, MODE, C16>(g0, len, fp, bA, gA, acc);
             __builtin_amdgcn_sched_barrier(0);
-            sell_gather<NOPS, UNR>(a, bB, gB);
+            sell_gather<NOPS, UNR, C16>(a, bB, gB, rb);
             __builtin_amdgcn_sched_barrier(0);
-            sell_load<NOPS, UNR>(a, cp, woff, g0 + 2 * UNR, len, bA);
+            sell_load<NOPS, UNR, C16>(cp, wp, gs, g0 + 2 * UNR, len, bA);
             __builtin_amdgcn_sched_barrier(0);
-            sell_fma<NOPS, UNR, MODE>(g0 + UNR, len, fp, bB, gB, acc);
+            sell_fma<NOPS, UNR, MODE, C16>(g0 + UNR, len, fp, bB, gB, acc);
             __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    if constexpr (C16) {
+        if (has_exc) {  // the far neighbours of this row (rare; after every pipelined load)
+            const int64_t i = s * 64 + lane;
+            for (int64_t k = a.exc_ptr[i]; k < a.exc_ptr[i + 1]; ++k) {
+                const double d = (double)a.f[a.exc_col[k]] - (double)fp;
+#pragma unroll
+                for (int o = 0; o < NOPS; ++o) acc[o] = fma((double)a.exc_W[o * a.n_exc + k], d, acc[o]);
+            }
         }
     }
     if (live) sell_epilogue<NOPS, MODE>(a, row, fp, acc);
@@ -243,22 +420,23 @@ __global__ __launch_bounds__(kSellWaves * kWave) void k_apply_sell(const SellArg
 // full step earlier (loads complete in issue order), at the cost of holding
 // two column, two weight and two gather batches in registers.
 template <int NOPS, int UNR>
-__device__ __forceinline__ void sell_load_cols(const int32_t* cp, int g0, int len, vi4a (&q)[UNR]) {
+__device__ __forceinline__ void sell_load_cols(const int32_t* cp, int64_t gs, int g0, int len, vi4a (&q)[UNR]) {
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
         const int g = g0 + u;
-        q[u] = *(const vi4a*)(cp + (4 * g < len ? (int64_t)g * 256 : 0));
+        q[u] = *(const vi4a*)(cp + (4 * g < len ? (int64_t)g * gs : 0));
     }
 }
 
 template <int NOPS, int UNR>
-__device__ __forceinline__ void sell_load_w(const SellArgs& a, int64_t woff, int g0, int len, vf4a (&w)[NOPS][UNR]) {
+__device__ __forceinline__ void sell_load_w(const float* const* wp, int64_t gs, int g0, int len,
+                                            vf4a (&w)[NOPS][UNR]) {
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
         const int g = g0 + u;
-        const int64_t off = 4 * g < len ? (int64_t)g * 256 : 0;
+        const int64_t off = 4 * g < len ? (int64_t)g * gs : 0;
 #pragma unroll
-        for (int o = 0; o < NOPS; ++o) w[o][u] = *(const vf4a*)(a.W[o] + woff + off);
+        for (int o = 0; o < NOPS; ++o) w[o][u] = *(const vf4a*)(wp[o] + off);
     }
 }
 
@@ -301,8 +479,9 @@ __global__ __launch_bounds__(kSellWaves * kWave) void k_apply_sell3(const SellAr
     const float fp = live ? a.f_self[row] : 0.0f;
     const int64_t base = a.slice_ptr[s];
     const int ng = (int)((a.slice_ptr[s + 1] - base) >> 8);  // groups of 4 slots
-    const int32_t* cp = a.sell_col + base + 4 * lane;
-    const int64_t woff = base + 4 * lane;
+    const int32_t* cp;
+    const float* wp[NOPS];
+    const int64_t gs = sell_streams<NOPS>(a, base, lane, cp, wp);
     double acc[NOPS];
 #pragma unroll
     for (int o = 0; o < NOPS; ++o) acc[o] = 0.0;
@@ -310,24 +489,24 @@ __global__ __launch_bounds__(kSellWaves * kWave) void k_apply_sell3(const SellAr
         vi4a cX[UNR], cY[UNR];
         vf4a wX[NOPS][UNR], wY[NOPS][UNR];
         SellGathered<UNR> gX, gY;
-        sell_load_cols<NOPS, UNR>(cp, 0, len, cX);
-        sell_load_cols<NOPS, UNR>(cp, UNR, len, cY);
-        sell_load_w<NOPS, UNR>(a, woff, 0, len, wX);
+        sell_load_cols<NOPS, UNR>(cp, gs, 0, len, cX);
+        sell_load_cols<NOPS, UNR>(cp, gs, UNR, len, cY);
+        sell_load_w<NOPS, UNR>(wp, gs, 0, len, wX);
         __builtin_amdgcn_sched_barrier(0);
         sell_gather_q<UNR>(a, cX, gX);
         __builtin_amdgcn_sched_barrier(0);
         for (int g0 = 0; g0 < ng; g0 += 2 * UNR) {
             // batch g0: W in wX, gathers in gX; batch g0+UNR: columns in cY
-            sell_load_cols<NOPS, UNR>(cp, g0 + 2 * UNR, len, cX);
-            sell_load_w<NOPS, UNR>(a, woff, g0 + UNR, len, wY);
+            sell_load_cols<NOPS, UNR>(cp, gs, g0 + 2 * UNR, len, cX);
+            sell_load_w<NOPS, UNR>(wp, gs, g0 + UNR, len, wY);
             __builtin_amdgcn_sched_barrier(0);
             sell_gather_q<UNR>(a, cY, gY);
             __builtin_amdgcn_sched_barrier(0);
             sell_fma_w<NOPS, UNR, MODE>(g0, len, fp, wX, gX, acc);
             __builtin_amdgcn_sched_barrier(0);
             // batch g0+UNR: W in wY, gathers in gY; batch g0+2UNR: columns in cX
-            sell_load_cols<NOPS, UNR>(cp, g0 + 3 * UNR, len, cY);
-            sell_load_w<NOPS, UNR>(a, woff, g0 + 2 * UNR, len, wX);
+            sell_load_cols<NOPS, UNR>(cp, gs, g0 + 3 * UNR, len, cY);
+            sell_load_w<NOPS, UNR>(wp, gs, g0 + 2 * UNR, len, wX);
             __builtin_amdgcn_sched_barrier(0);
             sell_gather_q<UNR>(a, cX, gX);
             __builtin_amdgcn_sched_barrier(0);
@@ -345,6 +524,26 @@ static void launch_sell(SellArgs a, hipStream_t st) {
     a.per_xcd = (a.blocks_needed + 7) / 8;
     if (DEPTH == 3) k_apply_sell3<NOPS, UNR, MODE><<<(unsigned)(a.per_xcd * 8), kSellWaves * kWave, 0, st>>>(a);
     else k_apply_sell<NOPS, UNR, MODE><<<(unsigned)(a.per_xcd * 8), kSellWaves * kWave, 0, st>>>(a);
+}
+
+template <int NOPS, int UNR>
+static void launch_sell16(SellArgs a, hipStream_t st) {
+    const int64_t n_slices = (a.n_rows + 63) / 64;
+    a.blocks_needed = (n_slices + kSellWaves - 1) / kSellWaves;
+    a.per_xcd = (a.blocks_needed + 7) / 8;
+    k_apply_sell<NOPS, UNR, 0, true><<<(unsigned)(a.per_xcd * 8), kSellWaves * kWave, 0, st>>>(a);
+}
+
+// 16-bit-offset bundles: forward apply, two-stage pipeline (groups 2 or 4).
+static bool dispatch_sell16(const SellArgs& a, int nops, int unr, hipStream_t st) {
+    const bool u4 = unr == 4;
+    switch (nops) {
+        case 1: u4 ? launch_sell16<1, 4>(a, st) : launch_sell16<1, 2>(a, st); return true;
+        case 2: u4 ? launch_sell16<2, 4>(a, st) : launch_sell16<2, 2>(a, st); return true;
+        case 3: u4 ? launch_sell16<3, 4>(a, st) : launch_sell16<3, 2>(a, st); return true;
+        case 4: u4 ? launch_sell16<4, 4>(a, st) : launch_sell16<4, 2>(a, st); return true;
+    }
+    return false;
 }
 
 template <int UNR, int DEPTH, int MODE>
@@ -444,24 +643,26 @@ dcpse_status dcpse_sell_fill_weights_f32(dcpse_ctx* ctx, int64_t n_rows, const i
 
 static dcpse_status sell_apply(dcpse_ctx* ctx, int mode, int64_t n_rows, const int64_t* row_ptr,
                                const int64_t* slice_ptr, const int32_t* sell_row, const int32_t* sell_col,
-                               int32_t n_ops, const float* const* sell_W, const float* f, const float* f_self,
-                               const float* diag, float* const* out, float alpha, float beta) {
+                               const int32_t* bundle, int32_t n_ops, const float* const* sell_W, const float* f,
+                               const float* f_self, const float* diag, float* const* out, float alpha, float beta) {
     if (!ctx) return DCPSE_ERR_INVALID;
-    if (!row_ptr || !slice_ptr || !sell_row || !sell_col || !sell_W || !f || !out || n_rows < 0 || n_ops < 1)
+    if (!row_ptr || !slice_ptr || !sell_row || !(bundle || (sell_col && sell_W)) || !f || !out || n_rows < 0 ||
+        n_ops < 1)
         return fail(ctx, DCPSE_ERR_INVALID, "null pointer or bad size");
     if (n_ops > 4) return fail(ctx, DCPSE_ERR_UNSUPPORTED, "at most 4 operators per pass");
     for (int o = 0; o < n_ops; ++o)
-        if (!sell_W[o] || !out[o]) return fail(ctx, DCPSE_ERR_INVALID, "null operator pointer");
+        if ((!bundle && !sell_W[o]) || !out[o]) return fail(ctx, DCPSE_ERR_INVALID, "null operator pointer");
     DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     if (n_rows == 0) return DCPSE_OK;
-    SellArgs a;
+    SellArgs a{};
     a.n_rows = n_rows;
     a.row_ptr = row_ptr;
     a.slice_ptr = slice_ptr;
     a.sell_row = sell_row;
     a.sell_col = sell_col;
+    a.bundle = bundle;
     for (int o = 0; o < 4; ++o) {
-        a.W[o] = o < n_ops ? sell_W[o] : nullptr;
+        a.W[o] = o < n_ops && sell_W ? sell_W[o] : nullptr;
         a.out[o] = o < n_ops ? out[o] : nullptr;
     }
     a.f = f;
@@ -479,6 +680,117 @@ static dcpse_status sell_apply(dcpse_ctx* ctx, int mode, int64_t n_rows, const i
     return DCPSE_OK;
 }
 
+dcpse_status dcpse_sell_fill_bundle_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int32_t* col,
+                                        int32_t n_ops, const float* const* W, const int64_t* slice_ptr,
+                                        const int32_t* sell_row, int32_t* bundle) {
+    if (!ctx) return DCPSE_ERR_INVALID;
+    if (!row_ptr || !col || !W || !slice_ptr || !sell_row || !bundle || n_rows < 0)
+        return fail(ctx, DCPSE_ERR_INVALID, "null pointer or negative size");
+    if (n_ops < 1 || n_ops > 4) return fail(ctx, DCPSE_ERR_UNSUPPORTED, "1 to 4 operators per bundle");
+    for (int o = 0; o < n_ops; ++o)
+        if (!W[o]) return fail(ctx, DCPSE_ERR_INVALID, "null weight pointer");
+    DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    if (n_rows == 0) return DCPSE_OK;
+    const int64_t b = std::min<int64_t>((n_rows + 63 + 255) / 256, 65536);
+    k_sell_fill_bundle<<<(unsigned)b, 256, 0, ctx->stream>>>(n_rows, row_ptr, col, n_ops, W[0],
+                                                             n_ops > 1 ? W[1] : nullptr, n_ops > 2 ? W[2] : nullptr,
+                                                             n_ops > 3 ? W[3] : nullptr, slice_ptr, sell_row, bundle);
+    DCPSE_LAUNCH_CHECK(ctx);
+    return DCPSE_OK;
+}
+
+dcpse_status dcpse_apply_sell_bundle_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr,
+                                         const int64_t* slice_ptr, const int32_t* sell_row, int32_t n_ops,
+                                         const int32_t* bundle, const float* f, const float* f_self,
+                                         float* const* out, float alpha, float beta) {
+    if (!bundle) return fail(ctx, DCPSE_ERR_INVALID, "bundle is null");
+    return sell_apply(ctx, 0, n_rows, row_ptr, slice_ptr, sell_row, nullptr, bundle, n_ops, nullptr, f, f_self,
+                      nullptr, out, alpha, beta);
+}
+
+dcpse_status dcpse_sell_bundle16_count(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int32_t* col,
+                                       const int32_t* sell_row, int64_t* exc_ptr, int64_t* n_exc) {
+    if (!ctx) return DCPSE_ERR_INVALID;
+    if (!row_ptr || !col || !sell_row || !exc_ptr || !n_exc || n_rows < 0)
+        return fail(ctx, DCPSE_ERR_INVALID, "null pointer or negative size");
+    if (n_rows >= kExcFlag) return fail(ctx, DCPSE_ERR_RANGE, "too many particles for the 16-bit bundle");
+    DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    const int64_t n_lanes = ((n_rows + 63) / 64) * 64;
+    Workspace w;
+    dcpse_status st = reserve_workspace(ctx, (size_t)(n_lanes + 1) * 4 + scan_workspace_bytes(n_lanes) + 1024, &w);
+    if (st) return st;
+    int32_t* counts = w.take<int32_t>(n_lanes + 1);
+    if (n_lanes > 0) {
+        const int64_t b = std::min<int64_t>((n_lanes + 255) / 256, 65536);
+        k_bundle16_count<<<(unsigned)b, 256, 0, ctx->stream>>>(n_rows, row_ptr, col, sell_row, counts);
+        DCPSE_LAUNCH_CHECK(ctx);
+    }
+    if ((st = scan_counts(ctx, counts, n_lanes, exc_ptr, &w))) return st;
+    int64_t total = 0;
+    DCPSE_HIP_CHECK(ctx, hipMemcpyAsync(&total, exc_ptr + n_lanes, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    DCPSE_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    *n_exc = total;
+    return DCPSE_OK;
+}
+
+dcpse_status dcpse_sell_fill_bundle16_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr,
+                                          const int32_t* col, int32_t n_ops, const float* const* W,
+                                          const int64_t* slice_ptr, const int32_t* sell_row,
+                                          const int64_t* exc_ptr, int64_t n_exc, int32_t* bundle,
+                                          int32_t* sell_row16, int32_t* exc_col, float* exc_W) {
+    if (!ctx) return DCPSE_ERR_INVALID;
+    if (!row_ptr || !col || !W || !slice_ptr || !sell_row || !exc_ptr || !bundle || !sell_row16 || n_rows < 0 ||
+        n_exc < 0 || (n_exc > 0 && (!exc_col || !exc_W)))
+        return fail(ctx, DCPSE_ERR_INVALID, "null pointer or negative size");
+    if (n_ops < 1 || n_ops > 4) return fail(ctx, DCPSE_ERR_UNSUPPORTED, "1 to 4 operators per bundle");
+    if (n_rows >= kExcFlag) return fail(ctx, DCPSE_ERR_RANGE, "too many particles for the 16-bit bundle");
+    for (int o = 0; o < n_ops; ++o)
+        if (!W[o]) return fail(ctx, DCPSE_ERR_INVALID, "null weight pointer");
+    DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    if (n_rows == 0) return DCPSE_OK;
+    const int64_t b = std::min<int64_t>((n_rows + 63 + 255) / 256, 65536);
+    k_sell_fill_bundle16<<<(unsigned)b, 256, 0, ctx->stream>>>(
+        n_rows, row_ptr, col, n_ops, W[0], n_ops > 1 ? W[1] : nullptr, n_ops > 2 ? W[2] : nullptr,
+        n_ops > 3 ? W[3] : nullptr, slice_ptr, sell_row, exc_ptr, n_exc, bundle, sell_row16, exc_col, exc_W);
+    DCPSE_LAUNCH_CHECK(ctx);
+    return DCPSE_OK;
+}
+
+dcpse_status dcpse_apply_sell_bundle16_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr,
+                                           const int64_t* slice_ptr, const int32_t* sell_row16, int32_t n_ops,
+                                           const int32_t* bundle, const int64_t* exc_ptr, const int32_t* exc_col,
+                                           const float* exc_W, int64_t n_exc, const float* f, const float* f_self,
+                                           float* const* out, float alpha, float beta) {
+    if (!ctx) return DCPSE_ERR_INVALID;
+    if (!row_ptr || !slice_ptr || !sell_row16 || !bundle || !exc_ptr || !f || !out || n_rows < 0 || n_ops < 1 ||
+        n_exc < 0 || (n_exc > 0 && (!exc_col || !exc_W)))
+        return fail(ctx, DCPSE_ERR_INVALID, "null pointer or bad size");
+    if (n_ops > 4) return fail(ctx, DCPSE_ERR_UNSUPPORTED, "at most 4 operators per pass");
+    for (int o = 0; o < n_ops; ++o)
+        if (!out[o]) return fail(ctx, DCPSE_ERR_INVALID, "null output pointer");
+    DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    if (n_rows == 0) return DCPSE_OK;
+    SellArgs a{};
+    a.n_rows = n_rows;
+    a.row_ptr = row_ptr;
+    a.slice_ptr = slice_ptr;
+    a.sell_row = sell_row16;
+    a.bundle = bundle;
+    a.exc_ptr = exc_ptr;
+    a.exc_col = exc_col;
+    a.exc_W = exc_W;
+    a.n_exc = n_exc;
+    for (int o = 0; o < 4; ++o) a.out[o] = o < n_ops ? out[o] : nullptr;
+    a.f = f;
+    a.f_self = f_self ? f_self : f;
+    a.alpha = alpha;
+    a.beta = beta;
+    if (!dispatch_sell16(a, n_ops, ctx->sell_groups == 4 ? 4 : 2, ctx->stream))
+        return fail(ctx, DCPSE_ERR_UNSUPPORTED, "no SELL kernel for %d operators", n_ops);
+    DCPSE_LAUNCH_CHECK(ctx);
+    return DCPSE_OK;
+}
+
 dcpse_status dcpse_stage_sell_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int64_t* slice_ptr,
                                   const int32_t* sell_row, const int32_t* sell_col, const float* sell_W,
                                   const float* u, float alpha, const float* y_a, float ca, float* out_a,
@@ -490,12 +802,13 @@ dcpse_status dcpse_stage_sell_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t*
     if (out_a == u || out_b == u) return fail(ctx, DCPSE_ERR_INVALID, "a stage output may not alias its input u");
     DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     if (n_rows == 0 || (!out_a && !out_b)) return DCPSE_OK;
-    SellArgs a;
+    SellArgs a{};
     a.n_rows = n_rows;
     a.row_ptr = row_ptr;
     a.slice_ptr = slice_ptr;
     a.sell_row = sell_row;
     a.sell_col = sell_col;
+    a.bundle = nullptr;
     for (int o = 0; o < 4; ++o) {
         a.W[o] = o == 0 ? sell_W : nullptr;
         a.out[o] = nullptr;
@@ -521,8 +834,8 @@ dcpse_status dcpse_apply_sell_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t*
                                   const int32_t* sell_row, const int32_t* sell_col, int32_t n_ops,
                                   const float* const* sell_W, const float* f, const float* f_self, float* const* out,
                                   float alpha, float beta) {
-    return sell_apply(ctx, 0, n_rows, row_ptr, slice_ptr, sell_row, sell_col, n_ops, sell_W, f, f_self, nullptr, out,
-                      alpha, beta);
+    return sell_apply(ctx, 0, n_rows, row_ptr, slice_ptr, sell_row, sell_col, nullptr, n_ops, sell_W, f, f_self,
+                      nullptr, out, alpha, beta);
 }
 
 dcpse_status dcpse_apply_transpose_sell_f32(dcpse_ctx* ctx, int64_t n_cols, const int64_t* rowT_ptr,
@@ -533,8 +846,8 @@ dcpse_status dcpse_apply_transpose_sell_f32(dcpse_ctx* ctx, int64_t n_cols, cons
     if (!diag) return fail(ctx, DCPSE_ERR_INVALID, "diag is null");
     const float* Wp[1] = {sell_WT};
     float* Op[1] = {out};
-    return sell_apply(ctx, 1, n_cols, rowT_ptr, slice_ptr, sell_rowT, sell_colT, 1, Wp, g, nullptr, diag, Op, alpha,
-                      beta);
+    return sell_apply(ctx, 1, n_cols, rowT_ptr, slice_ptr, sell_rowT, sell_colT, nullptr, 1, Wp, g, nullptr, diag, Op,
+                      alpha, beta);
 }
 
 dcpse_status dcpse_set_sell_pipeline(dcpse_ctx* ctx, int32_t groups, int32_t depth) {

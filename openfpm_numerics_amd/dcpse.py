@@ -435,6 +435,48 @@ def apply_ops(ops, f: torch.Tensor, outs=None, f_domain: torch.Tensor | None = N
     return [o.reshape(*f.shape[:-1], n_rows) for o in outs]
 
 
+def sell_bundle(ops) -> torch.Tensor:
+    """The interleaved sliced-ELL copy of several FP32 operators sharing one
+    support (columns and every operator's weights per slot group)."""
+    vl = ops[0].verletList
+    slice_ptr, sell_row, sell_col = vl.sell()
+    n_slots = sell_col.numel()
+    Ws = [op.weights(torch.float32) for op in ops]
+    bundle = torch.empty((1 + len(ops)) * n_slots, dtype=torch.int32, device=vl.row_ptr.device)
+    _lib.context(vl.row_ptr.device.index or 0).call(
+        "dcpse_sell_fill_bundle_f32", vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(vl.col), len(ops), _lib.ptr_array(Ws),
+        _lib.ptr(slice_ptr), _lib.ptr(sell_row), _lib.ptr(bundle))
+    return bundle
+
+
+def sell_bundle16(ops):
+    """The interleaved sliced-ELL copy with 16-bit column offsets (operators on
+    one particle set): (bundle, sell_row16, exc_ptr, exc_col, exc_W, n_exc),
+    the far neighbours (offset beyond +-32767 modulo N) listed per lane."""
+    vl = ops[0].verletList
+    if not all(op.same_set for op in ops):
+        raise ValueError("16-bit column offsets need operators whose support and domain sets are the same")
+    slice_ptr, sell_row, sell_col = vl.sell()
+    n_slots = sell_col.numel()
+    dev = vl.row_ptr.device
+    ctx = _lib.context(dev.index or 0)
+    n_lanes = sell_row.numel()
+    exc_ptr = torch.empty(n_lanes + 1, dtype=torch.int64, device=dev)
+    n_exc = np.zeros(1, np.int64)
+    ctx.call("dcpse_sell_bundle16_count", vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(vl.col), _lib.ptr(sell_row),
+             _lib.ptr(exc_ptr), n_exc.ctypes.data)
+    ne = int(n_exc[0])
+    Ws = [op.weights(torch.float32) for op in ops]
+    bundle = torch.empty(max(n_slots // 2 + len(ops) * n_slots, 4), dtype=torch.int32, device=dev)
+    sell_row16 = torch.empty_like(sell_row)
+    exc_col = torch.empty(max(ne, 1), dtype=torch.int32, device=dev)
+    exc_W = torch.empty(max(len(ops) * ne, 1), dtype=torch.float32, device=dev)
+    ctx.call("dcpse_sell_fill_bundle16_f32", vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(vl.col), len(ops),
+             _lib.ptr_array(Ws), _lib.ptr(slice_ptr), _lib.ptr(sell_row), _lib.ptr(exc_ptr), ne, _lib.ptr(bundle),
+             _lib.ptr(sell_row16), _lib.ptr(exc_col), _lib.ptr(exc_W))
+    return bundle, sell_row16, exc_ptr, exc_col, exc_W, ne
+
+
 class PreparedApply:
     """A fused apply with every pointer bound once: calling it costs one C call.
 
@@ -442,7 +484,7 @@ class PreparedApply:
     and keep their storage. ``f`` is (N,) or (B, N); ``outs`` one tensor per op.
     """
 
-    def __init__(self, ops, f: torch.Tensor, outs, alpha=1.0, beta=0.0, sell: bool = True):
+    def __init__(self, ops, f: torch.Tensor, outs, alpha=1.0, beta=0.0, sell=True):
         vl = ops[0].verletList
         if any(op.verletList is not vl for op in ops):
             raise ValueError("fused operators must share one Verlet list")
@@ -452,7 +494,24 @@ class PreparedApply:
         nb, n = f2.shape
         self.ctx = _lib.context(f.device.index or 0)
         self.O = _lib.ptr_array(outs)
-        if sell and f.dtype == torch.float32 and nb == 1:
+        if sell == "bundle16" and f.dtype == torch.float32 and nb == 1:
+            slice_ptr, sell_row, sell_col = vl.sell()
+            b16 = sell_bundle16(ops)
+            self._keep = (ops, f, outs, b16, slice_ptr)
+            bundle, sell_row16, exc_ptr, exc_col, exc_W, ne = b16
+            self.n_exc = ne
+            self.fn = self.ctx.lib.dcpse_apply_sell_bundle16_f32
+            self.args = (self.ctx.h, vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(slice_ptr), _lib.ptr(sell_row16),
+                         len(ops), _lib.ptr(bundle), _lib.ptr(exc_ptr), _lib.ptr(exc_col), _lib.ptr(exc_W), ne,
+                         _lib.ptr(f2), None, self.O, alpha, beta)
+        elif sell == "bundle" and f.dtype == torch.float32 and nb == 1:
+            slice_ptr, sell_row, sell_col = vl.sell()
+            bundle = sell_bundle(ops)
+            self._keep = (ops, f, outs, bundle, slice_ptr, sell_row)
+            self.fn = self.ctx.lib.dcpse_apply_sell_bundle_f32
+            self.args = (self.ctx.h, vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(slice_ptr), _lib.ptr(sell_row),
+                         len(ops), _lib.ptr(bundle), _lib.ptr(f2), None, self.O, alpha, beta)
+        elif sell and f.dtype == torch.float32 and nb == 1:
             slice_ptr, sell_row, sell_col = vl.sell()
             self._keep = (ops, f, outs, [op.sell_weights() for op in ops], slice_ptr, sell_row, sell_col)
             self.W = _lib.ptr_array(self._keep[3])
@@ -465,7 +524,12 @@ class PreparedApply:
             self.W = _lib.ptr_array(self._keep[3])
             self.args = (self.ctx.h, vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(vl.col), len(ops), self.W,
                          _lib.ptr(f2), n, nb, None, n, self.O, vl.n_rows, alpha, beta)
-        self.sell = self.fn == self.ctx.lib.dcpse_apply_sell_f32
+        lib = self.ctx.lib
+        self.layout = next((name for fn, name in ((lib.dcpse_apply_sell_f32, "sell"),
+                                                  (lib.dcpse_apply_sell_bundle_f32, "sell_bundle"),
+                                                  (lib.dcpse_apply_sell_bundle16_f32, "sell_bundle16"))
+                            if self.fn is fn), "csr")
+        self.sell = self.layout != "csr"
         self.ctx.bind_current_stream()
 
     def __call__(self):

@@ -64,14 +64,20 @@ class ParticleSet:
         return t
 
     def reorder(self, order: str = "morton") -> np.ndarray:
-        """Sorts the particles along a space-filling curve (Morton / Z-order over
-        the box), the counterpart of openfpm's vector_dist::reorder. Positions and
-        properties are permuted in place; returns perm with new[i] = old[perm[i]].
-        Verlet lists and operators built before must be rebuilt (map counter)."""
-        if order != "morton":
-            raise ValueError("only 'morton' ordering is implemented")
+        """Sorts the particles, the counterpart of openfpm's vector_dist::reorder:
+        "morton" along the Z-order curve over the box, "cell" row-major over a
+        grid of about N^(1/dim) cells per dimension (neighbours then lie within a
+        few cell rows of each other, which keeps column offsets small). Positions
+        and properties are permuted in place; returns perm with new[i] =
+        old[perm[i]]. Verlet lists and operators built before must be rebuilt
+        (map counter)."""
         pos = self.pos_soa.t().cpu().numpy()
-        perm = morton_order(pos, self.box_lo, self.box_hi)
+        if order == "morton":
+            perm = morton_order(pos, self.box_lo, self.box_hi)
+        elif order == "cell" or order.startswith("cell:"):
+            perm = cell_order(pos, self.box_lo, self.box_hi, int(order[5:]) if order.startswith("cell:") else None)
+        else:
+            raise ValueError("order must be 'morton', 'cell' or 'cell:<cells per dimension>'")
         idx = torch.as_tensor(perm, device=self.device)
         self.pos_soa = self.pos_soa[:, idx].contiguous()
         for k, v in self.props.items():
@@ -87,6 +93,21 @@ class ParticleSet:
         vl.__dict__.update(fresh.__dict__)
         vl._sell = None
         return vl
+
+
+def cell_order(pos: np.ndarray, box_lo, box_hi, cells: int | None = None) -> np.ndarray:
+    """Permutation sorting (N, dim) points row-major (last dimension fastest)
+    by their cell in a grid of `cells` per dimension (default about N^(1/dim))."""
+    pos = np.asarray(pos, dtype=np.float64)
+    n, dim = pos.shape
+    cells = cells or max(1, int(round(n ** (1.0 / dim))))
+    lo = np.asarray(box_lo, dtype=np.float64)
+    hi = np.asarray(box_hi, dtype=np.float64)
+    q = np.clip(np.floor((pos - lo) / (hi - lo) * cells), 0, cells - 1).astype(np.int64)
+    key = np.zeros(n, np.int64)
+    for d in range(dim):
+        key = key * cells + q[:, d]
+    return np.argsort(key, kind="stable")
 
 
 def morton_order(pos: np.ndarray, box_lo, box_hi, bits: int | None = None) -> np.ndarray:

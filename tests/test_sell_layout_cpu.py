@@ -123,3 +123,35 @@ def test_padding_lines_are_not_touched():
     need = lengths.sum() * 4 / 128
     assert fetched <= need * 1.06 + width.size * 2, (fetched, need, stored)
     assert stored > need * 1.08  # the layout itself is padded
+
+
+@pytest.mark.parametrize("n_ops", [1, 2, 4])
+def test_bundle16_addressing(n_ops):
+    """16-bit bundle: every word written once, every read inside the slice's blocks."""
+    rng = np.random.default_rng(3)
+    lengths = rng.integers(20, 60, 300)
+    L, width, slice_ptr, _ = layout(lengths)
+    blk = 128 + 256 * n_ops
+    total = slice_ptr[-1] // 256 * blk
+    assert total * 2 == slice_ptr[-1] + 2 * n_ops * slice_ptr[-1]  # (0.5 + n_ops) * n_slots words
+    hits16 = np.zeros(2 * total, np.int64)  # int16 units of the column part
+    hitsw = np.zeros(total, np.int64)
+    for i in range(L.size):
+        s, l = divmod(i, 64)
+        for j in range(width[s]):
+            gb = (slice_ptr[s] // 256 + (j >> 2)) * blk
+            hits16[2 * gb + 4 * l + (j & 3)] += 1
+            for o in range(n_ops):
+                hitsw[gb + 128 + o * 256 + 4 * l + (j & 3)] += 1
+    col_words = hits16.reshape(-1, 2).sum(axis=1)
+    assert np.all((col_words == 2) | (hitsw == 1))
+    assert np.all(col_words + 2 * hitsw == 2)  # each word is either 2 offsets or 1 weight
+    for s in range(width.size):
+        lo_w, hi_w = slice_ptr[s] // 256 * blk, slice_ptr[s + 1] // 256 * blk
+        for lane in range(64):
+            for g in range(width[s] // 4):
+                cw = lo_w + g * blk + 2 * lane  # this lane's 8 bytes of offsets
+                assert lo_w <= cw and cw + 1 < hi_w
+                for o in range(n_ops):
+                    ww = lo_w + g * blk + 128 + o * 256 + 4 * lane
+                    assert ww + 3 < hi_w
