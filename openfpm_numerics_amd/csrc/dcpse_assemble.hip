@@ -74,6 +74,9 @@ __global__ __launch_bounds__(64) void k_assemble(const AsmArgs a) {
     static_assert(M <= kWave, "basis larger than a wavefront");
     __shared__ double tab[TAB];
     __shared__ double mom[MG];
+    __shared__ double sidg[M];  // 1 / L_kk (uniform across the wave)
+    __shared__ double sx[M];    // solution a (uniform across the wave)
+    __shared__ double sdx[M];   // refinement correction
     constexpr BasisTables<DIM, L, AMIN> E{};
     const int lane = threadIdx.x;
 
@@ -119,6 +122,9 @@ __global__ __launch_bounds__(64) void k_assemble(const AsmArgs a) {
         };
 
         // ---- eps -------------------------------------------------------------
+        // lane j keeps the displacement of neighbour j (j < 64): the moment and
+        // kernel phases reuse it instead of gathering the positions again
+        double cdv[3] = {0.0, 0.0, 0.0};
         double l1 = 0.0;
         for (int j = lane; j < n; j += kWave) {
             double dv[3];
@@ -127,6 +133,10 @@ __global__ __launch_bounds__(64) void k_assemble(const AsmArgs a) {
 #pragma unroll
             for (int d = 0; d < DIM; ++d) s += fabs(dv[d]);
             l1 += s;
+            if (j == lane) {
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) cdv[d] = dv[d];
+            }
         }
         l1 = wave_sum(l1);
         const double eps = n > 0 ? (l1 / n) / a.h_over_eps : 0.0;
@@ -147,6 +157,8 @@ __global__ __launch_bounds__(64) void k_assemble(const AsmArgs a) {
             continue;
         }
 
+        const double ieps = 1.0 / eps;
+
         // ---- moments -----------------------------------------------------------
         double acc[ROUNDS];
 #pragma unroll
@@ -154,12 +166,14 @@ __global__ __launch_bounds__(64) void k_assemble(const AsmArgs a) {
         for (int c0 = 0; c0 < n; c0 += kChunk) {
             const int cnt = min(kChunk, n - c0);
             const int jl = lane & (kChunk - 1);
+            double dv[3];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) dv[d] = __shfl(cdv[d], (c0 + jl) & (kWave - 1), kWave);
             if (jl < cnt) {
-                double dv[3];
-                disp(c0 + jl, dv);
+                if (c0 + jl >= kWave) disp(c0 + jl, dv);
                 double z[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) z[d] = dv[d] / eps;
+                for (int d = 0; d < DIM; ++d) z[d] = dv[d] * ieps;
                 double* row = tab + jl * TROW;
                 if (lane < kChunk) {
 #pragma unroll
@@ -203,15 +217,21 @@ __global__ __launch_bounds__(64) void k_assemble(const AsmArgs a) {
         wave_lds_sync();
 
         // ---- A rows, Cholesky (lane i = row i) ------------------------------------
+        // `me` is the lane index made opaque per particle, so the lane == k
+        // tests below are recomputed (one compare) instead of being hoisted out
+        // of the particle loop as dozens of lane masks spilled to VGPR lanes.
+        int me = lane;
+        asm volatile("" : "+v"(me));
         double ar[M];
 #pragma unroll
         for (int k = 0; k < M; ++k) {
             const int idx = (ai0 + E.e[k][0]) + GW * (ai1 + E.e[k][1]) + GW * GW * (ai2 + E.e[k][2]);
             const double v = mom[idx];
-            ar[k] = lane < M ? v : 0.0;
+            ar[k] = me < M ? v : 0.0;
         }
+        // right-looking: step k scales column k and updates the trailing
+        // columns with its entries broadcast by v_readlane
         bool ok = true;
-        double idg[M];
 #pragma unroll
         for (int k = 0; k < M; ++k) {
             double dkk = readlane_d(ar[k], k);
@@ -221,65 +241,69 @@ __global__ __launch_bounds__(64) void k_assemble(const AsmArgs a) {
             }
             const double sk = sqrt(dkk);
             const double inv = 1.0 / sk;
-            idg[k] = inv;
+            if (me == 0) sidg[k] = inv;
             const double lik = ar[k] * inv;
-            ar[k] = (lane == k) ? sk : lik;
+            ar[k] = (me == k) ? sk : lik;
 #pragma unroll
             for (int j = k + 1; j < M; ++j) {
                 const double ljk = readlane_d(lik, j);
                 ar[j] = fma(-lik, ljk, ar[j]);
             }
+            // one step at a time: keeps the broadcasts of a step from piling up
+            // in scalar registers across steps
+            __builtin_amdgcn_sched_barrier(0);
         }
         // L rows to LDS (for column access in the backward solve)
         double* lmat = tab;
 #pragma unroll
         for (int k = 0; k < M; ++k)
-            if (k <= lane && lane < M) lmat[lane * M + k] = ar[k];
+            if (k <= me && me < M) lmat[me * M + k] = ar[k];
         wave_lds_sync();
 
+        // the uniform solution vectors live in LDS (x[j] written by lane 0),
+        // which keeps the register file to the factor rows
         auto chol_solve = [&](double rhs, double* x) {
             double r = rhs, t = 0.0;
 #pragma unroll
             for (int k = 0; k < M; ++k) {
-                const double yk = readlane_d(r, k) * idg[k];
-                if (lane == k) t = yk;
+                const double yk = readlane_d(r, k) * sidg[k];
+                if (me == k) t = yk;
                 r = fma(-ar[k], yk, r);
             }
 #pragma unroll
             for (int j = M - 1; j >= 0; --j) {
-                const double xj = readlane_d(t, j) * idg[j];
-                x[j] = xj;
-                const double ljk = lane < j ? lmat[j * M + lane] : 0.0;
+                const double xj = readlane_d(t, j) * sidg[j];
+                if (me == 0) x[j] = xj;
+                const double ljk = me < j ? lmat[j * M + me] : 0.0;
                 t = fma(-ljk, xj, t);
             }
+            wave_lds_sync();
         };
-        double av[M];
-        chol_solve(b_lane, av);
+        chol_solve(b_lane, sx);
         if (a.refine) {
             double rr = b_lane;
 #pragma unroll
             for (int k = 0; k < M; ++k) {
                 const int idx = (ai0 + E.e[k][0]) + GW * (ai1 + E.e[k][1]) + GW * GW * (ai2 + E.e[k][2]);
-                const double aik = lane < M ? mom[idx] : 0.0;
-                rr = fma(-aik, av[k], rr);
+                const double aik = me < M ? mom[idx] : 0.0;
+                rr = fma(-aik, sx[k], rr);
             }
-            double dx[M];
-            chol_solve(rr, dx);
-#pragma unroll
-            for (int k = 0; k < M; ++k) av[k] += dx[k];
+            chol_solve(rr, sdx);
+            if (me < M) sx[me] += sdx[me];
+            wave_lds_sync();
         }
 
         // ---- kernel weights ----------------------------------------------------
         const double eip = 1.0 / ipow_rt(eps, a.border);
         bool bad = false;
         for (int j = lane; j < n; j += kWave) {
-            double dv[3];
-            disp(j, dv);
+            double dv[3] = {cdv[0], cdv[1], cdv[2]};
+            if (j >= kWave) disp(j, dv);
             double z[3] = {0.0, 0.0, 0.0};
             double z2 = 0.0;
 #pragma unroll
             for (int d = 0; d < DIM; ++d) {
-                z[d] = dv[d] / eps;
+                z[d] = dv[d] * ieps;
                 z2 += z[d] * z[d];
             }
             const double w = exp(-z2);
@@ -296,7 +320,7 @@ __global__ __launch_bounds__(64) void k_assemble(const AsmArgs a) {
             for (int k = 0; k < M; ++k) {
                 double v = px[E.e[k][0]] * py[E.e[k][1]];
                 if (DIM == 3) v *= pz[E.e[k][2]];
-                s = fma(av[k], v, s);
+                s = fma(sx[k], v, s);
             }
             const double K = ok ? s * w : 0.0;
             bad |= !isfinite(K);
