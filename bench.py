@@ -330,10 +330,11 @@ def main():
         "fp64_error_vs_analytic": ana,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": ("k_apply_sell%s<2,%d,0>%s" % ("3" if (sell_pipe or (2, 2))[1] == 3 else "", (sell_pipe or (2, 2))[0],
-                                                        {"sell_bundle": " (interleaved bundle)",
-                                                         "sell_bundle16": " (interleaved bundle, 16-bit columns)"}
-                                                        .get(step.layout, "")))
+                     "kernel": ("k_apply_sell%s<2,%d,0%s>%s" % ("3" if (sell_pipe or (2, 2))[1] == 3 else "", (sell_pipe or (2, 2))[0],
+                                                          ",true" if step.layout == "sell_bundle16" else "",
+                                                          {"sell_bundle": " (interleaved bundle)",
+                                                           "sell_bundle16": " (interleaved bundle, 16-bit columns)"}
+                                                          .get(step.layout, "")))
                      if step.sell else "k_apply_v4<G,NCH,2,1,diff>", "alg_bytes_per_launch": alg_bytes, "layout_min_bytes_per_launch": layout_bytes,
                      "launch_ms": launch_ms},
         "roofline_assembly": {"bound": "fp64", "achieved": asm_tflops, "peak": FP64_PEAK_TFLOPS,
