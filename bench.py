@@ -6,8 +6,9 @@ removed at random (seed 3), periodic unit cube, rCut = 2.4h -> about 1.89M
 particles with 35..60 neighbours. Convergence order 2.
 
   setup     neighbour CSR on the GPU (support_ms), FP64 assembly of the
-            Laplacian (one operator, 19 monomials) and Dx (10 monomials)
-            (assembly_ms, HIP events).
+            Laplacian (one operator, 19 monomials) and Dx (10 monomials) in
+            one pass sharing gathers, epsilon and moments (assembly_ms, HIP
+            events; each operator alone in assembly_ms_by_operator).
   step      one fused FP32 pass applying Laplacian and Dx to
             f = sin(2 pi x) sin(2 pi y) sin(2 pi z) (inputs resident in HBM).
   value     M particle-op evals/s = particles * 2 operators / step time.
@@ -97,7 +98,7 @@ def main():
 
     from openfpm_numerics_amd import Derivative_x, Laplacian, ParticleSet, workloads as W
     from openfpm_numerics_amd.autotune import tune_apply, tune_assembly, tune_sell
-    from openfpm_numerics_amd.dcpse import Dcpse, PreparedApply
+    from openfpm_numerics_amd.dcpse import Dcpse, PreparedApply, assemble_together
 
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no GPU visible"}))
@@ -140,9 +141,8 @@ def main():
     dx = Derivative_x(ps, vl, 2, rcut)
     torch.cuda.synchronize()
 
-    def assemble_both():
-        lap.dcpse.initializeStaticSize(ps, ps, check=False)
-        dx.dcpse.initializeStaticSize(ps, ps, check=False)
+    def assemble_both():  # one pass for the pair (shared gathers, epsilon and moments)
+        assemble_together([lap.dcpse, dx.dcpse], ps, ps, check=False)
 
     if args.tune:
         (ppw, _), asm_tune = tune_assembly(assemble_both)

@@ -140,6 +140,23 @@ dcpse_status dcpse_assemble(dcpse_ctx* ctx, int32_t dim,
                             uint32_t order, double h_over_eps, uint32_t flags,
                             double* eps, double* eps_inv_pow, double* K64,
                             float* W32, double* W64, int32_t* info);
+/* n_ops operators on one support in one call: operator o has n_sig[o]
+ * signatures (betas/coeffs concatenated over operators) and its own output
+ * arrays (eps[o], ... info[o]; any array pointer or entry may be NULL).
+ * Operators whose bases pair up at order 2 (a second derivative or the
+ * Laplacian with a first derivative, two operators of one basis) are
+ * assembled two per kernel pass, sharing the neighbour gathers, epsilon and
+ * the moments - the reference builds each Dcpse separately
+ * (DCPSE_op.hpp:649-786, 1350-1487). Results equal separate dcpse_assemble calls. */
+dcpse_status dcpse_assemble_multi(dcpse_ctx* ctx, int32_t dim,
+                                  const double* pos_sup, int64_t n_sup,
+                                  const double* pos_dom, int64_t n_dom,
+                                  const double* box_lo, const double* box_hi, uint32_t periodic_mask,
+                                  const int64_t* row_ptr, const int32_t* col, int32_t n_ops,
+                                  const int32_t* n_sig, const uint32_t* betas, const double* coeffs,
+                                  uint32_t order, double h_over_eps, uint32_t flags,
+                                  double* const* eps, double* const* eps_inv_pow, double* const* K64,
+                                  float* const* W32, double* const* W64, int32_t* const* info);
 
 /* ---- operator application --------------------------------------------- */
 /* For n_ops operators sharing one support, each with pre-scaled weights

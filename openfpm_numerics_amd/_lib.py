@@ -44,6 +44,8 @@ SIGNATURES = {
     "dcpse_support_fill": (_i32, [_vp, _i32, _vp, _i64, _vp, _i64, _vp, _i32, _vp, _vp, _u32, _vp, _vp]),
     "dcpse_assemble": (_i32, [_vp, _i32, _vp, _i64, _vp, _i64, _vp, _vp, _u32, _vp, _vp,
                               _i32, _vp, _vp, _u32, _f64, _u32, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "dcpse_assemble_multi": (_i32, [_vp, _i32, _vp, _i64, _vp, _i64, _vp, _vp, _u32, _vp, _vp, _i32, _vp, _vp, _vp,
+                                    _u32, _f64, _u32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "dcpse_apply_f32": (_i32, [_vp, _i64, _vp, _vp, _i32, _vp, _vp, _i64, _i32, _vp, _i64, _vp, _i64,
                                _f32, _f32]),
     "dcpse_apply_f64": (_i32, [_vp, _i64, _vp, _vp, _i32, _vp, _vp, _i64, _i32, _vp, _i64, _vp, _i64,

@@ -26,6 +26,19 @@
 
 namespace dcpse {
 
+// One operator of a pass: its right-hand side (summed over its signatures)
+// and outputs.
+struct AsmOp {
+    double b[64];
+    unsigned border;  // |beta|
+    double* eps;
+    double* eps_inv_pow;
+    double* K64;
+    float* W32;
+    double* W64;
+    int32_t* info;
+};
+
 struct AsmArgs {
     const double* pos_sup;
     int64_t n_sup;
@@ -35,23 +48,47 @@ struct AsmArgs {
     const int32_t* col;
     double len[3];
     unsigned periodic;
-    unsigned border;
     double h_over_eps;
     int refine;
     int ppb;
-    double b[64];
-    double* eps;
-    double* eps_inv_pow;
-    double* K64;
-    float* W32;
-    double* W64;
-    int32_t* info;
+    AsmOp op[2];  // op[1] used by the two-operator kernels
 };
 
 constexpr int kChunk = 32;  // neighbours staged per LDS round
 
 template <int DIM, int L, int AMIN>
 __constant__ BasisTables<DIM, L, AMIN> kTab = BasisTables<DIM, L, AMIN>();
+
+// Moment exponents gamma with QMIN <= |gamma| <= P (dimension 0 fastest).
+template <int DIM, int P, int QMIN>
+struct MomentTable {
+    static constexpr int GW = P + 1;
+    static constexpr int count() {
+        int c = 0;
+        for (int lin = 0; lin < ipow_c(GW, DIM); ++lin) {
+            int s = 0, r = lin;
+            for (int d = 0; d < DIM; ++d) { s += r % GW; r /= GW; }
+            if (s >= QMIN && s <= P) ++c;
+        }
+        return c;
+    }
+    static constexpr int G = count();
+    int g[G][3];
+    constexpr MomentTable() : g{} {
+        int c = 0;
+        for (int lin = 0; lin < ipow_c(GW, DIM); ++lin) {
+            int k[3] = {0, 0, 0}, s = 0, r = lin;
+            for (int d = 0; d < DIM; ++d) { k[d] = r % GW; r /= GW; s += k[d]; }
+            if (s >= QMIN && s <= P) {
+                g[c][0] = k[0]; g[c][1] = k[1]; g[c][2] = k[2];
+                ++c;
+            }
+        }
+    }
+};
+
+template <int DIM, int P, int QMIN>
+__constant__ MomentTable<DIM, P, QMIN> kMom = MomentTable<DIM, P, QMIN>();
 
 __device__ __forceinline__ double ipow_rt(double x, unsigned e) {
     double r = 1.0;
@@ -63,21 +100,186 @@ __device__ __forceinline__ double ipow_rt(double x, unsigned e) {
     return r;
 }
 
-template <int DIM, int L, int AMIN>
-__global__ __launch_bounds__(64) void k_assemble(const AsmArgs a) {
+// Writes the "no kernel" result of one operator for particle p.
+__device__ __forceinline__ void asm_fail(const AsmOp& op, int64_t p, int64_t rs, int n, int status, int lane) {
+    for (int j = lane; j < n; j += kWave) {
+        if (op.K64) op.K64[rs + j] = 0.0;
+        if (op.W64) op.W64[rs + j] = 0.0;
+        if (op.W32) op.W32[rs + j] = 0.0f;
+    }
+    if (lane == 0) {
+        if (op.eps) op.eps[p] = 0.0;
+        if (op.eps_inv_pow) op.eps_inv_pow[p] = 0.0;
+        if (op.info) op.info[p] = status;
+    }
+}
+
+// One operator of basis <DIM, L, AMIN> for particle p, from the moments in
+// LDS (grid width GW): A rows, Cholesky, solve (+ refinement), kernel weights.
+// The lane (`me`, opaque so lane == k tests are not hoisted out of the
+// particle loop as spilled masks) holds row `me` of A in the factorisation.
+template <int DIM, int L, int AMIN, int GW, typename Disp>
+__device__ __forceinline__ void asm_solve(const AsmArgs& a, const AsmOp& op, int64_t p, int64_t rs, int n,
+                                          double eps, double ieps, const double (&cdv)[3], Disp disp,
+                                          const double* mom, double* lmat, double* sidg, double* sx, double* sdx,
+                                          int lane) {
     using S = BasisShape<DIM, L, AMIN>;
-    constexpr int M = S::M, P = S::P, GW = S::GW, MG = S::MG, G = S::G;
+    constexpr int M = S::M;
+    static_assert(M <= kWave, "basis larger than a wavefront");
+    constexpr BasisTables<DIM, L, AMIN> E{};
+    if (n < M) {
+        asm_fail(op, p, rs, n, DCPSE_INFO_TOO_FEW, lane);
+        return;
+    }
+    int me = lane;
+    asm volatile("" : "+v"(me));
+    const int li = me < M ? me : 0;
+    const int ai0 = kTab<DIM, L, AMIN>.e[li][0];
+    const int ai1 = kTab<DIM, L, AMIN>.e[li][1];
+    const int ai2 = DIM == 3 ? kTab<DIM, L, AMIN>.e[li][2] : 0;
+    double b_lane = 0.0;
+#pragma unroll
+    for (int k = 0; k < M; ++k)
+        if (me == k) b_lane = op.b[k];
+
+    double ar[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const int idx = (ai0 + E.e[k][0]) + GW * (ai1 + E.e[k][1]) + GW * GW * (ai2 + E.e[k][2]);
+        const double v = mom[idx];
+        ar[k] = me < M ? v : 0.0;
+    }
+    // right-looking: step k scales column k and updates the trailing columns
+    // with its entries broadcast by v_readlane
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        double dkk = readlane_d(ar[k], k);
+        if (!(dkk > 0.0) || !isfinite(dkk)) {
+            ok = false;
+            dkk = 1.0;
+        }
+        const double sk = sqrt(dkk);
+        const double inv = 1.0 / sk;
+        if (me == 0) sidg[k] = inv;
+        const double lik = ar[k] * inv;
+        ar[k] = (me == k) ? sk : lik;
+#pragma unroll
+        for (int j = k + 1; j < M; ++j) {
+            const double ljk = readlane_d(lik, j);
+            ar[j] = fma(-lik, ljk, ar[j]);
+        }
+        // one step at a time: keeps the broadcasts of a step from piling up in
+        // scalar registers across steps
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    // L rows to LDS (for column access in the backward solve)
+#pragma unroll
+    for (int k = 0; k < M; ++k)
+        if (k <= me && me < M) lmat[me * M + k] = ar[k];
+    wave_lds_sync();
+
+    // the uniform solution vectors live in LDS (x[j] written by lane 0), which
+    // keeps the register file to the factor rows
+    auto chol_solve = [&](double rhs, double* x) {
+        double r = rhs, t = 0.0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const double yk = readlane_d(r, k) * sidg[k];
+            if (me == k) t = yk;
+            r = fma(-ar[k], yk, r);
+        }
+#pragma unroll
+        for (int j = M - 1; j >= 0; --j) {
+            const double xj = readlane_d(t, j) * sidg[j];
+            if (me == 0) x[j] = xj;
+            const double ljk = me < j ? lmat[j * M + me] : 0.0;
+            t = fma(-ljk, xj, t);
+        }
+        wave_lds_sync();
+    };
+    chol_solve(b_lane, sx);
+    if (a.refine) {
+        double rr = b_lane;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const int idx = (ai0 + E.e[k][0]) + GW * (ai1 + E.e[k][1]) + GW * GW * (ai2 + E.e[k][2]);
+            const double aik = me < M ? mom[idx] : 0.0;
+            rr = fma(-aik, sx[k], rr);
+        }
+        chol_solve(rr, sdx);
+        if (me < M) sx[me] += sdx[me];
+        wave_lds_sync();
+    }
+
+    // ---- kernel weights ----------------------------------------------------
+    const double eip = 1.0 / ipow_rt(eps, op.border);
+    bool bad = false;
+    for (int j = lane; j < n; j += kWave) {
+        double dv[3] = {cdv[0], cdv[1], cdv[2]};
+        if (j >= kWave) disp(j, dv);
+        double z[3] = {0.0, 0.0, 0.0};
+        double z2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            z[d] = dv[d] * ieps;
+            z2 += z[d] * z[d];
+        }
+        const double w = exp(-z2);
+        double px[L], py[L], pz[L];
+        px[0] = py[0] = pz[0] = 1.0;
+#pragma unroll
+        for (int k = 1; k < L; ++k) {
+            px[k] = px[k - 1] * z[0];
+            py[k] = py[k - 1] * z[1];
+            pz[k] = pz[k - 1] * z[2];
+        }
+        double sum = 0.0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            double v = px[E.e[k][0]] * py[E.e[k][1]];
+            if (DIM == 3) v *= pz[E.e[k][2]];
+            sum = fma(sx[k], v, sum);
+        }
+        const double K = ok ? sum * w : 0.0;
+        bad |= !isfinite(K);
+        if (op.K64) op.K64[rs + j] = K;
+        if (op.W64) op.W64[rs + j] = K * eip;
+        if (op.W32) op.W32[rs + j] = (float)(K * eip);
+    }
+    const bool anybad = __ballot(bad) != 0ull;
+    if (lane == 0) {
+        if (op.eps) op.eps[p] = eps;
+        if (op.eps_inv_pow) op.eps_inv_pow[p] = eip;
+        if (op.info) op.info[p] = !ok ? DCPSE_INFO_NOT_SPD : (anybad ? DCPSE_INFO_NONFINITE : DCPSE_INFO_OK);
+    }
+    wave_lds_sync();
+}
+
+// One wavefront per particle; a block of one wavefront walks a.ppb particles.
+// Operator 0 has basis <DIM, L1, A1>; with L2 > 0 a second operator of basis
+// <DIM, L2, A2> on the same support is assembled in the same pass, sharing ε,
+// the neighbour displacements and the moments (the moment set covers both).
+template <int DIM, int L1, int A1, int L2 = 0, int A2 = 0>
+__global__ __launch_bounds__(64) void k_assemble(const AsmArgs a) {
+    constexpr int P1 = 2 * (L1 - 1), P2 = L2 > 0 ? 2 * (L2 - 1) : 0;
+    constexpr int P = P1 > P2 ? P1 : P2;
+    constexpr int QMIN = L2 > 0 ? (A1 < A2 ? 2 * A1 : 2 * A2) : 2 * A1;
+    using MT = MomentTable<DIM, P, QMIN>;
+    constexpr int G = MT::G, GW = MT::GW, MG = ipow_c(GW, DIM);
+    constexpr int M1 = BasisShape<DIM, L1, A1>::M;
+    constexpr int M2 = L2 > 0 ? BasisShape<DIM, (L2 > 0 ? L2 : 2), A2>::M : 0;
+    constexpr int MX = M1 > M2 ? M1 : M2;
+    constexpr int MN = L2 > 0 ? (M1 < M2 ? M1 : M2) : M1;
     constexpr int ROUNDS = (G + kWave - 1) / kWave;
     constexpr int PW = P + 1;
     constexpr int TROW = DIM * PW;
-    constexpr int TAB = (kChunk * TROW > M * M) ? kChunk * TROW : M * M;
-    static_assert(M <= kWave, "basis larger than a wavefront");
+    constexpr int TAB = (kChunk * TROW > MX * MX) ? kChunk * TROW : MX * MX;
     __shared__ double tab[TAB];
     __shared__ double mom[MG];
-    __shared__ double sidg[M];  // 1 / L_kk (uniform across the wave)
-    __shared__ double sx[M];    // solution a (uniform across the wave)
-    __shared__ double sdx[M];   // refinement correction
-    constexpr BasisTables<DIM, L, AMIN> E{};
+    __shared__ double sidg[MX];  // 1 / L_kk (uniform across the wave)
+    __shared__ double sx[MX];    // solution a (uniform across the wave)
+    __shared__ double sdx[MX];   // refinement correction
     const int lane = threadIdx.x;
 
     // moments owned by this lane: table offsets and moment-grid index
@@ -88,23 +290,14 @@ __global__ __launch_bounds__(64) void k_assemble(const AsmArgs a) {
         const int g = lane + kWave * r;
         gv[r] = g < G;
         const int gg = gv[r] ? g : 0;
-        const int e0 = kTab<DIM, L, AMIN>.g[gg][0];
-        const int e1 = kTab<DIM, L, AMIN>.g[gg][1];
-        const int e2 = DIM == 3 ? kTab<DIM, L, AMIN>.g[gg][2] : 0;
+        const int e0 = kMom<DIM, P, QMIN>.g[gg][0];
+        const int e1 = kMom<DIM, P, QMIN>.g[gg][1];
+        const int e2 = DIM == 3 ? kMom<DIM, P, QMIN>.g[gg][2] : 0;
         o0[r] = e0;
         o1[r] = PW + e1;
         o2[r] = 2 * PW + e2;
         gi[r] = e0 + GW * e1 + GW * GW * e2;
     }
-    // basis row owned by this lane in the Cholesky phase, and its RHS entry
-    const int li = lane < M ? lane : 0;
-    const int ai0 = kTab<DIM, L, AMIN>.e[li][0];
-    const int ai1 = kTab<DIM, L, AMIN>.e[li][1];
-    const int ai2 = DIM == 3 ? kTab<DIM, L, AMIN>.e[li][2] : 0;
-    double b_lane = 0.0;
-#pragma unroll
-    for (int k = 0; k < M; ++k)
-        if (lane == k) b_lane = a.b[k];
 
     const int64_t pbeg = (int64_t)blockIdx.x * a.ppb;
     const int64_t pend = min(pbeg + (int64_t)a.ppb, a.n_dom);
@@ -140,23 +333,13 @@ __global__ __launch_bounds__(64) void k_assemble(const AsmArgs a) {
         }
         l1 = wave_sum(l1);
         const double eps = n > 0 ? (l1 / n) / a.h_over_eps : 0.0;
-        int status = DCPSE_INFO_OK;
-        if (n < M) status = DCPSE_INFO_TOO_FEW;
-        else if (!(eps > 0.0) || !isfinite(eps)) status = DCPSE_INFO_EPS_ZERO;
-        if (status != DCPSE_INFO_OK) {
-            for (int j = lane; j < n; j += kWave) {
-                if (a.K64) a.K64[rs + j] = 0.0;
-                if (a.W64) a.W64[rs + j] = 0.0;
-                if (a.W32) a.W32[rs + j] = 0.0f;
-            }
-            if (lane == 0) {
-                if (a.eps) a.eps[p] = 0.0;
-                if (a.eps_inv_pow) a.eps_inv_pow[p] = 0.0;
-                if (a.info) a.info[p] = status;
-            }
+        const bool eps_bad = !(eps > 0.0) || !isfinite(eps);
+        if (n < MN || eps_bad) {  // no operator of the pass can be assembled
+            const int st0 = n < M1 ? DCPSE_INFO_TOO_FEW : DCPSE_INFO_EPS_ZERO;
+            asm_fail(a.op[0], p, rs, n, st0, lane);
+            if (L2 > 0) asm_fail(a.op[1], p, rs, n, n < M2 ? DCPSE_INFO_TOO_FEW : DCPSE_INFO_EPS_ZERO, lane);
             continue;
         }
-
         const double ieps = 1.0 / eps;
 
         // ---- moments -----------------------------------------------------------
@@ -216,125 +399,10 @@ __global__ __launch_bounds__(64) void k_assemble(const AsmArgs a) {
             if (gv[r]) mom[gi[r]] = acc[r];
         wave_lds_sync();
 
-        // ---- A rows, Cholesky (lane i = row i) ------------------------------------
-        // `me` is the lane index made opaque per particle, so the lane == k
-        // tests below are recomputed (one compare) instead of being hoisted out
-        // of the particle loop as dozens of lane masks spilled to VGPR lanes.
-        int me = lane;
-        asm volatile("" : "+v"(me));
-        double ar[M];
-#pragma unroll
-        for (int k = 0; k < M; ++k) {
-            const int idx = (ai0 + E.e[k][0]) + GW * (ai1 + E.e[k][1]) + GW * GW * (ai2 + E.e[k][2]);
-            const double v = mom[idx];
-            ar[k] = me < M ? v : 0.0;
-        }
-        // right-looking: step k scales column k and updates the trailing
-        // columns with its entries broadcast by v_readlane
-        bool ok = true;
-#pragma unroll
-        for (int k = 0; k < M; ++k) {
-            double dkk = readlane_d(ar[k], k);
-            if (!(dkk > 0.0) || !isfinite(dkk)) {
-                ok = false;
-                dkk = 1.0;
-            }
-            const double sk = sqrt(dkk);
-            const double inv = 1.0 / sk;
-            if (me == 0) sidg[k] = inv;
-            const double lik = ar[k] * inv;
-            ar[k] = (me == k) ? sk : lik;
-#pragma unroll
-            for (int j = k + 1; j < M; ++j) {
-                const double ljk = readlane_d(lik, j);
-                ar[j] = fma(-lik, ljk, ar[j]);
-            }
-            // one step at a time: keeps the broadcasts of a step from piling up
-            // in scalar registers across steps
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        // L rows to LDS (for column access in the backward solve)
-        double* lmat = tab;
-#pragma unroll
-        for (int k = 0; k < M; ++k)
-            if (k <= me && me < M) lmat[me * M + k] = ar[k];
-        wave_lds_sync();
-
-        // the uniform solution vectors live in LDS (x[j] written by lane 0),
-        // which keeps the register file to the factor rows
-        auto chol_solve = [&](double rhs, double* x) {
-            double r = rhs, t = 0.0;
-#pragma unroll
-            for (int k = 0; k < M; ++k) {
-                const double yk = readlane_d(r, k) * sidg[k];
-                if (me == k) t = yk;
-                r = fma(-ar[k], yk, r);
-            }
-#pragma unroll
-            for (int j = M - 1; j >= 0; --j) {
-                const double xj = readlane_d(t, j) * sidg[j];
-                if (me == 0) x[j] = xj;
-                const double ljk = me < j ? lmat[j * M + me] : 0.0;
-                t = fma(-ljk, xj, t);
-            }
-            wave_lds_sync();
-        };
-        chol_solve(b_lane, sx);
-        if (a.refine) {
-            double rr = b_lane;
-#pragma unroll
-            for (int k = 0; k < M; ++k) {
-                const int idx = (ai0 + E.e[k][0]) + GW * (ai1 + E.e[k][1]) + GW * GW * (ai2 + E.e[k][2]);
-                const double aik = me < M ? mom[idx] : 0.0;
-                rr = fma(-aik, sx[k], rr);
-            }
-            chol_solve(rr, sdx);
-            if (me < M) sx[me] += sdx[me];
-            wave_lds_sync();
-        }
-
-        // ---- kernel weights ----------------------------------------------------
-        const double eip = 1.0 / ipow_rt(eps, a.border);
-        bool bad = false;
-        for (int j = lane; j < n; j += kWave) {
-            double dv[3] = {cdv[0], cdv[1], cdv[2]};
-            if (j >= kWave) disp(j, dv);
-            double z[3] = {0.0, 0.0, 0.0};
-            double z2 = 0.0;
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) {
-                z[d] = dv[d] * ieps;
-                z2 += z[d] * z[d];
-            }
-            const double w = exp(-z2);
-            double px[L], py[L], pz[L];
-            px[0] = py[0] = pz[0] = 1.0;
-#pragma unroll
-            for (int k = 1; k < L; ++k) {
-                px[k] = px[k - 1] * z[0];
-                py[k] = py[k - 1] * z[1];
-                pz[k] = pz[k - 1] * z[2];
-            }
-            double s = 0.0;
-#pragma unroll
-            for (int k = 0; k < M; ++k) {
-                double v = px[E.e[k][0]] * py[E.e[k][1]];
-                if (DIM == 3) v *= pz[E.e[k][2]];
-                s = fma(sx[k], v, s);
-            }
-            const double K = ok ? s * w : 0.0;
-            bad |= !isfinite(K);
-            if (a.K64) a.K64[rs + j] = K;
-            if (a.W64) a.W64[rs + j] = K * eip;
-            if (a.W32) a.W32[rs + j] = (float)(K * eip);
-        }
-        const bool anybad = __ballot(bad) != 0ull;
-        if (lane == 0) {
-            if (a.eps) a.eps[p] = eps;
-            if (a.eps_inv_pow) a.eps_inv_pow[p] = eip;
-            if (a.info) a.info[p] = !ok ? DCPSE_INFO_NOT_SPD : (anybad ? DCPSE_INFO_NONFINITE : DCPSE_INFO_OK);
-        }
-        wave_lds_sync();
+        // ---- per operator: factorisation, solve, kernel weights ------------------
+        asm_solve<DIM, L1, A1, GW>(a, a.op[0], p, rs, n, eps, ieps, cdv, disp, mom, tab, sidg, sx, sdx, lane);
+        if constexpr (L2 > 0)
+            asm_solve<DIM, L2, A2, GW>(a, a.op[1], p, rs, n, eps, ieps, cdv, disp, mom, tab, sidg, sx, sdx, lane);
     }
 }
 
@@ -389,18 +457,20 @@ static void host_rhs(const HostBasis& B, const uint32_t* beta, double* b) {
     }
 }
 
-typedef void (*AsmKernel)(const AsmArgs);
-
-template <int DIM, int L, int AMIN>
+template <int DIM, int L1, int A1, int L2, int A2>
 static void launch_asm(const AsmArgs& args, int64_t blocks, hipStream_t s) {
-    k_assemble<DIM, L, AMIN><<<(unsigned)blocks, kWave, 0, s>>>(args);
+    k_assemble<DIM, L1, A1, L2, A2><<<(unsigned)blocks, kWave, 0, s>>>(args);
 }
 
 typedef void (*AsmLauncher)(const AsmArgs&, int64_t, hipStream_t);
 
-static AsmLauncher pick_launcher(int dim, int L, int amin) {
+// Single-operator kernels for every basis the library supports, and
+// two-operator kernels for the pairs assembled together most often
+// (second derivatives or the Laplacian with first derivatives, and two
+// operators of one basis), at convergence order 2.
+static AsmLauncher pick_launcher(int dim, int L, int amin, int L2 = 0, int amin2 = 0) {
 #define DCPSE_ASM_CASE(D, LL, A) \
-    if (dim == D && L == LL && amin == A) return &launch_asm<D, LL, A>;
+    if (dim == D && L == LL && amin == A && L2 == 0) return &launch_asm<D, LL, A, 0, 0>;
     DCPSE_ASM_CASE(2, 2, 0) DCPSE_ASM_CASE(2, 2, 1)
     DCPSE_ASM_CASE(2, 3, 0) DCPSE_ASM_CASE(2, 3, 1)
     DCPSE_ASM_CASE(2, 4, 0) DCPSE_ASM_CASE(2, 4, 1)
@@ -411,7 +481,33 @@ static AsmLauncher pick_launcher(int dim, int L, int amin) {
     DCPSE_ASM_CASE(3, 4, 0) DCPSE_ASM_CASE(3, 4, 1)
     DCPSE_ASM_CASE(3, 5, 0) DCPSE_ASM_CASE(3, 5, 1)
 #undef DCPSE_ASM_CASE
+#define DCPSE_ASM_PAIR(D, LA, AA, LB, AB) \
+    if (dim == D && L == LA && amin == AA && L2 == LB && amin2 == AB) return &launch_asm<D, LA, AA, LB, AB>;
+    DCPSE_ASM_PAIR(2, 4, 1, 3, 0) DCPSE_ASM_PAIR(2, 3, 0, 3, 0) DCPSE_ASM_PAIR(2, 4, 1, 4, 1)
+    DCPSE_ASM_PAIR(3, 4, 1, 3, 0) DCPSE_ASM_PAIR(3, 3, 0, 3, 0) DCPSE_ASM_PAIR(3, 4, 1, 4, 1)
+#undef DCPSE_ASM_PAIR
     return nullptr;
+}
+
+// The basis and summed right-hand side of one operator (n_sig signatures
+// sharing one basis) into op.b; returns false if the signatures do not share it.
+static bool setup_op(int dim, int n_sig, const uint32_t* betas, const double* coeffs, uint32_t order, AsmOp& op,
+                     HostBasis& B0) {
+    B0 = host_basis(dim, betas, order);
+    uint32_t border = 0;
+    for (int d = 0; d < dim; ++d) border += betas[d];
+    std::vector<double> bk(B0.m);
+    for (int k = 0; k < n_sig; ++k) {
+        HostBasis Bk = host_basis(dim, betas + k * dim, order);
+        uint32_t bo = 0;
+        for (int d = 0; d < dim; ++d) bo += betas[k * dim + d];
+        if (Bk.L != B0.L || Bk.amin != B0.amin || bo != border) return false;
+        host_rhs(Bk, betas + k * dim, bk.data());
+        const double c = coeffs ? coeffs[k] : 1.0;
+        for (int i = 0; i < B0.m && i < 64; ++i) op.b[i] += c * bk[i];
+    }
+    op.border = border;
+    return true;
 }
 
 }  // namespace dcpse
@@ -441,17 +537,19 @@ int32_t dcpse_rhs(int32_t dim, const uint32_t* beta, uint32_t order, double* b, 
     return B.m;
 }
 
-dcpse_status dcpse_assemble(dcpse_ctx* ctx, int32_t dim, const double* pos_sup, int64_t n_sup,
-                            const double* pos_dom, int64_t n_dom, const double* box_lo, const double* box_hi,
-                            uint32_t periodic_mask, const int64_t* row_ptr, const int32_t* col, int32_t n_sig,
-                            const uint32_t* betas, const double* coeffs, uint32_t order, double h_over_eps,
-                            uint32_t flags, double* eps, double* eps_inv_pow, double* K64, float* W32,
-                            double* W64, int32_t* info) {
+static dcpse_status assemble_ops(dcpse_ctx* ctx, int32_t dim, const double* pos_sup, int64_t n_sup,
+                                 const double* pos_dom, int64_t n_dom, const double* box_lo, const double* box_hi,
+                                 uint32_t periodic_mask, const int64_t* row_ptr, const int32_t* col, int32_t n_ops,
+                                 const int32_t* n_sig, const uint32_t* betas, const double* coeffs, uint32_t order,
+                                 double h_over_eps, uint32_t flags, double* const* eps, double* const* eps_inv_pow,
+                                 double* const* K64, float* const* W32, double* const* W64, int32_t* const* info) {
     if (!ctx) return DCPSE_ERR_INVALID;
     if (dim != 2 && dim != 3) return fail(ctx, DCPSE_ERR_UNSUPPORTED, "dim must be 2 or 3");
-    if (!pos_sup || !row_ptr || n_sup < 0 || n_dom < 0 || n_sig < 1 || !betas)
+    if (!pos_sup || !row_ptr || n_sup < 0 || n_dom < 0 || n_ops < 1 || !n_sig || !betas)
         return fail(ctx, DCPSE_ERR_INVALID, "null pointer or bad size");
     if (!(h_over_eps > 0.0)) return fail(ctx, DCPSE_ERR_INVALID, "h_over_eps must be > 0");
+    for (int o = 0; o < n_ops; ++o)
+        if (n_sig[o] < 1) return fail(ctx, DCPSE_ERR_INVALID, "operator %d has no signature", o);
     if (pos_dom == nullptr) {
         pos_dom = pos_sup;
         n_dom = n_sup;
@@ -460,27 +558,30 @@ dcpse_status dcpse_assemble(dcpse_ctx* ctx, int32_t dim, const double* pos_sup, 
         return fail(ctx, DCPSE_ERR_INVALID, "periodic boundaries need box_lo/box_hi");
     DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
 
-    HostBasis B0 = host_basis(dim, betas, order);
-    uint32_t border = 0;
-    for (int d = 0; d < dim; ++d) border += betas[d];
+    std::vector<AsmOp> ops(n_ops);
+    std::vector<HostBasis> bases(n_ops);
+    int64_t off_sig = 0, off_c = 0;
+    for (int o = 0; o < n_ops; ++o) {
+        AsmOp& op = ops[o];
+        memset(&op, 0, sizeof(op));
+        if (!setup_op(dim, n_sig[o], betas + off_sig * dim, coeffs ? coeffs + off_c : nullptr, order, op, bases[o]))
+            return fail(ctx, DCPSE_ERR_INVALID, "combined signatures must share |beta| (one basis)");
+        if (bases[o].m > 64)
+            return fail(ctx, DCPSE_ERR_UNSUPPORTED, "basis of %d monomials exceeds 64", bases[o].m);
+        if (!pick_launcher(dim, bases[o].L, bases[o].amin))
+            return fail(ctx, DCPSE_ERR_UNSUPPORTED, "no compiled assembly kernel for dim=%d |beta|+order=%d", dim,
+                        bases[o].L);
+        op.eps = eps ? eps[o] : nullptr;
+        op.eps_inv_pow = eps_inv_pow ? eps_inv_pow[o] : nullptr;
+        op.K64 = K64 ? K64[o] : nullptr;
+        op.W32 = W32 ? W32[o] : nullptr;
+        op.W64 = W64 ? W64[o] : nullptr;
+        op.info = info ? info[o] : nullptr;
+        off_sig += n_sig[o];
+        off_c += n_sig[o];
+    }
     AsmArgs args;
     memset(&args, 0, sizeof(args));
-    std::vector<double> bk(B0.m);
-    for (int k = 0; k < n_sig; ++k) {
-        HostBasis Bk = host_basis(dim, betas + k * dim, order);
-        uint32_t bo = 0;
-        for (int d = 0; d < dim; ++d) bo += betas[k * dim + d];
-        if (Bk.L != B0.L || Bk.amin != B0.amin || bo != border)
-            return fail(ctx, DCPSE_ERR_INVALID, "combined signatures must share |beta| (one basis)");
-        host_rhs(Bk, betas + k * dim, bk.data());
-        const double c = coeffs ? coeffs[k] : 1.0;
-        for (int i = 0; i < B0.m; ++i) args.b[i] += c * bk[i];
-    }
-    if (B0.m > 64) return fail(ctx, DCPSE_ERR_UNSUPPORTED, "basis of %d monomials exceeds 64", B0.m);
-    AsmLauncher launch = pick_launcher(dim, B0.L, B0.amin);
-    if (!launch)
-        return fail(ctx, DCPSE_ERR_UNSUPPORTED, "no compiled assembly kernel for dim=%d |beta|+order=%d", dim,
-                    B0.L);
     args.pos_sup = pos_sup;
     args.n_sup = n_sup;
     args.pos_dom = pos_dom;
@@ -489,21 +590,56 @@ dcpse_status dcpse_assemble(dcpse_ctx* ctx, int32_t dim, const double* pos_sup, 
     args.col = col;
     for (int d = 0; d < 3; ++d) args.len[d] = (d < dim && box_lo && box_hi) ? box_hi[d] - box_lo[d] : 1.0;
     args.periodic = periodic_mask;
-    args.border = border;
     args.h_over_eps = h_over_eps;
     args.refine = (flags & DCPSE_ASM_REFINE) ? 1 : 0;
     args.ppb = std::max(1, ctx->asm_ppb);
-    args.eps = eps;
-    args.eps_inv_pow = eps_inv_pow;
-    args.K64 = K64;
-    args.W32 = W32;
-    args.W64 = W64;
-    args.info = info;
     if (n_dom == 0) return DCPSE_OK;
     const int64_t blocks = (n_dom + args.ppb - 1) / args.ppb;
-    launch(args, blocks, ctx->stream);
-    DCPSE_LAUNCH_CHECK(ctx);
+    // operators in pairs sharing one pass where a two-operator kernel exists
+    // (larger basis first), one at a time otherwise
+    int o = 0;
+    while (o < n_ops) {
+        AsmLauncher pair = nullptr;
+        int first = o, second = o + 1;
+        if (o + 1 < n_ops) {
+            if (bases[o + 1].L > bases[o].L || (bases[o + 1].L == bases[o].L && bases[o + 1].amin > bases[o].amin))
+                std::swap(first, second);
+            pair = pick_launcher(dim, bases[first].L, bases[first].amin, bases[second].L, bases[second].amin);
+        }
+        if (pair) {
+            args.op[0] = ops[first];
+            args.op[1] = ops[second];
+            pair(args, blocks, ctx->stream);
+            o += 2;
+        } else {
+            args.op[0] = ops[o];
+            memset(&args.op[1], 0, sizeof(AsmOp));
+            pick_launcher(dim, bases[o].L, bases[o].amin)(args, blocks, ctx->stream);
+            o += 1;
+        }
+        DCPSE_LAUNCH_CHECK(ctx);
+    }
     return DCPSE_OK;
+}
+
+dcpse_status dcpse_assemble(dcpse_ctx* ctx, int32_t dim, const double* pos_sup, int64_t n_sup,
+                            const double* pos_dom, int64_t n_dom, const double* box_lo, const double* box_hi,
+                            uint32_t periodic_mask, const int64_t* row_ptr, const int32_t* col, int32_t n_sig,
+                            const uint32_t* betas, const double* coeffs, uint32_t order, double h_over_eps,
+                            uint32_t flags, double* eps, double* eps_inv_pow, double* K64, float* W32,
+                            double* W64, int32_t* info) {
+    return assemble_ops(ctx, dim, pos_sup, n_sup, pos_dom, n_dom, box_lo, box_hi, periodic_mask, row_ptr, col, 1,
+                        &n_sig, betas, coeffs, order, h_over_eps, flags, &eps, &eps_inv_pow, &K64, &W32, &W64, &info);
+}
+
+dcpse_status dcpse_assemble_multi(dcpse_ctx* ctx, int32_t dim, const double* pos_sup, int64_t n_sup,
+                                  const double* pos_dom, int64_t n_dom, const double* box_lo, const double* box_hi,
+                                  uint32_t periodic_mask, const int64_t* row_ptr, const int32_t* col, int32_t n_ops,
+                                  const int32_t* n_sig, const uint32_t* betas, const double* coeffs, uint32_t order,
+                                  double h_over_eps, uint32_t flags, double* const* eps, double* const* eps_inv_pow,
+                                  double* const* K64, float* const* W32, double* const* W64, int32_t* const* info) {
+    return assemble_ops(ctx, dim, pos_sup, n_sup, pos_dom, n_dom, box_lo, box_hi, periodic_mask, row_ptr, col,
+                        n_ops, n_sig, betas, coeffs, order, h_over_eps, flags, eps, eps_inv_pow, K64, W32, W64, info);
 }
 
 }  // extern "C"

@@ -109,40 +109,25 @@ class Dcpse:
 
     def initializeStaticSize(self, particlesSupport: ParticleSet, particlesDomain: ParticleSet,
                              check: bool = True):
-        vl = self.verletList
-        sup, dom = particlesSupport, particlesDomain
-        dev = sup.device
-        n_dom = dom.size_local()
-        if vl.n_rows != n_dom:
-            raise ValueError(f"Verlet list has {vl.n_rows} rows, domain has {n_dom} particles")
-        nnz = vl.nnz
-        eps = torch.empty(n_dom, dtype=torch.float64, device=dev)
-        eip = torch.empty(n_dom, dtype=torch.float64, device=dev)
-        K64 = torch.empty(max(nnz, 1), dtype=torch.float64, device=dev)
-        W64 = torch.empty(max(nnz, 1), dtype=torch.float64, device=dev)
-        W32 = torch.empty(max(nnz, 1), dtype=torch.float32, device=dev)
-        info = torch.empty(n_dom, dtype=torch.int32, device=dev)
-        betas = np.ascontiguousarray(np.array(self.signatures, dtype=np.uint32).ravel())
-        coeffs = np.ascontiguousarray(np.array(self.coeffs, dtype=np.float64))
-        lo = np.ascontiguousarray(sup.box_lo)
-        hi = np.ascontiguousarray(sup.box_hi)
-        ctx = _lib.context(dev.index or 0)
-        ctx.call("dcpse_assemble", sup.dims, _lib.ptr(sup.pos_soa), sup.size_local(),
-                 None if self.same_set else _lib.ptr(dom.pos_soa), n_dom,
-                 lo.ctypes.data, hi.ctypes.data, sup.periodic_mask,
-                 _lib.ptr(vl.row_ptr), _lib.ptr(vl.col) if nnz else None,
-                 len(self.signatures), betas.ctypes.data, coeffs.ctypes.data,
-                 self.convergenceOrder, float(self.HOverEpsilon),
-                 _lib.DCPSE_ASM_REFINE if self.refine else 0,
-                 _lib.ptr(eps), _lib.ptr(eip), _lib.ptr(K64), _lib.ptr(W32), _lib.ptr(W64), _lib.ptr(info))
+        assemble_together([self], particlesSupport, particlesDomain, check=check)
+
+    def _outputs(self, n_dom: int, dev):
+        nnz = self.verletList.nnz
+        return (torch.empty(n_dom, dtype=torch.float64, device=dev), torch.empty(n_dom, dtype=torch.float64, device=dev),
+                torch.empty(max(nnz, 1), dtype=torch.float64, device=dev),
+                torch.empty(max(nnz, 1), dtype=torch.float32, device=dev),
+                torch.empty(max(nnz, 1), dtype=torch.float64, device=dev),
+                torch.empty(n_dom, dtype=torch.int32, device=dev))
+
+    def _adopt(self, out, map_ctr):
+        eps, eip, K64, W32, W64, info = out
+        nnz = self.verletList.nnz
         self.localEps, self.localEpsInvPow = eps, eip
         self.calcKernels = K64[:nnz]
         self._W = {torch.float64: W64[:nnz], torch.float32: W32[:nnz]}
         self._T = {}
         self.info = info
-        self.update_ctr = sup.map_ctr
-        if check:
-            self.check_info()
+        self.update_ctr = map_ctr
 
     def check_info(self):
         """Raises like the reference on the first particle whose assembly failed."""
@@ -384,6 +369,48 @@ class Dcpse:
         rp, _ = vl._mirror()
         prop[vl.col[int(rp[p]):int(rp[p + 1])].long()] = 1.0
         return prop
+
+
+def assemble_together(ops, particlesSupport: ParticleSet | None = None, particlesDomain: ParticleSet | None = None,
+                      check: bool = True):
+    """(Re)assembles several operators on one Verlet list in one library call
+    (dcpse_assemble_multi): operators whose bases pair up (the Laplacian or a
+    second derivative with a first derivative, two operators of one basis)
+    share the neighbour gathers, epsilon and the moments in one kernel pass."""
+    if not ops:
+        return
+    d0 = ops[0]
+    vl = d0.verletList
+    for d in ops[1:]:
+        if d.verletList is not vl or d.same_set != d0.same_set or d.convergenceOrder != d0.convergenceOrder \
+                or d.HOverEpsilon != d0.HOverEpsilon or d.refine != d0.refine:
+            raise ValueError("operators assembled together must share the Verlet list, order, h/eps and flags")
+    sup = particlesSupport or d0.particlesSupport
+    dom = particlesDomain or (sup if d0.same_set else d0.particlesDomain)
+    dev = sup.device
+    n_dom = dom.size_local()
+    if vl.n_rows != n_dom:
+        raise ValueError(f"Verlet list has {vl.n_rows} rows, domain has {n_dom} particles")
+    outs = [d._outputs(n_dom, dev) for d in ops]
+    n_sig = np.array([len(d.signatures) for d in ops], dtype=np.int32)
+    betas = np.ascontiguousarray(np.concatenate([np.array(d.signatures, dtype=np.uint32).ravel() for d in ops]))
+    coeffs = np.ascontiguousarray(np.concatenate([np.array(d.coeffs, dtype=np.float64) for d in ops]))
+    lo = np.ascontiguousarray(sup.box_lo)
+    hi = np.ascontiguousarray(sup.box_hi)
+    arrays = [_lib.ptr_array([o[k] for o in outs]) for k in range(6)]
+    ctx = _lib.context(dev.index or 0)
+    ctx.call("dcpse_assemble_multi", sup.dims, _lib.ptr(sup.pos_soa), sup.size_local(),
+             None if d0.same_set else _lib.ptr(dom.pos_soa), n_dom,
+             lo.ctypes.data, hi.ctypes.data, sup.periodic_mask,
+             _lib.ptr(vl.row_ptr), _lib.ptr(vl.col) if vl.nnz else None,
+             len(ops), n_sig.ctypes.data, betas.ctypes.data, coeffs.ctypes.data,
+             d0.convergenceOrder, float(d0.HOverEpsilon), _lib.DCPSE_ASM_REFINE if d0.refine else 0,
+             arrays[0], arrays[1], arrays[2], arrays[3], arrays[4], arrays[5])
+    for d, o in zip(ops, outs):
+        d._adopt(o, sup.map_ctr)
+    if check:
+        for d in ops:
+            d.check_info()
 
 
 def apply_ops(ops, f: torch.Tensor, outs=None, f_domain: torch.Tensor | None = None, alpha=1.0, beta=0.0):

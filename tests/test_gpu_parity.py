@@ -317,3 +317,25 @@ def test_sell_bundle16_far_neighbours(oracle, cuda):
     want = sum(oracle.apply(r, rp, col, f64) for r in refs)
     got = c[0].double().cpu().numpy()
     assert np.linalg.norm(got - want) / np.linalg.norm(want) <= 1e-5
+
+
+@pytest.mark.parametrize("name", sorted(WORKLOADS))
+def test_assemble_together_equals_separate(name, cuda):
+    """Operators assembled in one pass (shared gathers, epsilon, moments) give
+    the kernels of separate assemblies, bit for bit."""
+    from openfpm_numerics_amd import Derivative_x, Derivative_y, Laplacian, workloads as W
+    from openfpm_numerics_amd.dcpse import LOAD, assemble_together
+
+    w = WORKLOADS[name](W)
+    ps = _ps(w, cuda)
+    vl = ps.getVerlet(w["rcut"])
+    sep = [Laplacian(ps, vl, 2, w["rcut"]).dcpse, Derivative_x(ps, vl, 2, w["rcut"]).dcpse,
+           Derivative_y(ps, vl, 2, w["rcut"]).dcpse]
+    tog = [Laplacian(ps, vl, 2, w["rcut"], LOAD).dcpse, Derivative_x(ps, vl, 2, w["rcut"], LOAD).dcpse,
+           Derivative_y(ps, vl, 2, w["rcut"], LOAD).dcpse]
+    assemble_together(tog)
+    for a, b in zip(sep, tog):
+        assert torch.equal(a.calcKernels, b.calcKernels), name
+        assert torch.equal(a.localEps, b.localEps) and torch.equal(a.localEpsInvPow, b.localEpsInvPow)
+        assert torch.equal(a.weights(torch.float32), b.weights(torch.float32))
+        assert int(b.info.abs().max()) == 0
