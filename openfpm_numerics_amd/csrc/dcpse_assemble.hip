@@ -59,36 +59,26 @@ constexpr int kChunk = 32;  // neighbours staged per LDS round
 template <int DIM, int L, int AMIN>
 __constant__ BasisTables<DIM, L, AMIN> kTab = BasisTables<DIM, L, AMIN>();
 
-// Moment exponents gamma with QMIN <= |gamma| <= P (dimension 0 fastest).
-template <int DIM, int P, int QMIN>
-struct MomentTable {
-    static constexpr int GW = P + 1;
-    static constexpr int count() {
+// The (y, z) exponent pairs (b, c) with b + c <= P (c = 0 in 2D), b fastest:
+// in the moment pass a lane owns one pair and accumulates M(a, b, c) for
+// every a <= P - b - c.
+template <int DIM, int P>
+struct PairTable {
+    static constexpr int N = DIM == 3 ? (P + 1) * (P + 2) / 2 : P + 1;
+    int bc[N][2];
+    constexpr PairTable() : bc{} {
         int c = 0;
-        for (int lin = 0; lin < ipow_c(GW, DIM); ++lin) {
-            int s = 0, r = lin;
-            for (int d = 0; d < DIM; ++d) { s += r % GW; r /= GW; }
-            if (s >= QMIN && s <= P) ++c;
-        }
-        return c;
-    }
-    static constexpr int G = count();
-    int g[G][3];
-    constexpr MomentTable() : g{} {
-        int c = 0;
-        for (int lin = 0; lin < ipow_c(GW, DIM); ++lin) {
-            int k[3] = {0, 0, 0}, s = 0, r = lin;
-            for (int d = 0; d < DIM; ++d) { k[d] = r % GW; r /= GW; s += k[d]; }
-            if (s >= QMIN && s <= P) {
-                g[c][0] = k[0]; g[c][1] = k[1]; g[c][2] = k[2];
+        for (int cc = 0; cc <= (DIM == 3 ? P : 0); ++cc)
+            for (int b = 0; b + cc <= P; ++b) {
+                bc[c][0] = b;
+                bc[c][1] = cc;
                 ++c;
             }
-        }
     }
 };
 
-template <int DIM, int P, int QMIN>
-__constant__ MomentTable<DIM, P, QMIN> kMom = MomentTable<DIM, P, QMIN>();
+template <int DIM, int P>
+__constant__ PairTable<DIM, P> kPair = PairTable<DIM, P>();
 
 __device__ __forceinline__ double ipow_rt(double x, unsigned e) {
     double r = 1.0;
@@ -265,16 +255,21 @@ __global__ __launch_bounds__(64) void k_assemble(const AsmArgs a) {
     constexpr int P1 = 2 * (L1 - 1), P2 = L2 > 0 ? 2 * (L2 - 1) : 0;
     constexpr int P = P1 > P2 ? P1 : P2;
     constexpr int QMIN = L2 > 0 ? (A1 < A2 ? 2 * A1 : 2 * A2) : 2 * A1;
-    using MT = MomentTable<DIM, P, QMIN>;
-    constexpr int G = MT::G, GW = MT::GW, MG = ipow_c(GW, DIM);
+    constexpr int GW = P + 1, MG = ipow_c(GW, DIM);  // moment grid M(a, b, c) at a + GW b + GW^2 c
     constexpr int M1 = BasisShape<DIM, L1, A1>::M;
     constexpr int M2 = L2 > 0 ? BasisShape<DIM, (L2 > 0 ? L2 : 2), A2>::M : 0;
     constexpr int MX = M1 > M2 ? M1 : M2;
     constexpr int MN = L2 > 0 ? (M1 < M2 ? M1 : M2) : M1;
-    constexpr int ROUNDS = (G + kWave - 1) / kWave;
     constexpr int PW = P + 1;
-    constexpr int TROW = DIM * PW;
+    // power-table row per staged neighbour: [x][y^0..y^P][w z^0..w z^P]
+    // (2D: [x][y^0..y^P][w])
+    constexpr int ZW = DIM == 3 ? PW : 1;
+    constexpr int TROW = 1 + PW + ZW;
     constexpr int TAB = (kChunk * TROW > MX * MX) ? kChunk * TROW : MX * MX;
+    constexpr int NPAIR = PairTable<DIM, P>::N;
+    constexpr int HALVES = NPAIR <= kWave / 2 ? 2 : 1;  // neighbours per pass
+    constexpr int LPN = kWave / HALVES;                 // lanes per neighbour
+    static_assert(NPAIR <= kWave, "moment pairs exceed a wavefront");
     __shared__ double tab[TAB];
     __shared__ double mom[MG];
     __shared__ double sidg[MX];  // 1 / L_kk (uniform across the wave)
@@ -282,22 +277,12 @@ __global__ __launch_bounds__(64) void k_assemble(const AsmArgs a) {
     __shared__ double sdx[MX];   // refinement correction
     const int lane = threadIdx.x;
 
-    // moments owned by this lane: table offsets and moment-grid index
-    int o0[ROUNDS], o1[ROUNDS], o2[ROUNDS], gi[ROUNDS];
-    bool gv[ROUNDS];
-#pragma unroll
-    for (int r = 0; r < ROUNDS; ++r) {
-        const int g = lane + kWave * r;
-        gv[r] = g < G;
-        const int gg = gv[r] ? g : 0;
-        const int e0 = kMom<DIM, P, QMIN>.g[gg][0];
-        const int e1 = kMom<DIM, P, QMIN>.g[gg][1];
-        const int e2 = DIM == 3 ? kMom<DIM, P, QMIN>.g[gg][2] : 0;
-        o0[r] = e0;
-        o1[r] = PW + e1;
-        o2[r] = 2 * PW + e2;
-        gi[r] = e0 + GW * e1 + GW * GW * e2;
-    }
+    // the (b, c) pair this lane accumulates, and which neighbour of a pass
+    const int half = lane / LPN;
+    const int pl = lane % LPN;
+    const bool pv = pl < NPAIR;
+    const int pb = kPair<DIM, P>.bc[pv ? pl : 0][0];
+    const int pc = kPair<DIM, P>.bc[pv ? pl : 0][1];
 
     const int64_t pbeg = (int64_t)blockIdx.x * a.ppb;
     const int64_t pend = min(pbeg + (int64_t)a.ppb, a.n_dom);
@@ -343,9 +328,14 @@ __global__ __launch_bounds__(64) void k_assemble(const AsmArgs a) {
         const double ieps = 1.0 / eps;
 
         // ---- moments -----------------------------------------------------------
-        double acc[ROUNDS];
+        // M(a, b, c) = sum_q w_q x_q^a y_q^b z_q^c (x, y, z = scaled displacement).
+        // Lanes own (b, c) pairs and read y^b and w z^c of the staged neighbour
+        // from LDS, forming the x powers in registers; a pass covers HALVES
+        // neighbours (one per half-wavefront), so each LDS read serves the
+        // whole moment column of its pair.
+        double acc[PW];
 #pragma unroll
-        for (int r = 0; r < ROUNDS; ++r) acc[r] = 0.0;
+        for (int k = 0; k < PW; ++k) acc[k] = 0.0;
         for (int c0 = 0; c0 < n; c0 += kChunk) {
             const int cnt = min(kChunk, n - c0);
             const int jl = lane & (kChunk - 1);
@@ -359,14 +349,12 @@ __global__ __launch_bounds__(64) void k_assemble(const AsmArgs a) {
                 for (int d = 0; d < DIM; ++d) z[d] = dv[d] * ieps;
                 double* row = tab + jl * TROW;
                 if (lane < kChunk) {
+                    row[0] = z[0];
+                    double pw = 1.0;
 #pragma unroll
-                    for (int d = 0; d < DIM - 1; ++d) {
-                        double pw = 1.0;
-#pragma unroll
-                        for (int k = 0; k <= P; ++k) {
-                            row[d * PW + k] = pw;
-                            pw *= z[d];
-                        }
+                    for (int k = 0; k <= P; ++k) {
+                        row[1 + k] = pw;
+                        pw *= z[1];
                     }
                 } else {
                     double z2 = 0.0;
@@ -374,29 +362,42 @@ __global__ __launch_bounds__(64) void k_assemble(const AsmArgs a) {
                     for (int d = 0; d < DIM; ++d) z2 += z[d] * z[d];
                     double pw = exp(-z2);
 #pragma unroll
-                    for (int k = 0; k <= P; ++k) {
-                        row[(DIM - 1) * PW + k] = pw;
+                    for (int k = 0; k < ZW; ++k) {
+                        row[1 + PW + k] = pw;
                         pw *= z[DIM - 1];
                     }
                 }
             }
             wave_lds_sync();
-#pragma unroll 4
-            for (int j = 0; j < cnt; ++j) {
-                const double* row = tab + j * TROW;
+#pragma unroll 2
+            for (int j = 0; j < cnt; j += HALVES) {
+                const int jj = j + half;
+                const bool v = pv && jj < cnt;
+                const double* row = tab + (jj < kChunk ? jj : 0) * TROW;
+                // rows past cnt hold another neighbour or nothing yet: both
+                // factors are replaced, so no stale value reaches the sums
+                const double x = v ? row[0] : 0.0;
+                double t = row[1 + pb] * row[1 + PW + pc];
+                t = v ? t : 0.0;
 #pragma unroll
-                for (int r = 0; r < ROUNDS; ++r) {
-                    if (gv[r]) {
-                        if (DIM == 3) acc[r] = fma(row[o0[r]] * row[o1[r]], row[o2[r]], acc[r]);
-                        else acc[r] = fma(row[o0[r]], row[o1[r]], acc[r]);
-                    }
+                for (int k = 0; k < PW; ++k) {
+                    acc[k] += t;
+                    t *= x;
                 }
             }
             wave_lds_sync();
         }
+        if (HALVES == 2) {
 #pragma unroll
-        for (int r = 0; r < ROUNDS; ++r)
-            if (gv[r]) mom[gi[r]] = acc[r];
+            for (int k = 0; k < PW; ++k) acc[k] += __shfl_down(acc[k], LPN, kWave);
+        }
+        if (pv && half == 0) {
+#pragma unroll
+            for (int k = 0; k < PW; ++k) {
+                const int s = k + pb + pc;
+                if (s >= QMIN && s <= P) mom[k + GW * pb + GW * GW * pc] = acc[k];
+            }
+        }
         wave_lds_sync();
 
         // ---- per operator: factorisation, solve, kernel weights ------------------
