@@ -149,8 +149,15 @@ __device__ __forceinline__ void asm_solve(const AsmArgs& a, const AsmOp& op, int
             ok = false;
             dkk = 1.0;
         }
-        const double sk = sqrt(dkk);
-        const double inv = 1.0 / sk;
+        // 1/sqrt(d) from the hardware estimate and two Newton-Raphson steps
+        // r <- r (3 - d r^2) / 2 (quadratic convergence: ~23 -> 46 -> full
+        // bits), then sqrt(d) = d * r: one estimate and a few FMAs instead of
+        // a square-root and a division sequence per step
+        double inv = __builtin_amdgcn_rsq(dkk);
+        const double hd = 0.5 * dkk;
+        inv = inv * fma(-hd * inv, inv, 1.5);
+        inv = inv * fma(-hd * inv, inv, 1.5);
+        const double sk = dkk * inv;
         if (me == 0) sidg[k] = inv;
         const double lik = ar[k] * inv;
         ar[k] = (me == k) ? sk : lik;
@@ -286,124 +293,157 @@ __global__ __launch_bounds__(64) void k_assemble(const AsmArgs a) {
 
     const int64_t pbeg = (int64_t)blockIdx.x * a.ppb;
     const int64_t pend = min(pbeg + (int64_t)a.ppb, a.n_dom);
-    for (int64_t p = pbeg; p < pend; ++p) {
-        const int64_t rs = a.row_ptr[p];
-        const int n = (int)(a.row_ptr[p + 1] - rs);
-        double xp[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) xp[d] = a.pos_dom[d * a.n_dom + p];
-        auto disp = [&](int j, double* dv) {
-            const int q = a.col[rs + j];
+    // Particles of the block are software-pipelined: while particle p is
+    // assembled, the row bounds, first 64 columns and their displacements of
+    // particle p + 1 are loaded (row bounds at the top, columns after the eps
+    // pass, positions after the moments), so their memory latency overlaps the
+    // moments and the factorisation of p.
+    auto load_disp = [&](int64_t rs_, int n_, const double* xp_, int j, double* dv) {
+        if (j < n_) {
+            const int q = a.col[rs_ + j];
 #pragma unroll
             for (int d = 0; d < DIM; ++d)
-                dv[d] = min_image(xp[d] - a.pos_sup[d * a.n_sup + q], a.len[d], (a.periodic >> d) & 1u);
-        };
+                dv[d] = min_image(xp_[d] - a.pos_sup[d * a.n_sup + q], a.len[d], (a.periodic >> d) & 1u);
+        }
+    };
+    int64_t rs = 0;
+    int n = 0;
+    double xp[3] = {0.0, 0.0, 0.0};
+    double cdv[3] = {0.0, 0.0, 0.0};  // lane j: displacement of neighbour j (j < 64)
+    if (pbeg < pend) {
+        rs = a.row_ptr[pbeg];
+        n = (int)(a.row_ptr[pbeg + 1] - rs);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) xp[d] = a.pos_dom[d * a.n_dom + pbeg];
+        load_disp(rs, n, xp, lane, cdv);
+    }
+    for (int64_t p = pbeg; p < pend; ++p) {
+        const bool has_next = p + 1 < pend;
+        int64_t rs_n = 0;
+        int n_n = 0;
+        double xp_n[3] = {0.0, 0.0, 0.0};
+        if (has_next) {
+            rs_n = a.row_ptr[p + 1];
+            n_n = (int)(a.row_ptr[p + 2] - rs_n);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) xp_n[d] = a.pos_dom[d * a.n_dom + p + 1];
+        }
+        auto disp = [&](int j, double* dv) { load_disp(rs, n, xp, j, dv); };
 
         // ---- eps -------------------------------------------------------------
-        // lane j keeps the displacement of neighbour j (j < 64): the moment and
-        // kernel phases reuse it instead of gathering the positions again
-        double cdv[3] = {0.0, 0.0, 0.0};
         double l1 = 0.0;
         for (int j = lane; j < n; j += kWave) {
-            double dv[3];
-            disp(j, dv);
+            double dv[3] = {cdv[0], cdv[1], cdv[2]};
+            if (j >= kWave) disp(j, dv);
             double s = 0.0;
 #pragma unroll
             for (int d = 0; d < DIM; ++d) s += fabs(dv[d]);
             l1 += s;
-            if (j == lane) {
-#pragma unroll
-                for (int d = 0; d < DIM; ++d) cdv[d] = dv[d];
-            }
         }
         l1 = wave_sum(l1);
         const double eps = n > 0 ? (l1 / n) / a.h_over_eps : 0.0;
         const bool eps_bad = !(eps > 0.0) || !isfinite(eps);
+        // next particle: first 64 columns
+        const int q_n = has_next && lane < n_n ? a.col[rs_n + lane] : 0;
         if (n < MN || eps_bad) {  // no operator of the pass can be assembled
             const int st0 = n < M1 ? DCPSE_INFO_TOO_FEW : DCPSE_INFO_EPS_ZERO;
             asm_fail(a.op[0], p, rs, n, st0, lane);
             if (L2 > 0) asm_fail(a.op[1], p, rs, n, n < M2 ? DCPSE_INFO_TOO_FEW : DCPSE_INFO_EPS_ZERO, lane);
-            continue;
-        }
-        const double ieps = 1.0 / eps;
+        } else {
+            const double ieps = 1.0 / eps;
 
-        // ---- moments -----------------------------------------------------------
-        // M(a, b, c) = sum_q w_q x_q^a y_q^b z_q^c (x, y, z = scaled displacement).
-        // Lanes own (b, c) pairs and read y^b and w z^c of the staged neighbour
-        // from LDS, forming the x powers in registers; a pass covers HALVES
-        // neighbours (one per half-wavefront), so each LDS read serves the
-        // whole moment column of its pair.
-        double acc[PW];
+            // ---- moments -----------------------------------------------------------
+            // M(a, b, c) = sum_q w_q x_q^a y_q^b z_q^c (x, y, z = scaled displacement).
+            // Lanes own (b, c) pairs and read y^b and w z^c of the staged neighbour
+            // from LDS, forming the x powers in registers; a pass covers HALVES
+            // neighbours (one per half-wavefront), so each LDS read serves the
+            // whole moment column of its pair.
+            double acc[PW];
 #pragma unroll
-        for (int k = 0; k < PW; ++k) acc[k] = 0.0;
-        for (int c0 = 0; c0 < n; c0 += kChunk) {
-            const int cnt = min(kChunk, n - c0);
-            const int jl = lane & (kChunk - 1);
-            double dv[3];
+            for (int k = 0; k < PW; ++k) acc[k] = 0.0;
+            for (int c0 = 0; c0 < n; c0 += kChunk) {
+                const int cnt = min(kChunk, n - c0);
+                const int jl = lane & (kChunk - 1);
+                double dv[3];
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) dv[d] = __shfl(cdv[d], (c0 + jl) & (kWave - 1), kWave);
-            if (jl < cnt) {
-                if (c0 + jl >= kWave) disp(c0 + jl, dv);
-                double z[3] = {0.0, 0.0, 0.0};
+                for (int d = 0; d < DIM; ++d) dv[d] = __shfl(cdv[d], (c0 + jl) & (kWave - 1), kWave);
+                if (jl < cnt) {
+                    if (c0 + jl >= kWave) disp(c0 + jl, dv);
+                    double z[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) z[d] = dv[d] * ieps;
-                double* row = tab + jl * TROW;
-                if (lane < kChunk) {
-                    row[0] = z[0];
-                    double pw = 1.0;
+                    for (int d = 0; d < DIM; ++d) z[d] = dv[d] * ieps;
+                    double* row = tab + jl * TROW;
+                    if (lane < kChunk) {
+                        row[0] = z[0];
+                        double pw = 1.0;
 #pragma unroll
-                    for (int k = 0; k <= P; ++k) {
-                        row[1 + k] = pw;
-                        pw *= z[1];
-                    }
-                } else {
-                    double z2 = 0.0;
+                        for (int k = 0; k <= P; ++k) {
+                            row[1 + k] = pw;
+                            pw *= z[1];
+                        }
+                    } else {
+                        double z2 = 0.0;
 #pragma unroll
-                    for (int d = 0; d < DIM; ++d) z2 += z[d] * z[d];
-                    double pw = exp(-z2);
+                        for (int d = 0; d < DIM; ++d) z2 += z[d] * z[d];
+                        double pw = exp(-z2);
 #pragma unroll
-                    for (int k = 0; k < ZW; ++k) {
-                        row[1 + PW + k] = pw;
-                        pw *= z[DIM - 1];
+                        for (int k = 0; k < ZW; ++k) {
+                            row[1 + PW + k] = pw;
+                            pw *= z[DIM - 1];
+                        }
                     }
                 }
-            }
-            wave_lds_sync();
+                wave_lds_sync();
 #pragma unroll 2
-            for (int j = 0; j < cnt; j += HALVES) {
-                const int jj = j + half;
-                const bool v = pv && jj < cnt;
-                const double* row = tab + (jj < kChunk ? jj : 0) * TROW;
-                // rows past cnt hold another neighbour or nothing yet: both
-                // factors are replaced, so no stale value reaches the sums
-                const double x = v ? row[0] : 0.0;
-                double t = row[1 + pb] * row[1 + PW + pc];
-                t = v ? t : 0.0;
+                for (int j = 0; j < cnt; j += HALVES) {
+                    const int jj = j + half;
+                    const bool v = pv && jj < cnt;
+                    const double* row = tab + (jj < kChunk ? jj : 0) * TROW;
+                    // rows past cnt hold another neighbour or nothing yet: both
+                    // factors are replaced, so no stale value reaches the sums
+                    const double x = v ? row[0] : 0.0;
+                    double t = row[1 + pb] * row[1 + PW + pc];
+                    t = v ? t : 0.0;
+#pragma unroll
+                    for (int k = 0; k < PW; ++k) {
+                        acc[k] += t;
+                        t *= x;
+                    }
+                }
+                wave_lds_sync();
+            }
+            if (HALVES == 2) {
+#pragma unroll
+                for (int k = 0; k < PW; ++k) acc[k] += __shfl_down(acc[k], LPN, kWave);
+            }
+            if (pv && half == 0) {
 #pragma unroll
                 for (int k = 0; k < PW; ++k) {
-                    acc[k] += t;
-                    t *= x;
+                    const int s = k + pb + pc;
+                    if (s >= QMIN && s <= P) mom[k + GW * pb + GW * GW * pc] = acc[k];
                 }
             }
             wave_lds_sync();
-        }
-        if (HALVES == 2) {
-#pragma unroll
-            for (int k = 0; k < PW; ++k) acc[k] += __shfl_down(acc[k], LPN, kWave);
-        }
-        if (pv && half == 0) {
-#pragma unroll
-            for (int k = 0; k < PW; ++k) {
-                const int s = k + pb + pc;
-                if (s >= QMIN && s <= P) mom[k + GW * pb + GW * GW * pc] = acc[k];
-            }
-        }
-        wave_lds_sync();
 
-        // ---- per operator: factorisation, solve, kernel weights ------------------
-        asm_solve<DIM, L1, A1, GW>(a, a.op[0], p, rs, n, eps, ieps, cdv, disp, mom, tab, sidg, sx, sdx, lane);
-        if constexpr (L2 > 0)
-            asm_solve<DIM, L2, A2, GW>(a, a.op[1], p, rs, n, eps, ieps, cdv, disp, mom, tab, sidg, sx, sdx, lane);
+            // ---- per operator: factorisation, solve, kernel weights ------------------
+            asm_solve<DIM, L1, A1, GW>(a, a.op[0], p, rs, n, eps, ieps, cdv, disp, mom, tab, sidg, sx, sdx, lane);
+            if constexpr (L2 > 0)
+                asm_solve<DIM, L2, A2, GW>(a, a.op[1], p, rs, n, eps, ieps, cdv, disp, mom, tab, sidg, sx, sdx, lane);
+        }
+        // next particle: displacements of its first 64 neighbours
+        double dv_n[3] = {0.0, 0.0, 0.0};
+        if (has_next && lane < n_n) {
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+                dv_n[d] = min_image(xp_n[d] - a.pos_sup[d * a.n_sup + q_n], a.len[d], (a.periodic >> d) & 1u);
+        }
+        rs = rs_n;
+        n = n_n;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            xp[d] = xp_n[d];
+            cdv[d] = dv_n[d];
+        }
     }
 }
 
