@@ -185,14 +185,20 @@ __global__ void k_cell_assign(const double* __restrict__ pos, int64_t n, CellGri
     }
 }
 
-__global__ void k_cell_fill(const int32_t* __restrict__ cell_id, int64_t n,
+// cell_part[slot] = particle, and a copy of its position in cell order
+// (pos_cell[d * n + slot]) so the row kernels read candidate positions
+// contiguously per cell, without waiting for the particle index first.
+template <int DIM>
+__global__ void k_cell_fill(const int32_t* __restrict__ cell_id, int64_t n, const double* __restrict__ pos,
                             const int64_t* __restrict__ cell_start, int32_t* __restrict__ cursor,
-                            int32_t* __restrict__ cell_part) {
+                            int32_t* __restrict__ cell_part, double* __restrict__ pos_cell) {
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
          q += (int64_t)gridDim.x * blockDim.x) {
         int32_t c = cell_id[q];
         int64_t at = cell_start[c] + atomicAdd(&cursor[c], 1);
         cell_part[at] = (int32_t)q;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) pos_cell[d * n + at] = pos[d * n + q];
     }
 }
 
@@ -206,8 +212,8 @@ __global__ __launch_bounds__(kRowWaves * kWave) void k_support_rows(
     const double* __restrict__ pos_sup, int64_t n_sup, const double* __restrict__ pos_dom,
     int64_t n_dom, int same_set, const double* __restrict__ rcut, int rcut_pp, CellGrid g,
     const int64_t* __restrict__ cell_start, const int32_t* __restrict__ cell_part,
-    int32_t* __restrict__ counts, int32_t* __restrict__ overflow, const int64_t* __restrict__ row_ptr,
-    int32_t* __restrict__ col) {
+    const double* __restrict__ pos_cell, int32_t* __restrict__ counts, int32_t* __restrict__ overflow,
+    const int64_t* __restrict__ row_ptr, int32_t* __restrict__ col) {
     __shared__ int32_t buf[FILL ? kRowWaves : 1][FILL ? DCPSE_MAX_ROW : 1];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
@@ -268,11 +274,12 @@ __global__ __launch_bounds__(kRowWaves * kWave) void k_support_rows(
             }
             const long long s_cst = __shfl(cst, seg, kWave);
             const int s_ex = __shfl(excl, seg, kWave);
-            const int q = cell_part[s_cst + (k - s_ex)];
+            const int64_t slot = s_cst + (k - s_ex);
+            const int q = cell_part[slot];
             double d[3];
 #pragma unroll
             for (int dd = 0; dd < DIM; ++dd)
-                d[dd] = min_image(xp[dd] - pos_sup[dd * n_sup + q], g.len[dd], (g.periodic >> dd) & 1u);
+                d[dd] = min_image(xp[dd] - pos_cell[dd * n_sup + slot], g.len[dd], (g.periodic >> dd) & 1u);
             double d2 = d[DIM - 1] * d[DIM - 1];
 #pragma unroll
             for (int dd = DIM - 2; dd >= 0; --dd) d2 = fma(d[dd], d[dd], d2);
@@ -367,11 +374,13 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
     if (st) return st;
 
     Workspace w;
-    size_t need = (size_t)n_sup * 4 * 2 + (size_t)(g.ncell + 1) * 8 + (size_t)g.ncell * 4 * 2 +
-                  (size_t)n_dom * 4 + scan_workspace_bytes(std::max(g.ncell, n_dom)) + 64 * 256;
+    size_t need = (size_t)n_sup * 4 * 2 + (size_t)dim * (n_sup + 1) * 8 + (size_t)(g.ncell + 1) * 8 +
+                  (size_t)g.ncell * 4 * 2 + (size_t)n_dom * 4 + scan_workspace_bytes(std::max(g.ncell, n_dom)) +
+                  64 * 256;
     if ((st = reserve_workspace(ctx, need, &w))) return st;
     int32_t* cell_id = w.take<int32_t>(n_sup + 1);
     int32_t* cell_part = w.take<int32_t>(n_sup + 1);
+    double* pos_cell = w.take<double>((size_t)dim * (n_sup + 1));
     int32_t* cell_cnt = w.take<int32_t>(g.ncell + 1);
     int64_t* cell_start = w.take<int64_t>(g.ncell + 1);
     int32_t* counts = w.take<int32_t>(n_dom + 1);
@@ -391,7 +400,12 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
     if ((st = scan_counts(ctx, cell_cnt, g.ncell, cell_start, &ws_scan))) return st;
     fill_i32(s, cell_cnt, g.ncell + 1, 0);  // reuse as cursor
     if (n_sup > 0) {
-        k_cell_fill<<<(unsigned)tb, 256, 0, s>>>(cell_id, n_sup, cell_start, cell_cnt, cell_part);
+        if (dim == 2)
+            k_cell_fill<2><<<(unsigned)tb, 256, 0, s>>>(cell_id, n_sup, pos_sup, cell_start, cell_cnt, cell_part,
+                                                        pos_cell);
+        else
+            k_cell_fill<3><<<(unsigned)tb, 256, 0, s>>>(cell_id, n_sup, pos_sup, cell_start, cell_cnt, cell_part,
+                                                        pos_cell);
         DCPSE_LAUNCH_CHECK(ctx);
     }
     const int64_t rb = std::max<int64_t>(1, std::min<int64_t>((n_dom + kRowWaves - 1) / kRowWaves,
@@ -400,11 +414,11 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
         if (n_dom > 0) {
             if (dim == 2)
                 k_support_rows<2, false><<<(unsigned)rb, kRowWaves * kWave, 0, s>>>(
-                    pos_sup, n_sup, pos_dom, n_dom, same, rcut, rcut_pp, g, cell_start, cell_part, counts,
+                    pos_sup, n_sup, pos_dom, n_dom, same, rcut, rcut_pp, g, cell_start, cell_part, pos_cell, counts,
                     flags, nullptr, nullptr);
             else
                 k_support_rows<3, false><<<(unsigned)rb, kRowWaves * kWave, 0, s>>>(
-                    pos_sup, n_sup, pos_dom, n_dom, same, rcut, rcut_pp, g, cell_start, cell_part, counts,
+                    pos_sup, n_sup, pos_dom, n_dom, same, rcut, rcut_pp, g, cell_start, cell_part, pos_cell, counts,
                     flags, nullptr, nullptr);
             DCPSE_LAUNCH_CHECK(ctx);
         }
@@ -425,11 +439,11 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
         if (n_dom > 0) {
             if (dim == 2)
                 k_support_rows<2, true><<<(unsigned)rb, kRowWaves * kWave, 0, s>>>(
-                    pos_sup, n_sup, pos_dom, n_dom, same, rcut, rcut_pp, g, cell_start, cell_part, counts,
+                    pos_sup, n_sup, pos_dom, n_dom, same, rcut, rcut_pp, g, cell_start, cell_part, pos_cell, counts,
                     flags, row_ptr, col);
             else
                 k_support_rows<3, true><<<(unsigned)rb, kRowWaves * kWave, 0, s>>>(
-                    pos_sup, n_sup, pos_dom, n_dom, same, rcut, rcut_pp, g, cell_start, cell_part, counts,
+                    pos_sup, n_sup, pos_dom, n_dom, same, rcut, rcut_pp, g, cell_start, cell_part, pos_cell, counts,
                     flags, row_ptr, col);
             DCPSE_LAUNCH_CHECK(ctx);
         }
