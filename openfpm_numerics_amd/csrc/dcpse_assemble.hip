@@ -9,9 +9,9 @@
 //     so A_ij = M(alpha_i + alpha_j). Neighbours are staged 32 at a time as
 //     per-dimension power tables in LDS; each lane owns one or more moments.
 //   * A is factored by an in-register FP64 Cholesky (lane i holds row i,
-//     broadcasts by v_readlane) instead of the reference's
-//     colPivHouseholderQr (Dcpse.hpp:766); an optional FP64 iterative
-//     refinement step recomputes the residual against the moments.
+//     pivots by v_readlane, columns broadcast through LDS) instead of the
+//     reference's colPivHouseholderQr (Dcpse.hpp:766); an optional FP64
+//     iterative refinement step recomputes the residual against the moments.
 //   * epilogue: lanes own neighbours again and write
 //        K_pq = w_q * sum_i a_i z_q^alpha_i              (Dcpse.hpp:798-809)
 //     coalesced, together with the eps^-|beta| pre-scaled FP64/FP32 weights.
@@ -111,8 +111,8 @@ __device__ __forceinline__ void asm_fail(const AsmOp& op, int64_t p, int64_t rs,
 template <int DIM, int L, int AMIN, int GW, typename Disp>
 __device__ __forceinline__ void asm_solve(const AsmArgs& a, const AsmOp& op, int64_t p, int64_t rs, int n,
                                           double eps, double ieps, const double (&cdv)[3], Disp disp,
-                                          const double* mom, double* lmat, double* sidg, double* sx, double* sdx,
-                                          int lane) {
+                                          const double* mom, double* lmat, double* lcol, double* sx,
+                                          double b_lane, int lane) {
     using S = BasisShape<DIM, L, AMIN>;
     constexpr int M = S::M;
     static_assert(M <= kWave, "basis larger than a wavefront");
@@ -127,10 +127,6 @@ __device__ __forceinline__ void asm_solve(const AsmArgs& a, const AsmOp& op, int
     const int ai0 = kTab<DIM, L, AMIN>.e[li][0];
     const int ai1 = kTab<DIM, L, AMIN>.e[li][1];
     const int ai2 = DIM == 3 ? kTab<DIM, L, AMIN>.e[li][2] : 0;
-    double b_lane = 0.0;
-#pragma unroll
-    for (int k = 0; k < M; ++k)
-        if (me == k) b_lane = op.b[k];
 
     double ar[M];
 #pragma unroll
@@ -139,64 +135,74 @@ __device__ __forceinline__ void asm_solve(const AsmArgs& a, const AsmOp& op, int
         const double v = mom[idx];
         ar[k] = me < M ? v : 0.0;
     }
-    // right-looking: step k scales column k and updates the trailing columns
-    // with its entries broadcast by v_readlane
-    bool ok = true;
+    // ---- A = L L^T, right-looking; lane i holds row i ---------------------
+    // Step k takes the pivot from lane k (v_readlane), scales column k and
+    // publishes it in LDS; every lane then reads the trailing entries l_jk as
+    // broadcasts (one address for the wave) and updates its row. Entries right
+    // of the diagonal are updated too and never read as part of L.
+    // Pivot checks accumulate in vector registers (no per-step scalar masks):
+    // the smallest pivot, and pivot - pivot, which is NaN once a pivot is not
+    // finite; a failed factorisation writes zero weights whatever it computed.
+    double dmin = 1.0, dfin = 0.0;
+    double dinv = 0.0;  // lane k: 1 / L_kk
 #pragma unroll
     for (int k = 0; k < M; ++k) {
-        double dkk = readlane_d(ar[k], k);
-        if (!(dkk > 0.0) || !isfinite(dkk)) {
-            ok = false;
-            dkk = 1.0;
-        }
+        const double dkk = readlane_d(ar[k], k);
+        dmin = fmin(dmin, dkk);
+        dfin += dkk - dkk;
+        asm volatile("" : "+v"(dmin), "+v"(dfin));  // consumed here, not kept as scalars
         // 1/sqrt(d) from the hardware estimate and two Newton-Raphson steps
         // r <- r (3 - d r^2) / 2 (quadratic convergence: ~23 -> 46 -> full
-        // bits), then sqrt(d) = d * r: one estimate and a few FMAs instead of
-        // a square-root and a division sequence per step
+        // bits): one estimate and a few FMAs instead of a square-root and a
+        // division sequence per step
         double inv = __builtin_amdgcn_rsq(dkk);
         const double hd = 0.5 * dkk;
         inv = inv * fma(-hd * inv, inv, 1.5);
         inv = inv * fma(-hd * inv, inv, 1.5);
-        const double sk = dkk * inv;
-        if (me == 0) sidg[k] = inv;
-        const double lik = ar[k] * inv;
-        ar[k] = (me == k) ? sk : lik;
+        dinv = (me == k) ? inv : dinv;
+        // lane k's own entry becomes dkk * inv = L_kk
+        ar[k] *= inv;
+        const double lik = ar[k];
+        if (k + 1 < M) {
+            double* col = lcol + (k & 1) * kWave;  // alternate buffers between steps
+            col[me] = lik;
+            wave_lds_sync();
 #pragma unroll
-        for (int j = k + 1; j < M; ++j) {
-            const double ljk = readlane_d(lik, j);
-            ar[j] = fma(-lik, ljk, ar[j]);
+            for (int j = k + 1; j < M; ++j) ar[j] = fma(-lik, col[j], ar[j]);
         }
-        // one step at a time: keeps the broadcasts of a step from piling up in
-        // scalar registers across steps
         __builtin_amdgcn_sched_barrier(0);
     }
-    // L rows to LDS (for column access in the backward solve)
+    const bool ok = dmin > 0.0 && dfin == 0.0;
+    // keep the strictly lower part of L (the diagonal is in dinv): a lane's
+    // partial sums then stop changing once its unknown is determined, so the
+    // substitutions need no per-lane capture of their results
 #pragma unroll
-    for (int k = 0; k < M; ++k)
-        if (k <= me && me < M) lmat[me * M + k] = ar[k];
+    for (int k = 0; k < M; ++k) ar[k] = k < me ? ar[k] : 0.0;
+    // L rows to LDS (row stride MS): the backward solve reads column `me`
+    constexpr int MS = M + (M & 1);
+    if (me < M) {
+#pragma unroll
+        for (int k = 0; k < M; ++k) lmat[me * MS + k] = ar[k];
+    }
     wave_lds_sync();
 
-    // the uniform solution vectors live in LDS (x[j] written by lane 0), which
-    // keeps the register file to the factor rows
-    auto chol_solve = [&](double rhs, double* x) {
-        double r = rhs, t = 0.0;
+    // forward (L y = r) and backward (L^T x = y) substitution: step k
+    // broadcasts unknown k from lane k and every lane subtracts its multiple;
+    // lane k's own sum is final from step k on, so it ends holding y_k, then x_k
+    auto chol_solve = [&](double rhs) {
+        double r = rhs;
 #pragma unroll
-        for (int k = 0; k < M; ++k) {
-            const double yk = readlane_d(r, k) * sidg[k];
-            if (me == k) t = yk;
-            r = fma(-ar[k], yk, r);
-        }
+        for (int k = 0; k < M; ++k) r = fma(-ar[k], readlane_d(r * dinv, k), r);
+        double t = r * dinv;
 #pragma unroll
-        for (int j = M - 1; j >= 0; --j) {
-            const double xj = readlane_d(t, j) * sidg[j];
-            if (me == 0) x[j] = xj;
-            const double ljk = me < j ? lmat[j * M + me] : 0.0;
-            t = fma(-ljk, xj, t);
-        }
-        wave_lds_sync();
+        for (int j = M - 1; j >= 0; --j) t = fma(-lmat[j * MS + li], readlane_d(t * dinv, j), t);
+        return t * dinv;
     };
-    chol_solve(b_lane, sx);
+    double xv = chol_solve(b_lane);
     if (a.refine) {
+        // one step of iterative refinement against the moments: x += A^-1 (b - A x)
+        if (me < M) sx[me] = xv;
+        wave_lds_sync();
         double rr = b_lane;
 #pragma unroll
         for (int k = 0; k < M; ++k) {
@@ -204,10 +210,10 @@ __device__ __forceinline__ void asm_solve(const AsmArgs& a, const AsmOp& op, int
             const double aik = me < M ? mom[idx] : 0.0;
             rr = fma(-aik, sx[k], rr);
         }
-        chol_solve(rr, sdx);
-        if (me < M) sx[me] += sdx[me];
-        wave_lds_sync();
+        xv += chol_solve(rr);
     }
+    if (me < M) sx[me] = xv;
+    wave_lds_sync();
 
     // ---- kernel weights ----------------------------------------------------
     const double eip = 1.0 / ipow_rt(eps, op.border);
@@ -272,16 +278,15 @@ __global__ __launch_bounds__(64) void k_assemble(const AsmArgs a) {
     // (2D: [x][y^0..y^P][w])
     constexpr int ZW = DIM == 3 ? PW : 1;
     constexpr int TROW = 1 + PW + ZW;
-    constexpr int TAB = (kChunk * TROW > MX * MX) ? kChunk * TROW : MX * MX;
+    constexpr int TAB = (kChunk * TROW > MX * (MX + 1)) ? kChunk * TROW : MX * (MX + 1);
     constexpr int NPAIR = PairTable<DIM, P>::N;
     constexpr int HALVES = NPAIR <= kWave / 2 ? 2 : 1;  // neighbours per pass
     constexpr int LPN = kWave / HALVES;                 // lanes per neighbour
     static_assert(NPAIR <= kWave, "moment pairs exceed a wavefront");
     __shared__ double tab[TAB];
     __shared__ double mom[MG];
-    __shared__ double sidg[MX];  // 1 / L_kk (uniform across the wave)
-    __shared__ double sx[MX];    // solution a (uniform across the wave)
-    __shared__ double sdx[MX];   // refinement correction
+    __shared__ double lcol[2 * kWave];  // column k of L, broadcast to the wave
+    __shared__ double sx[MX];           // solution a (uniform across the wave)
     const int lane = threadIdx.x;
 
     // the (b, c) pair this lane accumulates, and which neighbour of a pass
@@ -290,6 +295,10 @@ __global__ __launch_bounds__(64) void k_assemble(const AsmArgs a) {
     const bool pv = pl < NPAIR;
     const int pb = kPair<DIM, P>.bc[pv ? pl : 0][0];
     const int pc = kPair<DIM, P>.bc[pv ? pl : 0][1];
+
+    // lane i: entry i of each operator's right-hand side
+    const double b1 = lane < M1 ? a.op[0].b[lane] : 0.0;
+    const double b2 = L2 > 0 && lane < M2 ? a.op[1].b[lane] : 0.0;
 
     const int64_t pbeg = (int64_t)blockIdx.x * a.ppb;
     const int64_t pend = min(pbeg + (int64_t)a.ppb, a.n_dom);
@@ -426,9 +435,10 @@ __global__ __launch_bounds__(64) void k_assemble(const AsmArgs a) {
             wave_lds_sync();
 
             // ---- per operator: factorisation, solve, kernel weights ------------------
-            asm_solve<DIM, L1, A1, GW>(a, a.op[0], p, rs, n, eps, ieps, cdv, disp, mom, tab, sidg, sx, sdx, lane);
+            asm_solve<DIM, L1, A1, GW>(a, a.op[0], p, rs, n, eps, ieps, cdv, disp, mom, tab, lcol, sx, b1, lane);
             if constexpr (L2 > 0)
-                asm_solve<DIM, L2, A2, GW>(a, a.op[1], p, rs, n, eps, ieps, cdv, disp, mom, tab, sidg, sx, sdx, lane);
+                asm_solve<DIM, L2, A2, GW>(a, a.op[1], p, rs, n, eps, ieps, cdv, disp, mom, tab, lcol, sx, b2,
+                                           lane);
         }
         // next particle: displacements of its first 64 neighbours
         double dv_n[3] = {0.0, 0.0, 0.0};
