@@ -90,6 +90,27 @@ __device__ __forceinline__ double ipow_rt(double x, unsigned e) {
     return r;
 }
 
+// Lane i's offset of A row i in the moment grid of width GW: the moment index
+// of monomial alpha_i (lanes past the basis take row 0).
+template <int DIM, int L, int AMIN, int GW>
+__device__ __forceinline__ int mom_row_base(int lane) {
+    constexpr int M = BasisShape<DIM, L, AMIN>::M;
+    const int li = lane < M ? lane : 0;
+    const int e2 = DIM == 3 ? kTab<DIM, L, AMIN>.e[li][2] : 0;
+    return kTab<DIM, L, AMIN>.e[li][0] + GW * kTab<DIM, L, AMIN>.e[li][1] + GW * GW * e2;
+}
+
+template <int LL, int AA>
+struct BasisTag {
+    static constexpr int L = LL, A = AA;
+};
+
+__device__ __forceinline__ int64_t readlane_i64(int64_t v, int lane) {
+    const int lo = __builtin_amdgcn_readlane((int)v, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(v >> 32), lane);
+    return ((int64_t)hi << 32) | (uint32_t)lo;
+}
+
 // Writes the "no kernel" result of one operator for particle p.
 __device__ __forceinline__ void asm_fail(const AsmOp& op, int64_t p, int64_t rs, int n, int status, int lane) {
     for (int j = lane; j < n; j += kWave) {
@@ -104,53 +125,60 @@ __device__ __forceinline__ void asm_fail(const AsmOp& op, int64_t p, int64_t rs,
     }
 }
 
-// One operator of basis <DIM, L, AMIN> for particle p, from the moments in
-// LDS (grid width GW): A rows, Cholesky, solve (+ refinement), kernel weights.
-// The lane (`me`, opaque so lane == k tests are not hoisted out of the
-// particle loop as spilled masks) holds row `me` of A in the factorisation.
-template <int DIM, int L, int AMIN, int GW, typename Disp>
-__device__ __forceinline__ void asm_solve(const AsmArgs& a, const AsmOp& op, int64_t p, int64_t rs, int n,
-                                          double eps, double ieps, const double (&cdv)[3], Disp disp,
-                                          const double* mom, double* lmat, double* lcol, double* sx,
-                                          double b_lane, int lane) {
-    using S = BasisShape<DIM, L, AMIN>;
-    constexpr int M = S::M;
-    static_assert(M <= kWave, "basis larger than a wavefront");
+// Moment-system factorisation and solve of one operator for the particles of
+// SLOTS lane groups at once (LS = 64 / SLOTS lanes each, one particle per
+// group): lane slot * LS + i holds row i of its particle's A = (M(alpha_i +
+// alpha_k))_k, read from the moment grid (width GW) at abase + offset(k).
+// A = L L^T by a right-looking Cholesky; forward and backward substitution;
+// optionally one step of FP64 iterative refinement against the moments. The
+// solution entry x_i is left in sx[lane]; okl is false when a pivot of the
+// lane's system is not positive and finite. The groups share every
+// instruction and read only their own LDS entries, so a failed or absent
+// particle in one group cannot reach another.
+template <int DIM, int L, int AMIN, int GW, int SLOTS>
+__device__ __forceinline__ void asm_factor_solve(const double* mom, int abase, double b_lane, int refine,
+                                                 double* lmat, double* lcol, double* sx, int lane, bool& okl) {
+    constexpr int M = BasisShape<DIM, L, AMIN>::M;
+    constexpr int LS = kWave / SLOTS;
+    static_assert(M <= LS, "basis larger than a lane group");
     constexpr BasisTables<DIM, L, AMIN> E{};
-    if (n < M) {
-        asm_fail(op, p, rs, n, DCPSE_INFO_TOO_FEW, lane);
-        return;
-    }
-    int me = lane;
+    const int slot = lane / LS;
+    const int base = slot * LS;
+    // row index, opaque so the lane == k tests are not hoisted out of the
+    // particle loop as spilled masks
+    int me = lane - base;
     asm volatile("" : "+v"(me));
     const int li = me < M ? me : 0;
-    const int ai0 = kTab<DIM, L, AMIN>.e[li][0];
-    const int ai1 = kTab<DIM, L, AMIN>.e[li][1];
-    const int ai2 = DIM == 3 ? kTab<DIM, L, AMIN>.e[li][2] : 0;
 
     double ar[M];
 #pragma unroll
-    for (int k = 0; k < M; ++k) {
-        const int idx = (ai0 + E.e[k][0]) + GW * (ai1 + E.e[k][1]) + GW * GW * (ai2 + E.e[k][2]);
-        const double v = mom[idx];
-        ar[k] = me < M ? v : 0.0;
-    }
+    for (int k = 0; k < M; ++k) ar[k] = mom[abase + E.e[k][0] + GW * E.e[k][1] + GW * GW * E.e[k][2]];
+
     // ---- A = L L^T, right-looking; lane i holds row i ---------------------
-    // Step k takes the pivot from lane k (v_readlane), scales column k and
-    // publishes it in LDS; every lane then reads the trailing entries l_jk as
-    // broadcasts (one address for the wave) and updates its row. Entries right
-    // of the diagonal are updated too and never read as part of L.
-    // Pivot checks accumulate in vector registers (no per-step scalar masks):
-    // the smallest pivot, and pivot - pivot, which is NaN once a pivot is not
-    // finite; a failed factorisation writes zero weights whatever it computed.
-    double dmin = 1.0, dfin = 0.0;
+    // One group: step k takes the pivot from lane k (v_readlane), scales
+    // column k and publishes it in LDS; every lane reads the trailing entries
+    // l_jk as broadcasts and updates its row. Two groups: step k publishes
+    // the unscaled column, each lane reads its group's pivot and entries from
+    // it, and updates with (a_ik / d_kk) a_jk = l_ik l_jk. Entries right of
+    // the diagonal are updated too and never read as part of L. The pivot
+    // check accumulates in a vector register (no per-step scalar masks); a
+    // failed factorisation writes zero weights whatever it computed.
+    double dfin = 0.0;  // NaN once a pivot is not positive and finite
     double dinv = 0.0;  // lane k: 1 / L_kk
 #pragma unroll
     for (int k = 0; k < M; ++k) {
-        const double dkk = readlane_d(ar[k], k);
-        dmin = fmin(dmin, dkk);
-        dfin += dkk - dkk;
-        asm volatile("" : "+v"(dmin), "+v"(dfin));  // consumed here, not kept as scalars
+        double* col = lcol + (k & 1) * kWave;  // alternate buffers between steps
+        double dkk;
+        double cv[M];  // two groups: the unscaled column k of the lane's group
+        if constexpr (SLOTS == 1) {
+            dkk = readlane_d(ar[k], k);
+        } else {
+            col[lane] = ar[k];
+            wave_lds_sync();
+            dkk = col[base + k];
+#pragma unroll
+            for (int j = k + 1; j < M; ++j) cv[j] = col[base + j];
+        }
         // 1/sqrt(d) from the hardware estimate and two Newton-Raphson steps
         // r <- r (3 - d r^2) / 2 (quadratic convergence: ~23 -> 46 -> full
         // bits): one estimate and a few FMAs instead of a square-root and a
@@ -159,67 +187,99 @@ __device__ __forceinline__ void asm_solve(const AsmArgs& a, const AsmOp& op, int
         const double hd = 0.5 * dkk;
         inv = inv * fma(-hd * inv, inv, 1.5);
         inv = inv * fma(-hd * inv, inv, 1.5);
+        // a pivot that is not positive and finite makes inv NaN (negative or
+        // NaN: the estimate is NaN; zero or +inf: 0 * inf in the Newton steps),
+        // and inv - inv is NaN exactly then
+        dfin += inv - inv;
         dinv = (me == k) ? inv : dinv;
         // lane k's own entry becomes dkk * inv = L_kk
         ar[k] *= inv;
-        const double lik = ar[k];
-        if (k + 1 < M) {
-            double* col = lcol + (k & 1) * kWave;  // alternate buffers between steps
-            col[me] = lik;
-            wave_lds_sync();
+        if constexpr (SLOTS == 1) {
+            const double lik = ar[k];
+            if (k + 1 < M) {
+                col[lane] = lik;
+                wave_lds_sync();
 #pragma unroll
-            for (int j = k + 1; j < M; ++j) ar[j] = fma(-lik, col[j], ar[j]);
+                for (int j = k + 1; j < M; ++j) ar[j] = fma(-lik, col[j], ar[j]);
+            }
+        } else {
+            const double t = ar[k] * inv;
+#pragma unroll
+            for (int j = k + 1; j < M; ++j) ar[j] = fma(-t, cv[j], ar[j]);
         }
         __builtin_amdgcn_sched_barrier(0);
     }
-    const bool ok = dmin > 0.0 && dfin == 0.0;
+    okl = dfin == 0.0;
     // keep the strictly lower part of L (the diagonal is in dinv): a lane's
     // partial sums then stop changing once its unknown is determined, so the
     // substitutions need no per-lane capture of their results
 #pragma unroll
     for (int k = 0; k < M; ++k) ar[k] = k < me ? ar[k] : 0.0;
-    // L rows to LDS (row stride MS): the backward solve reads column `me`
+    // L rows to LDS (row stride MS, one block per group): the backward solve
+    // reads column `me`
     constexpr int MS = M + (M & 1);
+    double* lm = lmat + slot * (M * MS);
     if (me < M) {
 #pragma unroll
-        for (int k = 0; k < M; ++k) lmat[me * MS + k] = ar[k];
+        for (int k = 0; k < M; ++k) lm[me * MS + k] = ar[k];
     }
     wave_lds_sync();
 
+    // unknown k of each group to all lanes of the group
+    auto bcast = [&](double v, int k) -> double {
+        if constexpr (SLOTS == 1) {
+            return readlane_d(v, k);
+        } else {
+            double* buf = lcol + (k & 1) * kWave;
+            buf[lane] = v;
+            wave_lds_sync();
+            return buf[base + k];
+        }
+    };
     // forward (L y = r) and backward (L^T x = y) substitution: step k
-    // broadcasts unknown k from lane k and every lane subtracts its multiple;
-    // lane k's own sum is final from step k on, so it ends holding y_k, then x_k
+    // broadcasts unknown k and every lane subtracts its multiple; lane k's own
+    // sum is final from step k on, so it ends holding y_k, then x_k
     auto chol_solve = [&](double rhs) {
         double r = rhs;
 #pragma unroll
-        for (int k = 0; k < M; ++k) r = fma(-ar[k], readlane_d(r * dinv, k), r);
+        for (int k = 0; k < M; ++k) r = fma(-ar[k], bcast(r * dinv, k), r);
         double t = r * dinv;
 #pragma unroll
-        for (int j = M - 1; j >= 0; --j) t = fma(-lmat[j * MS + li], readlane_d(t * dinv, j), t);
+        for (int j = M - 1; j >= 0; --j) t = fma(-lm[j * MS + li], bcast(t * dinv, j), t);
         return t * dinv;
     };
     double xv = chol_solve(b_lane);
-    if (a.refine) {
+    if (refine) {
         // one step of iterative refinement against the moments: x += A^-1 (b - A x)
-        if (me < M) sx[me] = xv;
+        sx[lane] = xv;
         wave_lds_sync();
         double rr = b_lane;
 #pragma unroll
         for (int k = 0; k < M; ++k) {
-            const int idx = (ai0 + E.e[k][0]) + GW * (ai1 + E.e[k][1]) + GW * GW * (ai2 + E.e[k][2]);
-            const double aik = me < M ? mom[idx] : 0.0;
-            rr = fma(-aik, sx[k], rr);
+            const double aik = mom[abase + E.e[k][0] + GW * E.e[k][1] + GW * GW * E.e[k][2]];
+            rr = fma(-aik, sx[base + k], rr);
         }
         xv += chol_solve(rr);
     }
-    if (me < M) sx[me] = xv;
+    sx[lane] = xv;
     wave_lds_sync();
+}
 
-    // ---- kernel weights ----------------------------------------------------
+// Kernel weights of one operator for particle p (Dcpse.hpp:798-809): lanes
+// own neighbours, K_pq = w_q * sum_i a_i z_q^alpha_i with the solution a in
+// xs (wave-uniform), written with the eps^-|beta| pre-scaled copies; the
+// displacements of the first 64 neighbours come from dsl[d * 64 + lane].
+template <int DIM, int L, int AMIN, typename Disp>
+__device__ __forceinline__ void asm_weights(const AsmOp& op, int64_t p, int64_t rs, int n, double eps, double ieps,
+                                            bool ok, const double* xs, const double* dsl, Disp disp, int lane) {
+    constexpr int M = BasisShape<DIM, L, AMIN>::M;
+    constexpr BasisTables<DIM, L, AMIN> E{};
     const double eip = 1.0 / ipow_rt(eps, op.border);
     bool bad = false;
     for (int j = lane; j < n; j += kWave) {
-        double dv[3] = {cdv[0], cdv[1], cdv[2]};
+        double dv[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) dv[d] = dsl[d * kWave + lane];
         if (j >= kWave) disp(j, dv);
         double z[3] = {0.0, 0.0, 0.0};
         double z2 = 0.0;
@@ -242,7 +302,7 @@ __device__ __forceinline__ void asm_solve(const AsmArgs& a, const AsmOp& op, int
         for (int k = 0; k < M; ++k) {
             double v = px[E.e[k][0]] * py[E.e[k][1]];
             if (DIM == 3) v *= pz[E.e[k][2]];
-            sum = fma(sx[k], v, sum);
+            sum = fma(xs[k], v, sum);
         }
         const double K = ok ? sum * w : 0.0;
         bad |= !isfinite(K);
@@ -256,37 +316,57 @@ __device__ __forceinline__ void asm_solve(const AsmArgs& a, const AsmOp& op, int
         if (op.eps_inv_pow) op.eps_inv_pow[p] = eip;
         if (op.info) op.info[p] = !ok ? DCPSE_INFO_NOT_SPD : (anybad ? DCPSE_INFO_NONFINITE : DCPSE_INFO_OK);
     }
-    wave_lds_sync();
 }
 
-// One wavefront per particle; a block of one wavefront walks a.ppb particles.
-// Operator 0 has basis <DIM, L1, A1>; with L2 > 0 a second operator of basis
-// <DIM, L2, A2> on the same support is assembled in the same pass, sharing ε,
-// the neighbour displacements and the moments (the moment set covers both).
+// Waves per SIMD each instantiation is compiled for: the largest bases need
+// the whole register file of one wave; up to 10 monomials three waves fit.
+template <int DIM, int L1, int A1, int L2, int A2>
+constexpr int asm_waves() {
+    constexpr int m1 = BasisShape<DIM, L1, A1>::M;
+    constexpr int m2 = L2 > 0 ? BasisShape<DIM, (L2 > 0 ? L2 : 2), A2>::M : 0;
+    constexpr int m = m1 > m2 ? m1 : m2;
+    return m > 24 ? 1 : (m > 10 ? 2 : 3);
+}
+
+// One wavefront walks a.ppb particles (at most 63). Operator 0 has basis
+// <DIM, L1, A1>; with L2 > 0 a second operator of basis <DIM, L2, A2> on the
+// same support is assembled in the same pass, sharing ε, the neighbour
+// displacements and the moments (the moment set covers both). ε and the
+// moments are computed one particle at a time by the whole wave; when the
+// bases fit in half a wave (m <= 32), two particles' systems are factored and
+// solved together, one per half.
 template <int DIM, int L1, int A1, int L2 = 0, int A2 = 0>
-__global__ __launch_bounds__(64) void k_assemble(const AsmArgs a) {
+__global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_assemble(const AsmArgs a) {
     constexpr int P1 = 2 * (L1 - 1), P2 = L2 > 0 ? 2 * (L2 - 1) : 0;
     constexpr int P = P1 > P2 ? P1 : P2;
     constexpr int QMIN = L2 > 0 ? (A1 < A2 ? 2 * A1 : 2 * A2) : 2 * A1;
     constexpr int GW = P + 1, MG = ipow_c(GW, DIM);  // moment grid M(a, b, c) at a + GW b + GW^2 c
+    constexpr int MGP = MG + (MG & 1);
     constexpr int M1 = BasisShape<DIM, L1, A1>::M;
     constexpr int M2 = L2 > 0 ? BasisShape<DIM, (L2 > 0 ? L2 : 2), A2>::M : 0;
     constexpr int MX = M1 > M2 ? M1 : M2;
     constexpr int MN = L2 > 0 ? (M1 < M2 ? M1 : M2) : M1;
+    constexpr int SLOTS = MX <= kWave / 2 ? 2 : 1;  // particles factored together
+    constexpr int LS = kWave / SLOTS;
+    constexpr int MSX = MX + (MX & 1);
     constexpr int PW = P + 1;
     // power-table row per staged neighbour: [x][y^0..y^P][w z^0..w z^P]
     // (2D: [x][y^0..y^P][w])
     constexpr int ZW = DIM == 3 ? PW : 1;
     constexpr int TROW = 1 + PW + ZW;
-    constexpr int TAB = (kChunk * TROW > MX * (MX + 1)) ? kChunk * TROW : MX * (MX + 1);
+    constexpr int TAB = (kChunk * TROW > SLOTS * MX * MSX) ? kChunk * TROW : SLOTS * MX * MSX;
     constexpr int NPAIR = PairTable<DIM, P>::N;
-    constexpr int HALVES = NPAIR <= kWave / 2 ? 2 : 1;  // neighbours per pass
-    constexpr int LPN = kWave / HALVES;                 // lanes per neighbour
+    // lanes per neighbour: the smallest power of two covering the pairs; a
+    // pass covers HALVES = 64 / LPN neighbours
+    constexpr int LPN = NPAIR <= 8 ? 8 : (NPAIR <= 16 ? 16 : (NPAIR <= 32 ? 32 : 64));
+    constexpr int HALVES = kWave / LPN;
+    static_assert(kChunk % HALVES == 0, "a chunk holds whole passes");
     static_assert(NPAIR <= kWave, "moment pairs exceed a wavefront");
-    __shared__ double tab[TAB];
-    __shared__ double mom[MG];
-    __shared__ double lcol[2 * kWave];  // column k of L, broadcast to the wave
-    __shared__ double sx[MX];           // solution a (uniform across the wave)
+    __shared__ double tab[TAB];                   // staged powers; L factors in the solve
+    __shared__ double mom[SLOTS * MGP];           // moments of each slot's particle
+    __shared__ double lcol[2 * kWave];            // broadcast buffers of the factorisation
+    __shared__ double sx[kWave];                  // solutions, one entry per lane
+    __shared__ double dsl[SLOTS * 3 * kWave];     // first 64 displacements per slot
     const int lane = threadIdx.x;
 
     // the (b, c) pair this lane accumulates, and which neighbour of a pass
@@ -296,17 +376,41 @@ __global__ __launch_bounds__(64) void k_assemble(const AsmArgs a) {
     const int pb = kPair<DIM, P>.bc[pv ? pl : 0][0];
     const int pc = kPair<DIM, P>.bc[pv ? pl : 0][1];
 
-    // lane i: entry i of each operator's right-hand side
-    const double b1 = lane < M1 ? a.op[0].b[lane] : 0.0;
-    const double b2 = L2 > 0 && lane < M2 ? a.op[1].b[lane] : 0.0;
+    // lane slot * LS + i: row i of each operator's system (right-hand side
+    // entry and offset of A row i in its slot's moment grid)
+    const int lslot = lane / LS, lrow = lane % LS;
+    const double b1 = lrow < M1 ? a.op[0].b[lrow] : 0.0;
+    const double b2 = L2 > 0 && lrow < M2 ? a.op[1].b[lrow] : 0.0;
+    const int ab1 = lslot * MGP + mom_row_base<DIM, L1, A1, GW>(lrow);
+    const int ab2 = L2 > 0 ? lslot * MGP + mom_row_base<DIM, (L2 > 0 ? L2 : 2), A2, GW>(lrow) : 0;
 
+    // The block's particles: lane i holds row_ptr[pbeg + i] (i <= pcnt) and
+    // the position of particle pbeg + i, loaded once; a particle takes its
+    // values by v_readlane. Neighbour data is software-pipelined: while
+    // particle i's ε and moments are computed, the positions of the first 64
+    // neighbours of particle i + 1 and the columns of particle i + 2 are in
+    // flight, and are first used one particle later.
     const int64_t pbeg = (int64_t)blockIdx.x * a.ppb;
-    const int64_t pend = min(pbeg + (int64_t)a.ppb, a.n_dom);
-    // Particles of the block are software-pipelined: while particle p is
-    // assembled, the row bounds, first 64 columns and their displacements of
-    // particle p + 1 are loaded (row bounds at the top, columns after the eps
-    // pass, positions after the moments), so their memory latency overlaps the
-    // moments and the factorisation of p.
+    const int pcnt = (int)max((int64_t)0, min((int64_t)a.ppb, a.n_dom - pbeg));
+    const int64_t rpv = lane <= pcnt ? a.row_ptr[pbeg + lane] : 0;
+    double xpv[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) xpv[d] = lane < pcnt ? a.pos_dom[d * a.n_dom + pbeg + lane] : 0.0;
+    auto row_of = [&](int i, int64_t& rs_, int& n_, double* xp_) {
+        rs_ = readlane_i64(rpv, i);
+        n_ = (int)(readlane_i64(rpv, i + 1) - rs_);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) xp_[d] = d < DIM ? readlane_d(xpv[d], i) : 0.0;
+    };
+    // columns of particle i's first 64 neighbours (lanes past its count repeat
+    // its last column)
+    auto load_cols = [&](int i) -> unsigned {
+        if (i >= pcnt) return 0u;
+        const int64_t rs_ = readlane_i64(rpv, i);
+        const int n_ = (int)(readlane_i64(rpv, i + 1) - rs_);
+        if (n_ <= 0) return 0u;
+        return (unsigned)a.col[rs_ + min(lane, n_ - 1)];
+    };
     auto load_disp = [&](int64_t rs_, int n_, const double* xp_, int j, double* dv) {
         if (j < n_) {
             const int q = a.col[rs_ + j];
@@ -319,68 +423,97 @@ __global__ __launch_bounds__(64) void k_assemble(const AsmArgs a) {
     int n = 0;
     double xp[3] = {0.0, 0.0, 0.0};
     double cdv[3] = {0.0, 0.0, 0.0};  // lane j: displacement of neighbour j (j < 64)
-    if (pbeg < pend) {
-        rs = a.row_ptr[pbeg];
-        n = (int)(a.row_ptr[pbeg + 1] - rs);
+    unsigned q_n = 0;                 // lane j: column of neighbour j of the next particle
+    if (pcnt > 0) {
+        row_of(0, rs, n, xp);
+        const unsigned q0 = load_cols(0);
+        if (n > 0) {
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) xp[d] = a.pos_dom[d * a.n_dom + pbeg];
-        load_disp(rs, n, xp, lane, cdv);
+            for (int d = 0; d < DIM; ++d)
+                cdv[d] = min_image(xp[d] - a.pos_sup[(size_t)d * a.n_sup + q0], a.len[d], (a.periodic >> d) & 1u);
+        }
+        q_n = load_cols(1);
     }
-    for (int64_t p = pbeg; p < pend; ++p) {
-        const bool has_next = p + 1 < pend;
-        int64_t rs_n = 0;
-        int n_n = 0;
-        double xp_n[3] = {0.0, 0.0, 0.0};
-        if (has_next) {
-            rs_n = a.row_ptr[p + 1];
-            n_n = (int)(a.row_ptr[p + 2] - rs_n);
+    for (int i = 0; i < pcnt; i += SLOTS) {
+        // per slot: row, position, ε and whether its operators are assembled
+        int64_t rs_s[SLOTS];
+        int n_s[SLOTS];
+        double xp_s[SLOTS][3];
+        double eps_s[SLOTS], ieps_s[SLOTS];
+        bool go_s[SLOTS];
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) xp_n[d] = a.pos_dom[d * a.n_dom + p + 1];
-        }
-        auto disp = [&](int j, double* dv) { load_disp(rs, n, xp, j, dv); };
+        for (int s = 0; s < SLOTS; ++s) {
+            rs_s[s] = rs;
+            n_s[s] = n;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) xp_s[s][d] = xp[d];
+            eps_s[s] = ieps_s[s] = 0.0;
+            go_s[s] = false;
+            const int ii = i + s;
+            if (ii >= pcnt) {
+                n_s[s] = 0;
+                continue;
+            }
+            const int64_t p = pbeg + ii;
+            int64_t rs_n = 0;
+            int n_n = 0;
+            double xp_n[3] = {0.0, 0.0, 0.0};
+            if (ii + 1 < pcnt) row_of(ii + 1, rs_n, n_n, xp_n);
+            double pq_n[3] = {0.0, 0.0, 0.0};
+            if (n_n > 0) {
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) pq_n[d] = a.pos_sup[(size_t)d * a.n_sup + q_n];
+            }
+            const unsigned q_nn = load_cols(ii + 2);
+            auto disp = [&](int j, double* dv) { load_disp(rs, n, xp, j, dv); };
 
-        // ---- eps -------------------------------------------------------------
-        double l1 = 0.0;
-        for (int j = lane; j < n; j += kWave) {
-            double dv[3] = {cdv[0], cdv[1], cdv[2]};
-            if (j >= kWave) disp(j, dv);
-            double s = 0.0;
+            // ---- eps -------------------------------------------------------------
+            double l1 = 0.0;
+            for (int j = lane; j < n; j += kWave) {
+                double dv[3] = {cdv[0], cdv[1], cdv[2]};
+                if (j >= kWave) disp(j, dv);
+                double sa = 0.0;
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) s += fabs(dv[d]);
-            l1 += s;
-        }
-        l1 = wave_sum(l1);
-        const double eps = n > 0 ? (l1 / n) / a.h_over_eps : 0.0;
-        const bool eps_bad = !(eps > 0.0) || !isfinite(eps);
-        // next particle: first 64 columns
-        const int q_n = has_next && lane < n_n ? a.col[rs_n + lane] : 0;
-        if (n < MN || eps_bad) {  // no operator of the pass can be assembled
-            const int st0 = n < M1 ? DCPSE_INFO_TOO_FEW : DCPSE_INFO_EPS_ZERO;
-            asm_fail(a.op[0], p, rs, n, st0, lane);
-            if (L2 > 0) asm_fail(a.op[1], p, rs, n, n < M2 ? DCPSE_INFO_TOO_FEW : DCPSE_INFO_EPS_ZERO, lane);
-        } else {
-            const double ieps = 1.0 / eps;
+                for (int d = 0; d < DIM; ++d) sa += fabs(dv[d]);
+                l1 += sa;
+            }
+            l1 = wave_sum(l1);
+            const double eps = n > 0 ? (l1 / n) / a.h_over_eps : 0.0;
+            const bool eps_bad = !(eps > 0.0) || !isfinite(eps);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) dsl[(s * 3 + d) * kWave + lane] = cdv[d];
+            if (n < MN || eps_bad) {  // no operator of the pass can be assembled
+                const int st0 = n < M1 ? DCPSE_INFO_TOO_FEW : DCPSE_INFO_EPS_ZERO;
+                asm_fail(a.op[0], p, rs, n, st0, lane);
+                if (L2 > 0) asm_fail(a.op[1], p, rs, n, n < M2 ? DCPSE_INFO_TOO_FEW : DCPSE_INFO_EPS_ZERO, lane);
+            } else {
+                const double ieps = 1.0 / eps;
+                eps_s[s] = eps;
+                ieps_s[s] = ieps;
+                go_s[s] = true;
 
-            // ---- moments -----------------------------------------------------------
-            // M(a, b, c) = sum_q w_q x_q^a y_q^b z_q^c (x, y, z = scaled displacement).
-            // Lanes own (b, c) pairs and read y^b and w z^c of the staged neighbour
-            // from LDS, forming the x powers in registers; a pass covers HALVES
-            // neighbours (one per half-wavefront), so each LDS read serves the
-            // whole moment column of its pair.
-            double acc[PW];
+                // ---- moments -----------------------------------------------------------
+                // M(a, b, c) = sum_q w_q x_q^a y_q^b z_q^c (x, y, z = scaled displacement).
+                // Lanes own (b, c) pairs and read y^b and w z^c of the staged neighbour
+                // from LDS, forming the x powers in registers; a pass covers HALVES
+                // neighbours (one per group of LPN lanes), so each LDS read serves
+                // the whole moment column of its pair.
+                double acc[PW];
 #pragma unroll
-            for (int k = 0; k < PW; ++k) acc[k] = 0.0;
-            for (int c0 = 0; c0 < n; c0 += kChunk) {
-                const int cnt = min(kChunk, n - c0);
-                const int jl = lane & (kChunk - 1);
-                double dv[3];
+                for (int k = 0; k < PW; ++k) acc[k] = 0.0;
+                for (int c0 = 0; c0 < n; c0 += kChunk) {
+                    const int cnt = min(kChunk, n - c0);
+                    const int jl = lane & (kChunk - 1);
+                    double dv[3];
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) dv[d] = __shfl(cdv[d], (c0 + jl) & (kWave - 1), kWave);
-                if (jl < cnt) {
-                    if (c0 + jl >= kWave) disp(c0 + jl, dv);
+                    for (int d = 0; d < DIM; ++d) dv[d] = __shfl(cdv[d], (c0 + jl) & (kWave - 1), kWave);
+                    // rows past cnt are written as zero neighbours (x = 0, w = 0):
+                    // the passes then read every row of the chunk unconditionally
+                    const bool vj = jl < cnt;
+                    if (vj && c0 + jl >= kWave) disp(c0 + jl, dv);
                     double z[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-                    for (int d = 0; d < DIM; ++d) z[d] = dv[d] * ieps;
+                    for (int d = 0; d < DIM; ++d) z[d] = vj ? dv[d] * ieps : 0.0;
                     double* row = tab + jl * TROW;
                     if (lane < kChunk) {
                         row[0] = z[0];
@@ -394,66 +527,85 @@ __global__ __launch_bounds__(64) void k_assemble(const AsmArgs a) {
                         double z2 = 0.0;
 #pragma unroll
                         for (int d = 0; d < DIM; ++d) z2 += z[d] * z[d];
-                        double pw = exp(-z2);
+                        double pw = vj ? exp(-z2) : 0.0;
 #pragma unroll
                         for (int k = 0; k < ZW; ++k) {
                             row[1 + PW + k] = pw;
                             pw *= z[DIM - 1];
                         }
                     }
-                }
-                wave_lds_sync();
+                    wave_lds_sync();
+                    // lanes of an invalid pair accumulate pair 0 and are not stored
 #pragma unroll 2
-                for (int j = 0; j < cnt; j += HALVES) {
-                    const int jj = j + half;
-                    const bool v = pv && jj < cnt;
-                    const double* row = tab + (jj < kChunk ? jj : 0) * TROW;
-                    // rows past cnt hold another neighbour or nothing yet: both
-                    // factors are replaced, so no stale value reaches the sums
-                    const double x = v ? row[0] : 0.0;
-                    double t = row[1 + pb] * row[1 + PW + pc];
-                    t = v ? t : 0.0;
+                    for (int j = 0; j < cnt; j += HALVES) {
+                        const double* rw = tab + (j + half) * TROW;
+                        const double x = rw[0];
+                        double t = rw[1 + pb] * rw[1 + PW + pc];
+#pragma unroll
+                        for (int k = 0; k < PW; ++k) {
+                            acc[k] += t;
+                            t *= x;
+                        }
+                    }
+                    wave_lds_sync();
+                }
+#pragma unroll
+                for (int off = kWave / 2; off >= LPN; off /= 2) {
+#pragma unroll
+                    for (int k = 0; k < PW; ++k) acc[k] += __shfl_down(acc[k], off, kWave);
+                }
+                if (pv && half == 0) {
+                    double* ms = mom + s * MGP;
 #pragma unroll
                     for (int k = 0; k < PW; ++k) {
-                        acc[k] += t;
-                        t *= x;
+                        const int sm = k + pb + pc;
+                        if (sm >= QMIN && sm <= P) ms[k + GW * pb + GW * GW * pc] = acc[k];
                     }
                 }
                 wave_lds_sync();
             }
-            if (HALVES == 2) {
+            // next particle: displacements of its first 64 neighbours
 #pragma unroll
-                for (int k = 0; k < PW; ++k) acc[k] += __shfl_down(acc[k], LPN, kWave);
+            for (int d = 0; d < 3; ++d) {
+                cdv[d] = d < DIM ? min_image(xp_n[d] - pq_n[d], a.len[d], (a.periodic >> d) & 1u) : 0.0;
+                xp[d] = xp_n[d];
             }
-            if (pv && half == 0) {
+            rs = rs_n;
+            n = n_n;
+            q_n = q_nn;
+        }
+
+        // ---- per operator: factorisation and solve of every slot, weights ----
+        auto weights = [&](auto tag, const AsmOp& op, int m, int abase, double b_lane) {
+            constexpr int LL = decltype(tag)::L, AA = decltype(tag)::A;
+            bool any = false;
 #pragma unroll
-                for (int k = 0; k < PW; ++k) {
-                    const int s = k + pb + pc;
-                    if (s >= QMIN && s <= P) mom[k + GW * pb + GW * GW * pc] = acc[k];
+            for (int s = 0; s < SLOTS; ++s) any |= go_s[s] && n_s[s] >= m;
+            if (!any) {
+#pragma unroll
+                for (int s = 0; s < SLOTS; ++s)
+                    if (go_s[s]) asm_fail(op, pbeg + i + s, rs_s[s], n_s[s], DCPSE_INFO_TOO_FEW, lane);
+                return;
+            }
+            bool okl;
+            asm_factor_solve<DIM, LL, AA, GW, SLOTS>(mom, abase, b_lane, a.refine, tab, lcol, sx, lane, okl);
+            const uint64_t okm = __ballot(okl);
+#pragma unroll
+            for (int s = 0; s < SLOTS; ++s) {
+                if (!go_s[s]) continue;
+                const int64_t p = pbeg + i + s;
+                if (n_s[s] < m) {
+                    asm_fail(op, p, rs_s[s], n_s[s], DCPSE_INFO_TOO_FEW, lane);
+                    continue;
                 }
+                auto disp = [&](int j, double* dv) { load_disp(rs_s[s], n_s[s], xp_s[s], j, dv); };
+                asm_weights<DIM, LL, AA>(op, p, rs_s[s], n_s[s], eps_s[s], ieps_s[s], (okm >> (s * LS)) & 1ull,
+                                         sx + s * LS, dsl + s * 3 * kWave, disp, lane);
             }
             wave_lds_sync();
-
-            // ---- per operator: factorisation, solve, kernel weights ------------------
-            asm_solve<DIM, L1, A1, GW>(a, a.op[0], p, rs, n, eps, ieps, cdv, disp, mom, tab, lcol, sx, b1, lane);
-            if constexpr (L2 > 0)
-                asm_solve<DIM, L2, A2, GW>(a, a.op[1], p, rs, n, eps, ieps, cdv, disp, mom, tab, lcol, sx, b2,
-                                           lane);
-        }
-        // next particle: displacements of its first 64 neighbours
-        double dv_n[3] = {0.0, 0.0, 0.0};
-        if (has_next && lane < n_n) {
-#pragma unroll
-            for (int d = 0; d < DIM; ++d)
-                dv_n[d] = min_image(xp_n[d] - a.pos_sup[d * a.n_sup + q_n], a.len[d], (a.periodic >> d) & 1u);
-        }
-        rs = rs_n;
-        n = n_n;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            xp[d] = xp_n[d];
-            cdv[d] = dv_n[d];
-        }
+        };
+        weights(BasisTag<L1, A1>{}, a.op[0], M1, ab1, b1);
+        if constexpr (L2 > 0) weights(BasisTag<(L2 > 0 ? L2 : 2), A2>{}, a.op[1], M2, ab2, b2);
     }
 }
 
@@ -643,7 +795,7 @@ static dcpse_status assemble_ops(dcpse_ctx* ctx, int32_t dim, const double* pos_
     args.periodic = periodic_mask;
     args.h_over_eps = h_over_eps;
     args.refine = (flags & DCPSE_ASM_REFINE) ? 1 : 0;
-    args.ppb = std::max(1, ctx->asm_ppb);
+    args.ppb = std::min(63, std::max(1, ctx->asm_ppb));  // one lane per particle's row bound
     if (n_dom == 0) return DCPSE_OK;
     const int64_t blocks = (n_dom + args.ppb - 1) / args.ppb;
     // operators in pairs sharing one pass where a two-operator kernel exists

@@ -322,7 +322,9 @@ def test_sell_bundle16_far_neighbours(oracle, cuda):
 @pytest.mark.parametrize("name", sorted(WORKLOADS))
 def test_assemble_together_equals_separate(name, cuda):
     """Operators assembled in one pass (shared gathers, epsilon, moments) give
-    the kernels of separate assemblies, bit for bit."""
+    the kernels of separate assemblies: epsilon bit for bit; the kernels to
+    rounding, since an operator with a smaller basis sums its moments in
+    another order when assembled alone (more neighbours per moment pass)."""
     from openfpm_numerics_amd import Derivative_x, Derivative_y, Laplacian, workloads as W
     from openfpm_numerics_amd.dcpse import LOAD, assemble_together
 
@@ -335,7 +337,9 @@ def test_assemble_together_equals_separate(name, cuda):
            Derivative_y(ps, vl, 2, w["rcut"], LOAD).dcpse]
     assemble_together(tog)
     for a, b in zip(sep, tog):
-        assert torch.equal(a.calcKernels, b.calcKernels), name
+        scale = float(a.calcKernels.abs().max())
+        assert float((a.calcKernels - b.calcKernels).abs().max()) <= 1e-13 * scale, name
         assert torch.equal(a.localEps, b.localEps) and torch.equal(a.localEpsInvPow, b.localEpsInvPow)
-        assert torch.equal(a.weights(torch.float32), b.weights(torch.float32))
+        w32a, w32b = a.weights(torch.float32), b.weights(torch.float32)
+        assert float((w32a - w32b).abs().max()) <= 2.0 ** -22 * float(w32a.abs().max()), name
         assert int(b.info.abs().max()) == 0
