@@ -47,6 +47,7 @@ struct AsmArgs {
     const int64_t* row_ptr;
     const int32_t* col;
     double len[3];
+    double ilen[3];  // 1 / len
     unsigned periodic;
     double h_over_eps;
     int refine;
@@ -137,7 +138,7 @@ __device__ __forceinline__ void asm_fail(const AsmOp& op, int64_t p, int64_t rs,
 // particle in one group cannot reach another.
 template <int DIM, int L, int AMIN, int GW, int SLOTS>
 __device__ __forceinline__ void asm_factor_solve(const double* mom, int abase, double b_lane, int refine,
-                                                 double* lmat, double* lcol, double* sx, int lane, bool& okl) {
+                                                 double* lmat, double2* lbuf, double* sx, int lane, bool& okl) {
     constexpr int M = BasisShape<DIM, L, AMIN>::M;
     constexpr int LS = kWave / SLOTS;
     static_assert(M <= LS, "basis larger than a lane group");
@@ -158,26 +159,35 @@ __device__ __forceinline__ void asm_factor_solve(const double* mom, int abase, d
     // One group: step k takes the pivot from lane k (v_readlane), scales
     // column k and publishes it in LDS; every lane reads the trailing entries
     // l_jk as broadcasts and updates its row. Two groups: step k publishes
-    // the unscaled column, each lane reads its group's pivot and entries from
-    // it, and updates with (a_ik / d_kk) a_jk = l_ik l_jk. Entries right of
+    // the unscaled column (with the lane's right-hand side entry), each lane
+    // reads its group's pivot and entries from it, and updates with
+    // (a_ik / d_kk) a_jk = l_ik l_jk. Entries right of
     // the diagonal are updated too and never read as part of L. The pivot
     // check accumulates in a vector register (no per-step scalar masks); a
     // failed factorisation writes zero weights whatever it computed.
     double dfin = 0.0;  // NaN once a pivot is not positive and finite
     double dinv = 0.0;  // lane k: 1 / L_kk
+    // two groups: the right-hand side rides along as column M of A, so the
+    // factorisation also performs the forward substitution L y = b (lane k
+    // keeps y_k)
+    double rb = b_lane, yb = 0.0;
 #pragma unroll
     for (int k = 0; k < M; ++k) {
-        double* col = lcol + (k & 1) * kWave;  // alternate buffers between steps
-        double dkk;
+        // alternate buffers between steps
+        double* col = reinterpret_cast<double*>(lbuf) + (k & 1) * kWave;
+        double2* col2 = lbuf + (k & 1) * kWave;
+        double dkk, rk = 0.0;
         double cv[M];  // two groups: the unscaled column k of the lane's group
         if constexpr (SLOTS == 1) {
             dkk = readlane_d(ar[k], k);
         } else {
-            col[lane] = ar[k];
+            col2[lane] = make_double2(ar[k], rb);
             wave_lds_sync();
-            dkk = col[base + k];
+            const double2 pk = col2[base + k];
+            dkk = pk.x;
+            rk = pk.y;
 #pragma unroll
-            for (int j = k + 1; j < M; ++j) cv[j] = col[base + j];
+            for (int j = k + 1; j < M; ++j) cv[j] = col2[base + j].x;
         }
         // 1/sqrt(d) from the hardware estimate and two Newton-Raphson steps
         // r <- r (3 - d r^2) / 2 (quadratic convergence: ~23 -> 46 -> full
@@ -204,6 +214,9 @@ __device__ __forceinline__ void asm_factor_solve(const double* mom, int abase, d
             }
         } else {
             const double t = ar[k] * inv;
+            const double yk = rk * inv;
+            yb = (me == k) ? yk : yb;
+            rb = fma(-ar[k], yk, rb);
 #pragma unroll
             for (int j = k + 1; j < M; ++j) ar[j] = fma(-t, cv[j], ar[j]);
         }
@@ -225,30 +238,30 @@ __device__ __forceinline__ void asm_factor_solve(const double* mom, int abase, d
     }
     wave_lds_sync();
 
-    // unknown k of each group to all lanes of the group
+    // unknown k of each group to all lanes of the group (v_readlane per group)
     auto bcast = [&](double v, int k) -> double {
         if constexpr (SLOTS == 1) {
             return readlane_d(v, k);
         } else {
-            double* buf = lcol + (k & 1) * kWave;
-            buf[lane] = v;
-            wave_lds_sync();
-            return buf[base + k];
+            const double y0 = readlane_d(v, k), y1 = readlane_d(v, LS + k);
+            return slot ? y1 : y0;
         }
     };
     // forward (L y = r) and backward (L^T x = y) substitution: step k
     // broadcasts unknown k and every lane subtracts its multiple; lane k's own
     // sum is final from step k on, so it ends holding y_k, then x_k
-    auto chol_solve = [&](double rhs) {
-        double r = rhs;
-#pragma unroll
-        for (int k = 0; k < M; ++k) r = fma(-ar[k], bcast(r * dinv, k), r);
-        double t = r * dinv;
+    auto back_solve = [&](double t) {
 #pragma unroll
         for (int j = M - 1; j >= 0; --j) t = fma(-lm[j * MS + li], bcast(t * dinv, j), t);
         return t * dinv;
     };
-    double xv = chol_solve(b_lane);
+    auto chol_solve = [&](double rhs) {
+        double r = rhs;
+#pragma unroll
+        for (int k = 0; k < M; ++k) r = fma(-ar[k], bcast(r * dinv, k), r);
+        return back_solve(r * dinv);
+    };
+    double xv = SLOTS == 1 ? chol_solve(b_lane) : back_solve(yb);
     if (refine) {
         // one step of iterative refinement against the moments: x += A^-1 (b - A x)
         sx[lane] = xv;
@@ -268,10 +281,12 @@ __device__ __forceinline__ void asm_factor_solve(const double* mom, int abase, d
 // Kernel weights of one operator for particle p (Dcpse.hpp:798-809): lanes
 // own neighbours, K_pq = w_q * sum_i a_i z_q^alpha_i with the solution a in
 // xs (wave-uniform), written with the eps^-|beta| pre-scaled copies; the
-// displacements of the first 64 neighbours come from dsl[d * 64 + lane].
+// displacements and weights w of the first 64 neighbours come from
+// dsl[d * 64 + lane] and wsl[lane] (as the moment pass computed them).
 template <int DIM, int L, int AMIN, typename Disp>
 __device__ __forceinline__ void asm_weights(const AsmOp& op, int64_t p, int64_t rs, int n, double eps, double ieps,
-                                            bool ok, const double* xs, const double* dsl, Disp disp, int lane) {
+                                            bool ok, const double* xs, const double* dsl, const double* wsl,
+                                            Disp disp, int lane) {
     constexpr int M = BasisShape<DIM, L, AMIN>::M;
     constexpr BasisTables<DIM, L, AMIN> E{};
     const double eip = 1.0 / ipow_rt(eps, op.border);
@@ -288,7 +303,7 @@ __device__ __forceinline__ void asm_weights(const AsmOp& op, int64_t p, int64_t 
             z[d] = dv[d] * ieps;
             z2 += z[d] * z[d];
         }
-        const double w = exp(-z2);
+        const double w = j < kWave ? wsl[j] : exp(-z2);
         double px[L], py[L], pz[L];
         px[0] = py[0] = pz[0] = 1.0;
 #pragma unroll
@@ -364,9 +379,10 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
     static_assert(NPAIR <= kWave, "moment pairs exceed a wavefront");
     __shared__ double tab[TAB];                   // staged powers; L factors in the solve
     __shared__ double mom[SLOTS * MGP];           // moments of each slot's particle
-    __shared__ double lcol[2 * kWave];            // broadcast buffers of the factorisation
+    __shared__ double2 lbuf[2 * kWave];           // broadcast buffers of the factorisation
     __shared__ double sx[kWave];                  // solutions, one entry per lane
     __shared__ double dsl[SLOTS * 3 * kWave];     // first 64 displacements per slot
+    __shared__ double wsl[SLOTS * kWave];         // and their weights exp(-|z|^2)
     const int lane = threadIdx.x;
 
     // the (b, c) pair this lane accumulates, and which neighbour of a pass
@@ -416,7 +432,7 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
             const int q = a.col[rs_ + j];
 #pragma unroll
             for (int d = 0; d < DIM; ++d)
-                dv[d] = min_image(xp_[d] - a.pos_sup[d * a.n_sup + q], a.len[d], (a.periodic >> d) & 1u);
+                dv[d] = min_image_r(xp_[d] - a.pos_sup[d * a.n_sup + q], a.len[d], a.ilen[d], (a.periodic >> d) & 1u);
         }
     };
     int64_t rs = 0;
@@ -430,7 +446,8 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
         if (n > 0) {
 #pragma unroll
             for (int d = 0; d < DIM; ++d)
-                cdv[d] = min_image(xp[d] - a.pos_sup[(size_t)d * a.n_sup + q0], a.len[d], (a.periodic >> d) & 1u);
+                cdv[d] = min_image_r(xp[d] - a.pos_sup[(size_t)d * a.n_sup + q0], a.len[d], a.ilen[d],
+                                    (a.periodic >> d) & 1u);
         }
         q_n = load_cols(1);
     }
@@ -528,6 +545,7 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
 #pragma unroll
                         for (int d = 0; d < DIM; ++d) z2 += z[d] * z[d];
                         double pw = vj ? exp(-z2) : 0.0;
+                        if (c0 + jl < kWave) wsl[s * kWave + c0 + jl] = pw;
 #pragma unroll
                         for (int k = 0; k < ZW; ++k) {
                             row[1 + PW + k] = pw;
@@ -567,7 +585,7 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
             // next particle: displacements of its first 64 neighbours
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                cdv[d] = d < DIM ? min_image(xp_n[d] - pq_n[d], a.len[d], (a.periodic >> d) & 1u) : 0.0;
+                cdv[d] = d < DIM ? min_image_r(xp_n[d] - pq_n[d], a.len[d], a.ilen[d], (a.periodic >> d) & 1u) : 0.0;
                 xp[d] = xp_n[d];
             }
             rs = rs_n;
@@ -588,7 +606,7 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
                 return;
             }
             bool okl;
-            asm_factor_solve<DIM, LL, AA, GW, SLOTS>(mom, abase, b_lane, a.refine, tab, lcol, sx, lane, okl);
+            asm_factor_solve<DIM, LL, AA, GW, SLOTS>(mom, abase, b_lane, a.refine, tab, lbuf, sx, lane, okl);
             const uint64_t okm = __ballot(okl);
 #pragma unroll
             for (int s = 0; s < SLOTS; ++s) {
@@ -600,7 +618,7 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
                 }
                 auto disp = [&](int j, double* dv) { load_disp(rs_s[s], n_s[s], xp_s[s], j, dv); };
                 asm_weights<DIM, LL, AA>(op, p, rs_s[s], n_s[s], eps_s[s], ieps_s[s], (okm >> (s * LS)) & 1ull,
-                                         sx + s * LS, dsl + s * 3 * kWave, disp, lane);
+                                         sx + s * LS, dsl + s * 3 * kWave, wsl + s * kWave, disp, lane);
             }
             wave_lds_sync();
         };
@@ -791,7 +809,10 @@ static dcpse_status assemble_ops(dcpse_ctx* ctx, int32_t dim, const double* pos_
     args.n_dom = n_dom;
     args.row_ptr = row_ptr;
     args.col = col;
-    for (int d = 0; d < 3; ++d) args.len[d] = (d < dim && box_lo && box_hi) ? box_hi[d] - box_lo[d] : 1.0;
+    for (int d = 0; d < 3; ++d) {
+        args.len[d] = (d < dim && box_lo && box_hi) ? box_hi[d] - box_lo[d] : 1.0;
+        args.ilen[d] = 1.0 / args.len[d];
+    }
     args.periodic = periodic_mask;
     args.h_over_eps = h_over_eps;
     args.refine = (flags & DCPSE_ASM_REFINE) ? 1 : 0;

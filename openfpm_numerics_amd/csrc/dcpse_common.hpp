@@ -17,10 +17,33 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// v from another lane of the same 16-lane row by a DPP pattern (both halves).
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+    const long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)x, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(x >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over the 64 lanes (all active), wave-uniform: within each 16-lane row
+// by DPP exchanges (lane ^ 1, lane ^ 2, mirrored half-rows, mirrored rows;
+// every lane of a row ends with the same bits), then the four row sums by
+// v_readlane.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-    return v;
+    v += dpp_d<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+    v += dpp_d<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+    v += dpp_d<0x141>(v);  // row_half_mirror
+    v += dpp_d<0x140>(v);  // row_mirror
+    return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
+}
+
+// Minimum-image displacement component with the reciprocal box length:
+// rint(v * ilen) equals rint(v / len) unless v / len lies within an ulp of a
+// half-integer, i.e. a separation of half the box, which no support of
+// radius below half the box contains.
+__device__ __forceinline__ double min_image_r(double v, double len, double ilen, bool periodic) {
+    return periodic ? v - len * rint(v * ilen) : v;
 }
 
 __device__ __forceinline__ int lane_prefix(unsigned long long mask) {
