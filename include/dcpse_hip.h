@@ -89,7 +89,7 @@ dcpse_status dcpse_sync(dcpse_ctx* ctx);
 dcpse_status dcpse_set_apply_config(dcpse_ctx* ctx, int32_t group_lanes, int32_t elems_per_lane,
                                     int32_t vector_width, int32_t block_threads);
 /* Assembly launch shape: wavefronts per block (must be 1) and particles each
- * wavefront assembles in sequence (0 = automatic). */
+ * wavefront assembles in sequence (0 = automatic; values above 63 act as 63). */
 dcpse_status dcpse_set_assembly_config(dcpse_ctx* ctx, int32_t waves_per_block, int32_t particles_per_wave);
 
 /* ---- host-only helpers (no device work) -------------------------------- */

@@ -375,9 +375,9 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
     // pass covers HALVES = 64 / LPN neighbours
     constexpr int LPN = NPAIR <= 8 ? 8 : (NPAIR <= 16 ? 16 : (NPAIR <= 32 ? 32 : 64));
     constexpr int HALVES = kWave / LPN;
-    static_assert(kChunk % HALVES == 0, "a chunk holds whole passes");
+    static_assert(kChunk % (2 * HALVES) == 0, "a chunk holds whole passes");
     static_assert(NPAIR <= kWave, "moment pairs exceed a wavefront");
-    __shared__ double tab[TAB];                   // staged powers; L factors in the solve
+    __shared__ __align__(16) double tab[TAB];     // staged powers; L factors in the solve
     __shared__ double mom[SLOTS * MGP];           // moments of each slot's particle
     __shared__ double2 lbuf[2 * kWave];           // broadcast buffers of the factorisation
     __shared__ double sx[kWave];                  // solutions, one entry per lane
@@ -531,13 +531,14 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
                     double z[3] = {0.0, 0.0, 0.0};
 #pragma unroll
                     for (int d = 0; d < DIM; ++d) z[d] = vj ? dv[d] * ieps : 0.0;
-                    double* row = tab + jl * TROW;
+                    // neighbours 2i and 2i + 1 share a row of (even, odd) pairs
+                    double* row = tab + (jl >> 1) * 2 * TROW + (jl & 1);
                     if (lane < kChunk) {
                         row[0] = z[0];
                         double pw = 1.0;
 #pragma unroll
                         for (int k = 0; k <= P; ++k) {
-                            row[1 + k] = pw;
+                            row[2 * (1 + k)] = pw;
                             pw *= z[1];
                         }
                     } else {
@@ -548,21 +549,23 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
                         if (c0 + jl < kWave) wsl[s * kWave + c0 + jl] = pw;
 #pragma unroll
                         for (int k = 0; k < ZW; ++k) {
-                            row[1 + PW + k] = pw;
+                            row[2 * (1 + PW + k)] = pw;
                             pw *= z[DIM - 1];
                         }
                     }
                     wave_lds_sync();
-                    // lanes of an invalid pair accumulate pair 0 and are not stored
-#pragma unroll 2
-                    for (int j = 0; j < cnt; j += HALVES) {
-                        const double* rw = tab + (j + half) * TROW;
-                        const double x = rw[0];
-                        double t = rw[1 + pb] * rw[1 + PW + pc];
+                    // a pass covers two neighbours per group; lanes of an
+                    // invalid pair accumulate pair 0 and are not stored
+                    for (int j = 0; j < cnt; j += 2 * HALVES) {
+                        const double2* rw = reinterpret_cast<const double2*>(tab) + ((j >> 1) + half) * TROW;
+                        const double2 x = rw[0];
+                        const double2 yb = rw[1 + pb], wc = rw[1 + PW + pc];
+                        double t0 = yb.x * wc.x, t1 = yb.y * wc.y;
 #pragma unroll
                         for (int k = 0; k < PW; ++k) {
-                            acc[k] += t;
-                            t *= x;
+                            acc[k] += t0 + t1;
+                            t0 *= x.x;
+                            t1 *= x.y;
                         }
                     }
                     wave_lds_sync();
