@@ -8,7 +8,9 @@ particles with 35..60 neighbours. Convergence order 2.
   setup     neighbour CSR on the GPU (support_ms), FP64 assembly of the
             Laplacian (one operator, 19 monomials) and Dx (10 monomials) in
             one pass sharing gathers, epsilon and moments (assembly_ms, HIP
-            events; each operator alone in assembly_ms_by_operator).
+            events; each operator alone in assembly_ms_by_operator; the pair
+            with one step of FP64 iterative refinement per solve in
+            assembly_ms_with_refinement).
   step      one fused FP32 pass applying Laplacian and Dx to
             f = sin(2 pi x) sin(2 pi y) sin(2 pi z) (inputs resident in HBM).
   value     M particle-op evals/s = particles * 2 operators / step time.
@@ -150,10 +152,16 @@ def main():
             + ", ".join(f"{k}:{v:.2f}ms" for k, v in asm_tune.items()))
     asm_lap_ms, _ = events_ms(lambda: lap.dcpse.initializeStaticSize(ps, ps, check=False), 3, torch)
     asm_dx_ms, _ = events_ms(lambda: dx.dcpse.initializeStaticSize(ps, ps, check=False), 3, torch)
+    # the pair with one step of FP64 iterative refinement per solve (an option,
+    # off by default as in the reference), then the default again
+    lap.dcpse.refine = dx.dcpse.refine = True
+    assembly_refined_ms, _ = events_ms(assemble_both, 3, torch)
+    lap.dcpse.refine = dx.dcpse.refine = False
     assembly_ms, _ = events_ms(assemble_both, 3, torch)
     lap.dcpse.check_info()
     dx.dcpse.check_info()
-    log(f"[bench] assembly Lap {asm_lap_ms:.3f} ms, Dx {asm_dx_ms:.3f} ms, both {assembly_ms:.3f} ms")
+    log(f"[bench] assembly Lap {asm_lap_ms:.3f} ms, Dx {asm_dx_ms:.3f} ms, both {assembly_ms:.3f} ms "
+        f"(with refinement {assembly_refined_ms:.3f} ms)")
 
     # ---- apply --------------------------------------------------------------------
     f64, dx_exact, _, lap_exact = W.sin_field(w["pos"])
@@ -325,6 +333,7 @@ def main():
                                      "vector_width": tuned[0][2], "block": tuned[0][3]})},
         "assembly_ms": assembly_ms,
         "assembly_ms_by_operator": {"laplacian": asm_lap_ms, "dx": asm_dx_ms},
+        "assembly_ms_with_refinement": assembly_refined_ms,
         "support_ms": support_ms,
         "fp32_error_vs_fp64": err,
         "fp64_error_vs_analytic": ana,

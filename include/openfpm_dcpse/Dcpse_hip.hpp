@@ -49,6 +49,9 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
     typedef vector_type vtype;
 
     T HOverEpsilon = 0.9;
+    // one step of FP64 iterative refinement of each particle's solve
+    // (DCPSE_ASM_REFINE); off as in the reference, whose QR solve has none
+    bool refine = false;
 
     template <typename PointT>
     Dcpse_hip(vector_type& particles, VerletList_type& verletList, PointT differentialSignature,
@@ -243,7 +246,7 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
         c.check(dcpse_assemble(c.get(), dim, particles_.devPos(), (int64_t)n, nullptr, (int64_t)n, lo, hi,
                                particles_.periodicMask(), verletList_.devRowPtr(), verletList_.devCol(),
                                (int32_t)coeffs_.size(), sigs_.data(), coeffs_.data(), convergenceOrder_,
-                               (double)HOverEpsilon, DCPSE_ASM_REFINE, eps_.data(), eip_.data(), K64_.data(),
+                               (double)HOverEpsilon, refine ? DCPSE_ASM_REFINE : 0u, eps_.data(), eip_.data(), K64_.data(),
                                W32_.data(), W64_.data(), info_.data()),
                 "dcpse_assemble");
         const std::vector<int32_t> info = info_.to_host();

@@ -67,7 +67,7 @@ class Dcpse:
 
     def __init__(self, particles: ParticleSet, verletList: VerletList, differentialSignature,
                  convergenceOrder: int, rCut, opt: int = RADIUS, *, particlesDomain: ParticleSet | None = None,
-                 signatures=None, coeffs=None, h_over_eps: float | None = None, refine: bool = True):
+                 signatures=None, coeffs=None, h_over_eps: float | None = None, refine: bool = False):
         self.particlesSupport = particles
         self.particlesDomain = particles if particlesDomain is None else particlesDomain
         self.same_set = particlesDomain is None

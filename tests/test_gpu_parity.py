@@ -343,3 +343,33 @@ def test_assemble_together_equals_separate(name, cuda):
         w32a, w32b = a.weights(torch.float32), b.weights(torch.float32)
         assert float((w32a - w32b).abs().max()) <= 2.0 ** -22 * float(w32a.abs().max()), name
         assert int(b.info.abs().max()) == 0
+
+
+@pytest.mark.parametrize("name", ["2d_jitter_96", "3d_holes_20"])
+def test_refinement_step_agrees(name, cuda):
+    """The optional FP64 refinement step (DCPSE_ASM_REFINE) and the plain
+    Cholesky solve give the same kernels to 1e-12 row-relative on these
+    well-conditioned moment systems, so refinement stays an option."""
+    from openfpm_numerics_amd import workloads as W
+    from openfpm_numerics_amd.dcpse import LOAD, Dcpse
+
+    w = WORKLOADS[name](W)
+    ps = _ps(w, cuda)
+    vl = ps.getVerlet(w["rcut"])
+    dim = len(w["box_lo"])
+    lap = [tuple(2 if e == d else 0 for e in range(dim)) for d in range(dim)]
+    rp = vl.row_ptr.cpu().numpy()
+    rows = np.repeat(np.arange(rp.size - 1), np.diff(rp))
+    for sigs in (lap, [tuple(1 if e == 0 else 0 for e in range(dim))]):
+        K = []
+        for refine in (False, True):
+            d = Dcpse(ps, vl, sigs[0], 2, w["rcut"], LOAD, signatures=sigs, coeffs=[1.0] * len(sigs), refine=refine)
+            d.initializeStaticSize(ps, ps)
+            K.append(d.calcKernels.cpu().numpy())
+        num = np.zeros(rp.size - 1)
+        den = np.zeros(rp.size - 1)
+        np.maximum.at(num, rows, np.abs(K[0] - K[1]))
+        np.maximum.at(den, rows, np.abs(K[1]))
+        rel = float(np.max(num / np.maximum(den, 1e-300)))
+        print(f"[refine] {name} {sigs}: row-relative {rel:.2e}", flush=True)
+        assert rel <= 1e-12
