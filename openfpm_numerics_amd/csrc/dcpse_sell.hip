@@ -61,28 +61,53 @@ __global__ __launch_bounds__(256) void k_sell_width(int64_t n_rows, const int64_
     }
 }
 
-// One thread per lane of every slice: writes every slot, padding included,
-// also for the lanes of the last slice that have no row, so no slot of the
-// layout is ever left uninitialised.
+// Thread layout of the layout builds: a block of 256 threads walks slices
+// (grid-stride); thread t takes lane t % 64 of the slice and its slot groups
+// t / 64, t / 64 + 4, ... (4 slots each), so the loads of a row's entries are
+// independent and every store is a 16-byte (or 8-byte) vector store at
+// consecutive addresses across the lanes.
+struct SliceLane {
+    int32_t row;
+    int64_t rs;
+    int len;
+};
+
+__device__ __forceinline__ SliceLane slice_lane(const int64_t* row_ptr, const int32_t* sell_row, int64_t i) {
+    SliceLane r;
+    r.row = sell_row[i];
+    r.rs = r.row >= 0 ? row_ptr[r.row] : 0;
+    r.len = r.row >= 0 ? (int)(row_ptr[r.row + 1] - r.rs) : 0;
+    return r;
+}
+
+// Every slot is written, padding included, also for the lanes of the last
+// slice that have no row, so no slot of the layout is ever left
+// uninitialised.
 template <bool COLS>
-__global__ void k_sell_fill(int64_t n_rows, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
-                            const float* __restrict__ W, const int64_t* __restrict__ slice_ptr,
-                            const int32_t* __restrict__ sell_row, int32_t* __restrict__ sell_col,
-                            float* __restrict__ sell_W) {
-    const int64_t n_lanes = ((n_rows + 63) / 64) * 64;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_lanes;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t s = i >> 6;
-        const int l = (int)(i & 63);
+__global__ __launch_bounds__(256) void k_sell_fill(int64_t n_rows, const int64_t* __restrict__ row_ptr,
+                                                   const int32_t* __restrict__ col, const float* __restrict__ W,
+                                                   const int64_t* __restrict__ slice_ptr,
+                                                   const int32_t* __restrict__ sell_row,
+                                                   int32_t* __restrict__ sell_col, float* __restrict__ sell_W) {
+    const int64_t n_slices = (n_rows + 63) / 64;
+    const int l = threadIdx.x & 63, gsub = threadIdx.x >> 6;
+    for (int64_t s = blockIdx.x; s < n_slices; s += gridDim.x) {
+        const SliceLane r = slice_lane(row_ptr, sell_row, s * 64 + l);
         const int64_t base = slice_ptr[s];
-        const int width = (int)((slice_ptr[s + 1] - base) / 64);
-        const int32_t row = sell_row[i];
-        const int64_t rs = row >= 0 ? row_ptr[row] : 0;
-        const int len = row >= 0 ? (int)(row_ptr[row + 1] - rs) : 0;
-        for (int j = 0; j < width; ++j) {
-            const int64_t dst = base + (int64_t)(j >> 2) * 256 + 4 * l + (j & 3);
-            if (COLS) sell_col[dst] = j < len ? col[rs + j] : 0;
-            else sell_W[dst] = j < len ? W[rs + j] : 0.0f;
+        const int ng = (int)((slice_ptr[s + 1] - base) >> 8);
+        for (int g = gsub; g < ng; g += 4) {
+            const int64_t dst = base + (int64_t)g * 256 + 4 * l;
+            if constexpr (COLS) {
+                vi4a v;
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) v[jj] = 4 * g + jj < r.len ? col[r.rs + 4 * g + jj] : 0;
+                *reinterpret_cast<vi4a*>(sell_col + dst) = v;
+            } else {
+                vf4a v;
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) v[jj] = 4 * g + jj < r.len ? W[r.rs + 4 * g + jj] : 0.0f;
+                *reinterpret_cast<vf4a*>(sell_W + dst) = v;
+            }
         }
     }
 }
@@ -93,25 +118,31 @@ __global__ void k_sell_fill(int64_t n_rows, const int64_t* __restrict__ row_ptr,
 // (1 + n_ops) * slice_ptr[s] (weights stored as their float bits). A
 // wavefront then reads one contiguous (1 + n_ops) KB span per slot group
 // instead of 1 KB from each of 1 + n_ops arrays. Every word is written.
-__global__ void k_sell_fill_bundle(int64_t n_rows, const int64_t* __restrict__ row_ptr,
-                                   const int32_t* __restrict__ col, int n_ops, const float* W0, const float* W1,
-                                   const float* W2, const float* W3, const int64_t* __restrict__ slice_ptr,
-                                   const int32_t* __restrict__ sell_row, int32_t* __restrict__ bundle) {
-    const int64_t n_lanes = ((n_rows + 63) / 64) * 64;
+__global__ __launch_bounds__(256) void k_sell_fill_bundle(int64_t n_rows, const int64_t* __restrict__ row_ptr,
+                                                          const int32_t* __restrict__ col, int n_ops,
+                                                          const float* W0, const float* W1, const float* W2,
+                                                          const float* W3, const int64_t* __restrict__ slice_ptr,
+                                                          const int32_t* __restrict__ sell_row,
+                                                          int32_t* __restrict__ bundle) {
+    const int64_t n_slices = (n_rows + 63) / 64;
     const float* W[4] = {W0, W1, W2, W3};
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_lanes;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t s = i >> 6;
-        const int l = (int)(i & 63);
+    const int l = threadIdx.x & 63, gsub = threadIdx.x >> 6;
+    for (int64_t s = blockIdx.x; s < n_slices; s += gridDim.x) {
+        const SliceLane r = slice_lane(row_ptr, sell_row, s * 64 + l);
         const int64_t base = slice_ptr[s];
-        const int width = (int)((slice_ptr[s + 1] - base) / 64);
-        const int32_t row = sell_row[i];
-        const int64_t rs = row >= 0 ? row_ptr[row] : 0;
-        const int len = row >= 0 ? (int)(row_ptr[row + 1] - rs) : 0;
-        for (int j = 0; j < width; ++j) {
-            const int64_t dst = (int64_t)(1 + n_ops) * (base + (int64_t)(j >> 2) * 256) + 4 * l + (j & 3);
-            bundle[dst] = j < len ? col[rs + j] : 0;
-            for (int o = 0; o < n_ops; ++o) bundle[dst + (1 + o) * 256] = j < len ? __float_as_int(W[o][rs + j]) : 0;
+        const int ng = (int)((slice_ptr[s + 1] - base) >> 8);
+        for (int g = gsub; g < ng; g += 4) {
+            const int64_t dst = (int64_t)(1 + n_ops) * (base + (int64_t)g * 256) + 4 * l;
+            vi4a v;
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) v[jj] = 4 * g + jj < r.len ? col[r.rs + 4 * g + jj] : 0;
+            *reinterpret_cast<vi4a*>(bundle + dst) = v;
+            for (int o = 0; o < n_ops; ++o) {
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj)
+                    v[jj] = 4 * g + jj < r.len ? __float_as_int(W[o][r.rs + 4 * g + jj]) : 0;
+                *reinterpret_cast<vi4a*>(bundle + dst + (1 + o) * 256) = v;
+            }
         }
     }
 }
@@ -138,60 +169,91 @@ __device__ __forceinline__ int64_t col_offset(int32_t q, int32_t p, int64_t n) {
     return d;
 }
 
-// exc_count[i] = far neighbours of the row of lane i.
-__global__ void k_bundle16_count(int64_t n_rows, const int64_t* __restrict__ row_ptr,
-                                 const int32_t* __restrict__ col, const int32_t* __restrict__ sell_row,
-                                 int32_t* __restrict__ exc_count) {
-    const int64_t n_lanes = ((n_rows + 63) / 64) * 64;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_lanes;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t row = sell_row[i];
+__device__ __forceinline__ bool far16(int64_t d) { return d > 32767 || d < -32768; }
+
+// exc_count[i] = far neighbours of the row of lane i: the four threads of a
+// lane count their slot groups and the first adds the four counts.
+__global__ __launch_bounds__(256) void k_bundle16_count(int64_t n_rows, const int64_t* __restrict__ row_ptr,
+                                                        const int32_t* __restrict__ col,
+                                                        const int32_t* __restrict__ sell_row,
+                                                        int32_t* __restrict__ exc_count) {
+    __shared__ int cnt[256];
+    const int64_t n_slices = (n_rows + 63) / 64;
+    const int l = threadIdx.x & 63, gsub = threadIdx.x >> 6;
+    for (int64_t s = blockIdx.x; s < n_slices; s += gridDim.x) {
+        const SliceLane r = slice_lane(row_ptr, sell_row, s * 64 + l);
         int c = 0;
-        if (row >= 0)
-            for (int64_t j = row_ptr[row]; j < row_ptr[row + 1]; ++j) {
-                const int64_t d = col_offset(col[j], row, n_rows);
-                c += d > 32767 || d < -32768;
-            }
-        exc_count[i] = c;
+        for (int g = gsub; 4 * g < r.len; g += 4)
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj)
+                if (4 * g + jj < r.len) c += far16(col_offset(col[r.rs + 4 * g + jj], r.row, n_rows));
+        cnt[threadIdx.x] = c;
+        __syncthreads();
+        if (gsub == 0) exc_count[s * 64 + l] = cnt[l] + cnt[64 + l] + cnt[128 + l] + cnt[192 + l];
+        __syncthreads();
     }
 }
 
-__global__ void k_sell_fill_bundle16(int64_t n_rows, const int64_t* __restrict__ row_ptr,
-                                     const int32_t* __restrict__ col, int n_ops, const float* W0, const float* W1,
-                                     const float* W2, const float* W3, const int64_t* __restrict__ slice_ptr,
-                                     const int32_t* __restrict__ sell_row, const int64_t* __restrict__ exc_ptr,
-                                     int64_t n_exc, int32_t* __restrict__ bundle, int32_t* __restrict__ sell_row16,
-                                     int32_t* __restrict__ exc_col, float* __restrict__ exc_W) {
-    const int64_t n_lanes = ((n_rows + 63) / 64) * 64;
+__global__ __launch_bounds__(256) void k_sell_fill_bundle16(
+    int64_t n_rows, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ col, int n_ops,
+    const float* W0, const float* W1, const float* W2, const float* W3, const int64_t* __restrict__ slice_ptr,
+    const int32_t* __restrict__ sell_row, const int64_t* __restrict__ exc_ptr, int32_t* __restrict__ bundle,
+    int32_t* __restrict__ sell_row16) {
+    const int64_t n_slices = (n_rows + 63) / 64;
     const float* W[4] = {W0, W1, W2, W3};
     const int64_t blk = 128 + 256 * (int64_t)n_ops;
     int16_t* cols16 = reinterpret_cast<int16_t*>(bundle);
+    const int l = threadIdx.x & 63, gsub = threadIdx.x >> 6;
+    for (int64_t s = blockIdx.x; s < n_slices; s += gridDim.x) {
+        const int64_t i = s * 64 + l;
+        const SliceLane r = slice_lane(row_ptr, sell_row, i);
+        if (gsub == 0) sell_row16[i] = r.row < 0 ? r.row : (exc_ptr[i + 1] > exc_ptr[i] ? (r.row | kExcFlag) : r.row);
+        const int64_t base = slice_ptr[s];
+        const int ng = (int)((slice_ptr[s + 1] - base) >> 8);
+        for (int g = gsub; g < ng; g += 4) {
+            const int64_t gb = (base / 256 + g) * blk;  // word offset of the slot group's block
+            vs4a d;
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                int64_t dj = 0;
+                if (4 * g + jj < r.len) {
+                    dj = col_offset(col[r.rs + 4 * g + jj], r.row, n_rows);
+                    if (far16(dj)) dj = 0;  // in the lane's exception list instead
+                }
+                d[jj] = (int16_t)dj;
+            }
+            *reinterpret_cast<vs4a*>(cols16 + 2 * gb + 4 * l) = d;
+            for (int o = 0; o < n_ops; ++o) {
+                vi4a v;
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj)
+                    v[jj] = 4 * g + jj < r.len ? __float_as_int(W[o][r.rs + 4 * g + jj]) : 0;
+                *reinterpret_cast<vi4a*>(bundle + gb + 128 + o * 256 + 4 * l) = v;
+            }
+        }
+    }
+}
+
+// The exception lists: one thread per lane whose row has far neighbours
+// walks the row in order.
+__global__ void k_bundle16_exc(int64_t n_rows, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
+                               int n_ops, const float* W0, const float* W1, const float* W2, const float* W3,
+                               const int32_t* __restrict__ sell_row, const int64_t* __restrict__ exc_ptr,
+                               int64_t n_exc, int32_t* __restrict__ exc_col, float* __restrict__ exc_W) {
+    const int64_t n_lanes = ((n_rows + 63) / 64) * 64;
+    const float* W[4] = {W0, W1, W2, W3};
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_lanes;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t s = i >> 6;
-        const int l = (int)(i & 63);
-        const int64_t base = slice_ptr[s];
-        const int width = (int)((slice_ptr[s + 1] - base) / 64);
-        const int32_t row = sell_row[i];
-        const int64_t rs = row >= 0 ? row_ptr[row] : 0;
-        const int len = row >= 0 ? (int)(row_ptr[row + 1] - rs) : 0;
         int64_t e = exc_ptr[i];
-        sell_row16[i] = row < 0 ? row : (exc_ptr[i + 1] > e ? (row | kExcFlag) : row);
-        for (int j = 0; j < width; ++j) {
-            const int64_t gb = (base / 256 + (j >> 2)) * blk;  // word offset of the slot group's block
-            int64_t d = 0;
-            if (j < len) {
-                d = col_offset(col[rs + j], row, n_rows);
-                if (d > 32767 || d < -32768) {
-                    exc_col[e] = col[rs + j];
-                    for (int o = 0; o < n_ops; ++o) exc_W[o * n_exc + e] = W[o][rs + j];
-                    ++e;
-                    d = 0;
-                }
+        if (exc_ptr[i + 1] == e) continue;
+        const SliceLane r = slice_lane(row_ptr, sell_row, i);
+        for (int j = 0; j < r.len; ++j) {
+            const int32_t q = col[r.rs + j];
+            if (far16(col_offset(q, r.row, n_rows))) {
+                exc_col[e] = q;
+                for (int o = 0; o < n_ops; ++o) exc_W[o * n_exc + e] = W[o][r.rs + j];
+                ++e;
             }
-            cols16[2 * gb + 4 * l + (j & 3)] = (int16_t)d;
-            for (int o = 0; o < n_ops; ++o)
-                bundle[gb + 128 + o * 256 + 4 * l + (j & 3)] = j < len ? __float_as_int(W[o][rs + j]) : 0;
         }
     }
 }
@@ -620,7 +682,7 @@ dcpse_status dcpse_sell_fill_cols(dcpse_ctx* ctx, int64_t n_rows, const int64_t*
         return fail(ctx, DCPSE_ERR_INVALID, "null pointer or negative size");
     DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     if (n_rows == 0) return DCPSE_OK;
-    const int64_t b = std::min<int64_t>((n_rows + 63 + 255) / 256, 65536);
+    const int64_t b = std::min<int64_t>((n_rows + 63) / 64, 65536);
     k_sell_fill<true><<<(unsigned)b, 256, 0, ctx->stream>>>(n_rows, row_ptr, col, nullptr, slice_ptr, sell_row,
                                                             sell_col, nullptr);
     DCPSE_LAUNCH_CHECK(ctx);
@@ -634,7 +696,7 @@ dcpse_status dcpse_sell_fill_weights_f32(dcpse_ctx* ctx, int64_t n_rows, const i
         return fail(ctx, DCPSE_ERR_INVALID, "null pointer or negative size");
     DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     if (n_rows == 0) return DCPSE_OK;
-    const int64_t b = std::min<int64_t>((n_rows + 63 + 255) / 256, 65536);
+    const int64_t b = std::min<int64_t>((n_rows + 63) / 64, 65536);
     k_sell_fill<false><<<(unsigned)b, 256, 0, ctx->stream>>>(n_rows, row_ptr, nullptr, W, slice_ptr, sell_row,
                                                              nullptr, sell_W);
     DCPSE_LAUNCH_CHECK(ctx);
@@ -691,7 +753,7 @@ dcpse_status dcpse_sell_fill_bundle_f32(dcpse_ctx* ctx, int64_t n_rows, const in
         if (!W[o]) return fail(ctx, DCPSE_ERR_INVALID, "null weight pointer");
     DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     if (n_rows == 0) return DCPSE_OK;
-    const int64_t b = std::min<int64_t>((n_rows + 63 + 255) / 256, 65536);
+    const int64_t b = std::min<int64_t>((n_rows + 63) / 64, 65536);
     k_sell_fill_bundle<<<(unsigned)b, 256, 0, ctx->stream>>>(n_rows, row_ptr, col, n_ops, W[0],
                                                              n_ops > 1 ? W[1] : nullptr, n_ops > 2 ? W[2] : nullptr,
                                                              n_ops > 3 ? W[3] : nullptr, slice_ptr, sell_row, bundle);
@@ -721,7 +783,7 @@ dcpse_status dcpse_sell_bundle16_count(dcpse_ctx* ctx, int64_t n_rows, const int
     if (st) return st;
     int32_t* counts = w.take<int32_t>(n_lanes + 1);
     if (n_lanes > 0) {
-        const int64_t b = std::min<int64_t>((n_lanes + 255) / 256, 65536);
+        const int64_t b = std::min<int64_t>(n_lanes / 64, 65536);
         k_bundle16_count<<<(unsigned)b, 256, 0, ctx->stream>>>(n_rows, row_ptr, col, sell_row, counts);
         DCPSE_LAUNCH_CHECK(ctx);
     }
@@ -748,10 +810,17 @@ dcpse_status dcpse_sell_fill_bundle16_f32(dcpse_ctx* ctx, int64_t n_rows, const 
         if (!W[o]) return fail(ctx, DCPSE_ERR_INVALID, "null weight pointer");
     DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     if (n_rows == 0) return DCPSE_OK;
-    const int64_t b = std::min<int64_t>((n_rows + 63 + 255) / 256, 65536);
+    const int64_t b = std::min<int64_t>((n_rows + 63) / 64, 65536);
     k_sell_fill_bundle16<<<(unsigned)b, 256, 0, ctx->stream>>>(
         n_rows, row_ptr, col, n_ops, W[0], n_ops > 1 ? W[1] : nullptr, n_ops > 2 ? W[2] : nullptr,
-        n_ops > 3 ? W[3] : nullptr, slice_ptr, sell_row, exc_ptr, n_exc, bundle, sell_row16, exc_col, exc_W);
+        n_ops > 3 ? W[3] : nullptr, slice_ptr, sell_row, exc_ptr, bundle, sell_row16);
+    DCPSE_LAUNCH_CHECK(ctx);
+    if (n_exc > 0) {
+        const int64_t be = std::min<int64_t>((n_rows + 63 + 255) / 256, 65536);
+        k_bundle16_exc<<<(unsigned)be, 256, 0, ctx->stream>>>(
+            n_rows, row_ptr, col, n_ops, W[0], n_ops > 1 ? W[1] : nullptr, n_ops > 2 ? W[2] : nullptr,
+            n_ops > 3 ? W[3] : nullptr, sell_row, exc_ptr, n_exc, exc_col, exc_W);
+    }
     DCPSE_LAUNCH_CHECK(ctx);
     return DCPSE_OK;
 }
