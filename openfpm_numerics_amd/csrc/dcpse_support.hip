@@ -151,6 +151,7 @@ dcpse_status reserve_workspace(dcpse_ctx* ctx, size_t bytes, Workspace* w) {
 // ---------------------------------------------------------------------------
 struct CellGrid {
     int nc[3];
+    int kmax[3];  // cells scanned on each side: ceil(largest rcut / cs), capped so a periodic wrap visits no cell twice
     double lo[3], cs[3], len[3];
     unsigned periodic;
     int64_t ncell;
@@ -207,6 +208,12 @@ constexpr int kRowWaves = 4;  // wavefronts per block in the row kernels
 // One wavefront per domain particle. FILL = false: count the support of p into
 // counts[p] (and raise *overflow if it exceeds DCPSE_MAX_ROW). FILL = true:
 // write the support of p, sorted ascending, to col[row_ptr[p] ..].
+// The cells within k_d = ceil(rcut_p / cell edge) of p's cell in each
+// dimension are scanned as rows along dimension 0: cells of one row are
+// consecutive in the cell order, so a row is one contiguous candidate segment
+// (two where a periodic wrap splits it). Up to 32 rows at a time, one segment
+// per lane, are flattened into one candidate list by a prefix sum over the
+// lanes.
 template <int DIM, bool FILL>
 __global__ __launch_bounds__(kRowWaves * kWave) void k_support_rows(
     const double* __restrict__ pos_sup, int64_t n_sup, const double* __restrict__ pos_dom,
@@ -217,11 +224,10 @@ __global__ __launch_bounds__(kRowWaves * kWave) void k_support_rows(
     __shared__ int32_t buf[FILL ? kRowWaves : 1][FILL ? DCPSE_MAX_ROW : 1];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
-    constexpr int NCELLS = DIM == 2 ? 9 : 27;
     for (int64_t p = (int64_t)blockIdx.x * kRowWaves + wv; p < n_dom;
          p += (int64_t)gridDim.x * kRowWaves) {
         double xp[3];
-        int c[3];
+        int c[3] = {0, 0, 0};
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
             xp[d] = pos_dom[d * n_dom + p];
@@ -229,67 +235,89 @@ __global__ __launch_bounds__(kRowWaves * kWave) void k_support_rows(
         }
         const double r = rcut_pp ? rcut[p] : rcut[0];
         const double r2 = r * r;
-        // lanes < NCELLS describe one neighbouring cell each
-        long long cst = 0;
-        int ccnt = 0;
-        if (lane < NCELLS) {
-            int o[3] = {lane % 3 - 1, (lane / 3) % 3 - 1, lane / 9 - 1};
-            bool ok = true;
-            int64_t lin = 0, stride = 1;
+        int kd[3] = {0, 0, 0};
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) {
-                int cc = c[d] + o[d];
-                if ((g.periodic >> d) & 1u) {
-                    cc = (cc + g.nc[d]) % g.nc[d];
-                } else if (cc < 0 || cc >= g.nc[d]) {
-                    ok = false;
-                }
-                lin += stride * cc;
-                stride *= g.nc[d];
-            }
-            if (ok) {
-                cst = cell_start[lin];
-                ccnt = (int)(cell_start[lin + 1] - cst);
-            }
-        }
-        // inclusive prefix over lanes of the candidate counts
-        int incl = ccnt;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            int t = __shfl_up(incl, off, kWave);
-            if (lane >= off) incl += t;
-        }
-        const int total = __shfl(incl, 63, kWave);
-        const int excl = lane < NCELLS ? incl - ccnt : 0x7fffffff;
+        for (int d = 0; d < DIM; ++d) kd[d] = min((int)ceil(r / g.cs[d]), g.kmax[d]);
+        const int ny = 2 * kd[1] + 1;
+        const int rows = DIM == 3 ? ny * (2 * kd[2] + 1) : ny;
         int nacc = 0;
-        for (int base = 0; base < total; base += kWave) {
-            const int k = min(base + lane, total - 1);
-            // largest segment s with excl_s <= k
-            int seg = 0;
-#pragma unroll
-            for (int step = 16; step > 0; step >>= 1) {
-                int cand = seg + step;
-                int ex = __shfl(excl, cand & 63, kWave);
-                if (cand < 32 && ex <= k) seg = cand;
+        for (int rb = 0; rb < rows; rb += kWave / 2) {
+            // lane 2i (+1): row rb + i (its part past a periodic wrap)
+            long long cst = 0;
+            int ccnt = 0;
+            const int row = rb + (lane >> 1), part = lane & 1;
+            if (row < rows) {
+                const int o1 = row % ny - kd[1];
+                const int o2 = DIM == 3 ? row / ny - kd[2] : 0;
+                int cy = c[1] + o1, cz = DIM == 3 ? c[2] + o2 : 0;
+                bool ok = true;
+                if ((g.periodic >> 1) & 1u) cy = (cy % g.nc[1] + g.nc[1]) % g.nc[1];
+                else ok &= cy >= 0 && cy < g.nc[1];
+                if (DIM == 3) {
+                    if ((g.periodic >> 2) & 1u) cz = (cz % g.nc[2] + g.nc[2]) % g.nc[2];
+                    else ok &= cz >= 0 && cz < g.nc[2];
+                }
+                int xa = c[0] - kd[0], xb = c[0] + kd[0];
+                if (g.periodic & 1u) {
+                    if (xa < 0) {
+                        if (part == 0) xa += g.nc[0], xb = g.nc[0] - 1;
+                        else xa = 0;
+                    } else if (xb >= g.nc[0]) {
+                        if (part == 0) xb = g.nc[0] - 1;
+                        else xa = 0, xb -= g.nc[0];
+                    } else if (part == 1) {
+                        ok = false;
+                    }
+                } else {
+                    xa = max(xa, 0);
+                    xb = min(xb, g.nc[0] - 1);
+                    ok &= part == 0;
+                }
+                if (ok && xa <= xb) {
+                    const int64_t lin = (int64_t)g.nc[0] * ((int64_t)cy + (int64_t)g.nc[1] * cz);
+                    cst = cell_start[lin + xa];
+                    ccnt = (int)(cell_start[lin + xb + 1] - cst);
+                }
             }
-            const long long s_cst = __shfl(cst, seg, kWave);
-            const int s_ex = __shfl(excl, seg, kWave);
-            const int64_t slot = s_cst + (k - s_ex);
-            const int q = cell_part[slot];
-            double d[3];
+            // inclusive prefix over lanes of the candidate counts
+            int incl = ccnt;
 #pragma unroll
-            for (int dd = 0; dd < DIM; ++dd)
-                d[dd] = min_image(xp[dd] - pos_cell[dd * n_sup + slot], g.len[dd], (g.periodic >> dd) & 1u);
-            double d2 = d[DIM - 1] * d[DIM - 1];
-#pragma unroll
-            for (int dd = DIM - 2; dd >= 0; --dd) d2 = fma(d[dd], d[dd], d2);
-            const bool take = (base + lane < total) && !(same_set && q == (int)p) && d2 < r2;
-            const unsigned long long m = __ballot(take);
-            if (FILL) {
-                const int at = nacc + lane_prefix(m);
-                if (take && at < DCPSE_MAX_ROW) buf[wv][at] = q;
+            for (int off = 1; off < 64; off <<= 1) {
+                int t = __shfl_up(incl, off, kWave);
+                if (lane >= off) incl += t;
             }
-            nacc += __popcll(m);
+            const int total = __shfl(incl, 63, kWave);
+            const int excl = incl - ccnt;
+            for (int base = 0; base < total; base += kWave) {
+                const int k = min(base + lane, total - 1);
+                // last segment s with excl_s <= k (empty segments are skipped
+                // by taking the last one)
+                int seg = 0;
+#pragma unroll
+                for (int step = 32; step > 0; step >>= 1) {
+                    int cand = seg + step;
+                    int ex = __shfl(excl, cand & 63, kWave);
+                    if (cand < 64 && ex <= k) seg = cand;
+                }
+                const long long s_cst = __shfl(cst, seg, kWave);
+                const int s_ex = __shfl(excl, seg, kWave);
+                const int64_t slot = s_cst + (k - s_ex);
+                const int q = cell_part[slot];
+                double d[3];
+#pragma unroll
+                for (int dd = 0; dd < DIM; ++dd)
+                    d[dd] = min_image(xp[dd] - pos_cell[dd * n_sup + slot], g.len[dd], (g.periodic >> dd) & 1u);
+                double d2 = d[DIM - 1] * d[DIM - 1];
+#pragma unroll
+                for (int dd = DIM - 2; dd >= 0; --dd) d2 = fma(d[dd], d[dd], d2);
+                const bool take = (base + lane < total) && !(same_set && q == (int)p) && d2 < r2;
+                const unsigned long long m = __ballot(take);
+                if (FILL) {
+                    const int at = nacc + lane_prefix(m);
+                    if (take && at < DCPSE_MAX_ROW) buf[wv][at] = q;
+                }
+                nacc += __popcll(m);
+            }
         }
         if (!FILL) {
             if (lane == 0) {
@@ -315,8 +343,13 @@ __global__ __launch_bounds__(kRowWaves * kWave) void k_support_rows(
     }
 }
 
+// Cell edge about rmax / K: smaller cells than the cut-off shrink the scanned
+// volume around a particle ((2k+1)^dim cells of edge rmax/K instead of 3^dim
+// of edge rmax), and let particles with a smaller cut-off scan fewer cells.
+// A periodic dimension needs at least 2k+1 cells; where the box is too small
+// for that, the edge falls back to rmax (k = 1, at least 3 cells).
 static dcpse_status make_grid(dcpse_ctx* ctx, int dim, const double* box_lo, const double* box_hi,
-                              unsigned periodic, double rmax, CellGrid* g) {
+                              unsigned periodic, double rmax, int K, CellGrid* g) {
     if (!(rmax > 0.0) || !std::isfinite(rmax)) return fail(ctx, DCPSE_ERR_INVALID, "rcut must be > 0");
     g->ncell = 1;
     g->periodic = periodic;
@@ -324,20 +357,26 @@ static dcpse_status make_grid(dcpse_ctx* ctx, int dim, const double* box_lo, con
         if (d < dim) {
             double len = box_hi[d] - box_lo[d];
             if (!(len > 0.0)) return fail(ctx, DCPSE_ERR_INVALID, "box_hi must exceed box_lo");
-            int n = (int)std::floor(len / rmax);
+            int n = (int)std::floor(len * K / rmax);
             if (n < 1) n = 1;
-            if (((periodic >> d) & 1u) && n < 3)
-                return fail(ctx, DCPSE_ERR_INVALID,
-                            "periodic dimension %d has %d cells of edge >= rcut (need 3)", d, n);
             // keep the cell grid bounded (cells beyond ~2^26 add nothing but memory)
             while ((double)g->ncell * n > 6.7e7 && n / 2 >= 3) n /= 2;
+            int kmax = (int)std::ceil(rmax / (len / n));
+            if (((periodic >> d) & 1u) && n < 2 * kmax + 1) {
+                n = (int)std::floor(len / rmax);
+                if (n < 3)
+                    return fail(ctx, DCPSE_ERR_INVALID,
+                                "periodic dimension %d has %d cells of edge >= rcut (need 3)", d, n);
+                kmax = 1;
+            }
             g->nc[d] = n;
+            g->kmax[d] = kmax;
             g->lo[d] = box_lo[d];
             g->len[d] = len;
             g->cs[d] = len / n;
             g->ncell *= n;
         } else {
-            g->nc[d] = 1; g->lo[d] = 0; g->len[d] = 1; g->cs[d] = 1;
+            g->nc[d] = 1; g->kmax[d] = 0; g->lo[d] = 0; g->len[d] = 1; g->cs[d] = 1;
         }
     }
     return DCPSE_OK;
@@ -355,7 +394,7 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
     const int same = pos_dom == nullptr;
     if (same) { pos_dom = pos_sup; n_dom = n_sup; }
     // the largest cut-off fixes the cell edge
-    double rmax = 0.0;
+    double rmax = 0.0, rmin = 0.0;
     if (rcut_pp) {
         // reduce on the host: one copy of n_dom doubles is cheap next to the build
         double* h = (double*)malloc(sizeof(double) * (size_t)std::max<int64_t>(n_dom, 1));
@@ -363,14 +402,25 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
         hipError_t e = hipMemcpyAsync(h, rcut, sizeof(double) * n_dom, hipMemcpyDeviceToHost, ctx->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) { free(h); DCPSE_HIP_CHECK(ctx, e); }
-        for (int64_t i = 0; i < n_dom; ++i) rmax = std::max(rmax, h[i]);
+        rmin = n_dom > 0 ? h[0] : 0.0;
+        for (int64_t i = 0; i < n_dom; ++i) {
+            rmax = std::max(rmax, h[i]);
+            rmin = std::min(rmin, h[i]);
+        }
         free(h);
     } else {
         DCPSE_HIP_CHECK(ctx, hipMemcpyAsync(&rmax, rcut, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         DCPSE_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        rmin = rmax;
     }
+    // cells per cut-off: 2 (3D) or 4 (2D) for one cut-off; with a spread of
+    // cut-offs, about rmax / rmin so the smallest scan only their own
+    // neighbourhood (bounded: a wave handles 32 cell rows per round)
+    const double spread = rmin > 0.0 ? rmax / rmin : 1.0;
+    const int K = dim == 3 ? std::min(4, std::max(2, (int)std::ceil(spread - 1e-9)))
+                           : std::min(8, std::max(4, (int)std::ceil(spread - 1e-9)));
     CellGrid g;
-    dcpse_status st = make_grid(ctx, dim, box_lo, box_hi, periodic, rmax, &g);
+    dcpse_status st = make_grid(ctx, dim, box_lo, box_hi, periodic, rmax, K, &g);
     if (st) return st;
 
     Workspace w;
