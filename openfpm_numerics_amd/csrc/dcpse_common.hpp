@@ -38,6 +38,20 @@ __device__ __forceinline__ double wave_sum(double v) {
     return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
 }
 
+// Inclusive prefix sum over the 64 lanes (all active) by DPP: a window of
+// four lanes (row_shr 1..3), then row_shr 4 and 8 inside each 16-lane row
+// (upper banks only), then row_bcast 15 and 31 across rows.
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    int s = v + __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, true);
+    s += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, true);
+    s += __builtin_amdgcn_update_dpp(0, v, 0x113, 0xF, 0xF, true);
+    s += __builtin_amdgcn_update_dpp(0, s, 0x114, 0xF, 0xE, false);
+    s += __builtin_amdgcn_update_dpp(0, s, 0x118, 0xF, 0xC, false);
+    s += __builtin_amdgcn_update_dpp(0, s, 0x142, 0xA, 0xF, false);
+    s += __builtin_amdgcn_update_dpp(0, s, 0x143, 0xC, 0xF, false);
+    return s;
+}
+
 // Minimum-image displacement component with the reciprocal box length:
 // rint(v * ilen) equals rint(v / len) unless v / len lies within an ulp of a
 // half-integer, i.e. a separation of half the box, which no support of

@@ -152,7 +152,7 @@ dcpse_status reserve_workspace(dcpse_ctx* ctx, size_t bytes, Workspace* w) {
 struct CellGrid {
     int nc[3];
     int kmax[3];  // cells scanned on each side: ceil(largest rcut / cs), capped so a periodic wrap visits no cell twice
-    double lo[3], cs[3], len[3];
+    double lo[3], cs[3], len[3], ilen[3];
     unsigned periodic;
     int64_t ncell;
 };
@@ -280,13 +280,8 @@ __global__ __launch_bounds__(kRowWaves * kWave) void k_support_rows(
                 }
             }
             // inclusive prefix over lanes of the candidate counts
-            int incl = ccnt;
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                int t = __shfl_up(incl, off, kWave);
-                if (lane >= off) incl += t;
-            }
-            const int total = __shfl(incl, 63, kWave);
+            const int incl = wave_incl_scan(ccnt);
+            const int total = __builtin_amdgcn_readlane(incl, 63);
             const int excl = incl - ccnt;
             for (int base = 0; base < total; base += kWave) {
                 const int k = min(base + lane, total - 1);
@@ -306,7 +301,8 @@ __global__ __launch_bounds__(kRowWaves * kWave) void k_support_rows(
                 double d[3];
 #pragma unroll
                 for (int dd = 0; dd < DIM; ++dd)
-                    d[dd] = min_image(xp[dd] - pos_cell[dd * n_sup + slot], g.len[dd], (g.periodic >> dd) & 1u);
+                    d[dd] = min_image_r(xp[dd] - pos_cell[dd * n_sup + slot], g.len[dd], g.ilen[dd],
+                                        (g.periodic >> dd) & 1u);
                 double d2 = d[DIM - 1] * d[DIM - 1];
 #pragma unroll
                 for (int dd = DIM - 2; dd >= 0; --dd) d2 = fma(d[dd], d[dd], d2);
@@ -373,10 +369,11 @@ static dcpse_status make_grid(dcpse_ctx* ctx, int dim, const double* box_lo, con
             g->kmax[d] = kmax;
             g->lo[d] = box_lo[d];
             g->len[d] = len;
+            g->ilen[d] = 1.0 / len;
             g->cs[d] = len / n;
             g->ncell *= n;
         } else {
-            g->nc[d] = 1; g->kmax[d] = 0; g->lo[d] = 0; g->len[d] = 1; g->cs[d] = 1;
+            g->nc[d] = 1; g->kmax[d] = 0; g->lo[d] = 0; g->len[d] = 1; g->ilen[d] = 1; g->cs[d] = 1;
         }
     }
     return DCPSE_OK;
