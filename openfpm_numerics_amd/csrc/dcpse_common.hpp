@@ -52,6 +52,19 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
     return s;
 }
 
+// Inclusive prefix maximum over the 64 lanes (all active) of values >= -1,
+// by the same DPP pattern as wave_incl_scan (lanes without a source keep -1).
+__device__ __forceinline__ int wave_incl_max(int v) {
+    int s = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x111, 0xF, 0xF, false));
+    s = max(s, __builtin_amdgcn_update_dpp(-1, v, 0x112, 0xF, 0xF, false));
+    s = max(s, __builtin_amdgcn_update_dpp(-1, v, 0x113, 0xF, 0xF, false));
+    s = max(s, __builtin_amdgcn_update_dpp(-1, s, 0x114, 0xF, 0xE, false));
+    s = max(s, __builtin_amdgcn_update_dpp(-1, s, 0x118, 0xF, 0xC, false));
+    s = max(s, __builtin_amdgcn_update_dpp(-1, s, 0x142, 0xA, 0xF, false));
+    s = max(s, __builtin_amdgcn_update_dpp(-1, s, 0x143, 0xC, 0xF, false));
+    return s;
+}
+
 // Minimum-image displacement component with the reciprocal box length:
 // rint(v * ilen) equals rint(v / len) unless v / len lies within an ulp of a
 // half-integer, i.e. a separation of half the box, which no support of

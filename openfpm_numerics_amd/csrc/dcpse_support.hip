@@ -222,6 +222,9 @@ __global__ __launch_bounds__(kRowWaves * kWave) void k_support_rows(
     const double* __restrict__ pos_cell, int32_t* __restrict__ counts, int32_t* __restrict__ overflow,
     const int64_t* __restrict__ row_ptr, int32_t* __restrict__ col) {
     __shared__ int32_t buf[FILL ? kRowWaves : 1][FILL ? DCPSE_MAX_ROW : 1];
+    __shared__ int32_t mark[kRowWaves][kWave];      // segment starting at a candidate of the round
+    __shared__ long long seg_cst[kRowWaves][kWave];  // first cell slot of each segment
+    __shared__ int32_t seg_ex[kRowWaves][kWave];     // candidates before each segment
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     for (int64_t p = (int64_t)blockIdx.x * kRowWaves + wv; p < n_dom;
@@ -283,19 +286,23 @@ __global__ __launch_bounds__(kRowWaves * kWave) void k_support_rows(
             const int incl = wave_incl_scan(ccnt);
             const int total = __builtin_amdgcn_readlane(incl, 63);
             const int excl = incl - ccnt;
+            seg_cst[wv][lane] = cst;
+            seg_ex[wv][lane] = excl;
+            // segment of candidate k: the last non-empty segment starting at
+            // or before k. Each round marks the segments that start inside its
+            // 64 candidates, a prefix maximum spreads them, and a candidate
+            // before the round's first start continues the previous segment.
+            int carry = 0;
             for (int base = 0; base < total; base += kWave) {
                 const int k = min(base + lane, total - 1);
-                // last segment s with excl_s <= k (empty segments are skipped
-                // by taking the last one)
-                int seg = 0;
-#pragma unroll
-                for (int step = 32; step > 0; step >>= 1) {
-                    int cand = seg + step;
-                    int ex = __shfl(excl, cand & 63, kWave);
-                    if (cand < 64 && ex <= k) seg = cand;
-                }
-                const long long s_cst = __shfl(cst, seg, kWave);
-                const int s_ex = __shfl(excl, seg, kWave);
+                mark[wv][lane] = -1;
+                wave_lds_sync();
+                if (ccnt > 0 && excl >= base && excl < base + kWave) mark[wv][excl - base] = lane;
+                wave_lds_sync();
+                const int seg = max(wave_incl_max(mark[wv][lane]), carry);
+                carry = __builtin_amdgcn_readlane(seg, 63);
+                const long long s_cst = seg_cst[wv][seg];
+                const int s_ex = seg_ex[wv][seg];
                 const int64_t slot = s_cst + (k - s_ex);
                 const int q = cell_part[slot];
                 double d[3];
