@@ -314,6 +314,13 @@ dcpse_status dcpse_stage_sell_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t*
                                   const int32_t* sell_row, const int32_t* sell_col, const float* sell_W,
                                   const float* u, float alpha, const float* y_a, float ca, float* out_a,
                                   const float* y_b, float cb, float* out_b);
+/* The stage on the 16-bit bundle of ONE operator (dcpse_sell_fill_bundle16_f32
+ * with n_ops = 1; support and domain one particle set). */
+dcpse_status dcpse_stage_bundle16_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr,
+                                      const int64_t* slice_ptr, const int32_t* sell_row16, const int32_t* bundle,
+                                      const int64_t* exc_ptr, const int32_t* exc_col, const float* exc_W,
+                                      int64_t n_exc, const float* u, float alpha, const float* y_a, float ca,
+                                      float* out_a, const float* y_b, float cb, float* out_b);
 /* n_steps classic RK4 steps of df/dt = alpha * D f (the integration of
  * src/OdeIntegrators/tests/OdeIntegratores_base_tests.cpp:478-577 with
  * runge_kutta4), state f updated in place; 4 stage launches per step.
@@ -325,6 +332,10 @@ dcpse_status dcpse_rk4_f64(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_pt
 dcpse_status dcpse_rk4_sell_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int64_t* slice_ptr,
                                 const int32_t* sell_row, const int32_t* sell_col, const float* sell_W, float alpha,
                                 double dt, int64_t n_steps, float* f, float* work);
+dcpse_status dcpse_rk4_bundle16_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr,
+                                    const int64_t* slice_ptr, const int32_t* sell_row16, const int32_t* bundle,
+                                    const int64_t* exc_ptr, const int32_t* exc_col, const float* exc_W,
+                                    int64_t n_exc, float alpha, double dt, int64_t n_steps, float* f, float* work);
 
 #ifdef __cplusplus
 }

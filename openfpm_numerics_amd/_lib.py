@@ -80,6 +80,10 @@ SIGNATURES = {
     "dcpse_rk4_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _f32, _f64, _i64, _vp, _vp]),
     "dcpse_rk4_f64": (_i32, [_vp, _i64, _vp, _vp, _vp, _f64, _f64, _i64, _vp, _vp]),
     "dcpse_rk4_sell_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _f32, _f64, _i64, _vp, _vp]),
+    "dcpse_stage_bundle16_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _f32, _vp, _f32,
+                                        _vp, _vp, _f32, _vp]),
+    "dcpse_rk4_bundle16_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _f32, _f64, _i64, _vp,
+                                      _vp]),
 }
 
 _LIB = None

@@ -121,6 +121,17 @@ dcpse_status dcpse_rk4_sell_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* r
     return DCPSE_OK;
 }
 
+dcpse_status dcpse_rk4_bundle16_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr,
+                                    const int64_t* slice_ptr, const int32_t* sell_row16, const int32_t* bundle,
+                                    const int64_t* exc_ptr, const int32_t* exc_col, const float* exc_W,
+                                    int64_t n_exc, float alpha, double dt, int64_t n_steps, float* f, float* work) {
+    if (!ctx) return DCPSE_ERR_INVALID;
+    if (!f || !work || n_steps < 0 || n_rows < 0) return fail(ctx, DCPSE_ERR_INVALID, "null pointer or bad size");
+    DCPSE_RK4_STEPS(dcpse_stage_bundle16_f32, float, ctx, n_rows, row_ptr, slice_ptr, sell_row16, bundle, exc_ptr,
+                    exc_col, exc_W, n_exc);
+    return DCPSE_OK;
+}
+
 dcpse_status dcpse_rk4_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int32_t* col,
                            const float* W, float alpha, double dt, int64_t n_steps, float* f, float* work) {
     if (!ctx) return DCPSE_ERR_INVALID;

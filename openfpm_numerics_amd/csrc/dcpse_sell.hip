@@ -588,12 +588,12 @@ static void launch_sell(SellArgs a, hipStream_t st) {
     else k_apply_sell<NOPS, UNR, MODE><<<(unsigned)(a.per_xcd * 8), kSellWaves * kWave, 0, st>>>(a);
 }
 
-template <int NOPS, int UNR>
+template <int NOPS, int UNR, int MODE = 0>
 static void launch_sell16(SellArgs a, hipStream_t st) {
     const int64_t n_slices = (a.n_rows + 63) / 64;
     a.blocks_needed = (n_slices + kSellWaves - 1) / kSellWaves;
     a.per_xcd = (a.blocks_needed + 7) / 8;
-    k_apply_sell<NOPS, UNR, 0, true><<<(unsigned)(a.per_xcd * 8), kSellWaves * kWave, 0, st>>>(a);
+    k_apply_sell<NOPS, UNR, MODE, true><<<(unsigned)(a.per_xcd * 8), kSellWaves * kWave, 0, st>>>(a);
 }
 
 // 16-bit-offset bundles: forward apply, two-stage pipeline (groups 2 or 4).
@@ -895,6 +895,44 @@ dcpse_status dcpse_stage_sell_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t*
     a.cb = cb;
     if (!dispatch_sell<2>(a, 1, ctx->sell_groups, ctx->sell_depth, ctx->stream))
         return fail(ctx, DCPSE_ERR_UNSUPPORTED, "no SELL stage kernel for this pipeline");
+    DCPSE_LAUNCH_CHECK(ctx);
+    return DCPSE_OK;
+}
+
+dcpse_status dcpse_stage_bundle16_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr,
+                                      const int64_t* slice_ptr, const int32_t* sell_row16, const int32_t* bundle,
+                                      const int64_t* exc_ptr, const int32_t* exc_col, const float* exc_W,
+                                      int64_t n_exc, const float* u, float alpha, const float* y_a, float ca,
+                                      float* out_a, const float* y_b, float cb, float* out_b) {
+    if (!ctx) return DCPSE_ERR_INVALID;
+    if (!row_ptr || !slice_ptr || !sell_row16 || !bundle || !exc_ptr || !u || n_rows < 0 || n_exc < 0 ||
+        (n_exc > 0 && (!exc_col || !exc_W)))
+        return fail(ctx, DCPSE_ERR_INVALID, "null pointer or bad size");
+    if ((out_a && !y_a) || (out_b && !y_b)) return fail(ctx, DCPSE_ERR_INVALID, "an output without its input");
+    if (out_a == u || out_b == u) return fail(ctx, DCPSE_ERR_INVALID, "a stage output may not alias its input u");
+    DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    if (n_rows == 0 || (!out_a && !out_b)) return DCPSE_OK;
+    SellArgs a{};
+    a.n_rows = n_rows;
+    a.row_ptr = row_ptr;
+    a.slice_ptr = slice_ptr;
+    a.sell_row = sell_row16;
+    a.bundle = bundle;
+    a.exc_ptr = exc_ptr;
+    a.exc_col = exc_col;
+    a.exc_W = exc_W;
+    a.n_exc = n_exc;
+    a.f = u;
+    a.f_self = u;
+    a.alpha = alpha;
+    a.y_a = y_a;
+    a.y_b = y_b;
+    a.out_a = out_a;
+    a.out_b = out_b;
+    a.ca = ca;
+    a.cb = cb;
+    if (ctx->sell_groups == 4) launch_sell16<1, 4, 2>(a, ctx->stream);
+    else launch_sell16<1, 2, 2>(a, ctx->stream);
     DCPSE_LAUNCH_CHECK(ctx);
     return DCPSE_OK;
 }

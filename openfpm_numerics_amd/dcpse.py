@@ -162,6 +162,13 @@ class Dcpse:
                                            self.verletList.n_rows)
         return self._W["sell"]
 
+    def sell_bundle16(self):
+        """This operator alone as a 16-bit-offset bundle (see sell_bundle16),
+        built once per assembly."""
+        if self._W.get("bundle16") is None:
+            self._W["bundle16"] = sell_bundle16([self])
+        return self._W["bundle16"]
+
     def computeDifferentialOperator(self, f: torch.Tensor, out: torch.Tensor | None = None,
                                     f_domain: torch.Tensor | None = None, alpha=1.0, beta=0.0):
         """D f for a field (N,) or a batch of fields (B, N) over the support set."""

@@ -27,9 +27,10 @@ def rk4(op, f: torch.Tensor, dt: float, n_steps: int, alpha: float = 1.0, layout
         work: torch.Tensor | None = None) -> torch.Tensor:
     """Advances f (N,) in place by n_steps RK4 steps of df/dt = alpha * D f and returns it.
 
-    float32 state runs the FP32 kernels (sliced-ELL layout unless layout="csr"),
-    float64 state the FP64 CSR kernels. ``work`` (3N of f's dtype) may be passed
-    to avoid an allocation per call.
+    float32 state runs the FP32 kernels: layout "auto" or "bundle16" (sliced-ELL
+    with 16-bit column offsets, the fewest bytes), "sell" (32-bit columns) or
+    "csr"; float64 state the FP64 CSR kernels. ``work`` (3N of f's dtype) may
+    be passed to avoid an allocation per call.
     """
     d = _dcpse(op)
     vl = d.verletList
@@ -47,7 +48,13 @@ def rk4(op, f: torch.Tensor, dt: float, n_steps: int, alpha: float = 1.0, layout
     if f.dtype == torch.float64:
         ctx.call("dcpse_rk4_f64", n, _lib.ptr(vl.row_ptr), _lib.ptr(vl.col), _lib.ptr(d.weights(torch.float64)),
                  float(alpha), float(dt), int(n_steps), _lib.ptr(f), _lib.ptr(work))
-    elif layout in ("auto", "sell"):
+    elif layout in ("auto", "bundle16"):
+        slice_ptr = vl.sell()[0]
+        bundle, row16, exc_ptr, exc_col, exc_W, ne = d.sell_bundle16()
+        ctx.call("dcpse_rk4_bundle16_f32", n, _lib.ptr(vl.row_ptr), _lib.ptr(slice_ptr), _lib.ptr(row16),
+                 _lib.ptr(bundle), _lib.ptr(exc_ptr), _lib.ptr(exc_col), _lib.ptr(exc_W), ne, float(alpha),
+                 float(dt), int(n_steps), _lib.ptr(f), _lib.ptr(work))
+    elif layout == "sell":
         slice_ptr, sell_row, sell_col = vl.sell()
         ctx.call("dcpse_rk4_sell_f32", n, _lib.ptr(vl.row_ptr), _lib.ptr(slice_ptr), _lib.ptr(sell_row),
                  _lib.ptr(sell_col), _lib.ptr(d.sell_weights()), float(alpha), float(dt), int(n_steps), _lib.ptr(f),
@@ -72,12 +79,20 @@ def stage(op, u: torch.Tensor, alpha: float, y_a: torch.Tensor | None, ca: float
         ctx.call("dcpse_stage_f64", vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(vl.col),
                  _lib.ptr(d.weights(torch.float64)), _lib.ptr(u), float(alpha), p(y_a), float(ca), p(out_a), p(y_b),
                  float(cb), p(out_b))
-    elif layout in ("auto", "sell"):
+    elif layout in ("auto", "bundle16"):
+        slice_ptr = vl.sell()[0]
+        bundle, row16, exc_ptr, exc_col, exc_W, ne = d.sell_bundle16()
+        ctx.call("dcpse_stage_bundle16_f32", vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(slice_ptr), _lib.ptr(row16),
+                 _lib.ptr(bundle), _lib.ptr(exc_ptr), _lib.ptr(exc_col), _lib.ptr(exc_W), ne, _lib.ptr(u),
+                 float(alpha), p(y_a), float(ca), p(out_a), p(y_b), float(cb), p(out_b))
+    elif layout == "sell":
         slice_ptr, sell_row, sell_col = vl.sell()
         ctx.call("dcpse_stage_sell_f32", vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(slice_ptr), _lib.ptr(sell_row),
                  _lib.ptr(sell_col), _lib.ptr(d.sell_weights()), _lib.ptr(u), float(alpha), p(y_a), float(ca),
                  p(out_a), p(y_b), float(cb), p(out_b))
-    else:
+    elif layout == "csr":
         ctx.call("dcpse_stage_f32", vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(vl.col),
                  _lib.ptr(d.weights(torch.float32)), _lib.ptr(u), float(alpha), p(y_a), float(ca), p(out_a), p(y_b),
                  float(cb), p(out_b))
+    else:
+        raise ValueError(f"unknown layout {layout!r}")

@@ -70,7 +70,7 @@ def test_stage_epilogue_and_aliasing(cuda):
     _, ps, _, lap = _small(cuda, n=48, seed=3)
     gen = torch.Generator(device="cpu").manual_seed(1)
     n = ps.size_local()
-    for dtype, layouts, tol in ((torch.float64, ("csr",), 1e-13), (torch.float32, ("sell", "csr"), 2e-6)):
+    for dtype, layouts, tol in ((torch.float64, ("csr",), 1e-13), (torch.float32, ("bundle16", "sell", "csr"), 2e-6)):
         u = torch.randn(n, generator=gen, dtype=torch.float64).to(cuda, dtype)
         ya = torch.randn(n, generator=gen, dtype=torch.float64).to(cuda, dtype)
         yb = torch.randn(n, generator=gen, dtype=torch.float64).to(cuda, dtype)
@@ -97,7 +97,8 @@ def test_rk4_fp32_layouts_agree(cuda):
     ref = rk4(lap, f0.clone(), 1e-4, 20, alpha=0.01)
     a = rk4(lap, f0.float().clone(), 1e-4, 20, alpha=0.01, layout="sell")
     b = rk4(lap, f0.float().clone(), 1e-4, 20, alpha=0.01, layout="csr")
-    for got in (a, b):
+    c = rk4(lap, f0.float().clone(), 1e-4, 20, alpha=0.01, layout="bundle16")
+    for got in (a, b, c):
         rel = float(torch.linalg.vector_norm(got.double() - ref) / torch.linalg.vector_norm(ref))
         assert rel <= 1e-6, rel
 

@@ -347,6 +347,18 @@ def test_sell_bundle16_far_neighbours(oracle, cuda):
     want = sum(oracle.apply(r, rp, col, f64) for r in refs)
     got = c[0].double().cpu().numpy()
     assert np.linalg.norm(got - want) / np.linalg.norm(want) <= 1e-5
+    # the time-integration stage on the one-operator bundle, far lists included
+    from openfpm_numerics_amd.ode import stage
+
+    ya = torch.randn(f.numel(), generator=torch.Generator().manual_seed(3)).to(cuda)
+    outs = {}
+    for layout in ("sell", "bundle16"):
+        o = torch.empty_like(f)
+        stage(ops[0], f, 0.5, ya, 0.25, o, None, 0.0, None, layout=layout)
+        outs[layout] = o
+    assert ops[0].sell_bundle16()[5] > vl.nnz // 10
+    scale = float((outs["sell"] - ya).abs().max())
+    assert float((outs["sell"] - outs["bundle16"]).abs().max()) <= 1e-6 * scale
 
 
 @pytest.mark.parametrize("name", sorted(WORKLOADS))
