@@ -35,7 +35,9 @@ AT_LEAST_N_PARTICLES = 3
 
 _FILE_MAGIC = b"DCPSEHIP"
 
-#: FP32 single-field applies run on the sliced-ELL layout (set False to use the CSR kernels)
+#: FP32 single-field applies run on the sliced-ELL layouts (one operator on one
+#: particle set: the 16-bit-offset bundle; otherwise 32-bit columns); set False
+#: to use the CSR kernels
 USE_SELL = True
 _FILE_VERSION = 1
 
@@ -452,6 +454,14 @@ def apply_ops(ops, f: torch.Tensor, outs=None, f_domain: torch.Tensor | None = N
     else:
         outs = [o.reshape(nb, n_rows) for o in outs]
     ctx = _lib.context(f.device.index or 0)
+    if f.dtype == torch.float32 and nb == 1 and USE_SELL and len(ops) == 1 and ops[0].same_set:
+        # one operator: its 16-bit-offset bundle, built once per assembly
+        slice_ptr = vl.sell()[0]
+        bundle, row16, exc_ptr, exc_col, exc_W, ne = ops[0].sell_bundle16()
+        ctx.call("dcpse_apply_sell_bundle16_f32", n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(slice_ptr), _lib.ptr(row16),
+                 1, _lib.ptr(bundle), _lib.ptr(exc_ptr), _lib.ptr(exc_col), _lib.ptr(exc_W), ne, _lib.ptr(f2),
+                 _lib.ptr(fd), _lib.ptr_array(outs), alpha, beta)
+        return [o.reshape(*f.shape[:-1], n_rows) for o in outs]
     if f.dtype == torch.float32 and nb == 1 and USE_SELL:
         slice_ptr, sell_row, sell_col = vl.sell()
         SW = [op.sell_weights() for op in ops]
