@@ -520,22 +520,32 @@ __global__ void k_tcount(const int32_t* __restrict__ col, int64_t nnz, int32_t* 
         atomicAdd(&cnt[col[j]], 1);
 }
 
+// One wavefront per source row p, a lane per entry: each entry takes its
+// slot in transposed row col[j] by an atomic cursor (the order is undone by
+// k_tsort) and the row sum for diag[p] is a wave sum in FP64.
 template <typename T>
-__global__ void k_tfill(int64_t n_rows, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
-                        const T* __restrict__ W, const int64_t* __restrict__ rowT_ptr,
-                        int32_t* __restrict__ cursor, int32_t* __restrict__ colT, T* __restrict__ WT,
-                        T* __restrict__ diag) {
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n_rows;
-         p += (int64_t)gridDim.x * blockDim.x) {
+__global__ __launch_bounds__(kRowWaves * kWave) void k_tfill(int64_t n_rows, const int64_t* __restrict__ row_ptr,
+                                                             const int32_t* __restrict__ col,
+                                                             const T* __restrict__ W,
+                                                             const int64_t* __restrict__ rowT_ptr,
+                                                             int32_t* __restrict__ cursor,
+                                                             int32_t* __restrict__ colT, T* __restrict__ WT,
+                                                             T* __restrict__ diag) {
+    const int lane = threadIdx.x & 63;
+    for (int64_t p = (int64_t)blockIdx.x * kRowWaves + (threadIdx.x >> 6); p < n_rows;
+         p += (int64_t)gridDim.x * kRowWaves) {
+        const int64_t rs = row_ptr[p], re = row_ptr[p + 1];
         double acc = 0.0;
-        for (int64_t j = row_ptr[p]; j < row_ptr[p + 1]; ++j) {
+        for (int64_t j = rs + lane; j < re; j += kWave) {
             const int32_t q = col[j];
+            const T wj = W[j];
             const int64_t at = rowT_ptr[q] + atomicAdd(&cursor[q], 1);
             colT[at] = (int32_t)p;
-            WT[at] = W[j];
-            acc += (double)W[j];
+            WT[at] = wj;
+            acc += (double)wj;
         }
-        diag[p] = (T)(-acc);
+        acc = wave_sum(acc);
+        if (lane == 0) diag[p] = (T)(-acc);
     }
 }
 
@@ -590,8 +600,10 @@ static dcpse_status transpose_fill(dcpse_ctx* ctx, int64_t n_rows, int64_t n_col
     fill_i32(s, cursor, n_cols + 1, 0);
     fill_i32(s, flags, 16, 0);
     if (n_rows > 0) {
-        int64_t b = std::max<int64_t>(1, std::min<int64_t>((n_rows + 255) / 256, 65536));
-        k_tfill<T><<<(unsigned)b, 256, 0, s>>>(n_rows, row_ptr, col, W, rowT_ptr, cursor, colT, WT, diag);
+        int64_t b = std::max<int64_t>(1, std::min<int64_t>((n_rows + kRowWaves - 1) / kRowWaves,
+                                                           (int64_t)ctx->num_cus * 32));
+        k_tfill<T><<<(unsigned)b, kRowWaves * kWave, 0, s>>>(n_rows, row_ptr, col, W, rowT_ptr, cursor, colT, WT,
+                                                              diag);
         DCPSE_LAUNCH_CHECK(ctx);
         int64_t rb = std::max<int64_t>(1, std::min<int64_t>((n_cols + kRowWaves - 1) / kRowWaves,
                                                             (int64_t)ctx->num_cus * 16));
