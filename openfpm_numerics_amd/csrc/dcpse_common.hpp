@@ -65,10 +65,11 @@ __device__ __forceinline__ int wave_incl_max(int v) {
     return s;
 }
 
-// Minimum-image displacement component with the reciprocal box length:
-// rint(v * ilen) equals rint(v / len) unless v / len lies within an ulp of a
-// half-integer, i.e. a separation of half the box, which no support of
-// radius below half the box contains.
+// Minimum-image displacement component, v - len * rint(v / len) as the
+// oracle computes it, with the reciprocal box length: rint(v * ilen) equals
+// rint(v / len) unless v / len lies within an ulp of a half-integer, i.e. a
+// separation of half the box, which no support of radius below half the box
+// contains, so the displacement is the same double.
 __device__ __forceinline__ double min_image_r(double v, double len, double ilen, bool periodic) {
     return periodic ? v - len * rint(v * ilen) : v;
 }
@@ -84,11 +85,6 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Minimum-image displacement component (same arithmetic as the oracle).
-__device__ __forceinline__ double min_image(double v, double len, bool periodic) {
-    return periodic ? v - len * rint(v / len) : v;
 }
 
 // ---- compile-time basis / moment tables ---------------------------------------
