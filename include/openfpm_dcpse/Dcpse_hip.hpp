@@ -11,6 +11,7 @@
 // applies (see DCPSE_op.hpp in this directory).
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <fstream>
@@ -89,8 +90,17 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
         check_fresh();
         const int64_t n = (int64_t)verletList_.size();
         if constexpr (std::is_same<prop_type, float>::value) {
-            const float* W[1] = {W32_.data()};
             float* O[1] = {out};
+            if (n > 0 && !b16_tried_) build_bundle16();
+            if (b16_) {
+                c.check(dcpse_apply_sell_bundle16_f32(c.get(), n, verletList_.devRowPtr(), slice_ptr_.data(),
+                                                      row16_.data(), 1, bundle_.data(), exc_ptr_.data(),
+                                                      exc_col_.data(), exc_W_.data(), n_exc_, f, nullptr, O, alpha,
+                                                      beta),
+                        "dcpse_apply_sell_bundle16_f32");
+                return;
+            }
+            const float* W[1] = {W32_.data()};
             c.check(dcpse_apply_f32(c.get(), n, verletList_.devRowPtr(), verletList_.devCol(), 1, W, f, n, 1,
                                     nullptr, n, O, n, alpha, beta),
                     "dcpse_apply_f32");
@@ -208,11 +218,44 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
         K64_.upload(K_host_);
         W64_.upload(W64);
         W32_.upload(W32);
+        b16_ = b16_tried_ = false;
         mirrored_ = true;
         ctr_ = particles_.getMapCtr();
     }
 
     const std::vector<uint32_t>& signatures() const { return sigs_; }
+
+    // FP32 applies run on a sliced-ELL copy of the weights with 16-bit column
+    // offsets (dcpse_sell_count + dcpse_sell_bundle16_count/fill), built at
+    // the first apply after each assembly or load; where it cannot be built
+    // the CSR kernels run instead.
+    void build_bundle16() {
+        auto& c = openfpm_dcpse::context();
+        b16_tried_ = true;
+        b16_ = false;
+        const int64_t n = (int64_t)verletList_.size();
+        const int64_t n_lanes = 64 * ((n + 63) / 64);
+        slice_ptr_.resize(n_lanes / 64 + 1);
+        sell_row_.resize(n_lanes);
+        int64_t n_slots = 0, n_exc = 0;
+        if (dcpse_sell_count(c.get(), n, verletList_.devRowPtr(), slice_ptr_.data(), sell_row_.data(), &n_slots))
+            return;
+        exc_ptr_.resize(n_lanes + 1);
+        if (dcpse_sell_bundle16_count(c.get(), n, verletList_.devRowPtr(), verletList_.devCol(), sell_row_.data(),
+                                      exc_ptr_.data(), &n_exc))
+            return;
+        bundle_.resize((size_t)std::max<int64_t>(n_slots / 2 + n_slots, 4));
+        row16_.resize(n_lanes);
+        exc_col_.resize((size_t)std::max<int64_t>(n_exc, 1));
+        exc_W_.resize((size_t)std::max<int64_t>(n_exc, 1));
+        const float* W[1] = {W32_.data()};
+        if (dcpse_sell_fill_bundle16_f32(c.get(), n, verletList_.devRowPtr(), verletList_.devCol(), 1, W,
+                                         slice_ptr_.data(), sell_row_.data(), exc_ptr_.data(), n_exc,
+                                         bundle_.data(), row16_.data(), exc_col_.data(), exc_W_.data()))
+            return;
+        n_exc_ = n_exc;
+        b16_ = true;
+    }
 
    private:
     unsigned order_of_derivative() const {
@@ -257,6 +300,7 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
                 throw openfpm_dcpse::dcpse_error("DC-PSE assembly failed at particle " + std::to_string(p) +
                                                  " (info " + std::to_string(info[p]) + ")");
         mirrored_ = false;
+        b16_ = b16_tried_ = false;
         ctr_ = particles_.getMapCtr();
     }
     void mirror() {
@@ -285,6 +329,12 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
     openfpm_dcpse::device_array<double> eps_, eip_, K64_, W64_;
     openfpm_dcpse::device_array<float> W32_;
     openfpm_dcpse::device_array<int32_t> info_;
+    // the 16-bit sliced-ELL copy used by FP32 applies
+    openfpm_dcpse::device_array<int64_t> slice_ptr_, exc_ptr_;
+    openfpm_dcpse::device_array<int32_t> sell_row_, row16_, bundle_, exc_col_;
+    openfpm_dcpse::device_array<float> exc_W_;
+    int64_t n_exc_ = 0;
+    bool b16_ = false, b16_tried_ = false;
     std::vector<double> eps_host_, eip_host_, K_host_;
     std::vector<int64_t> rp_host_;
     std::vector<int32_t> col_host_;
