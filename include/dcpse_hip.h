@@ -109,7 +109,11 @@ int32_t dcpse_rhs(int32_t dim, const uint32_t* beta, uint32_t order, double* b, 
  * periodic dimension (bit d of periodic_mask) uses minimum-image
  * displacements and needs at least 3 cells of edge >= max rcut.
  * count: writes row_ptr[0..n_dom] (device int64) and *nnz (host; this call
- * synchronises the stream). fill: writes col[nnz] (device int32). */
+ * synchronises the stream). fill: writes col[nnz] (device int32), for the
+ * same arguments as the count and unchanged positions. With one set and one
+ * cut-off, the context keeps the count's cell list and per-particle take
+ * masks (about 64 bytes per particle) and a fill that directly follows a
+ * count with the same arguments gathers its rows from them. */
 dcpse_status dcpse_support_count(dcpse_ctx* ctx, int32_t dim,
                                  const double* pos_sup, int64_t n_sup,
                                  const double* pos_dom, int64_t n_dom,

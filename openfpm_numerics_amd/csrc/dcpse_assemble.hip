@@ -106,12 +106,6 @@ struct BasisTag {
     static constexpr int L = LL, A = AA;
 };
 
-__device__ __forceinline__ int64_t readlane_i64(int64_t v, int lane) {
-    const int lo = __builtin_amdgcn_readlane((int)v, lane);
-    const int hi = __builtin_amdgcn_readlane((int)(v >> 32), lane);
-    return ((int64_t)hi << 32) | (uint32_t)lo;
-}
-
 // Writes the "no kernel" result of one operator for particle p.
 __device__ __forceinline__ void asm_fail(const AsmOp& op, int64_t p, int64_t rs, int n, int status, int lane) {
     for (int j = lane; j < n; j += kWave) {

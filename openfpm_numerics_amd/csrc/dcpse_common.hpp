@@ -17,6 +17,12 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+__device__ __forceinline__ int64_t readlane_i64(int64_t v, int lane) {
+    const int lo = __builtin_amdgcn_readlane((int)v, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(v >> 32), lane);
+    return ((int64_t)hi << 32) | (uint32_t)lo;
+}
+
 // v from another lane of the same 16-lane row by a DPP pattern (both halves).
 template <int CTRL>
 __device__ __forceinline__ double dpp_d(double v) {

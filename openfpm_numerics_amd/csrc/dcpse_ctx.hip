@@ -32,6 +32,7 @@ dcpse_status dcpse_ctx_destroy(dcpse_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->ws) (void)hipFree(ctx->ws);
+    if (ctx->sup.buf) (void)hipFree(ctx->sup.buf);
     delete ctx;
     return DCPSE_OK;
 }

@@ -9,6 +9,22 @@
 
 #include "dcpse_hip.h"
 
+// Cell list and per-particle take masks of the last same-set, single-cut-off
+// support count; the fill that follows with the same arguments gathers its
+// rows from them instead of repeating the distance tests.
+struct dcpse_support_cache {
+    void* buf = nullptr;  // grow-only device buffer
+    size_t cap = 0;
+    bool valid = false;
+    bool spill = false;   // some cell's candidates exceed one LDS copy
+    int dim = 0;
+    const double* pos = nullptr;
+    int64_t n = 0;
+    double rcut = 0.0;
+    double lo[3] = {0.0, 0.0, 0.0}, hi[3] = {0.0, 0.0, 0.0};
+    unsigned periodic = 0;
+};
+
 struct dcpse_ctx {
     int device = 0;
     hipStream_t stream = nullptr;  // nullptr = the device's default stream
@@ -28,6 +44,7 @@ struct dcpse_ctx {
     int asm_ppb = 8;       // particles per block (per wave) batch
     void* ws = nullptr;    // grow-only device workspace
     size_t ws_size = 0;
+    dcpse_support_cache sup;
     char err[1024] = {0};
 };
 

@@ -11,6 +11,7 @@
 // row comes out in ascending particle index (deterministic, independent of
 // atomic ordering).
 #include <algorithm>
+#include <climits>
 #include <cmath>
 
 #include "dcpse_common.hpp"
@@ -205,6 +206,68 @@ __global__ void k_cell_fill(const int32_t* __restrict__ cell_id, int64_t n, cons
 
 constexpr int kRowWaves = 4;  // wavefronts per block in the row kernels
 
+// v of lane ^ S (S a power of two below 64): DPP within quads and rows,
+// ds_swizzle within 32-lane halves, ds_bpermute across them.
+template <int S>
+__device__ __forceinline__ int lane_xor(int v, int lane) {
+    if constexpr (S == 1) return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);   // quad_perm 1,0,3,2
+    else if constexpr (S == 2) return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);  // quad_perm 2,3,0,1
+    else if constexpr (S == 8) return __builtin_amdgcn_update_dpp(0, v, 0x128, 0xF, 0xF, false);  // row_ror 8
+    else if constexpr (S == 4 || S == 16) return __builtin_amdgcn_ds_swizzle(v, 0x1F | (S << 10));  // and 31, xor S
+    else return __builtin_amdgcn_ds_bpermute((lane ^ S) << 2, v);
+}
+
+// Bitonic network over the 64 lanes (element index = lane): merge steps of
+// block size SIZE from stride STRIDE down to 1. A block sorts ascending when
+// (lane & SIZE) == 0, flipped by `desc`; in each pair the lower lane keeps
+// the minimum of an ascending block.
+template <int SIZE, int STRIDE>
+__device__ __forceinline__ void bitonic_merge(int& v, int lane, bool desc) {
+    const int p = lane_xor<STRIDE>(v, lane);
+    const bool asc = ((lane & SIZE) == 0) != desc;
+    const bool lower = (lane & STRIDE) == 0;
+    v = (asc == lower) ? min(v, p) : max(v, p);
+    if constexpr (STRIDE > 1) bitonic_merge<SIZE, STRIDE / 2>(v, lane, desc);
+}
+template <int SIZE>
+__device__ __forceinline__ void bitonic_sort(int& v, int lane, bool desc) {
+    if constexpr (SIZE > 2) bitonic_sort<SIZE / 2>(v, lane, desc);
+    bitonic_merge<SIZE, SIZE / 2>(v, lane, desc);
+}
+
+// Writes the n distinct keys keys[0..n) of a wave's LDS list to out[0..n) in
+// ascending order. Up to 128 keys are sorted in registers by a bitonic
+// network (one or two per lane, padded with INT_MAX, which no particle index
+// reaches) and stored contiguously; longer lists write each key at its rank
+// (the number of smaller keys), counted from LDS.
+__device__ __forceinline__ void store_ranked(const int32_t* keys, int n, int32_t* __restrict__ out, int lane) {
+    if (n <= kWave) {
+        int a = lane < n ? keys[lane] : INT_MAX;
+        bitonic_sort<kWave>(a, lane, false);
+        if (lane < n) out[lane] = a;
+    } else if (n <= 2 * kWave) {
+        // a: elements 0..63 ascending, b: 64..127 descending, then one merge
+        int a = keys[lane];
+        int b = lane + kWave < n ? keys[lane + kWave] : INT_MAX;
+        bitonic_sort<kWave>(a, lane, false);
+        bitonic_sort<kWave>(b, lane, true);
+        const int lo = min(a, b), hi = max(a, b);
+        a = lo;
+        b = hi;
+        bitonic_merge<2 * kWave, kWave / 2>(a, lane, false);
+        bitonic_merge<2 * kWave, kWave / 2>(b, lane, false);
+        out[lane] = a;
+        if (lane + kWave < n) out[lane + kWave] = b;
+    } else {
+        for (int e = lane; e < n; e += kWave) {
+            const int key = keys[e];
+            int rank = 0;
+            for (int j = 0; j < n; ++j) rank += keys[j] < key;
+            out[rank] = key;
+        }
+    }
+}
+
 // One wavefront per domain particle. FILL = false: count the support of p into
 // counts[p] (and raise *overflow if it exceeds DCPSE_MAX_ROW). FILL = true:
 // write the support of p, sorted ascending, to col[row_ptr[p] ..].
@@ -334,15 +397,353 @@ __global__ __launch_bounds__(kRowWaves * kWave) void k_support_rows(
             if ((int64_t)nacc != row_ptr[p + 1] - rs) {
                 if (lane == 0) atomicOr(overflow, 2);
             } else {
-                for (int e = lane; e < n; e += kWave) {
-                    const int key = buf[wv][e];
-                    int rank = 0;
-                    for (int j = 0; j < n; ++j) rank += buf[wv][j] < key;
-                    col[rs + rank] = key;
-                }
+                store_ranked(buf[wv], n, col + rs, lane);
             }
             wave_lds_sync();
         }
+    }
+}
+
+// Support rows for one cut-off and one particle set (the same-set, uniform
+// rcut case), one wavefront per cell of a grid with cell edge >= rcut. The
+// candidates of a cell are its 3^dim neighbouring cells, scanned as 3^(dim-1)
+// rows along dimension 0 (contiguous in the cell order, split in two where a
+// periodic wrap cuts a row). They are copied once into the wave's LDS slice
+// (particle index and position, kCellCap of them) and then tested against
+// every particle of the cell, so the cell's particles share the candidate
+// setup and global loads. A cell with more candidates than CellCap restages
+// them chunk by chunk for each of its particles (and raises *overflow bit 4).
+// Same results as k_support_rows: the same distance formula on the same
+// position copies, and the same ascending rank order of a row.
+// Count pass with masks != NULL: also stores, for each particle of a cell that
+// fits, which of the cell's candidates it takes, as CellCap / 64 words at
+// masks[slot * (CellCap / 64)], for k_support_cells_masked. Fill pass with
+// spill_only: only the cells that do not fit.
+constexpr int kCellWaves = 2;
+template <int DIM>
+struct CellCap {
+    static constexpr int v = DIM == 3 ? 512 : 256;
+};
+
+template <int DIM, bool FILL>
+__global__ __launch_bounds__(kCellWaves * kWave) void k_support_cells(
+    int64_t n, const double* __restrict__ rcut, CellGrid g, const int64_t* __restrict__ cell_start,
+    const int32_t* __restrict__ cell_part, const double* __restrict__ pos_cell, int32_t* __restrict__ counts,
+    int32_t* __restrict__ overflow, const int64_t* __restrict__ row_ptr, int32_t* __restrict__ col,
+    uint64_t* __restrict__ masks, int spill_only) {
+    constexpr int CAP = CellCap<DIM>::v;
+    constexpr int NW = CAP / kWave;
+    __shared__ int32_t sq[kCellWaves][CAP];        // candidate particle
+    __shared__ double sx[kCellWaves][DIM][CAP];    // candidate position
+    __shared__ int32_t buf[FILL ? kCellWaves : 1][FILL ? DCPSE_MAX_ROW : 1];
+    __shared__ int32_t mark[kCellWaves][kWave];
+    __shared__ long long seg_cst[kCellWaves][kWave];
+    __shared__ int32_t seg_ex[kCellWaves][kWave];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const double r = rcut[0];
+    const double r2 = r * r;
+    // consecutive logical blocks on one XCD (blocks are dealt round-robin over
+    // the 8 XCDs), so neighbouring cells, which share most candidates, meet in
+    // one L2
+    const unsigned G = gridDim.x;
+    const unsigned lb = (G % 8u == 0u) ? (blockIdx.x % 8u) * (G / 8u) + blockIdx.x / 8u : blockIdx.x;
+    for (int64_t c = (int64_t)lb * kCellWaves + wv; c < g.ncell; c += (int64_t)G * kCellWaves) {
+        const int64_t pst = cell_start[c];
+        const int np = (int)(cell_start[c + 1] - pst);
+        if (np == 0) continue;
+        int cc[3] = {0, 0, 0};
+        {
+            int64_t rem = c;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                cc[d] = (int)(rem % g.nc[d]);
+                rem /= g.nc[d];
+            }
+        }
+        constexpr int ROWS = DIM == 3 ? 9 : 3;
+        long long cst = 0;
+        int ccnt = 0;
+        {
+            // lane 2i (+1): row i (its part past a periodic wrap)
+            const int row = lane >> 1, part = lane & 1;
+            if (row < ROWS) {
+                int cy = cc[1] + row % 3 - 1, cz = DIM == 3 ? cc[2] + row / 3 - 1 : 0;
+                bool ok = true;
+                if ((g.periodic >> 1) & 1u) cy = (cy + g.nc[1]) % g.nc[1];
+                else ok &= cy >= 0 && cy < g.nc[1];
+                if (DIM == 3) {
+                    if ((g.periodic >> 2) & 1u) cz = (cz + g.nc[2]) % g.nc[2];
+                    else ok &= cz >= 0 && cz < g.nc[2];
+                }
+                int xa = cc[0] - 1, xb = cc[0] + 1;
+                if (g.periodic & 1u) {
+                    if (xa < 0) {
+                        if (part == 0) xa += g.nc[0], xb = g.nc[0] - 1;
+                        else xa = 0;
+                    } else if (xb >= g.nc[0]) {
+                        if (part == 0) xb = g.nc[0] - 1;
+                        else xa = 0, xb -= g.nc[0];
+                    } else if (part == 1) {
+                        ok = false;
+                    }
+                } else {
+                    xa = max(xa, 0);
+                    xb = min(xb, g.nc[0] - 1);
+                    ok &= part == 0;
+                }
+                if (ok && xa <= xb) {
+                    const int64_t lin = (int64_t)g.nc[0] * ((int64_t)cy + (int64_t)g.nc[1] * cz);
+                    cst = cell_start[lin + xa];
+                    ccnt = (int)(cell_start[lin + xb + 1] - cst);
+                }
+            }
+        }
+        const int incl = wave_incl_scan(ccnt);
+        const int total = __builtin_amdgcn_readlane(incl, 63);
+        const int excl = incl - ccnt;
+        seg_cst[wv][lane] = cst;
+        seg_ex[wv][lane] = excl;
+        // copies candidates [c0, c0 + cn) to the LDS slice (c0 a multiple of
+        // 64); a candidate's segment is the last non-empty one starting at or
+        // before it (start marks spread by a prefix maximum, carried across
+        // rounds of 64)
+        auto stage = [&](int c0, int cn) {
+            int carry = 0;
+            for (int base = 0; base < c0 + cn; base += kWave) {
+                mark[wv][lane] = -1;
+                wave_lds_sync();
+                if (ccnt > 0 && excl >= base && excl < base + kWave) mark[wv][excl - base] = lane;
+                wave_lds_sync();
+                const int seg = max(wave_incl_max(mark[wv][lane]), carry);
+                carry = __builtin_amdgcn_readlane(seg, 63);
+                const int k = base + lane;
+                if (base >= c0 && k < c0 + cn) {
+                    const int64_t slot = seg_cst[wv][seg] + (k - seg_ex[wv][seg]);
+                    sq[wv][k - c0] = cell_part[slot];
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) sx[wv][d][k - c0] = pos_cell[d * n + slot];
+                }
+            }
+            wave_lds_sync();
+        };
+        // the cell's first 64 particles, one per lane: index, position and
+        // (FILL) row bounds, loaded together ahead of the per-particle loop
+        int pid_l = 0;
+        double x_l[3] = {0.0, 0.0, 0.0};
+        int64_t rs_l = 0, re_l = 0;
+        if (lane < np) {
+            pid_l = cell_part[pst + lane];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) x_l[d] = pos_cell[d * n + pst + lane];
+            if (FILL) {
+                rs_l = row_ptr[pid_l];
+                re_l = row_ptr[pid_l + 1];
+            }
+        }
+        const bool fits = total <= CAP;
+        if (FILL && spill_only && fits) continue;
+        if (!FILL && !fits && lane == 0) atomicOr(overflow, 4);
+        if (fits) stage(0, total);
+        for (int i = 0; i < np; ++i) {
+            int pid;
+            double xp[3];
+            int64_t rs = 0, re = 0;
+            if (i < kWave) {
+                pid = __builtin_amdgcn_readlane(pid_l, i);
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) xp[d] = readlane_d(x_l[d], i);
+                if (FILL) {
+                    rs = readlane_i64(rs_l, i);
+                    re = readlane_i64(re_l, i);
+                }
+            } else {
+                const int64_t ps = pst + i;
+                pid = cell_part[ps];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) xp[d] = pos_cell[d * n + ps];
+                if (FILL) {
+                    rs = row_ptr[pid];
+                    re = row_ptr[pid + 1];
+                }
+            }
+            int nacc = 0;
+            uint64_t mword = 0;  // lane r: the take mask of candidates [64 r, 64 r + 64)
+            for (int c0 = 0; c0 < total; c0 += CAP) {
+                const int cn = min(CAP, total - c0);
+                if (!fits) {
+                    wave_lds_sync();
+                    stage(c0, cn);
+                }
+                for (int t0 = 0; t0 < cn; t0 += kWave) {
+                    const int t = min(t0 + lane, cn - 1);
+                    const int q = sq[wv][t];
+                    double d[3];
+#pragma unroll
+                    for (int dd = 0; dd < DIM; ++dd)
+                        d[dd] = min_image_r(xp[dd] - sx[wv][dd][t], g.len[dd], g.ilen[dd], (g.periodic >> dd) & 1u);
+                    double d2 = d[DIM - 1] * d[DIM - 1];
+#pragma unroll
+                    for (int dd = DIM - 2; dd >= 0; --dd) d2 = fma(d[dd], d[dd], d2);
+                    const bool take = t0 + lane < cn && q != pid && d2 < r2;
+                    const unsigned long long m = __ballot(take);
+                    if (FILL) {
+                        const int at = nacc + lane_prefix(m);
+                        if (take && at < DCPSE_MAX_ROW) buf[wv][at] = q;
+                    } else {
+                        if (lane == (t0 >> 6)) mword = m;
+                    }
+                    nacc += __popcll(m);
+                }
+            }
+            if (!FILL) {
+                if (lane == 0) {
+                    counts[pid] = nacc;
+                    if (nacc > DCPSE_MAX_ROW) atomicOr(overflow, 1);
+                }
+                if (masks && fits && lane < NW) masks[(pst + i) * NW + lane] = mword;
+            } else {
+                wave_lds_sync();
+                const int nn = min(nacc, DCPSE_MAX_ROW);
+                if ((int64_t)nacc != re - rs) {
+                    if (lane == 0) atomicOr(overflow, 2);
+                } else {
+                    store_ranked(buf[wv], nn, col + rs, lane);
+                }
+                wave_lds_sync();
+            }
+        }
+        wave_lds_sync();
+    }
+}
+
+// Fill pass of k_support_cells for the cells whose candidates fit one LDS
+// copy, from the take masks its count pass stored: the candidate indices of a
+// cell are staged once and each particle's row is gathered by its mask, with
+// no distance work. The cell list must be the one the count pass used.
+template <int DIM>
+__global__ __launch_bounds__(kCellWaves * kWave) void k_support_cells_masked(
+    CellGrid g, const int64_t* __restrict__ cell_start, const int32_t* __restrict__ cell_part,
+    const uint64_t* __restrict__ masks, int32_t* __restrict__ overflow, const int64_t* __restrict__ row_ptr,
+    int32_t* __restrict__ col) {
+    constexpr int CAP = CellCap<DIM>::v;
+    constexpr int NW = CAP / kWave;
+    __shared__ int32_t sq[kCellWaves][CAP];
+    __shared__ int32_t buf[kCellWaves][DCPSE_MAX_ROW];
+    __shared__ int32_t mark[kCellWaves][kWave];
+    __shared__ long long seg_cst[kCellWaves][kWave];
+    __shared__ int32_t seg_ex[kCellWaves][kWave];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const unsigned G = gridDim.x;
+    const unsigned lb = (G % 8u == 0u) ? (blockIdx.x % 8u) * (G / 8u) + blockIdx.x / 8u : blockIdx.x;
+    for (int64_t c = (int64_t)lb * kCellWaves + wv; c < g.ncell; c += (int64_t)G * kCellWaves) {
+        const int64_t pst = cell_start[c];
+        const int np = (int)(cell_start[c + 1] - pst);
+        if (np == 0) continue;
+        int cc[3] = {0, 0, 0};
+        {
+            int64_t rem = c;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                cc[d] = (int)(rem % g.nc[d]);
+                rem /= g.nc[d];
+            }
+        }
+        constexpr int ROWS = DIM == 3 ? 9 : 3;
+        long long cst = 0;
+        int ccnt = 0;
+        {
+            const int row = lane >> 1, part = lane & 1;
+            if (row < ROWS) {
+                int cy = cc[1] + row % 3 - 1, cz = DIM == 3 ? cc[2] + row / 3 - 1 : 0;
+                bool ok = true;
+                if ((g.periodic >> 1) & 1u) cy = (cy + g.nc[1]) % g.nc[1];
+                else ok &= cy >= 0 && cy < g.nc[1];
+                if (DIM == 3) {
+                    if ((g.periodic >> 2) & 1u) cz = (cz + g.nc[2]) % g.nc[2];
+                    else ok &= cz >= 0 && cz < g.nc[2];
+                }
+                int xa = cc[0] - 1, xb = cc[0] + 1;
+                if (g.periodic & 1u) {
+                    if (xa < 0) {
+                        if (part == 0) xa += g.nc[0], xb = g.nc[0] - 1;
+                        else xa = 0;
+                    } else if (xb >= g.nc[0]) {
+                        if (part == 0) xb = g.nc[0] - 1;
+                        else xa = 0, xb -= g.nc[0];
+                    } else if (part == 1) {
+                        ok = false;
+                    }
+                } else {
+                    xa = max(xa, 0);
+                    xb = min(xb, g.nc[0] - 1);
+                    ok &= part == 0;
+                }
+                if (ok && xa <= xb) {
+                    const int64_t lin = (int64_t)g.nc[0] * ((int64_t)cy + (int64_t)g.nc[1] * cz);
+                    cst = cell_start[lin + xa];
+                    ccnt = (int)(cell_start[lin + xb + 1] - cst);
+                }
+            }
+        }
+        const int incl = wave_incl_scan(ccnt);
+        const int total = __builtin_amdgcn_readlane(incl, 63);
+        if (total > CAP) continue;  // filled from positions (spill_only pass)
+        const int excl = incl - ccnt;
+        seg_cst[wv][lane] = cst;
+        seg_ex[wv][lane] = excl;
+        int carry = 0;
+        for (int base = 0; base < total; base += kWave) {
+            mark[wv][lane] = -1;
+            wave_lds_sync();
+            if (ccnt > 0 && excl >= base && excl < base + kWave) mark[wv][excl - base] = lane;
+            wave_lds_sync();
+            const int seg = max(wave_incl_max(mark[wv][lane]), carry);
+            carry = __builtin_amdgcn_readlane(seg, 63);
+            const int k = base + lane;
+            if (k < total) sq[wv][k] = cell_part[seg_cst[wv][seg] + (k - seg_ex[wv][seg])];
+        }
+        int pid_l = 0;
+        int64_t rs_l = 0, re_l = 0;
+        if (lane < np) {
+            pid_l = cell_part[pst + lane];
+            rs_l = row_ptr[pid_l];
+            re_l = row_ptr[pid_l + 1];
+        }
+        const int nr = (total + kWave - 1) / kWave;
+        uint64_t mw = lane < nr ? masks[pst * NW + lane] : 0;
+        wave_lds_sync();
+        for (int i = 0; i < np; ++i) {
+            int64_t rs, re;
+            if (i < kWave) {
+                rs = readlane_i64(rs_l, i);
+                re = readlane_i64(re_l, i);
+            } else {
+                const int pid = cell_part[pst + i];
+                rs = row_ptr[pid];
+                re = row_ptr[pid + 1];
+            }
+            const uint64_t cur = mw;
+            if (i + 1 < np) mw = lane < nr ? masks[(pst + i + 1) * NW + lane] : 0;
+            int nacc = 0;
+            for (int r = 0; r < nr; ++r) {
+                const uint64_t m = (uint64_t)readlane_i64((int64_t)cur, r);
+                const bool take = (m >> lane) & 1u;
+                const int at = nacc + lane_prefix(m);
+                if (take && at < DCPSE_MAX_ROW) buf[wv][at] = sq[wv][r * kWave + lane];
+                nacc += __popcll(m);
+            }
+            wave_lds_sync();
+            const int nn = min(nacc, DCPSE_MAX_ROW);
+            if ((int64_t)nacc != re - rs) {
+                if (lane == 0) atomicOr(overflow, 2);
+            } else {
+                store_ranked(buf[wv], nn, col + rs, lane);
+            }
+            wave_lds_sync();
+        }
+        wave_lds_sync();
     }
 }
 
@@ -424,8 +825,31 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
     const int K = dim == 3 ? std::min(4, std::max(2, (int)std::ceil(spread - 1e-9)))
                            : std::min(8, std::max(4, (int)std::ceil(spread - 1e-9)));
     CellGrid g;
-    dcpse_status st = make_grid(ctx, dim, box_lo, box_hi, periodic, rmax, K, &g);
-    if (st) return st;
+    dcpse_status st;
+    // one particle set and one cut-off: a wavefront per cell of edge >= rcut
+    // (k_support_cells) while the mean candidate list of a cell fits its LDS
+    // slice with room to spare; otherwise a wavefront per particle
+    bool by_cell = same && !rcut_pp && n_dom > 0;
+    if (by_cell) {
+        if ((st = make_grid(ctx, dim, box_lo, box_hi, periodic, rmax, 1, &g))) return st;
+        const double cand = (double)n_sup * (dim == 3 ? 27.0 : 9.0) / (double)g.ncell;
+        by_cell = cand <= 0.75 * (dim == 3 ? CellCap<3>::v : CellCap<2>::v);
+    }
+    if (!by_cell && (st = make_grid(ctx, dim, box_lo, box_hi, periodic, rmax, K, &g))) return st;
+    const int64_t cb = by_cell ? std::max<int64_t>(1, std::min<int64_t>((g.ncell + kCellWaves - 1) / kCellWaves,
+                                                                         (int64_t)ctx->num_cus * 16))
+                               : 0;
+    const unsigned cell_blocks = (unsigned)(cb >= 8 ? cb / 8 * 8 : cb);
+
+    // the same-set, single-cut-off fill reuses the cell list and take masks of
+    // the count just before it when the arguments match
+    dcpse_support_cache& sc = ctx->sup;
+    const bool count_pass = col == nullptr;
+    const bool cached = by_cell && !count_pass && sc.valid && sc.dim == dim && sc.pos == pos_sup &&
+                        sc.n == n_sup && sc.rcut == rmax && sc.periodic == periodic &&
+                        std::equal(box_lo, box_lo + dim, sc.lo) && std::equal(box_hi, box_hi + dim, sc.hi);
+    sc.valid = false;
+    const int mask_words = dim == 3 ? CellCap<3>::v / kWave : CellCap<2>::v / kWave;
 
     Workspace w;
     size_t need = (size_t)n_sup * 4 * 2 + (size_t)dim * (n_sup + 1) * 8 + (size_t)(g.ncell + 1) * 8 +
@@ -441,31 +865,66 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
     int32_t* flags = w.take<int32_t>(64);
     Workspace ws_scan = w;
     if (!flags) return fail(ctx, DCPSE_ERR_INVALID, "workspace sizing error");
-
     hipStream_t s = ctx->stream;
-    fill_i32(s, cell_cnt, g.ncell + 1, 0);
+
+    uint64_t* masks = nullptr;
+    if (by_cell && (count_pass || cached)) {
+        // cell list and masks in the context's support cache
+        const size_t cap = (((size_t)(g.ncell + 1) * 8 + 255) & ~(size_t)255) +
+                           (((size_t)(n_sup + 1) * 4 + 255) & ~(size_t)255) +
+                           (((size_t)dim * (n_sup + 1) * 8 + 255) & ~(size_t)255) +
+                           (size_t)(n_sup + 1) * mask_words * 8;
+        if (count_pass && sc.cap < cap) {
+            DCPSE_HIP_CHECK(ctx, hipStreamSynchronize(s));
+            if (sc.buf) (void)hipFree(sc.buf);
+            sc.buf = nullptr;
+            sc.cap = 0;
+            DCPSE_HIP_CHECK(ctx, hipMalloc(&sc.buf, cap));
+            sc.cap = cap;
+        }
+        Workspace cw;
+        cw.base = static_cast<char*>(sc.buf);
+        cw.cap = sc.cap;
+        cell_start = cw.take<int64_t>(g.ncell + 1);
+        cell_part = cw.take<int32_t>(n_sup + 1);
+        pos_cell = cw.take<double>((size_t)dim * (n_sup + 1));
+        masks = cw.take<uint64_t>((size_t)(n_sup + 1) * mask_words);
+        if (!masks) return fail(ctx, DCPSE_ERR_INVALID, "support cache sizing error");
+    }
+
     fill_i32(s, flags, 64, 0);
     const int64_t tb = std::max<int64_t>(1, std::min<int64_t>((n_sup + 255) / 256, 16384));
-    if (n_sup > 0) {
-        if (dim == 2) k_cell_assign<2><<<(unsigned)tb, 256, 0, s>>>(pos_sup, n_sup, g, cell_id, cell_cnt);
-        else k_cell_assign<3><<<(unsigned)tb, 256, 0, s>>>(pos_sup, n_sup, g, cell_id, cell_cnt);
-        DCPSE_LAUNCH_CHECK(ctx);
-    }
-    if ((st = scan_counts(ctx, cell_cnt, g.ncell, cell_start, &ws_scan))) return st;
-    fill_i32(s, cell_cnt, g.ncell + 1, 0);  // reuse as cursor
-    if (n_sup > 0) {
-        if (dim == 2)
-            k_cell_fill<2><<<(unsigned)tb, 256, 0, s>>>(cell_id, n_sup, pos_sup, cell_start, cell_cnt, cell_part,
-                                                        pos_cell);
-        else
-            k_cell_fill<3><<<(unsigned)tb, 256, 0, s>>>(cell_id, n_sup, pos_sup, cell_start, cell_cnt, cell_part,
-                                                        pos_cell);
-        DCPSE_LAUNCH_CHECK(ctx);
+    if (!cached) {
+        fill_i32(s, cell_cnt, g.ncell + 1, 0);
+        if (n_sup > 0) {
+            if (dim == 2) k_cell_assign<2><<<(unsigned)tb, 256, 0, s>>>(pos_sup, n_sup, g, cell_id, cell_cnt);
+            else k_cell_assign<3><<<(unsigned)tb, 256, 0, s>>>(pos_sup, n_sup, g, cell_id, cell_cnt);
+            DCPSE_LAUNCH_CHECK(ctx);
+        }
+        if ((st = scan_counts(ctx, cell_cnt, g.ncell, cell_start, &ws_scan))) return st;
+        fill_i32(s, cell_cnt, g.ncell + 1, 0);  // reuse as cursor
+        if (n_sup > 0) {
+            if (dim == 2)
+                k_cell_fill<2><<<(unsigned)tb, 256, 0, s>>>(cell_id, n_sup, pos_sup, cell_start, cell_cnt,
+                                                            cell_part, pos_cell);
+            else
+                k_cell_fill<3><<<(unsigned)tb, 256, 0, s>>>(cell_id, n_sup, pos_sup, cell_start, cell_cnt,
+                                                            cell_part, pos_cell);
+            DCPSE_LAUNCH_CHECK(ctx);
+        }
     }
     const int64_t rb = std::max<int64_t>(1, std::min<int64_t>((n_dom + kRowWaves - 1) / kRowWaves,
                                                               (int64_t)ctx->num_cus * 16));
-    if (col == nullptr) {
-        if (n_dom > 0) {
+    if (count_pass) {
+        if (by_cell) {
+            if (dim == 2)
+                k_support_cells<2, false><<<cell_blocks, kCellWaves * kWave, 0, s>>>(
+                    n_sup, rcut, g, cell_start, cell_part, pos_cell, counts, flags, nullptr, nullptr, masks, 0);
+            else
+                k_support_cells<3, false><<<cell_blocks, kCellWaves * kWave, 0, s>>>(
+                    n_sup, rcut, g, cell_start, cell_part, pos_cell, counts, flags, nullptr, nullptr, masks, 0);
+            DCPSE_LAUNCH_CHECK(ctx);
+        } else if (n_dom > 0) {
             if (dim == 2)
                 k_support_rows<2, false><<<(unsigned)rb, kRowWaves * kWave, 0, s>>>(
                     pos_sup, n_sup, pos_dom, n_dom, same, rcut, rcut_pp, g, cell_start, cell_part, pos_cell, counts,
@@ -489,8 +948,41 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
         if (nnz >= ((int64_t)1 << 31) * 64)
             return fail(ctx, DCPSE_ERR_RANGE, "nnz out of range");
         if (nnz_out) *nnz_out = nnz;
+        if (by_cell) {
+            sc.valid = true;
+            sc.spill = (ovf & 4) != 0;
+            sc.dim = dim;
+            sc.pos = pos_sup;
+            sc.n = n_sup;
+            sc.rcut = rmax;
+            sc.periodic = periodic;
+            for (int d = 0; d < 3; ++d) {
+                sc.lo[d] = d < dim ? box_lo[d] : 0.0;
+                sc.hi[d] = d < dim ? box_hi[d] : 0.0;
+            }
+        }
     } else {
-        if (n_dom > 0) {
+        if (cached) {
+            if (dim == 2)
+                k_support_cells_masked<2><<<cell_blocks, kCellWaves * kWave, 0, s>>>(g, cell_start, cell_part, masks,
+                                                                                      flags, row_ptr, col);
+            else
+                k_support_cells_masked<3><<<cell_blocks, kCellWaves * kWave, 0, s>>>(g, cell_start, cell_part, masks,
+                                                                                      flags, row_ptr, col);
+            DCPSE_LAUNCH_CHECK(ctx);
+        }
+        if (by_cell && (!cached || sc.spill)) {
+            const int spill_only = cached ? 1 : 0;
+            if (dim == 2)
+                k_support_cells<2, true><<<cell_blocks, kCellWaves * kWave, 0, s>>>(
+                    n_sup, rcut, g, cell_start, cell_part, pos_cell, counts, flags, row_ptr, col, nullptr,
+                    spill_only);
+            else
+                k_support_cells<3, true><<<cell_blocks, kCellWaves * kWave, 0, s>>>(
+                    n_sup, rcut, g, cell_start, cell_part, pos_cell, counts, flags, row_ptr, col, nullptr,
+                    spill_only);
+            DCPSE_LAUNCH_CHECK(ctx);
+        } else if (!by_cell && n_dom > 0) {
             if (dim == 2)
                 k_support_rows<2, true><<<(unsigned)rb, kRowWaves * kWave, 0, s>>>(
                     pos_sup, n_sup, pos_dom, n_dom, same, rcut, rcut_pp, g, cell_start, cell_part, pos_cell, counts,
@@ -504,7 +996,7 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
         int32_t ovf = 0;
         DCPSE_HIP_CHECK(ctx, hipMemcpyAsync(&ovf, flags, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         DCPSE_HIP_CHECK(ctx, hipStreamSynchronize(s));
-        if (ovf)
+        if (ovf & 3)
             return fail(ctx, DCPSE_ERR_INVALID,
                         "support fill disagrees with row_ptr (positions or rcut changed since count?)");
     }

@@ -81,6 +81,78 @@ def test_support_cutoff_spread_and_two_sets_match_oracle(oracle, cuda):
         assert np.array_equal(vl2.col.cpu().numpy(), col2), periodic
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_support_clustered_cells_match_oracle(dim, oracle, cuda):
+    """One cut-off, one set, a uniform background plus a dense cluster: the
+    cluster's cells hold several times the candidates one LDS copy takes, so
+    the per-cell walk restages them in chunks per particle; mixed periodic and
+    open dimensions, a periodic dimension only 3 cells wide."""
+    from openfpm_numerics_amd import ParticleSet
+
+    rng = np.random.default_rng(21 + dim)
+    if dim == 3:
+        hi = np.array([1.0, 1.0, 0.35])
+        bg = rng.random((2400, 3)) * hi
+        cl = 0.3 + 0.2 * rng.random((600, 3))
+        cl[:, 2] = 0.1 + 0.1 * rng.random(600)
+        rcut, periodic = 0.1, 0b101
+    else:
+        hi = np.array([1.0, 0.16])
+        bg = rng.random((600, 2)) * hi
+        cl = 0.4 + 0.1 * rng.random((500, 2))
+        cl[:, 1] = 0.05 + 0.05 * rng.random(500)
+        rcut, periodic = 0.05, 0b10
+    pos = np.concatenate([bg, cl])
+    lo = np.zeros(dim)
+    bc = [(periodic >> d) & 1 for d in range(dim)]
+    ps = ParticleSet(pos, lo, hi, bc=bc, device=cuda)
+    vl = ps.getVerlet(rcut)
+    rp, col = oracle.support(pos, rcut, lo, hi, periodic)
+    print(f"[support] clustered dim={dim} N={len(pos)} nnz={col.size} max row={np.diff(rp).max()}", flush=True)
+    assert np.array_equal(vl.row_ptr.cpu().numpy(), rp)
+    assert np.array_equal(vl.col.cpu().numpy(), col)
+
+
+def test_support_fill_after_other_count_matches_oracle(oracle, cuda):
+    """The fill reuses the cell list and take masks of the count just before it
+    only when the arguments match: a count of another cut-off in between, and a
+    second fill of the same rows, take the full distance path and agree."""
+    from openfpm_numerics_amd import _lib, workloads as W
+
+    w = W.jittered_3d_holes(20, seed=11)
+    rp, col = oracle.support(w["pos"], w["rcut"], w["box_lo"], w["box_hi"], w["periodic"])
+    ctx = _lib.context(0)
+    pos = torch.as_tensor(np.ascontiguousarray(w["pos"].T), device=cuda)
+    n = pos.shape[1]
+    lo = np.ascontiguousarray(w["box_lo"], dtype=np.float64)
+    hi = np.ascontiguousarray(w["box_hi"], dtype=np.float64)
+
+    def count(rc):
+        rct = torch.tensor([rc], dtype=torch.float64, device=cuda)
+        row_ptr = torch.empty(n + 1, dtype=torch.int64, device=cuda)
+        nnz = np.zeros(1, np.int64)
+        ctx.call("dcpse_support_count", 3, _lib.ptr(pos), n, None, n, _lib.ptr(rct), 0, lo.ctypes.data,
+                 hi.ctypes.data, w["periodic"], _lib.ptr(row_ptr), nnz.ctypes.data)
+        return row_ptr, int(nnz[0])
+
+    def fill(rc, row_ptr, nnz):
+        rct = torch.tensor([rc], dtype=torch.float64, device=cuda)
+        c = torch.full((nnz,), -1, dtype=torch.int32, device=cuda)
+        ctx.call("dcpse_support_fill", 3, _lib.ptr(pos), n, None, n, _lib.ptr(rct), 0, lo.ctypes.data,
+                 hi.ctypes.data, w["periodic"], _lib.ptr(row_ptr), _lib.ptr(c))
+        return c.cpu().numpy()
+
+    row_ptr, nnz = count(w["rcut"])
+    assert np.array_equal(row_ptr.cpu().numpy(), rp)
+    rp_b, nnz_b = count(0.8 * w["rcut"])
+    assert np.array_equal(fill(w["rcut"], row_ptr, nnz), col)
+    assert np.array_equal(fill(w["rcut"], row_ptr, nnz), col)
+    row_ptr, nnz = count(w["rcut"])
+    assert np.array_equal(fill(w["rcut"], row_ptr, nnz), col)
+    rp_s, col_s = oracle.support(w["pos"], 0.8 * w["rcut"], w["box_lo"], w["box_hi"], w["periodic"])
+    assert np.array_equal(rp_b.cpu().numpy(), rp_s)
+
+
 SIGS = {
     2: [(1, 0), (0, 1), (2, 0), (0, 2), (1, 1)],
     3: [(1, 0, 0), (0, 0, 1), (2, 0, 0), (0, 1, 1)],
