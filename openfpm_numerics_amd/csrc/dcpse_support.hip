@@ -217,53 +217,96 @@ __device__ __forceinline__ int lane_xor(int v, int lane) {
     else return __builtin_amdgcn_ds_bpermute((lane ^ S) << 2, v);
 }
 
-// Bitonic network over the 64 lanes (element index = lane): merge steps of
-// block size SIZE from stride STRIDE down to 1. A block sorts ascending when
-// (lane & SIZE) == 0, flipped by `desc`; in each pair the lower lane keeps
-// the minimum of an ascending block.
-template <int SIZE, int STRIDE>
-__device__ __forceinline__ void bitonic_merge(int& v, int lane, bool desc) {
-    const int p = lane_xor<STRIDE>(v, lane);
-    const bool asc = ((lane & SIZE) == 0) != desc;
-    const bool lower = (lane & STRIDE) == 0;
-    v = (asc == lower) ? min(v, p) : max(v, p);
-    if constexpr (STRIDE > 1) bitonic_merge<SIZE, STRIDE / 2>(v, lane, desc);
+// Bitonic network over R * 64 keys held R per lane (element r * 64 + lane in
+// v[r]): merge steps of block size SIZE from stride STRIDE down to 1. A block
+// sorts ascending when (element & SIZE) == 0; in each pair the lower element
+// keeps the minimum of an ascending block. Strides below 64 pair lanes
+// (lane_xor), larger ones pair registers of one lane.
+template <int R, int SIZE, int STRIDE>
+__device__ __forceinline__ void bitonic_merge(int (&v)[R], int lane) {
+    if constexpr (STRIDE >= kWave) {
+        constexpr int RS = STRIDE / kWave;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if ((r & RS) == 0) {
+                const bool asc = ((r * kWave) & SIZE) == 0;
+                const int lo = min(v[r], v[r | RS]), hi = max(v[r], v[r | RS]);
+                v[r] = asc ? lo : hi;
+                v[r | RS] = asc ? hi : lo;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int p = lane_xor<STRIDE>(v[r], lane);
+            const bool asc = ((r * kWave + lane) & SIZE) == 0;
+            const bool lower = (lane & STRIDE) == 0;
+            v[r] = (asc == lower) ? min(v[r], p) : max(v[r], p);
+        }
+    }
+    if constexpr (STRIDE > 1) bitonic_merge<R, SIZE, STRIDE / 2>(v, lane);
 }
-template <int SIZE>
-__device__ __forceinline__ void bitonic_sort(int& v, int lane, bool desc) {
-    if constexpr (SIZE > 2) bitonic_sort<SIZE / 2>(v, lane, desc);
-    bitonic_merge<SIZE, SIZE / 2>(v, lane, desc);
+template <int R, int SIZE>
+__device__ __forceinline__ void bitonic_sort(int (&v)[R], int lane) {
+    if constexpr (SIZE > 2) bitonic_sort<R, SIZE / 2>(v, lane);
+    bitonic_merge<R, SIZE, SIZE / 2>(v, lane);
 }
 
-// Writes the n distinct keys keys[0..n) of a wave's LDS list to out[0..n) in
-// ascending order. Up to 128 keys are sorted in registers by a bitonic
-// network (one or two per lane, padded with INT_MAX, which no particle index
-// reaches) and stored contiguously; longer lists write each key at its rank
-// (the number of smaller keys), counted from LDS.
+// Sorts the n <= R * 64 distinct keys keys[0..n) in registers (padded with
+// INT_MAX, which no particle index reaches) and stores them ascending to
+// out[0..n) (out may alias keys).
+template <int R>
+__device__ __forceinline__ void sort_store(const int32_t* keys, int n, int32_t* out, int lane) {
+    int v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = r * kWave + lane < n ? keys[r * kWave + lane] : INT_MAX;
+    bitonic_sort<R, R * kWave>(v, lane);
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (r * kWave + lane < n) out[r * kWave + lane] = v[r];
+}
+
+// Rows longer than this leave the row kernels unsorted; k_sort_long_rows
+// sorts them afterwards, so the row kernels keep the registers of two keys
+// per lane.
+constexpr int kShortRow = 2 * kWave;
+
+// Writes the n distinct keys of a wave's LDS list to out[0..n): sorted
+// ascending when n <= kShortRow, as they are otherwise.
 __device__ __forceinline__ void store_ranked(const int32_t* keys, int n, int32_t* __restrict__ out, int lane) {
     if (n <= kWave) {
-        int a = lane < n ? keys[lane] : INT_MAX;
-        bitonic_sort<kWave>(a, lane, false);
-        if (lane < n) out[lane] = a;
-    } else if (n <= 2 * kWave) {
-        // a: elements 0..63 ascending, b: 64..127 descending, then one merge
-        int a = keys[lane];
-        int b = lane + kWave < n ? keys[lane + kWave] : INT_MAX;
-        bitonic_sort<kWave>(a, lane, false);
-        bitonic_sort<kWave>(b, lane, true);
-        const int lo = min(a, b), hi = max(a, b);
-        a = lo;
-        b = hi;
-        bitonic_merge<2 * kWave, kWave / 2>(a, lane, false);
-        bitonic_merge<2 * kWave, kWave / 2>(b, lane, false);
-        out[lane] = a;
-        if (lane + kWave < n) out[lane + kWave] = b;
+        sort_store<1>(keys, n, out, lane);
+    } else if (n <= kShortRow) {
+        sort_store<2>(keys, n, out, lane);
     } else {
-        for (int e = lane; e < n; e += kWave) {
-            const int key = keys[e];
-            int rank = 0;
-            for (int j = 0; j < n; ++j) rank += keys[j] < key;
-            out[rank] = key;
+        for (int e = lane; e < n; e += kWave) out[e] = keys[e];
+    }
+}
+
+// Sorts, in place, every row of the CSR longer than kShortRow (at most
+// DCPSE_MAX_ROW = 1024 entries). Each wavefront reads 64 row lengths at a
+// time and sorts the long ones among them in registers, up to 16 keys per lane.
+__global__ __launch_bounds__(kRowWaves * kWave) void k_sort_long_rows(int64_t n_rows,
+                                                                      const int64_t* __restrict__ row_ptr,
+                                                                      int32_t* col) {
+    static_assert(DCPSE_MAX_ROW <= 16 * kWave, "the row sort covers at most 1024 keys");
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)blockIdx.x * kRowWaves + (threadIdx.x >> 6);
+    const int64_t nw = (int64_t)gridDim.x * kRowWaves;
+    for (int64_t base = wave * kWave; base < n_rows; base += nw * kWave) {
+        const int64_t p = base + lane;
+        const int64_t rs_l = p < n_rows ? row_ptr[p] : 0;
+        const int len_l = p < n_rows ? (int)(row_ptr[p + 1] - rs_l) : 0;
+        unsigned long long m = __ballot(len_l > kShortRow);
+        while (m) {
+            const int j = __builtin_ctzll(m);
+            m &= m - 1;
+            const int64_t rs = readlane_i64(rs_l, j);
+            const int n = min(__builtin_amdgcn_readlane(len_l, j), DCPSE_MAX_ROW);
+            int32_t* row = col + rs;
+            if (n <= 4 * kWave) sort_store<4>(row, n, row, lane);
+            else if (n <= 8 * kWave) sort_store<8>(row, n, row, lane);
+            else sort_store<16>(row, n, row, lane);
         }
     }
 }
@@ -991,6 +1034,12 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
                 k_support_rows<3, true><<<(unsigned)rb, kRowWaves * kWave, 0, s>>>(
                     pos_sup, n_sup, pos_dom, n_dom, same, rcut, rcut_pp, g, cell_start, cell_part, pos_cell, counts,
                     flags, row_ptr, col);
+            DCPSE_LAUNCH_CHECK(ctx);
+        }
+        if (n_dom > 0) {
+            const int64_t lb = std::max<int64_t>(
+                1, std::min<int64_t>((n_dom + kRowWaves * kWave - 1) / (kRowWaves * kWave), (int64_t)ctx->num_cus * 8));
+            k_sort_long_rows<<<(unsigned)lb, kRowWaves * kWave, 0, s>>>(n_dom, row_ptr, col);
             DCPSE_LAUNCH_CHECK(ctx);
         }
         int32_t ovf = 0;
