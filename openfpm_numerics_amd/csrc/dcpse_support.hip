@@ -1090,6 +1090,66 @@ __global__ __launch_bounds__(kRowWaves * kWave) void k_tfill(int64_t n_rows, con
     }
 }
 
+// Transpose of a structurally symmetric CSR with strictly ascending rows (a
+// support of one set with one cut-off): the transpose has the same row_ptr
+// and col, and its entry k = (q, p) of row q takes the weight of entry (p, q),
+// found by searching q in row p. One wavefront per output row q, a lane per
+// entry, so the writes are coalesced and there are no atomics and no sort.
+// The search starts at the mirror position len_p - 1 - m (exact for a
+// lattice stencil, m = index of p in row q), then bisects. Raises *asym,
+// leaving the outputs to the general path, when a row is not strictly
+// ascending, when a transposed length differs, or when q is missing from
+// row p; each row checks its own entries, so both directions of every pair
+// are covered.
+template <typename T>
+__global__ __launch_bounds__(kRowWaves * kWave) void k_tfill_sym(int64_t n_rows, const int64_t* __restrict__ row_ptr,
+                                                                 const int32_t* __restrict__ col,
+                                                                 const T* __restrict__ W,
+                                                                 const int64_t* __restrict__ rowT_ptr,
+                                                                 int32_t* __restrict__ colT, T* __restrict__ WT,
+                                                                 T* __restrict__ diag, int32_t* __restrict__ asym) {
+    const int lane = threadIdx.x & 63;
+    // consecutive logical blocks on one XCD (blocks are dealt round-robin over
+    // the 8 XCDs): the rows searched by neighbouring rows q share cache lines
+    const unsigned G = gridDim.x;
+    const unsigned lb = (G % 8u == 0u) ? (blockIdx.x % 8u) * (G / 8u) + blockIdx.x / 8u : blockIdx.x;
+    for (int64_t q = (int64_t)lb * kRowWaves + (threadIdx.x >> 6); q < n_rows; q += (int64_t)G * kRowWaves) {
+        const int64_t rs = row_ptr[q], re = row_ptr[q + 1];
+        bool bad = lane == 0 && (rowT_ptr[q] != rs || (q + 1 == n_rows && rowT_ptr[n_rows] != re));
+        double acc = 0.0;
+        for (int64_t k = rs + lane; k < re; k += kWave) {
+            const int32_t p = col[k];
+            acc += (double)W[k];
+            colT[k] = p;
+            if (k + 1 < re && col[k + 1] <= p) bad = true;
+            if (p < 0 || p >= n_rows) {
+                bad = true;
+                continue;
+            }
+            const int64_t ps = row_ptr[p], pe = row_ptr[p + 1];
+            const int64_t mirror = pe - 1 - (k - rs);
+            int64_t lo = ps, hi = pe;
+            if (mirror >= ps && mirror < pe) {
+                const int32_t c = col[mirror];
+                if (c == (int32_t)q) lo = hi = mirror;
+                else if (c < (int32_t)q) lo = mirror + 1;
+                else hi = mirror;
+            }
+            // first position in [lo, hi) with col >= q
+            while (lo < hi) {
+                const int64_t mid = lo + ((hi - lo) >> 1);
+                if (col[mid] < (int32_t)q) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo < pe && col[lo] == (int32_t)q) WT[k] = W[lo];
+            else bad = true;
+        }
+        acc = wave_sum(acc);
+        if (lane == 0) diag[q] = (T)(-acc);
+        if (__ballot(bad) && lane == 0) atomicOr(asym, 1);
+    }
+}
+
 // Sorts each transposed row by ascending source row (rank sort in LDS); rows
 // longer than DCPSE_MAX_ROW raise *overflow and are left unsorted.
 template <typename T>
@@ -1143,6 +1203,15 @@ static dcpse_status transpose_fill(dcpse_ctx* ctx, int64_t n_rows, int64_t n_col
     if (n_rows > 0) {
         int64_t b = std::max<int64_t>(1, std::min<int64_t>((n_rows + kRowWaves - 1) / kRowWaves,
                                                            (int64_t)ctx->num_cus * 32));
+        // a symmetric pattern with sorted rows first; the general path below
+        // only if it reports otherwise
+        const unsigned bs = (unsigned)(b >= 8 ? b / 8 * 8 : b);
+        k_tfill_sym<T><<<bs, kRowWaves * kWave, 0, s>>>(n_rows, row_ptr, col, W, rowT_ptr, colT, WT, diag, flags + 1);
+        DCPSE_LAUNCH_CHECK(ctx);
+        int32_t asym = 0;
+        DCPSE_HIP_CHECK(ctx, hipMemcpyAsync(&asym, flags + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        DCPSE_HIP_CHECK(ctx, hipStreamSynchronize(s));
+        if (!asym) return DCPSE_OK;
         k_tfill<T><<<(unsigned)b, kRowWaves * kWave, 0, s>>>(n_rows, row_ptr, col, W, rowT_ptr, cursor, colT, WT,
                                                               diag);
         DCPSE_LAUNCH_CHECK(ctx);

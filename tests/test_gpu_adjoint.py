@@ -7,18 +7,27 @@ import torch
 pytestmark = pytest.mark.gpu
 
 
-def _setup(cuda, n=64, seed=4):
+def _setup(cuda, n=64, seed=4, spread=False):
     from openfpm_numerics_amd import Laplacian, ParticleSet, workloads as W
+    from openfpm_numerics_amd.particles import VerletList
 
     w = W.jittered_2d(n, seed=seed)
     ps = ParticleSet(w["pos"], w["box_lo"], w["box_hi"], bc=[1, 1], device=cuda)
-    vl = ps.getVerlet(w["rcut"])
+    if spread:
+        # per-particle cut-offs: an unsymmetric support pattern
+        rc = w["rcut"] * (1.0 + 0.4 * np.random.default_rng(seed).random(ps.size_local()))
+        vl = VerletList.build(ps, torch.as_tensor(rc, device=cuda))
+    else:
+        vl = ps.getVerlet(w["rcut"])
     return w, ps, vl, Laplacian(ps, vl, 2, w["rcut"])
 
 
-def test_transpose_matches_value_nz_transposed(oracle, cuda):
-    """A^T g on the GPU == (sum_d value_nz(D_d))^T g from the oracle (scipy)."""
-    w, ps, vl, lap = _setup(cuda)
+@pytest.mark.parametrize("spread", [False, True], ids=["one_cutoff", "per_particle_cutoff"])
+def test_transpose_matches_value_nz_transposed(spread, oracle, cuda):
+    """A^T g on the GPU == (sum_d value_nz(D_d))^T g from the oracle (scipy),
+    for a symmetric support pattern (one cut-off: the transpose reuses the
+    pattern) and an unsymmetric one (per-particle cut-offs)."""
+    w, ps, vl, lap = _setup(cuda, spread=spread)
     rp, col = vl.row_ptr.cpu().numpy(), vl.col.cpu().numpy()
     A = None
     for d in range(2):
