@@ -16,13 +16,14 @@ struct dcpse_support_cache {
     void* buf = nullptr;  // grow-only device buffer
     size_t cap = 0;
     bool valid = false;
-    bool spill = false;   // some cell's candidates exceed one LDS copy
+    bool spill = false;   // some cells are left to the position kernel
     int dim = 0;
     const double* pos = nullptr;
     int64_t n = 0;
     double rcut = 0.0;
     double lo[3] = {0.0, 0.0, 0.0}, hi[3] = {0.0, 0.0, 0.0};
     unsigned periodic = 0;
+    int nws = 0;          // mask words per particle
 };
 
 struct dcpse_ctx {

@@ -447,37 +447,254 @@ __global__ __launch_bounds__(kRowWaves * kWave) void k_support_rows(
     }
 }
 
-// Support rows for one cut-off and one particle set (the same-set, uniform
-// rcut case), one wavefront per cell of a grid with cell edge >= rcut. The
-// candidates of a cell are its 3^dim neighbouring cells, scanned as 3^(dim-1)
-// rows along dimension 0 (contiguous in the cell order, split in two where a
-// periodic wrap cuts a row). They are copied once into the wave's LDS slice
-// (particle index and position, kCellCap of them) and then tested against
-// every particle of the cell, so the cell's particles share the candidate
-// setup and global loads. A cell with more candidates than CellCap restages
-// them chunk by chunk for each of its particles (and raises *overflow bit 4).
-// Same results as k_support_rows: the same distance formula on the same
-// position copies, and the same ascending rank order of a row.
-// Count pass with masks != NULL: also stores, for each particle of a cell that
-// fits, which of the cell's candidates it takes, as CellCap / 64 words at
-// masks[slot * (CellCap / 64)], for k_support_cells_masked. Fill pass with
-// spill_only: only the cells that do not fit.
+// ---- one set, one cut-off: a wavefront per cell of edge >= rcut -----------
+// The candidates of a cell are its 3^dim neighbouring cells, scanned as
+// 3^(dim-1) rows along dimension 0 (contiguous in the cell order, split in two
+// where a periodic wrap cuts a row), flattened into one list. The count pass
+// (k_support_cells_count) copies them, CellCap at a time, into the wave's LDS
+// slice (particle index and position) and tests every particle of the cell
+// against each copy, so the cell's particles share the candidate setup and
+// global loads; it stores, per particle, which candidates it takes (one bit
+// each, nws 64-bit words at masks[slot * nws]). The fill pass
+// (k_support_cells_masked) stages only the candidate indices and gathers each
+// row by its mask, with no distance work. Cells with more than kCellMaxPart
+// particles or more candidates than the masks cover (nws * 64) are left to
+// k_support_cells, which tests each particle against positions (restaging
+// them per particle when they exceed one LDS copy). All three use the
+// distance formula of k_support_rows on the same position copies and write
+// rows in ascending order, so the results are the same.
 constexpr int kCellWaves = 2;
+constexpr int kCellMaxPart = 2 * kWave;  // particles of a masked cell
 template <int DIM>
-struct CellCap {
+struct CellCap {  // candidates per LDS copy with positions
     static constexpr int v = DIM == 3 ? 512 : 256;
 };
+template <int DIM>
+struct MaskCap {  // candidates a particle's mask covers at most
+    static constexpr int v = DIM == 3 ? 2048 : 1024;
+};
 
+__device__ __forceinline__ bool masked_cell(int np, int total, int mask_cap) {
+    return np <= kCellMaxPart && total <= mask_cap;
+}
+
+// Block b's cell range: consecutive logical blocks on one XCD (blocks are
+// dealt round-robin over the 8 XCDs), so neighbouring cells, which share most
+// candidates, meet in one L2.
+__device__ __forceinline__ unsigned xcd_block() {
+    const unsigned G = gridDim.x;
+    return (G % 8u == 0u) ? (blockIdx.x % 8u) * (G / 8u) + blockIdx.x / 8u : blockIdx.x;
+}
+
+// The candidate segments of cell c: lane 2i (+1) gets row i of the 3^(dim-1)
+// rows of three cells along dimension 0 (its part past a periodic wrap), as
+// the first cell slot cst and the length ccnt (0 for no segment).
+template <int DIM>
+__device__ __forceinline__ void cell_segments(const CellGrid& g, const int64_t* __restrict__ cell_start, int64_t c,
+                                              int lane, long long& cst, int& ccnt) {
+    int cc[3] = {0, 0, 0};
+    int64_t rem = c;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        cc[d] = (int)(rem % g.nc[d]);
+        rem /= g.nc[d];
+    }
+    constexpr int ROWS = DIM == 3 ? 9 : 3;
+    cst = 0;
+    ccnt = 0;
+    const int row = lane >> 1, part = lane & 1;
+    if (row >= ROWS) return;
+    int cy = cc[1] + row % 3 - 1, cz = DIM == 3 ? cc[2] + row / 3 - 1 : 0;
+    bool ok = true;
+    if ((g.periodic >> 1) & 1u) cy = (cy + g.nc[1]) % g.nc[1];
+    else ok &= cy >= 0 && cy < g.nc[1];
+    if (DIM == 3) {
+        if ((g.periodic >> 2) & 1u) cz = (cz + g.nc[2]) % g.nc[2];
+        else ok &= cz >= 0 && cz < g.nc[2];
+    }
+    int xa = cc[0] - 1, xb = cc[0] + 1;
+    if (g.periodic & 1u) {
+        if (xa < 0) {
+            if (part == 0) xa += g.nc[0], xb = g.nc[0] - 1;
+            else xa = 0;
+        } else if (xb >= g.nc[0]) {
+            if (part == 0) xb = g.nc[0] - 1;
+            else xa = 0, xb -= g.nc[0];
+        } else if (part == 1) {
+            ok = false;
+        }
+    } else {
+        xa = max(xa, 0);
+        xb = min(xb, g.nc[0] - 1);
+        ok &= part == 0;
+    }
+    if (ok && xa <= xb) {
+        const int64_t lin = (int64_t)g.nc[0] * ((int64_t)cy + (int64_t)g.nc[1] * cz);
+        cst = cell_start[lin + xa];
+        ccnt = (int)(cell_start[lin + xb + 1] - cst);
+    }
+}
+
+// Copies candidates [c0, c0 + cn) of a cell (c0 a multiple of 64) to the
+// wave's LDS slice: sq[k - c0] = particle, and with POS its position
+// sx[d * sx_stride + k - c0]. A candidate's segment is the last non-empty one
+// starting at or before it: the segment starts of each round of 64 are
+// marked, a prefix maximum spreads them, and the last one carries over.
+// seg_cst / seg_ex (each lane's segment start slot and candidates before it)
+// must be written before the call; ends with the slice readable.
+template <int DIM, bool POS>
+__device__ __forceinline__ void stage_candidates(int c0, int cn, int ccnt, int excl, int32_t* mark,
+                                                 const long long* seg_cst, const int32_t* seg_ex,
+                                                 const int32_t* __restrict__ cell_part,
+                                                 const double* __restrict__ pos_cell, int64_t n, int32_t* sq,
+                                                 double* sx, int sx_stride, int lane) {
+    int carry = 0;
+    for (int base = 0; base < c0 + cn; base += kWave) {
+        mark[lane] = -1;
+        wave_lds_sync();
+        if (ccnt > 0 && excl >= base && excl < base + kWave) mark[excl - base] = lane;
+        wave_lds_sync();
+        const int seg = max(wave_incl_max(mark[lane]), carry);
+        carry = __builtin_amdgcn_readlane(seg, 63);
+        const int k = base + lane;
+        if (base >= c0 && k < c0 + cn) {
+            const int64_t slot = seg_cst[seg] + (k - seg_ex[seg]);
+            sq[k - c0] = cell_part[slot];
+            if (POS) {
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) sx[d * sx_stride + k - c0] = pos_cell[d * n + slot];
+            }
+        }
+    }
+    wave_lds_sync();
+}
+
+// The distance test of candidate slot t of the LDS copy against xp (the
+// minimum image in periodic dimensions), as in k_support_rows.
+template <int DIM>
+__device__ __forceinline__ bool within_cut(const double* xp, const double* sx, int sx_stride, int t, const CellGrid& g,
+                                           double r2) {
+    double d[3];
+#pragma unroll
+    for (int dd = 0; dd < DIM; ++dd)
+        d[dd] = min_image_r(xp[dd] - sx[dd * sx_stride + t], g.len[dd], g.ilen[dd], (g.periodic >> dd) & 1u);
+    double d2 = d[DIM - 1] * d[DIM - 1];
+#pragma unroll
+    for (int dd = DIM - 2; dd >= 0; --dd) d2 = fma(d[dd], d[dd], d2);
+    return d2 < r2;
+}
+
+// Count pass over the masked cells (see above): row lengths to counts[p] and
+// take masks. The other cells are appended to spill[1 ..] (spill[0] = their
+// number) for k_support_cells.
+template <int DIM>
+__global__ __launch_bounds__(kCellWaves * kWave) void k_support_cells_count(
+    int64_t n, const double* __restrict__ rcut, CellGrid g, const int64_t* __restrict__ cell_start,
+    const int32_t* __restrict__ cell_part, const double* __restrict__ pos_cell, int32_t* __restrict__ counts,
+    int32_t* __restrict__ overflow, uint64_t* __restrict__ masks, int nws, int32_t* __restrict__ spill) {
+    constexpr int CAP = CellCap<DIM>::v;
+    __shared__ int32_t sq[kCellWaves][CAP];
+    __shared__ double sx[kCellWaves][DIM * CAP];
+    __shared__ int32_t mark[kCellWaves][kWave];
+    __shared__ long long seg_cst[kCellWaves][kWave];
+    __shared__ int32_t seg_ex[kCellWaves][kWave];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const double r = rcut[0];
+    const double r2 = r * r;
+    const unsigned G = gridDim.x;
+    for (int64_t c = (int64_t)xcd_block() * kCellWaves + wv; c < g.ncell; c += (int64_t)G * kCellWaves) {
+        const int64_t pst = cell_start[c];
+        const int np = (int)(cell_start[c + 1] - pst);
+        if (np == 0) continue;
+        long long cst;
+        int ccnt;
+        cell_segments<DIM>(g, cell_start, c, lane, cst, ccnt);
+        const int incl = wave_incl_scan(ccnt);
+        const int total = __builtin_amdgcn_readlane(incl, 63);
+        if (!masked_cell(np, total, nws * kWave)) {
+            if (lane == 0) spill[1 + atomicAdd(spill, 1)] = (int32_t)c;
+            continue;
+        }
+        const int excl = incl - ccnt;
+        seg_cst[wv][lane] = cst;
+        seg_ex[wv][lane] = excl;
+        // the cell's particles, one per lane in two rounds of 64
+        int pid0 = 0, pid1 = 0;
+        double x0[3] = {0.0, 0.0, 0.0}, x1[3] = {0.0, 0.0, 0.0};
+        if (lane < np) {
+            pid0 = cell_part[pst + lane];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) x0[d] = pos_cell[d * n + pst + lane];
+        }
+        if (lane + kWave < np) {
+            pid1 = cell_part[pst + kWave + lane];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) x1[d] = pos_cell[d * n + pst + kWave + lane];
+        }
+        int cnt0 = 0, cnt1 = 0;  // lane i: the row length of particle i (i + 64)
+        for (int c0 = 0; c0 < total; c0 += CAP) {
+            const int cn = min(CAP, total - c0);
+            if (c0 > 0) wave_lds_sync();
+            stage_candidates<DIM, true>(c0, cn, ccnt, excl, mark[wv], seg_cst[wv], seg_ex[wv], cell_part, pos_cell,
+                                        n, sq[wv], sx[wv], CAP, lane);
+            // particle i against the copy: its take count, and its mask words
+            // of the copy (lane w: word w), stored together
+            auto test = [&](int i, int pid, const double* xp) {
+                uint64_t mword = 0;
+                int nacc = 0;
+                for (int t0 = 0; t0 < cn; t0 += kWave) {
+                    const int t = min(t0 + lane, cn - 1);
+                    // evaluated without short-circuit: no branches in the loop
+                    const bool near = within_cut<DIM>(xp, sx[wv], CAP, t, g, r2);
+                    const bool take = (t0 + lane < cn) & (sq[wv][t] != pid) & near;
+                    const unsigned long long m = __ballot(take);
+                    if (lane == (t0 >> 6)) mword = m;
+                    nacc += __popcll(m);
+                }
+                if (lane < ((cn + kWave - 1) >> 6)) masks[(pst + i) * nws + (c0 >> 6) + lane] = mword;
+                return nacc;
+            };
+            const int n0 = min(np, kWave);
+            for (int i = 0; i < n0; ++i) {
+                double xp[3];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) xp[d] = readlane_d(x0[d], i);
+                const int nacc = test(i, __builtin_amdgcn_readlane(pid0, i), xp);
+                if (lane == i) cnt0 += nacc;
+            }
+            for (int i = kWave; i < np; ++i) {
+                double xp[3];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) xp[d] = readlane_d(x1[d], i - kWave);
+                const int nacc = test(i, __builtin_amdgcn_readlane(pid1, i - kWave), xp);
+                if (lane == i - kWave) cnt1 += nacc;
+            }
+        }
+        if (lane < np) {
+            counts[pid0] = cnt0;
+            if (cnt0 > DCPSE_MAX_ROW) atomicOr(overflow, 1);
+        }
+        if (lane + kWave < np) {
+            counts[pid1] = cnt1;
+            if (cnt1 > DCPSE_MAX_ROW) atomicOr(overflow, 1);
+        }
+        wave_lds_sync();
+    }
+}
+
+// Count (FILL = false: counts[p]) or fill (FILL = true: col[row_ptr[p] ..])
+// from positions, one particle of the cell at a time: every cell, or with
+// spill != NULL the spill[0] cells listed at spill[1 ..].
 template <int DIM, bool FILL>
 __global__ __launch_bounds__(kCellWaves * kWave) void k_support_cells(
     int64_t n, const double* __restrict__ rcut, CellGrid g, const int64_t* __restrict__ cell_start,
     const int32_t* __restrict__ cell_part, const double* __restrict__ pos_cell, int32_t* __restrict__ counts,
     int32_t* __restrict__ overflow, const int64_t* __restrict__ row_ptr, int32_t* __restrict__ col,
-    uint64_t* __restrict__ masks, int spill_only) {
+    const int32_t* __restrict__ spill) {
     constexpr int CAP = CellCap<DIM>::v;
-    constexpr int NW = CAP / kWave;
-    __shared__ int32_t sq[kCellWaves][CAP];        // candidate particle
-    __shared__ double sx[kCellWaves][DIM][CAP];    // candidate position
+    __shared__ int32_t sq[kCellWaves][CAP];
+    __shared__ double sx[kCellWaves][DIM * CAP];
     __shared__ int32_t buf[FILL ? kCellWaves : 1][FILL ? DCPSE_MAX_ROW : 1];
     __shared__ int32_t mark[kCellWaves][kWave];
     __shared__ long long seg_cst[kCellWaves][kWave];
@@ -486,155 +703,48 @@ __global__ __launch_bounds__(kCellWaves * kWave) void k_support_cells(
     const int wv = threadIdx.x >> 6;
     const double r = rcut[0];
     const double r2 = r * r;
-    // consecutive logical blocks on one XCD (blocks are dealt round-robin over
-    // the 8 XCDs), so neighbouring cells, which share most candidates, meet in
-    // one L2
     const unsigned G = gridDim.x;
-    const unsigned lb = (G % 8u == 0u) ? (blockIdx.x % 8u) * (G / 8u) + blockIdx.x / 8u : blockIdx.x;
-    for (int64_t c = (int64_t)lb * kCellWaves + wv; c < g.ncell; c += (int64_t)G * kCellWaves) {
+    const int64_t ncells = spill ? (int64_t)spill[0] : g.ncell;
+    for (int64_t ci = (int64_t)xcd_block() * kCellWaves + wv; ci < ncells; ci += (int64_t)G * kCellWaves) {
+        const int64_t c = spill ? (int64_t)spill[1 + ci] : ci;
         const int64_t pst = cell_start[c];
         const int np = (int)(cell_start[c + 1] - pst);
         if (np == 0) continue;
-        int cc[3] = {0, 0, 0};
-        {
-            int64_t rem = c;
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) {
-                cc[d] = (int)(rem % g.nc[d]);
-                rem /= g.nc[d];
-            }
-        }
-        constexpr int ROWS = DIM == 3 ? 9 : 3;
-        long long cst = 0;
-        int ccnt = 0;
-        {
-            // lane 2i (+1): row i (its part past a periodic wrap)
-            const int row = lane >> 1, part = lane & 1;
-            if (row < ROWS) {
-                int cy = cc[1] + row % 3 - 1, cz = DIM == 3 ? cc[2] + row / 3 - 1 : 0;
-                bool ok = true;
-                if ((g.periodic >> 1) & 1u) cy = (cy + g.nc[1]) % g.nc[1];
-                else ok &= cy >= 0 && cy < g.nc[1];
-                if (DIM == 3) {
-                    if ((g.periodic >> 2) & 1u) cz = (cz + g.nc[2]) % g.nc[2];
-                    else ok &= cz >= 0 && cz < g.nc[2];
-                }
-                int xa = cc[0] - 1, xb = cc[0] + 1;
-                if (g.periodic & 1u) {
-                    if (xa < 0) {
-                        if (part == 0) xa += g.nc[0], xb = g.nc[0] - 1;
-                        else xa = 0;
-                    } else if (xb >= g.nc[0]) {
-                        if (part == 0) xb = g.nc[0] - 1;
-                        else xa = 0, xb -= g.nc[0];
-                    } else if (part == 1) {
-                        ok = false;
-                    }
-                } else {
-                    xa = max(xa, 0);
-                    xb = min(xb, g.nc[0] - 1);
-                    ok &= part == 0;
-                }
-                if (ok && xa <= xb) {
-                    const int64_t lin = (int64_t)g.nc[0] * ((int64_t)cy + (int64_t)g.nc[1] * cz);
-                    cst = cell_start[lin + xa];
-                    ccnt = (int)(cell_start[lin + xb + 1] - cst);
-                }
-            }
-        }
+        long long cst;
+        int ccnt;
+        cell_segments<DIM>(g, cell_start, c, lane, cst, ccnt);
         const int incl = wave_incl_scan(ccnt);
         const int total = __builtin_amdgcn_readlane(incl, 63);
         const int excl = incl - ccnt;
         seg_cst[wv][lane] = cst;
         seg_ex[wv][lane] = excl;
-        // copies candidates [c0, c0 + cn) to the LDS slice (c0 a multiple of
-        // 64); a candidate's segment is the last non-empty one starting at or
-        // before it (start marks spread by a prefix maximum, carried across
-        // rounds of 64)
-        auto stage = [&](int c0, int cn) {
-            int carry = 0;
-            for (int base = 0; base < c0 + cn; base += kWave) {
-                mark[wv][lane] = -1;
-                wave_lds_sync();
-                if (ccnt > 0 && excl >= base && excl < base + kWave) mark[wv][excl - base] = lane;
-                wave_lds_sync();
-                const int seg = max(wave_incl_max(mark[wv][lane]), carry);
-                carry = __builtin_amdgcn_readlane(seg, 63);
-                const int k = base + lane;
-                if (base >= c0 && k < c0 + cn) {
-                    const int64_t slot = seg_cst[wv][seg] + (k - seg_ex[wv][seg]);
-                    sq[wv][k - c0] = cell_part[slot];
-#pragma unroll
-                    for (int d = 0; d < DIM; ++d) sx[wv][d][k - c0] = pos_cell[d * n + slot];
-                }
-            }
-            wave_lds_sync();
-        };
-        // the cell's first 64 particles, one per lane: index, position and
-        // (FILL) row bounds, loaded together ahead of the per-particle loop
-        int pid_l = 0;
-        double x_l[3] = {0.0, 0.0, 0.0};
-        int64_t rs_l = 0, re_l = 0;
-        if (lane < np) {
-            pid_l = cell_part[pst + lane];
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) x_l[d] = pos_cell[d * n + pst + lane];
-            if (FILL) {
-                rs_l = row_ptr[pid_l];
-                re_l = row_ptr[pid_l + 1];
-            }
-        }
         const bool fits = total <= CAP;
-        if (FILL && spill_only && fits) continue;
-        if (!FILL && !fits && lane == 0) atomicOr(overflow, 4);
-        if (fits) stage(0, total);
+        if (fits)
+            stage_candidates<DIM, true>(0, total, ccnt, excl, mark[wv], seg_cst[wv], seg_ex[wv], cell_part, pos_cell,
+                                        n, sq[wv], sx[wv], CAP, lane);
         for (int i = 0; i < np; ++i) {
-            int pid;
+            const int64_t ps = pst + i;
+            const int pid = cell_part[ps];
             double xp[3];
-            int64_t rs = 0, re = 0;
-            if (i < kWave) {
-                pid = __builtin_amdgcn_readlane(pid_l, i);
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) xp[d] = readlane_d(x_l[d], i);
-                if (FILL) {
-                    rs = readlane_i64(rs_l, i);
-                    re = readlane_i64(re_l, i);
-                }
-            } else {
-                const int64_t ps = pst + i;
-                pid = cell_part[ps];
-#pragma unroll
-                for (int d = 0; d < DIM; ++d) xp[d] = pos_cell[d * n + ps];
-                if (FILL) {
-                    rs = row_ptr[pid];
-                    re = row_ptr[pid + 1];
-                }
-            }
+            for (int d = 0; d < DIM; ++d) xp[d] = pos_cell[d * n + ps];
             int nacc = 0;
-            uint64_t mword = 0;  // lane r: the take mask of candidates [64 r, 64 r + 64)
             for (int c0 = 0; c0 < total; c0 += CAP) {
                 const int cn = min(CAP, total - c0);
                 if (!fits) {
                     wave_lds_sync();
-                    stage(c0, cn);
+                    stage_candidates<DIM, true>(c0, cn, ccnt, excl, mark[wv], seg_cst[wv], seg_ex[wv], cell_part,
+                                                pos_cell, n, sq[wv], sx[wv], CAP, lane);
                 }
                 for (int t0 = 0; t0 < cn; t0 += kWave) {
                     const int t = min(t0 + lane, cn - 1);
                     const int q = sq[wv][t];
-                    double d[3];
-#pragma unroll
-                    for (int dd = 0; dd < DIM; ++dd)
-                        d[dd] = min_image_r(xp[dd] - sx[wv][dd][t], g.len[dd], g.ilen[dd], (g.periodic >> dd) & 1u);
-                    double d2 = d[DIM - 1] * d[DIM - 1];
-#pragma unroll
-                    for (int dd = DIM - 2; dd >= 0; --dd) d2 = fma(d[dd], d[dd], d2);
-                    const bool take = t0 + lane < cn && q != pid && d2 < r2;
+                    const bool near = within_cut<DIM>(xp, sx[wv], CAP, t, g, r2);
+                    const bool take = (t0 + lane < cn) & (q != pid) & near;
                     const unsigned long long m = __ballot(take);
                     if (FILL) {
                         const int at = nacc + lane_prefix(m);
                         if (take && at < DCPSE_MAX_ROW) buf[wv][at] = q;
-                    } else {
-                        if (lane == (t0 >> 6)) mword = m;
                     }
                     nacc += __popcll(m);
                 }
@@ -644,11 +754,11 @@ __global__ __launch_bounds__(kCellWaves * kWave) void k_support_cells(
                     counts[pid] = nacc;
                     if (nacc > DCPSE_MAX_ROW) atomicOr(overflow, 1);
                 }
-                if (masks && fits && lane < NW) masks[(pst + i) * NW + lane] = mword;
             } else {
                 wave_lds_sync();
+                const int64_t rs = row_ptr[pid];
                 const int nn = min(nacc, DCPSE_MAX_ROW);
-                if ((int64_t)nacc != re - rs) {
+                if ((int64_t)nacc != row_ptr[pid + 1] - rs) {
                     if (lane == 0) atomicOr(overflow, 2);
                 } else {
                     store_ranked(buf[wv], nn, col + rs, lane);
@@ -660,18 +770,16 @@ __global__ __launch_bounds__(kCellWaves * kWave) void k_support_cells(
     }
 }
 
-// Fill pass of k_support_cells for the cells whose candidates fit one LDS
-// copy, from the take masks its count pass stored: the candidate indices of a
-// cell are staged once and each particle's row is gathered by its mask, with
-// no distance work. The cell list must be the one the count pass used.
+// Fill pass over the masked cells from the masks of k_support_cells_count
+// (nws words per particle); the cell list must be the one that pass used.
+// Dynamic LDS: kCellWaves * nws * 64 candidate indices.
 template <int DIM>
 __global__ __launch_bounds__(kCellWaves * kWave) void k_support_cells_masked(
     CellGrid g, const int64_t* __restrict__ cell_start, const int32_t* __restrict__ cell_part,
-    const uint64_t* __restrict__ masks, int32_t* __restrict__ overflow, const int64_t* __restrict__ row_ptr,
+    const uint64_t* __restrict__ masks, int nws, int32_t* __restrict__ overflow, const int64_t* __restrict__ row_ptr,
     int32_t* __restrict__ col) {
-    constexpr int CAP = CellCap<DIM>::v;
-    constexpr int NW = CAP / kWave;
-    __shared__ int32_t sq[kCellWaves][CAP];
+    constexpr int MCAP = MaskCap<DIM>::v;
+    extern __shared__ int32_t sq_dyn[];
     __shared__ int32_t buf[kCellWaves][DCPSE_MAX_ROW];
     __shared__ int32_t mark[kCellWaves][kWave];
     __shared__ long long seg_cst[kCellWaves][kWave];
@@ -679,102 +787,50 @@ __global__ __launch_bounds__(kCellWaves * kWave) void k_support_cells_masked(
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     const unsigned G = gridDim.x;
-    const unsigned lb = (G % 8u == 0u) ? (blockIdx.x % 8u) * (G / 8u) + blockIdx.x / 8u : blockIdx.x;
-    for (int64_t c = (int64_t)lb * kCellWaves + wv; c < g.ncell; c += (int64_t)G * kCellWaves) {
+    const int mask_cap = min(nws * kWave, MCAP);
+    int32_t* sq = sq_dyn + wv * mask_cap;
+    for (int64_t c = (int64_t)xcd_block() * kCellWaves + wv; c < g.ncell; c += (int64_t)G * kCellWaves) {
         const int64_t pst = cell_start[c];
         const int np = (int)(cell_start[c + 1] - pst);
         if (np == 0) continue;
-        int cc[3] = {0, 0, 0};
-        {
-            int64_t rem = c;
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) {
-                cc[d] = (int)(rem % g.nc[d]);
-                rem /= g.nc[d];
-            }
-        }
-        constexpr int ROWS = DIM == 3 ? 9 : 3;
-        long long cst = 0;
-        int ccnt = 0;
-        {
-            const int row = lane >> 1, part = lane & 1;
-            if (row < ROWS) {
-                int cy = cc[1] + row % 3 - 1, cz = DIM == 3 ? cc[2] + row / 3 - 1 : 0;
-                bool ok = true;
-                if ((g.periodic >> 1) & 1u) cy = (cy + g.nc[1]) % g.nc[1];
-                else ok &= cy >= 0 && cy < g.nc[1];
-                if (DIM == 3) {
-                    if ((g.periodic >> 2) & 1u) cz = (cz + g.nc[2]) % g.nc[2];
-                    else ok &= cz >= 0 && cz < g.nc[2];
-                }
-                int xa = cc[0] - 1, xb = cc[0] + 1;
-                if (g.periodic & 1u) {
-                    if (xa < 0) {
-                        if (part == 0) xa += g.nc[0], xb = g.nc[0] - 1;
-                        else xa = 0;
-                    } else if (xb >= g.nc[0]) {
-                        if (part == 0) xb = g.nc[0] - 1;
-                        else xa = 0, xb -= g.nc[0];
-                    } else if (part == 1) {
-                        ok = false;
-                    }
-                } else {
-                    xa = max(xa, 0);
-                    xb = min(xb, g.nc[0] - 1);
-                    ok &= part == 0;
-                }
-                if (ok && xa <= xb) {
-                    const int64_t lin = (int64_t)g.nc[0] * ((int64_t)cy + (int64_t)g.nc[1] * cz);
-                    cst = cell_start[lin + xa];
-                    ccnt = (int)(cell_start[lin + xb + 1] - cst);
-                }
-            }
-        }
+        long long cst;
+        int ccnt;
+        cell_segments<DIM>(g, cell_start, c, lane, cst, ccnt);
         const int incl = wave_incl_scan(ccnt);
         const int total = __builtin_amdgcn_readlane(incl, 63);
-        if (total > CAP) continue;  // filled from positions (spill_only pass)
+        if (!masked_cell(np, total, mask_cap)) continue;  // filled from positions
         const int excl = incl - ccnt;
         seg_cst[wv][lane] = cst;
         seg_ex[wv][lane] = excl;
-        int carry = 0;
-        for (int base = 0; base < total; base += kWave) {
-            mark[wv][lane] = -1;
-            wave_lds_sync();
-            if (ccnt > 0 && excl >= base && excl < base + kWave) mark[wv][excl - base] = lane;
-            wave_lds_sync();
-            const int seg = max(wave_incl_max(mark[wv][lane]), carry);
-            carry = __builtin_amdgcn_readlane(seg, 63);
-            const int k = base + lane;
-            if (k < total) sq[wv][k] = cell_part[seg_cst[wv][seg] + (k - seg_ex[wv][seg])];
-        }
-        int pid_l = 0;
-        int64_t rs_l = 0, re_l = 0;
+        stage_candidates<DIM, false>(0, total, ccnt, excl, mark[wv], seg_cst[wv], seg_ex[wv], cell_part, nullptr, 0,
+                                     sq, nullptr, 0, lane);
+        // row bounds of the cell's particles, one per lane in two rounds of 64
+        int64_t rs0 = 0, re0 = 0, rs1 = 0, re1 = 0;
         if (lane < np) {
-            pid_l = cell_part[pst + lane];
-            rs_l = row_ptr[pid_l];
-            re_l = row_ptr[pid_l + 1];
+            const int pid = cell_part[pst + lane];
+            rs0 = row_ptr[pid];
+            re0 = row_ptr[pid + 1];
+        }
+        if (lane + kWave < np) {
+            const int pid = cell_part[pst + kWave + lane];
+            rs1 = row_ptr[pid];
+            re1 = row_ptr[pid + 1];
         }
         const int nr = (total + kWave - 1) / kWave;
-        uint64_t mw = lane < nr ? masks[pst * NW + lane] : 0;
-        wave_lds_sync();
+        uint64_t mw = lane < nr ? masks[pst * nws + lane] : 0;
         for (int i = 0; i < np; ++i) {
-            int64_t rs, re;
-            if (i < kWave) {
-                rs = readlane_i64(rs_l, i);
-                re = readlane_i64(re_l, i);
-            } else {
-                const int pid = cell_part[pst + i];
-                rs = row_ptr[pid];
-                re = row_ptr[pid + 1];
-            }
+            const bool second = i >= kWave;
+            const int il = i & (kWave - 1);
+            const int64_t rs = readlane_i64(second ? rs1 : rs0, il);
+            const int64_t re = readlane_i64(second ? re1 : re0, il);
             const uint64_t cur = mw;
-            if (i + 1 < np) mw = lane < nr ? masks[(pst + i + 1) * NW + lane] : 0;
+            if (i + 1 < np) mw = lane < nr ? masks[(pst + i + 1) * nws + lane] : 0;
             int nacc = 0;
-            for (int r = 0; r < nr; ++r) {
-                const uint64_t m = (uint64_t)readlane_i64((int64_t)cur, r);
+            for (int rr = 0; rr < nr; ++rr) {
+                const uint64_t m = (uint64_t)readlane_i64((int64_t)cur, rr);
                 const bool take = (m >> lane) & 1u;
                 const int at = nacc + lane_prefix(m);
-                if (take && at < DCPSE_MAX_ROW) buf[wv][at] = sq[wv][r * kWave + lane];
+                if (take && at < DCPSE_MAX_ROW) buf[wv][at] = sq[rr * kWave + lane];
                 nacc += __popcll(m);
             }
             wave_lds_sync();
@@ -870,13 +926,18 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
     CellGrid g;
     dcpse_status st;
     // one particle set and one cut-off: a wavefront per cell of edge >= rcut
-    // (k_support_cells) while the mean candidate list of a cell fits its LDS
-    // slice with room to spare; otherwise a wavefront per particle
+    // while the mean candidate list of a cell is well inside what a mask
+    // covers; otherwise a wavefront per particle. Mask words per particle:
+    // about 1.3x the mean candidate count, at least one LDS copy's worth.
     bool by_cell = same && !rcut_pp && n_dom > 0;
+    int nws = 0;
     if (by_cell) {
         if ((st = make_grid(ctx, dim, box_lo, box_hi, periodic, rmax, 1, &g))) return st;
         const double cand = (double)n_sup * (dim == 3 ? 27.0 : 9.0) / (double)g.ncell;
-        by_cell = cand <= 0.75 * (dim == 3 ? CellCap<3>::v : CellCap<2>::v);
+        const int mcap = dim == 3 ? MaskCap<3>::v : MaskCap<2>::v;
+        const int ccap = dim == 3 ? CellCap<3>::v : CellCap<2>::v;
+        by_cell = cand <= 0.8 * mcap;
+        nws = std::min(mcap, std::max(ccap, (int)std::ceil(1.3 * cand) + kWave)) / kWave;
     }
     if (!by_cell && (st = make_grid(ctx, dim, box_lo, box_hi, periodic, rmax, K, &g))) return st;
     const int64_t cb = by_cell ? std::max<int64_t>(1, std::min<int64_t>((g.ncell + kCellWaves - 1) / kCellWaves,
@@ -889,10 +950,9 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
     dcpse_support_cache& sc = ctx->sup;
     const bool count_pass = col == nullptr;
     const bool cached = by_cell && !count_pass && sc.valid && sc.dim == dim && sc.pos == pos_sup &&
-                        sc.n == n_sup && sc.rcut == rmax && sc.periodic == periodic &&
+                        sc.n == n_sup && sc.rcut == rmax && sc.periodic == periodic && sc.nws == nws &&
                         std::equal(box_lo, box_lo + dim, sc.lo) && std::equal(box_hi, box_hi + dim, sc.hi);
     sc.valid = false;
-    const int mask_words = dim == 3 ? CellCap<3>::v / kWave : CellCap<2>::v / kWave;
 
     Workspace w;
     size_t need = (size_t)n_sup * 4 * 2 + (size_t)dim * (n_sup + 1) * 8 + (size_t)(g.ncell + 1) * 8 +
@@ -911,12 +971,14 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
     hipStream_t s = ctx->stream;
 
     uint64_t* masks = nullptr;
+    int32_t* spill = nullptr;
     if (by_cell && (count_pass || cached)) {
-        // cell list and masks in the context's support cache
+        // cell list, masks and the list of cells left to the position kernel
+        // in the context's support cache
         const size_t cap = (((size_t)(g.ncell + 1) * 8 + 255) & ~(size_t)255) +
                            (((size_t)(n_sup + 1) * 4 + 255) & ~(size_t)255) +
                            (((size_t)dim * (n_sup + 1) * 8 + 255) & ~(size_t)255) +
-                           (size_t)(n_sup + 1) * mask_words * 8;
+                           (((size_t)(n_sup + 1) * nws * 8 + 255) & ~(size_t)255) + (size_t)(g.ncell + 2) * 4;
         if (count_pass && sc.cap < cap) {
             DCPSE_HIP_CHECK(ctx, hipStreamSynchronize(s));
             if (sc.buf) (void)hipFree(sc.buf);
@@ -931,8 +993,9 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
         cell_start = cw.take<int64_t>(g.ncell + 1);
         cell_part = cw.take<int32_t>(n_sup + 1);
         pos_cell = cw.take<double>((size_t)dim * (n_sup + 1));
-        masks = cw.take<uint64_t>((size_t)(n_sup + 1) * mask_words);
-        if (!masks) return fail(ctx, DCPSE_ERR_INVALID, "support cache sizing error");
+        masks = cw.take<uint64_t>((size_t)(n_sup + 1) * nws);
+        spill = cw.take<int32_t>(g.ncell + 2);
+        if (!spill) return fail(ctx, DCPSE_ERR_INVALID, "support cache sizing error");
     }
 
     fill_i32(s, flags, 64, 0);
@@ -960,12 +1023,19 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
                                                               (int64_t)ctx->num_cus * 16));
     if (count_pass) {
         if (by_cell) {
-            if (dim == 2)
+            // masked cells, then the listed others from positions
+            fill_i32(s, spill, 1, 0);
+            if (dim == 2) {
+                k_support_cells_count<2><<<cell_blocks, kCellWaves * kWave, 0, s>>>(
+                    n_sup, rcut, g, cell_start, cell_part, pos_cell, counts, flags, masks, nws, spill);
                 k_support_cells<2, false><<<cell_blocks, kCellWaves * kWave, 0, s>>>(
-                    n_sup, rcut, g, cell_start, cell_part, pos_cell, counts, flags, nullptr, nullptr, masks, 0);
-            else
+                    n_sup, rcut, g, cell_start, cell_part, pos_cell, counts, flags, nullptr, nullptr, spill);
+            } else {
+                k_support_cells_count<3><<<cell_blocks, kCellWaves * kWave, 0, s>>>(
+                    n_sup, rcut, g, cell_start, cell_part, pos_cell, counts, flags, masks, nws, spill);
                 k_support_cells<3, false><<<cell_blocks, kCellWaves * kWave, 0, s>>>(
-                    n_sup, rcut, g, cell_start, cell_part, pos_cell, counts, flags, nullptr, nullptr, masks, 0);
+                    n_sup, rcut, g, cell_start, cell_part, pos_cell, counts, flags, nullptr, nullptr, spill);
+            }
             DCPSE_LAUNCH_CHECK(ctx);
         } else if (n_dom > 0) {
             if (dim == 2)
@@ -981,9 +1051,11 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
         Workspace ws2 = ws_scan;
         if ((st = scan_counts(ctx, counts, n_dom, row_ptr, &ws2))) return st;
         int64_t nnz = 0;
-        int32_t ovf = 0;
+        int32_t ovf = 0, n_spill = 0;
         DCPSE_HIP_CHECK(ctx, hipMemcpyAsync(&nnz, row_ptr + n_dom, sizeof(int64_t), hipMemcpyDeviceToHost, s));
         DCPSE_HIP_CHECK(ctx, hipMemcpyAsync(&ovf, flags, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        if (spill)
+            DCPSE_HIP_CHECK(ctx, hipMemcpyAsync(&n_spill, spill, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         DCPSE_HIP_CHECK(ctx, hipStreamSynchronize(s));
         if (ovf & 1)
             return fail(ctx, DCPSE_ERR_RANGE, "a support row exceeds DCPSE_MAX_ROW=%d neighbours",
@@ -993,12 +1065,13 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
         if (nnz_out) *nnz_out = nnz;
         if (by_cell) {
             sc.valid = true;
-            sc.spill = (ovf & 4) != 0;
+            sc.spill = n_spill > 0;
             sc.dim = dim;
             sc.pos = pos_sup;
             sc.n = n_sup;
             sc.rcut = rmax;
             sc.periodic = periodic;
+            sc.nws = nws;
             for (int d = 0; d < 3; ++d) {
                 sc.lo[d] = d < dim ? box_lo[d] : 0.0;
                 sc.hi[d] = d < dim ? box_hi[d] : 0.0;
@@ -1006,24 +1079,24 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
         }
     } else {
         if (cached) {
+            const size_t lds = (size_t)kCellWaves * nws * kWave * sizeof(int32_t);
             if (dim == 2)
-                k_support_cells_masked<2><<<cell_blocks, kCellWaves * kWave, 0, s>>>(g, cell_start, cell_part, masks,
-                                                                                      flags, row_ptr, col);
+                k_support_cells_masked<2><<<cell_blocks, kCellWaves * kWave, lds, s>>>(g, cell_start, cell_part, masks,
+                                                                                        nws, flags, row_ptr, col);
             else
-                k_support_cells_masked<3><<<cell_blocks, kCellWaves * kWave, 0, s>>>(g, cell_start, cell_part, masks,
-                                                                                      flags, row_ptr, col);
+                k_support_cells_masked<3><<<cell_blocks, kCellWaves * kWave, lds, s>>>(g, cell_start, cell_part, masks,
+                                                                                        nws, flags, row_ptr, col);
             DCPSE_LAUNCH_CHECK(ctx);
         }
         if (by_cell && (!cached || sc.spill)) {
-            const int spill_only = cached ? 1 : 0;
+            // the cells the masks do not cover (cached), or all of them
+            const int32_t* list = cached ? spill : nullptr;
             if (dim == 2)
                 k_support_cells<2, true><<<cell_blocks, kCellWaves * kWave, 0, s>>>(
-                    n_sup, rcut, g, cell_start, cell_part, pos_cell, counts, flags, row_ptr, col, nullptr,
-                    spill_only);
+                    n_sup, rcut, g, cell_start, cell_part, pos_cell, counts, flags, row_ptr, col, list);
             else
                 k_support_cells<3, true><<<cell_blocks, kCellWaves * kWave, 0, s>>>(
-                    n_sup, rcut, g, cell_start, cell_part, pos_cell, counts, flags, row_ptr, col, nullptr,
-                    spill_only);
+                    n_sup, rcut, g, cell_start, cell_part, pos_cell, counts, flags, row_ptr, col, list);
             DCPSE_LAUNCH_CHECK(ctx);
         } else if (!by_cell && n_dom > 0) {
             if (dim == 2)

@@ -113,6 +113,23 @@ def test_support_clustered_cells_match_oracle(dim, oracle, cuda):
     assert np.array_equal(vl.col.cpu().numpy(), col)
 
 
+@pytest.mark.parametrize("dim,rcut_h", [(3, 3.9), (2, 6.0)])
+def test_support_large_cutoff_matches_oracle(dim, rcut_h, oracle, cuda):
+    """Cut-offs whose cells hold more candidates than one LDS copy (the count
+    walks them in chunks, particles of a cell in two rounds of 64) and rows
+    longer than 128 (sorted by the separate long-row pass)."""
+    from openfpm_numerics_amd import workloads as W
+
+    w = W.jittered_3d_holes(24, seed=5, rcut_h=rcut_h) if dim == 3 else W.jittered_2d(64, seed=5, rcut_h=rcut_h)
+    ps = _ps(w, cuda)
+    vl = ps.getVerlet(w["rcut"])
+    rp, col = oracle.support(w["pos"], w["rcut"], w["box_lo"], w["box_hi"], w["periodic"])
+    print(f"[support] dim={dim} rcut={rcut_h}h N={len(w['pos'])} nnz={col.size} max row={np.diff(rp).max()}",
+          flush=True)
+    assert np.array_equal(vl.row_ptr.cpu().numpy(), rp)
+    assert np.array_equal(vl.col.cpu().numpy(), col)
+
+
 def test_support_fill_after_other_count_matches_oracle(oracle, cuda):
     """The fill reuses the cell list and take masks of the count just before it
     only when the arguments match: a count of another cut-off in between, and a
