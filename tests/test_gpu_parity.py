@@ -130,6 +130,31 @@ def test_support_large_cutoff_matches_oracle(dim, rcut_h, oracle, cuda):
     assert np.array_equal(vl.col.cpu().numpy(), col)
 
 
+def test_support_open_box_and_tiny_sets_match_oracle(oracle, cuda):
+    """Open (non-periodic) boxes with particles outside them (they fall in
+    the edge cells), and sets of one and two particles."""
+    from openfpm_numerics_amd import ParticleSet
+
+    rng = np.random.default_rng(13)
+    for dim, n, rcut in ((3, 6000, 0.12), (2, 3000, 0.05)):
+        pos = rng.random((n, dim)) * 1.1 - 0.05  # about 5% of each side outside [0, 1)
+        lo, hi = np.zeros(dim), np.ones(dim)
+        ps = ParticleSet(pos, lo, hi, bc=[0] * dim, device=cuda)
+        vl = ps.getVerlet(rcut)
+        rp, col = oracle.support(pos, rcut, lo, hi, 0)
+        assert np.array_equal(vl.row_ptr.cpu().numpy(), rp), dim
+        assert np.array_equal(vl.col.cpu().numpy(), col), dim
+    for n in (1, 2):
+        for periodic in (0b11, 0b00):
+            pos = np.array([[0.1, 0.1], [0.15, 0.12]])[:n]
+            bc = [(periodic >> d) & 1 for d in range(2)]
+            ps = ParticleSet(pos, [0.0, 0.0], [1.0, 1.0], bc=bc, device=cuda)
+            vl = ps.getVerlet(0.2)
+            rp, col = oracle.support(pos, 0.2, [0.0, 0.0], [1.0, 1.0], periodic)
+            assert np.array_equal(vl.row_ptr.cpu().numpy(), rp), (n, periodic)
+            assert np.array_equal(vl.col.cpu().numpy(), col), (n, periodic)
+
+
 def test_support_fill_after_other_count_matches_oracle(oracle, cuda):
     """The fill reuses the cell list and take masks of the count just before it
     only when the arguments match: a count of another cut-off in between, and a
