@@ -48,6 +48,9 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
     typedef typename vector_type::stype T;
     typedef typename vector_type::prop_type prop_type;
     typedef vector_type vtype;
+    // FP32 applies run the CSR kernel until this many applies since assembly,
+    // the operator's 16-bit sliced-ELL copy from then on
+    static constexpr int kSellAfter = 16;
 
     T HOverEpsilon = 0.9;
     // one step of FP64 iterative refinement of each particle's solve
@@ -91,7 +94,10 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
         const int64_t n = (int64_t)verletList_.size();
         if constexpr (std::is_same<prop_type, float>::value) {
             float* O[1] = {out};
-            if (n > 0 && !b16_tried_) build_bundle16();
+            // the sliced-ELL copy from the kSellAfter-th apply since assembly
+            // on: building it costs what some 50 applies save, so operators
+            // reassembled every few applies (moving particles) never build it
+            if (n > 0 && !b16_tried_ && ++n_apply32_ >= kSellAfter) build_bundle16();
             if (b16_) {
                 c.check(dcpse_apply_sell_bundle16_f32(c.get(), n, verletList_.devRowPtr(), slice_ptr_.data(),
                                                       row16_.data(), 1, bundle_.data(), exc_ptr_.data(),
@@ -219,6 +225,7 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
         W64_.upload(W64);
         W32_.upload(W32);
         b16_ = b16_tried_ = false;
+        n_apply32_ = 0;
         mirrored_ = true;
         ctr_ = particles_.getMapCtr();
     }
@@ -301,6 +308,7 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
                                                  " (info " + std::to_string(info[p]) + ")");
         mirrored_ = false;
         b16_ = b16_tried_ = false;
+        n_apply32_ = 0;
         ctr_ = particles_.getMapCtr();
     }
     void mirror() {
@@ -335,6 +343,7 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
     openfpm_dcpse::device_array<float> exc_W_;
     int64_t n_exc_ = 0;
     bool b16_ = false, b16_tried_ = false;
+    int n_apply32_ = 0;  // FP32 applies since assembly
     std::vector<double> eps_host_, eip_host_, K_host_;
     std::vector<int64_t> rp_host_;
     std::vector<int32_t> col_host_;

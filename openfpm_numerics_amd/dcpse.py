@@ -39,6 +39,11 @@ _FILE_MAGIC = b"DCPSEHIP"
 #: particle set: the 16-bit-offset bundle; otherwise 32-bit columns); set False
 #: to use the CSR kernels
 USE_SELL = True
+#: ... from an operator's SELL_AFTER-th such apply after its assembly on; the
+#: ones before run the CSR kernel. Building a sliced-ELL copy (about 2.4 ms for
+#: 1.9M particles and 105M neighbours) costs what some 50 applies save, so an
+#: operator reassembled every few applies (moving particles) never builds one.
+SELL_AFTER = 16
 _FILE_VERSION = 1
 
 
@@ -96,6 +101,7 @@ class Dcpse:
         self.info = None
         self._W = {}
         self._T = {}
+        self._n32 = {"fwd": 0, "adj": 0}  # FP32 single-field applies since assembly
         self.update_ctr = particles.map_ctr
         if opt != LOAD:
             self.initializeStaticSize(self.particlesSupport, self.particlesDomain)
@@ -128,6 +134,7 @@ class Dcpse:
         self.calcKernels = K64[:nnz]
         self._W = {torch.float64: W64[:nnz], torch.float32: W32[:nnz]}
         self._T = {}
+        self._n32 = {"fwd": 0, "adj": 0}
         self.info = info
         self.update_ctr = map_ctr
 
@@ -257,6 +264,12 @@ class Dcpse:
             self._T["sell"] = (*layout, sell_weights(rowT, WT, layout, n))
         return self._T["sell"]
 
+    def _count_apply32(self, kind: str) -> bool:
+        """Counts an FP32 single-field apply (kind "fwd" or "adj"); True once the
+        sliced-ELL copy is due (SELL_AFTER)."""
+        self._n32[kind] += 1
+        return self._n32[kind] >= SELL_AFTER
+
     def applyTranspose(self, g: torch.Tensor, out: torch.Tensor | None = None, alpha=1.0, beta=0.0):
         """A^T g for a field (N,) or a batch (B, N)."""
         if g.dtype not in (torch.float32, torch.float64):
@@ -268,7 +281,7 @@ class Dcpse:
             raise ValueError("field length differs from the support set")
         o = torch.empty_like(g2) if out is None else out.reshape(nb, n)
         ctx = _lib.context(g.device.index or 0)
-        if g.dtype == torch.float32 and nb == 1 and USE_SELL:
+        if g.dtype == torch.float32 and nb == 1 and USE_SELL and self._count_apply32("adj"):
             slice_ptr, sell_rowT, sell_colT, sell_WT = self.transposed_sell()
             ctx.call("dcpse_apply_transpose_sell_f32", n, _lib.ptr(rowT), _lib.ptr(slice_ptr), _lib.ptr(sell_rowT),
                      _lib.ptr(sell_colT), _lib.ptr(sell_WT), _lib.ptr(diag), _lib.ptr(g2), _lib.ptr(o), alpha, beta)
@@ -335,6 +348,7 @@ class Dcpse:
         W64 = self.calcKernels * self.localEpsInvPow[rows]
         self._W = {torch.float64: W64, torch.float32: W64.to(torch.float32)}
         self._T = {}
+        self._n32 = {"fwd": 0, "adj": 0}
         self.info = torch.zeros(n, dtype=torch.int32, device=dev)
         self.update_ctr = self.particlesSupport.map_ctr
 
@@ -454,7 +468,10 @@ def apply_ops(ops, f: torch.Tensor, outs=None, f_domain: torch.Tensor | None = N
     else:
         outs = [o.reshape(nb, n_rows) for o in outs]
     ctx = _lib.context(f.device.index or 0)
-    if f.dtype == torch.float32 and nb == 1 and USE_SELL and len(ops) == 1 and ops[0].same_set:
+    use_sell = False
+    if f.dtype == torch.float32 and nb == 1 and USE_SELL:
+        use_sell = all([op._count_apply32("fwd") for op in ops])
+    if use_sell and len(ops) == 1 and ops[0].same_set:
         # one operator: its 16-bit-offset bundle, built once per assembly
         slice_ptr = vl.sell()[0]
         bundle, row16, exc_ptr, exc_col, exc_W, ne = ops[0].sell_bundle16()
@@ -462,7 +479,7 @@ def apply_ops(ops, f: torch.Tensor, outs=None, f_domain: torch.Tensor | None = N
                  1, _lib.ptr(bundle), _lib.ptr(exc_ptr), _lib.ptr(exc_col), _lib.ptr(exc_W), ne, _lib.ptr(f2),
                  _lib.ptr(fd), _lib.ptr_array(outs), alpha, beta)
         return [o.reshape(*f.shape[:-1], n_rows) for o in outs]
-    if f.dtype == torch.float32 and nb == 1 and USE_SELL:
+    if use_sell:
         slice_ptr, sell_row, sell_col = vl.sell()
         SW = [op.sell_weights() for op in ops]
         ctx.call("dcpse_apply_sell_f32", n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(slice_ptr), _lib.ptr(sell_row),

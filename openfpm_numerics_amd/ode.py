@@ -27,10 +27,12 @@ def rk4(op, f: torch.Tensor, dt: float, n_steps: int, alpha: float = 1.0, layout
         work: torch.Tensor | None = None) -> torch.Tensor:
     """Advances f (N,) in place by n_steps RK4 steps of df/dt = alpha * D f and returns it.
 
-    float32 state runs the FP32 kernels: layout "auto" or "bundle16" (sliced-ELL
-    with 16-bit column offsets, the fewest bytes), "sell" (32-bit columns) or
-    "csr"; float64 state the FP64 CSR kernels. ``work`` (3N of f's dtype) may
-    be passed to avoid an allocation per call.
+    float32 state runs the FP32 kernels: layout "bundle16" (sliced-ELL with
+    16-bit column offsets, the fewest bytes), "sell" (32-bit columns), "csr",
+    or "auto": the 4 n_steps stage applies count towards the operator's
+    ``dcpse.SELL_AFTER`` and run on "bundle16" once it is reached (its copy is
+    then built), on "csr" before. float64 state runs the FP64 CSR kernels.
+    ``work`` (3N of f's dtype) may be passed to avoid an allocation per call.
     """
     d = _dcpse(op)
     vl = d.verletList
@@ -45,10 +47,13 @@ def rk4(op, f: torch.Tensor, dt: float, n_steps: int, alpha: float = 1.0, layout
         raise ValueError("work must be a contiguous tensor of 3N elements of f's dtype")
     ctx = _lib.context(f.device.index or 0)
     ctx.bind_current_stream()
+    if f.dtype == torch.float32 and layout == "auto":
+        d._n32["fwd"] += 4 * int(n_steps) - 1
+        layout = "bundle16" if d._count_apply32("fwd") else "csr"
     if f.dtype == torch.float64:
         ctx.call("dcpse_rk4_f64", n, _lib.ptr(vl.row_ptr), _lib.ptr(vl.col), _lib.ptr(d.weights(torch.float64)),
                  float(alpha), float(dt), int(n_steps), _lib.ptr(f), _lib.ptr(work))
-    elif layout in ("auto", "bundle16"):
+    elif layout == "bundle16":
         slice_ptr = vl.sell()[0]
         bundle, row16, exc_ptr, exc_col, exc_W, ne = d.sell_bundle16()
         ctx.call("dcpse_rk4_bundle16_f32", n, _lib.ptr(vl.row_ptr), _lib.ptr(slice_ptr), _lib.ptr(row16),
@@ -69,17 +74,21 @@ def rk4(op, f: torch.Tensor, dt: float, n_steps: int, alpha: float = 1.0, layout
 
 def stage(op, u: torch.Tensor, alpha: float, y_a: torch.Tensor | None, ca: float, out_a: torch.Tensor | None,
           y_b: torch.Tensor | None, cb: float, out_b: torch.Tensor | None, layout: str = "auto"):
-    """One fused stage: k = alpha * D u; out_a = y_a + ca k; out_b = y_b + cb k (outputs may be None)."""
+    """One fused stage: k = alpha * D u; out_a = y_a + ca k; out_b = y_b + cb k (outputs may be None).
+
+    Layouts as for rk4; "auto" counts one apply towards ``dcpse.SELL_AFTER``."""
     d = _dcpse(op)
     vl = d.verletList
     ctx = _lib.context(u.device.index or 0)
     ctx.bind_current_stream()
     p = _lib.ptr
+    if u.dtype == torch.float32 and layout == "auto":
+        layout = "bundle16" if d._count_apply32("fwd") else "csr"
     if u.dtype == torch.float64:
         ctx.call("dcpse_stage_f64", vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(vl.col),
                  _lib.ptr(d.weights(torch.float64)), _lib.ptr(u), float(alpha), p(y_a), float(ca), p(out_a), p(y_b),
                  float(cb), p(out_b))
-    elif layout in ("auto", "bundle16"):
+    elif layout == "bundle16":
         slice_ptr = vl.sell()[0]
         bundle, row16, exc_ptr, exc_col, exc_W, ne = d.sell_bundle16()
         ctx.call("dcpse_stage_bundle16_f32", vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(slice_ptr), _lib.ptr(row16),

@@ -248,16 +248,21 @@ static bool dcpse_fp32_fields() {
     Laplacian<decltype(vl32)> L32(d32, vl32, 2, rCut);
     Derivative_x<decltype(vl32)> X32(d32, vl32, 2, rCut);
     getV<1>(d64) = L64(getV<0>(d64)) + 0.5 * X64(getV<0>(d64));
-    getV<1>(d32) = L32(getV<0>(d32)) + 0.5 * X32(getV<0>(d32));
-    double num = 0, den = 0;
-    for (size_t p = 0; p < d64.size_local(); ++p) {
-        const double a = d64.getProp<1>(p), b = d32.getProp<1>(p);
-        num += (a - b) * (a - b);
-        den += a * a;
+    // 20 evaluations: the FP32 operators switch from the CSR kernel to their
+    // sliced-ELL copies after 16 applies
+    double worst = 0;
+    for (int it = 0; it < 20; ++it) {
+        getV<1>(d32) = L32(getV<0>(d32)) + 0.5 * X32(getV<0>(d32));
+        double num = 0, den = 0;
+        for (size_t p = 0; p < d64.size_local(); ++p) {
+            const double a = d64.getProp<1>(p), b = d32.getProp<1>(p);
+            num += (a - b) * (a - b);
+            den += a * a;
+        }
+        worst = std::fmax(worst, std::sqrt(num / den));
     }
-    const double rel = std::sqrt(num / den);
-    std::printf("         FP32 vs FP64 rel L2 %.3e\n", rel);
-    return rel < 1e-5;
+    std::printf("         FP32 vs FP64 rel L2 %.3e (worst of 20 evaluations)\n", worst);
+    return worst < 1e-5;
 }
 
 // An assignment whose target is also an operand reads the unmodified field;

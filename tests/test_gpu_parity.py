@@ -378,6 +378,8 @@ def test_sell_layout_apply(name, oracle, cuda):
     want_dx = oracle.apply(ref_dx, rp, col, f64)
     ctx = _lib.context(0)
     ft = torch.as_tensor(f32, device=cuda)
+    sell_after = D.SELL_AFTER
+    D.SELL_AFTER = 1  # the sliced-ELL copies from the first apply on
     try:
         for gd in ((2, 2), (4, 2), (1, 3), (2, 3)):
             print(f"[sell] {name} groups,depth={gd}", flush=True)
@@ -400,8 +402,34 @@ def test_sell_layout_apply(name, oracle, cuda):
             want_t = A.T @ f64
             got_t = lap.applyTranspose(ft).cpu().numpy().astype(np.float64)
             assert np.linalg.norm(got_t - want_t) / np.linalg.norm(want_t) <= 1e-5
+            assert dx.dcpse._W.get("sell") is not None and lap.dcpse._T.get("sell") is not None
     finally:
+        D.SELL_AFTER = sell_after
         ctx.check(ctx.lib.dcpse_set_sell_pipeline(ctx.h, 2, 2), "pipeline reset")
+
+
+def test_sell_copy_built_from_the_nth_apply(cuda):
+    """FP32 single-field applies run the CSR kernel until SELL_AFTER applies
+    since assembly, then the 16-bit sliced-ELL copy (built then); a
+    reassembly starts the count again. Both give the same result to FP32
+    rounding of the row sums."""
+    from openfpm_numerics_amd import Laplacian, dcpse as D, workloads as W
+
+    w = W.jittered_2d(96, seed=7)
+    ps = _ps(w, cuda)
+    vl = ps.getVerlet(w["rcut"])
+    lap = Laplacian(ps, vl, 2, w["rcut"])
+    f = torch.randn(ps.size_local(), generator=torch.Generator().manual_seed(2)).to(cuda)
+    outs = []
+    for i in range(D.SELL_AFTER + 2):
+        outs.append(lap(f))
+        assert ("bundle16" in lap.dcpse._W) == (i + 1 >= D.SELL_AFTER), i
+    ref = outs[0].double()
+    for o in outs[1:]:
+        assert float((o.double() - ref).abs().max()) <= 1e-6 * float(ref.abs().max())
+    lap.dcpse.initializeStaticSize(ps, ps, check=False)
+    lap(f)
+    assert "bundle16" not in lap.dcpse._W
 
 
 @pytest.mark.parametrize("name", sorted(WORKLOADS))
