@@ -60,9 +60,8 @@ constexpr int kChunk = 32;  // neighbours staged per LDS round
 template <int DIM, int L, int AMIN>
 __constant__ BasisTables<DIM, L, AMIN> kTab = BasisTables<DIM, L, AMIN>();
 
-// The (y, z) exponent pairs (b, c) with b + c <= P (c = 0 in 2D), b fastest:
-// in the moment pass a lane owns one pair and accumulates M(a, b, c) for
-// every a <= P - b - c.
+// The (y, z) exponent pairs (b, c) with b + c <= P (c = 0 in 2D): the moment
+// pass accumulates M(a, b, c) for every a <= P - b - c of each.
 template <int DIM, int P>
 struct PairTable {
     static constexpr int N = DIM == 3 ? (P + 1) * (P + 2) / 2 : P + 1;
@@ -78,8 +77,75 @@ struct PairTable {
     }
 };
 
+// Lanes per neighbour in the moment pass: the smallest power of two covering
+// the pairs.
 template <int DIM, int P>
-__constant__ PairTable<DIM, P> kPair = PairTable<DIM, P>();
+constexpr int moment_lpn() {
+    constexpr int n = PairTable<DIM, P>::N;
+    return n <= 8 ? 8 : (n <= 16 ? 16 : (n <= 32 ? 32 : 64));
+}
+
+// The moment pass's lane segments: pair (b, c) needs the x powers
+// a = 0 .. P - b - c; chains longer than T are split in two, [0, T) and
+// [T, P - b - c], so that no lane runs more than T steps. T is the smallest
+// length for which the segments still fit the lanes of a neighbour (and at
+// most two segments per chain). A second segment starts from x^T y^b, staged
+// in the power table after the y powers (b <= P - T), so it costs no extra
+// arithmetic in the loop. Per segment: the y-factor and wz-factor slots of
+// the table row, its exponents and a0, len.
+template <int DIM, int P>
+struct SegTable {
+    static constexpr int LPN = moment_lpn<DIM, P>();
+    static constexpr int find_t() {
+        int t = P + 1;
+        for (int tt = P + 1; tt >= 1; --tt) {
+            if (2 * tt < P + 1) break;
+            int n = 0;
+            for (int c = 0; c <= (DIM == 3 ? P : 0); ++c)
+                for (int b = 0; b + c <= P; ++b) n += (P - b - c + 1 > tt) ? 2 : 1;
+            if (n <= LPN) t = tt;
+        }
+        return t;
+    }
+    static constexpr int T = find_t();
+    static constexpr int NX = T < P + 1 ? P - T + 1 : 0;  // x^T y^b slots, b = 0 .. P - T
+    static constexpr int count() {
+        int n = 0;
+        for (int c = 0; c <= (DIM == 3 ? P : 0); ++c)
+            for (int b = 0; b + c <= P; ++b) n += (P - b - c + 1 > T) ? 2 : 1;
+        return n;
+    }
+    static constexpr int N = count();
+    int yslot[N], zslot[N], b[N], c[N], a0[N], len[N];
+    constexpr SegTable() : yslot{}, zslot{}, b{}, c{}, a0{}, len{} {
+        int k = 0;
+        for (int cc = 0; cc <= (DIM == 3 ? P : 0); ++cc)
+            for (int bb = 0; bb + cc <= P; ++bb) {
+                const int full = P - bb - cc + 1;
+                // row slots: 0 = x, 1 + e = y^e, 1 + (P + 1) + e = x^T y^e,
+                // 1 + (P + 1) + NX + e = w z^e
+                yslot[k] = 1 + bb;
+                zslot[k] = 1 + (P + 1) + NX + cc;
+                b[k] = bb;
+                c[k] = cc;
+                a0[k] = 0;
+                len[k] = full > T ? T : full;
+                ++k;
+                if (full > T) {
+                    yslot[k] = 1 + (P + 1) + bb;
+                    zslot[k] = 1 + (P + 1) + NX + cc;
+                    b[k] = bb;
+                    c[k] = cc;
+                    a0[k] = T;
+                    len[k] = full - T;
+                    ++k;
+                }
+            }
+    }
+};
+
+template <int DIM, int P>
+__constant__ SegTable<DIM, P> kSeg = SegTable<DIM, P>();
 
 __device__ __forceinline__ double ipow_rt(double x, unsigned e) {
     double r = 1.0;
@@ -359,18 +425,21 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
     constexpr int LS = kWave / SLOTS;
     constexpr int MSX = MX + (MX & 1);
     constexpr int PW = P + 1;
-    // power-table row per staged neighbour: [x][y^0..y^P][w z^0..w z^P]
-    // (2D: [x][y^0..y^P][w])
+    using Seg = SegTable<DIM, P>;
+    constexpr int ST = Seg::T;    // x powers per lane segment
+    constexpr int NX = Seg::NX;   // x^T y^b slots
+    // power-table row per staged neighbour: [x][y^0..y^P][x^T y^0..x^T y^(P-T)]
+    // [w z^0..w z^P] (2D: [x][y^0..y^P][x^T y^b..][w])
     constexpr int ZW = DIM == 3 ? PW : 1;
-    constexpr int TROW = 1 + PW + ZW;
+    constexpr int TROW = 1 + PW + NX + ZW;
     constexpr int TAB = (kChunk * TROW > SLOTS * MX * MSX) ? kChunk * TROW : SLOTS * MX * MSX;
-    constexpr int NPAIR = PairTable<DIM, P>::N;
-    // lanes per neighbour: the smallest power of two covering the pairs; a
-    // pass covers HALVES = 64 / LPN neighbours
-    constexpr int LPN = NPAIR <= 8 ? 8 : (NPAIR <= 16 ? 16 : (NPAIR <= 32 ? 32 : 64));
+    // lanes per neighbour (the segments of the moment pass fit them); a pass
+    // covers HALVES = 64 / LPN neighbours
+    constexpr int NSEG = Seg::N;
+    constexpr int LPN = Seg::LPN;
     constexpr int HALVES = kWave / LPN;
     static_assert(kChunk % (2 * HALVES) == 0, "a chunk holds whole passes");
-    static_assert(NPAIR <= kWave, "moment pairs exceed a wavefront");
+    static_assert(NSEG <= LPN, "moment segments exceed the lanes of a neighbour");
     __shared__ __align__(16) double tab[TAB];     // staged powers; L factors in the solve
     __shared__ double mom[SLOTS * MGP];           // moments of each slot's particle
     __shared__ double2 lbuf[2 * kWave];           // broadcast buffers of the factorisation
@@ -379,12 +448,14 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
     __shared__ double wsl[SLOTS * kWave];         // and their weights exp(-|z|^2)
     const int lane = threadIdx.x;
 
-    // the (b, c) pair this lane accumulates, and which neighbour of a pass
+    // the moment segment this lane accumulates, and which neighbour of a pass
     const int half = lane / LPN;
     const int pl = lane % LPN;
-    const bool pv = pl < NPAIR;
-    const int pb = kPair<DIM, P>.bc[pv ? pl : 0][0];
-    const int pc = kPair<DIM, P>.bc[pv ? pl : 0][1];
+    const bool pv = pl < NSEG;
+    const int sgi = pv ? pl : 0;
+    const int pys = kSeg<DIM, P>.yslot[sgi], pzs = kSeg<DIM, P>.zslot[sgi];
+    const int pb = kSeg<DIM, P>.b[sgi], pc = kSeg<DIM, P>.c[sgi];
+    const int pa0 = kSeg<DIM, P>.a0[sgi], plen = kSeg<DIM, P>.len[sgi];
 
     // lane slot * LS + i: row i of each operator's system (right-hand side
     // entry and offset of A row i in its slot's moment grid)
@@ -509,9 +580,9 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
                 // from LDS, forming the x powers in registers; a pass covers HALVES
                 // neighbours (one per group of LPN lanes), so each LDS read serves
                 // the whole moment column of its pair.
-                double acc[PW];
+                double acc[ST];
 #pragma unroll
-                for (int k = 0; k < PW; ++k) acc[k] = 0.0;
+                for (int k = 0; k < ST; ++k) acc[k] = 0.0;
                 for (int c0 = 0; c0 < n; c0 += kChunk) {
                     const int cnt = min(kChunk, n - c0);
                     const int jl = lane & (kChunk - 1);
@@ -535,6 +606,16 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
                             row[2 * (1 + k)] = pw;
                             pw *= z[1];
                         }
+                        if (NX > 0) {
+                            double xt = 1.0;
+#pragma unroll
+                            for (int k = 0; k < ST; ++k) xt *= z[0];
+#pragma unroll
+                            for (int k = 0; k < NX; ++k) {
+                                row[2 * (1 + PW + k)] = xt;
+                                xt *= z[1];
+                            }
+                        }
                     } else {
                         double z2 = 0.0;
 #pragma unroll
@@ -543,7 +624,7 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
                         if (c0 + jl < kWave) wsl[s * kWave + c0 + jl] = pw;
 #pragma unroll
                         for (int k = 0; k < ZW; ++k) {
-                            row[2 * (1 + PW + k)] = pw;
+                            row[2 * (1 + PW + NX + k)] = pw;
                             pw *= z[DIM - 1];
                         }
                     }
@@ -553,10 +634,10 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
                     for (int j = 0; j < cnt; j += 2 * HALVES) {
                         const double2* rw = reinterpret_cast<const double2*>(tab) + ((j >> 1) + half) * TROW;
                         const double2 x = rw[0];
-                        const double2 yb = rw[1 + pb], wc = rw[1 + PW + pc];
+                        const double2 yb = rw[pys], wc = rw[pzs];
                         double t0 = yb.x * wc.x, t1 = yb.y * wc.y;
 #pragma unroll
-                        for (int k = 0; k < PW; ++k) {
+                        for (int k = 0; k < ST; ++k) {
                             acc[k] += t0 + t1;
                             t0 *= x.x;
                             t1 *= x.y;
@@ -567,14 +648,14 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
 #pragma unroll
                 for (int off = kWave / 2; off >= LPN; off /= 2) {
 #pragma unroll
-                    for (int k = 0; k < PW; ++k) acc[k] += __shfl_down(acc[k], off, kWave);
+                    for (int k = 0; k < ST; ++k) acc[k] += __shfl_down(acc[k], off, kWave);
                 }
                 if (pv && half == 0) {
                     double* ms = mom + s * MGP;
 #pragma unroll
-                    for (int k = 0; k < PW; ++k) {
-                        const int sm = k + pb + pc;
-                        if (sm >= QMIN && sm <= P) ms[k + GW * pb + GW * GW * pc] = acc[k];
+                    for (int k = 0; k < ST; ++k) {
+                        const int a = pa0 + k;
+                        if (k < plen && a + pb + pc >= QMIN) ms[a + GW * pb + GW * GW * pc] = acc[k];
                     }
                 }
                 wave_lds_sync();
