@@ -234,8 +234,8 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
 
     // FP32 applies run on a sliced-ELL copy of the weights with 16-bit column
     // offsets (dcpse_sell_count + dcpse_sell_bundle16_count/fill), built at
-    // the first apply after each assembly or load; where it cannot be built
-    // the CSR kernels run instead.
+    // the kSellAfter-th apply after each assembly or load; where it cannot be
+    // built the CSR kernels run instead.
     void build_bundle16() {
         auto& c = openfpm_dcpse::context();
         b16_tried_ = true;
