@@ -473,7 +473,7 @@ def apply_ops(ops, f: torch.Tensor, outs=None, f_domain: torch.Tensor | None = N
         use_sell = all([op._count_apply32("fwd") for op in ops])
     if use_sell and len(ops) == 1 and ops[0].same_set:
         # one operator: its 16-bit-offset bundle, built once per assembly
-        slice_ptr = vl.sell()[0]
+        slice_ptr = vl.sell_slices()[0]
         bundle, row16, exc_ptr, exc_col, exc_W, ne = ops[0].sell_bundle16()
         ctx.call("dcpse_apply_sell_bundle16_f32", n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(slice_ptr), _lib.ptr(row16),
                  1, _lib.ptr(bundle), _lib.ptr(exc_ptr), _lib.ptr(exc_col), _lib.ptr(exc_W), ne, _lib.ptr(f2),
@@ -500,8 +500,7 @@ def sell_bundle(ops) -> torch.Tensor:
     """The interleaved sliced-ELL copy of several FP32 operators sharing one
     support (columns and every operator's weights per slot group)."""
     vl = ops[0].verletList
-    slice_ptr, sell_row, sell_col = vl.sell()
-    n_slots = sell_col.numel()
+    slice_ptr, sell_row, n_slots = vl.sell_slices()
     Ws = [op.weights(torch.float32) for op in ops]
     bundle = torch.empty((1 + len(ops)) * n_slots, dtype=torch.int32, device=vl.row_ptr.device)
     _lib.context(vl.row_ptr.device.index or 0).call(
@@ -517,8 +516,7 @@ def sell_bundle16(ops):
     vl = ops[0].verletList
     if not all(op.same_set for op in ops):
         raise ValueError("16-bit column offsets need operators whose support and domain sets are the same")
-    slice_ptr, sell_row, sell_col = vl.sell()
-    n_slots = sell_col.numel()
+    slice_ptr, sell_row, n_slots = vl.sell_slices()
     dev = vl.row_ptr.device
     ctx = _lib.context(dev.index or 0)
     n_lanes = sell_row.numel()
@@ -556,7 +554,7 @@ class PreparedApply:
         self.ctx = _lib.context(f.device.index or 0)
         self.O = _lib.ptr_array(outs)
         if sell == "bundle16" and f.dtype == torch.float32 and nb == 1:
-            slice_ptr, sell_row, sell_col = vl.sell()
+            slice_ptr = vl.sell_slices()[0]
             b16 = sell_bundle16(ops)
             self._keep = (ops, f, outs, b16, slice_ptr)
             bundle, sell_row16, exc_ptr, exc_col, exc_W, ne = b16
@@ -566,7 +564,7 @@ class PreparedApply:
                          len(ops), _lib.ptr(bundle), _lib.ptr(exc_ptr), _lib.ptr(exc_col), _lib.ptr(exc_W), ne,
                          _lib.ptr(f2), None, self.O, alpha, beta)
         elif sell == "bundle" and f.dtype == torch.float32 and nb == 1:
-            slice_ptr, sell_row, sell_col = vl.sell()
+            slice_ptr, sell_row, _ = vl.sell_slices()
             bundle = sell_bundle(ops)
             self._keep = (ops, f, outs, bundle, slice_ptr, sell_row)
             self.fn = self.ctx.lib.dcpse_apply_sell_bundle_f32

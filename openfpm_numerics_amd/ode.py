@@ -54,7 +54,7 @@ def rk4(op, f: torch.Tensor, dt: float, n_steps: int, alpha: float = 1.0, layout
         ctx.call("dcpse_rk4_f64", n, _lib.ptr(vl.row_ptr), _lib.ptr(vl.col), _lib.ptr(d.weights(torch.float64)),
                  float(alpha), float(dt), int(n_steps), _lib.ptr(f), _lib.ptr(work))
     elif layout == "bundle16":
-        slice_ptr = vl.sell()[0]
+        slice_ptr = vl.sell_slices()[0]
         bundle, row16, exc_ptr, exc_col, exc_W, ne = d.sell_bundle16()
         ctx.call("dcpse_rk4_bundle16_f32", n, _lib.ptr(vl.row_ptr), _lib.ptr(slice_ptr), _lib.ptr(row16),
                  _lib.ptr(bundle), _lib.ptr(exc_ptr), _lib.ptr(exc_col), _lib.ptr(exc_W), ne, float(alpha),
@@ -89,7 +89,7 @@ def stage(op, u: torch.Tensor, alpha: float, y_a: torch.Tensor | None, ca: float
                  _lib.ptr(d.weights(torch.float64)), _lib.ptr(u), float(alpha), p(y_a), float(ca), p(out_a), p(y_b),
                  float(cb), p(out_b))
     elif layout == "bundle16":
-        slice_ptr = vl.sell()[0]
+        slice_ptr = vl.sell_slices()[0]
         bundle, row16, exc_ptr, exc_col, exc_W, ne = d.sell_bundle16()
         ctx.call("dcpse_stage_bundle16_f32", vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(slice_ptr), _lib.ptr(row16),
                  _lib.ptr(bundle), _lib.ptr(exc_ptr), _lib.ptr(exc_col), _lib.ptr(exc_W), ne, _lib.ptr(u),

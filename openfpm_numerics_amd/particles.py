@@ -92,6 +92,7 @@ class ParticleSet:
         fresh = VerletList.build(self, rCut, domain=vl.domain)
         vl.__dict__.update(fresh.__dict__)
         vl._sell = None
+        vl._slices = None
         return vl
 
 
@@ -138,6 +139,7 @@ class VerletList:
         self.domain = domain
         self._host = None
         self._sell = None
+        self._slices = None
         self.map_ctr = -1
 
     @property
@@ -191,18 +193,28 @@ class VerletList:
     def row_lengths(self) -> torch.Tensor:
         return self.row_ptr[1:] - self.row_ptr[:-1]
 
+    def sell_slices(self):
+        """(slice_ptr, sell_row, n_slots): the slice structure of the
+        sliced-ELL layout (what the 16-bit bundles need); built on first use."""
+        if getattr(self, "_slices", None) is None:
+            self._slices = sell_slices(self.row_ptr, self.n_rows)
+        return self._slices
+
     def sell(self):
         """(slice_ptr, sell_row, sell_col): the sliced-ELL execution layout of
-        this support, shared by every FP32 operator assembled on it; built on
-        first use."""
+        this support with 32-bit columns, shared by every FP32 operator
+        assembled on it; built on first use."""
         if getattr(self, "_sell", None) is None:
-            self._sell = sell_layout(self.row_ptr, self.col, self.n_rows)
+            slice_ptr, sell_row, n_slots = self.sell_slices()
+            self._sell = (slice_ptr, sell_row, sell_cols(self.row_ptr, self.col, self.n_rows, slice_ptr, sell_row,
+                                                         n_slots))
         return self._sell
 
 
-def sell_layout(row_ptr: torch.Tensor, col: torch.Tensor, n_rows: int):
-    """Sliced-ELL layout of a CSR: slices of 64 rows, rows ordered by
-    decreasing length inside a slice (sell_row maps lane -> row, -1 = none)."""
+def sell_slices(row_ptr: torch.Tensor, n_rows: int):
+    """Slice structure of the sliced-ELL layout of a CSR: slices of 64 rows,
+    rows ordered by decreasing length inside a slice (sell_row maps lane ->
+    row, -1 = none); (slice_ptr, sell_row, n_slots)."""
     ctx = _lib.context(row_ptr.device.index or 0)
     dev = row_ptr.device
     n_slices = (n_rows + 63) // 64
@@ -211,11 +223,24 @@ def sell_layout(row_ptr: torch.Tensor, col: torch.Tensor, n_rows: int):
     n_slots = np.zeros(1, np.int64)
     ctx.call("dcpse_sell_count", n_rows, _lib.ptr(row_ptr), _lib.ptr(slice_ptr), _lib.ptr(sell_row),
              n_slots.ctypes.data)
-    sell_col = torch.empty(max(int(n_slots[0]), 4), dtype=torch.int32, device=dev)
+    return slice_ptr, sell_row, int(n_slots[0])
+
+
+def sell_cols(row_ptr: torch.Tensor, col: torch.Tensor, n_rows: int, slice_ptr, sell_row, n_slots: int):
+    """The 32-bit columns of the sliced-ELL layout (padding slots included)."""
+    sell_col = torch.empty(max(n_slots, 4), dtype=torch.int32, device=row_ptr.device)
     if n_rows:
-        ctx.call("dcpse_sell_fill_cols", n_rows, _lib.ptr(row_ptr), _lib.ptr(col), _lib.ptr(slice_ptr),
-                 _lib.ptr(sell_row), _lib.ptr(sell_col))
-    return slice_ptr, sell_row, sell_col
+        _lib.context(row_ptr.device.index or 0).call(
+            "dcpse_sell_fill_cols", n_rows, _lib.ptr(row_ptr), _lib.ptr(col), _lib.ptr(slice_ptr), _lib.ptr(sell_row),
+            _lib.ptr(sell_col))
+    return sell_col
+
+
+def sell_layout(row_ptr: torch.Tensor, col: torch.Tensor, n_rows: int):
+    """Sliced-ELL layout of a CSR: (slice_ptr, sell_row, sell_col), see
+    sell_slices and sell_cols."""
+    slice_ptr, sell_row, n_slots = sell_slices(row_ptr, n_rows)
+    return slice_ptr, sell_row, sell_cols(row_ptr, col, n_rows, slice_ptr, sell_row, n_slots)
 
 
 def sell_weights(row_ptr: torch.Tensor, W: torch.Tensor, layout, n_rows: int):
