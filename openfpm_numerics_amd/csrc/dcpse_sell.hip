@@ -194,42 +194,76 @@ __global__ __launch_bounds__(256) void k_bundle16_count(int64_t n_rows, const in
     }
 }
 
+// The 16-bit bundle, one block per slice, through an LDS tile of 32 slots x
+// 64 lanes: the four waves read 32 entries of two rows at a time (columns and
+// each operator's weights, coalesced along the row) into the tile, then each
+// thread writes its lane's slot groups of the tile to the bundle, 16-bit
+// column offsets (0 where the offset does not fit: the lane's exception list
+// holds that neighbour) and the weights, padding slots zero.
+constexpr int kB16Chunk = 32;  // slots per tile
+constexpr int kB16Pitch = 65;  // tile row pitch in words (no bank conflicts)
 __global__ __launch_bounds__(256) void k_sell_fill_bundle16(
     int64_t n_rows, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ col, int n_ops,
     const float* W0, const float* W1, const float* W2, const float* W3, const int64_t* __restrict__ slice_ptr,
     const int32_t* __restrict__ sell_row, const int64_t* __restrict__ exc_ptr, int32_t* __restrict__ bundle,
     int32_t* __restrict__ sell_row16) {
+    __shared__ int32_t tc[kB16Chunk * kB16Pitch];      // column offsets
+    __shared__ float tw[4][kB16Chunk * kB16Pitch];     // weights per operator
+    __shared__ int32_t srow[64];
+    __shared__ int64_t srs[64];
+    __shared__ int32_t slen[64];
     const int64_t n_slices = (n_rows + 63) / 64;
     const float* W[4] = {W0, W1, W2, W3};
     const int64_t blk = 128 + 256 * (int64_t)n_ops;
     int16_t* cols16 = reinterpret_cast<int16_t*>(bundle);
-    const int l = threadIdx.x & 63, gsub = threadIdx.x >> 6;
+    const int tid = threadIdx.x;
+    const int l = tid & 63, gsub = tid >> 6;
     for (int64_t s = blockIdx.x; s < n_slices; s += gridDim.x) {
-        const int64_t i = s * 64 + l;
-        const SliceLane r = slice_lane(row_ptr, sell_row, i);
-        if (gsub == 0) sell_row16[i] = r.row < 0 ? r.row : (exc_ptr[i + 1] > exc_ptr[i] ? (r.row | kExcFlag) : r.row);
+        __syncthreads();  // the previous slice's tile and lane table are consumed
+        if (tid < 64) {
+            const int64_t i = s * 64 + tid;
+            const SliceLane r = slice_lane(row_ptr, sell_row, i);
+            srow[tid] = r.row;
+            srs[tid] = r.rs;
+            slen[tid] = r.len;
+            sell_row16[i] = r.row < 0 ? r.row : (exc_ptr[i + 1] > exc_ptr[i] ? (r.row | kExcFlag) : r.row);
+        }
+        __syncthreads();
         const int64_t base = slice_ptr[s];
         const int ng = (int)((slice_ptr[s + 1] - base) >> 8);
-        for (int g = gsub; g < ng; g += 4) {
-            const int64_t gb = (base / 256 + g) * blk;  // word offset of the slot group's block
-            vs4a d;
-#pragma unroll
-            for (int jj = 0; jj < 4; ++jj) {
-                int64_t dj = 0;
-                if (4 * g + jj < r.len) {
-                    dj = col_offset(col[r.rs + 4 * g + jj], r.row, n_rows);
-                    if (far16(dj)) dj = 0;  // in the lane's exception list instead
+        for (int g0 = 0; g0 < ng; g0 += kB16Chunk / 4) {
+            // load: wave w takes lanes 16 w .. 16 w + 15, two rows at a time
+            const int e = tid & 31;
+            const int j = 4 * g0 + e;
+            for (int it = 0; it < 8; ++it) {
+                const int ll = 16 * gsub + 2 * it + ((tid >> 5) & 1);
+                const int len = slen[ll];
+                const int64_t rs = srs[ll];
+                int32_t d = 0;
+                if (j < len) {
+                    const int64_t dj = col_offset(col[rs + j], srow[ll], n_rows);
+                    d = far16(dj) ? 0 : (int32_t)dj;
                 }
-                d[jj] = (int16_t)dj;
+                tc[e * kB16Pitch + ll] = d;
+                for (int o = 0; o < n_ops; ++o) tw[o][e * kB16Pitch + ll] = j < len ? W[o][rs + j] : 0.0f;
             }
-            *reinterpret_cast<vs4a*>(cols16 + 2 * gb + 4 * l) = d;
-            for (int o = 0; o < n_ops; ++o) {
-                vi4a v;
+            __syncthreads();
+            // write: thread (lane l, gsub) writes the tile's groups gsub, gsub + 4
+            const int gn = min(kB16Chunk / 4, ng - g0);
+            for (int gl = gsub; gl < gn; gl += 4) {
+                const int64_t gb = (base / 256 + g0 + gl) * blk;  // word offset of the slot group's block
+                vs4a dv;
 #pragma unroll
-                for (int jj = 0; jj < 4; ++jj)
-                    v[jj] = 4 * g + jj < r.len ? __float_as_int(W[o][r.rs + 4 * g + jj]) : 0;
-                *reinterpret_cast<vi4a*>(bundle + gb + 128 + o * 256 + 4 * l) = v;
+                for (int jj = 0; jj < 4; ++jj) dv[jj] = (int16_t)tc[(4 * gl + jj) * kB16Pitch + l];
+                *reinterpret_cast<vs4a*>(cols16 + 2 * gb + 4 * l) = dv;
+                for (int o = 0; o < n_ops; ++o) {
+                    vi4a v;
+#pragma unroll
+                    for (int jj = 0; jj < 4; ++jj) v[jj] = __float_as_int(tw[o][(4 * gl + jj) * kB16Pitch + l]);
+                    *reinterpret_cast<vi4a*>(bundle + gb + 128 + o * 256 + 4 * l) = v;
+                }
             }
+            __syncthreads();
         }
     }
 }
