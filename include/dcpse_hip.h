@@ -279,6 +279,17 @@ dcpse_status dcpse_apply_transpose_sell_f32(dcpse_ctx* ctx, int64_t n_cols, cons
                                             const int64_t* slice_ptr, const int32_t* sell_rowT,
                                             const int32_t* sell_colT, const float* sell_WT, const float* diag,
                                             const float* g, float* out, float alpha, float beta);
+/* Same result as dcpse_apply_transpose_f32 with one field, from a 16-bit
+ * bundle (dcpse_sell_bundle16_count / dcpse_sell_fill_bundle16_f32) of the
+ * transposed weights WT on the operator's own row_ptr / col and slices: for
+ * a structurally symmetric support (one set, one cut-off) the transposed CSR
+ * has the same row_ptr and col. out = alpha (WT g + diag .* g) + beta out. */
+dcpse_status dcpse_apply_transpose_bundle16_f32(dcpse_ctx* ctx, int64_t n, const int64_t* row_ptr,
+                                                const int64_t* slice_ptr, const int32_t* sell_row16,
+                                                const int32_t* bundle, const int64_t* exc_ptr,
+                                                const int32_t* exc_col, const float* exc_W, int64_t n_exc,
+                                                const float* diag, const float* g, float* out, float alpha,
+                                                float beta);
 /* Software pipeline of the SELL apply: groups of 4 slots per batch and
  * stages. depth 2: (groups 2 or 4) the stream loads of one batch overlap the
  * gathers and FMAs of the other; depth 3: (groups 1 or 2) columns are loaded

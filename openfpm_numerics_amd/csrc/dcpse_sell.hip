@@ -501,7 +501,7 @@ __global__ __launch_bounds__(kSellWaves * kWave) void k_apply_sell(const SellArg
         if (has_exc) {  // the far neighbours of this row (rare; after every pipelined load)
             const int64_t i = s * 64 + lane;
             for (int64_t k = a.exc_ptr[i]; k < a.exc_ptr[i + 1]; ++k) {
-                const double d = (double)a.f[a.exc_col[k]] - (double)fp;
+                const double d = MODE == 1 ? (double)a.f[a.exc_col[k]] : (double)a.f[a.exc_col[k]] - (double)fp;
 #pragma unroll
                 for (int o = 0; o < NOPS; ++o) acc[o] = fma((double)a.exc_W[o * a.n_exc + k], d, acc[o]);
             }
@@ -989,6 +989,40 @@ dcpse_status dcpse_apply_transpose_sell_f32(dcpse_ctx* ctx, int64_t n_cols, cons
     float* Op[1] = {out};
     return sell_apply(ctx, 1, n_cols, rowT_ptr, slice_ptr, sell_rowT, sell_colT, nullptr, 1, Wp, g, nullptr, diag, Op,
                       alpha, beta);
+}
+
+dcpse_status dcpse_apply_transpose_bundle16_f32(dcpse_ctx* ctx, int64_t n, const int64_t* row_ptr,
+                                                const int64_t* slice_ptr, const int32_t* sell_row16,
+                                                const int32_t* bundle, const int64_t* exc_ptr,
+                                                const int32_t* exc_col, const float* exc_W, int64_t n_exc,
+                                                const float* diag, const float* g, float* out, float alpha,
+                                                float beta) {
+    if (!ctx) return DCPSE_ERR_INVALID;
+    if (!row_ptr || !slice_ptr || !sell_row16 || !bundle || !exc_ptr || !diag || !g || !out || n < 0 || n_exc < 0 ||
+        (n_exc > 0 && (!exc_col || !exc_W)))
+        return fail(ctx, DCPSE_ERR_INVALID, "null pointer or bad size");
+    DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    if (n == 0) return DCPSE_OK;
+    SellArgs a{};
+    a.n_rows = n;
+    a.row_ptr = row_ptr;
+    a.slice_ptr = slice_ptr;
+    a.sell_row = sell_row16;
+    a.bundle = bundle;
+    a.exc_ptr = exc_ptr;
+    a.exc_col = exc_col;
+    a.exc_W = exc_W;
+    a.n_exc = n_exc;
+    a.f = g;
+    a.f_self = g;
+    a.diag = diag;
+    a.out[0] = out;
+    a.alpha = alpha;
+    a.beta = beta;
+    if (ctx->sell_groups == 4) launch_sell16<1, 4, 1>(a, ctx->stream);
+    else launch_sell16<1, 2, 1>(a, ctx->stream);
+    DCPSE_LAUNCH_CHECK(ctx);
+    return DCPSE_OK;
 }
 
 dcpse_status dcpse_set_sell_pipeline(dcpse_ctx* ctx, int32_t groups, int32_t depth) {

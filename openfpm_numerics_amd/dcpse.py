@@ -255,6 +255,18 @@ class Dcpse:
         self._T[dtype] = (rowT, colT[: vl.nnz], WT[: vl.nnz], diag)
         return self._T[dtype]
 
+    def transposed_bundle16(self):
+        """The FP32 transposed weights as a 16-bit bundle on this operator's own
+        support (sell_bundle16's tuple), when the transposed CSR has the same
+        row_ptr and col (a symmetric support: one set, one cut-off); else None."""
+        if "bundle16" not in self._T:
+            rowT, colT, WT, _ = self.transposed(torch.float32)
+            vl = self.verletList
+            same = (self.same_set and torch.equal(rowT, vl.row_ptr) and colT.numel() == vl.col.numel()
+                    and torch.equal(colT, vl.col))
+            self._T["bundle16"] = bundle16(vl, [WT]) if same else None
+        return self._T["bundle16"]
+
     def transposed_sell(self):
         """SELL layout of the FP32 transposed operator: (slice_ptr, sell_rowT, sell_colT, sell_WT)."""
         if self._T.get("sell") is None:
@@ -282,6 +294,14 @@ class Dcpse:
         o = torch.empty_like(g2) if out is None else out.reshape(nb, n)
         ctx = _lib.context(g.device.index or 0)
         if g.dtype == torch.float32 and nb == 1 and USE_SELL and self._count_apply32("adj"):
+            b16 = self.transposed_bundle16()
+            if b16 is not None:
+                bundle, row16, exc_ptr, exc_col, exc_W, ne = b16
+                vl = self.verletList
+                ctx.call("dcpse_apply_transpose_bundle16_f32", n, _lib.ptr(vl.row_ptr), _lib.ptr(vl.sell_slices()[0]),
+                         _lib.ptr(row16), _lib.ptr(bundle), _lib.ptr(exc_ptr), _lib.ptr(exc_col), _lib.ptr(exc_W), ne,
+                         _lib.ptr(diag), _lib.ptr(g2), _lib.ptr(o), alpha, beta)
+                return o.reshape(g.shape)
             slice_ptr, sell_rowT, sell_colT, sell_WT = self.transposed_sell()
             ctx.call("dcpse_apply_transpose_sell_f32", n, _lib.ptr(rowT), _lib.ptr(slice_ptr), _lib.ptr(sell_rowT),
                      _lib.ptr(sell_colT), _lib.ptr(sell_WT), _lib.ptr(diag), _lib.ptr(g2), _lib.ptr(o), alpha, beta)
@@ -513,9 +533,13 @@ def sell_bundle16(ops):
     """The interleaved sliced-ELL copy with 16-bit column offsets (operators on
     one particle set): (bundle, sell_row16, exc_ptr, exc_col, exc_W, n_exc),
     the far neighbours (offset beyond +-32767 modulo N) listed per lane."""
-    vl = ops[0].verletList
     if not all(op.same_set for op in ops):
         raise ValueError("16-bit column offsets need operators whose support and domain sets are the same")
+    return bundle16(ops[0].verletList, [op.weights(torch.float32) for op in ops])
+
+
+def bundle16(vl: VerletList, Ws):
+    """sell_bundle16 of FP32 weight arrays Ws in the CSR order of vl."""
     slice_ptr, sell_row, n_slots = vl.sell_slices()
     dev = vl.row_ptr.device
     ctx = _lib.context(dev.index or 0)
@@ -525,12 +549,11 @@ def sell_bundle16(ops):
     ctx.call("dcpse_sell_bundle16_count", vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(vl.col), _lib.ptr(sell_row),
              _lib.ptr(exc_ptr), n_exc.ctypes.data)
     ne = int(n_exc[0])
-    Ws = [op.weights(torch.float32) for op in ops]
-    bundle = torch.empty(max(n_slots // 2 + len(ops) * n_slots, 4), dtype=torch.int32, device=dev)
+    bundle = torch.empty(max(n_slots // 2 + len(Ws) * n_slots, 4), dtype=torch.int32, device=dev)
     sell_row16 = torch.empty_like(sell_row)
     exc_col = torch.empty(max(ne, 1), dtype=torch.int32, device=dev)
-    exc_W = torch.empty(max(len(ops) * ne, 1), dtype=torch.float32, device=dev)
-    ctx.call("dcpse_sell_fill_bundle16_f32", vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(vl.col), len(ops),
+    exc_W = torch.empty(max(len(Ws) * ne, 1), dtype=torch.float32, device=dev)
+    ctx.call("dcpse_sell_fill_bundle16_f32", vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(vl.col), len(Ws),
              _lib.ptr_array(Ws), _lib.ptr(slice_ptr), _lib.ptr(sell_row), _lib.ptr(exc_ptr), ne, _lib.ptr(bundle),
              _lib.ptr(sell_row16), _lib.ptr(exc_col), _lib.ptr(exc_W))
     return bundle, sell_row16, exc_ptr, exc_col, exc_W, ne

@@ -45,6 +45,22 @@ def test_transpose_matches_value_nz_transposed(spread, oracle, cuda):
     want32 = (A.T @ g.astype(np.float32).astype(np.float64).T).T
     rel32 = np.linalg.norm(got32 - want32) / np.linalg.norm(want32)
     assert rel32 <= 1e-5, f"FP32 transpose apply rel L2 {rel32:.3e}"
+    # single fields: the CSR kernel, then the sliced-ELL copy (16-bit bundle
+    # on the operator's own support when the pattern is symmetric, the
+    # transposed 32-bit layout otherwise)
+    from openfpm_numerics_amd import dcpse as D
+
+    sell_after = D.SELL_AFTER
+    D.SELL_AFTER = 2
+    try:
+        one = [lap.applyTranspose(torch.as_tensor(g[1], dtype=torch.float32, device=cuda)).cpu().numpy()
+               for _ in range(2)]
+    finally:
+        D.SELL_AFTER = sell_after
+    assert (lap.dcpse._T.get("bundle16") is None) == spread
+    for o in one:
+        rel1 = np.linalg.norm(o - want32[1]) / np.linalg.norm(want32[1])
+        assert rel1 <= 1e-5, f"FP32 single-field transpose apply rel L2 {rel1:.3e}"
 
 
 def test_inner_product_identity(cuda):

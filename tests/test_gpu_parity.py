@@ -402,7 +402,7 @@ def test_sell_layout_apply(name, oracle, cuda):
             want_t = A.T @ f64
             got_t = lap.applyTranspose(ft).cpu().numpy().astype(np.float64)
             assert np.linalg.norm(got_t - want_t) / np.linalg.norm(want_t) <= 1e-5
-            assert dx.dcpse._W.get("sell") is not None and lap.dcpse._T.get("sell") is not None
+            assert dx.dcpse._W.get("sell") is not None and lap.dcpse._T.get("bundle16") is not None
     finally:
         D.SELL_AFTER = sell_after
         ctx.check(ctx.lib.dcpse_set_sell_pipeline(ctx.h, 2, 2), "pipeline reset")
