@@ -466,8 +466,8 @@ __global__ __launch_bounds__(kRowWaves * kWave) void k_support_rows(
 constexpr int kCellWaves = 2;
 constexpr int kCellMaxPart = 2 * kWave;  // particles of a masked cell
 template <int DIM>
-struct CellCap {  // candidates per LDS copy with positions
-    static constexpr int v = DIM == 3 ? 512 : 256;
+struct CellCap {  // candidates per LDS copy with positions (384: 3.5 waves per SIMD in the count)
+    static constexpr int v = DIM == 3 ? 384 : 256;
 };
 template <int DIM>
 struct MaskCap {  // candidates a particle's mask covers at most
