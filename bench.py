@@ -20,7 +20,7 @@ particles with 35..60 neighbours. Convergence order 2.
             reference-form FP64 apply, Laplacian as 3 separate operators) timed
             on the host cores on a bounded sample, same process.
 
-Prints one JSON line. --gpus above 1 exits: this build runs on one GPU.
+Prints one JSON line. --gpus above 1 exits: the benchmark runs on one GPU.
 """
 from __future__ import annotations
 
@@ -90,7 +90,7 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this DC-PSE build runs on exactly one GPU; "
+        print(json.dumps({"error": "the DC-PSE benchmark runs on exactly one GPU; "
                                    f"--gpus {args.gpus} (WORLD_SIZE={world}) is not supported"}))
         sys.exit(2)
 
