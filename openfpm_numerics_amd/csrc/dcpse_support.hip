@@ -972,7 +972,8 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
 
     uint64_t* masks = nullptr;
     int32_t* spill = nullptr;
-    if (by_cell && (count_pass || cached)) {
+    bool use_cache = by_cell && (count_pass || cached);
+    if (use_cache) {
         // cell list, masks and the list of cells left to the position kernel
         // in the context's support cache
         const size_t cap = (((size_t)(g.ncell + 1) * 8 + 255) & ~(size_t)255) +
@@ -984,9 +985,18 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
             if (sc.buf) (void)hipFree(sc.buf);
             sc.buf = nullptr;
             sc.cap = 0;
-            DCPSE_HIP_CHECK(ctx, hipMalloc(&sc.buf, cap));
-            sc.cap = cap;
+            if (hipMalloc(&sc.buf, cap) == hipSuccess) {
+                sc.cap = cap;
+            } else {
+                // no room for the masks: count from positions, and the fill
+                // (finding no matching cache) fills from positions too
+                (void)hipGetLastError();
+                sc.buf = nullptr;
+                use_cache = false;
+            }
         }
+    }
+    if (use_cache) {
         Workspace cw;
         cw.base = static_cast<char*>(sc.buf);
         cw.cap = sc.cap;
@@ -1022,7 +1032,15 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
     const int64_t rb = std::max<int64_t>(1, std::min<int64_t>((n_dom + kRowWaves - 1) / kRowWaves,
                                                               (int64_t)ctx->num_cus * 16));
     if (count_pass) {
-        if (by_cell) {
+        if (by_cell && !use_cache) {
+            if (dim == 2)
+                k_support_cells<2, false><<<cell_blocks, kCellWaves * kWave, 0, s>>>(
+                    n_sup, rcut, g, cell_start, cell_part, pos_cell, counts, flags, nullptr, nullptr, nullptr);
+            else
+                k_support_cells<3, false><<<cell_blocks, kCellWaves * kWave, 0, s>>>(
+                    n_sup, rcut, g, cell_start, cell_part, pos_cell, counts, flags, nullptr, nullptr, nullptr);
+            DCPSE_LAUNCH_CHECK(ctx);
+        } else if (by_cell) {
             // masked cells, then the listed others from positions
             fill_i32(s, spill, 1, 0);
             if (dim == 2) {
@@ -1063,7 +1081,7 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
         if (nnz >= ((int64_t)1 << 31) * 64)
             return fail(ctx, DCPSE_ERR_RANGE, "nnz out of range");
         if (nnz_out) *nnz_out = nnz;
-        if (by_cell) {
+        if (use_cache) {
             sc.valid = true;
             sc.spill = n_spill > 0;
             sc.dim = dim;
