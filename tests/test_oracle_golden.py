@@ -265,6 +265,20 @@ def test_support_brute_force(oracle, dim, periodic):
         assert np.array_equal(col[rp[p]:rp[p + 1]], want)
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_support_open_box_outside_particles_brute_force(oracle, dim):
+    """Open box with particles up to 0.05 outside it on every side: they
+    fall in the edge cells, and the rows still equal brute force."""
+    rng = np.random.default_rng(40 + dim)
+    pos = rng.random((700 if dim == 2 else 900, dim)) * 1.1 - 0.05
+    rc = 0.09 if dim == 2 else 0.17
+    rp, col = oracle.support(pos, rc, [0] * dim, [1] * dim, 0)
+    d2 = ((pos[:, None, :] - pos[None, :, :]) ** 2).sum(-1)
+    np.fill_diagonal(d2, np.inf)
+    for p in range(pos.shape[0]):
+        assert np.array_equal(col[rp[p]:rp[p + 1]], np.nonzero(d2[p] < rc * rc)[0])
+
+
 def test_support_per_particle_rcut(oracle):
     rng = np.random.default_rng(4)
     pos = rng.random((600, 2))
