@@ -886,6 +886,33 @@ static dcpse_status make_grid(dcpse_ctx* ctx, int dim, const double* box_lo, con
     return DCPSE_OK;
 }
 
+// out[0] = min and out[1] = max(0, max) of v[0..n) (NaNs ignored), one block.
+constexpr int kMinMaxThreads = 1024;
+__global__ __launch_bounds__(kMinMaxThreads) void k_minmax(const double* __restrict__ v, int64_t n,
+                                                          double* __restrict__ out) {
+    __shared__ double smin[kMinMaxThreads], smax[kMinMaxThreads];
+    const int t = threadIdx.x;
+    double lo = n > 0 ? v[0] : 0.0, hi = 0.0;
+    for (int64_t i = t; i < n; i += kMinMaxThreads) {
+        lo = fmin(lo, v[i]);
+        hi = fmax(hi, v[i]);
+    }
+    smin[t] = lo;
+    smax[t] = hi;
+    __syncthreads();
+    for (int off = kMinMaxThreads / 2; off > 0; off >>= 1) {
+        if (t < off) {
+            smin[t] = fmin(smin[t], smin[t + off]);
+            smax[t] = fmax(smax[t], smax[t + off]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[0] = smin[0];
+        out[1] = smax[0];
+    }
+}
+
 static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup, int64_t n_sup,
                                  const double* pos_dom, int64_t n_dom, const double* rcut, int rcut_pp,
                                  const double* box_lo, const double* box_hi, unsigned periodic,
@@ -900,18 +927,18 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
     // the largest cut-off fixes the cell edge
     double rmax = 0.0, rmin = 0.0;
     if (rcut_pp) {
-        // reduce on the host: one copy of n_dom doubles is cheap next to the build
-        double* h = (double*)malloc(sizeof(double) * (size_t)std::max<int64_t>(n_dom, 1));
-        if (!h) return fail(ctx, DCPSE_ERR_INVALID, "host allocation failed");
-        hipError_t e = hipMemcpyAsync(h, rcut, sizeof(double) * n_dom, hipMemcpyDeviceToHost, ctx->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        if (e != hipSuccess) { free(h); DCPSE_HIP_CHECK(ctx, e); }
-        rmin = n_dom > 0 ? h[0] : 0.0;
-        for (int64_t i = 0; i < n_dom; ++i) {
-            rmax = std::max(rmax, h[i]);
-            rmin = std::min(rmin, h[i]);
-        }
-        free(h);
+        // reduced on the device; only the two extremes come back
+        Workspace w0;
+        dcpse_status st0 = reserve_workspace(ctx, 1024, &w0);
+        if (st0) return st0;
+        double* mm = w0.take<double>(2);
+        k_minmax<<<1, kMinMaxThreads, 0, ctx->stream>>>(rcut, n_dom, mm);
+        DCPSE_LAUNCH_CHECK(ctx);
+        double h[2] = {0.0, 0.0};
+        DCPSE_HIP_CHECK(ctx, hipMemcpyAsync(h, mm, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+        DCPSE_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        rmin = h[0];
+        rmax = h[1];
     } else {
         DCPSE_HIP_CHECK(ctx, hipMemcpyAsync(&rmax, rcut, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         DCPSE_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
