@@ -44,6 +44,11 @@ USE_SELL = True
 #: 1.9M particles and 105M neighbours) costs what some 50 applies save, so an
 #: operator reassembled every few applies (moving particles) never builds one.
 SELL_AFTER = 16
+#: The 16-bit-offset bundle serves neighbours whose index offset does not fit
+#: 16 bits from a per-lane exception list; above this fraction of such
+#: neighbours (e.g. Morton-ordered particles: 6.7% at 128^3) the 32-bit
+#: sliced-ELL layout is faster and the automatic choice takes it.
+BUNDLE16_MAX_EXC = 0.01
 _FILE_VERSION = 1
 
 
@@ -178,6 +183,23 @@ class Dcpse:
             self._W["bundle16"] = sell_bundle16([self])
         return self._W["bundle16"]
 
+    def bundle16_suits(self) -> bool:
+        """Whether the 16-bit bundle is the faster FP32 layout of this operator:
+        one particle set, and at most BUNDLE16_MAX_EXC of the neighbours in the
+        exception lists (counted once per support)."""
+        if not self.same_set:
+            return False
+        vl = self.verletList
+        if getattr(vl, "_n_exc16", None) is None:
+            sell_row = vl.sell_slices()[1]
+            exc_ptr = torch.empty(sell_row.numel() + 1, dtype=torch.int64, device=vl.row_ptr.device)
+            n_exc = np.zeros(1, np.int64)
+            _lib.context(vl.row_ptr.device.index or 0).call(
+                "dcpse_sell_bundle16_count", vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(vl.col), _lib.ptr(sell_row),
+                _lib.ptr(exc_ptr), n_exc.ctypes.data)
+            vl._n_exc16 = int(n_exc[0])
+        return vl._n_exc16 <= BUNDLE16_MAX_EXC * max(vl.nnz, 1)
+
     def computeDifferentialOperator(self, f: torch.Tensor, out: torch.Tensor | None = None,
                                     f_domain: torch.Tensor | None = None, alpha=1.0, beta=0.0):
         """D f for a field (N,) or a batch of fields (B, N) over the support set."""
@@ -294,7 +316,7 @@ class Dcpse:
         o = torch.empty_like(g2) if out is None else out.reshape(nb, n)
         ctx = _lib.context(g.device.index or 0)
         if g.dtype == torch.float32 and nb == 1 and USE_SELL and self._count_apply32("adj"):
-            b16 = self.transposed_bundle16()
+            b16 = self.transposed_bundle16() if self.bundle16_suits() else None
             if b16 is not None:
                 bundle, row16, exc_ptr, exc_col, exc_W, ne = b16
                 vl = self.verletList
@@ -491,7 +513,7 @@ def apply_ops(ops, f: torch.Tensor, outs=None, f_domain: torch.Tensor | None = N
     use_sell = False
     if f.dtype == torch.float32 and nb == 1 and USE_SELL:
         use_sell = all([op._count_apply32("fwd") for op in ops])
-    if use_sell and len(ops) == 1 and ops[0].same_set:
+    if use_sell and len(ops) == 1 and ops[0].bundle16_suits():
         # one operator: its 16-bit-offset bundle, built once per assembly
         slice_ptr = vl.sell_slices()[0]
         bundle, row16, exc_ptr, exc_col, exc_W, ne = ops[0].sell_bundle16()

@@ -31,7 +31,8 @@ def rk4(op, f: torch.Tensor, dt: float, n_steps: int, alpha: float = 1.0, layout
     16-bit column offsets, the fewest bytes), "sell" (32-bit columns), "csr",
     or "auto": the 4 n_steps stage applies count towards the operator's
     ``dcpse.SELL_AFTER`` and run on "bundle16" once it is reached (its copy is
-    then built), on "csr" before. float64 state runs the FP64 CSR kernels.
+    then built; "sell" where too many neighbours fall outside 16-bit offsets,
+    ``dcpse.BUNDLE16_MAX_EXC``), on "csr" before. float64 state runs the FP64 CSR kernels.
     ``work`` (3N of f's dtype) may be passed to avoid an allocation per call.
     """
     d = _dcpse(op)
@@ -49,7 +50,7 @@ def rk4(op, f: torch.Tensor, dt: float, n_steps: int, alpha: float = 1.0, layout
     ctx.bind_current_stream()
     if f.dtype == torch.float32 and layout == "auto":
         d._n32["fwd"] += 4 * int(n_steps) - 1
-        layout = "bundle16" if d._count_apply32("fwd") else "csr"
+        layout = ("bundle16" if d.bundle16_suits() else "sell") if d._count_apply32("fwd") else "csr"
     if f.dtype == torch.float64:
         ctx.call("dcpse_rk4_f64", n, _lib.ptr(vl.row_ptr), _lib.ptr(vl.col), _lib.ptr(d.weights(torch.float64)),
                  float(alpha), float(dt), int(n_steps), _lib.ptr(f), _lib.ptr(work))
@@ -83,7 +84,7 @@ def stage(op, u: torch.Tensor, alpha: float, y_a: torch.Tensor | None, ca: float
     ctx.bind_current_stream()
     p = _lib.ptr
     if u.dtype == torch.float32 and layout == "auto":
-        layout = "bundle16" if d._count_apply32("fwd") else "csr"
+        layout = ("bundle16" if d.bundle16_suits() else "sell") if d._count_apply32("fwd") else "csr"
     if u.dtype == torch.float64:
         ctx.call("dcpse_stage_f64", vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(vl.col),
                  _lib.ptr(d.weights(torch.float64)), _lib.ptr(u), float(alpha), p(y_a), float(ca), p(out_a), p(y_b),

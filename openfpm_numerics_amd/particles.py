@@ -93,6 +93,7 @@ class ParticleSet:
         vl.__dict__.update(fresh.__dict__)
         vl._sell = None
         vl._slices = None
+        vl._n_exc16 = None
         return vl
 
 
@@ -140,6 +141,7 @@ class VerletList:
         self._host = None
         self._sell = None
         self._slices = None
+        self._n_exc16 = None
         self.map_ctr = -1
 
     @property

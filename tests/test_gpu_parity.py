@@ -501,6 +501,18 @@ def test_sell_bundle16_far_neighbours(oracle, cuda):
     assert ops[0].sell_bundle16()[5] > vl.nnz // 10
     scale = float((outs["sell"] - ya).abs().max())
     assert float((outs["sell"] - outs["bundle16"]).abs().max()) <= 1e-6 * scale
+    # with this many far neighbours the automatic choice takes the 32-bit layout
+    from openfpm_numerics_amd import dcpse as D
+
+    assert not ops[0].bundle16_suits()
+    sell_after = D.SELL_AFTER
+    D.SELL_AFTER = 1
+    try:
+        lap_auto = ops[0].computeDifferentialOperator(f)
+    finally:
+        D.SELL_AFTER = sell_after
+    assert ops[0]._W.get("sell") is not None
+    assert float((lap_auto - a[0]).abs().max()) <= 1e-6 * float(a[0].abs().max())
 
 
 @pytest.mark.parametrize("name", sorted(WORKLOADS))
