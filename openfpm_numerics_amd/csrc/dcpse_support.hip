@@ -569,19 +569,28 @@ __device__ __forceinline__ void stage_candidates(int c0, int cn, int ccnt, int e
     wave_lds_sync();
 }
 
-// The distance test of candidate slot t of the LDS copy against xp (the
-// minimum image in periodic dimensions), as in k_support_rows.
+// The distance test of a candidate at xq against xp (the minimum image in
+// periodic dimensions), as in k_support_rows.
 template <int DIM>
-__device__ __forceinline__ bool within_cut(const double* xp, const double* sx, int sx_stride, int t, const CellGrid& g,
-                                           double r2) {
+__device__ __forceinline__ bool within_cut_at(const double* xp, const double* xq, const CellGrid& g, double r2) {
     double d[3];
 #pragma unroll
     for (int dd = 0; dd < DIM; ++dd)
-        d[dd] = min_image_r(xp[dd] - sx[dd * sx_stride + t], g.len[dd], g.ilen[dd], (g.periodic >> dd) & 1u);
+        d[dd] = min_image_r(xp[dd] - xq[dd], g.len[dd], g.ilen[dd], (g.periodic >> dd) & 1u);
     double d2 = d[DIM - 1] * d[DIM - 1];
 #pragma unroll
     for (int dd = DIM - 2; dd >= 0; --dd) d2 = fma(d[dd], d[dd], d2);
     return d2 < r2;
+}
+
+// ... of candidate slot t of the LDS copy.
+template <int DIM>
+__device__ __forceinline__ bool within_cut(const double* xp, const double* sx, int sx_stride, int t, const CellGrid& g,
+                                           double r2) {
+    double xq[3];
+#pragma unroll
+    for (int dd = 0; dd < DIM; ++dd) xq[dd] = sx[dd * sx_stride + t];
+    return within_cut_at<DIM>(xp, xq, g, r2);
 }
 
 // Count pass over the masked cells (see above): row lengths to counts[p] and
@@ -638,37 +647,71 @@ __global__ __launch_bounds__(kCellWaves * kWave) void k_support_cells_count(
             if (c0 > 0) wave_lds_sync();
             stage_candidates<DIM, true>(c0, cn, ccnt, excl, mark[wv], seg_cst[wv], seg_ex[wv], cell_part, pos_cell,
                                         n, sq[wv], sx[wv], CAP, lane);
-            // particle i against the copy: its take count, and its mask words
-            // of the copy (lane w: word w), stored together
-            auto test = [&](int i, int pid, const double* xp) {
-                uint64_t mword = 0;
-                int nacc = 0;
+            // particles i and i + 1 against the copy (each candidate read
+            // from LDS once for both): their take counts, and their mask words
+            // of the copy (lane w: word w), stored together. With `two` false
+            // only particle i.
+            auto test2 = [&](int i, int pa, const double* xa, int pb, const double* xb, bool two, int& na,
+                             int& nb) {
+                uint64_t mwa = 0, mwb = 0;
+                na = nb = 0;
                 for (int t0 = 0; t0 < cn; t0 += kWave) {
                     const int t = min(t0 + lane, cn - 1);
+                    double xq[3];
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) xq[d] = sx[wv][d * CAP + t];
+                    const int q = sq[wv][t];
+                    const bool valid = t0 + lane < cn;
                     // evaluated without short-circuit: no branches in the loop
-                    const bool near = within_cut<DIM>(xp, sx[wv], CAP, t, g, r2);
-                    const bool take = (t0 + lane < cn) & (sq[wv][t] != pid) & near;
-                    const unsigned long long m = __ballot(take);
-                    if (lane == (t0 >> 6)) mword = m;
-                    nacc += __popcll(m);
+                    const bool neara = within_cut_at<DIM>(xa, xq, g, r2);
+                    const bool nearb = within_cut_at<DIM>(xb, xq, g, r2);
+                    const unsigned long long ma = __ballot(valid & (q != pa) & neara);
+                    const unsigned long long mb = __ballot(valid & (q != pb) & nearb & two);
+                    if (lane == (t0 >> 6)) {
+                        mwa = ma;
+                        mwb = mb;
+                    }
+                    na += __popcll(ma);
+                    nb += __popcll(mb);
                 }
-                if (lane < ((cn + kWave - 1) >> 6)) masks[(pst + i) * nws + (c0 >> 6) + lane] = mword;
-                return nacc;
+                const int nw = (cn + kWave - 1) >> 6;
+                if (lane < nw) masks[(pst + i) * nws + (c0 >> 6) + lane] = mwa;
+                if (two && lane < nw) masks[(pst + i + 1) * nws + (c0 >> 6) + lane] = mwb;
             };
-            const int n0 = min(np, kWave);
-            for (int i = 0; i < n0; ++i) {
-                double xp[3];
+            // particle i of the cell: its index and position (lane i, or
+            // lane i - 64 of the second set)
+            auto part = [&](int i, int& pid, double* xp) {
+                if (i < kWave) {
+                    pid = __builtin_amdgcn_readlane(pid0, i);
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) xp[d] = readlane_d(x0[d], i);
-                const int nacc = test(i, __builtin_amdgcn_readlane(pid0, i), xp);
-                if (lane == i) cnt0 += nacc;
-            }
-            for (int i = kWave; i < np; ++i) {
-                double xp[3];
+                    for (int d = 0; d < DIM; ++d) xp[d] = readlane_d(x0[d], i);
+                } else {
+                    pid = __builtin_amdgcn_readlane(pid1, i - kWave);
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) xp[d] = readlane_d(x1[d], i - kWave);
-                const int nacc = test(i, __builtin_amdgcn_readlane(pid1, i - kWave), xp);
-                if (lane == i - kWave) cnt1 += nacc;
+                    for (int d = 0; d < DIM; ++d) xp[d] = readlane_d(x1[d], i - kWave);
+                }
+            };
+            for (int i = 0; i < np; i += 2) {
+                const bool two = i + 1 < np;
+                int pa, pb;
+                double xa[3] = {0.0, 0.0, 0.0}, xb[3] = {0.0, 0.0, 0.0};
+                part(i, pa, xa);
+                if (two) part(i + 1, pb, xb);
+                else pb = pa;
+                int na, nb;
+                test2(i, pa, xa, pb, xb, two, na, nb);
+                if (i < kWave) {
+                    if (lane == i) cnt0 += na;
+                } else if (lane == i - kWave) {
+                    cnt1 += na;
+                }
+                if (two) {
+                    if (i + 1 < kWave) {
+                        if (lane == i + 1) cnt0 += nb;
+                    } else if (lane == i + 1 - kWave) {
+                        cnt1 += nb;
+                    }
+                }
             }
         }
         if (lane < np) {
