@@ -220,6 +220,10 @@ class dcpse_operator_base {
         get<V>()->initializeUpdate(parts);
     }
     template <typename V>
+    auto checkMomenta(V& parts) {
+        return get<V>()->checkMomenta(parts);
+    }
+    template <typename V>
     void save(V&, const std::string& file) {
         get<V>()->save(file);
     }

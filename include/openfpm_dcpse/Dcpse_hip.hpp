@@ -5,18 +5,21 @@
 // (src/DCPSE/Dcpse.hpp:43-826): the same constructors (:91-130), the bulk
 // computeDifferentialOperator<f, Df> (:467-499), the row accessors getNumNN /
 // getCoeffNN / getIndexNN / getSign / getEpsilonInvPrefactor (:507-550),
-// initializeUpdate (:661-687), save / load (:302-379) and DrawKernel
-// (:151-193). The per-particle computeDifferentialOperator(key, expr) used by
+// initializeUpdate (:661-687), save / load (:302-379), checkMomenta (:381-457)
+// and DrawKernel (:151-193). The per-particle computeDifferentialOperator(key, expr) used by
 // the reference's host expression loop is replaced by whole-field device
 // applies (see DCPSE_op.hpp in this directory).
 #pragma once
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <fstream>
+#include <iostream>
 #include <string>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "dcpse_runtime.hpp"
@@ -152,6 +155,58 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
         mirror();
         for (int64_t j = rp_host_[p]; j < rp_host_[p + 1]; ++j)
             particles.template getProp<prp>((size_t)col_host_[j]) = prop_type(1);
+    }
+
+    // Per basis monomial, the smallest and largest over particles of the moment
+    // sum_q K_pq m_alpha((x_p - x_q) / eps_p), printed one "MOMENTA:" line per
+    // monomial and returned (Dcpse.hpp:381-457). A valid operator gives
+    // b_alpha (DcpseRhs.hpp:27-60) at every particle. Periodic neighbours are
+    // taken at their nearest image, as the support build found them; both
+    // extremes are of the sum itself (the reference's maximum keeps the sum
+    // only when an unset slot exceeds it).
+    std::vector<std::pair<double, double>> checkMomenta(vector_type& particles) {
+        mirror();
+        const uint32_t* beta = sigs_.data();
+        const int32_t m = dcpse_basis_size(dim, beta, convergenceOrder_);
+        std::vector<uint32_t> ex((size_t)std::max(m, 1) * dim);
+        if (m < 0 || dcpse_basis(dim, beta, convergenceOrder_, ex.data(), m) != m)
+            throw openfpm_dcpse::dcpse_error("checkMomenta: no monomial basis");
+        std::vector<std::pair<double, double>> mm((size_t)m, {HUGE_VAL, -HUGE_VAL});
+        std::vector<double> acc((size_t)m);
+        double len[dim];
+        for (unsigned d = 0; d < dim; ++d)
+            len[d] = (double)particles.boxHigh()[d] - (double)particles.boxLow()[d];
+        const unsigned per = particles.periodicMask();
+        const size_t n = eps_host_.size();
+        for (size_t p = 0; p < n; ++p) {
+            std::fill(acc.begin(), acc.end(), 0.0);
+            const auto xp = particles.getPos(vect_dist_key_dx(p));
+            for (int64_t j = rp_host_[p]; j < rp_host_[p + 1]; ++j) {
+                const auto xq = particles.getPos(vect_dist_key_dx((size_t)col_host_[j]));
+                double z[dim];
+                for (unsigned d = 0; d < dim; ++d) {
+                    double dd = (double)xp[d] - (double)xq[d];
+                    if ((per >> d) & 1u) dd -= len[d] * std::nearbyint(dd / len[d]);
+                    z[d] = dd / eps_host_[p];
+                }
+                for (int32_t i = 0; i < m; ++i) {
+                    double v = K_host_[j];
+                    for (unsigned d = 0; d < dim; ++d)
+                        for (uint32_t e = 0; e < ex[(size_t)i * dim + d]; ++e) v *= z[d];
+                    acc[(size_t)i] += v;
+                }
+            }
+            for (int32_t i = 0; i < m; ++i) {
+                mm[(size_t)i].first = std::min(mm[(size_t)i].first, acc[(size_t)i]);
+                mm[(size_t)i].second = std::max(mm[(size_t)i].second, acc[(size_t)i]);
+            }
+        }
+        for (int32_t i = 0; i < m; ++i) {
+            std::cout << "MOMENTA: ";
+            for (unsigned d = 0; d < dim; ++d) std::cout << (d ? " " : "(") << ex[(size_t)i * dim + d];
+            std::cout << ")Min: " << mm[(size_t)i].first << "  Max: " << mm[(size_t)i].second << std::endl;
+        }
+        return mm;
     }
 
     void save(const std::string& file) {

@@ -3,8 +3,9 @@
 //   Dcpse_2D_test, Dcpse_3D_test        src/DCPSE/tests/Dcpse_unit_tests.cpp:36-133, 243-337
 //   dcpse_op_tests, dcpse_op_save_load,
 //   dcpse_op_test_lap                   src/DCPSE/DCPSE_op/tests/DCPSE_op_test_base_tests.cpp:30-175, 394-468
-// plus FP32 fields, expression aliasing and the row accessors (value_nz
-// contract) checked against the apply. Prints one line per case and exits
+// plus FP32 fields, expression aliasing, the row accessors (value_nz
+// contract) checked against the apply, and checkMomenta (Dcpse.hpp:381-457)
+// against the right-hand side b_alpha the kernels are built to reproduce. Prints one line per case and exits
 // non-zero if any fails.
 #include <cmath>
 #include <cstdio>
@@ -339,6 +340,75 @@ static bool dcpse_row_accessors() {
     return worst <= 1e-10 * scale;
 }
 
+// checkMomenta: on a jittered periodic lattice every particle's moments
+// sum_q K_pq m_alpha((x_p - x_q)/eps_p) equal b_alpha (DcpseRhs.hpp:27-60), so
+// the smallest and the largest over particles both sit at b_alpha.
+template <unsigned dim, typename Op>
+static bool momenta_match(Op& op, vector_dist_hip<dim, double, 3>& domain, const uint32_t* beta, unsigned order) {
+    const auto mm = op.checkMomenta(domain);
+    const int32_t m = dcpse_basis_size(dim, beta, order);
+    std::vector<double> b((size_t)m);
+    dcpse_rhs(dim, beta, order, b.data(), m);
+    if ((int32_t)mm.size() != m) return false;
+    double worst = 0;
+    for (int32_t i = 0; i < m; ++i)
+        worst = std::fmax(worst, std::fmax(std::fabs(mm[(size_t)i].first - b[(size_t)i]),
+                                           std::fabs(mm[(size_t)i].second - b[(size_t)i])) /
+                                     std::fmax(1.0, std::fabs(b[(size_t)i])));
+    std::printf("         %d moments, worst deviation from b_alpha %.3e\n", m, worst);
+    return worst <= 1e-9;
+}
+
+static bool dcpse_check_momenta() {
+    bool ok = true;
+    {
+        const size_t sz[2] = {32, 32};
+        Box<2, double> box({0, 0}, {1, 1});
+        size_t bc[2] = {PERIODIC, PERIODIC};
+        const double h = 1.0 / sz[0];
+        const double rCut = 3.1 * h;
+        vector_dist_hip<2, double, 3> domain(0, box, bc, Ghost<2, double>(rCut));
+        auto it = domain.getGridIterator(sz);
+        while (it.isNext()) {
+            auto key = it.get();
+            domain.add();
+            const double i = (double)key.get(0), j = (double)key.get(1);
+            domain.getLastPos()[0] = (i + 0.1 * std::sin(7.3 * i + 3.1 * j)) * h;
+            domain.getLastPos()[1] = (j + 0.1 * std::cos(5.7 * i - 2.3 * j)) * h;
+            ++it;
+        }
+        auto vl = domain.getVerlet<VL_NON_SYMMETRIC | VL_SKIP_REF_PART>(rCut);
+        Derivative_x<decltype(vl)> Dx(domain, vl, 2, rCut);
+        Derivative_xy<decltype(vl)> Dxy(domain, vl, 2, rCut);
+        const uint32_t bx[2] = {1, 0}, bxy[2] = {1, 1};
+        ok &= momenta_match<2>(Dx, domain, bx, 2);
+        ok &= momenta_match<2>(Dxy, domain, bxy, 2);
+    }
+    {
+        const size_t sz[3] = {12, 12, 12};
+        Box<3, double> box({0, 0, 0}, {1, 1, 1});
+        size_t bc[3] = {PERIODIC, PERIODIC, PERIODIC};
+        const double h = 1.0 / sz[0];
+        const double rCut = 2.4 * h;
+        vector_dist_hip<3, double, 3> domain(0, box, bc, Ghost<3, double>(rCut));
+        auto it = domain.getGridIterator(sz);
+        while (it.isNext()) {
+            auto key = it.get();
+            domain.add();
+            const double i = (double)key.get(0), j = (double)key.get(1), k = (double)key.get(2);
+            domain.getLastPos()[0] = (i + 0.1 * std::sin(7.3 * i + 3.1 * j + 1.7 * k)) * h;
+            domain.getLastPos()[1] = (j + 0.1 * std::cos(5.7 * i - 2.3 * j + 0.9 * k)) * h;
+            domain.getLastPos()[2] = (k + 0.1 * std::sin(1.9 * i + 4.1 * j - 3.3 * k)) * h;
+            ++it;
+        }
+        auto vl = domain.getVerlet<VL_NON_SYMMETRIC | VL_SKIP_REF_PART>(rCut);
+        Derivative_zz<decltype(vl)> Dzz(domain, vl, 2, rCut);
+        const uint32_t bzz[3] = {0, 0, 2};
+        ok &= momenta_match<3>(Dzz, domain, bzz, 2);
+    }
+    return ok;
+}
+
 int main() {
     run("Dcpse_2D_test", dcpse_2d_test);
     run("Dcpse_3D_test", dcpse_3d_test);
@@ -348,6 +418,7 @@ int main() {
     run("dcpse_fp32_fields", dcpse_fp32_fields);
     run("dcpse_expression_aliasing", dcpse_expression_aliasing);
     run("dcpse_row_accessors", dcpse_row_accessors);
+    run("dcpse_check_momenta", dcpse_check_momenta);
     std::printf("%d failure(s)\n", failures);
     return failures ? 1 : 0;
 }
