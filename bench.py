@@ -14,6 +14,8 @@ particles with 35..60 neighbours. Convergence order 2.
   step      one fused FP32 pass applying Laplacian and Dx to
             f = sin(2 pi x) sin(2 pi y) sin(2 pi z) (inputs resident in HBM).
   value     M particle-op evals/s = particles * 2 operators / step time.
+  fp64      the same fused pass in FP64 (fp64_apply_us; FP64 weights on the
+            sliced-ELL layout), the reference's own apply precision.
   roofline  the apply kernel against HBM (8 TB/s): algorithmic bytes per
             launch / mean launch duration (HIP events on the launch stream).
   cpu       the CPU oracle (faithful restatement of the reference's CPU Dcpse:
@@ -238,6 +240,28 @@ def main():
     # ---- accuracy -----------------------------------------------------------------
     lap64, dx64 = [t for t in __import__("openfpm_numerics_amd").apply_ops(
         [lap.dcpse, dx.dcpse], f32_t.to(torch.float64))]
+    # ---- FP64 apply (the reference's precision): the fused pass on the FP64
+    # sliced-ELL copy, which the SELL_AFTER-th FP64 apply builds (warm-up)
+    from openfpm_numerics_amd import dcpse as _dcpse
+    f64_d = f32_t.to(torch.float64)
+    o64 = [torch.empty_like(f64_d) for _ in range(2)]
+
+    def run64():
+        _dcpse.apply_ops([lap.dcpse, dx.dcpse], f64_d, outs=o64)
+
+    for _ in range(_dcpse.SELL_AFTER + 2):
+        run64()
+    torch.cuda.synchronize()
+    reps64 = 20
+    a.record()
+    for _ in range(reps64):
+        run64()
+    b.record()
+    torch.cuda.synchronize()
+    fp64_apply_us = a.elapsed_time(b) / reps64 * 1e3
+    if lap.dcpse._W.get("sell64") is None:
+        raise RuntimeError("the FP64 apply did not switch to its sliced-ELL copy")
+    log(f"[bench] FP64 fused apply {fp64_apply_us:.1f} us -> {2 * N / fp64_apply_us:.1f} M evals/s")
     err = {}
     for name, got, ref in (("laplacian", out_lap, lap64), ("dx", out_dx, dx64)):
         d = got.double() - ref
@@ -335,6 +359,8 @@ def main():
         "assembly_ms_by_operator": {"laplacian": asm_lap_ms, "dx": asm_dx_ms},
         "assembly_ms_with_refinement": assembly_refined_ms,
         "support_ms": support_ms,
+        "fp64_apply_us": fp64_apply_us,
+        "fp64_apply_evals_per_s_M": 2 * N / fp64_apply_us,
         "fp32_error_vs_fp64": err,
         "fp64_error_vs_analytic": ana,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -238,6 +238,15 @@ dcpse_status dcpse_apply_sell_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t*
                                   const int32_t* sell_row, const int32_t* sell_col, int32_t n_ops,
                                   const float* const* sell_W, const float* f, const float* f_self,
                                   float* const* out, float alpha, float beta);
+/* FP64 weights in the same slot order (sell_W 32-byte aligned), and the FP64
+ * apply from them and the columns of dcpse_sell_fill_cols: same result as
+ * dcpse_apply_f64 with one field (FP64 sums in another order). */
+dcpse_status dcpse_sell_fill_weights_f64(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const double* W,
+                                         const int64_t* slice_ptr, const int32_t* sell_row, double* sell_W);
+dcpse_status dcpse_apply_sell_f64(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int64_t* slice_ptr,
+                                  const int32_t* sell_row, const int32_t* sell_col, int32_t n_ops,
+                                  const double* const* sell_W, const double* f, const double* f_self,
+                                  double* const* out, double alpha, double beta);
 /* Interleaved form for n_ops (1..4) operators sharing the support: slot
  * group g of slice s is one block of (1 + n_ops) * 256 words - the 64 lanes'
  * columns, then each operator's weights (float bits) - starting at

@@ -63,6 +63,8 @@ SIGNATURES = {
     "dcpse_sell_fill_cols": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp]),
     "dcpse_sell_fill_weights_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp]),
     "dcpse_apply_sell_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _f32, _f32]),
+    "dcpse_sell_fill_weights_f64": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp]),
+    "dcpse_apply_sell_f64": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _f64, _f64]),
     "dcpse_apply_transpose_sell_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f32, _f32]),
     "dcpse_apply_transpose_bundle16_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp,
                                                   _f32, _f32]),

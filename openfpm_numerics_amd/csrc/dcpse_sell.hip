@@ -83,12 +83,13 @@ __device__ __forceinline__ SliceLane slice_lane(const int64_t* row_ptr, const in
 // Every slot is written, padding included, also for the lanes of the last
 // slice that have no row, so no slot of the layout is ever left
 // uninitialised.
-template <bool COLS>
+template <bool COLS, typename TW = float>
 __global__ __launch_bounds__(256) void k_sell_fill(int64_t n_rows, const int64_t* __restrict__ row_ptr,
-                                                   const int32_t* __restrict__ col, const float* __restrict__ W,
+                                                   const int32_t* __restrict__ col, const TW* __restrict__ W,
                                                    const int64_t* __restrict__ slice_ptr,
                                                    const int32_t* __restrict__ sell_row,
-                                                   int32_t* __restrict__ sell_col, float* __restrict__ sell_W) {
+                                                   int32_t* __restrict__ sell_col, TW* __restrict__ sell_W) {
+    typedef TW vw4 __attribute__((ext_vector_type(4)));
     const int64_t n_slices = (n_rows + 63) / 64;
     const int l = threadIdx.x & 63, gsub = threadIdx.x >> 6;
     for (int64_t s = blockIdx.x; s < n_slices; s += gridDim.x) {
@@ -103,10 +104,10 @@ __global__ __launch_bounds__(256) void k_sell_fill(int64_t n_rows, const int64_t
                 for (int jj = 0; jj < 4; ++jj) v[jj] = 4 * g + jj < r.len ? col[r.rs + 4 * g + jj] : 0;
                 *reinterpret_cast<vi4a*>(sell_col + dst) = v;
             } else {
-                vf4a v;
+                vw4 v;
 #pragma unroll
-                for (int jj = 0; jj < 4; ++jj) v[jj] = 4 * g + jj < r.len ? W[r.rs + 4 * g + jj] : 0.0f;
-                *reinterpret_cast<vf4a*>(sell_W + dst) = v;
+                for (int jj = 0; jj < 4; ++jj) v[jj] = 4 * g + jj < r.len ? W[r.rs + 4 * g + jj] : TW(0);
+                *reinterpret_cast<vw4*>(sell_W + dst) = v;
             }
         }
     }
@@ -613,6 +614,137 @@ __global__ __launch_bounds__(kSellWaves * kWave) void k_apply_sell3(const SellAr
     if (live) sell_epilogue<NOPS, MODE>(a, row, fp, acc);
 }
 
+// FP64 forward apply from the sliced-ELL layout: the columns of the FP32
+// layout (sell_col) and FP64 weights in the same slot order. The pipeline is
+// that of k_apply_sell (two register batches; the gathers of the loaded batch
+// before the next batch's stream loads); a slot group is 16 bytes of columns
+// and 32 bytes of weights per operator and lane.
+typedef double vd4a __attribute__((ext_vector_type(4)));
+
+struct Sell64Args {
+    int64_t n_rows;
+    const int64_t* row_ptr;
+    const int64_t* slice_ptr;
+    const int32_t* sell_row;
+    const int32_t* sell_col;
+    const double* W[4];
+    const double* f;
+    const double* f_self;
+    double* out[4];
+    double alpha, beta;
+    int64_t blocks_needed, per_xcd;
+};
+
+template <int NOPS, int UNR>
+struct Sell64Batch {
+    vi4a q[UNR];
+    vd4a w[NOPS][UNR];
+};
+
+template <int NOPS, int UNR>
+__device__ __forceinline__ void sell64_load(const int32_t* cp, const double* const* wp, int g0, int len,
+                                            Sell64Batch<NOPS, UNR>& b) {
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+        const int g = g0 + u;
+        const int64_t off = 4 * g < len ? (int64_t)g * 256 : 0;  // past the row: group 0 again
+        b.q[u] = *(const vi4a*)(cp + off);
+#pragma unroll
+        for (int o = 0; o < NOPS; ++o) b.w[o][u] = *(const vd4a*)(wp[o] + off);
+    }
+}
+
+template <int NOPS, int UNR>
+__device__ __forceinline__ void sell64_gather(const Sell64Args& a, const Sell64Batch<NOPS, UNR>& b,
+                                              double (&fq)[UNR][4]) {
+#pragma unroll
+    for (int u = 0; u < UNR; ++u)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) fq[u][i] = a.f[b.q[u][i]];
+}
+
+template <int NOPS, int UNR>
+__device__ __forceinline__ void sell64_fma(int g0, int len, double fp, const Sell64Batch<NOPS, UNR>& b,
+                                           const double (&fq)[UNR][4], double* acc) {
+#pragma unroll
+    for (int u = 0; u < UNR; ++u)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const bool v = 4 * (g0 + u) + i < len;
+            const double d = fq[u][i] - fp;
+#pragma unroll
+            for (int o = 0; o < NOPS; ++o) acc[o] = v ? fma(b.w[o][u][i], d, acc[o]) : acc[o];
+        }
+}
+
+template <int NOPS, int UNR>
+__global__ __launch_bounds__(kSellWaves * kWave) void k_apply_sell64(const Sell64Args a) {
+    const int64_t hb = blockIdx.x;
+    const int64_t lb = (hb & 7) * a.per_xcd + (hb >> 3);
+    if (lb >= a.blocks_needed) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t s = lb * kSellWaves + (threadIdx.x >> 6);
+    if (s * 64 >= a.n_rows) return;  // whole wavefront
+    const int32_t row = a.sell_row[s * 64 + lane];
+    const bool live = row >= 0;
+    const int len = live ? (int)(a.row_ptr[row + 1] - a.row_ptr[row]) : 0;
+    const double fp = live ? a.f_self[row] : 0.0;
+    const int64_t base = a.slice_ptr[s];
+    const int ng = (int)((a.slice_ptr[s + 1] - base) >> 8);
+    const int32_t* cp = a.sell_col + base + 4 * lane;
+    const double* wp[NOPS];
+#pragma unroll
+    for (int o = 0; o < NOPS; ++o) wp[o] = a.W[o] + base + 4 * lane;
+    double acc[NOPS];
+#pragma unroll
+    for (int o = 0; o < NOPS; ++o) acc[o] = 0.0;
+    if (ng > 0) {
+        Sell64Batch<NOPS, UNR> bA, bB;
+        double gA[UNR][4], gB[UNR][4];
+        sell64_load<NOPS, UNR>(cp, wp, 0, len, bA);
+        for (int g0 = 0; g0 < ng; g0 += 2 * UNR) {
+            sell64_gather<NOPS, UNR>(a, bA, gA);
+            __builtin_amdgcn_sched_barrier(0);
+            sell64_load<NOPS, UNR>(cp, wp, g0 + UNR, len, bB);
+            __builtin_amdgcn_sched_barrier(0);
+            sell64_fma<NOPS, UNR>(g0, len, fp, bA, gA, acc);
+            __builtin_amdgcn_sched_barrier(0);
+            sell64_gather<NOPS, UNR>(a, bB, gB);
+            __builtin_amdgcn_sched_barrier(0);
+            sell64_load<NOPS, UNR>(cp, wp, g0 + 2 * UNR, len, bA);
+            __builtin_amdgcn_sched_barrier(0);
+            sell64_fma<NOPS, UNR>(g0 + UNR, len, fp, bB, gB, acc);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    if (!live) return;
+#pragma unroll
+    for (int o = 0; o < NOPS; ++o) {
+        double val = acc[o] * a.alpha;
+        if (a.beta != 0.0) val = fma(a.beta, a.out[o][row], val);
+        a.out[o][row] = val;
+    }
+}
+
+template <int NOPS, int UNR>
+static void launch_sell64(Sell64Args a, hipStream_t st) {
+    const int64_t n_slices = (a.n_rows + 63) / 64;
+    a.blocks_needed = (n_slices + kSellWaves - 1) / kSellWaves;
+    a.per_xcd = (a.blocks_needed + 7) / 8;
+    k_apply_sell64<NOPS, UNR><<<(unsigned)(a.per_xcd * 8), kSellWaves * kWave, 0, st>>>(a);
+}
+
+template <int UNR>
+static bool dispatch_sell64(const Sell64Args& a, int nops, hipStream_t st) {
+    switch (nops) {
+        case 1: launch_sell64<1, UNR>(a, st); return true;
+        case 2: launch_sell64<2, UNR>(a, st); return true;
+        case 3: launch_sell64<3, UNR>(a, st); return true;
+        case 4: launch_sell64<4, UNR>(a, st); return true;
+    }
+    return false;
+}
+
 template <int NOPS, int UNR, int DEPTH, int MODE>
 static void launch_sell(SellArgs a, hipStream_t st) {
     const int64_t n_slices = (a.n_rows + 63) / 64;
@@ -717,7 +849,7 @@ dcpse_status dcpse_sell_fill_cols(dcpse_ctx* ctx, int64_t n_rows, const int64_t*
     DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     if (n_rows == 0) return DCPSE_OK;
     const int64_t b = std::min<int64_t>((n_rows + 63) / 64, 65536);
-    k_sell_fill<true><<<(unsigned)b, 256, 0, ctx->stream>>>(n_rows, row_ptr, col, nullptr, slice_ptr, sell_row,
+    k_sell_fill<true, float><<<(unsigned)b, 256, 0, ctx->stream>>>(n_rows, row_ptr, col, nullptr, slice_ptr, sell_row,
                                                             sell_col, nullptr);
     DCPSE_LAUNCH_CHECK(ctx);
     return DCPSE_OK;
@@ -731,8 +863,55 @@ dcpse_status dcpse_sell_fill_weights_f32(dcpse_ctx* ctx, int64_t n_rows, const i
     DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     if (n_rows == 0) return DCPSE_OK;
     const int64_t b = std::min<int64_t>((n_rows + 63) / 64, 65536);
-    k_sell_fill<false><<<(unsigned)b, 256, 0, ctx->stream>>>(n_rows, row_ptr, nullptr, W, slice_ptr, sell_row,
+    k_sell_fill<false, float><<<(unsigned)b, 256, 0, ctx->stream>>>(n_rows, row_ptr, nullptr, W, slice_ptr, sell_row,
                                                              nullptr, sell_W);
+    DCPSE_LAUNCH_CHECK(ctx);
+    return DCPSE_OK;
+}
+
+dcpse_status dcpse_sell_fill_weights_f64(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const double* W,
+                                         const int64_t* slice_ptr, const int32_t* sell_row, double* sell_W) {
+    if (!ctx) return DCPSE_ERR_INVALID;
+    if (!row_ptr || !W || !slice_ptr || !sell_row || !sell_W || n_rows < 0)
+        return fail(ctx, DCPSE_ERR_INVALID, "null pointer or negative size");
+    DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    if (n_rows == 0) return DCPSE_OK;
+    const int64_t b = std::min<int64_t>((n_rows + 63) / 64, 65536);
+    k_sell_fill<false, double><<<(unsigned)b, 256, 0, ctx->stream>>>(n_rows, row_ptr, nullptr, W, slice_ptr,
+                                                                     sell_row, nullptr, sell_W);
+    DCPSE_LAUNCH_CHECK(ctx);
+    return DCPSE_OK;
+}
+
+dcpse_status dcpse_apply_sell_f64(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int64_t* slice_ptr,
+                                  const int32_t* sell_row, const int32_t* sell_col, int32_t n_ops,
+                                  const double* const* sell_W, const double* f, const double* f_self,
+                                  double* const* out, double alpha, double beta) {
+    if (!ctx) return DCPSE_ERR_INVALID;
+    if (!row_ptr || !slice_ptr || !sell_row || !sell_col || !sell_W || !f || !out || n_rows < 0 || n_ops < 1)
+        return fail(ctx, DCPSE_ERR_INVALID, "null pointer or bad size");
+    if (n_ops > 4) return fail(ctx, DCPSE_ERR_UNSUPPORTED, "at most 4 operators per pass");
+    for (int o = 0; o < n_ops; ++o)
+        if (!sell_W[o] || !out[o]) return fail(ctx, DCPSE_ERR_INVALID, "null operator pointer");
+    DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    if (n_rows == 0) return DCPSE_OK;
+    Sell64Args a{};
+    a.n_rows = n_rows;
+    a.row_ptr = row_ptr;
+    a.slice_ptr = slice_ptr;
+    a.sell_row = sell_row;
+    a.sell_col = sell_col;
+    for (int o = 0; o < 4; ++o) {
+        a.W[o] = o < n_ops ? sell_W[o] : nullptr;
+        a.out[o] = o < n_ops ? out[o] : nullptr;
+    }
+    a.f = f;
+    a.f_self = f_self ? f_self : f;
+    a.alpha = alpha;
+    a.beta = beta;
+    // two slot groups per batch (437 us for the fused Laplacian + Dx at 128^3,
+    // one group 445 us; the CSR kernel takes 512 us)
+    if (!dispatch_sell64<2>(a, n_ops, ctx->stream)) return fail(ctx, DCPSE_ERR_UNSUPPORTED, "no SELL kernel for %d operators", n_ops);
     DCPSE_LAUNCH_CHECK(ctx);
     return DCPSE_OK;
 }

@@ -35,9 +35,10 @@ AT_LEAST_N_PARTICLES = 3
 
 _FILE_MAGIC = b"DCPSEHIP"
 
-#: FP32 single-field applies run on the sliced-ELL layouts (one operator on one
-#: particle set: the 16-bit-offset bundle; otherwise 32-bit columns); set False
-#: to use the CSR kernels
+#: Single-field applies run on the sliced-ELL layouts: FP32 on the 16-bit-offset
+#: bundle (one operator on one particle set) or 32-bit columns, FP64 on 32-bit
+#: columns with FP64 weights (fused Laplacian + Dx at 128^3: 437 us against the
+#: CSR kernel's 512 us); set False to use the CSR kernels
 USE_SELL = True
 #: ... from an operator's SELL_AFTER-th such apply after its assembly on; the
 #: ones before run the CSR kernel. Building a sliced-ELL copy (about 2.4 ms for
@@ -106,7 +107,7 @@ class Dcpse:
         self.info = None
         self._W = {}
         self._T = {}
-        self._n32 = {"fwd": 0, "adj": 0}  # FP32 single-field applies since assembly
+        self._n32 = {"fwd": 0, "adj": 0, "fwd64": 0}  # single-field applies since assembly
         self.update_ctr = particles.map_ctr
         if opt != LOAD:
             self.initializeStaticSize(self.particlesSupport, self.particlesDomain)
@@ -139,7 +140,7 @@ class Dcpse:
         self.calcKernels = K64[:nnz]
         self._W = {torch.float64: W64[:nnz], torch.float32: W32[:nnz]}
         self._T = {}
-        self._n32 = {"fwd": 0, "adj": 0}
+        self._n32 = {"fwd": 0, "adj": 0, "fwd64": 0}
         self.info = info
         self.update_ctr = map_ctr
 
@@ -175,6 +176,18 @@ class Dcpse:
             self._W["sell"] = sell_weights(self.verletList.row_ptr, self._W[torch.float32], self.verletList.sell(),
                                            self.verletList.n_rows)
         return self._W["sell"]
+
+    def sell_weights64(self) -> torch.Tensor:
+        """FP64 weights in the sliced-ELL layout of this operator's Verlet list."""
+        if self._W.get("sell64") is None:
+            vl = self.verletList
+            slice_ptr, sell_row, n_slots = vl.sell_slices()
+            sw = torch.empty(n_slots, dtype=torch.float64, device=vl.row_ptr.device)
+            _lib.context(vl.row_ptr.device.index or 0).call(
+                "dcpse_sell_fill_weights_f64", vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(self._W[torch.float64]),
+                _lib.ptr(slice_ptr), _lib.ptr(sell_row), _lib.ptr(sw))
+            self._W["sell64"] = sw
+        return self._W["sell64"]
 
     def sell_bundle16(self):
         """This operator alone as a 16-bit-offset bundle (see sell_bundle16),
@@ -299,8 +312,8 @@ class Dcpse:
         return self._T["sell"]
 
     def _count_apply32(self, kind: str) -> bool:
-        """Counts an FP32 single-field apply (kind "fwd" or "adj"); True once the
-        sliced-ELL copy is due (SELL_AFTER)."""
+        """Counts a single-field apply (kind "fwd" or "adj" in FP32, "fwd64" in
+        FP64); True once the sliced-ELL copy is due (SELL_AFTER)."""
         self._n32[kind] += 1
         return self._n32[kind] >= SELL_AFTER
 
@@ -390,7 +403,7 @@ class Dcpse:
         W64 = self.calcKernels * self.localEpsInvPow[rows]
         self._W = {torch.float64: W64, torch.float32: W64.to(torch.float32)}
         self._T = {}
-        self._n32 = {"fwd": 0, "adj": 0}
+        self._n32 = {"fwd": 0, "adj": 0, "fwd64": 0}
         self.info = torch.zeros(n, dtype=torch.int32, device=dev)
         self.update_ctr = self.particlesSupport.map_ctr
 
@@ -526,6 +539,14 @@ def apply_ops(ops, f: torch.Tensor, outs=None, f_domain: torch.Tensor | None = N
         SW = [op.sell_weights() for op in ops]
         ctx.call("dcpse_apply_sell_f32", n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(slice_ptr), _lib.ptr(sell_row),
                  _lib.ptr(sell_col), len(ops), _lib.ptr_array(SW), _lib.ptr(f2), _lib.ptr(fd), _lib.ptr_array(outs), alpha, beta)
+        return [o.reshape(*f.shape[:-1], n_rows) for o in outs]
+    if f.dtype == torch.float64 and nb == 1 and USE_SELL and all([op._count_apply32("fwd64") for op in ops]):
+        # FP64: the 32-bit-column layout with FP64 weights in its slot order
+        slice_ptr, sell_row, sell_col = vl.sell()
+        SW = [op.sell_weights64() for op in ops]
+        ctx.call("dcpse_apply_sell_f64", n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(slice_ptr), _lib.ptr(sell_row),
+                 _lib.ptr(sell_col), len(ops), _lib.ptr_array(SW), _lib.ptr(f2), _lib.ptr(fd), _lib.ptr_array(outs),
+                 alpha, beta)
         return [o.reshape(*f.shape[:-1], n_rows) for o in outs]
     W = [op.weights(f.dtype) for op in ops]
     name = "dcpse_apply_f32" if f.dtype == torch.float32 else "dcpse_apply_f64"

@@ -433,6 +433,52 @@ def test_sell_copy_built_from_the_nth_apply(cuda):
 
 
 @pytest.mark.parametrize("name", sorted(WORKLOADS))
+def test_sell_fp64_apply(name, oracle, cuda):
+    """FP64 single-field applies switch to the FP64 sliced-ELL copy at the
+    SELL_AFTER-th apply since assembly: fused Laplacian + Dx against the oracle
+    (1e-10 relative, the FP64 apply bar) and against the CSR kernel (FP64 sums
+    in another order: 1e-13 of the largest value)."""
+    from openfpm_numerics_amd import Derivative_x, Laplacian, dcpse as D, fused, workloads as W
+
+    w = WORKLOADS[name](W)
+    ps = _ps(w, cuda)
+    vl = ps.getVerlet(w["rcut"])
+    rp, col = vl.row_ptr.cpu().numpy(), vl.col.cpu().numpy()
+    dim = ps.dims
+    lap = Laplacian(ps, vl, 2, w["rcut"])
+    dx = Derivative_x(ps, vl, 2, w["rcut"])
+    refs = [oracle.assemble(w["pos"], rp, col, tuple(2 if k == d else 0 for k in range(dim)), 2,
+                            w["box_lo"], w["box_hi"], w["periodic"]) for d in range(dim)]
+    ref_dx = oracle.assemble(w["pos"], rp, col, (1,) + (0,) * (dim - 1), 2, w["box_lo"], w["box_hi"],
+                             w["periodic"])
+    f64 = np.random.default_rng(4).standard_normal(ps.size_local())
+    want = [sum(oracle.apply(r, rp, col, f64) for r in refs), oracle.apply(ref_dx, rp, col, f64)]
+    ft = torch.as_tensor(f64, device=cuda)
+    csr = fused(lap, dx, f=ft)  # first apply since assembly: the CSR kernel
+    assert "sell64" not in lap.dcpse._W and "sell64" not in dx.dcpse._W
+    sell_after = D.SELL_AFTER
+    D.SELL_AFTER = 2
+    try:
+        got = fused(lap, dx, f=ft)
+    finally:
+        D.SELL_AFTER = sell_after
+    assert lap.dcpse._W.get("sell64") is not None and dx.dcpse._W.get("sell64") is not None
+    for g, c, wv in zip(got, csr, want):
+        gn = g.cpu().numpy()
+        assert np.linalg.norm(gn - wv) / np.linalg.norm(wv) <= 1e-10
+        assert float((g - c).abs().max()) <= 1e-13 * float(c.abs().max())
+    # alpha / beta epilogue: out = 0.5 * A f + 2 * out
+    out = [c.clone() for c in csr]
+    D.SELL_AFTER = 1
+    try:
+        D.apply_ops([lap.dcpse, dx.dcpse], ft, outs=out, alpha=0.5, beta=2.0)
+    finally:
+        D.SELL_AFTER = sell_after
+    for o, g, c in zip(out, got, csr):
+        assert float((o - (0.5 * g + 2.0 * c)).abs().max()) <= 1e-13 * float(c.abs().max())
+
+
+@pytest.mark.parametrize("name", sorted(WORKLOADS))
 def test_sell_bundle_equals_sell(name, cuda):
     """The interleaved bundle layout reads the same slots in the same order:
     bitwise the same result as the separate-array SELL layout."""
