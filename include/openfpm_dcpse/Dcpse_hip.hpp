@@ -114,8 +114,16 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
                                     nullptr, n, O, n, alpha, beta),
                     "dcpse_apply_f32");
         } else {
-            const double* W[1] = {W64_.data()};
             double* O[1] = {out};
+            if (n > 0 && !s64_tried_ && ++n_apply64_ >= kSellAfter) build_sell64();
+            if (s64_) {
+                const double* SW[1] = {sellW64_.data()};
+                c.check(dcpse_apply_sell_f64(c.get(), n, verletList_.devRowPtr(), slice64_.data(), row64_.data(),
+                                             col64_.data(), 1, SW, f, nullptr, O, alpha, beta),
+                        "dcpse_apply_sell_f64");
+                return;
+            }
+            const double* W[1] = {W64_.data()};
             c.check(dcpse_apply_f64(c.get(), n, verletList_.devRowPtr(), verletList_.devCol(), 1, W, f, n, 1,
                                     nullptr, n, O, n, alpha, beta),
                     "dcpse_apply_f64");
@@ -281,6 +289,8 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
         W32_.upload(W32);
         b16_ = b16_tried_ = false;
         n_apply32_ = 0;
+        s64_ = s64_tried_ = false;
+        n_apply64_ = 0;
         mirrored_ = true;
         ctr_ = particles_.getMapCtr();
     }
@@ -317,6 +327,31 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
             return;
         n_exc_ = n_exc;
         b16_ = true;
+    }
+
+    // FP64 applies run on a sliced-ELL copy with 32-bit columns and FP64
+    // weights (dcpse_sell_count + dcpse_sell_fill_cols/fill_weights_f64),
+    // built at the kSellAfter-th FP64 apply after each assembly or load; where
+    // it cannot be built the CSR kernel runs instead.
+    bool sell64_built() const { return s64_; }
+    void build_sell64() {
+        auto& c = openfpm_dcpse::context();
+        s64_tried_ = true;
+        s64_ = false;
+        const int64_t n = (int64_t)verletList_.size();
+        const int64_t n_lanes = 64 * ((n + 63) / 64);
+        slice64_.resize(n_lanes / 64 + 1);
+        row64_.resize(n_lanes);
+        int64_t n_slots = 0;
+        if (dcpse_sell_count(c.get(), n, verletList_.devRowPtr(), slice64_.data(), row64_.data(), &n_slots)) return;
+        col64_.resize((size_t)std::max<int64_t>(n_slots, 4));
+        sellW64_.resize((size_t)std::max<int64_t>(n_slots, 4));
+        if (dcpse_sell_fill_cols(c.get(), n, verletList_.devRowPtr(), verletList_.devCol(), slice64_.data(),
+                                 row64_.data(), col64_.data()) ||
+            dcpse_sell_fill_weights_f64(c.get(), n, verletList_.devRowPtr(), W64_.data(), slice64_.data(),
+                                        row64_.data(), sellW64_.data()))
+            return;
+        s64_ = true;
     }
 
    private:
@@ -364,6 +399,8 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
         mirrored_ = false;
         b16_ = b16_tried_ = false;
         n_apply32_ = 0;
+        s64_ = s64_tried_ = false;
+        n_apply64_ = 0;
         ctr_ = particles_.getMapCtr();
     }
     void mirror() {
@@ -399,6 +436,12 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
     int64_t n_exc_ = 0;
     bool b16_ = false, b16_tried_ = false;
     int n_apply32_ = 0;  // FP32 applies since assembly
+    // the sliced-ELL copy used by FP64 applies
+    openfpm_dcpse::device_array<int64_t> slice64_;
+    openfpm_dcpse::device_array<int32_t> row64_, col64_;
+    openfpm_dcpse::device_array<double> sellW64_;
+    bool s64_ = false, s64_tried_ = false;
+    int n_apply64_ = 0;  // FP64 applies since assembly
     std::vector<double> eps_host_, eip_host_, K_host_;
     std::vector<int64_t> rp_host_;
     std::vector<int32_t> col_host_;

@@ -4,8 +4,9 @@
 //   dcpse_op_tests, dcpse_op_save_load,
 //   dcpse_op_test_lap                   src/DCPSE/DCPSE_op/tests/DCPSE_op_test_base_tests.cpp:30-175, 394-468
 // plus FP32 fields, expression aliasing, the row accessors (value_nz
-// contract) checked against the apply, and checkMomenta (Dcpse.hpp:381-457)
-// against the right-hand side b_alpha the kernels are built to reproduce. Prints one line per case and exits
+// contract) checked against the apply, checkMomenta (Dcpse.hpp:381-457)
+// against the right-hand side b_alpha the kernels are built to reproduce, and
+// FP64 applies on the sliced-ELL copy against the CSR kernel. Prints one line per case and exits
 // non-zero if any fails.
 #include <cmath>
 #include <cstdio>
@@ -409,6 +410,47 @@ static bool dcpse_check_momenta() {
     return ok;
 }
 
+// FP64 applies: the CSR kernel until the kSellAfter-th apply since assembly,
+// then the FP64 sliced-ELL copy; every apply agrees with the first to FP64
+// rounding of the row sums (another summation order).
+static bool dcpse_fp64_sell() {
+    const size_t sz[2] = {40, 40};
+    Box<2, double> box({0, 0}, {1, 1});
+    size_t bc[2] = {PERIODIC, PERIODIC};
+    const double h = 1.0 / sz[0];
+    const double rCut = 3.1 * h;
+    vector_dist_hip<2, double, 3> domain(0, box, bc, Ghost<2, double>(rCut));
+    auto it = domain.getGridIterator(sz);
+    while (it.isNext()) {
+        auto key = it.get();
+        domain.add();
+        const double x = key.get(0) * h, y = key.get(1) * h;
+        domain.getLastPos()[0] = x;
+        domain.getLastPos()[1] = y;
+        domain.getLastProp<0>() = std::sin(2 * M_PI * x) * std::cos(4 * M_PI * y);
+        ++it;
+    }
+    auto vl = domain.getVerlet<VL_NON_SYMMETRIC | VL_SKIP_REF_PART>(rCut);
+    Laplacian<decltype(vl)> Lap(domain, vl, 2, rCut);
+    getV<1>(domain) = Lap(getV<0>(domain));
+    std::vector<double> first(domain.size_local());
+    double scale = 0;
+    for (size_t p = 0; p < first.size(); ++p) {
+        first[p] = domain.getProp<1>(p);
+        scale = std::fmax(scale, std::fabs(first[p]));
+    }
+    double worst = 0;
+    for (int r = 0; r < 2 * Dcpse_hip<2, decltype(vl), vector_dist_hip<2, double, 3>>::kSellAfter; ++r) {
+        getV<2>(domain) = Lap(getV<0>(domain));
+        for (size_t p = 0; p < first.size(); ++p)
+            worst = std::fmax(worst, std::fabs(domain.getProp<2>(p) - first[p]));
+    }
+    const bool sell = Lap.dcpse(domain).sell64_built();
+    std::printf("         FP64 sliced-ELL copy %s, worst deviation %.3e (scale %.3e)\n", sell ? "built" : "missing",
+                worst, scale);
+    return sell && worst <= 1e-13 * scale;
+}
+
 int main() {
     run("Dcpse_2D_test", dcpse_2d_test);
     run("Dcpse_3D_test", dcpse_3d_test);
@@ -419,6 +461,7 @@ int main() {
     run("dcpse_expression_aliasing", dcpse_expression_aliasing);
     run("dcpse_row_accessors", dcpse_row_accessors);
     run("dcpse_check_momenta", dcpse_check_momenta);
+    run("dcpse_fp64_sell", dcpse_fp64_sell);
     std::printf("%d failure(s)\n", failures);
     return failures ? 1 : 0;
 }
