@@ -228,6 +228,17 @@ dcpse_status dcpse_axpby_f64(dcpse_ctx* ctx, int64_t n, double alpha, const doub
  * (device) from the CSR. */
 dcpse_status dcpse_sell_count(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, int64_t* slice_ptr,
                               int32_t* sell_row, int64_t* n_slots);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Same arguments and results as dcpse_sell_count, but its
+ * scratch (per-slice widths, overflow flag, scan tile sums) is a 4 MiB
+ * static device array that each GPU holds until the process ends, so it
+ * calls no hipMalloc/hipFree. n_rows above DCPSE_NOALLOC_MAX_ROWS is rejected
+ * with DCPSE_ERR_RANGE. Every call on one GPU shares that GPU's array, so
+ * calls that overlap on one GPU (from several contexts or host threads) race. */
+#define DCPSE_NOALLOC_MAX_ROWS 33554432
+dcpse_status dcpse_sell_count_noalloc(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, int64_t* slice_ptr,
+                                      int32_t* sell_row, int64_t* n_slots);
 dcpse_status dcpse_sell_fill_cols(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int32_t* col,
                                   const int64_t* slice_ptr, const int32_t* sell_row, int32_t* sell_col);
 dcpse_status dcpse_sell_fill_weights_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const float* W,

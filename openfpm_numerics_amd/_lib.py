@@ -60,6 +60,7 @@ SIGNATURES = {
     "dcpse_axpby_f32": (_i32, [_vp, _i64, _f32, _vp, _f32, _vp]),
     "dcpse_axpby_f64": (_i32, [_vp, _i64, _f64, _vp, _f64, _vp]),
     "dcpse_sell_count": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp]),
+    "dcpse_sell_count_noalloc": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp]),
     "dcpse_sell_fill_cols": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp]),
     "dcpse_sell_fill_weights_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp]),
     "dcpse_apply_sell_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _f32, _f32]),

@@ -23,6 +23,10 @@ dcpse_status dcpse_ctx_create(dcpse_ctx** out, int32_t device) {
         ctx->num_cus = prop.multiProcessorCount;
     // the device's default (null) stream until the caller binds another one
     ctx->stream = nullptr;
+    if (noalloc_scratch_init(ctx) != DCPSE_OK) {
+        delete ctx;
+        return DCPSE_ERR_HIP;
+    }
     *out = ctx;
     return DCPSE_OK;
 }

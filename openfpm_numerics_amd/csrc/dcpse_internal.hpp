@@ -45,6 +45,7 @@ struct dcpse_ctx {
     int asm_ppb = 8;       // particles per block (per wave) batch
     void* ws = nullptr;    // grow-only device workspace
     size_t ws_size = 0;
+    void* scratch = nullptr;  // this device's static no-alloc scratch (device address)
     dcpse_support_cache sup;
     char err[1024] = {0};
 };
@@ -89,6 +90,11 @@ dcpse_status reserve_workspace(dcpse_ctx* ctx, size_t bytes, Workspace* w);
 // Exclusive scan of n int32 counts into out[0..n] (int64), on ctx->stream.
 dcpse_status scan_counts(dcpse_ctx* ctx, const int32_t* in, int64_t n, int64_t* out, Workspace* w);
 size_t scan_workspace_bytes(int64_t n);
+
+// Looks up this device's address of the static scratch array of the no-alloc
+// entry points (once per device) and stores it in ctx->scratch; the device
+// must be current.
+dcpse_status noalloc_scratch_init(dcpse_ctx* ctx);
 
 // Device fill helpers (kernels, so they can be captured in a graph).
 void fill_i32(hipStream_t s, int32_t* p, int64_t n, int32_t v);

@@ -19,6 +19,7 @@
 // footprint but hardly any HBM traffic. Padded slots hold column 0 and
 // weight 0 and are masked out by the row length.
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "dcpse_common.hpp"
@@ -806,24 +807,54 @@ static bool dispatch_sell(const SellArgs& a, int nops, int unr, int depth, hipSt
     return false;
 }
 
-}  // namespace dcpse
+// Scratch of dcpse_sell_count_noalloc: the per-slice widths, the overflow
+// flag and the scan's tile sums live in this file-scope device array, so that
+// entry point allocates nothing. Each device holds its own copy for the life
+// of the process; host code never passes the array's name to a kernel (on the
+// host it is only a stand-in for the symbol) but the device address that
+// hipGetSymbolAddress returns, looked up once per device by
+// noalloc_scratch_init (called from dcpse_ctx_create with the device current).
+// Every no-alloc call on a device shares that device's copy: calls that
+// overlap on one GPU race.
+constexpr size_t kNoAllocScratchBytes = size_t(4) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
-using namespace dcpse;
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxScratchDevices] = {};
 
-extern "C" {
+dcpse_status noalloc_scratch_init(dcpse_ctx* ctx) {
+    if (ctx->device < 0 || ctx->device >= kMaxScratchDevices)
+        return fail(ctx, DCPSE_ERR_RANGE, "device index %d beyond the no-alloc scratch table", ctx->device);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[ctx->device]) {
+        void* ptr = nullptr;
+        DCPSE_HIP_CHECK(ctx, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(ctx, DCPSE_ERR_HIP, "hipGetSymbolAddress returned a null scratch address");
+        g_scratch_ptr[ctx->device] = ptr;
+    }
+    ctx->scratch = g_scratch_ptr[ctx->device];
+    return DCPSE_OK;
+}
 
-dcpse_status dcpse_sell_count(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, int64_t* slice_ptr,
-                              int32_t* sell_row, int64_t* n_slots) {
-    if (!ctx) return DCPSE_ERR_INVALID;
-    if (!row_ptr || !slice_ptr || !sell_row || n_rows < 0)
-        return fail(ctx, DCPSE_ERR_INVALID, "null pointer or negative size");
-    DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+// Bytes the slice-structure pass takes from its scratch: the same bump
+// allocations as sell_count_impl, then the scan's share.
+static size_t sell_count_scratch_bytes(int64_t n_slices) {
+    Workspace probe;
+    probe.cap = ~size_t(0) >> 1;
+    (void)probe.take<int32_t>((size_t)n_slices + 1);
+    (void)probe.take<int32_t>(16);
+    return probe.used + 256 + scan_workspace_bytes(n_slices);
+}
+
+// Slice structure of the sliced-ELL layout with its scratch taken from w.
+static dcpse_status sell_count_impl(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, int64_t* slice_ptr,
+                                    int32_t* sell_row, int64_t* n_slots, Workspace& w) {
     const int64_t n_slices = (n_rows + 63) / 64;
-    Workspace w;
-    dcpse_status st = reserve_workspace(ctx, (size_t)(n_slices + 1) * 4 + 64 + scan_workspace_bytes(n_slices) + 1024, &w);
-    if (st) return st;
+    dcpse_status st;
     int32_t* counts = w.take<int32_t>(n_slices + 1);
     int32_t* flags = w.take<int32_t>(16);
+    if (!flags) return fail(ctx, DCPSE_ERR_INVALID, "sliced-ELL count: scratch sizing error");
     fill_i32(ctx->stream, flags, 16, 0);
     if (n_slices > 0) {
         k_sell_width<<<(unsigned)((n_slices * 64 + 255) / 256), 256, 0, ctx->stream>>>(n_rows, row_ptr, counts,
@@ -839,6 +870,45 @@ dcpse_status dcpse_sell_count(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row
     if (ovf) return fail(ctx, DCPSE_ERR_RANGE, "a row exceeds DCPSE_MAX_ROW=%d", DCPSE_MAX_ROW);
     if (n_slots) *n_slots = total;
     return DCPSE_OK;
+}
+
+}  // namespace dcpse
+
+using namespace dcpse;
+
+extern "C" {
+
+dcpse_status dcpse_sell_count(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, int64_t* slice_ptr,
+                              int32_t* sell_row, int64_t* n_slots) {
+    if (!ctx) return DCPSE_ERR_INVALID;
+    if (!row_ptr || !slice_ptr || !sell_row || n_rows < 0)
+        return fail(ctx, DCPSE_ERR_INVALID, "null pointer or negative size");
+    DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    const int64_t n_slices = (n_rows + 63) / 64;
+    Workspace w;
+    dcpse_status st = reserve_workspace(ctx, sell_count_scratch_bytes(n_slices) + 1024, &w);
+    if (st) return st;
+    return sell_count_impl(ctx, n_rows, row_ptr, slice_ptr, sell_row, n_slots, w);
+}
+
+dcpse_status dcpse_sell_count_noalloc(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, int64_t* slice_ptr,
+                                      int32_t* sell_row, int64_t* n_slots) {
+    if (!ctx) return DCPSE_ERR_INVALID;
+    if (!row_ptr || !slice_ptr || !sell_row || n_rows < 0)
+        return fail(ctx, DCPSE_ERR_INVALID, "null pointer or negative size");
+    if (n_rows > DCPSE_NOALLOC_MAX_ROWS)
+        return fail(ctx, DCPSE_ERR_RANGE, "dcpse_sell_count_noalloc: %lld rows exceed DCPSE_NOALLOC_MAX_ROWS=%d",
+                    (long long)n_rows, DCPSE_NOALLOC_MAX_ROWS);
+    if (!ctx->scratch) return fail(ctx, DCPSE_ERR_HIP, "no-alloc scratch address was not set up for this device");
+    DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    const int64_t n_slices = (n_rows + 63) / 64;
+    if (sell_count_scratch_bytes(n_slices) > kNoAllocScratchBytes)
+        return fail(ctx, DCPSE_ERR_RANGE, "dcpse_sell_count_noalloc: scratch of %zu bytes too small",
+                    kNoAllocScratchBytes);
+    Workspace w;
+    w.base = static_cast<char*>(ctx->scratch);
+    w.cap = kNoAllocScratchBytes;
+    return sell_count_impl(ctx, n_rows, row_ptr, slice_ptr, sell_row, n_slots, w);
 }
 
 dcpse_status dcpse_sell_fill_cols(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int32_t* col,
