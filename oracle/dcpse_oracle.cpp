@@ -252,13 +252,18 @@ void orc_vandermonde_row(int dim, int m, const unsigned* exps, const double* x, 
     for (int j = 0; j < m; ++j) row[j] = mono_eval(dim, exps + j * dim, 1.0, x);
 }
 
-// Neighbour support: q != p (same-set) and |x_p - x_q| < rcut_p. A uniform
-// cell grid of edge >= max rcut; rows are sorted by ascending q.
+// Neighbour support: |x_p - x_q| < rcut_p, rows sorted by ascending q, over
+// a uniform cell grid of edge >= max rcut. The reference particle is skipped
+// (VL_SKIP_REF_PART): q != p when same_set, q != dom_self[p] when dom_self is
+// given (a domain that is a subset of the support set, dom_self[p] its index
+// there, as vector_dist_subset's getVerlet builds it,
+// src/DCPSE/DCPSE_op/tests/DCPSE_op_subset_test.cpp:138); a separate domain
+// set without dom_self skips nothing.
 // rcut has 1 entry (uniform) or ndom entries (per_particle != 0).
 // Pass 1 (col == nullptr) writes row_ptr[0..ndom] and returns nnz;
 // pass 2 (col != nullptr) fills col using the row_ptr from pass 1.
 int64_t orc_support(int dim, const double* pos_sup, int64_t nsup,
-                    const double* pos_dom, int64_t ndom, int same_set,
+                    const double* pos_dom, int64_t ndom, int same_set, const int32_t* dom_self,
                     const double* rcut, int per_particle,
                     const double* box_lo, const double* box_hi, unsigned periodic,
                     int64_t* row_ptr, int32_t* col, int nthreads) {
@@ -313,6 +318,7 @@ int64_t orc_support(int dim, const double* pos_sup, int64_t nsup,
             cell_coord(pos_dom, ndom, p, c);
             double r = per_particle ? rcut[p] : rcut[0];
             double r2 = r * r;
+            const int64_t self = same_set ? p : (dom_self ? (int64_t)dom_self[p] : -1);
             row.clear();
             int lo[kMaxDim], hi[kMaxDim];
             for (int d = 0; d < kMaxDim; ++d) {
@@ -335,7 +341,7 @@ int64_t orc_support(int dim, const double* pos_sup, int64_t nsup,
                         int64_t cl = cc[0] + (int64_t)nc[0] * (cc[1] + (int64_t)nc[1] * cc[2]);
                         for (int64_t k = cstart[cl]; k < cstart[cl + 1]; ++k) {
                             int32_t q = cpart[k];
-                            if (same_set && q == p) continue;
+                            if (q == self) continue;
                             double dv[kMaxDim];
                             displacement(dim, pos_dom, ndom, p, pos_sup, nsup, q, box_lo, box_hi,
                                          periodic, dv);
@@ -495,23 +501,6 @@ void orc_apply(int64_t ndom, const int64_t* row_ptr, const int32_t* col, const d
         double acc = 0.0;
         for (int64_t j = row_ptr[p]; j < row_ptr[p + 1]; ++j) acc += (f_sup[col[j]] + fxp) * K[j];
         out[p] = acc * eps_inv_pow[p];
-    }
-}
-
-// Same gather in single precision over pre-scaled FP32 weights W = eps^-|b| K,
-// difference form sum_q W_pq (f_q - f_p); the CPU counterpart of the device
-// FP32 apply, used only as a timed baseline.
-void orc_apply_f32(int64_t ndom, const int64_t* row_ptr, const int32_t* col, const float* W,
-                   const float* f, float* out, int nthreads) {
-#ifdef _OPENMP
-    if (nthreads > 0) omp_set_num_threads(nthreads);
-#endif
-#pragma omp parallel for schedule(static, 1024)
-    for (int64_t p = 0; p < ndom; ++p) {
-        float fp = f[p];
-        float acc = 0.0f;
-        for (int64_t j = row_ptr[p]; j < row_ptr[p + 1]; ++j) acc += W[j] * (f[col[j]] - fp);
-        out[p] = acc;
     }
 }
 

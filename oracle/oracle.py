@@ -20,7 +20,6 @@ _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
-_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 
 
 def build(force: bool = False) -> str:
@@ -51,7 +50,7 @@ def lib():
         L.orc_rhs.restype = C.c_int
         L.orc_vandermonde_row.argtypes = [C.c_int, C.c_int, _u32p, _f64p, _f64p]
         L.orc_vandermonde_row.restype = None
-        L.orc_support.argtypes = [C.c_int, _f64p, C.c_int64, _f64p, C.c_int64, C.c_int,
+        L.orc_support.argtypes = [C.c_int, _f64p, C.c_int64, _f64p, C.c_int64, C.c_int, C.c_void_p,
                                   _f64p, C.c_int, _f64p, _f64p, C.c_uint,
                                   _i64p, C.c_void_p, C.c_int]
         L.orc_support.restype = C.c_int64
@@ -64,8 +63,6 @@ def lib():
         L.orc_apply.argtypes = [C.c_int64, _i64p, _i32p, _f64p, _f64p, C.c_uint,
                                 _f64p, _f64p, _f64p, C.c_int]
         L.orc_apply.restype = None
-        L.orc_apply_f32.argtypes = [C.c_int64, _i64p, _i32p, _f32p, _f32p, _f32p, C.c_int]
-        L.orc_apply_f32.restype = None
         L.orc_value_nz.argtypes = [C.c_int64, _i64p, _i32p, _f64p, _f64p, C.c_uint,
                                    _i64p, _i64p, _f64p]
         L.orc_value_nz.restype = None
@@ -124,12 +121,21 @@ def _soa(pos):
     return np.ascontiguousarray(pos.T).ravel()
 
 
-def support(pos, rcut, box_lo, box_hi, periodic=0, pos_dom=None, nthreads=0):
+def support(pos, rcut, box_lo, box_hi, periodic=0, pos_dom=None, nthreads=0, dom_ids=None):
     """Neighbour CSR (row_ptr int64[Nd+1], col int32[nnz]); rows ascending.
 
-    ``rcut`` is a scalar or a per-domain-particle array.
+    ``rcut`` is a scalar or a per-domain-particle array. ``pos_dom``: a
+    separate domain set (nothing skipped); ``dom_ids``: a domain that is the
+    subset ``pos[dom_ids]`` of the support set, each row skipping its own
+    particle.
     """
     pos = np.asarray(pos, dtype=np.float64)
+    self_ids = None
+    if dom_ids is not None:
+        if pos_dom is not None:
+            raise ValueError("give pos_dom or dom_ids, not both")
+        self_ids = np.ascontiguousarray(dom_ids, dtype=np.int32)
+        pos_dom = pos[self_ids]
     same = pos_dom is None
     pdom = pos if same else np.asarray(pos_dom, dtype=np.float64)
     dim = pos.shape[1]
@@ -140,12 +146,13 @@ def support(pos, rcut, box_lo, box_hi, periodic=0, pos_dom=None, nthreads=0):
     hi = np.ascontiguousarray(box_hi, dtype=np.float64)
     nd = pdom.shape[0]
     row_ptr = np.zeros(nd + 1, np.int64)
-    nnz = lib().orc_support(dim, sup, pos.shape[0], dom, nd, int(same), rc, per, lo, hi,
+    sp = None if self_ids is None else self_ids.ctypes.data_as(C.c_void_p)
+    nnz = lib().orc_support(dim, sup, pos.shape[0], dom, nd, int(same), sp, rc, per, lo, hi,
                             int(periodic), row_ptr, None, nthreads)
     if nnz < 0:
         raise ValueError("periodic dimension has fewer than 3 cells; rCut too large for the box")
     col = np.zeros(max(nnz, 1), np.int32)
-    lib().orc_support(dim, sup, pos.shape[0], dom, nd, int(same), rc, per, lo, hi,
+    lib().orc_support(dim, sup, pos.shape[0], dom, nd, int(same), sp, rc, per, lo, hi,
                       int(periodic), row_ptr, col.ctypes.data_as(C.c_void_p), nthreads)
     return row_ptr, col[:nnz]
 
@@ -192,15 +199,6 @@ def apply(asm: Assembly, row_ptr, col, f, f_dom=None, nthreads=0):
                     np.ascontiguousarray(col, dtype=np.int32) if nnz else np.zeros(1, np.int32),
                     np.ascontiguousarray(asm.K) if nnz else np.zeros(1),
                     asm.eps_inv_pow, asm.diff_order, f, fd, out, nthreads)
-    return out
-
-
-def apply_f32(row_ptr, col, W32, f32, nthreads=0):
-    f32 = np.ascontiguousarray(f32, dtype=np.float32)
-    out = np.zeros(f32.size, np.float32)
-    lib().orc_apply_f32(f32.size, np.ascontiguousarray(row_ptr, dtype=np.int64),
-                        np.ascontiguousarray(col, dtype=np.int32),
-                        np.ascontiguousarray(W32, dtype=np.float32), f32, out, nthreads)
     return out
 
 

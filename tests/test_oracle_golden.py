@@ -298,3 +298,116 @@ def test_too_few_neighbours_reported(oracle):
     rp, col = oracle.support(pos, 0.05, [0, 0], [1, 1], 0)
     a = oracle.assemble(pos, rp, col, (1, 0), 2, [0, 0], [1, 1], 0)
     assert list(a.info) == [1, 1, 1]
+
+
+# ---- two particle sets: support != domain (Dcpse.hpp:111-130) ---------------------------------
+def _brute_rows(pos_sup, pos_dom, rc, periodic, skip=None):
+    dim = pos_sup.shape[1]
+    d = pos_dom[:, None, :] - pos_sup[None, :, :]
+    for k in range(dim):
+        if periodic >> k & 1:
+            d[..., k] -= np.round(d[..., k])
+    d2 = (d ** 2).sum(-1)
+    if skip is not None:
+        d2[np.arange(pos_dom.shape[0]), skip] = np.inf
+    r2 = np.broadcast_to(np.asarray(rc, dtype=np.float64) ** 2, (pos_dom.shape[0],))
+    return [np.nonzero(d2[p] < r2[p])[0] for p in range(pos_dom.shape[0])]
+
+
+@pytest.mark.parametrize("dim,periodic,per_particle", [(2, 3, False), (2, 0, True), (3, 7, True), (3, 2, False)])
+def test_support_two_sets_brute_force(oracle, dim, periodic, per_particle):
+    """A domain set distinct from the support set: nothing is skipped, so a
+    domain point on top of a support particle lists it (distance 0), and
+    cells come from the domain positions (some domain points outside the
+    open box)."""
+    rng = np.random.default_rng(60 + dim + periodic)
+    ns, nd = (700, 500) if dim == 2 else (900, 600)
+    sup = rng.random((ns, dim))
+    dom = rng.random((nd, dim)) * 1.1 - 0.05
+    if periodic:
+        dom = np.mod(dom, 1.0)
+    dom[:40] = sup[100:140]  # coincident points
+    base = 0.09 if dim == 2 else 0.17
+    rc = base * (0.6 + 0.8 * rng.random(nd)) if per_particle else base
+    rp, col = oracle.support(sup, rc, [0] * dim, [1] * dim, periodic, pos_dom=dom)
+    want = _brute_rows(sup, dom, rc, periodic)
+    assert rp[-1] == sum(len(w) for w in want)
+    for p in range(nd):
+        assert np.array_equal(col[rp[p]:rp[p + 1]], want[p])
+    assert all(100 + p in col[rp[p]:rp[p + 1]] for p in range(40))
+
+
+@pytest.mark.parametrize("dim,periodic", [(2, 3), (3, 0)])
+def test_support_subset_brute_force(oracle, dim, periodic):
+    """A domain that is a subset of the support set (vector_dist_subset,
+    DCPSE_op_subset_test.cpp:113-141): each row skips its own particle and
+    nothing else."""
+    rng = np.random.default_rng(80 + dim)
+    n = 800 if dim == 2 else 900
+    pos = rng.random((n, dim))
+    ids = np.sort(rng.choice(n, size=n // 2, replace=False)).astype(np.int32)
+    rc = 0.09 if dim == 2 else 0.17
+    rp, col = oracle.support(pos, rc, [0] * dim, [1] * dim, periodic, dom_ids=ids)
+    want = _brute_rows(pos, pos[ids], rc, periodic, skip=ids)
+    for p in range(ids.size):
+        assert np.array_equal(col[rp[p]:rp[p + 1]], want[p])
+    # the subset's rows are exactly those rows of the one-set list
+    rpf, colf = oracle.support(pos, rc, [0] * dim, [1] * dim, periodic)
+    for p, i in enumerate(ids):
+        assert np.array_equal(col[rp[p]:rp[p + 1]], colf[rpf[i]:rpf[i + 1]])
+
+
+@pytest.mark.parametrize("mode,beta", [("separate", (1, 0)), ("separate", (0, 2)), ("subset", (1, 1)),
+                                       ("subset", (2, 0))])
+def test_kernel_moments_two_sets(oracle, mode, beta):
+    """Two-set assembly: x_pq = x_p(domain) - x_q(support). The moment
+    conditions V^T K = b hold and K equals the SVD minimum-norm solution of
+    the weighted system, as for one set."""
+    pos, h = _jitter_cloud(2, 24, 17)
+    rng = np.random.default_rng(5)
+    rc = 3.1 * h
+    if mode == "separate":
+        dom = np.mod(pos[rng.choice(pos.shape[0], 150, replace=False)] + rng.uniform(-0.5 * h, 0.5 * h, (150, 2)),
+                     1.0)
+        rp, col = oracle.support(pos, rc, [0, 0], [1, 1], 3, pos_dom=dom)
+    else:
+        ids = np.sort(rng.choice(pos.shape[0], 150, replace=False)).astype(np.int32)
+        dom = pos[ids]
+        rp, col = oracle.support(pos, rc, [0, 0], [1, 1], 3, dom_ids=ids)
+    a = oracle.assemble(pos, rp, col, beta, 2, [0, 0], [1, 1], 3, pos_dom=dom)
+    assert (a.info == 0).all()
+    B = oracle.basis(beta, 2)
+    b = oracle.rhs(beta, 2)
+    worst_m = worst_k = 0.0
+    for p in range(dom.shape[0]):
+        q = col[rp[p]:rp[p + 1]]
+        x = _min_image(dom[p] - pos[q])
+        eps = np.abs(x).sum(1).mean() / 0.9
+        assert a.eps[p] == pytest.approx(eps, rel=1e-13)
+        z = x / eps
+        V = np.prod(z[:, None, :] ** B[None, :, :], axis=2)
+        E = np.exp(-(z ** 2).sum(1) / 2.0)
+        K = a.K[rp[p]:rp[p + 1]]
+        worst_m = max(worst_m, np.abs(V.T @ K - b).max() / np.abs(b).max())
+        u = np.linalg.pinv((E[:, None] * V).T) @ b
+        worst_k = max(worst_k, np.abs(K - E * u).max() / np.abs(E * u).max())
+    assert worst_m <= 1e-10, worst_m
+    assert worst_k <= 1e-9, worst_k
+
+
+def test_apply_two_sets_reference_form(oracle):
+    """Two-set apply: Df_p = eps_p^-|b| sum_q (f_sup[q] + s f_dom[p]) K_pq
+    (Dcpse.hpp:467-499 with the field read from the support set at q and
+    from the domain set at p); exact for polynomials of the basis."""
+    pos, h = _jitter_cloud(2, 24, 19)
+    rng = np.random.default_rng(6)
+    dom = 0.25 + 0.5 * rng.random((120, 2))
+    rp, col = oracle.support(pos, 3.1 * h, [0, 0], [1, 1], 0, pos_dom=dom)
+    fx = lambda P: P[:, 0] ** 2 + 3 * P[:, 0] * P[:, 1] - P[:, 1] ** 2  # noqa: E731
+    for beta, want in (((1, 0), 2 * dom[:, 0] + 3 * dom[:, 1]), ((0, 1), 3 * dom[:, 0] - 2 * dom[:, 1])):
+        a = oracle.assemble(pos, rp, col, beta, 2, [0, 0], [1, 1], 0, pos_dom=dom)
+        got = oracle.apply(a, rp, col, fx(pos), f_dom=fx(dom))
+        assert np.abs(got - want).max() < 1e-9
+    a = oracle.assemble(pos, rp, col, (2, 0), 2, [0, 0], [1, 1], 0, pos_dom=dom)
+    got = oracle.apply(a, rp, col, fx(pos), f_dom=fx(dom))
+    assert np.abs(got - 2.0).max() < 1e-7
