@@ -126,6 +126,26 @@ dcpse_status dcpse_support_fill(dcpse_ctx* ctx, int32_t dim,
                                 const double* rcut, int32_t rcut_per_particle,
                                 const double* box_lo, const double* box_hi, uint32_t periodic_mask,
                                 const int64_t* row_ptr, int32_t* col);
+/* A domain that is a subset of the support set: domain particle p is
+ * support particle dom_ids[p] (device int32[n_dom]) at pos_dom (device, its
+ * positions gathered from pos_sup). Row p lists every support particle
+ * q != dom_ids[p] with |x_p - x_q| < rcut_p, ascending: the rows of
+ * vector_dist_subset::getVerlet<VL_NON_SYMMETRIC|VL_SKIP_REF_PART>, which the
+ * reference's two-set operators run on
+ * (src/DCPSE/DCPSE_op/tests/DCPSE_op_subset_test.cpp:113-141,
+ * src/DCPSE/Dcpse.hpp:111-130). Same count/fill protocol as above. */
+dcpse_status dcpse_support_count_subset(dcpse_ctx* ctx, int32_t dim,
+                                        const double* pos_sup, int64_t n_sup,
+                                        const double* pos_dom, const int32_t* dom_ids, int64_t n_dom,
+                                        const double* rcut, int32_t rcut_per_particle,
+                                        const double* box_lo, const double* box_hi, uint32_t periodic_mask,
+                                        int64_t* row_ptr, int64_t* nnz);
+dcpse_status dcpse_support_fill_subset(dcpse_ctx* ctx, int32_t dim,
+                                       const double* pos_sup, int64_t n_sup,
+                                       const double* pos_dom, const int32_t* dom_ids, int64_t n_dom,
+                                       const double* rcut, int32_t rcut_per_particle,
+                                       const double* box_lo, const double* box_hi, uint32_t periodic_mask,
+                                       const int64_t* row_ptr, int32_t* col);
 
 /* ---- kernel assembly (FP64) ------------------------------------------- */
 /* Assembles sum_k coeffs[k] * D^{beta_k} for n_sig signatures that share one

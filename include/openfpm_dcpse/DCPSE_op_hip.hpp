@@ -20,6 +20,7 @@
 #pragma once
 
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "Dcpse_hip.hpp"
@@ -187,24 +188,114 @@ field_expr<V> operator-(const field_ref<p1, V>& a, const field_ref<p2, V>& b) {
 }
 
 // ---- operator classes ------------------------------------------------------------------------
-// Owns one Dcpse_type instance over particle set type V; forwards the
-// lifecycle calls of the reference operator classes.
+// Field operator over a Dcpse_type that offers only the reference Dcpse
+// contract (per-particle computeDifferentialOperator(key, o1)): the whole
+// field is evaluated particle by particle on the host, as the reference's
+// expression loop does (vector_dist_operators.hpp:1113-1138 over
+// VECT_DCPSE / VECT_DCPSE_V_SUM nodes, DCPSE_op.hpp:29-106, 364-442), summing
+// the terms' results (the Laplacian's dim operators).
+template <typename V, typename D>
+class keyed_operator : public field_operator<typename V::prop_type> {
+    typedef typename V::prop_type P;
+
+   public:
+    struct host_field {
+        const std::vector<P>* f;
+        V* parts;
+        P value(const vect_dist_key_dx& k) const { return (*f)[k.getKey()]; }
+        V& getVector() const { return *parts; }
+    };
+    keyed_operator(V& parts, std::vector<D*> terms) : parts_(parts), terms_(std::move(terms)) {}
+    void apply(const P* f, P* out, P alpha, P beta) override {
+        const size_t n = parts_.size_local();
+        std::vector<P> fh(n), oh(n, P(0));
+        if (n) hip_check(hipMemcpy(fh.data(), f, n * sizeof(P), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        if (n && beta != P(0)) hip_check(hipMemcpy(oh.data(), out, n * sizeof(P), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        host_field o1{&fh, &parts_};
+        for (size_t p = 0; p < n; ++p) {
+            P acc = P(0);
+            for (D* t : terms_) acc += (P)t->computeDifferentialOperator(vect_dist_key_dx(p), o1);
+            oh[p] = alpha * acc + (beta != P(0) ? beta * oh[p] : P(0));
+        }
+        if (n) hip_check(hipMemcpy(out, oh.data(), n * sizeof(P), hipMemcpyHostToDevice), "hipMemcpy H2D");
+    }
+
+   private:
+    V& parts_;
+    std::vector<D*> terms_;
+};
+
+// Sum of device field operators (a Laplacian of dim separate Dcpse_type
+// objects that do apply whole fields).
+template <typename P>
+class summed_operator : public field_operator<P> {
+   public:
+    explicit summed_operator(std::vector<field_operator<P>*> terms) : terms_(std::move(terms)) {}
+    void apply(const P* f, P* out, P alpha, P beta) override {
+        bool first = true;
+        for (auto* t : terms_) {
+            t->apply(f, out, alpha, first ? beta : P(1));
+            first = false;
+        }
+    }
+
+   private:
+    std::vector<field_operator<P>*> terms_;
+};
+
+// Owns the Dcpse_type instance(s) over particle set type V and the field
+// operator that expressions apply; forwards the lifecycle calls of the
+// reference operator classes.
 template <typename VerletList_type, template <unsigned int, typename, typename...> class Dcpse_type>
 class dcpse_operator_base {
    protected:
-    std::shared_ptr<void> dcpse_;
+    std::vector<std::shared_ptr<void>> dcpse_;  // one Dcpse_type per term
+    std::shared_ptr<void> fop_;                 // field_operator<P> the expressions apply
 
     template <typename V>
     using dcpse_t = Dcpse_type<V::dims, VerletList_type, V>;
 
     template <typename V>
-    dcpse_t<V>* get() const {
-        return static_cast<dcpse_t<V>*>(dcpse_.get());
+    dcpse_t<V>* get(size_t i = 0) const {
+        return static_cast<dcpse_t<V>*>(dcpse_.at(i).get());
+    }
+    template <typename D, typename... Args>
+    D* add_term(Args&&... args) {
+        D* d = new D(std::forward<Args>(args)...);
+        dcpse_.emplace_back(d, [](void* p) { delete static_cast<D*>(p); });
+        return d;
+    }
+    // the field operator over the terms: the Dcpse_type itself when it applies
+    // whole fields on the device, else the per-particle host evaluation
+    template <typename V, typename D>
+    void bind(V& parts) {
+        typedef typename V::prop_type P;
+        std::vector<D*> terms;
+        for (auto& d : dcpse_) terms.push_back(static_cast<D*>(d.get()));
+        if constexpr (std::is_base_of<field_operator<P>, D>::value) {
+            if (terms.size() == 1) {
+                fop_ = std::shared_ptr<void>(dcpse_[0], static_cast<field_operator<P>*>(terms[0]));
+            } else {
+                std::vector<field_operator<P>*> fo(terms.begin(), terms.end());
+                auto* so = new summed_operator<P>(fo);
+                fop_ = std::shared_ptr<void>(so, [](void* p) { delete static_cast<summed_operator<P>*>(p); });
+            }
+        } else {
+            auto* ko = new keyed_operator<V, D>(parts, terms);
+            fop_ = std::shared_ptr<void>(ko, [](void* p) { delete static_cast<keyed_operator<V, D>*>(p); });
+        }
     }
     template <typename V, typename... Args>
-    void make(Args&&... args) {
-        dcpse_ = std::shared_ptr<void>(new dcpse_t<V>(std::forward<Args>(args)...),
-                                       [](void* p) { delete static_cast<dcpse_t<V>*>(p); });
+    void make(V& parts, Args&&... args) {
+        add_term<dcpse_t<V>>(parts, std::forward<Args>(args)...);
+        bind<V, dcpse_t<V>>(parts);
+    }
+    // two sets: support parts, domain subset dom
+    template <typename V, typename S, typename... Args>
+    void make2(V& parts, S& dom, Args&&... args) {
+        typedef Dcpse_type<V::dims, VerletList_type, V, S> D;
+        add_term<D>(parts, dom, std::forward<Args>(args)...);
+        bind<V, D>(parts);
     }
 
    public:
@@ -212,12 +303,12 @@ class dcpse_operator_base {
     field_expr<V> operator()(const field_ref<prp, V>& f) const {
         field_expr<V> e;
         e.parts = &f.parts;
-        e.ops.push_back({get<V>(), prp, 1.0});
+        e.ops.push_back({static_cast<field_operator<typename V::prop_type>*>(fop_.get()), prp, 1.0});
         return e;
     }
     template <typename V>
     void update(V& parts) {
-        get<V>()->initializeUpdate(parts);
+        for (size_t i = 0; i < dcpse_.size(); ++i) get<V>(i)->initializeUpdate(parts);
     }
     template <typename V>
     auto checkMomenta(V& parts) {
@@ -241,30 +332,49 @@ class dcpse_operator_base {
     }
     template <typename V>
     void deallocate(V&) {
-        dcpse_.reset();
+        fop_.reset();
+        dcpse_.clear();
     }
-    // the underlying Dcpse_type object (accessors getNumNN, getCoeffNN, ...)
+    // the underlying Dcpse_type object (accessors getNumNN, getCoeffNN, ...);
+    // i selects a term of a Laplacian held as dim operators
     template <typename V>
-    dcpse_t<V>& dcpse(V&) {
-        return *get<V>();
+    dcpse_t<V>& dcpse(V&, size_t i = 0) {
+        return *get<V>(i);
     }
+    // the Dcpse_type of a two-set operator
+    template <typename V, typename S>
+    Dcpse_type<V::dims, VerletList_type, V, S>& dcpse(V&, S&) {
+        return *static_cast<Dcpse_type<V::dims, VerletList_type, V, S>*>(dcpse_.at(0).get());
+    }
+    size_t terms() const { return dcpse_.size(); }
 };
 
 // One operator of a fixed signature (unit exponent vectors and their pairs).
 template <typename VerletList_type, template <unsigned int, typename, typename...> class Dcpse_type, unsigned int a0,
           unsigned int a1, unsigned int a2>
 class dcpse_signature_operator : public dcpse_operator_base<VerletList_type, Dcpse_type> {
+    template <unsigned int dims>
+    static Point<dims, unsigned int> signature() {
+        static_assert(dims == 3 || a2 == 0, "z derivative of a 2D particle set");
+        Point<dims, unsigned int> p;
+        p.zero();
+        p.get(0) = a0;
+        p.get(1) = a1;
+        if constexpr (dims == 3) p.get(2) = a2;
+        return p;
+    }
+
    public:
     template <typename V>
     dcpse_signature_operator(V& parts, VerletList_type& verletList, unsigned int ord, typename V::stype rCut,
                              support_options opt = support_options::RADIUS) {
-        static_assert(V::dims == 3 || a2 == 0, "z derivative of a 2D particle set");
-        Point<V::dims, unsigned int> p;
-        p.zero();
-        p.get(0) = a0;
-        p.get(1) = a1;
-        if constexpr (V::dims == 3) p.get(2) = a2;
-        this->template make<V>(parts, verletList, p, ord, rCut, opt);
+        this->template make<V>(parts, verletList, signature<V::dims>(), ord, rCut, opt);
+    }
+    // support set parts, domain subset dom (DCPSE_op.hpp:693-718)
+    template <typename V, typename S, typename = typename S::parent_t>
+    dcpse_signature_operator(V& parts, S& dom, VerletList_type& verletList, unsigned int ord, typename V::stype rCut,
+                             support_options opt = support_options::RADIUS) {
+        this->template make2<V, S>(parts, dom, verletList, signature<V::dims>(), ord, rCut, opt);
     }
 };
 
@@ -298,17 +408,38 @@ class Derivative_G_T : public dcpse_operator_base<VerletList_type, Dcpse_type> {
     }
 };
 
-// Laplacian as one operator of signature sum_d 2 e_d.
+// Laplacian. With a Dcpse_type that takes a combination of signatures
+// (Dcpse_hip) it is one operator of signature sum_d 2 e_d; with one that
+// offers only the reference constructor it is dim operators of signature
+// 2 e_d whose results are summed, as the reference builds it
+// (DCPSE_op.hpp:1369-1389, VECT_DCPSE_V_SUM :364-442).
 template <typename VerletList_type, template <unsigned int, typename, typename...> class Dcpse_type = Dcpse_hip>
 class Laplacian_T : public dcpse_operator_base<VerletList_type, Dcpse_type> {
+    template <typename V>
+    static constexpr bool fused_ctor =
+        std::is_constructible<Dcpse_type<V::dims, VerletList_type, V>, V&, VerletList_type&,
+                              const std::vector<uint32_t>&, const std::vector<double>&, unsigned int,
+                              typename V::stype, support_options>::value;
+
    public:
     template <typename V>
     Laplacian_T(V& parts, VerletList_type& verletList, unsigned int ord, typename V::stype rCut,
                 support_options opt = support_options::RADIUS) {
-        std::vector<uint32_t> sigs(V::dims * V::dims, 0u);
-        for (unsigned d = 0; d < V::dims; ++d) sigs[d * V::dims + d] = 2;
-        std::vector<double> coeffs(V::dims, 1.0);
-        this->template make<V>(parts, verletList, sigs, coeffs, ord, rCut, opt);
+        typedef Dcpse_type<V::dims, VerletList_type, V> D;
+        if constexpr (fused_ctor<V>) {
+            std::vector<uint32_t> sigs(V::dims * V::dims, 0u);
+            for (unsigned d = 0; d < V::dims; ++d) sigs[d * V::dims + d] = 2;
+            std::vector<double> coeffs(V::dims, 1.0);
+            this->template make<V>(parts, verletList, sigs, coeffs, ord, rCut, opt);
+        } else {
+            for (unsigned i = 0; i < V::dims; ++i) {
+                Point<V::dims, unsigned int> p;
+                p.zero();
+                p.get(i) = 2;
+                this->template add_term<D>(parts, verletList, p, ord, rCut, opt);
+            }
+            this->template bind<V, D>(parts);
+        }
     }
 };
 

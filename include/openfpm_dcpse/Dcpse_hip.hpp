@@ -6,9 +6,11 @@
 // computeDifferentialOperator<f, Df> (:467-499), the row accessors getNumNN /
 // getCoeffNN / getIndexNN / getSign / getEpsilonInvPrefactor (:507-550),
 // initializeUpdate (:661-687), save / load (:302-379), checkMomenta (:381-457)
-// and DrawKernel (:151-193). The per-particle computeDifferentialOperator(key, expr) used by
-// the reference's host expression loop is replaced by whole-field device
-// applies (see DCPSE_op.hpp in this directory).
+// and DrawKernel (:151-193). The per-particle computeDifferentialOperator(key, o1) and
+// (key, o1, i) (:560-659) that the reference's expression nodes call
+// (VECT_DCPSE, DCPSE_op.hpp:64-67) are served from a host copy of the
+// kernels; the operator classes of DCPSE_op_hip.hpp evaluate whole fields
+// with device applies instead.
 #pragma once
 
 #include <algorithm>
@@ -18,6 +20,7 @@
 #include <fstream>
 #include <iostream>
 #include <string>
+#include <tuple>
 #include <type_traits>
 #include <utility>
 #include <vector>
@@ -41,19 +44,25 @@ constexpr uint32_t kFileVersion = 1;
 }  // namespace openfpm_dcpse
 
 // The trailing pack matches the reference plug point's
-// template<unsigned int, typename, typename...> signature; a second particle
-// set (support != domain) is not supported by this container.
+// template<unsigned int, typename, typename...> signature; its one optional
+// entry is the domain set type of a two-set operator (a
+// vector_dist_subset_hip of the support set), as the reference's
+// Dcpse<dim, VerletList, vector_type, vector_type2 = vector_type>.
 template <unsigned int dim, typename VerletList_type, typename vector_type, typename... vector_type2>
 class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::prop_type> {
-    static_assert(sizeof...(vector_type2) == 0, "separate support and domain particle sets are not supported");
+    static_assert(sizeof...(vector_type2) <= 1, "one domain set at most");
 
    public:
     typedef typename vector_type::stype T;
     typedef typename vector_type::prop_type prop_type;
     typedef vector_type vtype;
-    // FP32 applies run the CSR kernel until this many applies since assembly,
-    // the operator's 16-bit sliced-ELL copy from then on
-    static constexpr int kSellAfter = 16;
+    typedef std::tuple_element_t<0, std::tuple<vector_type2..., vector_type>> domain_type;
+    static constexpr bool two_sets = sizeof...(vector_type2) == 1;
+    // Single-field applies run the operator's sliced-ELL copy (FP32: 16-bit
+    // column offsets, FP64: 32-bit columns) from this apply since assembly
+    // on, the CSR kernel before. 1: from the first, so every apply of an
+    // operator runs one kernel and repeated applies agree bit for bit.
+    static constexpr int kSellAfter = 1;
 
     T HOverEpsilon = 0.9;
     // one step of FP64 iterative refinement of each particle's solve
@@ -64,6 +73,22 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
     Dcpse_hip(vector_type& particles, VerletList_type& verletList, PointT differentialSignature,
               unsigned int convergenceOrder, T rCut, support_options opt = support_options::RADIUS)
         : particles_(particles), verletList_(verletList), convergenceOrder_(convergenceOrder), rCut_(rCut) {
+        std::vector<uint32_t> sig(dim);
+        for (unsigned d = 0; d < dim; ++d) sig[d] = (uint32_t)differentialSignature.value(d);
+        sigs_ = sig;
+        coeffs_ = {1.0};
+        finish_init(opt);
+    }
+
+    // Two sets (Dcpse.hpp:111-130): kernels of the domain particles over
+    // neighbours in the support set; the domain is a subset of the support
+    // set (vector_dist_subset_hip) and the Verlet list its getVerlet().
+    template <typename PointT, bool B = two_sets, typename = std::enable_if_t<B>>
+    Dcpse_hip(vector_type& particlesSupport, domain_type& particlesDomain, VerletList_type& verletList,
+              PointT differentialSignature, unsigned int convergenceOrder, T rCut,
+              support_options opt = support_options::RADIUS)
+        : particles_(particlesSupport), verletList_(verletList), convergenceOrder_(convergenceOrder), rCut_(rCut),
+          domain_(&particlesDomain) {
         std::vector<uint32_t> sig(dim);
         for (unsigned d = 0; d < dim; ++d) sig[d] = (uint32_t)differentialSignature.value(d);
         sigs_ = sig;
@@ -82,24 +107,72 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
     }
 
     // ---- apply ---------------------------------------------------------------------------
+    // Df = D f over the whole set (Dcpse.hpp:467-499). With two sets, f is read
+    // from the support set and Df written at the domain's particles.
     template <unsigned int fValuePos, unsigned int DfValuePos>
     void computeDifferentialOperator(vector_type& particles) {
-        const prop_type* f = particles.devProp(fValuePos);
-        prop_type* out = particles.devProp(DfValuePos);
-        apply(f, out, prop_type(1), prop_type(0));
-        particles.devPropWritten(DfValuePos);
+        if constexpr (two_sets) {
+            const size_t nd = domain_->size_local();
+            std::vector<prop_type> self(nd);
+            for (size_t p = 0; p < nd; ++p)
+                self[p] = particles.template getProp<fValuePos>(domain_->supportIndex(p));
+            openfpm_dcpse::device_array<prop_type> fs, out(nd);
+            fs.upload(self);
+            apply_two_set(particles.devProp(fValuePos), fs.data(), out.data(), prop_type(1), prop_type(0));
+            const std::vector<prop_type> h = out.to_host();
+            for (size_t p = 0; p < nd; ++p) domain_->template getProp<DfValuePos>(p) = h[p];
+        } else {
+            const prop_type* f = particles.devProp(fValuePos);
+            prop_type* out = particles.devProp(DfValuePos);
+            apply(f, out, prop_type(1), prop_type(0));
+            particles.devPropWritten(DfValuePos);
+        }
+    }
+
+    // Two sets: out[p] = alpha * eps_p^-|b| sum_q K_pq (f[q] - f_self[p]) + beta * out[p]
+    // over the domain rows, f a support-set field, f_self its value at each
+    // domain particle (device arrays).
+    void apply_two_set(const prop_type* f, const prop_type* f_self, prop_type* out, prop_type alpha,
+                       prop_type beta) {
+        auto& c = openfpm_dcpse::context();
+        check_fresh();
+        const int64_t n = (int64_t)verletList_.size(), n_sup = (int64_t)particles_.size_local();
+        prop_type* O[1] = {out};
+        if constexpr (std::is_same<prop_type, float>::value) {
+            const float* W[1] = {W32_.data()};
+            c.check(dcpse_apply_f32(c.get(), n, verletList_.devRowPtr(), verletList_.devCol(), 1, W, f, n_sup, 1,
+                                    f_self, n, O, n, alpha, beta),
+                    "dcpse_apply_f32");
+        } else {
+            const double* W[1] = {W64_.data()};
+            c.check(dcpse_apply_f64(c.get(), n, verletList_.devRowPtr(), verletList_.devCol(), 1, W, f, n_sup, 1,
+                                    f_self, n, O, n, alpha, beta),
+                    "dcpse_apply_f64");
+        }
     }
 
     // out = alpha * D f + beta * out on device arrays of the particle count
+    // (two sets: f over the support set, out over the domain; the values at
+    // the domain particles are gathered from f on the host)
     void apply(const prop_type* f, prop_type* out, prop_type alpha, prop_type beta) override {
         auto& c = openfpm_dcpse::context();
         check_fresh();
         const int64_t n = (int64_t)verletList_.size();
+        if constexpr (two_sets) {
+            std::vector<prop_type> fh(particles_.size_local()), self(domain_->size_local());
+            if (!fh.empty())
+                openfpm_dcpse::hip_check(hipMemcpy(fh.data(), f, fh.size() * sizeof(prop_type), hipMemcpyDeviceToHost),
+                                         "hipMemcpy D2H");
+            for (size_t p = 0; p < self.size(); ++p) self[p] = fh[domain_->supportIndex(p)];
+            openfpm_dcpse::device_array<prop_type> fs;
+            fs.upload(self);
+            apply_two_set(f, fs.data(), out, alpha, beta);
+            return;
+        }
         if constexpr (std::is_same<prop_type, float>::value) {
             float* O[1] = {out};
             // the sliced-ELL copy from the kSellAfter-th apply since assembly
-            // on: building it costs what some 50 applies save, so operators
-            // reassembled every few applies (moving particles) never build it
+            // on (building it costs what some 50 applies save)
             if (n > 0 && !b16_tried_ && ++n_apply32_ >= kSellAfter) build_bundle16();
             if (b16_) {
                 c.check(dcpse_apply_sell_bundle16_f32(c.get(), n, verletList_.devRowPtr(), slice_ptr_.data(),
@@ -146,9 +219,51 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
         return (T)eip_host_[p.getKey()];
     }
 
+    // ---- per-particle apply (Dcpse.hpp:560-659) -------------------------------------------
+    // D o1 at domain particle p, o1 an expression with value(key) over the
+    // support set (and getVector()): eps_p^-|b| sum_q (o1(q) + s o1(p)) K_pq,
+    // s = -1 for even |b|. With two sets o1(p) is read at p's index in the
+    // support set. Served from a host copy of the kernels (the reference's
+    // host expression loop); whole fields go through the device applies.
+    template <typename op_type>
+    auto computeDifferentialOperator(const vect_dist_key_dx& p, op_type& o1)
+        -> std::decay_t<decltype(o1.value(p))> {
+        typedef std::decay_t<decltype(o1.value(p))> expr_type;
+        check_fresh();
+        mirror();
+        const size_t k = p.getKey();
+        const expr_type fxp = apply_sign() * o1.value(vect_dist_key_dx(self_index(k)));
+        expr_type Dfxp = expr_type(0);
+        for (int64_t j = rp_host_[k]; j < rp_host_[k + 1]; ++j)
+            Dfxp = Dfxp + (o1.value(vect_dist_key_dx((size_t)col_host_[j])) + fxp) * (T)K_host_[j];
+        return Dfxp * (T)eip_host_[k];
+    }
+    // component i of a vector-valued o1
+    template <typename op_type>
+    auto computeDifferentialOperator(const vect_dist_key_dx& p, op_type& o1, int i)
+        -> std::decay_t<decltype(o1.value(p)[i])> {
+        typedef std::decay_t<decltype(o1.value(p)[i])> expr_type;
+        check_fresh();
+        mirror();
+        const size_t k = p.getKey();
+        const expr_type fxp = apply_sign() * o1.value(vect_dist_key_dx(self_index(k)))[i];
+        expr_type Dfxp = expr_type(0);
+        for (int64_t j = rp_host_[k]; j < rp_host_[k + 1]; ++j)
+            Dfxp = Dfxp + (o1.value(vect_dist_key_dx((size_t)col_host_[j]))[i] + fxp) * (T)K_host_[j];
+        return Dfxp * (T)eip_host_[k];
+    }
+
     // ---- lifecycle ------------------------------------------------------------------------
+    // Reassembles on the current positions (Dcpse.hpp:661-687); call after the
+    // particles moved and the Verlet list was rebuilt.
     void initializeUpdate(vector_type& particles) {
         (void)particles;
+        assemble();
+    }
+    template <bool B = two_sets, typename = std::enable_if_t<B>>
+    void initializeUpdate(vector_type& particlesSupport, domain_type& particlesDomain) {
+        (void)particlesSupport;
+        domain_ = &particlesDomain;
         assemble();
     }
 
@@ -188,7 +303,7 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
         const size_t n = eps_host_.size();
         for (size_t p = 0; p < n; ++p) {
             std::fill(acc.begin(), acc.end(), 0.0);
-            const auto xp = particles.getPos(vect_dist_key_dx(p));
+            const auto xp = dom_pos(particles, p);
             for (int64_t j = rp_host_[p]; j < rp_host_[p + 1]; ++j) {
                 const auto xq = particles.getPos(vect_dist_key_dx((size_t)col_host_[j]));
                 double z[dim];
@@ -273,6 +388,8 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
         in.read((char*)rp_host_.data(), 8 * (n + 1));
         in.read((char*)col_host_.data(), 4 * nnz);
         if (!in) throw openfpm_dcpse::dcpse_error(file + "_0 is truncated");
+        if (rp_host_ != verletList_.hostRowPtr() || col_host_ != verletList_.hostCol())
+            throw openfpm_dcpse::dcpse_error("kernel file support differs from this operator's Verlet list");
         convergenceOrder_ = hdr[3];
         HOverEpsilon = (T)hoe;
         coeffs_ = co;
@@ -355,6 +472,16 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
     }
 
    private:
+    // the reference apply's s: -1 for even |beta| (zero included), +1 for odd
+    T apply_sign() const { return order_of_derivative() % 2 == 0 ? T(-1) : T(1); }
+    size_t self_index(size_t p) const {
+        if constexpr (two_sets) return domain_->supportIndex(p);
+        return p;
+    }
+    Point<dim, T> dom_pos(vector_type& particles, size_t p) const {
+        if constexpr (two_sets) return domain_->getPos(vect_dist_key_dx(p));
+        return particles.getPos(vect_dist_key_dx(p));
+    }
     unsigned order_of_derivative() const {
         unsigned o = 0;
         for (unsigned d = 0; d < dim; ++d) o += sigs_[d];
@@ -383,7 +510,10 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
             lo[d] = (double)particles_.boxLow()[d];
             hi[d] = (double)particles_.boxHigh()[d];
         }
-        c.check(dcpse_assemble(c.get(), dim, particles_.devPos(), (int64_t)n, nullptr, (int64_t)n, lo, hi,
+        const double* pdom = nullptr;
+        if constexpr (two_sets) pdom = domain_->devPos();
+        c.check(dcpse_assemble(c.get(), dim, particles_.devPos(), (int64_t)particles_.size_local(), pdom, (int64_t)n,
+                               lo, hi,
                                particles_.periodicMask(), verletList_.devRowPtr(), verletList_.devCol(),
                                (int32_t)coeffs_.size(), sigs_.data(), coeffs_.data(), convergenceOrder_,
                                (double)HOverEpsilon, refine ? DCPSE_ASM_REFINE : 0u, eps_.data(), eip_.data(), K64_.data(),
@@ -424,6 +554,7 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
     VerletList_type& verletList_;
     unsigned int convergenceOrder_;
     T rCut_;
+    domain_type* domain_ = nullptr;  // two sets: the domain subset
     std::vector<uint32_t> sigs_;
     std::vector<double> coeffs_;
     openfpm_dcpse::device_array<double> eps_, eip_, K64_, W64_;

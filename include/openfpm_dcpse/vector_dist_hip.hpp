@@ -112,6 +112,8 @@ class grid_key_iterator {
 
 template <typename vector_type>
 class VerletList_hip;
+template <typename parent_type>
+class vector_dist_subset_hip;
 
 // Scalar properties only (the DC-PSE operators act on scalar fields); nprop of
 // them, of type P (double or float: float fields run the FP32 kernels).
@@ -236,6 +238,13 @@ class vector_dist_hip {
     }
     void updateVerlet(VerletList_hip<vector_dist_hip>& vl, St rCut) { vl = VerletList_hip<vector_dist_hip>(*this, rCut); }
 
+    // Moves particle key to x (dim coordinates); the positions changed, so
+    // operators assembled before must be updated (map counter).
+    void setPos(size_t key, const St* x) {
+        St* p = getPosWrite(key);
+        for (unsigned d = 0; d < dim; ++d) p[d] = x[d];
+    }
+
    private:
     struct Prop {
         std::vector<P> host;
@@ -260,12 +269,115 @@ class vector_dist_hip {
     size_t map_ctr_ = 0;
 };
 
+// The particles ids of a vector_dist_hip, the counterpart of openfpm's
+// vector_dist_subset (the domain of two-set operators whose support is the
+// whole set, src/DCPSE/DCPSE_op/tests/DCPSE_op_subset_test.cpp:113-141). It
+// is a view: positions and properties are the parent's at ids; its Verlet
+// list holds, for subset particle p, every parent particle q != ids[p]
+// within rCut. The reference selects the subset by a per-particle subset
+// number; here the caller gives the indices.
+template <typename parent_type>
+class vector_dist_subset_hip {
+   public:
+    static constexpr unsigned int dims = parent_type::dims;
+    typedef typename parent_type::stype stype;
+    typedef typename parent_type::prop_type prop_type;
+    typedef parent_type parent_t;
+
+    vector_dist_subset_hip(parent_type& parent, std::vector<int32_t> ids) : parent_(parent), ids_(std::move(ids)) {
+        for (int32_t i : ids_)
+            if (i < 0 || (size_t)i >= parent_.size_local()) throw dcpse_error("subset index out of range");
+    }
+    size_t size_local() const { return ids_.size(); }
+    const std::vector<int32_t>& getIds() const { return ids_; }
+    // index in the parent set of subset particle p
+    size_t supportIndex(size_t p) const { return (size_t)ids_[p]; }
+    parent_type& parent() { return parent_; }
+
+    Point<dims, stype> getPos(vect_dist_key_dx key) const { return parent_.getPos(vect_dist_key_dx(ids_[key.getKey()])); }
+    template <unsigned int prp>
+    prop_type& getProp(vect_dist_key_dx key) {
+        return parent_.template getProp<prp>(vect_dist_key_dx((size_t)ids_[key.getKey()]));
+    }
+    template <unsigned int prp>
+    prop_type& getProp(size_t key) {
+        return getProp<prp>(vect_dist_key_dx(key));
+    }
+    typename parent_type::DomainIterator getDomainIterator() const { return {0, size_local()}; }
+
+    // device copies: the subset positions (SoA, gathered again after the
+    // parent moved) and the ids
+    const stype* devPos() {
+        if (pos_ctr_ != parent_.getMapCtr() || pos_dev_.size() != dims * ids_.size()) {
+            const size_t n = ids_.size();
+            std::vector<stype> soa(n * dims);
+            for (size_t i = 0; i < n; ++i) {
+                const auto x = parent_.getPos(vect_dist_key_dx((size_t)ids_[i]));
+                for (unsigned d = 0; d < dims; ++d) soa[d * n + i] = x[d];
+            }
+            pos_dev_.upload(soa);
+            pos_ctr_ = parent_.getMapCtr();
+        }
+        return pos_dev_.data();
+    }
+    const int32_t* devIds() {
+        if (ids_dev_.size() != ids_.size()) ids_dev_.upload(ids_);
+        return ids_dev_.data();
+    }
+    const stype* boxLow() const { return parent_.boxLow(); }
+    const stype* boxHigh() const { return parent_.boxHigh(); }
+    unsigned periodicMask() const { return parent_.periodicMask(); }
+    size_t getMapCtr() const { return parent_.getMapCtr(); }
+
+    template <unsigned int opt = VL_NON_SYMMETRIC | VL_SKIP_REF_PART>
+    VerletList_hip<parent_type> getVerlet(stype rCut) {
+        static_assert(opt & VL_SKIP_REF_PART, "DC-PSE supports exclude the particle itself");
+        return VerletList_hip<parent_type>(parent_, *this, rCut);
+    }
+    void updateVerlet(VerletList_hip<parent_type>& vl, stype rCut) { vl = VerletList_hip<parent_type>(parent_, *this, rCut); }
+
+   private:
+    parent_type& parent_;
+    std::vector<int32_t> ids_;
+    device_array<stype> pos_dev_;
+    device_array<int32_t> ids_dev_;
+    size_t pos_ctr_ = ~size_t(0);
+};
+
 // Neighbour CSR in device memory; row p holds every q != p with |x_p - x_q| < rCut
-// in ascending index (dcpse_support_count / dcpse_support_fill).
+// in ascending index (dcpse_support_count / dcpse_support_fill), or, for a
+// subset domain, every parent particle q != ids[p] (dcpse_support_*_subset).
 template <typename vector_type>
 class VerletList_hip {
    public:
     typedef typename vector_type::stype St;
+    template <typename subset_type>
+    VerletList_hip(vector_type& parts, subset_type& dom, St rCut) : n_rows_(dom.size_local()) {
+        Context& c = context();
+        device_array<double> rc(1);
+        double r = (double)rCut;
+        rc.upload(&r, 1);
+        std::vector<double> lo(vector_type::dims), hi(vector_type::dims);
+        for (unsigned d = 0; d < vector_type::dims; ++d) {
+            lo[d] = (double)parts.boxLow()[d];
+            hi[d] = (double)parts.boxHigh()[d];
+        }
+        const double* pos = pos_double(parts);
+        const double* pdom = dom.devPos();
+        const int32_t* ids = dom.devIds();
+        row_ptr_.resize(n_rows_ + 1);
+        int64_t nnz = 0;
+        c.check(dcpse_support_count_subset(c.get(), vector_type::dims, pos, (int64_t)parts.size_local(), pdom, ids,
+                                           (int64_t)n_rows_, rc.data(), 0, lo.data(), hi.data(), parts.periodicMask(),
+                                           row_ptr_.data(), &nnz),
+                "dcpse_support_count_subset");
+        col_.resize((size_t)nnz);
+        if (nnz)
+            c.check(dcpse_support_fill_subset(c.get(), vector_type::dims, pos, (int64_t)parts.size_local(), pdom, ids,
+                                              (int64_t)n_rows_, rc.data(), 0, lo.data(), hi.data(),
+                                              parts.periodicMask(), row_ptr_.data(), col_.data()),
+                    "dcpse_support_fill_subset");
+    }
     VerletList_hip(vector_type& parts, St rCut) : n_rows_(parts.size_local()) {
         Context& c = context();
         device_array<double> rc(1);
@@ -300,6 +412,12 @@ class VerletList_hip {
     size_t nnz() const { return col_.size(); }
     const int64_t* devRowPtr() const { return row_ptr_.data(); }
     const int32_t* devCol() const { return col_.data(); }
+    // host copies of the CSR (read back once)
+    const std::vector<int64_t>& hostRowPtr() { return mirror_rp(); }
+    const std::vector<int32_t>& hostCol() {
+        mirror();
+        return col_host_;
+    }
 
    private:
     static const double* pos_double(vector_type& parts) {

@@ -10,11 +10,11 @@ from .dcpse import ADAPTIVE, AT_LEAST_N_PARTICLES, LOAD, RADIUS, Dcpse, apply_op
 from .operators import (Derivative_G, Derivative_x, Derivative_xx, Derivative_xy, Derivative_xz, Derivative_y,
                         Derivative_yy, Derivative_yz, Derivative_z, Derivative_zz, Gradient, Laplacian, fused)
 from .ode import rk4
-from .particles import NON_PERIODIC, PERIODIC, ParticleSet, VerletList
+from .particles import NON_PERIODIC, PERIODIC, ParticleSet, ParticleSubset, VerletList
 
 __all__ = [
     "ADAPTIVE", "AT_LEAST_N_PARTICLES", "LOAD", "RADIUS", "NON_PERIODIC", "PERIODIC",
-    "Dcpse", "DcpseError", "ParticleSet", "VerletList", "apply_ops", "basis", "rhs", "dcpse_apply", "fused",
+    "Dcpse", "DcpseError", "ParticleSet", "ParticleSubset", "VerletList", "apply_ops", "basis", "rhs", "dcpse_apply", "fused",
     "Derivative_G", "Derivative_x", "Derivative_xx", "Derivative_xy", "Derivative_xz", "Derivative_y",
     "Derivative_yy", "Derivative_yz", "Derivative_z", "Derivative_zz", "Gradient", "Laplacian",
     "load_library", "rk4",

@@ -42,6 +42,10 @@ SIGNATURES = {
     "dcpse_rhs": (_i32, [_i32, _vp, _u32, _vp, _i32]),
     "dcpse_support_count": (_i32, [_vp, _i32, _vp, _i64, _vp, _i64, _vp, _i32, _vp, _vp, _u32, _vp, _vp]),
     "dcpse_support_fill": (_i32, [_vp, _i32, _vp, _i64, _vp, _i64, _vp, _i32, _vp, _vp, _u32, _vp, _vp]),
+    "dcpse_support_count_subset": (_i32, [_vp, _i32, _vp, _i64, _vp, _vp, _i64, _vp, _i32, _vp, _vp, _u32, _vp,
+                                          _vp]),
+    "dcpse_support_fill_subset": (_i32, [_vp, _i32, _vp, _i64, _vp, _vp, _i64, _vp, _i32, _vp, _vp, _u32, _vp,
+                                         _vp]),
     "dcpse_assemble": (_i32, [_vp, _i32, _vp, _i64, _vp, _i64, _vp, _vp, _u32, _vp, _vp,
                               _i32, _vp, _vp, _u32, _f64, _u32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "dcpse_assemble_multi": (_i32, [_vp, _i32, _vp, _i64, _vp, _i64, _vp, _vp, _u32, _vp, _vp, _i32, _vp, _vp, _vp,

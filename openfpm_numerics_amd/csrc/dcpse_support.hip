@@ -323,8 +323,8 @@ __global__ __launch_bounds__(kRowWaves * kWave) void k_sort_long_rows(int64_t n_
 template <int DIM, bool FILL>
 __global__ __launch_bounds__(kRowWaves * kWave) void k_support_rows(
     const double* __restrict__ pos_sup, int64_t n_sup, const double* __restrict__ pos_dom,
-    int64_t n_dom, int same_set, const double* __restrict__ rcut, int rcut_pp, CellGrid g,
-    const int64_t* __restrict__ cell_start, const int32_t* __restrict__ cell_part,
+    int64_t n_dom, int same_set, const int32_t* __restrict__ dom_self, const double* __restrict__ rcut, int rcut_pp,
+    CellGrid g, const int64_t* __restrict__ cell_start, const int32_t* __restrict__ cell_part,
     const double* __restrict__ pos_cell, int32_t* __restrict__ counts, int32_t* __restrict__ overflow,
     const int64_t* __restrict__ row_ptr, int32_t* __restrict__ col) {
     __shared__ int32_t buf[FILL ? kRowWaves : 1][FILL ? DCPSE_MAX_ROW : 1];
@@ -343,6 +343,9 @@ __global__ __launch_bounds__(kRowWaves * kWave) void k_support_rows(
             c[d] = cell_of(xp[d], d, g);
         }
         const double r = rcut_pp ? rcut[p] : rcut[0];
+        // the reference particle skipped in this row (VL_SKIP_REF_PART): p
+        // itself for one set, its support index for a subset domain
+        const int self_q = same_set ? (int)p : (dom_self ? dom_self[p] : -1);
         const double r2 = r * r;
         int kd[3] = {0, 0, 0};
 #pragma unroll
@@ -419,7 +422,7 @@ __global__ __launch_bounds__(kRowWaves * kWave) void k_support_rows(
                 double d2 = d[DIM - 1] * d[DIM - 1];
 #pragma unroll
                 for (int dd = DIM - 2; dd >= 0; --dd) d2 = fma(d[dd], d[dd], d2);
-                const bool take = (base + lane < total) && !(same_set && q == (int)p) && d2 < r2;
+                const bool take = (base + lane < total) && q != self_q && d2 < r2;
                 const unsigned long long m = __ballot(take);
                 if (FILL) {
                     const int at = nacc + lane_prefix(m);
@@ -957,7 +960,8 @@ __global__ __launch_bounds__(kMinMaxThreads) void k_minmax(const double* __restr
 }
 
 static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup, int64_t n_sup,
-                                 const double* pos_dom, int64_t n_dom, const double* rcut, int rcut_pp,
+                                 const double* pos_dom, int64_t n_dom, const int32_t* dom_self,
+                                 const double* rcut, int rcut_pp,
                                  const double* box_lo, const double* box_hi, unsigned periodic,
                                  int64_t* row_ptr, int32_t* col, int64_t* nnz_out) {
     if (dim != 2 && dim != 3) return fail(ctx, DCPSE_ERR_UNSUPPORTED, "dim must be 2 or 3");
@@ -1128,11 +1132,11 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
         } else if (n_dom > 0) {
             if (dim == 2)
                 k_support_rows<2, false><<<(unsigned)rb, kRowWaves * kWave, 0, s>>>(
-                    pos_sup, n_sup, pos_dom, n_dom, same, rcut, rcut_pp, g, cell_start, cell_part, pos_cell, counts,
+                    pos_sup, n_sup, pos_dom, n_dom, same, dom_self, rcut, rcut_pp, g, cell_start, cell_part, pos_cell, counts,
                     flags, nullptr, nullptr);
             else
                 k_support_rows<3, false><<<(unsigned)rb, kRowWaves * kWave, 0, s>>>(
-                    pos_sup, n_sup, pos_dom, n_dom, same, rcut, rcut_pp, g, cell_start, cell_part, pos_cell, counts,
+                    pos_sup, n_sup, pos_dom, n_dom, same, dom_self, rcut, rcut_pp, g, cell_start, cell_part, pos_cell, counts,
                     flags, nullptr, nullptr);
             DCPSE_LAUNCH_CHECK(ctx);
         }
@@ -1189,11 +1193,11 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
         } else if (!by_cell && n_dom > 0) {
             if (dim == 2)
                 k_support_rows<2, true><<<(unsigned)rb, kRowWaves * kWave, 0, s>>>(
-                    pos_sup, n_sup, pos_dom, n_dom, same, rcut, rcut_pp, g, cell_start, cell_part, pos_cell, counts,
+                    pos_sup, n_sup, pos_dom, n_dom, same, dom_self, rcut, rcut_pp, g, cell_start, cell_part, pos_cell, counts,
                     flags, row_ptr, col);
             else
                 k_support_rows<3, true><<<(unsigned)rb, kRowWaves * kWave, 0, s>>>(
-                    pos_sup, n_sup, pos_dom, n_dom, same, rcut, rcut_pp, g, cell_start, cell_part, pos_cell, counts,
+                    pos_sup, n_sup, pos_dom, n_dom, same, dom_self, rcut, rcut_pp, g, cell_start, cell_part, pos_cell, counts,
                     flags, row_ptr, col);
             DCPSE_LAUNCH_CHECK(ctx);
         }
@@ -1400,7 +1404,7 @@ dcpse_status dcpse_support_count(dcpse_ctx* ctx, int32_t dim, const double* pos_
                                  uint32_t periodic_mask, int64_t* row_ptr, int64_t* nnz) {
     if (!ctx) return DCPSE_ERR_INVALID;
     DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
-    return support_pass(ctx, dim, pos_sup, n_sup, pos_dom, n_dom, rcut, rcut_per_particle, box_lo, box_hi,
+    return support_pass(ctx, dim, pos_sup, n_sup, pos_dom, n_dom, nullptr, rcut, rcut_per_particle, box_lo, box_hi,
                         periodic_mask, row_ptr, nullptr, nnz);
 }
 
@@ -1411,7 +1415,32 @@ dcpse_status dcpse_support_fill(dcpse_ctx* ctx, int32_t dim, const double* pos_s
     if (!ctx) return DCPSE_ERR_INVALID;
     if (!col) return fail(ctx, DCPSE_ERR_INVALID, "col is null");
     DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
-    return support_pass(ctx, dim, pos_sup, n_sup, pos_dom, n_dom, rcut, rcut_per_particle, box_lo, box_hi,
+    return support_pass(ctx, dim, pos_sup, n_sup, pos_dom, n_dom, nullptr, rcut, rcut_per_particle, box_lo, box_hi,
+                        periodic_mask, const_cast<int64_t*>(row_ptr), col, nullptr);
+}
+
+dcpse_status dcpse_support_count_subset(dcpse_ctx* ctx, int32_t dim, const double* pos_sup, int64_t n_sup,
+                                        const double* pos_dom, const int32_t* dom_ids, int64_t n_dom,
+                                        const double* rcut, int32_t rcut_per_particle, const double* box_lo,
+                                        const double* box_hi, uint32_t periodic_mask, int64_t* row_ptr,
+                                        int64_t* nnz) {
+    if (!ctx) return DCPSE_ERR_INVALID;
+    if (!pos_dom || !dom_ids) return fail(ctx, DCPSE_ERR_INVALID, "a subset needs pos_dom and dom_ids");
+    DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    return support_pass(ctx, dim, pos_sup, n_sup, pos_dom, n_dom, dom_ids, rcut, rcut_per_particle, box_lo, box_hi,
+                        periodic_mask, row_ptr, nullptr, nnz);
+}
+
+dcpse_status dcpse_support_fill_subset(dcpse_ctx* ctx, int32_t dim, const double* pos_sup, int64_t n_sup,
+                                       const double* pos_dom, const int32_t* dom_ids, int64_t n_dom,
+                                       const double* rcut, int32_t rcut_per_particle, const double* box_lo,
+                                       const double* box_hi, uint32_t periodic_mask, const int64_t* row_ptr,
+                                       int32_t* col) {
+    if (!ctx) return DCPSE_ERR_INVALID;
+    if (!col) return fail(ctx, DCPSE_ERR_INVALID, "col is null");
+    if (!pos_dom || !dom_ids) return fail(ctx, DCPSE_ERR_INVALID, "a subset needs pos_dom and dom_ids");
+    DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    return support_pass(ctx, dim, pos_sup, n_sup, pos_dom, n_dom, dom_ids, rcut, rcut_per_particle, box_lo, box_hi,
                         periodic_mask, const_cast<int64_t*>(row_ptr), col, nullptr);
 }
 

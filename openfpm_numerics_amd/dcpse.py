@@ -25,7 +25,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .particles import ParticleSet, VerletList, sell_layout, sell_weights
+from .particles import ParticleSet, ParticleSubset, VerletList, sell_layout, sell_weights
 
 # support_options (src/DCPSE/SupportBuilder.hpp:13-19)
 RADIUS = 0
@@ -41,10 +41,14 @@ _FILE_MAGIC = b"DCPSEHIP"
 #: CSR kernel's 512 us); set False to use the CSR kernels
 USE_SELL = True
 #: ... from an operator's SELL_AFTER-th such apply after its assembly on; the
-#: ones before run the CSR kernel. Building a sliced-ELL copy (about 2.4 ms for
-#: 1.9M particles and 105M neighbours) costs what some 50 applies save, so an
-#: operator reassembled every few applies (moving particles) never builds one.
-SELL_AFTER = 16
+#: ones before run the CSR kernel. The default, 1, builds the copy at the first
+#: apply, so every single-field apply of an operator runs the same kernel and
+#: repeated applies (and the autograd forward) agree bit for bit. Building the
+#: copy costs about 2.4 ms for 1.9M particles and 105M neighbours, what some 50
+#: applies save: a caller that reassembles every few applies (moving
+#: particles) can raise SELL_AFTER to skip it, and its results then change at
+#: rounding level (summation order) from that apply on.
+SELL_AFTER = 1
 #: The 16-bit-offset bundle serves neighbours whose index offset does not fit
 #: 16 bits from a per-lane exception list; above this fraction of such
 #: neighbours (e.g. Morton-ordered particles: 6.7% at 128^3) the 32-bit
@@ -388,13 +392,16 @@ class Dcpse:
             raise ValueError("kernel file holds a different differential signature")
         dev = self.particlesSupport.device
         vl = self.verletList
-        if vl.n_rows != n or vl.nnz != nnz or not np.array_equal(vl.row_ptr.cpu().numpy(), rp) \
-                or not np.array_equal(vl.col.cpu().numpy(), col):
-            # the file's support replaces the current Verlet rows
-            vl.row_ptr = torch.as_tensor(rp.copy(), device=dev)
-            vl.col = torch.as_tensor(col.copy(), device=dev)
-            vl.n_rows = n
-            vl._host = None
+        # the kernels are only meaningful on the support they were assembled on
+        # (the reference loads them against the Verlet list it is given,
+        # Dcpse.hpp:336-379); a file for another support is refused, and the
+        # Verlet list, which other operators may share, is never touched
+        if vl.n_rows != n or vl.nnz != nnz:
+            raise ValueError(f"kernel file support ({n} rows, {nnz} neighbours) differs from this operator's "
+                             f"Verlet list ({vl.n_rows} rows, {vl.nnz} neighbours)")
+        hrp, hcol = vl._mirror()
+        if not np.array_equal(hrp, rp) or not np.array_equal(hcol, col):
+            raise ValueError("kernel file support differs from this operator's Verlet list (neighbour lists)")
         self.convergenceOrder, self.HOverEpsilon, self.coeffs = int(order), float(hoe), list(coeffs)
         self.localEps = torch.as_tensor(eps.copy(), device=dev)
         self.localEpsInvPow = torch.as_tensor(eip.copy(), device=dev)
@@ -515,6 +522,9 @@ def apply_ops(ops, f: torch.Tensor, outs=None, f_domain: torch.Tensor | None = N
     fd = None
     if f_domain is not None:
         fd = f_domain.reshape(nb, -1).contiguous().to(f.dtype)
+    elif isinstance(ops[0].particlesDomain, ParticleSubset):
+        # a subset domain reads the field at its own particles of the support set
+        fd = ops[0].particlesDomain.gather(f2).contiguous()
     elif not ops[0].same_set:
         raise ValueError("two-set operators need f_domain (the field on the domain particles)")
     n_rows = vl.n_rows

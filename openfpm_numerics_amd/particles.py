@@ -97,6 +97,56 @@ class ParticleSet:
         return vl
 
 
+class ParticleSubset:
+    """The particles ``ids`` of a ParticleSet, the counterpart of openfpm's
+    ``vector_dist_subset``: the domain of two-set operators whose support is
+    the whole set (src/DCPSE/DCPSE_op/tests/DCPSE_op_subset_test.cpp:113-141).
+    Its Verlet list skips each particle's own index in the parent
+    (VL_SKIP_REF_PART); fields over the subset are the parent's at ``ids``."""
+
+    def __init__(self, parent: ParticleSet, ids):
+        self.parent = parent
+        ids_t = torch.as_tensor(np.asarray(ids) if not torch.is_tensor(ids) else ids, device=parent.device)
+        if ids_t.dim() != 1:
+            raise ValueError("ids must be one-dimensional")
+        if ids_t.numel() and (int(ids_t.min()) < 0 or int(ids_t.max()) >= parent.size_local()):
+            raise ValueError("subset ids out of range")
+        self.ids = ids_t.to(torch.int32).contiguous()
+        self._pos = None
+        self._pos_ctr = None
+
+    dims = property(lambda self: self.parent.dims)
+    box_lo = property(lambda self: self.parent.box_lo)
+    box_hi = property(lambda self: self.parent.box_hi)
+    periodic_mask = property(lambda self: self.parent.periodic_mask)
+    device = property(lambda self: self.parent.device)
+    map_ctr = property(lambda self: self.parent.map_ctr)
+
+    @property
+    def pos_soa(self) -> torch.Tensor:
+        """Positions of the subset (gathered again after the parent moved)."""
+        if self._pos is None or self._pos_ctr != self.parent.map_ctr:
+            self._pos = self.parent.pos_soa[:, self.ids.long()].contiguous()
+            self._pos_ctr = self.parent.map_ctr
+        return self._pos
+
+    def getIds(self) -> torch.Tensor:
+        return self.ids
+
+    def size_local(self) -> int:
+        return int(self.ids.numel())
+
+    def getPos(self, p: int) -> np.ndarray:
+        return self.pos_soa[:, p].cpu().numpy()
+
+    def gather(self, f: torch.Tensor) -> torch.Tensor:
+        """A field over the parent restricted to the subset (last axis)."""
+        return f[..., self.ids.long()]
+
+    def getVerlet(self, rCut) -> "VerletList":
+        return VerletList.build(self.parent, rCut, domain=self)
+
+
 def cell_order(pos: np.ndarray, box_lo, box_hi, cells: int | None = None) -> np.ndarray:
     """Permutation sorting (N, dim) points row-major (last dimension fastest)
     by their cell in a grid of `cells` per dimension (default about N^(1/dim))."""
@@ -149,7 +199,7 @@ class VerletList:
         return int(self.col.numel())
 
     @classmethod
-    def build(cls, support: ParticleSet, rCut, domain: ParticleSet | None = None) -> "VerletList":
+    def build(cls, support: ParticleSet, rCut, domain: "ParticleSet | ParticleSubset | None" = None) -> "VerletList":
         ctx = _lib.context(support.device.index or 0)
         dev = support.device
         dom = support if domain is None else domain
@@ -165,13 +215,20 @@ class VerletList:
         pdom = None if domain is None else _lib.ptr(dom.pos_soa)
         lo = np.ascontiguousarray(support.box_lo)
         hi = np.ascontiguousarray(support.box_hi)
-        ctx.call("dcpse_support_count", support.dims, _lib.ptr(support.pos_soa), support.size_local(),
-                 pdom, n_dom, _lib.ptr(rc), per, lo.ctypes.data, hi.ctypes.data,
+        if isinstance(domain, ParticleSubset):
+            if domain.parent is not support:
+                raise ValueError("a subset domain must be a subset of the support set")
+            count, fill, dom_args = "dcpse_support_count_subset", "dcpse_support_fill_subset", (
+                pdom, _lib.ptr(domain.ids), n_dom)
+        else:
+            count, fill, dom_args = "dcpse_support_count", "dcpse_support_fill", (pdom, n_dom)
+        ctx.call(count, support.dims, _lib.ptr(support.pos_soa), support.size_local(),
+                 *dom_args, _lib.ptr(rc), per, lo.ctypes.data, hi.ctypes.data,
                  support.periodic_mask, _lib.ptr(row_ptr), nnz.ctypes.data)
         col = torch.empty(max(int(nnz[0]), 1), dtype=torch.int32, device=dev)
         if int(nnz[0]) > 0:
-            ctx.call("dcpse_support_fill", support.dims, _lib.ptr(support.pos_soa), support.size_local(),
-                     pdom, n_dom, _lib.ptr(rc), per, lo.ctypes.data, hi.ctypes.data,
+            ctx.call(fill, support.dims, _lib.ptr(support.pos_soa), support.size_local(),
+                     *dom_args, _lib.ptr(rc), per, lo.ctypes.data, hi.ctypes.data,
                      support.periodic_mask, _lib.ptr(row_ptr), _lib.ptr(col))
         col = col[: int(nnz[0])]
         vl = cls(row_ptr, col, support.size_local(), rc, domain)

@@ -8,15 +8,20 @@
 // against the right-hand side b_alpha the kernels are built to reproduce, and
 // FP64 applies on the sliced-ELL copy against the CSR kernel. Prints one line per case and exits
 // non-zero if any fails.
+#include <array>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
+#include <fstream>
 #include <functional>
+#include <map>
 #include <string>
 #include <vector>
 
 #include "openfpm_dcpse/DCPSE_op_hip.hpp"
 
 using openfpm_dcpse::vector_dist_hip;
+using openfpm_dcpse::vector_dist_subset_hip;
 
 static int failures = 0;
 
@@ -250,8 +255,7 @@ static bool dcpse_fp32_fields() {
     Laplacian<decltype(vl32)> L32(d32, vl32, 2, rCut);
     Derivative_x<decltype(vl32)> X32(d32, vl32, 2, rCut);
     getV<1>(d64) = L64(getV<0>(d64)) + 0.5 * X64(getV<0>(d64));
-    // 20 evaluations: the FP32 operators switch from the CSR kernel to their
-    // sliced-ELL copies after 16 applies
+    // 20 evaluations, all on the operators' sliced-ELL copies
     double worst = 0;
     for (int it = 0; it < 20; ++it) {
         getV<1>(d32) = L32(getV<0>(d32)) + 0.5 * X32(getV<0>(d32));
@@ -451,7 +455,375 @@ static bool dcpse_fp64_sell() {
     return sell && worst <= 1e-13 * scale;
 }
 
-int main() {
+// ---- parity cases: outputs written for the oracle comparison ------------------------------
+// With an output directory (argv[1]) these cases write raw little-endian
+// arrays <dir>/<name>.<f64|i64|i32>, listed in <dir>/manifest.txt, for
+// tests/test_gpu_dropin_cpp.py to compare with the oracle.
+static std::string g_dir;
+
+template <typename T>
+static void dump(const std::string& name, const std::vector<T>& v) {
+    const char* ext = std::is_same<T, double>::value ? "f64" : (std::is_same<T, int64_t>::value ? "i64" : "i32");
+    std::ofstream out(g_dir + "/" + name + "." + ext, std::ios::binary);
+    out.write((const char*)v.data(), (std::streamsize)(v.size() * sizeof(T)));
+    std::ofstream man(g_dir + "/manifest.txt", std::ios::app);
+    man << name << "." << ext << " " << v.size() << "\n";
+}
+
+template <typename V, unsigned int prp>
+static std::vector<double> prop_of(V& d) {
+    std::vector<double> out(d.size_local());
+    for (size_t p = 0; p < out.size(); ++p) out[p] = d.template getProp<prp>(p);
+    return out;
+}
+
+template <typename V>
+static std::vector<double> pos_of(V& d) {
+    const unsigned dim = V::dims;
+    std::vector<double> out(d.size_local() * dim);
+    for (size_t p = 0; p < d.size_local(); ++p) {
+        const auto x = d.getPos(vect_dist_key_dx(p));
+        for (unsigned k = 0; k < dim; ++k) out[p * dim + k] = x[k];
+    }
+    return out;
+}
+
+// 48 x 48 jittered periodic lattice of [0, 1]^2 (deterministic jitter), the
+// particle set of the parity cases; prop 0 = f
+template <typename V>
+static void fill_parity_set(V& domain) {
+    const size_t sz[2] = {48, 48};
+    const double h = 1.0 / 48;
+    auto it = domain.getGridIterator(sz);
+    while (it.isNext()) {
+        auto key = it.get();
+        domain.add();
+        const double i = (double)key.get(0), j = (double)key.get(1);
+        double x = (i + 0.5 + 0.1 * std::sin(7.3 * i + 3.1 * j)) * h;
+        double y = (j + 0.5 + 0.1 * std::cos(5.7 * i - 2.3 * j)) * h;
+        domain.getLastPos()[0] = x;
+        domain.getLastPos()[1] = y;
+        domain.template getLastProp<0>() = std::sin(2 * M_PI * x) * std::cos(2 * M_PI * y) + x * x;
+        ++it;
+    }
+    domain.map();
+}
+
+// The reference's VECT_DCPSE expression node (DCPSE_op.hpp:29-106) as a
+// caller of the Dcpse_type contract: value(key) is the per-particle apply,
+// value_nz the row of the assembled operator from the accessors.
+template <typename exp1, typename DCPSE_type>
+struct vect_dcpse_node {
+    const exp1 o1;
+    DCPSE_type& dcp;
+    auto value(const vect_dist_key_dx& key) const { return dcp.computeDifferentialOperator(key, o1); }
+    void value_nz(const vect_dist_key_dx& key, std::map<size_t, double>& cols, double coeff) const {
+        for (int j = 0; j < dcp.getNumNN(key); j++) {
+            const double coeff_dc = dcp.getCoeffNN(key, j);
+            const size_t k = dcp.getIndexNN(key, j);
+            cols[k] += coeff_dc * coeff * dcp.getEpsilonInvPrefactor(key);
+            cols[key.getKey()] += dcp.getSign() * coeff_dc * coeff * dcp.getEpsilonInvPrefactor(key);
+        }
+    }
+};
+// a scalar property read on the host, and a two-component one (props a, b)
+template <typename V, unsigned int prp>
+struct prop_expr {
+    V* v;
+    double value(const vect_dist_key_dx& k) const { return v->template getProp<prp>(k); }
+    V& getVector() const { return *v; }
+};
+template <typename V, unsigned int a, unsigned int b>
+struct vec_expr {
+    V* v;
+    std::array<double, 2> value(const vect_dist_key_dx& k) const {
+        return {v->template getProp<a>(k), v->template getProp<b>(k)};
+    }
+    V& getVector() const { return *v; }
+};
+
+// A Dcpse_type that offers only the reference Dcpse contract (the 6-argument
+// constructor, per-particle applies, accessors, initializeUpdate), as the
+// reference's own Dcpse does: the operator classes must evaluate expressions
+// over it particle by particle and build the Laplacian from dim operators.
+template <unsigned int dim, typename VL, typename V, typename... Rest>
+class RefShapedDcpse {
+   public:
+    typedef typename V::stype T;
+    typedef V vtype;
+    RefShapedDcpse(V& parts, VL& vl, Point<dim, unsigned int> sig, unsigned int ord, T rCut,
+                   support_options opt = support_options::RADIUS)
+        : in_(parts, vl, sig, ord, rCut, opt) {}
+    template <typename op_type>
+    auto computeDifferentialOperator(const vect_dist_key_dx& p, op_type& o1) {
+        return in_.computeDifferentialOperator(p, o1);
+    }
+    template <unsigned int f, unsigned int Df>
+    void computeDifferentialOperator(V& parts) {
+        in_.template computeDifferentialOperator<f, Df>(parts);
+    }
+    int getNumNN(const vect_dist_key_dx& p) { return in_.getNumNN(p); }
+    T getCoeffNN(const vect_dist_key_dx& p, int j) { return in_.getCoeffNN(p, j); }
+    size_t getIndexNN(const vect_dist_key_dx& p, int j) { return in_.getIndexNN(p, j); }
+    T getSign() { return in_.getSign(); }
+    T getEpsilonInvPrefactor(const vect_dist_key_dx& p) { return in_.getEpsilonInvPrefactor(p); }
+    void initializeUpdate(V& parts) { in_.initializeUpdate(parts); }
+    auto checkMomenta(V& parts) { return in_.checkMomenta(parts); }
+    void save(const std::string& f) { in_.save(f); }
+    void load(const std::string& f) { in_.load(f); }
+    template <unsigned int prp>
+    void DrawKernel(V& parts, int k) { in_.template DrawKernel<prp>(parts, k); }
+    template <unsigned int prp>
+    void DrawKernelNN(V& parts, int k) { in_.template DrawKernelNN<prp>(parts, k); }
+
+   private:
+    Dcpse_hip<dim, VL, V> in_;
+};
+
+// v = 2 Dx(P) + Dy(P), Lap(P), a mixed sum and an in-place update, by the
+// device operator classes (vector_dist_operators.hpp:1113-1138 semantics).
+static bool parity_expressions() {
+    Box<2, double> box({0, 0}, {1, 1});
+    size_t bc[2] = {PERIODIC, PERIODIC};
+    const double rCut = 3.1 / 48;
+    vector_dist_hip<2, double, 6> d(0, box, bc, Ghost<2, double>(rCut));
+    fill_parity_set(d);
+    auto vl = d.getVerlet<VL_NON_SYMMETRIC | VL_SKIP_REF_PART>(rCut);
+    Derivative_x<decltype(vl)> Dx(d, vl, 2, rCut);
+    Derivative_y<decltype(vl)> Dy(d, vl, 2, rCut);
+    Laplacian<decltype(vl)> Lap(d, vl, 2, rCut);
+    auto P = getV<0>(d);
+    auto v = getV<1>(d);
+    auto lap = getV<2>(d);
+    auto w = getV<3>(d);
+    auto q = getV<4>(d);
+    v = 2 * Dx(P) + Dy(P);
+    lap = Lap(P);
+    w = P - 0.5 * Lap(P) + Dx(P);
+    q = P;
+    q = q + 0.25 * Dy(q);
+    dump("expr_pos", pos_of(d));
+    dump("expr_P", prop_of<decltype(d), 0>(d));
+    dump("expr_v", prop_of<decltype(d), 1>(d));
+    dump("expr_lap", prop_of<decltype(d), 2>(d));
+    dump("expr_w", prop_of<decltype(d), 3>(d));
+    dump("expr_inplace", prop_of<decltype(d), 4>(d));
+    Dx.save(d, g_dir + "/cpp_dx");
+    return true;
+}
+
+// The per-particle contract driven by a VECT_DCPSE-shaped node: Dx and Dxy
+// at every particle, the component form on a two-component field, and the
+// assembled rows of Dxy from value_nz.
+static bool parity_keyed() {
+    Box<2, double> box({0, 0}, {1, 1});
+    size_t bc[2] = {PERIODIC, PERIODIC};
+    const double rCut = 3.1 / 48;
+    typedef vector_dist_hip<2, double, 6> V;
+    V d(0, box, bc, Ghost<2, double>(rCut));
+    fill_parity_set(d);
+    for (size_t p = 0; p < d.size_local(); ++p) d.getProp<5>(p) = std::cos(3.0 * d.getProp<0>(p));
+    auto vl = d.getVerlet<VL_NON_SYMMETRIC | VL_SKIP_REF_PART>(rCut);
+    typedef decltype(vl) VL;
+    Dcpse_hip<2, VL, V> dx(d, vl, Point<2, unsigned int>({1, 0}), 2, rCut);
+    Dcpse_hip<2, VL, V> dxy(d, vl, Point<2, unsigned int>({1, 1}), 2, rCut);
+    typedef prop_expr<V, 0> E0;
+    vect_dcpse_node<E0, Dcpse_hip<2, VL, V>> ndx{E0{&d}, dx}, ndxy{E0{&d}, dxy};
+    typedef vec_expr<V, 0, 5> EV;
+    const EV ev{&d};
+    const size_t n = d.size_local();
+    std::vector<double> kdx(n), kdxy(n), kc0(n), kc1(n);
+    std::vector<int64_t> rp(1, 0);
+    std::vector<int64_t> cols;
+    std::vector<double> vals;
+    for (size_t p = 0; p < n; ++p) {
+        const vect_dist_key_dx key(p);
+        kdx[p] = ndx.value(key);
+        kdxy[p] = ndxy.value(key);
+        kc0[p] = dx.computeDifferentialOperator(key, ev, 0);
+        kc1[p] = dx.computeDifferentialOperator(key, ev, 1);
+        std::map<size_t, double> row;
+        ndxy.value_nz(key, row, 1.0);
+        for (auto& kv : row) {
+            cols.push_back((int64_t)kv.first);
+            vals.push_back(kv.second);
+        }
+        rp.push_back((int64_t)cols.size());
+    }
+    dump("keyed_dx", kdx);
+    dump("keyed_dxy", kdxy);
+    dump("keyed_comp0", kc0);
+    dump("keyed_comp1", kc1);
+    dump("keyed_f5", prop_of<V, 5>(d));
+    dump("nz_rowptr", rp);
+    dump("nz_col", cols);
+    dump("nz_val", vals);
+    return true;
+}
+
+// The operator classes over a Dcpse_type with only the reference contract:
+// expressions evaluated per particle, the Laplacian as dim operators.
+static bool parity_ref_shaped() {
+    Box<2, double> box({0, 0}, {1, 1});
+    size_t bc[2] = {PERIODIC, PERIODIC};
+    const double rCut = 3.1 / 48;
+    vector_dist_hip<2, double, 6> d(0, box, bc, Ghost<2, double>(rCut));
+    fill_parity_set(d);
+    auto vl = d.getVerlet<VL_NON_SYMMETRIC | VL_SKIP_REF_PART>(rCut);
+    typedef decltype(vl) VL;
+    openfpm_dcpse::Derivative_x_T<VL, RefShapedDcpse> Dx(d, vl, 2, rCut);
+    openfpm_dcpse::Derivative_y_T<VL, RefShapedDcpse> Dy(d, vl, 2, rCut);
+    openfpm_dcpse::Laplacian_T<VL, RefShapedDcpse> Lap(d, vl, 2, rCut);
+    auto P = getV<0>(d);
+    getV<1>(d) = 2 * Dx(P) + Dy(P);
+    getV<2>(d) = Lap(P);
+    dump("ref_v", prop_of<decltype(d), 1>(d));
+    dump("ref_lap", prop_of<decltype(d), 2>(d));
+    std::printf("         Laplacian over the reference-shaped Dcpse_type holds %zu operators\n", Lap.terms());
+    return Lap.terms() == 2;
+}
+
+// dcpse_op_convection (DCPSE_op_test_base_tests.cpp:1025-1156): diffusion
+// steps with Dxx + Dyy while the particles move by a rotating velocity; after
+// each move the operators refuse to apply until update() reassembles them on
+// the new Verlet list.
+static bool parity_update() {
+    const size_t sz[2] = {41, 41};
+    Box<2, double> box({-1, -1}, {1, 1});
+    size_t bc[2] = {NON_PERIODIC, NON_PERIODIC};
+    const double h = 2.0 / (sz[0] - 1);
+    const double rCut = 3.1 * h;
+    typedef vector_dist_hip<2, double, 3> V;
+    V d(0, box, bc, Ghost<2, double>(rCut));
+    auto it = d.getGridIterator(sz);
+    while (it.isNext()) {
+        auto key = it.get();
+        d.add();
+        const double x = -1.0 + key.get(0) * h, y = -1.0 + key.get(1) * h;
+        d.getLastPos()[0] = x;
+        d.getLastPos()[1] = y;
+        d.getLastProp<0>() = std::exp(-8.0 * (x * x + (y - 0.2) * (y - 0.2)));
+        ++it;
+    }
+    d.map();
+    auto vl = d.getVerlet<VL_NON_SYMMETRIC | VL_SKIP_REF_PART>(rCut);
+    Derivative_xx<decltype(vl)> Dxx(d, vl, 2, rCut);
+    Derivative_yy<decltype(vl)> Dyy(d, vl, 2, rCut);
+    auto C = getV<0>(d);
+    auto Cnew = getV<1>(d);
+    const double dt = 1e-2;
+    bool refused = true;
+    for (int step = 0; step < 5; ++step) {
+        Cnew = C + dt * 0.01 * (Dxx(C) + Dyy(C));
+        C = Cnew;
+        for (size_t p = 0; p < d.size_local(); ++p) {
+            const auto xp = d.getPos(vect_dist_key_dx(p));
+            const double x = xp[0], y = xp[1], g = std::exp(-10 * (x * x + y * y));
+            const double inside = (x > -1 && y > -1 && x < 1 && y < 1) ? 1.0 : 0.0;
+            const double nx[2] = {x + dt * inside * (-y) * g, y + dt * inside * x * g};
+            d.setPos(p, nx);
+        }
+        d.map();
+        bool threw = false;
+        try {
+            getV<2>(d) = Dxx(C);
+        } catch (const openfpm_dcpse::dcpse_error&) {
+            threw = true;
+        }
+        refused = refused && threw;
+        d.updateVerlet(vl, rCut);
+        Dxx.update(d);
+        Dyy.update(d);
+    }
+    getV<1>(d) = Dxx(C);
+    getV<2>(d) = Dyy(C);
+    dump("upd_pos", pos_of(d));
+    dump("upd_C", prop_of<V, 0>(d));
+    dump("upd_dxx", prop_of<V, 1>(d));
+    dump("upd_dyy", prop_of<V, 2>(d));
+    std::printf("         stale operators refused before update: %s\n", refused ? "yes" : "no");
+    return refused;
+}
+
+// Two sets: the support is the whole lattice, the domain the subset of
+// particles with (i + j) % 3 != 0 (DCPSE_op_subset_test.cpp:113-141); Dx
+// per particle of the subset and in bulk.
+static bool parity_two_sets() {
+    Box<2, double> box({0, 0}, {1, 1});
+    size_t bc[2] = {PERIODIC, PERIODIC};
+    const double rCut = 3.1 / 48;
+    typedef vector_dist_hip<2, double, 4> V;
+    V d(0, box, bc, Ghost<2, double>(rCut));
+    fill_parity_set(d);
+    std::vector<int32_t> ids;
+    for (int32_t p = 0; p < (int32_t)d.size_local(); ++p)
+        if (((p / 48) + (p % 48)) % 3 != 0) ids.push_back(p);
+    typedef vector_dist_subset_hip<V> S;
+    S bulk(d, ids);
+    auto vl = bulk.getVerlet<VL_NON_SYMMETRIC | VL_SKIP_REF_PART>(rCut);
+    typedef decltype(vl) VL;
+    Derivative_x<VL> Dx_bulk(d, bulk, vl, 2, rCut, support_options::RADIUS);
+    auto& dc = Dx_bulk.dcpse(d, bulk);
+    typedef prop_expr<V, 0> E0;
+    vect_dcpse_node<E0, std::remove_reference_t<decltype(dc)>> node{E0{&d}, dc};
+    std::vector<double> keyed(ids.size());
+    for (size_t p = 0; p < ids.size(); ++p) keyed[p] = node.value(vect_dist_key_dx(p));
+    for (size_t p = 0; p < d.size_local(); ++p) d.getProp<1>(p) = -7.0;
+    dc.computeDifferentialOperator<0, 1>(d);  // writes prop 1 at the subset's particles
+    std::vector<double> bulk_out(ids.size());
+    double untouched = 0.0;
+    for (size_t p = 0; p < ids.size(); ++p) bulk_out[p] = bulk.getProp<1>(p);
+    for (size_t p = 0; p < d.size_local(); ++p)
+        if (((p / 48) + (p % 48)) % 3 == 0) untouched = std::fmax(untouched, std::fabs(d.getProp<1>(p) + 7.0));
+    dump("two_ids", ids);
+    dump("two_keyed", keyed);
+    dump("two_bulk", bulk_out);
+    std::printf("         %zu domain particles, the others left untouched: %s\n", ids.size(),
+                untouched == 0.0 ? "yes" : "no");
+    return untouched == 0.0;
+}
+
+// Kernels saved by the Python package for the parity set, loaded here and
+// applied (argv: <dir> load-py).
+static bool parity_load_python() {
+    Box<2, double> box({0, 0}, {1, 1});
+    size_t bc[2] = {PERIODIC, PERIODIC};
+    const double rCut = 3.1 / 48;
+    vector_dist_hip<2, double, 3> d(0, box, bc, Ghost<2, double>(rCut));
+    fill_parity_set(d);
+    auto vl = d.getVerlet<VL_NON_SYMMETRIC | VL_SKIP_REF_PART>(rCut);
+    Derivative_x<decltype(vl)> DxLoaded(d, vl, 2, rCut, support_options::LOAD);
+    DxLoaded.load(d, g_dir + "/py_dx");
+    getV<1>(d) = DxLoaded(getV<0>(d));
+    dump("pyload_dx", prop_of<decltype(d), 1>(d));
+    // a kernel file of another support is refused
+    Derivative_x<decltype(vl)> Other(d, vl, 2, rCut, support_options::LOAD);
+    bool refused = false;
+    try {
+        Other.load(d, g_dir + "/py_dx_other");
+    } catch (const openfpm_dcpse::dcpse_error&) {
+        refused = true;
+    }
+    std::printf("         kernel file of another support refused: %s\n", refused ? "yes" : "no");
+    return refused;
+}
+
+int main(int argc, char** argv) {
+    if (argc > 1) {
+        g_dir = argv[1];
+        std::remove((g_dir + "/manifest.txt").c_str());
+        if (argc > 2 && std::strcmp(argv[2], "load-py") == 0) {
+            run("parity_load_python", parity_load_python);
+        } else {
+            run("parity_expressions", parity_expressions);
+            run("parity_keyed", parity_keyed);
+            run("parity_ref_shaped", parity_ref_shaped);
+            run("parity_update", parity_update);
+            run("parity_two_sets", parity_two_sets);
+        }
+        std::printf("%d failure(s)\n", failures);
+        return failures ? 1 : 0;
+    }
     run("Dcpse_2D_test", dcpse_2d_test);
     run("Dcpse_3D_test", dcpse_3d_test);
     run("dcpse_op_tests", dcpse_op_tests);

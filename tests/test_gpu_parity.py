@@ -54,7 +54,8 @@ def test_support_matches_oracle(name, oracle, cuda):
 def test_support_cutoff_spread_and_two_sets_match_oracle(oracle, cuda):
     """The cell walk with per-particle reach: cut-offs over a 4x range (3D,
     x not periodic, cells of rmax/4), and a 2D domain set distinct from the
-    support set with per-domain-particle cut-offs, against brute force."""
+    support set with per-domain-particle cut-offs, against the oracle and a
+    numpy brute-force search over all pairs."""
     from openfpm_numerics_amd import ParticleSet, workloads as W
     from openfpm_numerics_amd.particles import VerletList
 
@@ -79,6 +80,14 @@ def test_support_cutoff_spread_and_two_sets_match_oracle(oracle, cuda):
         rp2, col2 = oracle.support(sup, rc, [0.0, 0.0], [1.0, 1.0], periodic, pos_dom=dom)
         assert np.array_equal(vl2.row_ptr.cpu().numpy(), rp2), periodic
         assert np.array_equal(vl2.col.cpu().numpy(), col2), periodic
+        d = dom[:, None, :] - sup[None, :, :]
+        for k in range(2):
+            if periodic >> k & 1:
+                d[..., k] -= np.round(d[..., k])
+        d2 = (d ** 2).sum(-1)
+        gp, gc = vl2.row_ptr.cpu().numpy(), vl2.col.cpu().numpy()
+        for p in range(len(dom)):
+            assert np.array_equal(gc[gp[p]:gp[p + 1]], np.nonzero(d2[p] < rc[p] ** 2)[0]), (periodic, p)
 
 
 @pytest.mark.parametrize("dim", [2, 3])
@@ -285,7 +294,7 @@ def test_fused_apply_equals_separate(cuda):
 
 
 def test_multi_field_apply_equals_single(cuda):
-    from openfpm_numerics_amd import Laplacian, workloads as W
+    from openfpm_numerics_amd import Laplacian, dcpse as D, workloads as W
 
     w = W.jittered_2d(64, seed=3)
     ps = _ps(w, cuda)
@@ -294,8 +303,18 @@ def test_multi_field_apply_equals_single(cuda):
     g = torch.Generator(device="cpu").manual_seed(1)
     F = torch.randn(13, ps.size_local(), generator=g, dtype=torch.float64).to(cuda)
     batch = lap(F)
+    # one field at a time through the same CSR kernel (the sliced-ELL copy,
+    # which single fields run by default, sums in another order)
+    sell_after = D.SELL_AFTER
+    D.SELL_AFTER = 1 << 30
+    try:
+        for b in range(F.shape[0]):
+            assert torch.equal(batch[b], lap(F[b]))
+    finally:
+        D.SELL_AFTER = sell_after
     for b in range(F.shape[0]):
-        assert torch.equal(batch[b], lap(F[b]))
+        single = lap(F[b])
+        assert float((batch[b] - single).abs().max()) <= 1e-13 * float(single.abs().max())
     # FP32: the batch runs the CSR kernel, a single field the sliced-ELL one
     # (different summation order), so they agree to FP32 rounding only
     F32 = F.to(torch.float32)
@@ -409,10 +428,10 @@ def test_sell_layout_apply(name, oracle, cuda):
 
 
 def test_sell_copy_built_from_the_nth_apply(cuda):
-    """FP32 single-field applies run the CSR kernel until SELL_AFTER applies
-    since assembly, then the 16-bit sliced-ELL copy (built then); a
-    reassembly starts the count again. Both give the same result to FP32
-    rounding of the row sums."""
+    """With SELL_AFTER raised to 4, FP32 single-field applies run the CSR
+    kernel for 3 applies since assembly, then the 16-bit sliced-ELL copy
+    (built then); a reassembly starts the count again. Both give the same
+    result to FP32 rounding of the row sums."""
     from openfpm_numerics_amd import Laplacian, dcpse as D, workloads as W
 
     w = W.jittered_2d(96, seed=7)
@@ -420,16 +439,51 @@ def test_sell_copy_built_from_the_nth_apply(cuda):
     vl = ps.getVerlet(w["rcut"])
     lap = Laplacian(ps, vl, 2, w["rcut"])
     f = torch.randn(ps.size_local(), generator=torch.Generator().manual_seed(2)).to(cuda)
-    outs = []
-    for i in range(D.SELL_AFTER + 2):
-        outs.append(lap(f))
-        assert ("bundle16" in lap.dcpse._W) == (i + 1 >= D.SELL_AFTER), i
-    ref = outs[0].double()
-    for o in outs[1:]:
-        assert float((o.double() - ref).abs().max()) <= 1e-6 * float(ref.abs().max())
-    lap.dcpse.initializeStaticSize(ps, ps, check=False)
-    lap(f)
-    assert "bundle16" not in lap.dcpse._W
+    sell_after = D.SELL_AFTER
+    D.SELL_AFTER = 4
+    try:
+        outs = []
+        for i in range(D.SELL_AFTER + 2):
+            outs.append(lap(f))
+            assert ("bundle16" in lap.dcpse._W) == (i + 1 >= D.SELL_AFTER), i
+        ref = outs[0].double()
+        for o in outs[1:]:
+            assert float((o.double() - ref).abs().max()) <= 1e-6 * float(ref.abs().max())
+        lap.dcpse.initializeStaticSize(ps, ps, check=False)
+        lap(f)
+        assert "bundle16" not in lap.dcpse._W
+    finally:
+        D.SELL_AFTER = sell_after
+
+
+@pytest.mark.parametrize("name", ["2d_jitter_96", "3d_holes_20"])
+def test_repeated_applies_are_bitwise_equal(name, cuda):
+    """With the default SELL_AFTER every single-field apply of an operator runs
+    one kernel on one layout, so repeated applies, the autograd forward and
+    the transposed apply give identical bits, FP32 and FP64, after a
+    reassembly too."""
+    from openfpm_numerics_amd import Derivative_x, Laplacian, dcpse_apply, workloads as W
+
+    w = WORKLOADS[name](W)
+    ps = _ps(w, cuda)
+    vl = ps.getVerlet(w["rcut"])
+    ops = [Laplacian(ps, vl, 2, w["rcut"]), Derivative_x(ps, vl, 2, w["rcut"])]
+    g = torch.Generator().manual_seed(5)
+    for dt in (torch.float32, torch.float64):
+        f = torch.randn(ps.size_local(), generator=g, dtype=torch.float64).to(dt).to(cuda)
+        for op in ops:
+            first = op(f)
+            firstT = op.applyTranspose(f)
+            for rep in range(20):
+                assert torch.equal(op(f), first), (name, dt, rep)
+                assert torch.equal(op.applyTranspose(f), firstT), (name, dt, rep)
+            x = f.clone().requires_grad_(True)
+            y = dcpse_apply(op, x)
+            assert torch.equal(y.detach(), first)
+            y.backward(f)
+            assert torch.equal(x.grad, firstT)
+            op.dcpse.initializeStaticSize(ps, ps)
+            assert torch.equal(op(f), first), (name, dt, "after reassembly")
 
 
 @pytest.mark.parametrize("name", sorted(WORKLOADS))
@@ -454,11 +508,11 @@ def test_sell_fp64_apply(name, oracle, cuda):
     f64 = np.random.default_rng(4).standard_normal(ps.size_local())
     want = [sum(oracle.apply(r, rp, col, f64) for r in refs), oracle.apply(ref_dx, rp, col, f64)]
     ft = torch.as_tensor(f64, device=cuda)
-    csr = fused(lap, dx, f=ft)  # first apply since assembly: the CSR kernel
-    assert "sell64" not in lap.dcpse._W and "sell64" not in dx.dcpse._W
     sell_after = D.SELL_AFTER
     D.SELL_AFTER = 2
     try:
+        csr = fused(lap, dx, f=ft)  # first apply since assembly: the CSR kernel
+        assert "sell64" not in lap.dcpse._W and "sell64" not in dx.dcpse._W
         got = fused(lap, dx, f=ft)
     finally:
         D.SELL_AFTER = sell_after
