@@ -89,6 +89,10 @@ def main():
                     help="particle storage order (morton: ParticleSet.reorder, as openfpm's reorder)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run torch's copy kernel on 1 GiB (HBM counter calibration)")
+    ap.add_argument("--asm-only", type=int, default=0, metavar="REPS",
+                    help="only build the support and run REPS Laplacian + Dx assembly passes with the default "
+                         "launch (assembly counter runs)")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-support assembly workload")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -101,7 +105,8 @@ def main():
     import torch
 
     from openfpm_numerics_amd import Derivative_x, Laplacian, ParticleSet, workloads as W
-    from openfpm_numerics_amd.autotune import tune_apply, tune_assembly, tune_sell
+    from openfpm_numerics_amd.autotune import (ASSEMBLY_DEFAULT, set_assembly_launch, tune_apply, tune_assembly,
+                                               tune_sell)
     from openfpm_numerics_amd.dcpse import Dcpse, PreparedApply, assemble_together
 
     if not torch.cuda.is_available():
@@ -148,10 +153,12 @@ def main():
     def assemble_both():  # one pass for the pair (shared gathers, epsilon and moments)
         assemble_together([lap.dcpse, dx.dcpse], ps, ps, check=False)
 
-    if args.tune:
-        (ppw, _), asm_tune = tune_assembly(assemble_both)
-        log(f"[bench] assembly autotune particles/wave -> {ppw}: "
-            + ", ".join(f"{k}:{v:.2f}ms" for k, v in asm_tune.items()))
+    if args.asm_only:
+        for _ in range(args.asm_only):
+            assemble_both()
+        torch.cuda.synchronize()
+        print(json.dumps({"asm_only": args.asm_only, "particles": N, "nnz": nnz}))
+        return
     asm_lap_ms, _ = events_ms(lambda: lap.dcpse.initializeStaticSize(ps, ps, check=False), 3, torch)
     asm_dx_ms, _ = events_ms(lambda: dx.dcpse.initializeStaticSize(ps, ps, check=False), 3, torch)
     # the pair with one step of FP64 iterative refinement per solve (an option,
@@ -159,11 +166,49 @@ def main():
     lap.dcpse.refine = dx.dcpse.refine = True
     assembly_refined_ms, _ = events_ms(assemble_both, 3, torch)
     lap.dcpse.refine = dx.dcpse.refine = False
+    assembly_default_ms, _ = events_ms(assemble_both, 3, torch)
+    # the launch shape chosen by the autotuner (static or persistent grid,
+    # particles per batch, neighbour chunk, bucketing); the weights are the same
+    (asm_cfg, _), asm_tune = tune_assembly(assemble_both)
     assembly_ms, _ = events_ms(assemble_both, 3, torch)
     lap.dcpse.check_info()
     dx.dcpse.check_info()
-    log(f"[bench] assembly Lap {asm_lap_ms:.3f} ms, Dx {asm_dx_ms:.3f} ms, both {assembly_ms:.3f} ms "
-        f"(with refinement {assembly_refined_ms:.3f} ms)")
+    log(f"[bench] assembly Lap {asm_lap_ms:.3f} ms, Dx {asm_dx_ms:.3f} ms, both {assembly_default_ms:.3f} ms "
+        f"(static grid), {assembly_ms:.3f} ms with the tuned launch {asm_cfg} (with refinement "
+        f"{assembly_refined_ms:.3f} ms)")
+    set_assembly_launch(*ASSEMBLY_DEFAULT)
+
+    # ---- batched uneven supports: about 20 to 200 neighbours in one launch ---------
+    uneven = None
+    if not args.no_uneven:
+        wu = W.uneven_3d(1_000_000)
+        psu = ParticleSet(wu["pos"], wu["box_lo"], wu["box_hi"], bc=[1, 1, 1], device=dev)
+        vlu = psu.getVerlet(wu["rcut"])
+        lu = vlu.row_lengths().cpu().numpy()
+        lapu = Laplacian(psu, vlu, 2, wu["rcut"])
+        dxu = Derivative_x(psu, vlu, 2, wu["rcut"])
+
+        def assemble_uneven():
+            assemble_together([lapu.dcpse, dxu.dcpse], psu, psu, check=False)
+
+        uneven_static_ms, _ = events_ms(assemble_uneven, 3, torch)
+        (ucfg, _), utune = tune_assembly(assemble_uneven)
+        uneven_ms, _ = events_ms(assemble_uneven, 3, torch)
+        lapu.dcpse.check_info()
+        dxu.dcpse.check_info()
+        set_assembly_launch(*ASSEMBLY_DEFAULT)
+        uneven = {"workload": "3d_uneven_h0(1+1.1x)_periodic_one_rcut_order2_lap+dx", "particles": int(len(lu)),
+                  "nnz": int(vlu.nnz),
+                  "neighbours_min_p5_p50_p95_max": [int(lu.min()), float(np.percentile(lu, 5)),
+                                                    float(np.median(lu)), float(np.percentile(lu, 95)),
+                                                    int(lu.max())],
+                  "assembly_ms": uneven_ms, "assembly_ms_static_grid": uneven_static_ms,
+                  "launch": {"persistent": ucfg[0], "particles_per_batch": ucfg[1], "chunk": ucfg[2],
+                             "bucket_width": ucfg[3]},
+                  "autotune_ms": {"%d/%d/%d/%d" % k: v for k, v in utune.items()}}
+        log(f"[bench] uneven supports N={len(lu)} n {lu.min()}..{lu.max()}: Lap+Dx assembly {uneven_ms:.3f} ms "
+            f"(tuned {ucfg}), static grid {uneven_static_ms:.3f} ms")
+        del lapu, dxu, vlu, psu
 
     # ---- apply --------------------------------------------------------------------
     f64, dx_exact, _, lap_exact = W.sin_field(w["pos"])
@@ -288,7 +333,11 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
 
-        cores = min(16, os.cpu_count() or 1)
+        # every core this process may run on, capped by OMP_NUM_THREADS where the
+        # host sets it (the GPU box gives one GPU's job a 16-core share that way)
+        visible = len(os.sched_getaffinity(0))
+        omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        cores = min(visible, omp) if omp > 0 else visible
         rp = vl.row_ptr.cpu().numpy()
         col = vl.col.cpu().numpy()
         sigs = [(2, 0, 0), (0, 2, 0), (0, 0, 2), (1, 0, 0)]
@@ -330,8 +379,9 @@ def main():
                          f"assembly: first {S} particles x 4 operators (dense E*V, pivoted QR), "
                          f"extrapolated to N",
                "assembly_ms_extrapolated": cpu_asm_ms,
-               "step_ms": cpu_step_s * 1e3}
-        log(f"[bench] CPU ({cores} threads): step {cpu_step_s * 1e3:.1f} ms -> {cpu['value']:.1f} M evals/s; "
+               "step_ms": cpu_step_s * 1e3,
+               "host_cores_visible": visible, "omp_num_threads": omp or None}
+        log(f"[bench] CPU ({cores} threads of {visible} visible cores, OMP_NUM_THREADS={omp or 'unset'}): step {cpu_step_s * 1e3:.1f} ms -> {cpu['value']:.1f} M evals/s; "
             f"assembly {cpu_asm_ms:.0f} ms (extrapolated from {S})")
 
     out = {
@@ -356,7 +406,11 @@ def main():
                                     {"group_lanes": tuned[0][0], "elems_per_lane": tuned[0][1],
                                      "vector_width": tuned[0][2], "block": tuned[0][3]})},
         "assembly_ms": assembly_ms,
+        "assembly_launch": {"persistent": asm_cfg[0], "particles_per_batch": asm_cfg[1], "chunk": asm_cfg[2],
+                            "bucket_width": asm_cfg[3]},
+        "assembly_ms_static_grid": assembly_default_ms,
         "assembly_ms_by_operator": {"laplacian": asm_lap_ms, "dx": asm_dx_ms},
+        "assembly_uneven": uneven,
         "assembly_ms_with_refinement": assembly_refined_ms,
         "support_ms": support_ms,
         "fp64_apply_us": fp64_apply_us,

@@ -37,6 +37,7 @@ SIGNATURES = {
     "dcpse_sync": (_i32, [_vp]),
     "dcpse_set_apply_config": (_i32, [_vp, _i32, _i32, _i32, _i32]),
     "dcpse_set_assembly_config": (_i32, [_vp, _i32, _i32]),
+    "dcpse_set_assembly_launch": (_i32, [_vp, _i32, _i32, _i32, _i32]),
     "dcpse_basis_size": (_i32, [_i32, _vp, _u32]),
     "dcpse_basis": (_i32, [_i32, _vp, _u32, _vp, _i32]),
     "dcpse_rhs": (_i32, [_i32, _vp, _u32, _vp, _i32]),

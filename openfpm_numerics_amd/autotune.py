@@ -4,7 +4,9 @@ Times each candidate with HIP events on the current stream (the stream the
 library launches on) and keeps the fastest in the library context:
   apply:    CSR (lanes per row, nonzeros per lane, load width) x threads per block;
             sliced-ELL groups of 4 slots loaded ahead {2, 4}
-  assembly: particles each wavefront assembles in sequence {1, 2, 4, 8, 16}
+  assembly: grid (static: a block per batch; persistent: a counter hands out
+            batches), particles per batch, neighbour chunk {32, 64} and
+            bucketing of the particles by row length (longest first)
 """
 from __future__ import annotations
 
@@ -71,16 +73,34 @@ def tune_sell(fn, pipelines=SELL_PIPELINES, reps=10, device=0):
     return best, results
 
 
-def tune_assembly(fn, ppws=(1, 2, 4, 8, 16), reps=2, device=0):
-    """fn() runs one assembly; returns (particles_per_wave, ms) of the fastest."""
+#: the default assembly launch: static grid, 8 particles per batch, chunks of
+#: 32 neighbours, index order
+ASSEMBLY_DEFAULT = (0, 8, 32, 0)
+
+#: (persistent, particles per batch, chunk, bucket width) tried by tune_assembly
+ASSEMBLY_SPACE = tuple((pers, ppb, ch, bw) for pers in (0, 1) for ppb in (4, 8, 16) for ch in (32, 64)
+                       for bw in ((0, 16) if pers else (0,)))
+
+
+def set_assembly_launch(persistent=0, particles_per_batch=8, chunk=32, bucket_width=0, device=0):
+    """Launch shape of the assembly kernels (dcpse_set_assembly_launch); the
+    results do not depend on it."""
     ctx = _lib.context(device)
+    ctx.check(ctx.lib.dcpse_set_assembly_launch(ctx.h, int(persistent), int(particles_per_batch), int(chunk),
+                                                int(bucket_width)), "dcpse_set_assembly_launch")
+
+
+def tune_assembly(fn, space=ASSEMBLY_SPACE, reps=2, device=0):
+    """fn() runs one assembly; every launch shape of ``space`` (persistent,
+    particles per batch, chunk, bucket width) is timed with HIP events and the
+    fastest is kept in the library context. Returns ((shape), ms), results."""
     best = None
     results = {}
-    for p in ppws:
-        ctx.check(ctx.lib.dcpse_set_assembly_config(ctx.h, 1, p), "dcpse_set_assembly_config")
+    for cfg in space:
+        set_assembly_launch(*cfg, device=device)
         ms = _time(fn, reps)
-        results[p] = ms
+        results[cfg] = ms
         if best is None or ms < best[1]:
-            best = (p, ms)
-    ctx.check(ctx.lib.dcpse_set_assembly_config(ctx.h, 1, best[0]), "dcpse_set_assembly_config")
+            best = (cfg, ms)
+    set_assembly_launch(*best[0], device=device)
     return best, results

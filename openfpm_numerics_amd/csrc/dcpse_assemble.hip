@@ -51,11 +51,12 @@ struct AsmArgs {
     unsigned periodic;
     double h_over_eps;
     int refine;
-    int ppb;
+    int ppb;              // particles per batch (one wavefront walks a batch)
+    const int32_t* order; // batch b holds particles order[b * ppb ...]; nullptr: b * ppb ...
+    unsigned* queue;      // persistent launch: counter handing out batches; nullptr: batch = block
+    int64_t n_batches;
     AsmOp op[2];  // op[1] used by the two-operator kernels
 };
-
-constexpr int kChunk = 32;  // neighbours staged per LDS round
 
 template <int DIM, int L, int AMIN>
 __constant__ BasisTables<DIM, L, AMIN> kTab = BasisTables<DIM, L, AMIN>();
@@ -361,7 +362,7 @@ __device__ __forceinline__ void asm_weights(const AsmOp& op, int64_t p, int64_t 
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
             z[d] = dv[d] * ieps;
-            z2 += z[d] * z[d];
+            z2 = fma(z[d], z[d], z2);  // as the moment pass forms it
         }
         const double w = j < kWave ? wsl[j] : exp(-z2);
         double px[L], py[L], pz[L];
@@ -410,7 +411,7 @@ constexpr int asm_waves() {
 // moments are computed one particle at a time by the whole wave; when the
 // bases fit in half a wave (m <= 32), two particles' systems are factored and
 // solved together, one per half.
-template <int DIM, int L1, int A1, int L2 = 0, int A2 = 0>
+template <int DIM, int L1, int A1, int L2 = 0, int A2 = 0, int CH = 32, bool PQ = false>
 __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_assemble(const AsmArgs a) {
     constexpr int P1 = 2 * (L1 - 1), P2 = L2 > 0 ? 2 * (L2 - 1) : 0;
     constexpr int P = P1 > P2 ? P1 : P2;
@@ -432,13 +433,14 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
     // [w z^0..w z^P] (2D: [x][y^0..y^P][x^T y^b..][w])
     constexpr int ZW = DIM == 3 ? PW : 1;
     constexpr int TROW = 1 + PW + NX + ZW;
-    constexpr int TAB = (kChunk * TROW > SLOTS * MX * MSX) ? kChunk * TROW : SLOTS * MX * MSX;
+    static_assert(CH == 32 || CH == 64, "neighbour chunk of 32 or 64");
+    constexpr int TAB = (CH * TROW > SLOTS * MX * MSX) ? CH * TROW : SLOTS * MX * MSX;
     // lanes per neighbour (the segments of the moment pass fit them); a pass
     // covers HALVES = 64 / LPN neighbours
     constexpr int NSEG = Seg::N;
     constexpr int LPN = Seg::LPN;
     constexpr int HALVES = kWave / LPN;
-    static_assert(kChunk % (2 * HALVES) == 0, "a chunk holds whole passes");
+    static_assert(CH % (2 * HALVES) == 0, "a chunk holds whole passes");
     static_assert(NSEG <= LPN, "moment segments exceed the lanes of a neighbour");
     __shared__ __align__(16) double tab[TAB];     // staged powers; L factors in the solve
     __shared__ double mom[SLOTS * MGP];           // moments of each slot's particle
@@ -465,21 +467,48 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
     const int ab1 = lslot * MGP + mom_row_base<DIM, L1, A1, GW>(lrow);
     const int ab2 = L2 > 0 ? lslot * MGP + mom_row_base<DIM, (L2 > 0 ? L2 : 2), A2, GW>(lrow) : 0;
 
-    // The block's particles: lane i holds row_ptr[pbeg + i] (i <= pcnt) and
-    // the position of particle pbeg + i, loaded once; a particle takes its
-    // values by v_readlane. Neighbour data is software-pipelined: while
-    // particle i's ε and moments are computed, the positions of the first 64
-    // neighbours of particle i + 1 and the columns of particle i + 2 are in
-    // flight, and are first used one particle later.
-    const int64_t pbeg = (int64_t)blockIdx.x * a.ppb;
+    // Batches of a.ppb particles. Static grid (PQ false): batch = block, its
+    // particles b * ppb + i. Persistent grid (PQ true): batches taken from a
+    // counter until none is left (every wave leaves the loop once the counter
+    // passes n_batches), particles order[b * ppb + i] when bucketed by row
+    // length. The static kernel runs the loop body once.
+    for (int64_t batch = blockIdx.x;;) {
+    if constexpr (PQ) {
+        unsigned bq = 0;
+        if (lane == 0) bq = atomicAdd(a.queue, 1u);
+        batch = (int64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)bq);  // wave-uniform (SGPR)
+    }
+    if (batch >= a.n_batches) break;
+    // The batch's particles: lane i holds particle i's row bounds and position
+    // (static: row_ptr[pbeg + i] for i <= pcnt; persistent: the particle's
+    // index, row start and length), loaded once; a particle takes its values
+    // by v_readlane. Neighbour data is software-pipelined: while particle i's
+    // ε and moments are computed, the positions of the first 64 neighbours of
+    // particle i + 1 and the columns of particle i + 2 are in flight, and are
+    // first used one particle later.
+    const int64_t pbeg = batch * a.ppb;
     const int pcnt = (int)max((int64_t)0, min((int64_t)a.ppb, a.n_dom - pbeg));
-    const int64_t rpv = lane <= pcnt ? a.row_ptr[pbeg + lane] : 0;
+    // (particle counts are below 2^31, so indices and row lengths are 32-bit)
+    int pidv = 0, nv = 0;
+    int64_t rsv = 0;
+    if constexpr (PQ) {
+        pidv = lane < pcnt ? (a.order ? a.order[pbeg + lane] : (int)(pbeg + lane)) : 0;
+        rsv = lane < pcnt ? a.row_ptr[pidv] : 0;
+        nv = lane < pcnt ? (int)(a.row_ptr[pidv + 1] - rsv) : 0;
+    } else {
+        pidv = (int)(pbeg + lane);
+        rsv = lane <= pcnt ? a.row_ptr[pbeg + lane] : 0;
+    }
     double xpv[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) xpv[d] = lane < pcnt ? a.pos_dom[d * a.n_dom + pbeg + lane] : 0.0;
+    for (int d = 0; d < DIM; ++d) xpv[d] = lane < pcnt ? a.pos_dom[d * a.n_dom + pidv] : 0.0;
+    auto row_bounds = [&](int i, int64_t& rs_, int& n_) {
+        rs_ = readlane_i64(rsv, i);
+        if constexpr (PQ) n_ = __builtin_amdgcn_readlane(nv, i);
+        else n_ = (int)(readlane_i64(rsv, i + 1) - rs_);
+    };
     auto row_of = [&](int i, int64_t& rs_, int& n_, double* xp_) {
-        rs_ = readlane_i64(rpv, i);
-        n_ = (int)(readlane_i64(rpv, i + 1) - rs_);
+        row_bounds(i, rs_, n_);
 #pragma unroll
         for (int d = 0; d < 3; ++d) xp_[d] = d < DIM ? readlane_d(xpv[d], i) : 0.0;
     };
@@ -487,8 +516,9 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
     // its last column)
     auto load_cols = [&](int i) -> unsigned {
         if (i >= pcnt) return 0u;
-        const int64_t rs_ = readlane_i64(rpv, i);
-        const int n_ = (int)(readlane_i64(rpv, i + 1) - rs_);
+        int64_t rs_;
+        int n_;
+        row_bounds(i, rs_, n_);
         if (n_ <= 0) return 0u;
         return (unsigned)a.col[rs_ + min(lane, n_ - 1)];
     };
@@ -518,7 +548,7 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
     }
     for (int i = 0; i < pcnt; i += SLOTS) {
         // per slot: row, position, ε and whether its operators are assembled
-        int64_t rs_s[SLOTS];
+        int64_t rs_s[SLOTS], pid_s[SLOTS];
         int n_s[SLOTS];
         double xp_s[SLOTS][3];
         double eps_s[SLOTS], ieps_s[SLOTS];
@@ -531,12 +561,16 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
             for (int d = 0; d < 3; ++d) xp_s[s][d] = xp[d];
             eps_s[s] = ieps_s[s] = 0.0;
             go_s[s] = false;
+            pid_s[s] = 0;
             const int ii = i + s;
             if (ii >= pcnt) {
                 n_s[s] = 0;
                 continue;
             }
-            const int64_t p = pbeg + ii;
+            // (no scalar load here: its wait would also hold back the LDS work)
+            int64_t p = pbeg + ii;
+            if constexpr (PQ) p = a.order ? (int64_t)__builtin_amdgcn_readlane(pidv, ii) : pbeg + ii;
+            pid_s[s] = p;
             int64_t rs_n = 0;
             int n_n = 0;
             double xp_n[3] = {0.0, 0.0, 0.0};
@@ -583,9 +617,9 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
                 double acc[ST];
 #pragma unroll
                 for (int k = 0; k < ST; ++k) acc[k] = 0.0;
-                for (int c0 = 0; c0 < n; c0 += kChunk) {
-                    const int cnt = min(kChunk, n - c0);
-                    const int jl = lane & (kChunk - 1);
+                for (int c0 = 0; c0 < n; c0 += CH) {
+                    const int cnt = min(CH, n - c0);
+                    const int jl = lane & (CH - 1);
                     double dv[3];
 #pragma unroll
                     for (int d = 0; d < DIM; ++d) dv[d] = __shfl(cdv[d], (c0 + jl) & (kWave - 1), kWave);
@@ -598,7 +632,9 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
                     for (int d = 0; d < DIM; ++d) z[d] = vj ? dv[d] * ieps : 0.0;
                     // neighbours 2i and 2i + 1 share a row of (even, odd) pairs
                     double* row = tab + (jl >> 1) * 2 * TROW + (jl & 1);
-                    if (lane < kChunk) {
+                    // a chunk of 32: lanes 0-31 stage the y powers, lanes 32-63
+                    // the w z powers of the same neighbour; of 64: every lane both
+                    if (CH == kWave || lane < CH) {
                         row[0] = z[0];
                         double pw = 1.0;
 #pragma unroll
@@ -616,10 +652,11 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
                                 xt *= z[1];
                             }
                         }
-                    } else {
+                    }
+                    if (CH == kWave || lane >= CH) {
                         double z2 = 0.0;
 #pragma unroll
-                        for (int d = 0; d < DIM; ++d) z2 += z[d] * z[d];
+                        for (int d = 0; d < DIM; ++d) z2 = fma(z[d], z[d], z2);  // explicit: no contraction choice
                         double pw = vj ? exp(-z2) : 0.0;
                         if (c0 + jl < kWave) wsl[s * kWave + c0 + jl] = pw;
 #pragma unroll
@@ -680,7 +717,7 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
             if (!any) {
 #pragma unroll
                 for (int s = 0; s < SLOTS; ++s)
-                    if (go_s[s]) asm_fail(op, pbeg + i + s, rs_s[s], n_s[s], DCPSE_INFO_TOO_FEW, lane);
+                    if (go_s[s]) asm_fail(op, pid_s[s], rs_s[s], n_s[s], DCPSE_INFO_TOO_FEW, lane);
                 return;
             }
             bool okl;
@@ -689,7 +726,7 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
 #pragma unroll
             for (int s = 0; s < SLOTS; ++s) {
                 if (!go_s[s]) continue;
-                const int64_t p = pbeg + i + s;
+                const int64_t p = pid_s[s];
                 if (n_s[s] < m) {
                     asm_fail(op, p, rs_s[s], n_s[s], DCPSE_INFO_TOO_FEW, lane);
                     continue;
@@ -703,6 +740,8 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
         weights(BasisTag<L1, A1>{}, a.op[0], M1, ab1, b1);
         if constexpr (L2 > 0) weights(BasisTag<(L2 > 0 ? L2 : 2), A2>{}, a.op[1], M2, ab2, b2);
     }
+    if constexpr (!PQ) break;
+    }  // batches
 }
 
 // ---------------------------------------------------------------------------
@@ -756,36 +795,112 @@ static void host_rhs(const HostBasis& B, const uint32_t* beta, double* b) {
     }
 }
 
-template <int DIM, int L1, int A1, int L2, int A2>
-static void launch_asm(const AsmArgs& args, int64_t blocks, hipStream_t s) {
-    k_assemble<DIM, L1, A1, L2, A2><<<(unsigned)blocks, kWave, 0, s>>>(args);
+// Static grid: one block (wavefront) per batch. Persistent grid: as many
+// wavefronts as the GPU holds at once (the kernel's occupancy times the CU
+// count), each taking batches from args.queue until none is left.
+template <int DIM, int L1, int A1, int L2, int A2, int CH>
+static hipError_t launch_asm(const AsmArgs& args, int num_cus, hipStream_t s) {
+    int64_t blocks = args.n_batches;
+    if (args.queue) {
+        int per_cu = 0;
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, k_assemble<DIM, L1, A1, L2, A2, CH, true>, kWave, 0);
+        if (e != hipSuccess) return e;
+        blocks = std::min<int64_t>(blocks, (int64_t)std::max(per_cu, 1) * num_cus);
+        if (blocks <= 0) return hipSuccess;
+        k_assemble<DIM, L1, A1, L2, A2, CH, true><<<(unsigned)blocks, kWave, 0, s>>>(args);
+        return hipGetLastError();
+    }
+    if (blocks <= 0) return hipSuccess;
+    k_assemble<DIM, L1, A1, L2, A2, CH, false><<<(unsigned)blocks, kWave, 0, s>>>(args);
+    return hipGetLastError();
 }
 
-typedef void (*AsmLauncher)(const AsmArgs&, int64_t, hipStream_t);
+typedef hipError_t (*AsmLauncher)(const AsmArgs&, int, hipStream_t);
 
 // Single-operator kernels for every basis the library supports, and
 // two-operator kernels for the pairs assembled together most often
 // (second derivatives or the Laplacian with first derivatives, and two
-// operators of one basis), at convergence order 2.
-static AsmLauncher pick_launcher(int dim, int L, int amin, int L2 = 0, int amin2 = 0) {
+// operators of one basis), at convergence order 2. Neighbour chunks of 32
+// for every kernel, of 64 for the order-2 ones.
+static AsmLauncher pick_launcher(int dim, int L, int amin, int L2 = 0, int amin2 = 0, int ch = 32) {
 #define DCPSE_ASM_CASE(D, LL, A) \
-    if (dim == D && L == LL && amin == A && L2 == 0) return &launch_asm<D, LL, A, 0, 0>;
+    if (dim == D && L == LL && amin == A && L2 == 0 && ch == 32) return &launch_asm<D, LL, A, 0, 0, 32>;
+#define DCPSE_ASM_CASE64(D, LL, A) \
+    DCPSE_ASM_CASE(D, LL, A)       \
+    if (dim == D && L == LL && amin == A && L2 == 0 && ch == 64) return &launch_asm<D, LL, A, 0, 0, 64>;
     DCPSE_ASM_CASE(2, 2, 0) DCPSE_ASM_CASE(2, 2, 1)
-    DCPSE_ASM_CASE(2, 3, 0) DCPSE_ASM_CASE(2, 3, 1)
-    DCPSE_ASM_CASE(2, 4, 0) DCPSE_ASM_CASE(2, 4, 1)
+    DCPSE_ASM_CASE64(2, 3, 0) DCPSE_ASM_CASE(2, 3, 1)
+    DCPSE_ASM_CASE(2, 4, 0) DCPSE_ASM_CASE64(2, 4, 1)
     DCPSE_ASM_CASE(2, 5, 0) DCPSE_ASM_CASE(2, 5, 1)
     DCPSE_ASM_CASE(2, 6, 0) DCPSE_ASM_CASE(2, 6, 1)
     DCPSE_ASM_CASE(3, 2, 0) DCPSE_ASM_CASE(3, 2, 1)
-    DCPSE_ASM_CASE(3, 3, 0) DCPSE_ASM_CASE(3, 3, 1)
-    DCPSE_ASM_CASE(3, 4, 0) DCPSE_ASM_CASE(3, 4, 1)
+    DCPSE_ASM_CASE64(3, 3, 0) DCPSE_ASM_CASE(3, 3, 1)
+    DCPSE_ASM_CASE(3, 4, 0) DCPSE_ASM_CASE64(3, 4, 1)
     DCPSE_ASM_CASE(3, 5, 0) DCPSE_ASM_CASE(3, 5, 1)
+#undef DCPSE_ASM_CASE64
 #undef DCPSE_ASM_CASE
-#define DCPSE_ASM_PAIR(D, LA, AA, LB, AB) \
-    if (dim == D && L == LA && amin == AA && L2 == LB && amin2 == AB) return &launch_asm<D, LA, AA, LB, AB>;
+#define DCPSE_ASM_PAIR(D, LA, AA, LB, AB)                                                  \
+    if (dim == D && L == LA && amin == AA && L2 == LB && amin2 == AB)                     \
+        return ch == 64 ? &launch_asm<D, LA, AA, LB, AB, 64> : &launch_asm<D, LA, AA, LB, AB, 32>;
     DCPSE_ASM_PAIR(2, 4, 1, 3, 0) DCPSE_ASM_PAIR(2, 3, 0, 3, 0) DCPSE_ASM_PAIR(2, 4, 1, 4, 1)
     DCPSE_ASM_PAIR(3, 4, 1, 3, 0) DCPSE_ASM_PAIR(3, 3, 0, 3, 0) DCPSE_ASM_PAIR(3, 4, 1, 4, 1)
 #undef DCPSE_ASM_PAIR
     return nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// batches bucketed by row length: particles of longer rows first, so the
+// persistent grid ends on short rows (longest-first scheduling); a stable
+// counting sort by tiles, bucket-major, keeps the particles of a bucket in
+// index order within each tile
+// ---------------------------------------------------------------------------
+constexpr int kBucketTile = 2048;
+
+__global__ __launch_bounds__(256) void k_asm_bucket_hist(const int64_t* __restrict__ row_ptr, int64_t n, int width,
+                                                         int nb, int64_t n_tiles, int32_t* __restrict__ counts) {
+    extern __shared__ int32_t h[];
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) h[b] = 0;
+    __syncthreads();
+    const int64_t t = blockIdx.x;
+    for (int64_t p = t * kBucketTile + threadIdx.x; p < min(n, (t + 1) * kBucketTile); p += blockDim.x) {
+        const int len = (int)(row_ptr[p + 1] - row_ptr[p]);
+        const int b = nb - 1 - min(len / width, nb - 1);  // longest rows in bucket 0
+        atomicAdd(&h[b], 1);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) counts[(int64_t)b * n_tiles + t] = h[b];
+}
+
+// One wavefront per tile writes its particles bucket by bucket in index
+// order: the tile's bucket numbers are staged in LDS once, and a ballot per
+// bucket and round of 64 particles gives each lane its rank among the tile's
+// particles of that bucket.
+__global__ __launch_bounds__(64) void k_asm_bucket_scatter(const int64_t* __restrict__ row_ptr, int64_t n, int width,
+                                                           int nb, int64_t n_tiles, const int64_t* __restrict__ start,
+                                                           int32_t* __restrict__ order) {
+    __shared__ uint16_t bk[kBucketTile];
+    const int64_t t = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int64_t p0 = t * kBucketTile, p1 = min(n, (t + 1) * kBucketTile);
+    const int cnt = (int)(p1 - p0);
+    for (int i = lane; i < cnt; i += kWave) {
+        const int len = (int)(row_ptr[p0 + i + 1] - row_ptr[p0 + i]);
+        bk[i] = (uint16_t)(nb - 1 - min(len / width, nb - 1));
+    }
+    wave_lds_sync();
+    for (int b = 0; b < nb; ++b) {
+        int64_t at = start[(int64_t)b * n_tiles + t];
+        const int64_t end = start[(int64_t)b * n_tiles + t + 1];
+        if (at == end) continue;
+        for (int i0 = 0; i0 < cnt; i0 += kWave) {
+            const int i = i0 + lane;
+            const bool mine = i < cnt && bk[i] == b;
+            const unsigned long long m = __ballot(mine);
+            if (mine) order[at + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)(p0 + i);
+            at += __popcll(m);
+        }
+    }
 }
 
 // The basis and summed right-hand side of one operator (n_sig signatures
@@ -847,6 +962,8 @@ static dcpse_status assemble_ops(dcpse_ctx* ctx, int32_t dim, const double* pos_
     if (!pos_sup || !row_ptr || n_sup < 0 || n_dom < 0 || n_ops < 1 || !n_sig || !betas)
         return fail(ctx, DCPSE_ERR_INVALID, "null pointer or bad size");
     if (!(h_over_eps > 0.0)) return fail(ctx, DCPSE_ERR_INVALID, "h_over_eps must be > 0");
+    if (n_sup >= ((int64_t)1 << 31) || n_dom >= ((int64_t)1 << 31))
+        return fail(ctx, DCPSE_ERR_RANGE, "particle counts must be < 2^31");
     for (int o = 0; o < n_ops; ++o)
         if (n_sig[o] < 1) return fail(ctx, DCPSE_ERR_INVALID, "operator %d has no signature", o);
     if (pos_dom == nullptr) {
@@ -894,32 +1011,72 @@ static dcpse_status assemble_ops(dcpse_ctx* ctx, int32_t dim, const double* pos_
     args.periodic = periodic_mask;
     args.h_over_eps = h_over_eps;
     args.refine = (flags & DCPSE_ASM_REFINE) ? 1 : 0;
-    args.ppb = std::min(63, std::max(1, ctx->asm_ppb));  // one lane per particle's row bound
+    args.ppb = std::min(64, std::max(1, ctx->asm_ppb));  // one lane per particle of a batch
     if (n_dom == 0) return DCPSE_OK;
-    const int64_t blocks = (n_dom + args.ppb - 1) / args.ppb;
+    args.n_batches = (n_dom + args.ppb - 1) / args.ppb;
+    const int ch = ctx->asm_chunk;
+    // launch shape (dcpse_set_assembly_launch): persistent grid with a batch
+    // counter, and particles bucketed by row length
+    const bool persistent = ctx->asm_persistent != 0 && n_ops <= 64;
+    const int bw = persistent ? ctx->asm_bucket : 0;  // the static grid runs index order
+    const int nb = bw > 0 ? std::max(1, (DCPSE_MAX_ROW + bw) / bw) : 0;
+    const int64_t n_tiles = (n_dom + kBucketTile - 1) / kBucketTile;
+    if (persistent || bw > 0) {
+        Workspace w;
+        const size_t need = 64 * 4 + (bw > 0 ? (size_t)n_dom * 4 + ((size_t)nb * n_tiles + 1) * 12 +
+                                                   scan_workspace_bytes((int64_t)nb * n_tiles) + 1024
+                                             : 0) + 1024;
+        dcpse_status st = reserve_workspace(ctx, need, &w);
+        if (st) return st;
+        unsigned* queues = w.take<unsigned>(64);
+        if (!queues) return fail(ctx, DCPSE_ERR_INVALID, "assembly workspace sizing error");
+        if (persistent) {
+            fill_i32(ctx->stream, reinterpret_cast<int32_t*>(queues), 64, 0);
+            args.queue = queues;
+        }
+        if (bw > 0) {
+            int32_t* order = w.take<int32_t>((size_t)n_dom);
+            int32_t* counts = w.take<int32_t>((size_t)nb * n_tiles + 1);
+            int64_t* start = w.take<int64_t>((size_t)nb * n_tiles + 1);
+            if (!start) return fail(ctx, DCPSE_ERR_INVALID, "assembly workspace sizing error");
+            k_asm_bucket_hist<<<(unsigned)n_tiles, 256, (size_t)nb * 4, ctx->stream>>>(row_ptr, n_dom, bw, nb,
+                                                                                      n_tiles, counts);
+            DCPSE_LAUNCH_CHECK(ctx);
+            if ((st = scan_counts(ctx, counts, (int64_t)nb * n_tiles, start, &w))) return st;
+            k_asm_bucket_scatter<<<(unsigned)n_tiles, kWave, 0, ctx->stream>>>(row_ptr, n_dom, bw, nb, n_tiles, start,
+                                                                              order);
+            DCPSE_LAUNCH_CHECK(ctx);
+            args.order = order;
+        }
+    }
     // operators in pairs sharing one pass where a two-operator kernel exists
-    // (larger basis first), one at a time otherwise
-    int o = 0;
+    // (larger basis first), one at a time otherwise; a basis without a kernel
+    // for the chosen chunk runs with chunks of 32
+    int o = 0, pass = 0;
     while (o < n_ops) {
         AsmLauncher pair = nullptr;
         int first = o, second = o + 1;
         if (o + 1 < n_ops) {
             if (bases[o + 1].L > bases[o].L || (bases[o + 1].L == bases[o].L && bases[o + 1].amin > bases[o].amin))
                 std::swap(first, second);
-            pair = pick_launcher(dim, bases[first].L, bases[first].amin, bases[second].L, bases[second].amin);
+            pair = pick_launcher(dim, bases[first].L, bases[first].amin, bases[second].L, bases[second].amin, ch);
         }
+        AsmLauncher launch;
         if (pair) {
             args.op[0] = ops[first];
             args.op[1] = ops[second];
-            pair(args, blocks, ctx->stream);
+            launch = pair;
             o += 2;
         } else {
             args.op[0] = ops[o];
             memset(&args.op[1], 0, sizeof(AsmOp));
-            pick_launcher(dim, bases[o].L, bases[o].amin)(args, blocks, ctx->stream);
+            launch = pick_launcher(dim, bases[o].L, bases[o].amin, 0, 0, ch);
+            if (!launch) launch = pick_launcher(dim, bases[o].L, bases[o].amin, 0, 0, 32);
             o += 1;
         }
-        DCPSE_LAUNCH_CHECK(ctx);
+        if (args.queue && pass > 0) args.queue += 1;  // a fresh counter per pass
+        DCPSE_HIP_CHECK(ctx, launch(args, ctx->num_cus, ctx->stream));
+        ++pass;
     }
     return DCPSE_OK;
 }

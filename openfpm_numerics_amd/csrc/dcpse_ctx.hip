@@ -91,8 +91,24 @@ dcpse_status dcpse_set_assembly_config(dcpse_ctx* ctx, int32_t waves_per_block, 
         return fail(ctx, DCPSE_ERR_UNSUPPORTED, "the assembly kernel runs one wavefront per block");
     if (particles_per_wave < 0 || particles_per_wave > 4096)
         return fail(ctx, DCPSE_ERR_INVALID, "particles_per_wave must be in [0, 4096]");
+    particles_per_wave = particles_per_wave > 64 ? 64 : particles_per_wave;
     ctx->asm_wpb = waves_per_block;
     ctx->asm_ppb = particles_per_wave > 0 ? particles_per_wave : 8;
+    return DCPSE_OK;
+}
+
+dcpse_status dcpse_set_assembly_launch(dcpse_ctx* ctx, int32_t persistent, int32_t particles_per_batch,
+                                       int32_t chunk, int32_t bucket_width) {
+    if (!ctx) return DCPSE_ERR_INVALID;
+    if (particles_per_batch < 0 || particles_per_batch > 64)
+        return fail(ctx, DCPSE_ERR_INVALID, "particles_per_batch must be in [0, 64]");
+    if (chunk != 32 && chunk != 64) return fail(ctx, DCPSE_ERR_INVALID, "chunk must be 32 or 64");
+    if (bucket_width < 0 || bucket_width > DCPSE_MAX_ROW)
+        return fail(ctx, DCPSE_ERR_INVALID, "bucket_width must be in [0, DCPSE_MAX_ROW]");
+    ctx->asm_persistent = persistent ? 1 : 0;
+    ctx->asm_ppb = particles_per_batch > 0 ? particles_per_batch : 8;
+    ctx->asm_chunk = chunk;
+    ctx->asm_bucket = bucket_width;
     return DCPSE_OK;
 }
 

@@ -10,6 +10,10 @@ returns host arrays, so the GPU path and the oracle see identical inputs.
                         random, rCut=2.4h, periodic (~1.89M particles, 35..60 neighbours)
   adaptive_3d           3D particles with spacing h(x) = h0 (1 + 3x) along x,
                         per-particle rCut = 2.4 h(x_p)
+  uneven_3d             the same construction with h(x) = h0 (1 + x) and one
+                        rCut for all particles, so supports grow from about 25
+                        neighbours at the coarse end to about 200 at the fine
+                        end (batched uneven-size assembly)
 """
 from __future__ import annotations
 
@@ -103,3 +107,19 @@ def sin_field(pos):
     dy = TWO_PI * c[:, 1] * s[:, 0] * (np.prod(s[:, 2:], axis=1) if pos.shape[1] > 2 else 1.0)
     lap = -pos.shape[1] * TWO_PI ** 2 * f
     return f, dx, dy, lap
+
+
+def uneven_3d(n_target=1_000_000, seed=6, h_ratio=2.1, n_min=25, jitter=0.1):
+    """adaptive_3d's particles (spacing from h0 to h_ratio * h0 along x) under a
+    single cut-off sized so the coarsest particles keep about n_min
+    neighbours: rcut = (3 n_min / (4 pi))^(1/3) * h_max, which gives the finest
+    about h_ratio^3 * n_min (25 -> about 200 at h_ratio 2.1)."""
+    w = adaptive_3d(n_target, seed=seed, h_ratio=h_ratio, jitter=jitter)
+    h_max = float(np.max(w["h"]))
+    rcut = (3.0 * n_min / (4.0 * np.pi)) ** (1.0 / 3.0) * h_max
+    # periodic in x too (the fine end meets the coarse one across the wrap), so
+    # no particle has a one-sided support with fewer than n_min neighbours
+    pos = w["pos"].copy()
+    pos[:, 0] = np.mod(pos[:, 0], 1.0)
+    return dict(pos=pos, h=w["h"], rcut=rcut, box_lo=[0.0, 0.0, 0.0], box_hi=[1.0, 1.0, 1.0], periodic=0b111)
+
