@@ -239,6 +239,32 @@ dcpse_status dcpse_apply_transpose_f64(dcpse_ctx* ctx, int64_t n_cols, const int
                                        const double* g, int64_t ld_g, int32_t n_fields,
                                        double* out, int64_t ld_out, double alpha, double beta);
 
+/* ---- diagnostics -------------------------------------------------------- */
+/* Per-particle diagnostics of an assembled operator (any argument of
+ * dcpse_assemble, plus its outputs eps and K64), the reference's DCPSE_VERBOSE
+ * statistics (src/DCPSE/Dcpse.hpp:786-795, minSpacing of
+ * src/DCPSE/Vandermonde.hpp:63-81), checkMomenta's moment sums (Dcpse.hpp:381-457)
+ * and the conditioning its conditionNumber measures (Dcpse.hpp:812-824):
+ *   residual[p]    = max_alpha |sum_q K_pq m_alpha(x_pq/eps_p) - b_alpha| / max|b|
+ *   cond1[p]       = 1-norm condition number estimate of the solved system
+ *                    A = (E V)^T (E V) (exact ||A||_1, Hager-Higham estimate of
+ *                    ||A^-1||_1 as LAPACK dpocon); +inf where A is not
+ *                    positive definite or the particle has fewer than m neighbours
+ *   min_spacing[p] = min_q |x_pq|_2
+ * (device arrays of n_dom; any may be NULL) and summary (host, 10 doubles;
+ * synchronises): neighbours min, mean, max; <h> = h_over_eps * mean eps;
+ * min spacing mean, min, max; residual max; cond1 max over finite values;
+ * the number of particles with an infinite cond1. A diagnostic pass, a few
+ * times the assembly's cost; bases of up to 64 monomials. */
+dcpse_status dcpse_operator_diagnostics(dcpse_ctx* ctx, int32_t dim,
+                                        const double* pos_sup, int64_t n_sup,
+                                        const double* pos_dom, int64_t n_dom,
+                                        const double* box_lo, const double* box_hi, uint32_t periodic_mask,
+                                        const int64_t* row_ptr, const int32_t* col,
+                                        int32_t n_sig, const uint32_t* betas, const double* coeffs,
+                                        uint32_t order, double h_over_eps, const double* eps, const double* K64,
+                                        double* residual, double* cond1, double* min_spacing, double* summary);
+
 /* ---- elementwise helper of the expression evaluation ------------------ */
 /* y[i] = alpha * x[i] + beta * y[i] for i < n (beta == 0 never reads y); the
  * device counterpart of the per-particle SUM/MUL nodes the reference's host

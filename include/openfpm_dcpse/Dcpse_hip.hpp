@@ -332,6 +332,41 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
         return mm;
     }
 
+    // The reference's DCPSE_VERBOSE statistics (Dcpse.hpp:786-795) with the
+    // largest moment residual and condition estimate of the solved systems
+    // (dcpse_operator_diagnostics); print = true writes the reference's line.
+    struct diagnostics_t {
+        double n_min, n_mean, n_max, h_mean, min_spacing_mean, min_spacing_min, min_spacing_max, residual_max,
+            cond1_max, not_factored;
+    };
+    diagnostics_t diagnostics(bool print = false) {
+        auto& c = openfpm_dcpse::context();
+        check_fresh();
+        double lo[dim], hi[dim];
+        for (unsigned d = 0; d < dim; ++d) {
+            lo[d] = (double)particles_.boxLow()[d];
+            hi[d] = (double)particles_.boxHigh()[d];
+        }
+        const double* pdom = nullptr;
+        if constexpr (two_sets) pdom = domain_->devPos();
+        double out[10] = {0};
+        c.check(dcpse_operator_diagnostics(c.get(), dim, particles_.devPos(), (int64_t)particles_.size_local(), pdom,
+                                           (int64_t)verletList_.size(), lo, hi, particles_.periodicMask(),
+                                           verletList_.devRowPtr(), verletList_.devCol(), (int32_t)coeffs_.size(),
+                                           sigs_.data(), coeffs_.data(), convergenceOrder_, (double)HOverEpsilon,
+                                           eps_.data(), K64_.size() ? K64_.data() : eps_.data(), nullptr, nullptr,
+                                           nullptr, out),
+                "dcpse_operator_diagnostics");
+        diagnostics_t r{out[0], out[1], out[2], out[3], out[4], out[5], out[6], out[7], out[8], out[9]};
+        if (print)
+            std::cout << "DCPSE Operator Construction Complete. The global avg spacing in the support <h> is: "
+                      << r.h_mean << " (c=" << HOverEpsilon << "). Avg:" << r.min_spacing_mean << " Range:["
+                      << r.min_spacing_min << "," << r.min_spacing_max << "]. Neighbours " << r.n_min << ".."
+                      << r.n_max << " (mean " << r.n_mean << "), moment residual " << r.residual_max
+                      << ", cond_1(A) up to " << r.cond1_max << std::endl;
+        return r;
+    }
+
     void save(const std::string& file) {
         mirror();
         std::ofstream out(file + "_0", std::ios::binary);

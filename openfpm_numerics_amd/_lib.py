@@ -62,6 +62,8 @@ SIGNATURES = {
                                          _f32, _f32]),
     "dcpse_apply_transpose_f64": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _vp, _i64,
                                          _f64, _f64]),
+    "dcpse_operator_diagnostics": (_i32, [_vp, _i32, _vp, _i64, _vp, _i64, _vp, _vp, _u32, _vp, _vp, _i32, _vp, _vp,
+                                          _u32, _f64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "dcpse_axpby_f32": (_i32, [_vp, _i64, _f32, _vp, _f32, _vp]),
     "dcpse_axpby_f64": (_i32, [_vp, _i64, _f64, _vp, _f64, _vp]),
     "dcpse_sell_count": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp]),

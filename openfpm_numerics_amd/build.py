@@ -11,7 +11,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdcpse_hip.so")
 SOURCES = ["dcpse_ctx.hip", "dcpse_support.hip", "dcpse_assemble.hip", "dcpse_apply.hip", "dcpse_sell.hip",
-           "dcpse_ode.hip", "dcpse_export.hip"]
+           "dcpse_ode.hip", "dcpse_export.hip", "dcpse_diag.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

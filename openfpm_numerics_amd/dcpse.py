@@ -441,6 +441,37 @@ class Dcpse:
             out.append((float(mom.min()), float(mom.max())))
         return out
 
+    def diagnostics(self):
+        """Per-particle diagnostics of the assembled operator (dcpse_operator_diagnostics):
+        the reference's DCPSE_VERBOSE statistics (Dcpse.hpp:786-795), checkMomenta's
+        moment residual (:381-457) and a 1-norm condition estimate of each solved
+        system. Returns (summary dict, {"residual", "cond1", "min_spacing"} device
+        tensors of the domain particle count)."""
+        sup = self.particlesSupport
+        dom = self.particlesDomain
+        vl = self.verletList
+        dev = sup.device
+        n_dom = vl.n_rows
+        res = torch.empty(max(n_dom, 1), dtype=torch.float64, device=dev)
+        cond = torch.empty_like(res)
+        msp = torch.empty_like(res)
+        summary = np.zeros(10)
+        betas = np.ascontiguousarray(np.array(self.signatures, dtype=np.uint32).ravel())
+        coeffs = np.ascontiguousarray(np.array(self.coeffs, dtype=np.float64))
+        lo = np.ascontiguousarray(sup.box_lo)
+        hi = np.ascontiguousarray(sup.box_hi)
+        _lib.context(dev.index or 0).call(
+            "dcpse_operator_diagnostics", sup.dims, _lib.ptr(sup.pos_soa), sup.size_local(),
+            None if self.same_set else _lib.ptr(dom.pos_soa), n_dom, lo.ctypes.data, hi.ctypes.data,
+            sup.periodic_mask, _lib.ptr(vl.row_ptr), _lib.ptr(vl.col) if vl.nnz else None, len(self.signatures),
+            betas.ctypes.data, coeffs.ctypes.data, self.convergenceOrder, float(self.HOverEpsilon),
+            _lib.ptr(self.localEps), _lib.ptr(self.calcKernels) if vl.nnz else _lib.ptr(self.localEps),
+            _lib.ptr(res), _lib.ptr(cond), _lib.ptr(msp), summary.ctypes.data)
+        keys = ("neighbours_min", "neighbours_mean", "neighbours_max", "h_mean", "min_spacing_mean",
+                "min_spacing_min", "min_spacing_max", "residual_max", "cond1_max", "not_factored")
+        info = dict(zip(keys, (float(v) for v in summary)))
+        return info, {"residual": res[:n_dom], "cond1": cond[:n_dom], "min_spacing": msp[:n_dom]}
+
     def DrawKernel(self, prop: torch.Tensor, p: int):
         """prop[q] += K_pq for every neighbour q of p."""
         vl = self.verletList

@@ -808,6 +808,31 @@ static bool parity_load_python() {
     return refused;
 }
 
+// The DCPSE_VERBOSE statistics line and diagnostics of an assembled operator:
+// on a lattice of spacing h the smallest neighbour distance is h and every
+// moment residual is at rounding level.
+static bool dcpse_diagnostics() {
+    const size_t sz[2] = {32, 32};
+    Box<2, double> box({0, 0}, {1, 1});
+    size_t bc[2] = {PERIODIC, PERIODIC};
+    const double h = 1.0 / 32;
+    const double rCut = 3.1 * h;
+    vector_dist_hip<2, double, 3> domain(0, box, bc, Ghost<2, double>(rCut));
+    auto it = domain.getGridIterator(sz);
+    while (it.isNext()) {
+        auto key = it.get();
+        domain.add();
+        domain.getLastPos()[0] = (key.get(0) + 0.5) * h;
+        domain.getLastPos()[1] = (key.get(1) + 0.5) * h;
+        ++it;
+    }
+    auto vl = domain.getVerlet<VL_NON_SYMMETRIC | VL_SKIP_REF_PART>(rCut);
+    Laplacian<decltype(vl)> Lap(domain, vl, 2, rCut);
+    const auto r = Lap.dcpse(domain).diagnostics(true);
+    return std::fabs(r.min_spacing_min - h) < 1e-12 && std::fabs(r.min_spacing_max - h) < 1e-12 &&
+           r.residual_max < 1e-10 && r.n_min == 28 && r.n_max == 28 && r.not_factored == 0 && r.cond1_max > 1;
+}
+
 int main(int argc, char** argv) {
     if (argc > 1) {
         g_dir = argv[1];
@@ -834,6 +859,7 @@ int main(int argc, char** argv) {
     run("dcpse_row_accessors", dcpse_row_accessors);
     run("dcpse_check_momenta", dcpse_check_momenta);
     run("dcpse_fp64_sell", dcpse_fp64_sell);
+    run("dcpse_diagnostics", dcpse_diagnostics);
     std::printf("%d failure(s)\n", failures);
     return failures ? 1 : 0;
 }

@@ -1,7 +1,8 @@
 """Runs the C++ drop-in program (tests/cpp/dropin_test.cpp): the operator
 classes of include/openfpm_dcpse used as the reference's own tests use theirs
 (Dcpse_2D/3D_test, dcpse_op_tests, dcpse_op_save_load, dcpse_op_test_lap),
-plus FP32 fields, in-place expressions, the row accessors, checkMomenta and FP64 applies on the sliced-ELL copy."""
+plus FP32 fields, in-place expressions, the row accessors, checkMomenta, FP64 applies on the sliced-ELL copy
+and the DCPSE_VERBOSE diagnostics."""
 import os
 import subprocess
 
@@ -19,7 +20,7 @@ def test_cpp_dropin_program(cuda, tmp_path):
     print(proc.stderr)
     assert proc.returncode == 0, proc.stdout[-4000:] + proc.stderr[-2000:]
     assert "0 failure(s)" in proc.stdout
-    assert proc.stdout.count("[  OK  ]") == 10
+    assert proc.stdout.count("[  OK  ]") == 11
 
 
 def _read(d):
