@@ -745,6 +745,194 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
 }
 
 // ---------------------------------------------------------------------------
+// generic assembly for bases without a compiled kernel (any order, up to 64
+// monomials): one wavefront per particle. Lane i owns monomial i: it sums row
+// i of A = sum_q w_q v_q v_q^T over the neighbours (taken one at a time, the
+// values v_q broadcast through LDS), the wave factors A by Cholesky in LDS,
+// solves A a = b (optionally one refinement step against A), and lanes take
+// neighbours again for K_q = w_q sum_i a_i v_i(z_q). The same mathematics as
+// k_assemble, without its moment grid and compile-time tables.
+// ---------------------------------------------------------------------------
+struct GenericArgs {
+    const double* pos_sup;
+    int64_t n_sup;
+    const double* pos_dom;
+    int64_t n_dom;
+    const int64_t* row_ptr;
+    const int32_t* col;
+    double len[3], ilen[3];
+    unsigned periodic;
+    double h_over_eps;
+    int refine;
+    int M;
+    const uint32_t* exps;  // M * dim exponents (device)
+    AsmOp op;
+};
+
+__device__ __forceinline__ double mono_rt(const uint32_t* e, const double* z, int dim) {
+    double v = 1.0;
+    for (int d = 0; d < dim; ++d)
+        for (uint32_t k = 0; k < e[d]; ++k) v *= z[d];
+    return v;
+}
+
+template <int DIM, int MB>
+__global__ __launch_bounds__(64) void k_assemble_generic(const GenericArgs a) {
+    extern __shared__ double gsh[];
+    double* A = gsh;              // M x M, row-major
+    double* vq = gsh + MB * MB;   // broadcast values, then the solution
+    uint32_t* ex = reinterpret_cast<uint32_t*>(vq + kWave);  // M * DIM exponents
+    const int lane = threadIdx.x;
+    const int M = a.M;
+    const bool act = lane < M;
+    for (int i = lane; i < M * DIM; i += kWave) ex[i] = a.exps[i];
+    wave_lds_sync();
+    const double bi = act ? a.op.b[lane] : 0.0;
+    for (int64_t p = blockIdx.x; p < a.n_dom; p += gridDim.x) {
+        const int64_t rs = a.row_ptr[p];
+        const int n = (int)(a.row_ptr[p + 1] - rs);
+        double xp[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) xp[d] = a.pos_dom[d * a.n_dom + p];
+        auto disp = [&](int j, double* x) {
+            const int q = a.col[rs + j];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+                x[d] = min_image_r(xp[d] - a.pos_sup[d * a.n_sup + q], a.len[d], a.ilen[d], (a.periodic >> d) & 1u);
+        };
+        // eps = mean |x_pq|_1 / h_over_eps (Vandermonde.hpp:63-81)
+        double l1 = 0.0;
+        for (int j = lane; j < n; j += kWave) {
+            double x[3];
+            disp(j, x);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) l1 += fabs(x[d]);
+        }
+        l1 = wave_sum(l1);
+        const double eps = n > 0 ? (l1 / n) / a.h_over_eps : 0.0;
+        if (n < M || !(eps > 0.0) || !isfinite(eps)) {
+            asm_fail(a.op, p, rs, n, n < M ? DCPSE_INFO_TOO_FEW : DCPSE_INFO_EPS_ZERO, lane);
+            continue;
+        }
+        const double ieps = 1.0 / eps;
+        double arow[MB];
+#pragma unroll
+        for (int j = 0; j < MB; ++j) arow[j] = 0.0;
+        for (int q = 0; q < n; ++q) {
+            double x[3] = {0.0, 0.0, 0.0}, z[3] = {0.0, 0.0, 0.0}, z2 = 0.0;
+            disp(q, x);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                z[d] = x[d] * ieps;
+                z2 = fma(z[d], z[d], z2);
+            }
+            const double w = exp(-z2);
+            const double v = act ? mono_rt(ex + lane * DIM, z, DIM) : 0.0;
+            vq[lane] = v;
+            wave_lds_sync();
+            const double wv = w * v;
+#pragma unroll
+            for (int j = 0; j < MB; ++j)
+                if (j < M) arow[j] = fma(wv, vq[j], arow[j]);
+            wave_lds_sync();
+        }
+        if (act) {
+#pragma unroll
+            for (int j = 0; j < MB; ++j)
+                if (j < M) A[lane * M + j] = arow[j];
+        }
+        wave_lds_sync();
+        // Cholesky in place (lower triangle); a pivot that is not positive
+        // and finite ends it (no weights, DCPSE_INFO_NOT_SPD)
+        bool ok = true;
+        for (int k = 0; k < M; ++k) {
+            const double dkk = A[k * M + k];
+            if (!(dkk > 0.0) || !isfinite(dkk)) {
+                ok = false;
+                break;
+            }
+            const double lkk = sqrt(dkk);
+            wave_lds_sync();
+            if (lane > k && lane < M) A[lane * M + k] /= lkk;
+            if (lane == k) A[k * M + k] = lkk;
+            wave_lds_sync();
+            if (lane > k && lane < M) {
+                const double lik = A[lane * M + k];
+                for (int j = k + 1; j <= lane; ++j) A[lane * M + j] = fma(-lik, A[j * M + k], A[lane * M + j]);
+            }
+            wave_lds_sync();
+        }
+        if (!ok) {
+            asm_fail(a.op, p, rs, n, DCPSE_INFO_NOT_SPD, lane);
+            wave_lds_sync();
+            continue;
+        }
+        auto solve = [&](double r) {
+            for (int k = 0; k < M; ++k) {
+                if (lane == k) r /= A[k * M + k];
+                const double yk = readlane_d(r, k);
+                if (lane > k && lane < M) r = fma(-A[lane * M + k], yk, r);
+            }
+            for (int k = M - 1; k >= 0; --k) {
+                if (lane == k) r /= A[k * M + k];
+                const double xk = readlane_d(r, k);
+                if (lane < k) r = fma(-A[k * M + lane], xk, r);
+            }
+            return act ? r : 0.0;
+        };
+        double xa = solve(bi);
+        if (a.refine) {
+            // one step against the unfactored A (row i still in arow)
+            double rr = bi;
+            for (int j = 0; j < M; ++j) {
+                const double xj = readlane_d(xa, j);
+#pragma unroll
+                for (int jj = 0; jj < MB; ++jj)
+                    if (jj == j) rr = fma(-arow[jj], xj, rr);
+            }
+            xa += solve(act ? rr : 0.0);
+        }
+        vq[lane] = xa;
+        wave_lds_sync();
+        // kernel weights K_q = w_q sum_i a_i v_i(z_q) (Dcpse.hpp:798-809)
+        const double eip = 1.0 / ipow_rt(eps, a.op.border);
+        bool bad = false;
+        for (int j = lane; j < n; j += kWave) {
+            double x[3] = {0.0, 0.0, 0.0}, z[3] = {0.0, 0.0, 0.0}, z2 = 0.0;
+            disp(j, x);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                z[d] = x[d] * ieps;
+                z2 = fma(z[d], z[d], z2);
+            }
+            double sum = 0.0;
+            for (int i = 0; i < M; ++i) sum = fma(vq[i], mono_rt(ex + i * DIM, z, DIM), sum);
+            const double K = sum * exp(-z2);
+            bad |= !isfinite(K);
+            if (a.op.K64) a.op.K64[rs + j] = K;
+            if (a.op.W64) a.op.W64[rs + j] = K * eip;
+            if (a.op.W32) a.op.W32[rs + j] = (float)(K * eip);
+        }
+        const bool anybad = __ballot(bad) != 0ull;
+        if (lane == 0) {
+            if (a.op.eps) a.op.eps[p] = eps;
+            if (a.op.eps_inv_pow) a.op.eps_inv_pow[p] = eip;
+            if (a.op.info) a.op.info[p] = anybad ? DCPSE_INFO_NONFINITE : DCPSE_INFO_OK;
+        }
+        wave_lds_sync();
+    }
+}
+
+template <int DIM, int MB>
+static hipError_t launch_generic(const GenericArgs& a, int num_cus, hipStream_t s) {
+    const int64_t blocks = std::min<int64_t>(a.n_dom, (int64_t)num_cus * 16);
+    if (blocks <= 0) return hipSuccess;
+    const size_t lds = (size_t)(MB * MB + kWave) * sizeof(double) + (size_t)MB * DIM * sizeof(uint32_t);
+    k_assemble_generic<DIM, MB><<<(unsigned)blocks, kWave, lds, s>>>(a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // host-side basis rules (the device templates enumerate the same way)
 // ---------------------------------------------------------------------------
 struct HostBasis {
@@ -983,10 +1171,9 @@ static dcpse_status assemble_ops(dcpse_ctx* ctx, int32_t dim, const double* pos_
         if (!setup_op(dim, n_sig[o], betas + off_sig * dim, coeffs ? coeffs + off_c : nullptr, order, op, bases[o]))
             return fail(ctx, DCPSE_ERR_INVALID, "combined signatures must share |beta| (one basis)");
         if (bases[o].m > 64)
-            return fail(ctx, DCPSE_ERR_UNSUPPORTED, "basis of %d monomials exceeds 64", bases[o].m);
-        if (!pick_launcher(dim, bases[o].L, bases[o].amin))
-            return fail(ctx, DCPSE_ERR_UNSUPPORTED, "no compiled assembly kernel for dim=%d |beta|+order=%d", dim,
-                        bases[o].L);
+            return fail(ctx, DCPSE_ERR_UNSUPPORTED,
+                        "basis of %d monomials exceeds 64 (dim=%d, |beta|+order=%d; up to 6 in 3D, 10 in 2D)",
+                        bases[o].m, dim, bases[o].L);
         op.eps = eps ? eps[o] : nullptr;
         op.eps_inv_pow = eps_inv_pow ? eps_inv_pow[o] : nullptr;
         op.K64 = K64 ? K64[o] : nullptr;
@@ -1015,6 +1202,58 @@ static dcpse_status assemble_ops(dcpse_ctx* ctx, int32_t dim, const double* pos_
     if (n_dom == 0) return DCPSE_OK;
     args.n_batches = (n_dom + args.ppb - 1) / args.ppb;
     const int ch = ctx->asm_chunk;
+    // bases without a compiled kernel: the generic kernel, one operator at a time
+    std::vector<int> fast;
+    for (int k = 0; k < n_ops; ++k) {
+        if (pick_launcher(dim, bases[k].L, bases[k].amin)) {
+            fast.push_back(k);
+            continue;
+        }
+        GenericArgs g;
+        memset(&g, 0, sizeof(g));
+        g.pos_sup = pos_sup;
+        g.n_sup = n_sup;
+        g.pos_dom = pos_dom;
+        g.n_dom = n_dom;
+        g.row_ptr = row_ptr;
+        g.col = col;
+        for (int d = 0; d < 3; ++d) {
+            g.len[d] = args.len[d];
+            g.ilen[d] = args.ilen[d];
+        }
+        g.periodic = periodic_mask;
+        g.h_over_eps = h_over_eps;
+        g.refine = args.refine;
+        g.M = bases[k].m;
+        g.op = ops[k];
+        Workspace gw;
+        dcpse_status gst = reserve_workspace(ctx, (size_t)g.M * dim * 4 + 1024, &gw);
+        if (gst) return gst;
+        uint32_t* d_ex = gw.take<uint32_t>((size_t)g.M * dim);
+        DCPSE_HIP_CHECK(ctx, hipMemcpyAsync(d_ex, bases[k].e.data(), (size_t)g.M * dim * 4, hipMemcpyHostToDevice,
+                                            ctx->stream));
+        g.exps = d_ex;
+        hipError_t e;
+        if (dim == 2) e = g.M <= 32 ? launch_generic<2, 32>(g, ctx->num_cus, ctx->stream)
+                                    : launch_generic<2, 64>(g, ctx->num_cus, ctx->stream);
+        else e = g.M <= 32 ? launch_generic<3, 32>(g, ctx->num_cus, ctx->stream)
+                           : launch_generic<3, 64>(g, ctx->num_cus, ctx->stream);
+        DCPSE_HIP_CHECK(ctx, e);
+        // the exponents are read before the next call reuses the workspace
+        DCPSE_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    if (fast.size() < (size_t)n_ops) {
+        std::vector<AsmOp> fops;
+        std::vector<HostBasis> fbases;
+        for (int k : fast) {
+            fops.push_back(ops[k]);
+            fbases.push_back(bases[k]);
+        }
+        ops.swap(fops);
+        bases.swap(fbases);
+        n_ops = (int32_t)ops.size();
+        if (n_ops == 0) return DCPSE_OK;
+    }
     // launch shape (dcpse_set_assembly_launch): persistent grid with a batch
     // counter, and particles bucketed by row length
     const bool persistent = ctx->asm_persistent != 0 && n_ops <= 64;
