@@ -419,7 +419,7 @@ def main():
         "fp64_error_vs_analytic": ana,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": ("k_apply_sell%s<2,%d,0%s>%s" % ("3" if (sell_pipe or (2, 2))[1] == 3 else "", (sell_pipe or (2, 2))[0],
+                     "kernel": ("k_apply_sell%s<2,%d,0%s>%s" % ("", (sell_pipe or (2, 2))[0],
                                                           ",true" if step.layout == "sell_bundle16" else "",
                                                           {"sell_bundle": " (interleaved bundle)",
                                                            "sell_bundle16": " (interleaved bundle, 16-bit columns)"}

@@ -370,10 +370,9 @@ dcpse_status dcpse_apply_transpose_bundle16_f32(dcpse_ctx* ctx, int64_t n, const
                                                 const int32_t* exc_col, const float* exc_W, int64_t n_exc,
                                                 const float* diag, const float* g, float* out, float alpha,
                                                 float beta);
-/* Software pipeline of the SELL apply: groups of 4 slots per batch and
- * stages. depth 2: (groups 2 or 4) the stream loads of one batch overlap the
- * gathers and FMAs of the other; depth 3: (groups 1 or 2) columns are loaded
- * two batches ahead and weights one batch ahead. Default (2, 2). */
+/* Software pipeline of the SELL apply: groups of 4 slots per batch (2 or 4)
+ * and stages (2: the stream loads of one batch overlap the gathers and FMAs
+ * of the other). Default (2, 2). */
 dcpse_status dcpse_set_sell_pipeline(dcpse_ctx* ctx, int32_t groups, int32_t depth);
 
 /* ---- sparse-matrix export -------------------------------------------- */

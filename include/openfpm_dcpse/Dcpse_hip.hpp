@@ -175,7 +175,8 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
             // on (building it costs what some 50 applies save)
             if (n > 0 && !b16_tried_ && ++n_apply32_ >= kSellAfter) build_bundle16();
             if (b16_) {
-                c.check(dcpse_apply_sell_bundle16_f32(c.get(), n, verletList_.devRowPtr(), slice_ptr_.data(),
+                c.check(dcpse_apply_sell_bundle16_f32(c.get(), n, verletList_.devRowPtr(),
+                                                      verletList_.sell(false).slice_ptr.data(),
                                                       row16_.data(), 1, bundle_.data(), exc_ptr_.data(),
                                                       exc_col_.data(), exc_W_.data(), n_exc_, f, nullptr, O, alpha,
                                                       beta),
@@ -190,9 +191,10 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
             double* O[1] = {out};
             if (n > 0 && !s64_tried_ && ++n_apply64_ >= kSellAfter) build_sell64();
             if (s64_) {
+                const auto& L = verletList_.sell(true);
                 const double* SW[1] = {sellW64_.data()};
-                c.check(dcpse_apply_sell_f64(c.get(), n, verletList_.devRowPtr(), slice64_.data(), row64_.data(),
-                                             col64_.data(), 1, SW, f, nullptr, O, alpha, beta),
+                c.check(dcpse_apply_sell_f64(c.get(), n, verletList_.devRowPtr(), L.slice_ptr.data(),
+                                             L.sell_row.data(), L.sell_col.data(), 1, SW, f, nullptr, O, alpha, beta),
                         "dcpse_apply_sell_f64");
                 return;
             }
@@ -450,63 +452,73 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
     const std::vector<uint32_t>& signatures() const { return sigs_; }
 
     // FP32 applies run on a sliced-ELL copy of the weights with 16-bit column
-    // offsets (dcpse_sell_count + dcpse_sell_bundle16_count/fill), built at
-    // the kSellAfter-th apply after each assembly or load; where it cannot be
-    // built the CSR kernels run instead.
+    // offsets (dcpse_sell_bundle16_count/fill over the Verlet list's shared
+    // slice structure), built at the kSellAfter-th apply after each assembly
+    // or load. Where it cannot be built (no device memory for it, or a row
+    // count beyond the 16-bit layout's range) the CSR kernel runs instead.
     void build_bundle16() {
         auto& c = openfpm_dcpse::context();
         b16_tried_ = true;
         b16_ = false;
-        const int64_t n = (int64_t)verletList_.size();
-        const int64_t n_lanes = 64 * ((n + 63) / 64);
-        slice_ptr_.resize(n_lanes / 64 + 1);
-        sell_row_.resize(n_lanes);
-        int64_t n_slots = 0, n_exc = 0;
-        if (dcpse_sell_count(c.get(), n, verletList_.devRowPtr(), slice_ptr_.data(), sell_row_.data(), &n_slots))
-            return;
-        exc_ptr_.resize(n_lanes + 1);
-        if (dcpse_sell_bundle16_count(c.get(), n, verletList_.devRowPtr(), verletList_.devCol(), sell_row_.data(),
-                                      exc_ptr_.data(), &n_exc))
-            return;
-        bundle_.resize((size_t)std::max<int64_t>(n_slots / 2 + n_slots, 4));
-        row16_.resize(n_lanes);
-        exc_col_.resize((size_t)std::max<int64_t>(n_exc, 1));
-        exc_W_.resize((size_t)std::max<int64_t>(n_exc, 1));
-        const float* W[1] = {W32_.data()};
-        if (dcpse_sell_fill_bundle16_f32(c.get(), n, verletList_.devRowPtr(), verletList_.devCol(), 1, W,
-                                         slice_ptr_.data(), sell_row_.data(), exc_ptr_.data(), n_exc,
-                                         bundle_.data(), row16_.data(), exc_col_.data(), exc_W_.data()))
-            return;
-        n_exc_ = n_exc;
-        b16_ = true;
+        try {
+            const auto& L = verletList_.sell(false);
+            const int64_t n = (int64_t)verletList_.size();
+            const int64_t n_lanes = 64 * ((n + 63) / 64);
+            const int64_t n_slots = L.n_slots;
+            int64_t n_exc = 0;
+            exc_ptr_.resize(n_lanes + 1);
+            if (dcpse_sell_bundle16_count(c.get(), n, verletList_.devRowPtr(), verletList_.devCol(),
+                                          L.sell_row.data(), exc_ptr_.data(), &n_exc))
+                return release_bundle16();
+            bundle_.resize((size_t)std::max<int64_t>(n_slots / 2 + n_slots, 4));
+            row16_.resize(n_lanes);
+            exc_col_.resize((size_t)std::max<int64_t>(n_exc, 1));
+            exc_W_.resize((size_t)std::max<int64_t>(n_exc, 1));
+            const float* W[1] = {W32_.data()};
+            if (dcpse_sell_fill_bundle16_f32(c.get(), n, verletList_.devRowPtr(), verletList_.devCol(), 1, W,
+                                             L.slice_ptr.data(), L.sell_row.data(), exc_ptr_.data(), n_exc,
+                                             bundle_.data(), row16_.data(), exc_col_.data(), exc_W_.data()))
+                return release_bundle16();
+            n_exc_ = n_exc;
+            b16_ = true;
+        } catch (const openfpm_dcpse::dcpse_error&) {
+            release_bundle16();
+        }
     }
 
-    // FP64 applies run on a sliced-ELL copy with 32-bit columns and FP64
-    // weights (dcpse_sell_count + dcpse_sell_fill_cols/fill_weights_f64),
-    // built at the kSellAfter-th FP64 apply after each assembly or load; where
-    // it cannot be built the CSR kernel runs instead.
+    // FP64 applies run on FP64 weights in the slot order of the Verlet list's
+    // shared sliced-ELL layout (32-bit columns), built at the kSellAfter-th
+    // FP64 apply after each assembly or load; where it cannot be built the
+    // CSR kernel runs instead.
     bool sell64_built() const { return s64_; }
     void build_sell64() {
         auto& c = openfpm_dcpse::context();
         s64_tried_ = true;
         s64_ = false;
-        const int64_t n = (int64_t)verletList_.size();
-        const int64_t n_lanes = 64 * ((n + 63) / 64);
-        slice64_.resize(n_lanes / 64 + 1);
-        row64_.resize(n_lanes);
-        int64_t n_slots = 0;
-        if (dcpse_sell_count(c.get(), n, verletList_.devRowPtr(), slice64_.data(), row64_.data(), &n_slots)) return;
-        col64_.resize((size_t)std::max<int64_t>(n_slots, 4));
-        sellW64_.resize((size_t)std::max<int64_t>(n_slots, 4));
-        if (dcpse_sell_fill_cols(c.get(), n, verletList_.devRowPtr(), verletList_.devCol(), slice64_.data(),
-                                 row64_.data(), col64_.data()) ||
-            dcpse_sell_fill_weights_f64(c.get(), n, verletList_.devRowPtr(), W64_.data(), slice64_.data(),
-                                        row64_.data(), sellW64_.data()))
-            return;
-        s64_ = true;
+        try {
+            const auto& L = verletList_.sell(true);
+            const int64_t n = (int64_t)verletList_.size();
+            sellW64_.resize((size_t)std::max<int64_t>(L.n_slots, 4));
+            if (dcpse_sell_fill_weights_f64(c.get(), n, verletList_.devRowPtr(), W64_.data(), L.slice_ptr.data(),
+                                            L.sell_row.data(), sellW64_.data())) {
+                sellW64_ = openfpm_dcpse::device_array<double>();
+                return;
+            }
+            s64_ = true;
+        } catch (const openfpm_dcpse::dcpse_error&) {
+            sellW64_ = openfpm_dcpse::device_array<double>();
+        }
     }
 
    private:
+    void release_bundle16() {
+        b16_ = false;
+        exc_ptr_ = openfpm_dcpse::device_array<int64_t>();
+        row16_ = openfpm_dcpse::device_array<int32_t>();
+        bundle_ = openfpm_dcpse::device_array<int32_t>();
+        exc_col_ = openfpm_dcpse::device_array<int32_t>();
+        exc_W_ = openfpm_dcpse::device_array<float>();
+    }
     // the reference apply's s: -1 for even |beta| (zero included), +1 for odd
     T apply_sign() const { return order_of_derivative() % 2 == 0 ? T(-1) : T(1); }
     size_t self_index(size_t p) const {
@@ -595,16 +607,14 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
     openfpm_dcpse::device_array<double> eps_, eip_, K64_, W64_;
     openfpm_dcpse::device_array<float> W32_;
     openfpm_dcpse::device_array<int32_t> info_;
-    // the 16-bit sliced-ELL copy used by FP32 applies
-    openfpm_dcpse::device_array<int64_t> slice_ptr_, exc_ptr_;
-    openfpm_dcpse::device_array<int32_t> sell_row_, row16_, bundle_, exc_col_;
+    // the 16-bit sliced-ELL copy used by FP32 applies (over the list's slices)
+    openfpm_dcpse::device_array<int64_t> exc_ptr_;
+    openfpm_dcpse::device_array<int32_t> row16_, bundle_, exc_col_;
     openfpm_dcpse::device_array<float> exc_W_;
     int64_t n_exc_ = 0;
     bool b16_ = false, b16_tried_ = false;
     int n_apply32_ = 0;  // FP32 applies since assembly
-    // the sliced-ELL copy used by FP64 applies
-    openfpm_dcpse::device_array<int64_t> slice64_;
-    openfpm_dcpse::device_array<int32_t> row64_, col64_;
+    // FP64 weights in the list's sliced-ELL slot order, used by FP64 applies
     openfpm_dcpse::device_array<double> sellW64_;
     bool s64_ = false, s64_tried_ = false;
     int n_apply64_ = 0;  // FP64 applies since assembly

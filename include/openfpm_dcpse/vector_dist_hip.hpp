@@ -6,6 +6,7 @@
 // boundaries are handled by minimum-image displacements instead of ghosts.
 #pragma once
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdint>
@@ -419,6 +420,41 @@ class VerletList_hip {
         return col_host_;
     }
 
+    // The sliced-ELL structure of this support (dcpse_sell_count: slices of 64
+    // rows, lane -> row) and, with cols, its 32-bit column array
+    // (dcpse_sell_fill_cols); built on first use and shared by every operator
+    // on this list, which keep only their own weights in its slot order.
+    // Throws dcpse_error when device memory runs out.
+    struct sell_layout {
+        device_array<int64_t> slice_ptr;
+        device_array<int32_t> sell_row, sell_col;
+        int64_t n_slots = -1;
+    };
+    const sell_layout& sell(bool cols) {
+        Context& c = context();
+        const int64_t n = (int64_t)n_rows_;
+        if (sell_.n_slots < 0) {
+            const int64_t n_lanes = 64 * ((n + 63) / 64);
+            sell_layout L;
+            L.slice_ptr.resize((size_t)(n_lanes / 64 + 1));
+            L.sell_row.resize((size_t)std::max<int64_t>(n_lanes, 64));
+            int64_t n_slots = 0;
+            c.check(dcpse_sell_count(c.get(), n, row_ptr_.data(), L.slice_ptr.data(), L.sell_row.data(), &n_slots),
+                    "dcpse_sell_count");
+            L.n_slots = n_slots;
+            sell_ = std::move(L);
+        }
+        if (cols && sell_.sell_col.size() == 0) {
+            device_array<int32_t> sc((size_t)std::max<int64_t>(sell_.n_slots, 4));
+            if (n)
+                c.check(dcpse_sell_fill_cols(c.get(), n, row_ptr_.data(), col_.data(), sell_.slice_ptr.data(),
+                                             sell_.sell_row.data(), sc.data()),
+                        "dcpse_sell_fill_cols");
+            sell_.sell_col = std::move(sc);
+        }
+        return sell_;
+    }
+
    private:
     static const double* pos_double(vector_type& parts) {
         static_assert(std::is_same<St, double>::value, "positions are double precision");
@@ -439,6 +475,7 @@ class VerletList_hip {
     device_array<int32_t> col_;
     std::vector<int64_t> rp_host_;
     std::vector<int32_t> col_host_;
+    sell_layout sell_;
 };
 
 }  // namespace openfpm_dcpse

@@ -53,7 +53,7 @@ def tune_apply(fn, shapes=APPLY_SHAPES, blocks=(256, 512), reps=10, device=0):
     return best, results
 
 
-SELL_PIPELINES = ((2, 2), (4, 2), (1, 3), (2, 3))  # (groups per batch, stages)
+SELL_PIPELINES = ((2, 2), (4, 2))  # (groups per batch, stages)
 
 
 def tune_sell(fn, pipelines=SELL_PIPELINES, reps=10, device=0):

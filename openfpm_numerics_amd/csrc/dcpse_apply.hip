@@ -23,14 +23,10 @@
 
 namespace dcpse {
 
-// Loads of the col/W streams. Every byte of them is used once per apply, but
-// rows are not aligned to cache lines, so a line can be split between two load
-// instructions; the policy decides whether the second one finds it in L2.
-#ifdef DCPSE_APPLY_NONTEMPORAL
-#define DCPSE_STREAM_LOAD(p) __builtin_nontemporal_load(p)
-#else
+// Loads of the col/W streams: ordinary cached loads (rows are not aligned to
+// cache lines, so a line split between two load instructions is found in L2 by
+// the second).
 #define DCPSE_STREAM_LOAD(p) (*(p))
-#endif
 
 constexpr int kMaxOps = 4;
 

@@ -512,109 +512,6 @@ __global__ __launch_bounds__(kSellWaves * kWave) void k_apply_sell(const SellArg
     if (live) sell_epilogue<NOPS, MODE>(a, row, fp, acc);
 }
 
-// Three-stage variant: columns are loaded two batches ahead, weights one batch
-// ahead, and the gathers of the next batch go out before the FMAs of the
-// current one. Per step a wavefront waits only for memory operations issued a
-// full step earlier (loads complete in issue order), at the cost of holding
-// two column, two weight and two gather batches in registers.
-template <int NOPS, int UNR>
-__device__ __forceinline__ void sell_load_cols(const int32_t* cp, int64_t gs, int g0, int len, vi4a (&q)[UNR]) {
-#pragma unroll
-    for (int u = 0; u < UNR; ++u) {
-        const int g = g0 + u;
-        q[u] = *(const vi4a*)(cp + (4 * g < len ? (int64_t)g * gs : 0));
-    }
-}
-
-template <int NOPS, int UNR>
-__device__ __forceinline__ void sell_load_w(const float* const* wp, int64_t gs, int g0, int len,
-                                            vf4a (&w)[NOPS][UNR]) {
-#pragma unroll
-    for (int u = 0; u < UNR; ++u) {
-        const int g = g0 + u;
-        const int64_t off = 4 * g < len ? (int64_t)g * gs : 0;
-#pragma unroll
-        for (int o = 0; o < NOPS; ++o) w[o][u] = *(const vf4a*)(wp[o] + off);
-    }
-}
-
-template <int UNR>
-__device__ __forceinline__ void sell_gather_q(const SellArgs& a, const vi4a (&q)[UNR], SellGathered<UNR>& g) {
-#pragma unroll
-    for (int u = 0; u < UNR; ++u) {
-        g.fq[u][0] = a.f[q[u].x];
-        g.fq[u][1] = a.f[q[u].y];
-        g.fq[u][2] = a.f[q[u].z];
-        g.fq[u][3] = a.f[q[u].w];
-    }
-}
-
-template <int NOPS, int UNR, int MODE>
-__device__ __forceinline__ void sell_fma_w(int g0, int len, float fp, const vf4a (&w)[NOPS][UNR],
-                                           const SellGathered<UNR>& g, double* acc) {
-#pragma unroll
-    for (int u = 0; u < UNR; ++u)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const bool v = 4 * (g0 + u) + i < len;
-            const double d = MODE == 1 ? (double)g.fq[u][i] : (double)g.fq[u][i] - (double)fp;
-#pragma unroll
-            for (int o = 0; o < NOPS; ++o) acc[o] = v ? fma((double)w[o][u][i], d, acc[o]) : acc[o];
-        }
-}
-
-template <int NOPS, int UNR, int MODE>
-__global__ __launch_bounds__(kSellWaves * kWave) void k_apply_sell3(const SellArgs a) {
-    const int64_t hb = blockIdx.x;
-    const int64_t lb = (hb & 7) * a.per_xcd + (hb >> 3);
-    if (lb >= a.blocks_needed) return;
-    const int lane = threadIdx.x & 63;
-    const int64_t s = lb * kSellWaves + (threadIdx.x >> 6);
-    if (s * 64 >= a.n_rows) return;  // whole wavefront
-    const int32_t row = a.sell_row[s * 64 + lane];
-    const bool live = row >= 0;
-    const int len = live ? (int)(a.row_ptr[row + 1] - a.row_ptr[row]) : 0;
-    const float fp = live ? a.f_self[row] : 0.0f;
-    const int64_t base = a.slice_ptr[s];
-    const int ng = (int)((a.slice_ptr[s + 1] - base) >> 8);  // groups of 4 slots
-    const int32_t* cp;
-    const float* wp[NOPS];
-    const int64_t gs = sell_streams<NOPS>(a, base, lane, cp, wp);
-    double acc[NOPS];
-#pragma unroll
-    for (int o = 0; o < NOPS; ++o) acc[o] = 0.0;
-    if (ng > 0) {
-        vi4a cX[UNR], cY[UNR];
-        vf4a wX[NOPS][UNR], wY[NOPS][UNR];
-        SellGathered<UNR> gX, gY;
-        sell_load_cols<NOPS, UNR>(cp, gs, 0, len, cX);
-        sell_load_cols<NOPS, UNR>(cp, gs, UNR, len, cY);
-        sell_load_w<NOPS, UNR>(wp, gs, 0, len, wX);
-        __builtin_amdgcn_sched_barrier(0);
-        sell_gather_q<UNR>(a, cX, gX);
-        __builtin_amdgcn_sched_barrier(0);
-        for (int g0 = 0; g0 < ng; g0 += 2 * UNR) {
-            // batch g0: W in wX, gathers in gX; batch g0+UNR: columns in cY
-            sell_load_cols<NOPS, UNR>(cp, gs, g0 + 2 * UNR, len, cX);
-            sell_load_w<NOPS, UNR>(wp, gs, g0 + UNR, len, wY);
-            __builtin_amdgcn_sched_barrier(0);
-            sell_gather_q<UNR>(a, cY, gY);
-            __builtin_amdgcn_sched_barrier(0);
-            sell_fma_w<NOPS, UNR, MODE>(g0, len, fp, wX, gX, acc);
-            __builtin_amdgcn_sched_barrier(0);
-            // batch g0+UNR: W in wY, gathers in gY; batch g0+2UNR: columns in cX
-            sell_load_cols<NOPS, UNR>(cp, gs, g0 + 3 * UNR, len, cY);
-            sell_load_w<NOPS, UNR>(wp, gs, g0 + 2 * UNR, len, wX);
-            __builtin_amdgcn_sched_barrier(0);
-            sell_gather_q<UNR>(a, cX, gX);
-            __builtin_amdgcn_sched_barrier(0);
-            sell_fma_w<NOPS, UNR, MODE>(g0 + UNR, len, fp, wY, gY, acc);
-            __builtin_amdgcn_sched_barrier(0);
-        }
-    }
-    if (live) sell_epilogue<NOPS, MODE>(a, row, fp, acc);
-}
-
 // FP64 forward apply from the sliced-ELL layout: the columns of the FP32
 // layout (sell_col) and FP64 weights in the same slot order. The pipeline is
 // that of k_apply_sell (two register batches; the gathers of the loaded batch
@@ -746,13 +643,12 @@ static bool dispatch_sell64(const Sell64Args& a, int nops, hipStream_t st) {
     return false;
 }
 
-template <int NOPS, int UNR, int DEPTH, int MODE>
+template <int NOPS, int UNR, int MODE>
 static void launch_sell(SellArgs a, hipStream_t st) {
     const int64_t n_slices = (a.n_rows + 63) / 64;
     a.blocks_needed = (n_slices + kSellWaves - 1) / kSellWaves;
     a.per_xcd = (a.blocks_needed + 7) / 8;
-    if (DEPTH == 3) k_apply_sell3<NOPS, UNR, MODE><<<(unsigned)(a.per_xcd * 8), kSellWaves * kWave, 0, st>>>(a);
-    else k_apply_sell<NOPS, UNR, MODE><<<(unsigned)(a.per_xcd * 8), kSellWaves * kWave, 0, st>>>(a);
+    k_apply_sell<NOPS, UNR, MODE><<<(unsigned)(a.per_xcd * 8), kSellWaves * kWave, 0, st>>>(a);
 }
 
 template <int NOPS, int UNR, int MODE = 0>
@@ -775,34 +671,27 @@ static bool dispatch_sell16(const SellArgs& a, int nops, int unr, hipStream_t st
     return false;
 }
 
-template <int UNR, int DEPTH, int MODE>
+template <int UNR, int MODE>
 static bool dispatch_sell_ops(const SellArgs& a, int nops, hipStream_t st) {
     if constexpr (MODE == 2) {  // stages apply one operator
         if (nops != 1) return false;
-        launch_sell<1, UNR, DEPTH, MODE>(a, st);
+        launch_sell<1, UNR, MODE>(a, st);
         return true;
     }
     switch (nops) {
-        case 1: launch_sell<1, UNR, DEPTH, MODE>(a, st); return true;
-        case 2: launch_sell<2, UNR, DEPTH, MODE>(a, st); return true;
-        case 3: launch_sell<3, UNR, DEPTH, MODE>(a, st); return true;
-        case 4: launch_sell<4, UNR, DEPTH, MODE>(a, st); return true;
+        case 1: launch_sell<1, UNR, MODE>(a, st); return true;
+        case 2: launch_sell<2, UNR, MODE>(a, st); return true;
+        case 3: launch_sell<3, UNR, MODE>(a, st); return true;
+        case 4: launch_sell<4, UNR, MODE>(a, st); return true;
     }
     return false;
 }
 
 template <int MODE>
-static bool dispatch_sell(const SellArgs& a, int nops, int unr, int depth, hipStream_t st) {
-    if (depth == 3) {
-        switch (unr) {
-            case 1: return dispatch_sell_ops<1, 3, MODE>(a, nops, st);
-            case 2: return dispatch_sell_ops<2, 3, MODE>(a, nops, st);
-        }
-    } else {
-        switch (unr) {
-            case 2: return dispatch_sell_ops<2, 2, MODE>(a, nops, st);
-            case 4: return dispatch_sell_ops<4, 2, MODE>(a, nops, st);
-        }
+static bool dispatch_sell(const SellArgs& a, int nops, int unr, hipStream_t st) {
+    switch (unr) {
+        case 2: return dispatch_sell_ops<2, MODE>(a, nops, st);
+        case 4: return dispatch_sell_ops<4, MODE>(a, nops, st);
     }
     return false;
 }
@@ -1018,8 +907,8 @@ static dcpse_status sell_apply(dcpse_ctx* ctx, int mode, int64_t n_rows, const i
     a.y_a = a.y_b = nullptr;
     a.out_a = a.out_b = nullptr;
     a.ca = a.cb = 0.0f;
-    const bool ok = mode == 0 ? dispatch_sell<0>(a, n_ops, ctx->sell_groups, ctx->sell_depth, ctx->stream)
-                              : dispatch_sell<1>(a, n_ops, ctx->sell_groups, ctx->sell_depth, ctx->stream);
+    const bool ok = mode == 0 ? dispatch_sell<0>(a, n_ops, ctx->sell_groups, ctx->stream)
+                              : dispatch_sell<1>(a, n_ops, ctx->sell_groups, ctx->stream);
     if (!ok) return fail(ctx, DCPSE_ERR_UNSUPPORTED, "no SELL kernel for %d operators", n_ops);
     DCPSE_LAUNCH_CHECK(ctx);
     return DCPSE_OK;
@@ -1176,7 +1065,7 @@ dcpse_status dcpse_stage_sell_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t*
     a.out_b = out_b;
     a.ca = ca;
     a.cb = cb;
-    if (!dispatch_sell<2>(a, 1, ctx->sell_groups, ctx->sell_depth, ctx->stream))
+    if (!dispatch_sell<2>(a, 1, ctx->sell_groups, ctx->stream))
         return fail(ctx, DCPSE_ERR_UNSUPPORTED, "no SELL stage kernel for this pipeline");
     DCPSE_LAUNCH_CHECK(ctx);
     return DCPSE_OK;
@@ -1276,8 +1165,8 @@ dcpse_status dcpse_apply_transpose_bundle16_f32(dcpse_ctx* ctx, int64_t n, const
 
 dcpse_status dcpse_set_sell_pipeline(dcpse_ctx* ctx, int32_t groups, int32_t depth) {
     if (!ctx) return DCPSE_ERR_INVALID;
-    const bool ok = (depth == 2 && (groups == 2 || groups == 4)) || (depth == 3 && (groups == 1 || groups == 2));
-    if (!ok) return fail(ctx, DCPSE_ERR_INVALID, "(groups, depth) must be (2,2), (4,2), (1,3) or (2,3)");
+    const bool ok = depth == 2 && (groups == 2 || groups == 4);
+    if (!ok) return fail(ctx, DCPSE_ERR_INVALID, "(groups, depth) must be (2, 2) or (4, 2)");
     ctx->sell_groups = groups;
     ctx->sell_depth = depth;
     return DCPSE_OK;

@@ -684,6 +684,13 @@ class PreparedApply:
             self.fn = self.ctx.lib.dcpse_apply_sell_f32
             self.args = (self.ctx.h, vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(slice_ptr), _lib.ptr(sell_row),
                          _lib.ptr(sell_col), len(ops), self.W, _lib.ptr(f2), None, self.O, alpha, beta)
+        elif sell and f.dtype == torch.float64 and nb == 1:
+            slice_ptr, sell_row, sell_col = vl.sell()
+            self._keep = (ops, f, outs, [op.sell_weights64() for op in ops], slice_ptr, sell_row, sell_col)
+            self.W = _lib.ptr_array(self._keep[3])
+            self.fn = self.ctx.lib.dcpse_apply_sell_f64
+            self.args = (self.ctx.h, vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(slice_ptr), _lib.ptr(sell_row),
+                         _lib.ptr(sell_col), len(ops), self.W, _lib.ptr(f2), None, self.O, alpha, beta)
         else:
             self._keep = (ops, f, outs, [op.weights(f.dtype) for op in ops])
             self.fn = self.ctx.lib.dcpse_apply_f32 if f.dtype == torch.float32 else self.ctx.lib.dcpse_apply_f64
@@ -693,7 +700,8 @@ class PreparedApply:
         lib = self.ctx.lib
         self.layout = next((name for fn, name in ((lib.dcpse_apply_sell_f32, "sell"),
                                                   (lib.dcpse_apply_sell_bundle_f32, "sell_bundle"),
-                                                  (lib.dcpse_apply_sell_bundle16_f32, "sell_bundle16"))
+                                                  (lib.dcpse_apply_sell_bundle16_f32, "sell_bundle16"),
+                                                  (lib.dcpse_apply_sell_f64, "sell64"))
                             if self.fn is fn), "csr")
         self.sell = self.layout != "csr"
         self.ctx.bind_current_stream()

@@ -400,7 +400,7 @@ def test_sell_layout_apply(name, oracle, cuda):
     sell_after = D.SELL_AFTER
     D.SELL_AFTER = 1  # the sliced-ELL copies from the first apply on
     try:
-        for gd in ((2, 2), (4, 2), (1, 3), (2, 3)):
+        for gd in ((2, 2), (4, 2)):
             print(f"[sell] {name} groups,depth={gd}", flush=True)
             ctx.check(ctx.lib.dcpse_set_sell_pipeline(ctx.h, *gd), "pipeline")
             a, b = fused(lap, dx, f=ft)
@@ -669,3 +669,23 @@ def test_refinement_step_agrees(name, cuda):
         rel = float(np.max(num / np.maximum(den, 1e-300)))
         print(f"[refine] {name} {sigs}: row-relative {rel:.2e}", flush=True)
         assert rel <= 1e-12
+
+
+def test_prepared_apply_fp64_runs_the_sliced_ell_copy(cuda):
+    """PreparedApply with FP64 fields binds the FP64 sliced-ELL kernel and gives
+    the bits of the operators' own FP64 apply."""
+    from openfpm_numerics_amd import Derivative_x, Laplacian, fused, workloads as W
+    from openfpm_numerics_amd.dcpse import PreparedApply
+
+    w = W.jittered_2d(96, seed=8)
+    ps = _ps(w, cuda)
+    vl = ps.getVerlet(w["rcut"])
+    ops = [Laplacian(ps, vl, 2, w["rcut"]).dcpse, Derivative_x(ps, vl, 2, w["rcut"]).dcpse]
+    f = torch.randn(ps.size_local(), generator=torch.Generator().manual_seed(3), dtype=torch.float64).to(cuda)
+    outs = [torch.empty_like(f) for _ in ops]
+    p = PreparedApply(ops, f, outs, sell=True)
+    assert p.layout == "sell64"
+    p()
+    want = fused(*ops, f=f)
+    for o, wv in zip(outs, want):
+        assert torch.equal(o, wv)
