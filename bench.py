@@ -55,21 +55,25 @@ def events_ms(fn, reps, torch):
     return sum(d) / reps, d
 
 
-def read_traffic():
-    """HBM bytes per apply launch from the committed rocprofv3 --pmc summary, if any."""
-    path = os.path.join(ROOT, "profiles", "apply_fused_lap_dx_3d128.pmc.txt")
+def read_profile_value(name, key):
+    """A per-launch figure (``key = value`` line) of a committed rocprofv3 --pmc summary, if any."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
-    vals = {}
     with open(path) as fh:
         for line in fh:
-            if "=" in line and line.strip().startswith("traffic_"):
-                k, v = line.strip().split("=", 1)
+            k, sep, v = line.strip().partition("=")
+            if sep and k.strip() == key:
                 try:
-                    vals[k.strip()] = float(v.split()[0])
-                except ValueError:
-                    pass
-    return vals.get("traffic_bytes_per_launch")
+                    return float(v.split()[0])
+                except (ValueError, IndexError):
+                    return None
+    return None
+
+
+def read_traffic():
+    """HBM bytes per fused apply launch (FETCH_SIZE + WRITE_SIZE, calibrated)."""
+    return read_profile_value("apply_fused_lap_dx_3d128.pmc.txt", "traffic_bytes_per_launch")
 
 
 def main():
@@ -312,6 +316,31 @@ def main():
         d = got.double() - ref
         err[name] = {"rel_l2": float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(ref)),
                      "max_rel": float(d.abs().max() / ref.abs().max())}
+    # the same step with every operation of the row sums in FP32 (the
+    # reference's float-property apply); FP64 sums stay the default
+    from openfpm_numerics_amd import set_apply_accumulation
+    acc32_out = [torch.empty_like(out_lap), torch.empty_like(out_dx)]
+    step32 = PreparedApply([lap.dcpse, dx.dcpse], f32_t, acc32_out,
+                           sell={"sell": True, "sell_bundle": "bundle", "sell_bundle16": "bundle16",
+                                 "csr": False}[args.layout])
+    acc32 = None
+    if step32.sell:
+        set_apply_accumulation(32)
+        try:
+            for _ in range(3):
+                step32()
+            acc32_ms, _ = events_ms(step32, max(20, args.steps // 2), torch)
+        finally:
+            set_apply_accumulation(64)
+        err32 = {}
+        for name, got, ref in (("laplacian", acc32_out[0], lap64), ("dx", acc32_out[1], dx64)):
+            d = got.double() - ref
+            err32[name] = {"rel_l2": float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(ref)),
+                           "max_rel": float(d.abs().max() / ref.abs().max())}
+        acc32 = {"launch_us": acc32_ms * 1e3, "evals_per_s_M": evals / (acc32_ms * 1e-3) / 1e6,
+                 "fp32_error_vs_fp64": err32}
+        log(f"[bench] FP32 accumulation: {acc32_ms * 1e3:.1f} us per launch, error {err32}")
+    del step32
     ana = {
         "laplacian_max_rel": float((lap64.cpu().numpy() - lap_exact).__abs__().max() / np.abs(lap_exact).max()),
         "dx_max_rel": float((dx64.cpu().numpy() - dx_exact).__abs__().max() / np.abs(dx_exact).max()),
@@ -326,6 +355,9 @@ def main():
     flops_lap = N * 3 * asm_flops(nmean, 19)
     flops_dx = N * asm_flops(nmean, 10)
     asm_tflops = (flops_lap + flops_dx) / (assembly_ms * 1e-3) / 1e12
+    # FP64 flops the pair kernel executes (64 lanes x (2 FMA + ADD + MUL) wave
+    # instructions, rocprofv3 counters of the default launch shape)
+    asm_exec = read_profile_value("assemble_lap_dx_3d128.pmc.txt", "executed_fp64_flop_per_launch")
 
     # ---- CPU baseline -----------------------------------------------------------------
     cpu = None
@@ -416,6 +448,7 @@ def main():
         "fp64_apply_us": fp64_apply_us,
         "fp64_apply_evals_per_s_M": 2 * N / fp64_apply_us,
         "fp32_error_vs_fp64": err,
+        "fp32_accumulation": acc32,
         "fp64_error_vs_analytic": ana,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -428,7 +461,12 @@ def main():
                      "launch_ms": launch_ms},
         "roofline_assembly": {"bound": "fp64", "achieved": asm_tflops, "peak": FP64_PEAK_TFLOPS,
                               "unit": "TFLOP/s", "frac": asm_tflops / FP64_PEAK_TFLOPS,
-                              "flops": flops_lap + flops_dx},
+                              "flops": flops_lap + flops_dx,
+                              "flops_basis": "reference algorithm (Laplacian as three operators)",
+                              "executed_flops": asm_exec,
+                              "executed_achieved": (asm_exec / (assembly_ms * 1e-3) / 1e12) if asm_exec else None,
+                              "executed_frac": ((asm_exec / (assembly_ms * 1e-3) / 1e12) / FP64_PEAK_TFLOPS
+                                                if asm_exec else None)},
         "cpu_baseline": cpu,
         "wall_s_timed_region": wall_s,
         "host_buffers": {"ms_per_step": pcie_ms, "value": evals / (pcie_ms * 1e-3) / 1e6,

@@ -374,6 +374,13 @@ dcpse_status dcpse_apply_transpose_bundle16_f32(dcpse_ctx* ctx, int64_t n, const
  * and stages (2: the stream loads of one batch overlap the gathers and FMAs
  * of the other). Default (2, 2). */
 dcpse_status dcpse_set_sell_pipeline(dcpse_ctx* ctx, int32_t groups, int32_t depth);
+/* Accumulation of the FP32 sliced-ELL forward applies (dcpse_apply_sell_f32,
+ * dcpse_apply_sell_bundle_f32, dcpse_apply_sell_bundle16_f32): 64 (default)
+ * sums the FP32 products (f_q - f_p) W_pq in FP64 and rounds each row once;
+ * 32 does every operation in FP32, differences included, as the reference's
+ * apply does for float properties (src/DCPSE/Dcpse.hpp:560-659). Transposed
+ * applies, stages and the CSR kernels always sum in FP64. */
+dcpse_status dcpse_set_apply_accumulation(dcpse_ctx* ctx, int32_t bits);
 
 /* ---- sparse-matrix export -------------------------------------------- */
 /* The assembled operator as a CSR matrix in the reference's value_nz

@@ -77,6 +77,7 @@ SIGNATURES = {
     "dcpse_apply_transpose_bundle16_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp,
                                                   _f32, _f32]),
     "dcpse_set_sell_pipeline": (_i32, [_vp, _i32, _i32]),
+    "dcpse_set_apply_accumulation": (_i32, [_vp, _i32]),
     "dcpse_sell_fill_bundle_f32": (_i32, [_vp, _i64, _vp, _vp, _i32, _vp, _vp, _vp, _vp]),
     "dcpse_apply_sell_bundle_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _f32, _f32]),
     "dcpse_sell_bundle16_count": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp]),

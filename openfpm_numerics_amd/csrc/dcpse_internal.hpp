@@ -35,6 +35,7 @@ struct dcpse_ctx {
     int apply_vec = 1;     // 1: 4-byte loads, 4: 16-byte loads (FP32 only)
     int apply_block = 256;
     int sell_groups = 2;   // SELL apply: groups of 4 slots per pipeline batch
+    int apply_acc = 64;    // FP32 SELL forward applies: row sums in FP64 (64) or FP32 (32)
     int sell_depth = 2;    // SELL apply: pipeline stages (2)
     const void* grp_rows = nullptr;  // cache of the automatic apply shape
     int64_t grp_n = -1;

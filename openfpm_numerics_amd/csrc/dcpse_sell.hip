@@ -323,9 +323,22 @@ struct SellArgs {
 
 // Row epilogue shared by the SELL kernels: MODE 0 forward, 1 transposed
 // (explicit diagonal), 2 time-integration stage.
-// The row sums are accumulated in FP64 (the FP32 products are exact in FP64):
-// at high resolution the difference-form terms cancel to first order, and
-// FP32 accumulation rounding, not the FP32 weights, dominated the error.
+// The row sums are accumulated in FP64 by default (the FP32 products are
+// exact in FP64): at high resolution the difference-form terms cancel to
+// first order, and FP32 accumulation rounding, not the FP32 weights,
+// dominates the error. The forward kernels can also accumulate in FP32
+// (ACC32, dcpse_set_apply_accumulation), differences included.
+template <int NOPS, int MODE>
+__device__ __forceinline__ void sell_epilogue(const SellArgs& a, int32_t row, float fp, const float* acc) {
+    static_assert(MODE == 0, "FP32 accumulation is a forward-apply option");
+#pragma unroll
+    for (int o = 0; o < NOPS; ++o) {
+        float val = acc[o] * a.alpha;
+        if (a.beta != 0.0f) val = fmaf(a.beta, a.out[o][row], val);
+        a.out[o][row] = val;
+    }
+}
+
 template <int NOPS, int MODE>
 __device__ __forceinline__ void sell_epilogue(const SellArgs& a, int32_t row, float fp, const double* acc) {
     if (MODE == 2) {  // both inputs are read before either output is written
@@ -436,18 +449,19 @@ __device__ __forceinline__ void sell_gather(const SellArgs& a, const SellBatch<N
     }
 }
 
-// The FMAs of groups [g0, g0+UNR); slots past the row are masked.
-template <int NOPS, int UNR, int MODE, bool C16 = false>
+// The FMAs of groups [g0, g0+UNR); slots past the row are masked. Acc is
+// double (default) or float (every operation in FP32).
+template <int NOPS, int UNR, int MODE, bool C16, typename Acc>
 __device__ __forceinline__ void sell_fma(int g0, int len, float fp, const SellBatch<NOPS, UNR, C16>& b,
-                                         const SellGathered<UNR>& g, double* acc) {
+                                         const SellGathered<UNR>& g, Acc* acc) {
 #pragma unroll
     for (int u = 0; u < UNR; ++u)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const bool v = 4 * (g0 + u) + i < len;
-            const double d = MODE == 1 ? (double)g.fq[u][i] : (double)g.fq[u][i] - (double)fp;
+            const Acc d = MODE == 1 ? (Acc)g.fq[u][i] : (Acc)g.fq[u][i] - (Acc)fp;
 #pragma unroll
-            for (int o = 0; o < NOPS; ++o) acc[o] = v ? fma((double)b.w[o][u][i], d, acc[o]) : acc[o];
+            for (int o = 0; o < NOPS; ++o) acc[o] = v ? fma((Acc)b.w[o][u][i], d, acc[o]) : acc[o];
         }
 }
 
@@ -457,8 +471,9 @@ __device__ __forceinline__ void sell_fma(int g0, int len, float fp, const SellBa
 // waiting for the gathers does not wait for the stream loads issued after
 // them: the next batch's col/W loads stay in flight across the FMAs, and its
 // gathers go out as soon as its columns arrive.
-template <int NOPS, int UNR, int MODE, bool C16 = false>
+template <int NOPS, int UNR, int MODE, bool C16 = false, bool ACC32 = false>
 __global__ __launch_bounds__(kSellWaves * kWave) void k_apply_sell(const SellArgs a) {
+    using Acc = typename std::conditional<ACC32, float, double>::type;
     const int64_t hb = blockIdx.x;
     const int64_t lb = (hb & 7) * a.per_xcd + (hb >> 3);
     if (lb >= a.blocks_needed) return;
@@ -477,9 +492,9 @@ __global__ __launch_bounds__(kSellWaves * kWave) void k_apply_sell(const SellArg
     const float* wp[NOPS];
     const int64_t gs = sell_streams<NOPS, C16>(a, base, lane, cp, wp);
     const int32_t rb = live ? row : 0;  // 16-bit offsets are relative to the lane's row
-    double acc[NOPS];
+    Acc acc[NOPS];
 #pragma unroll
-    for (int o = 0; o < NOPS; ++o) acc[o] = 0.0;
+    for (int o = 0; o < NOPS; ++o) acc[o] = 0;
     if (ng > 0) {  // a slice of empty rows has no slots to read
         SellBatch<NOPS, UNR, C16> bA, bB;
         SellGathered<UNR> gA, gB;
@@ -489,13 +504,13 @@ __global__ __launch_bounds__(kSellWaves * kWave) void k_apply_sell(const SellArg
             __builtin_amdgcn_sched_barrier(0);
             sell_load<NOPS, UNR, C16>(cp, wp, gs, g0 + UNR, len, bB);
             __builtin_amdgcn_sched_barrier(0);
-            sell_fma<NOPS, UNR, MODE, C16>(g0, len, fp, bA, gA, acc);
+            sell_fma<NOPS, UNR, MODE, C16, Acc>(g0, len, fp, bA, gA, acc);
             __builtin_amdgcn_sched_barrier(0);
             sell_gather<NOPS, UNR, C16>(a, bB, gB, rb);
             __builtin_amdgcn_sched_barrier(0);
             sell_load<NOPS, UNR, C16>(cp, wp, gs, g0 + 2 * UNR, len, bA);
             __builtin_amdgcn_sched_barrier(0);
-            sell_fma<NOPS, UNR, MODE, C16>(g0 + UNR, len, fp, bB, gB, acc);
+            sell_fma<NOPS, UNR, MODE, C16, Acc>(g0 + UNR, len, fp, bB, gB, acc);
             __builtin_amdgcn_sched_barrier(0);
         }
     }
@@ -503,9 +518,9 @@ __global__ __launch_bounds__(kSellWaves * kWave) void k_apply_sell(const SellArg
         if (has_exc) {  // the far neighbours of this row (rare; after every pipelined load)
             const int64_t i = s * 64 + lane;
             for (int64_t k = a.exc_ptr[i]; k < a.exc_ptr[i + 1]; ++k) {
-                const double d = MODE == 1 ? (double)a.f[a.exc_col[k]] : (double)a.f[a.exc_col[k]] - (double)fp;
+                const Acc d = MODE == 1 ? (Acc)a.f[a.exc_col[k]] : (Acc)a.f[a.exc_col[k]] - (Acc)fp;
 #pragma unroll
-                for (int o = 0; o < NOPS; ++o) acc[o] = fma((double)a.exc_W[o * a.n_exc + k], d, acc[o]);
+                for (int o = 0; o < NOPS; ++o) acc[o] = fma((Acc)a.exc_W[o * a.n_exc + k], d, acc[o]);
             }
         }
     }
@@ -643,35 +658,40 @@ static bool dispatch_sell64(const Sell64Args& a, int nops, hipStream_t st) {
     return false;
 }
 
-template <int NOPS, int UNR, int MODE>
+template <int NOPS, int UNR, int MODE, bool ACC32 = false>
 static void launch_sell(SellArgs a, hipStream_t st) {
     const int64_t n_slices = (a.n_rows + 63) / 64;
     a.blocks_needed = (n_slices + kSellWaves - 1) / kSellWaves;
     a.per_xcd = (a.blocks_needed + 7) / 8;
-    k_apply_sell<NOPS, UNR, MODE><<<(unsigned)(a.per_xcd * 8), kSellWaves * kWave, 0, st>>>(a);
+    k_apply_sell<NOPS, UNR, MODE, false, ACC32><<<(unsigned)(a.per_xcd * 8), kSellWaves * kWave, 0, st>>>(a);
 }
 
-template <int NOPS, int UNR, int MODE = 0>
+template <int NOPS, int UNR, int MODE = 0, bool ACC32 = false>
 static void launch_sell16(SellArgs a, hipStream_t st) {
     const int64_t n_slices = (a.n_rows + 63) / 64;
     a.blocks_needed = (n_slices + kSellWaves - 1) / kSellWaves;
     a.per_xcd = (a.blocks_needed + 7) / 8;
-    k_apply_sell<NOPS, UNR, MODE, true><<<(unsigned)(a.per_xcd * 8), kSellWaves * kWave, 0, st>>>(a);
+    k_apply_sell<NOPS, UNR, MODE, true, ACC32><<<(unsigned)(a.per_xcd * 8), kSellWaves * kWave, 0, st>>>(a);
 }
 
 // 16-bit-offset bundles: forward apply, two-stage pipeline (groups 2 or 4).
-static bool dispatch_sell16(const SellArgs& a, int nops, int unr, hipStream_t st) {
+template <bool ACC32>
+static bool dispatch_sell16_acc(const SellArgs& a, int nops, int unr, hipStream_t st) {
     const bool u4 = unr == 4;
     switch (nops) {
-        case 1: u4 ? launch_sell16<1, 4>(a, st) : launch_sell16<1, 2>(a, st); return true;
-        case 2: u4 ? launch_sell16<2, 4>(a, st) : launch_sell16<2, 2>(a, st); return true;
-        case 3: u4 ? launch_sell16<3, 4>(a, st) : launch_sell16<3, 2>(a, st); return true;
-        case 4: u4 ? launch_sell16<4, 4>(a, st) : launch_sell16<4, 2>(a, st); return true;
+        case 1: u4 ? launch_sell16<1, 4, 0, ACC32>(a, st) : launch_sell16<1, 2, 0, ACC32>(a, st); return true;
+        case 2: u4 ? launch_sell16<2, 4, 0, ACC32>(a, st) : launch_sell16<2, 2, 0, ACC32>(a, st); return true;
+        case 3: u4 ? launch_sell16<3, 4, 0, ACC32>(a, st) : launch_sell16<3, 2, 0, ACC32>(a, st); return true;
+        case 4: u4 ? launch_sell16<4, 4, 0, ACC32>(a, st) : launch_sell16<4, 2, 0, ACC32>(a, st); return true;
     }
     return false;
 }
 
-template <int UNR, int MODE>
+static bool dispatch_sell16(const SellArgs& a, int nops, int unr, bool acc32, hipStream_t st) {
+    return acc32 ? dispatch_sell16_acc<true>(a, nops, unr, st) : dispatch_sell16_acc<false>(a, nops, unr, st);
+}
+
+template <int UNR, int MODE, bool ACC32 = false>
 static bool dispatch_sell_ops(const SellArgs& a, int nops, hipStream_t st) {
     if constexpr (MODE == 2) {  // stages apply one operator
         if (nops != 1) return false;
@@ -679,19 +699,19 @@ static bool dispatch_sell_ops(const SellArgs& a, int nops, hipStream_t st) {
         return true;
     }
     switch (nops) {
-        case 1: launch_sell<1, UNR, MODE>(a, st); return true;
-        case 2: launch_sell<2, UNR, MODE>(a, st); return true;
-        case 3: launch_sell<3, UNR, MODE>(a, st); return true;
-        case 4: launch_sell<4, UNR, MODE>(a, st); return true;
+        case 1: launch_sell<1, UNR, MODE, ACC32>(a, st); return true;
+        case 2: launch_sell<2, UNR, MODE, ACC32>(a, st); return true;
+        case 3: launch_sell<3, UNR, MODE, ACC32>(a, st); return true;
+        case 4: launch_sell<4, UNR, MODE, ACC32>(a, st); return true;
     }
     return false;
 }
 
-template <int MODE>
+template <int MODE, bool ACC32 = false>
 static bool dispatch_sell(const SellArgs& a, int nops, int unr, hipStream_t st) {
     switch (unr) {
-        case 2: return dispatch_sell_ops<2, MODE>(a, nops, st);
-        case 4: return dispatch_sell_ops<4, MODE>(a, nops, st);
+        case 2: return dispatch_sell_ops<2, MODE, ACC32>(a, nops, st);
+        case 4: return dispatch_sell_ops<4, MODE, ACC32>(a, nops, st);
     }
     return false;
 }
@@ -907,8 +927,9 @@ static dcpse_status sell_apply(dcpse_ctx* ctx, int mode, int64_t n_rows, const i
     a.y_a = a.y_b = nullptr;
     a.out_a = a.out_b = nullptr;
     a.ca = a.cb = 0.0f;
-    const bool ok = mode == 0 ? dispatch_sell<0>(a, n_ops, ctx->sell_groups, ctx->stream)
-                              : dispatch_sell<1>(a, n_ops, ctx->sell_groups, ctx->stream);
+    const bool ok = mode == 1               ? dispatch_sell<1>(a, n_ops, ctx->sell_groups, ctx->stream)
+                    : ctx->apply_acc == 32 ? dispatch_sell<0, true>(a, n_ops, ctx->sell_groups, ctx->stream)
+                                           : dispatch_sell<0>(a, n_ops, ctx->sell_groups, ctx->stream);
     if (!ok) return fail(ctx, DCPSE_ERR_UNSUPPORTED, "no SELL kernel for %d operators", n_ops);
     DCPSE_LAUNCH_CHECK(ctx);
     return DCPSE_OK;
@@ -1026,7 +1047,7 @@ dcpse_status dcpse_apply_sell_bundle16_f32(dcpse_ctx* ctx, int64_t n_rows, const
     a.f_self = f_self ? f_self : f;
     a.alpha = alpha;
     a.beta = beta;
-    if (!dispatch_sell16(a, n_ops, ctx->sell_groups == 4 ? 4 : 2, ctx->stream))
+    if (!dispatch_sell16(a, n_ops, ctx->sell_groups == 4 ? 4 : 2, ctx->apply_acc == 32, ctx->stream))
         return fail(ctx, DCPSE_ERR_UNSUPPORTED, "no SELL kernel for %d operators", n_ops);
     DCPSE_LAUNCH_CHECK(ctx);
     return DCPSE_OK;
@@ -1160,6 +1181,13 @@ dcpse_status dcpse_apply_transpose_bundle16_f32(dcpse_ctx* ctx, int64_t n, const
     if (ctx->sell_groups == 4) launch_sell16<1, 4, 1>(a, ctx->stream);
     else launch_sell16<1, 2, 1>(a, ctx->stream);
     DCPSE_LAUNCH_CHECK(ctx);
+    return DCPSE_OK;
+}
+
+dcpse_status dcpse_set_apply_accumulation(dcpse_ctx* ctx, int32_t bits) {
+    if (!ctx) return DCPSE_ERR_INVALID;
+    if (bits != 32 && bits != 64) return fail(ctx, DCPSE_ERR_INVALID, "accumulation must be 32 or 64 bits");
+    ctx->apply_acc = bits;
     return DCPSE_OK;
 }
 

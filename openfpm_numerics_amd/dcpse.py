@@ -69,6 +69,15 @@ def basis(beta, order):
     return out[: m * b.size].reshape(m, b.size)
 
 
+def set_apply_accumulation(bits: int, device: int = 0):
+    """Row sums of the FP32 sliced-ELL forward applies (the kernels of every
+    FP32 single-field apply and of apply_ops' sliced-ELL path): 64 (default)
+    accumulates the FP32 products in FP64, 32 does every operation in FP32.
+    Transposed applies, RK4 stages and the CSR kernels always sum in FP64."""
+    ctx = _lib.context(device)
+    ctx.check(ctx.lib.dcpse_set_apply_accumulation(ctx.h, int(bits)), "dcpse_set_apply_accumulation")
+
+
 def rhs(beta, order):
     lib = _lib.load()
     b = np.ascontiguousarray(beta, dtype=np.uint32)
