@@ -94,7 +94,7 @@ dcpse_status dcpse_set_assembly_config(dcpse_ctx* ctx, int32_t waves_per_block, 
 /* Assembly launch (kernels of src/DCPSE/Dcpse.hpp:690-809; replaces the
  * grid-stride assembleLocalMatrices_gpu<<<numSMs, 256>>> of
  * src/DCPSE/Dcpse.cuh:668-669). A wavefront assembles batches of
- * particles_per_batch particles (1..64; 0 = 8), one particle's ε and moments
+ * particles_per_batch particles (1..64; 0 = 16), one particle's ε and moments
  * at a time over all 64 lanes, staging `chunk` (32 or 64) neighbours per LDS
  * round; two particles share the factorisation when the basis fits half a
  * wavefront. persistent != 0: one launch of as many wavefronts as the GPU

@@ -73,9 +73,9 @@ def tune_sell(fn, pipelines=SELL_PIPELINES, reps=10, device=0):
     return best, results
 
 
-#: the default assembly launch: static grid, 8 particles per batch, chunks of
-#: 32 neighbours, index order
-ASSEMBLY_DEFAULT = (0, 8, 32, 0)
+#: the default assembly launch: static grid, 16 particles per batch, chunks of
+#: 64 neighbours, index order
+ASSEMBLY_DEFAULT = (0, 16, 64, 0)
 
 #: (persistent, particles per batch, chunk, bucket width) tried by tune_assembly
 ASSEMBLY_SPACE = tuple((pers, ppb, ch, bw) for pers in (0, 1) for ppb in (4, 8, 16) for ch in (32, 64)

@@ -158,14 +158,26 @@ __device__ __forceinline__ double ipow_rt(double x, unsigned e) {
     return r;
 }
 
-// Lane i's offset of A row i in the moment grid of width GW: the moment index
-// of monomial alpha_i (lanes past the basis take row 0).
+// The moment grid of the moments M(a, b, c), a + b + c <= P (GW = P + 1):
+// M(a, b, c) sits at a + GW b + S2 c with S2 = P GW + 1. The index is linear
+// in the exponents, so A_ik = M(alpha_i + alpha_k) is at index(alpha_i) +
+// index(alpha_k), and injective on the simplex a + b + c <= P (slice c spans
+// at most (P - c) GW < S2): P S2 + 1 entries (259 for P = 6) instead of GW^3.
+__host__ __device__ constexpr int grid_s2(int GW) { return (GW - 1) * GW + 1; }
+__host__ __device__ constexpr int grid_size(int DIM, int GW) {
+    return DIM == 3 ? (GW - 1) * grid_s2(GW) + 1 : (GW - 1) * GW + 1;
+}
+template <int GW>
+__device__ __forceinline__ constexpr int grid_index(int a, int b, int c) { return a + GW * b + grid_s2(GW) * c; }
+
+// Lane i's offset of A row i in the moment grid: the moment index of
+// monomial alpha_i (lanes past the basis take row 0).
 template <int DIM, int L, int AMIN, int GW>
 __device__ __forceinline__ int mom_row_base(int lane) {
     constexpr int M = BasisShape<DIM, L, AMIN>::M;
     const int li = lane < M ? lane : 0;
     const int e2 = DIM == 3 ? kTab<DIM, L, AMIN>.e[li][2] : 0;
-    return kTab<DIM, L, AMIN>.e[li][0] + GW * kTab<DIM, L, AMIN>.e[li][1] + GW * GW * e2;
+    return grid_index<GW>(kTab<DIM, L, AMIN>.e[li][0], kTab<DIM, L, AMIN>.e[li][1], e2);
 }
 
 template <int LL, int AA>
@@ -190,7 +202,7 @@ __device__ __forceinline__ void asm_fail(const AsmOp& op, int64_t p, int64_t rs,
 // Moment-system factorisation and solve of one operator for the particles of
 // SLOTS lane groups at once (LS = 64 / SLOTS lanes each, one particle per
 // group): lane slot * LS + i holds row i of its particle's A = (M(alpha_i +
-// alpha_k))_k, read from the moment grid (width GW) at abase + offset(k).
+// alpha_k))_k, read from the moment grid (grid_index) at abase + offset(k).
 // A = L L^T by a right-looking Cholesky; forward and backward substitution;
 // optionally one step of FP64 iterative refinement against the moments. The
 // solution entry x_i is left in sx[lane]; okl is false when a pivot of the
@@ -214,7 +226,7 @@ __device__ __forceinline__ void asm_factor_solve(const double* mom, int abase, d
 
     double ar[M];
 #pragma unroll
-    for (int k = 0; k < M; ++k) ar[k] = mom[abase + E.e[k][0] + GW * E.e[k][1] + GW * GW * E.e[k][2]];
+    for (int k = 0; k < M; ++k) ar[k] = mom[abase + grid_index<GW>(E.e[k][0], E.e[k][1], E.e[k][2])];
 
     // ---- A = L L^T, right-looking; lane i holds row i ---------------------
     // One group: step k takes the pivot from lane k (v_readlane), scales
@@ -260,8 +272,10 @@ __device__ __forceinline__ void asm_factor_solve(const double* mom, int abase, d
         inv = inv * fma(-hd * inv, inv, 1.5);
         // a pivot that is not positive and finite makes inv NaN (negative or
         // NaN: the estimate is NaN; zero or +inf: 0 * inf in the Newton steps),
-        // and inv - inv is NaN exactly then
-        dfin += inv - inv;
+        // and fma(inv, 0, s) is s exactly when inv is finite (an explicit fma:
+        // inv - inv may be contracted with the product that forms inv into
+        // that product's rounding error)
+        dfin = fma(inv, 0.0, dfin);
         dinv = (me == k) ? inv : dinv;
         // lane k's own entry becomes dkk * inv = L_kk
         ar[k] *= inv;
@@ -330,7 +344,7 @@ __device__ __forceinline__ void asm_factor_solve(const double* mom, int abase, d
         double rr = b_lane;
 #pragma unroll
         for (int k = 0; k < M; ++k) {
-            const double aik = mom[abase + E.e[k][0] + GW * E.e[k][1] + GW * GW * E.e[k][2]];
+            const double aik = mom[abase + grid_index<GW>(E.e[k][0], E.e[k][1], E.e[k][2])];
             rr = fma(-aik, sx[base + k], rr);
         }
         xv += chol_solve(rr);
@@ -339,19 +353,83 @@ __device__ __forceinline__ void asm_factor_solve(const double* mom, int abase, d
     wave_lds_sync();
 }
 
-// Kernel weights of one operator for particle p (Dcpse.hpp:798-809): lanes
-// own neighbours, K_pq = w_q * sum_i a_i z_q^alpha_i with the solution a in
-// xs (wave-uniform), written with the eps^-|beta| pre-scaled copies; the
-// displacements and weights w of the first 64 neighbours come from
-// dsl[d * 64 + lane] and wsl[lane] (as the moment pass computed them).
-template <int DIM, int L, int AMIN, typename Disp>
-__device__ __forceinline__ void asm_weights(const AsmOp& op, int64_t p, int64_t rs, int n, double eps, double ieps,
-                                            bool ok, const double* xs, const double* dsl, const double* wsl,
-                                            Disp disp, int lane) {
-    constexpr int M = BasisShape<DIM, L, AMIN>::M;
-    constexpr BasisTables<DIM, L, AMIN> E{};
-    const double eip = 1.0 / ipow_rt(eps, op.border);
-    bool bad = false;
+// Horner scheme of a basis polynomial sum_i x_i z^alpha_i: the monomials
+// grouped by their (y, z) exponents (b, c); each group's x exponents are
+// consecutive, a_min .. a_max, so the group is
+//   y^b z^c x^a_min (x_max + x (... )) evaluated innermost first.
+// idx lists the monomial indices of each group from the highest a down.
+template <int DIM, int L, int AMIN>
+struct HornerPlan {
+    static constexpr int M = BasisShape<DIM, L, AMIN>::M;
+    int ng;
+    int b[M], c[M], cnt[M], amin[M], start[M];
+    int idx[M];
+    constexpr HornerPlan() : ng(0), b{}, c{}, cnt{}, amin{}, start{}, idx{} {
+        const BasisTables<DIM, L, AMIN> E{};
+        int pos = 0;
+        for (int cc = 0; cc < (DIM == 3 ? L : 1); ++cc)
+            for (int bb = 0; bb + cc < L; ++bb) {
+                const int first = pos;
+                int lo = 0, count = 0;
+                for (int aa = L - 1; aa >= 0; --aa)
+                    for (int k = 0; k < M; ++k)
+                        if (E.e[k][0] == aa && E.e[k][1] == bb && E.e[k][2] == cc) {
+                            idx[pos++] = k;
+                            lo = aa;
+                            ++count;
+                        }
+                if (count) {
+                    b[ng] = bb;
+                    c[ng] = cc;
+                    cnt[ng] = count;
+                    amin[ng] = lo;
+                    start[ng] = first;
+                    ++ng;
+                }
+            }
+    }
+};
+
+// sum_i xs[i] z^alpha_i for the basis <DIM, L, AMIN>, from the powers
+// px[k] = x^k, py[k] = y^k, pz[k] = z^k (k < L).
+template <int DIM, int L, int AMIN>
+__device__ __forceinline__ double basis_sum(const double* xs, const double* px, const double* py, const double* pz) {
+    constexpr HornerPlan<DIM, L, AMIN> H{};
+    double sum = 0.0;
+#pragma unroll
+    for (int g = 0; g < H.ng; ++g) {
+        double h = xs[H.idx[H.start[g]]];
+#pragma unroll
+        for (int t = 1; t < H.cnt[g]; ++t) h = fma(h, px[1], xs[H.idx[H.start[g] + t]]);
+        if (H.amin[g] > 0) h *= px[H.amin[g]];
+        if (H.b[g] > 0 && H.c[g] > 0) h *= py[H.b[g]];
+        if (H.c[g] > 0) sum = fma(h, pz[H.c[g]], sum);
+        else if (H.b[g] > 0) sum = fma(h, py[H.b[g]], sum);
+        else sum += h;
+    }
+    return sum;
+}
+
+// Kernel weights of the pass's operators for particle p (Dcpse.hpp:798-809):
+// lanes own neighbours, K_pq = w_q * sum_i a_i z_q^alpha_i with each
+// operator's solution a in xs1 / xs2 (wave-uniform), written with the
+// eps^-|beta| pre-scaled copies. The displacement, weight w and powers of a
+// neighbour are formed once for both operators; those of the first 64
+// neighbours come from dsl[d * 64 + lane] and wsl[lane] (as the moment pass
+// computed them). Operator k is assembled when use_k (enough neighbours;
+// otherwise it is marked DCPSE_INFO_TOO_FEW with eps = 0) and solved when
+// ok_k (otherwise DCPSE_INFO_NOT_SPD and zero weights).
+template <int DIM, int L1, int A1, int L2, int A2, typename Disp>
+__device__ __forceinline__ void asm_weights(const AsmArgs& a, int64_t p, int64_t rs, int n, double eps,
+                                            double ieps, bool use1, bool ok1, bool use2, bool ok2,
+                                            const double* xs1, const double* xs2, const double* dsl,
+                                            const double* wsl, Disp disp, int lane) {
+    constexpr int LP = L1 > L2 ? L1 : L2;
+    const AsmOp& o1 = a.op[0];
+    const AsmOp& o2 = a.op[1];
+    const double eip1 = 1.0 / ipow_rt(eps, o1.border);
+    const double eip2 = L2 > 0 ? 1.0 / ipow_rt(eps, o2.border) : 0.0;
+    bool bad1 = false, bad2 = false;
     for (int j = lane; j < n; j += kWave) {
         double dv[3] = {0.0, 0.0, 0.0};
 #pragma unroll
@@ -365,43 +443,54 @@ __device__ __forceinline__ void asm_weights(const AsmOp& op, int64_t p, int64_t 
             z2 = fma(z[d], z[d], z2);  // as the moment pass forms it
         }
         const double w = j < kWave ? wsl[j] : exp(-z2);
-        double px[L], py[L], pz[L];
+        double px[LP], py[LP], pz[LP];
         px[0] = py[0] = pz[0] = 1.0;
 #pragma unroll
-        for (int k = 1; k < L; ++k) {
+        for (int k = 1; k < LP; ++k) {
             px[k] = px[k - 1] * z[0];
             py[k] = py[k - 1] * z[1];
             pz[k] = pz[k - 1] * z[2];
         }
-        double sum = 0.0;
-#pragma unroll
-        for (int k = 0; k < M; ++k) {
-            double v = px[E.e[k][0]] * py[E.e[k][1]];
-            if (DIM == 3) v *= pz[E.e[k][2]];
-            sum = fma(xs[k], v, sum);
+        {
+            const double K = ok1 ? basis_sum<DIM, L1, A1>(xs1, px, py, pz) * w : 0.0;
+            bad1 |= !isfinite(K);
+            if (o1.K64) o1.K64[rs + j] = K;
+            if (o1.W64) o1.W64[rs + j] = K * eip1;
+            if (o1.W32) o1.W32[rs + j] = (float)(K * eip1);
         }
-        const double K = ok ? sum * w : 0.0;
-        bad |= !isfinite(K);
-        if (op.K64) op.K64[rs + j] = K;
-        if (op.W64) op.W64[rs + j] = K * eip;
-        if (op.W32) op.W32[rs + j] = (float)(K * eip);
+        if constexpr (L2 > 0) {
+            const double K = ok2 ? basis_sum<DIM, L2, A2>(xs2, px, py, pz) * w : 0.0;
+            bad2 |= !isfinite(K);
+            if (o2.K64) o2.K64[rs + j] = K;
+            if (o2.W64) o2.W64[rs + j] = K * eip2;
+            if (o2.W32) o2.W32[rs + j] = (float)(K * eip2);
+        }
     }
-    const bool anybad = __ballot(bad) != 0ull;
+    const bool anybad1 = __ballot(bad1) != 0ull;
+    const bool anybad2 = __ballot(bad2) != 0ull;
     if (lane == 0) {
-        if (op.eps) op.eps[p] = eps;
-        if (op.eps_inv_pow) op.eps_inv_pow[p] = eip;
-        if (op.info) op.info[p] = !ok ? DCPSE_INFO_NOT_SPD : (anybad ? DCPSE_INFO_NONFINITE : DCPSE_INFO_OK);
+        auto mark = [&](const AsmOp& op, bool use, bool ok, bool anybad, double eip) {
+            if (op.eps) op.eps[p] = use ? eps : 0.0;
+            if (op.eps_inv_pow) op.eps_inv_pow[p] = use ? eip : 0.0;
+            if (op.info)
+                op.info[p] = !use  ? DCPSE_INFO_TOO_FEW
+                             : !ok ? DCPSE_INFO_NOT_SPD
+                                   : (anybad ? DCPSE_INFO_NONFINITE : DCPSE_INFO_OK);
+        };
+        mark(o1, use1, ok1, anybad1, eip1);
+        if constexpr (L2 > 0) mark(o2, use2, ok2, anybad2, eip2);
     }
 }
 
 // Waves per SIMD each instantiation is compiled for: the largest bases need
-// the whole register file of one wave; up to 10 monomials three waves fit.
+// the whole register file of one wave; one operator of up to 10 monomials
+// fits three waves.
 template <int DIM, int L1, int A1, int L2, int A2>
 constexpr int asm_waves() {
     constexpr int m1 = BasisShape<DIM, L1, A1>::M;
     constexpr int m2 = L2 > 0 ? BasisShape<DIM, (L2 > 0 ? L2 : 2), A2>::M : 0;
     constexpr int m = m1 > m2 ? m1 : m2;
-    return m > 24 ? 1 : (m > 10 ? 2 : 3);
+    return m > 24 ? 1 : (m > 10 || L2 > 0 ? 2 : 3);
 }
 
 // One wavefront walks a.ppb particles (at most 63). Operator 0 has basis
@@ -416,7 +505,7 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
     constexpr int P1 = 2 * (L1 - 1), P2 = L2 > 0 ? 2 * (L2 - 1) : 0;
     constexpr int P = P1 > P2 ? P1 : P2;
     constexpr int QMIN = L2 > 0 ? (A1 < A2 ? 2 * A1 : 2 * A2) : 2 * A1;
-    constexpr int GW = P + 1, MG = ipow_c(GW, DIM);  // moment grid M(a, b, c) at a + GW b + GW^2 c
+    constexpr int GW = P + 1, MG = grid_size(DIM, GW);  // moment grid (grid_index)
     constexpr int MGP = MG + (MG & 1);
     constexpr int M1 = BasisShape<DIM, L1, A1>::M;
     constexpr int M2 = L2 > 0 ? BasisShape<DIM, (L2 > 0 ? L2 : 2), A2>::M : 0;
@@ -445,7 +534,8 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
     __shared__ __align__(16) double tab[TAB];     // staged powers; L factors in the solve
     __shared__ double mom[SLOTS * MGP];           // moments of each slot's particle
     __shared__ double2 lbuf[2 * kWave];           // broadcast buffers of the factorisation
-    __shared__ double sx[kWave];                  // solutions, one entry per lane
+    __shared__ double sx[kWave];                  // solutions of operator 0, one entry per lane
+    __shared__ double sx2[L2 > 0 ? kWave : 1];    // and of operator 1
     __shared__ double dsl[SLOTS * 3 * kWave];     // first 64 displacements per slot
     __shared__ double wsl[SLOTS * kWave];         // and their weights exp(-|z|^2)
     const int lane = threadIdx.x;
@@ -692,7 +782,7 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
 #pragma unroll
                     for (int k = 0; k < ST; ++k) {
                         const int a = pa0 + k;
-                        if (k < plen && a + pb + pc >= QMIN) ms[a + GW * pb + GW * GW * pc] = acc[k];
+                        if (k < plen && a + pb + pc >= QMIN) ms[grid_index<GW>(a, pb, pc)] = acc[k];
                     }
                 }
                 wave_lds_sync();
@@ -709,36 +799,31 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
         }
 
         // ---- per operator: factorisation and solve of every slot, weights ----
-        auto weights = [&](auto tag, const AsmOp& op, int m, int abase, double b_lane) {
+        // the solution of each slot's system, per operator (sx, sx2)
+        auto solve = [&](auto tag, int m, int abase, double b_lane, double* xs) -> uint64_t {
             constexpr int LL = decltype(tag)::L, AA = decltype(tag)::A;
             bool any = false;
 #pragma unroll
             for (int s = 0; s < SLOTS; ++s) any |= go_s[s] && n_s[s] >= m;
-            if (!any) {
-#pragma unroll
-                for (int s = 0; s < SLOTS; ++s)
-                    if (go_s[s]) asm_fail(op, pid_s[s], rs_s[s], n_s[s], DCPSE_INFO_TOO_FEW, lane);
-                return;
-            }
+            if (!any) return 0ull;  // no slot has enough neighbours for this operator
             bool okl;
-            asm_factor_solve<DIM, LL, AA, GW, SLOTS>(mom, abase, b_lane, a.refine, tab, lbuf, sx, lane, okl);
-            const uint64_t okm = __ballot(okl);
-#pragma unroll
-            for (int s = 0; s < SLOTS; ++s) {
-                if (!go_s[s]) continue;
-                const int64_t p = pid_s[s];
-                if (n_s[s] < m) {
-                    asm_fail(op, p, rs_s[s], n_s[s], DCPSE_INFO_TOO_FEW, lane);
-                    continue;
-                }
-                auto disp = [&](int j, double* dv) { load_disp(rs_s[s], n_s[s], xp_s[s], j, dv); };
-                asm_weights<DIM, LL, AA>(op, p, rs_s[s], n_s[s], eps_s[s], ieps_s[s], (okm >> (s * LS)) & 1ull,
-                                         sx + s * LS, dsl + s * 3 * kWave, wsl + s * kWave, disp, lane);
-            }
-            wave_lds_sync();
+            asm_factor_solve<DIM, LL, AA, GW, SLOTS>(mom, abase, b_lane, a.refine, tab, lbuf, xs, lane, okl);
+            return __ballot(okl);
         };
-        weights(BasisTag<L1, A1>{}, a.op[0], M1, ab1, b1);
-        if constexpr (L2 > 0) weights(BasisTag<(L2 > 0 ? L2 : 2), A2>{}, a.op[1], M2, ab2, b2);
+        const uint64_t okm1 = solve(BasisTag<L1, A1>{}, M1, ab1, b1, sx);
+        uint64_t okm2 = 0ull;
+        if constexpr (L2 > 0) okm2 = solve(BasisTag<(L2 > 0 ? L2 : 2), A2>{}, M2, ab2, b2, sx2);
+#pragma unroll
+        for (int s = 0; s < SLOTS; ++s) {
+            if (!go_s[s]) continue;
+            const bool use1 = n_s[s] >= M1, use2 = L2 > 0 && n_s[s] >= M2;
+            const bool ok1 = use1 && ((okm1 >> (s * LS)) & 1ull), ok2 = use2 && ((okm2 >> (s * LS)) & 1ull);
+            auto disp = [&](int j, double* dv) { load_disp(rs_s[s], n_s[s], xp_s[s], j, dv); };
+            asm_weights<DIM, L1, A1, L2, A2>(a, pid_s[s], rs_s[s], n_s[s], eps_s[s], ieps_s[s], use1, ok1, use2, ok2,
+                                             sx + s * LS, sx2 + s * LS, dsl + s * 3 * kWave, wsl + s * kWave, disp,
+                                             lane);
+        }
+        wave_lds_sync();
     }
     if constexpr (!PQ) break;
     }  // batches
