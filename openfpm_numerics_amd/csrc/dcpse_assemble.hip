@@ -1232,7 +1232,8 @@ static dcpse_status assemble_ops(dcpse_ctx* ctx, int32_t dim, const double* pos_
                                  double* const* K64, float* const* W32, double* const* W64, int32_t* const* info) {
     if (!ctx) return DCPSE_ERR_INVALID;
     if (dim != 2 && dim != 3) return fail(ctx, DCPSE_ERR_UNSUPPORTED, "dim must be 2 or 3");
-    if (!pos_sup || !row_ptr || n_sup < 0 || n_dom < 0 || n_ops < 1 || !n_sig || !betas)
+    // positions of an empty set may be null (a particle set without storage)
+    if ((!pos_sup && n_sup > 0) || !row_ptr || n_sup < 0 || n_dom < 0 || n_ops < 1 || !n_sig || !betas)
         return fail(ctx, DCPSE_ERR_INVALID, "null pointer or bad size");
     if (!(h_over_eps > 0.0)) return fail(ctx, DCPSE_ERR_INVALID, "h_over_eps must be > 0");
     if (n_sup >= ((int64_t)1 << 31) || n_dom >= ((int64_t)1 << 31))

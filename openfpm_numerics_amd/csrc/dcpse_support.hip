@@ -132,6 +132,18 @@ void fill_i32(hipStream_t s, int32_t* p, int64_t n, int32_t v) {
     k_fill_i32<<<(unsigned)blocks, 256, 0, s>>>(p, n, v);
 }
 
+__global__ void k_fill_i64(int64_t* p, int64_t n, int64_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+static void fill_i64(hipStream_t s, int64_t* p, int64_t n, int64_t v) {
+    if (n <= 0) return;
+    int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+    k_fill_i64<<<(unsigned)blocks, 256, 0, s>>>(p, n, v);
+}
+
 dcpse_status reserve_workspace(dcpse_ctx* ctx, size_t bytes, Workspace* w) {
     if (ctx->ws_size < bytes) {
         DCPSE_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
@@ -965,12 +977,23 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
                                  const double* box_lo, const double* box_hi, unsigned periodic,
                                  int64_t* row_ptr, int32_t* col, int64_t* nnz_out) {
     if (dim != 2 && dim != 3) return fail(ctx, DCPSE_ERR_UNSUPPORTED, "dim must be 2 or 3");
-    if (!pos_sup || !rcut || !row_ptr || !box_lo || !box_hi || n_sup < 0 || n_dom < 0)
+    // positions of an empty set may be null (a particle set without storage)
+    if ((!pos_sup && n_sup > 0) || !row_ptr || !box_lo || !box_hi || n_sup < 0 || n_dom < 0)
         return fail(ctx, DCPSE_ERR_INVALID, "null pointer or negative size");
     if (n_sup >= (int64_t)1 << 31 || n_dom >= (int64_t)1 << 31)
         return fail(ctx, DCPSE_ERR_RANGE, "particle counts must be < 2^31");
     const int same = pos_dom == nullptr;
     if (same) { pos_dom = pos_sup; n_dom = n_sup; }
+    // empty sets: no rows, or rows without neighbours
+    if (n_dom == 0 || n_sup == 0) {
+        if (!col) {  // count pass: row_ptr = 0, nnz = 0
+            fill_i64(ctx->stream, row_ptr, n_dom + 1, 0);
+            DCPSE_LAUNCH_CHECK(ctx);
+            if (nnz_out) *nnz_out = 0;
+        }
+        return DCPSE_OK;
+    }
+    if (!rcut) return fail(ctx, DCPSE_ERR_INVALID, "rcut is null");
     // the largest cut-off fixes the cell edge
     double rmax = 0.0, rmin = 0.0;
     if (rcut_pp) {

@@ -18,6 +18,7 @@ sum_d D^{2 e_d} as one operator.
 """
 from __future__ import annotations
 
+import math
 import os
 import struct
 
@@ -55,6 +56,11 @@ SELL_AFTER = 1
 #: sliced-ELL layout is faster and the automatic choice takes it.
 BUNDLE16_MAX_EXC = 0.01
 _FILE_VERSION = 1
+
+
+def _batch2d(f: torch.Tensor) -> torch.Tensor:
+    """A field (N,) or batch (..., N) as (B, N), also for N = 0."""
+    return f.reshape(math.prod(f.shape[:-1]), f.shape[-1])
 
 
 def basis(beta, order):
@@ -334,11 +340,14 @@ class Dcpse:
         """A^T g for a field (N,) or a batch (B, N)."""
         if g.dtype not in (torch.float32, torch.float64):
             raise TypeError("fields must be float32 or float64")
-        rowT, colT, WT, diag = self.transposed(g.dtype)
-        g2 = g.reshape(-1, g.shape[-1]).contiguous()
+        g2 = _batch2d(g).contiguous()
         nb, n = g2.shape
         if n != self.verletList.n_cols:
             raise ValueError("field length differs from the support set")
+        if n == 0 or self.verletList.n_rows == 0:  # an empty set: A^T g = 0
+            o = torch.zeros_like(g2) if out is None else out.reshape(nb, n).zero_()
+            return o.reshape(g.shape)
+        rowT, colT, WT, diag = self.transposed(g.dtype)
         o = torch.empty_like(g2) if out is None else out.reshape(nb, n)
         ctx = _lib.context(g.device.index or 0)
         if g.dtype == torch.float32 and nb == 1 and USE_SELL and self._count_apply32("adj"):
@@ -555,7 +564,7 @@ def apply_ops(ops, f: torch.Tensor, outs=None, f_domain: torch.Tensor | None = N
     for op in ops:
         if op.update_ctr != op.particlesSupport.map_ctr:
             raise RuntimeError("particles moved since assembly: call update()/initializeUpdate() first")
-    f2 = f.reshape(-1, f.shape[-1]).contiguous()
+    f2 = _batch2d(f).contiguous()
     nb, n = f2.shape
     if n != vl.n_cols:
         raise ValueError(f"field has {n} values, support set has {vl.n_cols} particles")
@@ -572,6 +581,8 @@ def apply_ops(ops, f: torch.Tensor, outs=None, f_domain: torch.Tensor | None = N
         outs = [torch.empty((nb, n_rows), dtype=f.dtype, device=f.device) for _ in ops]
     else:
         outs = [o.reshape(nb, n_rows) for o in outs]
+    if n_rows == 0:  # an empty domain: nothing to compute
+        return [o.reshape(*f.shape[:-1], n_rows) for o in outs]
     ctx = _lib.context(f.device.index or 0)
     use_sell = False
     if f.dtype == torch.float32 and nb == 1 and USE_SELL:
@@ -665,7 +676,7 @@ class PreparedApply:
             raise ValueError("fused operators must share one Verlet list")
         if not f.is_contiguous() or any(not o.is_contiguous() for o in outs):
             raise ValueError("contiguous tensors only")
-        f2 = f.reshape(-1, f.shape[-1])
+        f2 = _batch2d(f)
         nb, n = f2.shape
         self.ctx = _lib.context(f.device.index or 0)
         self.O = _lib.ptr_array(outs)

@@ -210,6 +210,11 @@ class VerletList:
         if per and rc.numel() != dom.size_local():
             raise ValueError("per-particle rCut needs one value per domain particle")
         n_dom = dom.size_local()
+        if n_dom == 0:  # no rows (an empty set has no position storage to pass on)
+            vl = cls(torch.zeros(1, dtype=torch.int64, device=dev), torch.empty(0, dtype=torch.int32, device=dev),
+                     support.size_local(), rc, domain)
+            vl.map_ctr = support.map_ctr
+            return vl
         row_ptr = torch.empty(n_dom + 1, dtype=torch.int64, device=dev)
         nnz = np.zeros(1, np.int64)
         pdom = None if domain is None else _lib.ptr(dom.pos_soa)
