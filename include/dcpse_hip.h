@@ -118,7 +118,8 @@ int32_t dcpse_rhs(int32_t dim, const uint32_t* beta, uint32_t order, double* b, 
 /* ---- neighbour support (Verlet CSR) ------------------------------------ */
 /* Row p lists, in ascending index, every support particle q with
  * |x_p - x_q| < rcut_p (q != p when pos_dom == NULL, i.e. the domain set is
- * the support set). rcut is a device array of 1 value (uniform) or n_dom
+ * the support set; then n_dom must equal n_sup). A NULL pos_sup is accepted
+ * when n_sup == 0 (an empty set has no rows, or rows without neighbours). rcut is a device array of 1 value (uniform) or n_dom
  * values (rcut_per_particle != 0). box_lo/box_hi: host, dim doubles; a
  * periodic dimension (bit d of periodic_mask) uses minimum-image
  * displacements and needs at least 3 cells of edge >= max rcut.

@@ -158,6 +158,7 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
         auto& c = openfpm_dcpse::context();
         check_fresh();
         const int64_t n = (int64_t)verletList_.size();
+        if (n == 0) return;  // an empty domain: nothing to compute
         if constexpr (two_sets) {
             std::vector<prop_type> fh(particles_.size_local()), self(domain_->size_local());
             if (!fh.empty())
@@ -552,6 +553,11 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
         K64_.resize(nnz);
         W64_.resize(nnz);
         W32_.resize(nnz);
+        if (n == 0) {  // an empty domain: no rows to assemble
+            mirrored_ = false;
+            ctr_ = particles_.getMapCtr();
+            return;
+        }
         double lo[dim], hi[dim];
         for (unsigned d = 0; d < dim; ++d) {
             lo[d] = (double)particles_.boxLow()[d];

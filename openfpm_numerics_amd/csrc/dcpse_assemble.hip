@@ -1240,10 +1240,11 @@ static dcpse_status assemble_ops(dcpse_ctx* ctx, int32_t dim, const double* pos_
         return fail(ctx, DCPSE_ERR_RANGE, "particle counts must be < 2^31");
     for (int o = 0; o < n_ops; ++o)
         if (n_sig[o] < 1) return fail(ctx, DCPSE_ERR_INVALID, "operator %d has no signature", o);
-    if (pos_dom == nullptr) {
-        pos_dom = pos_sup;
-        n_dom = n_sup;
-    }
+    // a NULL domain is the support set itself: its row count is n_sup (a
+    // smaller n_dom would size the outputs for fewer rows than are written)
+    if (pos_dom == nullptr && n_dom != n_sup)
+        return fail(ctx, DCPSE_ERR_INVALID, "pos_dom NULL (the support set as domain) needs n_dom == n_sup");
+    if (pos_dom == nullptr) pos_dom = pos_sup;
     if (periodic_mask && (!box_lo || !box_hi))
         return fail(ctx, DCPSE_ERR_INVALID, "periodic boundaries need box_lo/box_hi");
     DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));

@@ -285,10 +285,11 @@ dcpse_status dcpse_operator_diagnostics(dcpse_ctx* ctx, int32_t dim, const doubl
         return fail(ctx, DCPSE_ERR_INVALID, "null pointer or bad size");
     if (periodic_mask && (!box_lo || !box_hi))
         return fail(ctx, DCPSE_ERR_INVALID, "periodic boundaries need box_lo/box_hi");
-    if (pos_dom == nullptr) {
-        pos_dom = pos_sup;
-        n_dom = n_sup;
-    }
+    // a NULL domain is the support set itself: its row count is n_sup (a
+    // smaller n_dom would size the outputs for fewer rows than are written)
+    if (pos_dom == nullptr && n_dom != n_sup)
+        return fail(ctx, DCPSE_ERR_INVALID, "pos_dom NULL (the support set as domain) needs n_dom == n_sup");
+    if (pos_dom == nullptr) pos_dom = pos_sup;
     const int M = dcpse_basis_size(dim, betas, order);
     if (M <= 0 || M > 64) return fail(ctx, DCPSE_ERR_UNSUPPORTED, "diagnostics need a basis of 1..64 monomials");
     std::vector<uint32_t> ex((size_t)M * dim);

@@ -983,7 +983,11 @@ static dcpse_status support_pass(dcpse_ctx* ctx, int dim, const double* pos_sup,
     if (n_sup >= (int64_t)1 << 31 || n_dom >= (int64_t)1 << 31)
         return fail(ctx, DCPSE_ERR_RANGE, "particle counts must be < 2^31");
     const int same = pos_dom == nullptr;
-    if (same) { pos_dom = pos_sup; n_dom = n_sup; }
+    // a NULL domain is the support set itself: its row count is n_sup (a
+    // smaller n_dom would size row_ptr for fewer rows than are written)
+    if (same && n_dom != n_sup)
+        return fail(ctx, DCPSE_ERR_INVALID, "pos_dom NULL (the support set as domain) needs n_dom == n_sup");
+    if (same) pos_dom = pos_sup;
     // empty sets: no rows, or rows without neighbours
     if (n_dom == 0 || n_sup == 0) {
         if (!col) {  // count pass: row_ptr = 0, nnz = 0
@@ -1448,8 +1452,14 @@ dcpse_status dcpse_support_count_subset(dcpse_ctx* ctx, int32_t dim, const doubl
                                         const double* box_hi, uint32_t periodic_mask, int64_t* row_ptr,
                                         int64_t* nnz) {
     if (!ctx) return DCPSE_ERR_INVALID;
-    if (!pos_dom || !dom_ids) return fail(ctx, DCPSE_ERR_INVALID, "a subset needs pos_dom and dom_ids");
     DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    if (n_dom == 0 && row_ptr) {  // an empty subset: no rows
+        fill_i64(ctx->stream, row_ptr, 1, 0);
+        DCPSE_LAUNCH_CHECK(ctx);
+        if (nnz) *nnz = 0;
+        return DCPSE_OK;
+    }
+    if (!pos_dom || !dom_ids) return fail(ctx, DCPSE_ERR_INVALID, "a subset needs pos_dom and dom_ids");
     return support_pass(ctx, dim, pos_sup, n_sup, pos_dom, n_dom, dom_ids, rcut, rcut_per_particle, box_lo, box_hi,
                         periodic_mask, row_ptr, nullptr, nnz);
 }
@@ -1460,6 +1470,7 @@ dcpse_status dcpse_support_fill_subset(dcpse_ctx* ctx, int32_t dim, const double
                                        const double* box_hi, uint32_t periodic_mask, const int64_t* row_ptr,
                                        int32_t* col) {
     if (!ctx) return DCPSE_ERR_INVALID;
+    if (n_dom == 0) return DCPSE_OK;  // an empty subset: no rows
     if (!col) return fail(ctx, DCPSE_ERR_INVALID, "col is null");
     if (!pos_dom || !dom_ids) return fail(ctx, DCPSE_ERR_INVALID, "a subset needs pos_dom and dom_ids");
     DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));

@@ -526,6 +526,10 @@ def assemble_together(ops, particlesSupport: ParticleSet | None = None, particle
     if vl.n_rows != n_dom:
         raise ValueError(f"Verlet list has {vl.n_rows} rows, domain has {n_dom} particles")
     outs = [d._outputs(n_dom, dev) for d in ops]
+    if n_dom == 0:  # an empty domain: no rows to assemble
+        for d, o in zip(ops, outs):
+            d._adopt(o, sup.map_ctr)
+        return
     n_sig = np.array([len(d.signatures) for d in ops], dtype=np.int32)
     betas = np.ascontiguousarray(np.concatenate([np.array(d.signatures, dtype=np.uint32).ravel() for d in ops]))
     coeffs = np.ascontiguousarray(np.concatenate([np.array(d.coeffs, dtype=np.float64) for d in ops]))

@@ -1,6 +1,7 @@
 """Edge cases of the support, assembly and apply paths: empty particle sets
 (the reference's loops simply run zero times, e.g. a process whose
-subdomain holds no particles), a domain whose support set is empty (every
+subdomain holds no particles), an empty domain of two sets, a domain whose
+support set is empty (every
 row has too few neighbours: the reference's Vandermonde.hpp:109-112 error),
 rows of several hundred neighbours (many LDS chunks in the assembly and a
 wide sliced-ELL slice in the apply), and rows above the documented limit of
@@ -46,6 +47,38 @@ def test_domain_without_support_particles(cuda):
     assert vl.nnz == 0 and vl.row_ptr.tolist() == [0] * 6
     with pytest.raises(ValueError):
         Derivative_x(sup, vl, 2, 0.2, particlesDomain=dom)
+
+
+def test_empty_domain_of_two_sets(cuda):
+    """A non-empty support set with an empty domain (a separate set, and an
+    empty subset): no rows, nothing assembled, empty results; the C ABI
+    refuses a NULL domain (meaning the support set itself) whose row count
+    differs from the support set's, before any device work."""
+    from openfpm_numerics_amd import DcpseError, Derivative_x, ParticleSet, ParticleSubset, _lib
+    from openfpm_numerics_amd.particles import VerletList
+
+    pos = np.random.default_rng(2).random((400, 2))
+    sup = ParticleSet(pos, [0.0, 0.0], [1.0, 1.0], bc=[1, 1], device=cuda)
+    f = torch.as_tensor(np.sin(2 * np.pi * pos[:, 0]), device=cuda)
+    dom = ParticleSet(np.zeros((0, 2)), [0.0, 0.0], [1.0, 1.0], bc=[1, 1], device=cuda)
+    vl = VerletList.build(sup, 0.2, dom)
+    assert vl.n_rows == 0 and vl.nnz == 0
+    dx = Derivative_x(sup, vl, 2, 0.2, particlesDomain=dom)
+    assert dx(f, f_domain=torch.zeros(0, dtype=f.dtype, device=cuda)).shape == (0,)
+    sub = ParticleSubset(sup, np.zeros(0, dtype=np.int32))
+    vls = sub.getVerlet(0.2)
+    assert vls.n_rows == 0 and vls.nnz == 0
+    dxs = Derivative_x(sup, vls, 2, 0.2, particlesDomain=sub)
+    assert dxs(f).shape == (0,)
+    ctx = _lib.context(0)
+    rc = torch.tensor([0.2], dtype=torch.float64, device=cuda)
+    row_ptr = torch.empty(1, dtype=torch.int64, device=cuda)
+    nnz = np.zeros(1, np.int64)
+    lo = np.zeros(2)
+    hi = np.ones(2)
+    with pytest.raises(DcpseError, match="n_dom == n_sup"):
+        ctx.call("dcpse_support_count", 2, _lib.ptr(sup.pos_soa), 400, None, 0, _lib.ptr(rc), 0, lo.ctypes.data,
+                 hi.ctypes.data, 3, _lib.ptr(row_ptr), nnz.ctypes.data)
 
 
 def test_long_rows_match_oracle(oracle, cuda):
