@@ -1,0 +1,102 @@
+"""The benchmark workload at full size: 128^3 lattice jittered by U(+-0.1h),
+10% removed (seed 3), periodic, rCut 2.4h, order 2 -- 1,886,902 particles,
+105 M neighbours. The oracle builds this support in seconds, so the support
+is compared whole; its assembly is compared on three ranges of 1,000
+particles; everything else through properties that hold at any size.
+
+Tolerances: support bit-exact; FP64 weights per row <= 1e-10 (or 16 cond(A) u,
+_dcpse_tol) on the sampled ranges; a constant field gives exactly 0 (the
+difference form); FP64 linearity to 1e-12 and the adjoint identity
+<A f, g> = <f, A^T g> to 1e-12 relative; FP32 against FP64 relative L2
+<= 1e-5; the FP64 operators against the analytic derivatives of
+sin 2pi x sin 2pi y sin 2pi z within 1.1x of their measured discretisation
+error (Laplacian 1.66e-3, Dx 3.25e-3 relative max); repeated applies bitwise
+equal.
+"""
+import numpy as np
+import pytest
+import torch
+from _dcpse_tol import check_weights
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c3(cuda):
+    from openfpm_numerics_amd import Derivative_x, Laplacian, ParticleSet, workloads as W
+
+    w = W.jittered_3d_holes(128, seed=3)
+    ps = ParticleSet(w["pos"], w["box_lo"], w["box_hi"], bc=[1, 1, 1], device=cuda)
+    vl = ps.getVerlet(w["rcut"])
+    lap = Laplacian(ps, vl, 2, w["rcut"])
+    dx = Derivative_x(ps, vl, 2, w["rcut"])
+    print(f"[full size] N={ps.size_local()} nnz={vl.nnz}", flush=True)
+    return w, ps, vl, lap, dx
+
+
+def test_support_and_sampled_weights_match_oracle(c3, oracle):
+    w, ps, vl, lap, dx = c3
+    rp, col = oracle.support(w["pos"], w["rcut"], w["box_lo"], w["box_hi"], w["periodic"])
+    assert np.array_equal(vl.row_ptr.cpu().numpy(), rp)
+    assert np.array_equal(vl.col.cpu().numpy(), col)
+    n = rp.size - 1
+    K_lap = lap.dcpse.calcKernels.cpu().numpy()
+    K_dx = dx.dcpse.calcKernels.cpu().numpy()
+    lo, hi, per = w["box_lo"], w["box_hi"], w["periodic"]
+    for p0 in (0, n // 2, n - 1000):
+        rng_ = (p0, p0 + 1000)
+        sl = slice(rp[p0], rp[p0 + 1000])
+        rp_s = rp[p0:p0 + 1001] - rp[p0]
+        refs = [oracle.assemble(w["pos"], rp, col, b, 2, lo, hi, per, dense_ev=False, p_range=rng_)
+                for b in ((2, 0, 0), (0, 2, 0), (0, 0, 2))]
+        ref_dx = oracle.assemble(w["pos"], rp, col, (1, 0, 0), 2, lo, hi, per, dense_ev=False, p_range=rng_)
+        pos_s = w["pos"]
+        # rows p0 .. p0 + 999 as their own CSR (columns index the whole set)
+        rel, _, _ = check_weights(K_lap[sl], sum(r.K for r in refs)[sl], rp_s, pos_s, pos_s[p0:p0 + 1000],
+                                  col[sl], oracle.basis((2, 0, 0), 2), period=[1.0, 1.0, 1.0], label="lap")
+        rel_dx, _, _ = check_weights(K_dx[sl], ref_dx.K[sl], rp_s, pos_s, pos_s[p0:p0 + 1000], col[sl],
+                                     oracle.basis((1, 0, 0), 2), period=[1.0, 1.0, 1.0], label="dx")
+        print(f"[full size] rows {p0}..{p0 + 999}: weights Laplacian {rel:.2e}, Dx {rel_dx:.2e}", flush=True)
+
+
+def test_properties_at_full_size(c3, cuda):
+    from openfpm_numerics_amd import apply_ops, workloads as W
+
+    w, ps, vl, lap, dx = c3
+    ops = [lap.dcpse, dx.dcpse]
+    n = ps.size_local()
+    # constant fields: exactly zero (difference form), FP32 and FP64
+    for dt in (torch.float32, torch.float64):
+        for out in apply_ops(ops, torch.full((n,), 3.25, dtype=dt, device=cuda)):
+            assert float(out.abs().max()) == 0.0
+    f64, dx_exact, _, lap_exact = W.sin_field(w["pos"])
+    f = torch.as_tensor(f64, device=cuda)
+    g = torch.as_tensor(np.cos(2 * np.pi * w["pos"][:, 1]) * w["pos"][:, 2], device=cuda)
+    Lf, Df = apply_ops(ops, f)
+    Lg, Dg = apply_ops(ops, g)
+    Lh, Dh = apply_ops(ops, 2.0 * f - 3.0 * g)
+    for h, a, b in ((Lh, Lf, Lg), (Dh, Df, Dg)):
+        want = 2.0 * a - 3.0 * b
+        assert float((h - want).abs().max()) <= 1e-12 * float(want.abs().max())
+    # analytic derivatives (discretisation error of order 2 at rCut 2.4h)
+    err_lap = float((Lf.cpu() - torch.as_tensor(lap_exact)).abs().max()) / float(np.abs(lap_exact).max())
+    err_dx = float((Df.cpu() - torch.as_tensor(dx_exact)).abs().max()) / float(np.abs(dx_exact).max())
+    print(f"[full size] analytic error: Laplacian {err_lap:.3e}, Dx {err_dx:.3e}", flush=True)
+    assert err_lap <= 1.1 * 1.66e-3 and err_dx <= 1.1 * 3.25e-3
+    # FP32 against FP64 on the FP32-rounded field
+    f32 = f.float()
+    L32, D32 = apply_ops(ops, f32)
+    L64r, D64r = apply_ops(ops, f32.double())
+    for a, b in ((L32, L64r), (D32, D64r)):
+        rel = float(torch.linalg.vector_norm(a.double() - b) / torch.linalg.vector_norm(b))
+        assert rel <= 1e-5, rel
+    # repeated applies bitwise equal (one kernel from the first apply on)
+    L32b, D32b = apply_ops(ops, f32)
+    assert torch.equal(L32, L32b) and torch.equal(D32, D32b)
+    # adjoint identity <A f, g> = <f, A^T g>
+    for op in ops:
+        Af = apply_ops([op], f)[0]
+        ATg = op.applyTranspose(g)
+        lhs, rhs = float(torch.dot(Af, g)), float(torch.dot(f, ATg))
+        scale = float(torch.linalg.vector_norm(Af) * torch.linalg.vector_norm(g))
+        assert abs(lhs - rhs) <= 1e-12 * scale, (lhs, rhs)
