@@ -1,8 +1,10 @@
 """The benchmark workload at full size: 128^3 lattice jittered by U(+-0.1h),
 10% removed (seed 3), periodic, rCut 2.4h, order 2 -- 1,886,902 particles,
-105 M neighbours. The oracle builds this support in seconds, so the support
-is compared whole; its assembly is compared on three ranges of 1,000
-particles; everything else through properties that hold at any size.
+105 M neighbours -- and the uneven-support workload at full size (1M
+particles, 20 to 226 neighbours). The oracle builds these supports in
+seconds, so they are compared whole; the assembly is compared on three
+ranges of 1,000 particles; everything else through properties that hold at
+any size.
 
 Tolerances: support bit-exact; FP64 weights per row <= 1e-10 (or 16 cond(A) u,
 _dcpse_tol) on the sampled ranges; a constant field gives exactly 0 (the
@@ -100,3 +102,39 @@ def test_properties_at_full_size(c3, cuda):
         lhs, rhs = float(torch.dot(Af, g)), float(torch.dot(f, ATg))
         scale = float(torch.linalg.vector_norm(Af) * torch.linalg.vector_norm(g))
         assert abs(lhs - rhs) <= 1e-12 * scale, (lhs, rhs)
+
+
+def test_uneven_full_size_matches_oracle(oracle, cuda):
+    """uneven_3d at 1M particles: the pair pass (default launch) on sampled
+    ranges at both ends of the spacing gradient and in the middle."""
+    from openfpm_numerics_amd import Derivative_x, Laplacian, ParticleSet, workloads as W
+    from openfpm_numerics_amd.dcpse import assemble_together
+
+    w = W.uneven_3d(1_000_000)
+    ps = ParticleSet(w["pos"], w["box_lo"], w["box_hi"], bc=[1, 1, 1], device=cuda)
+    vl = ps.getVerlet(w["rcut"])
+    rp, col = oracle.support(w["pos"], w["rcut"], w["box_lo"], w["box_hi"], w["periodic"])
+    assert np.array_equal(vl.row_ptr.cpu().numpy(), rp) and np.array_equal(vl.col.cpu().numpy(), col)
+    lens = np.diff(rp)
+    print(f"[full size] uneven N={lens.size} neighbours {lens.min()}..{lens.max()}", flush=True)
+    lap = Laplacian(ps, vl, 2, w["rcut"])
+    dx = Derivative_x(ps, vl, 2, w["rcut"])
+    assemble_together([lap.dcpse, dx.dcpse], ps, ps)
+    K_lap = lap.dcpse.calcKernels.cpu().numpy()
+    K_dx = dx.dcpse.calcKernels.cpu().numpy()
+    n = lens.size
+    lo, hi, per = w["box_lo"], w["box_hi"], w["periodic"]
+    for p0 in (0, n // 2, n - 1000):
+        rng_ = (p0, p0 + 1000)
+        sl = slice(rp[p0], rp[p0 + 1000])
+        rp_s = rp[p0:p0 + 1001] - rp[p0]
+        refs = [oracle.assemble(w["pos"], rp, col, b, 2, lo, hi, per, dense_ev=False, p_range=rng_)
+                for b in ((2, 0, 0), (0, 2, 0), (0, 0, 2))]
+        ref_dx = oracle.assemble(w["pos"], rp, col, (1, 0, 0), 2, lo, hi, per, dense_ev=False, p_range=rng_)
+        pos = w["pos"]
+        rel, _, _ = check_weights(K_lap[sl], sum(r.K for r in refs)[sl], rp_s, pos, pos[p0:p0 + 1000], col[sl],
+                                  oracle.basis((2, 0, 0), 2), period=[1.0, 1.0, 1.0], label="lap")
+        rel_dx, _, _ = check_weights(K_dx[sl], ref_dx.K[sl], rp_s, pos, pos[p0:p0 + 1000], col[sl],
+                                     oracle.basis((1, 0, 0), 2), period=[1.0, 1.0, 1.0], label="dx")
+        print(f"[full size] uneven rows {p0}..{p0 + 999} (neighbours {lens[p0:p0 + 1000].min()}.."
+              f"{lens[p0:p0 + 1000].max()}): weights Laplacian {rel:.2e}, Dx {rel_dx:.2e}", flush=True)
