@@ -6,15 +6,19 @@
 //   * the moment matrix A = B^T B with B = E V (Dcpse.hpp:755-757) is built
 //     from its distinct entries, the weighted moments
 //        M(gamma) = sum_q w_q z_q^gamma,  z = x_pq / eps,  w = exp(-|z|^2) = E_qq^2,
-//     so A_ij = M(alpha_i + alpha_j). Neighbours are staged 32 at a time as
-//     per-dimension power tables in LDS; each lane owns one or more moments.
+//     so A_ij = M(alpha_i + alpha_j). Neighbours are staged 64 (or 32) at a
+//     time as per-dimension power tables in LDS; each lane owns a segment of
+//     one moment column.
 //   * A is factored by an in-register FP64 Cholesky (lane i holds row i,
-//     pivots by v_readlane, columns broadcast through LDS) instead of the
-//     reference's colPivHouseholderQr (Dcpse.hpp:766); an optional FP64
-//     iterative refinement step recomputes the residual against the moments.
-//   * epilogue: lanes own neighbours again and write
+//     columns broadcast through LDS; two particles' systems at once for
+//     bases of up to 32 monomials) instead of the reference's
+//     colPivHouseholderQr (Dcpse.hpp:766); an optional FP64 iterative
+//     refinement step recomputes the residual against the moments.
+//   * epilogue: lanes own neighbours again and write, for both operators of
+//     a pass,
 //        K_pq = w_q * sum_i a_i z_q^alpha_i              (Dcpse.hpp:798-809)
-//     coalesced, together with the eps^-|beta| pre-scaled FP64/FP32 weights.
+//     (Horner form) coalesced, together with the eps^-|beta| pre-scaled
+//     FP64/FP32 weights.
 // There is no dense n x n E matrix and no per-particle heap traffic.
 #include <algorithm>
 #include <cmath>
