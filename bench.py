@@ -311,6 +311,16 @@ def main():
     if lap.dcpse._W.get("sell64") is None:
         raise RuntimeError("the FP64 apply did not switch to its sliced-ELL copy")
     log(f"[bench] FP64 fused apply {fp64_apply_us:.1f} us -> {2 * N / fp64_apply_us:.1f} M evals/s")
+    # the same fused FP64 pass on the 16-bit-offset FP64 bundle (18 instead of
+    # 20 bytes per neighbour and operator pair slot)
+    o64b = [torch.empty_like(f64_d) for _ in range(2)]
+    step64 = PreparedApply([lap.dcpse, dx.dcpse], f64_d, o64b, sell="bundle16")
+    for _ in range(4):
+        step64()
+    fp64_b16_ms, _ = events_ms(step64, reps64, torch)
+    fp64_b16_diff = max(float((x - y).abs().max() / y.abs().max()) for x, y in zip(o64b, o64))
+    log(f"[bench] FP64 fused apply, 16-bit bundle: {fp64_b16_ms * 1e3:.1f} us (max rel. diff {fp64_b16_diff:.1e})")
+    del step64, o64b
     err = {}
     for name, got, ref in (("laplacian", out_lap, lap64), ("dx", out_dx, dx64)):
         d = got.double() - ref
@@ -447,6 +457,7 @@ def main():
         "support_ms": support_ms,
         "fp64_apply_us": fp64_apply_us,
         "fp64_apply_evals_per_s_M": 2 * N / fp64_apply_us,
+        "fp64_apply_bundle16_us": fp64_b16_ms * 1e3,
         "fp32_error_vs_fp64": err,
         "fp32_accumulation": acc32,
         "fp64_error_vs_analytic": ana,

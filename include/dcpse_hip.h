@@ -354,6 +354,25 @@ dcpse_status dcpse_apply_sell_bundle16_f32(dcpse_ctx* ctx, int64_t n_rows, const
                                            const int32_t* bundle, const int64_t* exc_ptr, const int32_t* exc_col,
                                            const float* exc_W, int64_t n_exc, const float* f, const float* f_self,
                                            float* const* out, float alpha, float beta);
+/* The FP64 16-bit bundle: slot group g of slice s is one block of
+ * 128 + 512 * n_ops words (the 64 lanes' four 16-bit offsets, then each
+ * operator's four FP64 weights per lane) starting at
+ * (128 + 512 * n_ops) * slice_ptr[s] / 256; bundle holds
+ * (0.5 + 2 * n_ops) * n_slots words. Exceptions as for the FP32 bundle
+ * (dcpse_sell_bundle16_count; exc_W holds FP64 weights). 18 bytes per
+ * neighbour for one operator instead of 20 with 32-bit columns; the apply
+ * (out = alpha * D f + beta * out, the FP64 kernel of src/DCPSE/Dcpse.hpp:560-659)
+ * equals dcpse_apply_sell_f64 bit for bit when no row has far neighbours. */
+dcpse_status dcpse_sell_fill_bundle16_f64(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr,
+                                          const int32_t* col, int32_t n_ops, const double* const* W,
+                                          const int64_t* slice_ptr, const int32_t* sell_row,
+                                          const int64_t* exc_ptr, int64_t n_exc, int32_t* bundle,
+                                          int32_t* sell_row16, int32_t* exc_col, double* exc_W);
+dcpse_status dcpse_apply_sell_bundle16_f64(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr,
+                                           const int64_t* slice_ptr, const int32_t* sell_row16, int32_t n_ops,
+                                           const int32_t* bundle, const int64_t* exc_ptr, const int32_t* exc_col,
+                                           const double* exc_W, int64_t n_exc, const double* f,
+                                           const double* f_self, double* const* out, double alpha, double beta);
 /* Same result as dcpse_apply_transpose_f32 with one field, from the SELL
  * layout of the transposed CSR (built with the three calls above). */
 dcpse_status dcpse_apply_transpose_sell_f32(dcpse_ctx* ctx, int64_t n_cols, const int64_t* rowT_ptr,

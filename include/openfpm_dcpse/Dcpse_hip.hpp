@@ -191,6 +191,14 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
         } else {
             double* O[1] = {out};
             if (n > 0 && !s64_tried_ && ++n_apply64_ >= kSellAfter) build_sell64();
+            if (b64_) {
+                c.check(dcpse_apply_sell_bundle16_f64(c.get(), n, verletList_.devRowPtr(),
+                                                      verletList_.sell(false).slice_ptr.data(), row16_64_.data(), 1,
+                                                      bundle64_.data(), exc_ptr64_.data(), exc_col64_.data(),
+                                                      exc_W64_.data(), n_exc64_, f, nullptr, O, alpha, beta),
+                        "dcpse_apply_sell_bundle16_f64");
+                return;
+            }
             if (s64_) {
                 const auto& L = verletList_.sell(true);
                 const double* SW[1] = {sellW64_.data()};
@@ -444,7 +452,7 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
         W32_.upload(W32);
         b16_ = b16_tried_ = false;
         n_apply32_ = 0;
-        s64_ = s64_tried_ = false;
+        s64_ = s64_tried_ = b64_ = false;
         n_apply64_ = 0;
         mirrored_ = true;
         ctr_ = particles_.getMapCtr();
@@ -487,15 +495,19 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
         }
     }
 
-    // FP64 applies run on FP64 weights in the slot order of the Verlet list's
-    // shared sliced-ELL layout (32-bit columns), built at the kSellAfter-th
-    // FP64 apply after each assembly or load; where it cannot be built the
-    // CSR kernel runs instead.
-    bool sell64_built() const { return s64_; }
+    // FP64 applies run on a sliced-ELL copy over the Verlet list's shared
+    // slices, built at the kSellAfter-th FP64 apply after each assembly or
+    // load: the 16-bit-offset FP64 bundle (one set; 18 bytes per neighbour),
+    // else FP64 weights in the slot order of the 32-bit-column layout, else
+    // (no device memory for either) the CSR kernel.
+    bool sell64_built() const { return s64_ || b64_; }
     void build_sell64() {
         auto& c = openfpm_dcpse::context();
         s64_tried_ = true;
         s64_ = false;
+        if constexpr (!two_sets) {
+            if (build_bundle16_64()) return;
+        }
         try {
             const auto& L = verletList_.sell(true);
             const int64_t n = (int64_t)verletList_.size();
@@ -512,6 +524,48 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
     }
 
    private:
+    bool build_bundle16_64() {
+        auto& c = openfpm_dcpse::context();
+        b64_ = false;
+        try {
+            const auto& L = verletList_.sell(false);
+            const int64_t n = (int64_t)verletList_.size();
+            const int64_t n_lanes = 64 * ((n + 63) / 64);
+            const int64_t n_slots = L.n_slots;
+            int64_t n_exc = 0;
+            exc_ptr64_.resize(n_lanes + 1);
+            if (dcpse_sell_bundle16_count(c.get(), n, verletList_.devRowPtr(), verletList_.devCol(),
+                                          L.sell_row.data(), exc_ptr64_.data(), &n_exc)) {
+                release_bundle16_64();
+                return false;
+            }
+            bundle64_.resize((size_t)std::max<int64_t>(n_slots / 2 + 2 * n_slots, 8));
+            row16_64_.resize(n_lanes);
+            exc_col64_.resize((size_t)std::max<int64_t>(n_exc, 1));
+            exc_W64_.resize((size_t)std::max<int64_t>(n_exc, 1));
+            const double* W[1] = {W64_.data()};
+            if (dcpse_sell_fill_bundle16_f64(c.get(), n, verletList_.devRowPtr(), verletList_.devCol(), 1, W,
+                                             L.slice_ptr.data(), L.sell_row.data(), exc_ptr64_.data(), n_exc,
+                                             bundle64_.data(), row16_64_.data(), exc_col64_.data(),
+                                             exc_W64_.data())) {
+                release_bundle16_64();
+                return false;
+            }
+            n_exc64_ = n_exc;
+            b64_ = true;
+        } catch (const openfpm_dcpse::dcpse_error&) {
+            release_bundle16_64();
+        }
+        return b64_;
+    }
+    void release_bundle16_64() {
+        b64_ = false;
+        exc_ptr64_ = openfpm_dcpse::device_array<int64_t>();
+        row16_64_ = openfpm_dcpse::device_array<int32_t>();
+        bundle64_ = openfpm_dcpse::device_array<int32_t>();
+        exc_col64_ = openfpm_dcpse::device_array<int32_t>();
+        exc_W64_ = openfpm_dcpse::device_array<double>();
+    }
     void release_bundle16() {
         b16_ = false;
         exc_ptr_ = openfpm_dcpse::device_array<int64_t>();
@@ -582,7 +636,7 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
         mirrored_ = false;
         b16_ = b16_tried_ = false;
         n_apply32_ = 0;
-        s64_ = s64_tried_ = false;
+        s64_ = s64_tried_ = b64_ = false;
         n_apply64_ = 0;
         ctr_ = particles_.getMapCtr();
     }
@@ -623,6 +677,12 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
     // FP64 weights in the list's sliced-ELL slot order, used by FP64 applies
     openfpm_dcpse::device_array<double> sellW64_;
     bool s64_ = false, s64_tried_ = false;
+    // the 16-bit FP64 bundle used by FP64 applies of one set (over the list's slices)
+    openfpm_dcpse::device_array<int64_t> exc_ptr64_;
+    openfpm_dcpse::device_array<int32_t> row16_64_, bundle64_, exc_col64_;
+    openfpm_dcpse::device_array<double> exc_W64_;
+    int64_t n_exc64_ = 0;
+    bool b64_ = false;
     int n_apply64_ = 0;  // FP64 applies since assembly
     std::vector<double> eps_host_, eip_host_, K_host_;
     std::vector<int64_t> rp_host_;

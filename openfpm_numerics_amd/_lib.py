@@ -84,6 +84,9 @@ SIGNATURES = {
     "dcpse_sell_fill_bundle16_f32": (_i32, [_vp, _i64, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp]),
     "dcpse_apply_sell_bundle16_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp,
                                              _f32, _f32]),
+    "dcpse_sell_fill_bundle16_f64": (_i32, [_vp, _i64, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp]),
+    "dcpse_apply_sell_bundle16_f64": (_i32, [_vp, _i64, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp,
+                                             _f64, _f64]),
     "dcpse_export_csr_f64": (_i32, [_vp, _i64, _vp, _vp, _vp, _f64, _vp, _vp, _vp]),
     "dcpse_export_csr_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _f64, _vp, _vp, _vp]),
     "dcpse_stage_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _f32, _vp, _f32, _vp, _vp, _f32, _vp]),

@@ -272,12 +272,13 @@ __global__ __launch_bounds__(256) void k_sell_fill_bundle16(
 
 // The exception lists: one thread per lane whose row has far neighbours
 // walks the row in order.
+template <typename WT>
 __global__ void k_bundle16_exc(int64_t n_rows, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
-                               int n_ops, const float* W0, const float* W1, const float* W2, const float* W3,
+                               int n_ops, const WT* W0, const WT* W1, const WT* W2, const WT* W3,
                                const int32_t* __restrict__ sell_row, const int64_t* __restrict__ exc_ptr,
-                               int64_t n_exc, int32_t* __restrict__ exc_col, float* __restrict__ exc_W) {
+                               int64_t n_exc, int32_t* __restrict__ exc_col, WT* __restrict__ exc_W) {
     const int64_t n_lanes = ((n_rows + 63) / 64) * 64;
-    const float* W[4] = {W0, W1, W2, W3};
+    const WT* W[4] = {W0, W1, W2, W3};
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_lanes;
          i += (int64_t)gridDim.x * blockDim.x) {
         int64_t e = exc_ptr[i];
@@ -289,6 +290,53 @@ __global__ void k_bundle16_exc(int64_t n_rows, const int64_t* __restrict__ row_p
                 exc_col[e] = q;
                 for (int o = 0; o < n_ops; ++o) exc_W[o * n_exc + e] = W[o][r.rs + j];
                 ++e;
+            }
+        }
+    }
+}
+
+// The FP64 16-bit bundle: slot group g of slice s is one block of
+// 128 + 512 * n_ops words (the 64 lanes' four 16-bit offsets, then each
+// operator's four FP64 weights per lane, 32 bytes) at word offset
+// (128 + 512 n_ops) * slice_ptr[s] / 256 + g (128 + 512 n_ops). Thread
+// (lane l, t) of a 256-thread block writes lane l's slot groups t, t + 4, ...
+// of the slice directly from its CSR row (32 contiguous bytes per weight
+// stream; a once-per-assembly build).
+__global__ __launch_bounds__(256) void k_sell_fill_bundle16_f64(
+    int64_t n_rows, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ col, int n_ops,
+    const double* W0, const double* W1, const double* W2, const double* W3, const int64_t* __restrict__ slice_ptr,
+    const int32_t* __restrict__ sell_row, const int64_t* __restrict__ exc_ptr, int32_t* __restrict__ bundle,
+    int32_t* __restrict__ sell_row16) {
+    const int64_t n_slices = (n_rows + 63) / 64;
+    const double* W[4] = {W0, W1, W2, W3};
+    const int64_t blk = 128 + 512 * (int64_t)n_ops;
+    int16_t* cols16 = reinterpret_cast<int16_t*>(bundle);
+    const int l = threadIdx.x & 63, gsub = threadIdx.x >> 6;
+    for (int64_t s = blockIdx.x; s < n_slices; s += gridDim.x) {
+        const int64_t i = s * 64 + l;
+        const SliceLane r = slice_lane(row_ptr, sell_row, i);
+        if (gsub == 0)
+            sell_row16[i] = r.row < 0 ? r.row : (exc_ptr[i + 1] > exc_ptr[i] ? (r.row | kExcFlag) : r.row);
+        const int64_t base = slice_ptr[s];
+        const int ng = (int)((slice_ptr[s + 1] - base) >> 8);
+        for (int g = gsub; g < ng; g += 4) {
+            const int64_t gb = (base / 256 + g) * blk;  // word offset of the slot group's block
+            vs4a dv;
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                const int j = 4 * g + jj;
+                int32_t d = 0;
+                if (j < r.len) {
+                    const int64_t dj = col_offset(col[r.rs + j], r.row, n_rows);
+                    d = far16(dj) ? 0 : (int32_t)dj;
+                }
+                dv[jj] = (int16_t)d;
+            }
+            *reinterpret_cast<vs4a*>(cols16 + 2 * gb + 4 * l) = dv;
+            for (int o = 0; o < n_ops; ++o) {
+                double* wb = reinterpret_cast<double*>(bundle + gb + 128 + o * 512) + 4 * l;
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) wb[jj] = 4 * g + jj < r.len ? W[o][r.rs + 4 * g + jj] : 0.0;
             }
         }
     }
@@ -538,9 +586,14 @@ struct Sell64Args {
     int64_t n_rows;
     const int64_t* row_ptr;
     const int64_t* slice_ptr;
-    const int32_t* sell_row;
+    const int32_t* sell_row;  // 16-bit bundle: sell_row16 (bit 30: the lane has far neighbours)
     const int32_t* sell_col;
     const double* W[4];
+    const int32_t* bundle;    // 16-bit bundle (k_sell_fill_bundle16_f64), or null
+    const int64_t* exc_ptr;   // 16-bit bundle: far neighbours per lane
+    const int32_t* exc_col;
+    const double* exc_W;
+    int64_t n_exc;
     const double* f;
     const double* f_self;
     double* out[4];
@@ -548,36 +601,44 @@ struct Sell64Args {
     int64_t blocks_needed, per_xcd;
 };
 
-template <int NOPS, int UNR>
+template <int NOPS, int UNR, bool C16 = false>
 struct Sell64Batch {
-    vi4a q[UNR];
+    typename std::conditional<C16, vs4a, vi4a>::type q[UNR];  // columns, or 16-bit offsets
     vd4a w[NOPS][UNR];
 };
 
-template <int NOPS, int UNR>
-__device__ __forceinline__ void sell64_load(const int32_t* cp, const double* const* wp, int g0, int len,
-                                            Sell64Batch<NOPS, UNR>& b) {
+// gs: element stride between a lane's consecutive slot groups (the columns'
+// unit: int32 for the separate arrays, int16 for the bundle; weights double)
+template <int NOPS, int UNR, bool C16 = false>
+__device__ __forceinline__ void sell64_load(const void* cp, const double* const* wp, int64_t gsc, int64_t gsw,
+                                            int g0, int len, Sell64Batch<NOPS, UNR, C16>& b) {
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
         const int g = g0 + u;
-        const int64_t off = 4 * g < len ? (int64_t)g * 256 : 0;  // past the row: group 0 again
-        b.q[u] = *(const vi4a*)(cp + off);
+        const int64_t gg = 4 * g < len ? (int64_t)g : 0;  // past the row: group 0 again
+        if constexpr (C16)
+            b.q[u] = *(const vs4a*)(reinterpret_cast<const int16_t*>(cp) + gg * gsc);
+        else
+            b.q[u] = *(const vi4a*)(reinterpret_cast<const int32_t*>(cp) + gg * gsc);
 #pragma unroll
-        for (int o = 0; o < NOPS; ++o) b.w[o][u] = *(const vd4a*)(wp[o] + off);
+        for (int o = 0; o < NOPS; ++o) b.w[o][u] = *(const vd4a*)(wp[o] + gg * gsw);
     }
 }
 
-template <int NOPS, int UNR>
-__device__ __forceinline__ void sell64_gather(const Sell64Args& a, const Sell64Batch<NOPS, UNR>& b,
-                                              double (&fq)[UNR][4]) {
+template <int NOPS, int UNR, bool C16 = false>
+__device__ __forceinline__ void sell64_gather(const Sell64Args& a, const Sell64Batch<NOPS, UNR, C16>& b,
+                                              double (&fq)[UNR][4], int32_t rb) {
 #pragma unroll
     for (int u = 0; u < UNR; ++u)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) fq[u][i] = a.f[b.q[u][i]];
+        for (int i = 0; i < 4; ++i) {
+            if constexpr (C16) fq[u][i] = a.f[sell_col16(rb, b.q[u][i], (int32_t)a.n_rows)];
+            else fq[u][i] = a.f[b.q[u][i]];
+        }
 }
 
-template <int NOPS, int UNR>
-__device__ __forceinline__ void sell64_fma(int g0, int len, double fp, const Sell64Batch<NOPS, UNR>& b,
+template <int NOPS, int UNR, bool C16 = false>
+__device__ __forceinline__ void sell64_fma(int g0, int len, double fp, const Sell64Batch<NOPS, UNR, C16>& b,
                                            const double (&fq)[UNR][4], double* acc) {
 #pragma unroll
     for (int u = 0; u < UNR; ++u)
@@ -590,7 +651,10 @@ __device__ __forceinline__ void sell64_fma(int g0, int len, double fp, const Sel
         }
 }
 
-template <int NOPS, int UNR>
+// C16: the 16-bit bundle (k_sell_fill_bundle16_f64); offsets are relative to
+// the lane's row, and a lane flagged in sell_row16 adds its far neighbours
+// from the exception list after the slot loop.
+template <int NOPS, int UNR, bool C16 = false>
 __global__ __launch_bounds__(kSellWaves * kWave) void k_apply_sell64(const Sell64Args a) {
     const int64_t hb = blockIdx.x;
     const int64_t lb = (hb & 7) * a.per_xcd + (hb >> 3);
@@ -598,36 +662,62 @@ __global__ __launch_bounds__(kSellWaves * kWave) void k_apply_sell64(const Sell6
     const int lane = threadIdx.x & 63;
     const int64_t s = lb * kSellWaves + (threadIdx.x >> 6);
     if (s * 64 >= a.n_rows) return;  // whole wavefront
-    const int32_t row = a.sell_row[s * 64 + lane];
-    const bool live = row >= 0;
+    const int32_t row_raw = a.sell_row[s * 64 + lane];
+    const bool live = row_raw >= 0;
+    const bool has_exc = C16 && live && (row_raw & kExcFlag);
+    const int32_t row = C16 && live ? (row_raw & ~kExcFlag) : row_raw;
     const int len = live ? (int)(a.row_ptr[row + 1] - a.row_ptr[row]) : 0;
     const double fp = live ? a.f_self[row] : 0.0;
     const int64_t base = a.slice_ptr[s];
     const int ng = (int)((a.slice_ptr[s + 1] - base) >> 8);
-    const int32_t* cp = a.sell_col + base + 4 * lane;
+    const void* cp;
     const double* wp[NOPS];
+    int64_t gsc, gsw;
+    if constexpr (C16) {
+        const int64_t blk = 128 + 512 * NOPS;  // words per slot group
+        const int32_t* bp = a.bundle + (base / 256) * blk;
+        cp = reinterpret_cast<const int16_t*>(bp) + 4 * lane;
 #pragma unroll
-    for (int o = 0; o < NOPS; ++o) wp[o] = a.W[o] + base + 4 * lane;
+        for (int o = 0; o < NOPS; ++o) wp[o] = reinterpret_cast<const double*>(bp + 128 + o * 512) + 4 * lane;
+        gsc = 2 * blk;  // int16 units
+        gsw = blk / 2;  // double units
+    } else {
+        cp = a.sell_col + base + 4 * lane;
+#pragma unroll
+        for (int o = 0; o < NOPS; ++o) wp[o] = a.W[o] + base + 4 * lane;
+        gsc = gsw = 256;
+    }
+    const int32_t rb = live ? row : 0;
     double acc[NOPS];
 #pragma unroll
     for (int o = 0; o < NOPS; ++o) acc[o] = 0.0;
     if (ng > 0) {
-        Sell64Batch<NOPS, UNR> bA, bB;
+        Sell64Batch<NOPS, UNR, C16> bA, bB;
         double gA[UNR][4], gB[UNR][4];
-        sell64_load<NOPS, UNR>(cp, wp, 0, len, bA);
+        sell64_load<NOPS, UNR, C16>(cp, wp, gsc, gsw, 0, len, bA);
         for (int g0 = 0; g0 < ng; g0 += 2 * UNR) {
-            sell64_gather<NOPS, UNR>(a, bA, gA);
+            sell64_gather<NOPS, UNR, C16>(a, bA, gA, rb);
             __builtin_amdgcn_sched_barrier(0);
-            sell64_load<NOPS, UNR>(cp, wp, g0 + UNR, len, bB);
+            sell64_load<NOPS, UNR, C16>(cp, wp, gsc, gsw, g0 + UNR, len, bB);
             __builtin_amdgcn_sched_barrier(0);
-            sell64_fma<NOPS, UNR>(g0, len, fp, bA, gA, acc);
+            sell64_fma<NOPS, UNR, C16>(g0, len, fp, bA, gA, acc);
             __builtin_amdgcn_sched_barrier(0);
-            sell64_gather<NOPS, UNR>(a, bB, gB);
+            sell64_gather<NOPS, UNR, C16>(a, bB, gB, rb);
             __builtin_amdgcn_sched_barrier(0);
-            sell64_load<NOPS, UNR>(cp, wp, g0 + 2 * UNR, len, bA);
+            sell64_load<NOPS, UNR, C16>(cp, wp, gsc, gsw, g0 + 2 * UNR, len, bA);
             __builtin_amdgcn_sched_barrier(0);
-            sell64_fma<NOPS, UNR>(g0 + UNR, len, fp, bB, gB, acc);
+            sell64_fma<NOPS, UNR, C16>(g0 + UNR, len, fp, bB, gB, acc);
             __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    if constexpr (C16) {
+        if (has_exc) {  // the far neighbours of this row (rare; after every pipelined load)
+            const int64_t i = s * 64 + lane;
+            for (int64_t k = a.exc_ptr[i]; k < a.exc_ptr[i + 1]; ++k) {
+                const double d = a.f[a.exc_col[k]] - fp;
+#pragma unroll
+                for (int o = 0; o < NOPS; ++o) acc[o] = fma(a.exc_W[o * a.n_exc + k], d, acc[o]);
+            }
         }
     }
     if (!live) return;
@@ -639,21 +729,21 @@ __global__ __launch_bounds__(kSellWaves * kWave) void k_apply_sell64(const Sell6
     }
 }
 
-template <int NOPS, int UNR>
+template <int NOPS, int UNR, bool C16 = false>
 static void launch_sell64(Sell64Args a, hipStream_t st) {
     const int64_t n_slices = (a.n_rows + 63) / 64;
     a.blocks_needed = (n_slices + kSellWaves - 1) / kSellWaves;
     a.per_xcd = (a.blocks_needed + 7) / 8;
-    k_apply_sell64<NOPS, UNR><<<(unsigned)(a.per_xcd * 8), kSellWaves * kWave, 0, st>>>(a);
+    k_apply_sell64<NOPS, UNR, C16><<<(unsigned)(a.per_xcd * 8), kSellWaves * kWave, 0, st>>>(a);
 }
 
-template <int UNR>
+template <int UNR, bool C16 = false>
 static bool dispatch_sell64(const Sell64Args& a, int nops, hipStream_t st) {
     switch (nops) {
-        case 1: launch_sell64<1, UNR>(a, st); return true;
-        case 2: launch_sell64<2, UNR>(a, st); return true;
-        case 3: launch_sell64<3, UNR>(a, st); return true;
-        case 4: launch_sell64<4, UNR>(a, st); return true;
+        case 1: launch_sell64<1, UNR, C16>(a, st); return true;
+        case 2: launch_sell64<2, UNR, C16>(a, st); return true;
+        case 3: launch_sell64<3, UNR, C16>(a, st); return true;
+        case 4: launch_sell64<4, UNR, C16>(a, st); return true;
     }
     return false;
 }
@@ -895,6 +985,72 @@ dcpse_status dcpse_apply_sell_f64(dcpse_ctx* ctx, int64_t n_rows, const int64_t*
     return DCPSE_OK;
 }
 
+dcpse_status dcpse_sell_fill_bundle16_f64(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr,
+                                          const int32_t* col, int32_t n_ops, const double* const* W,
+                                          const int64_t* slice_ptr, const int32_t* sell_row,
+                                          const int64_t* exc_ptr, int64_t n_exc, int32_t* bundle,
+                                          int32_t* sell_row16, int32_t* exc_col, double* exc_W) {
+    if (!ctx) return DCPSE_ERR_INVALID;
+    if (!row_ptr || !col || !W || !slice_ptr || !sell_row || !exc_ptr || !bundle || !sell_row16 || n_rows < 0 ||
+        n_exc < 0 || (n_exc > 0 && (!exc_col || !exc_W)))
+        return fail(ctx, DCPSE_ERR_INVALID, "null pointer or negative size");
+    if (n_ops < 1 || n_ops > 4) return fail(ctx, DCPSE_ERR_UNSUPPORTED, "1 to 4 operators per bundle");
+    for (int o = 0; o < n_ops; ++o)
+        if (!W[o]) return fail(ctx, DCPSE_ERR_INVALID, "null weight array");
+    if (n_rows >= ((int64_t)1 << 30)) return fail(ctx, DCPSE_ERR_RANGE, "16-bit bundles need n_rows < 2^30");
+    DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    if (n_rows == 0) return DCPSE_OK;
+    const int64_t n_slices = (n_rows + 63) / 64;
+    const int64_t b = std::min<int64_t>(n_slices, (int64_t)ctx->num_cus * 16);
+    k_sell_fill_bundle16_f64<<<(unsigned)b, 256, 0, ctx->stream>>>(
+        n_rows, row_ptr, col, n_ops, W[0], n_ops > 1 ? W[1] : nullptr, n_ops > 2 ? W[2] : nullptr,
+        n_ops > 3 ? W[3] : nullptr, slice_ptr, sell_row, exc_ptr, bundle, sell_row16);
+    DCPSE_LAUNCH_CHECK(ctx);
+    if (n_exc > 0) {
+        const int64_t be = std::min<int64_t>((n_rows + 63 + 255) / 256, 65536);
+        k_bundle16_exc<double><<<(unsigned)be, 256, 0, ctx->stream>>>(
+            n_rows, row_ptr, col, n_ops, W[0], n_ops > 1 ? W[1] : nullptr, n_ops > 2 ? W[2] : nullptr,
+            n_ops > 3 ? W[3] : nullptr, sell_row, exc_ptr, n_exc, exc_col, exc_W);
+    }
+    DCPSE_LAUNCH_CHECK(ctx);
+    return DCPSE_OK;
+}
+
+dcpse_status dcpse_apply_sell_bundle16_f64(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr,
+                                           const int64_t* slice_ptr, const int32_t* sell_row16, int32_t n_ops,
+                                           const int32_t* bundle, const int64_t* exc_ptr, const int32_t* exc_col,
+                                           const double* exc_W, int64_t n_exc, const double* f,
+                                           const double* f_self, double* const* out, double alpha, double beta) {
+    if (!ctx) return DCPSE_ERR_INVALID;
+    if (!row_ptr || !slice_ptr || !sell_row16 || !bundle || !exc_ptr || !f || !out || n_rows < 0 || n_ops < 1 ||
+        n_exc < 0 || (n_exc > 0 && (!exc_col || !exc_W)))
+        return fail(ctx, DCPSE_ERR_INVALID, "null pointer or bad size");
+    if (n_ops > 4) return fail(ctx, DCPSE_ERR_UNSUPPORTED, "at most 4 operators per pass");
+    for (int o = 0; o < n_ops; ++o)
+        if (!out[o]) return fail(ctx, DCPSE_ERR_INVALID, "null output pointer");
+    DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    if (n_rows == 0) return DCPSE_OK;
+    Sell64Args a{};
+    a.n_rows = n_rows;
+    a.row_ptr = row_ptr;
+    a.slice_ptr = slice_ptr;
+    a.sell_row = sell_row16;
+    a.bundle = bundle;
+    a.exc_ptr = exc_ptr;
+    a.exc_col = exc_col;
+    a.exc_W = exc_W;
+    a.n_exc = n_exc;
+    for (int o = 0; o < 4; ++o) a.out[o] = o < n_ops ? out[o] : nullptr;
+    a.f = f;
+    a.f_self = f_self ? f_self : f;
+    a.alpha = alpha;
+    a.beta = beta;
+    if (!dispatch_sell64<2, true>(a, n_ops, ctx->stream))
+        return fail(ctx, DCPSE_ERR_UNSUPPORTED, "no SELL kernel for %d operators", n_ops);
+    DCPSE_LAUNCH_CHECK(ctx);
+    return DCPSE_OK;
+}
+
 static dcpse_status sell_apply(dcpse_ctx* ctx, int mode, int64_t n_rows, const int64_t* row_ptr,
                                const int64_t* slice_ptr, const int32_t* sell_row, const int32_t* sell_col,
                                const int32_t* bundle, int32_t n_ops, const float* const* sell_W, const float* f,
@@ -1010,7 +1166,7 @@ dcpse_status dcpse_sell_fill_bundle16_f32(dcpse_ctx* ctx, int64_t n_rows, const 
     DCPSE_LAUNCH_CHECK(ctx);
     if (n_exc > 0) {
         const int64_t be = std::min<int64_t>((n_rows + 63 + 255) / 256, 65536);
-        k_bundle16_exc<<<(unsigned)be, 256, 0, ctx->stream>>>(
+        k_bundle16_exc<float><<<(unsigned)be, 256, 0, ctx->stream>>>(
             n_rows, row_ptr, col, n_ops, W[0], n_ops > 1 ? W[1] : nullptr, n_ops > 2 ? W[2] : nullptr,
             n_ops > 3 ? W[3] : nullptr, sell_row, exc_ptr, n_exc, exc_col, exc_W);
     }

@@ -208,12 +208,13 @@ class Dcpse:
             self._W["sell64"] = sw
         return self._W["sell64"]
 
-    def sell_bundle16(self):
-        """This operator alone as a 16-bit-offset bundle (see sell_bundle16),
-        built once per assembly."""
-        if self._W.get("bundle16") is None:
-            self._W["bundle16"] = sell_bundle16([self])
-        return self._W["bundle16"]
+    def sell_bundle16(self, dtype=torch.float32):
+        """This operator alone as a 16-bit-offset bundle (see sell_bundle16)
+        with FP32 or FP64 weights, built once per assembly."""
+        key = "bundle16" if dtype == torch.float32 else "bundle16_64"
+        if self._W.get(key) is None:
+            self._W[key] = sell_bundle16([self], dtype)
+        return self._W[key]
 
     def bundle16_suits(self) -> bool:
         """Whether the 16-bit bundle is the faster FP32 layout of this operator:
@@ -606,6 +607,14 @@ def apply_ops(ops, f: torch.Tensor, outs=None, f_domain: torch.Tensor | None = N
                  _lib.ptr(sell_col), len(ops), _lib.ptr_array(SW), _lib.ptr(f2), _lib.ptr(fd), _lib.ptr_array(outs), alpha, beta)
         return [o.reshape(*f.shape[:-1], n_rows) for o in outs]
     if f.dtype == torch.float64 and nb == 1 and USE_SELL and all([op._count_apply32("fwd64") for op in ops]):
+        if len(ops) == 1 and ops[0].bundle16_suits():
+            # one operator: its FP64 16-bit-offset bundle (18 bytes per neighbour)
+            slice_ptr = vl.sell_slices()[0]
+            bundle, row16, exc_ptr, exc_col, exc_W, ne = ops[0].sell_bundle16(torch.float64)
+            ctx.call("dcpse_apply_sell_bundle16_f64", n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(slice_ptr),
+                     _lib.ptr(row16), 1, _lib.ptr(bundle), _lib.ptr(exc_ptr), _lib.ptr(exc_col), _lib.ptr(exc_W), ne,
+                     _lib.ptr(f2), _lib.ptr(fd), _lib.ptr_array(outs), alpha, beta)
+            return [o.reshape(*f.shape[:-1], n_rows) for o in outs]
         # FP64: the 32-bit-column layout with FP64 weights in its slot order
         slice_ptr, sell_row, sell_col = vl.sell()
         SW = [op.sell_weights64() for op in ops]
@@ -637,17 +646,19 @@ def sell_bundle(ops) -> torch.Tensor:
     return bundle
 
 
-def sell_bundle16(ops):
+def sell_bundle16(ops, dtype=torch.float32):
     """The interleaved sliced-ELL copy with 16-bit column offsets (operators on
-    one particle set): (bundle, sell_row16, exc_ptr, exc_col, exc_W, n_exc),
-    the far neighbours (offset beyond +-32767 modulo N) listed per lane."""
+    one particle set) with FP32 or FP64 weights: (bundle, sell_row16, exc_ptr,
+    exc_col, exc_W, n_exc), the far neighbours (offset beyond +-32767 modulo
+    N) listed per lane."""
     if not all(op.same_set for op in ops):
         raise ValueError("16-bit column offsets need operators whose support and domain sets are the same")
-    return bundle16(ops[0].verletList, [op.weights(torch.float32) for op in ops])
+    return bundle16(ops[0].verletList, [op.weights(dtype) for op in ops])
 
 
 def bundle16(vl: VerletList, Ws):
-    """sell_bundle16 of FP32 weight arrays Ws in the CSR order of vl."""
+    """sell_bundle16 of weight arrays Ws (all FP32 or all FP64) in the CSR
+    order of vl."""
     slice_ptr, sell_row, n_slots = vl.sell_slices()
     dev = vl.row_ptr.device
     ctx = _lib.context(dev.index or 0)
@@ -657,11 +668,15 @@ def bundle16(vl: VerletList, Ws):
     ctx.call("dcpse_sell_bundle16_count", vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(vl.col), _lib.ptr(sell_row),
              _lib.ptr(exc_ptr), n_exc.ctypes.data)
     ne = int(n_exc[0])
-    bundle = torch.empty(max(n_slots // 2 + len(Ws) * n_slots, 4), dtype=torch.int32, device=dev)
+    dt = Ws[0].dtype
+    if dt not in (torch.float32, torch.float64) or any(W.dtype != dt for W in Ws):
+        raise TypeError("bundle weights must be all float32 or all float64")
+    words = 1 if dt == torch.float32 else 2  # 32-bit words per weight
+    bundle = torch.empty(max(n_slots // 2 + words * len(Ws) * n_slots, 8), dtype=torch.int32, device=dev)
     sell_row16 = torch.empty_like(sell_row)
     exc_col = torch.empty(max(ne, 1), dtype=torch.int32, device=dev)
-    exc_W = torch.empty(max(len(Ws) * ne, 1), dtype=torch.float32, device=dev)
-    ctx.call("dcpse_sell_fill_bundle16_f32", vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(vl.col), len(Ws),
+    exc_W = torch.empty(max(len(Ws) * ne, 1), dtype=dt, device=dev)
+    ctx.call("dcpse_sell_fill_bundle16_f32" if dt == torch.float32 else "dcpse_sell_fill_bundle16_f64", vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(vl.col), len(Ws),
              _lib.ptr_array(Ws), _lib.ptr(slice_ptr), _lib.ptr(sell_row), _lib.ptr(exc_ptr), ne, _lib.ptr(bundle),
              _lib.ptr(sell_row16), _lib.ptr(exc_col), _lib.ptr(exc_W))
     return bundle, sell_row16, exc_ptr, exc_col, exc_W, ne
@@ -684,13 +699,14 @@ class PreparedApply:
         nb, n = f2.shape
         self.ctx = _lib.context(f.device.index or 0)
         self.O = _lib.ptr_array(outs)
-        if sell == "bundle16" and f.dtype == torch.float32 and nb == 1:
+        if sell == "bundle16" and nb == 1:
             slice_ptr = vl.sell_slices()[0]
-            b16 = sell_bundle16(ops)
+            b16 = sell_bundle16(ops, f.dtype)
             self._keep = (ops, f, outs, b16, slice_ptr)
             bundle, sell_row16, exc_ptr, exc_col, exc_W, ne = b16
             self.n_exc = ne
-            self.fn = self.ctx.lib.dcpse_apply_sell_bundle16_f32
+            self.fn = (self.ctx.lib.dcpse_apply_sell_bundle16_f32 if f.dtype == torch.float32
+                       else self.ctx.lib.dcpse_apply_sell_bundle16_f64)
             self.args = (self.ctx.h, vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(slice_ptr), _lib.ptr(sell_row16),
                          len(ops), _lib.ptr(bundle), _lib.ptr(exc_ptr), _lib.ptr(exc_col), _lib.ptr(exc_W), ne,
                          _lib.ptr(f2), None, self.O, alpha, beta)
@@ -725,6 +741,7 @@ class PreparedApply:
         self.layout = next((name for fn, name in ((lib.dcpse_apply_sell_f32, "sell"),
                                                   (lib.dcpse_apply_sell_bundle_f32, "sell_bundle"),
                                                   (lib.dcpse_apply_sell_bundle16_f32, "sell_bundle16"),
+                                                  (lib.dcpse_apply_sell_bundle16_f64, "sell64_bundle16"),
                                                   (lib.dcpse_apply_sell_f64, "sell64"))
                             if self.fn is fn), "csr")
         self.sell = self.layout != "csr"

@@ -689,3 +689,54 @@ def test_prepared_apply_fp64_runs_the_sliced_ell_copy(cuda):
     want = fused(*ops, f=f)
     for o, wv in zip(outs, want):
         assert torch.equal(o, wv)
+
+
+@pytest.mark.parametrize("shuffle", [False, True])
+def test_fp64_bundle16_apply(shuffle, oracle, cuda):
+    """The FP64 16-bit-offset bundle (dcpse_sell_fill_bundle16_f64 /
+    dcpse_apply_sell_bundle16_f64): one operator through the operator call
+    and the fused pair through PreparedApply, against the FP64 sliced-ELL
+    kernel (bit for bit without far neighbours; <= 1e-13 of max|ref| with a
+    shuffled order, where far neighbours go through the per-lane lists) and
+    against the oracle (<= 1e-10 of max|ref|)."""
+    from openfpm_numerics_amd import Derivative_x, Laplacian, dcpse as D, workloads as W
+    from openfpm_numerics_amd.dcpse import PreparedApply
+
+    w = W.jittered_3d_holes(24 if not shuffle else 48, seed=14)  # 48^3: offsets beyond 16 bits
+    if shuffle:
+        perm = np.random.default_rng(1).permutation(w["pos"].shape[0])
+        w["pos"] = np.ascontiguousarray(w["pos"][perm])
+    ps = _ps(w, cuda)
+    vl = ps.getVerlet(w["rcut"])
+    lap = Laplacian(ps, vl, 2, w["rcut"])
+    dx = Derivative_x(ps, vl, 2, w["rcut"])
+    ops = [lap.dcpse, dx.dcpse]
+    f = torch.as_tensor(np.sin(2 * np.pi * w["pos"][:, 0]) * np.cos(2 * np.pi * w["pos"][:, 1]), device=cuda)
+    a = [torch.empty_like(f) for _ in ops]
+    b = [torch.empty_like(f) for _ in ops]
+    PreparedApply(ops, f, a, sell=True)()  # FP64 sliced-ELL, 32-bit columns
+    p16 = PreparedApply(ops, f, b, sell="bundle16")
+    assert p16.layout == "sell64_bundle16"
+    print(f"[fp64 bundle16] shuffle={shuffle} N={ps.size_local()} nnz={vl.nnz} far neighbours {p16.n_exc}",
+          flush=True)
+    assert (p16.n_exc > 0) == shuffle
+    p16()
+    for x, y in zip(a, b):
+        if shuffle:
+            assert float((x - y).abs().max()) <= 1e-13 * float(x.abs().max())
+        else:
+            assert torch.equal(x, y)
+    rp, col = vl.row_ptr.cpu().numpy(), vl.col.cpu().numpy()
+    lo, hi, per = w["box_lo"], w["box_hi"], w["periodic"]
+    refs = [oracle.assemble(w["pos"], rp, col, s, 2, lo, hi, per) for s in ((2, 0, 0), (0, 2, 0), (0, 0, 2))]
+    ref_dx = oracle.assemble(w["pos"], rp, col, (1, 0, 0), 2, lo, hi, per)
+    fn = f.cpu().numpy()
+    wants = [sum(oracle.apply(r, rp, col, fn) for r in refs), oracle.apply(ref_dx, rp, col, fn)]
+    for got, want in zip(b, wants):
+        assert np.abs(got.cpu().numpy() - want).max() <= 1e-10 * np.abs(want).max()
+    # one operator through the operator call: the FP64 bundle from the first apply on
+    if not shuffle:
+        assert D.SELL_AFTER == 1
+        got = dx(f)
+        assert dx.dcpse._W.get("bundle16_64") is not None
+        assert torch.equal(got, a[1])
