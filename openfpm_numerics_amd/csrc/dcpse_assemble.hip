@@ -12,8 +12,10 @@
 //   * A is factored by an in-register FP64 Cholesky (lane i holds row i,
 //     columns broadcast through LDS; two particles' systems at once for
 //     bases of up to 32 monomials) instead of the reference's
-//     colPivHouseholderQr (Dcpse.hpp:766); an optional FP64 iterative
-//     refinement step recomputes the residual against the moments.
+//     colPivHouseholderQr (Dcpse.hpp:766); two operators whose bases nest
+//     (the Laplacian and Dx, two operators of one basis) share one
+//     factorisation; an optional FP64 iterative refinement step recomputes
+//     the residual against the moments.
 //   * epilogue: lanes own neighbours again and write, for both operators of
 //     a pass,
 //        K_pq = w_q * sum_i a_i z_q^alpha_i              (Dcpse.hpp:798-809)
@@ -203,6 +205,18 @@ __device__ __forceinline__ void asm_fail(const AsmOp& op, int64_t p, int64_t rs,
     }
 }
 
+// 1/sqrt(d) of a Cholesky pivot: the hardware estimate r (~23 bits) and one
+// third-order step r (1 + e/2 + 3e^2/8), e = 1 - d r^2, whose error O(e^3)
+// is below the final rounding; one estimate and four dependent FP64
+// operations instead of a square root and a division. A pivot that is not
+// positive and finite gives NaN (negative or NaN: the estimate is NaN; zero
+// or +inf: d * r is 0 * inf).
+__device__ __forceinline__ double pivot_rsqrt(double d) {
+    const double r = __builtin_amdgcn_rsq(d);
+    const double e = fma(-d * r, r, 1.0);
+    return fma(r * e, fma(e, 0.375, 0.5), r);
+}
+
 // Moment-system factorisation and solve of one operator for the particles of
 // SLOTS lane groups at once (LS = 64 / SLOTS lanes each, one particle per
 // group): lane slot * LS + i holds row i of its particle's A = (M(alpha_i +
@@ -266,17 +280,8 @@ __device__ __forceinline__ void asm_factor_solve(const double* mom, int abase, d
 #pragma unroll
             for (int j = k + 1; j < M; ++j) cv[j] = col2[base + j].x;
         }
-        // 1/sqrt(d) from the hardware estimate and two Newton-Raphson steps
-        // r <- r (3 - d r^2) / 2 (quadratic convergence: ~23 -> 46 -> full
-        // bits): one estimate and a few FMAs instead of a square-root and a
-        // division sequence per step
-        double inv = __builtin_amdgcn_rsq(dkk);
-        const double hd = 0.5 * dkk;
-        inv = inv * fma(-hd * inv, inv, 1.5);
-        inv = inv * fma(-hd * inv, inv, 1.5);
-        // a pivot that is not positive and finite makes inv NaN (negative or
-        // NaN: the estimate is NaN; zero or +inf: 0 * inf in the Newton steps),
-        // and fma(inv, 0, s) is s exactly when inv is finite (an explicit fma:
+        const double inv = pivot_rsqrt(dkk);
+        // fma(inv, 0, s) is s exactly when inv is finite (an explicit fma:
         // inv - inv may be contracted with the product that forms inv into
         // that product's rounding error)
         dfin = fma(inv, 0.0, dfin);
@@ -354,6 +359,200 @@ __device__ __forceinline__ void asm_factor_solve(const double* mom, int abase, d
         xv += chol_solve(rr);
     }
     sx[lane] = xv;
+    wave_lds_sync();
+}
+
+// Two operators on one support whose bases nest: operator 1's basis is
+// operator 0's monomials that it shares (S of them) plus at most the constant
+// monomial (E = 1), e.g. Dx (degrees 0-2) beside the Laplacian (degrees 1-3),
+// or two operators of one basis (S = M0, E = 0). Operator 1's moment matrix is
+// then, in the row order [shared, constant], the leading S x S block of
+// operator 0's (rows ordered [shared, the rest]) bordered by the constant's
+// row, and its Cholesky factor is the first S columns of operator 0's factor
+// plus that row eliminated by the same S pivots: one factorisation of
+// operator 0's system with the constant's row appended as row M0 serves both.
+// p1[i], q2[i]: the operator-0 and operator-1 basis index of solve row i (-1
+// when the row is not in that basis); ex[i]: its exponents.
+template <int DIM, int L1, int A1, int L2, int A2>
+struct FusePlan {
+    static constexpr int M1 = BasisShape<DIM, L1, A1>::M;
+    static constexpr int M2 = BasisShape<DIM, (L2 > 0 ? L2 : 2), A2>::M;
+    bool ok;
+    int S, E;
+    int p1[M1 + 1], q2[M1 + 1], ex[M1 + 1][3];
+    static constexpr int find(const int (*e)[3], int m, const int* x) {
+        for (int k = 0; k < m; ++k)
+            if (e[k][0] == x[0] && e[k][1] == x[1] && e[k][2] == x[2]) return k;
+        return -1;
+    }
+    constexpr FusePlan() : ok(false), S(0), E(0), p1{}, q2{}, ex{} {
+        const BasisTables<DIM, L1, A1> B1{};
+        const BasisTables<DIM, (L2 > 0 ? L2 : 2), A2> B2{};
+        int s = 0;
+        for (int pass = 0; pass < 2; ++pass)
+            for (int i = 0; i < M1; ++i)
+                if ((find(B2.e, M2, B1.e[i]) >= 0) == (pass == 0)) p1[s++] = i;
+        for (int i = 0; i < M1; ++i) {
+            if (find(B2.e, M2, B1.e[i]) >= 0) ++S;
+            for (int d = 0; d < 3; ++d) ex[i][d] = B1.e[p1[i]][d];
+            q2[i] = find(B2.e, M2, ex[i]);
+        }
+        int extra = 0, idx = -1;
+        bool konst = true;
+        for (int k = 0; k < M2; ++k)
+            if (find(B1.e, M1, B2.e[k]) < 0) {
+                ++extra;
+                idx = k;
+                konst = konst && B2.e[k][0] == 0 && B2.e[k][1] == 0 && B2.e[k][2] == 0;
+            }
+        E = extra;
+        ok = L2 > 0 && S > 0 && (extra == 0 || (extra == 1 && konst));
+        p1[M1] = -1;
+        q2[M1] = extra == 1 ? idx : -1;
+        ex[M1][0] = ex[M1][1] = ex[M1][2] = 0;
+    }
+};
+template <int DIM, int L1, int A1, int L2, int A2>
+__constant__ FusePlan<DIM, L1, A1, L2, A2> kFuse = FusePlan<DIM, L1, A1, L2, A2>();
+
+// The factorisation and both solves of a nested pair (FusePlan) for two
+// particles at once, one per half-wave: lane slot * 32 + i holds solve row i
+// (M0 + E rows; abase: its moment row in its slot's grid; b1v, b2v: its
+// right-hand side entries of both operators). Right-looking Cholesky over the
+// M0 pivots of operator 0 as in asm_factor_solve, with both right-hand sides
+// riding along (operator 1's over the S shared pivots); the constant row's
+// Schur complement after the S shared pivots is operator 1's last pivot. The
+// backward substitutions share the L rows of the shared pivots; the optional
+// refinement step (x += A^-1 (b - A x) against the moments) does the same
+// for both operators. Solutions go to sx (operator 0) and sx2 (operator 1)
+// in basis order; ok1 / ok2 are false when a pivot of that operator's system
+// is not positive and finite.
+template <int DIM, int L1, int A1, int L2, int A2, int GW>
+__device__ __forceinline__ void asm_factor_solve_fused(const double* mom, int abase, double b1v, double b2v,
+                                                       int p1, int q2, int refine, double* lmat, double2* lbuf,
+                                                       double* sx, double* sx2, int lane, bool& ok1, bool& ok2) {
+    using FP = FusePlan<DIM, L1, A1, L2, A2>;
+    constexpr FP F{};
+    constexpr int M1 = FP::M1, S = F.S, E = F.E, MA = M1 + E;
+    constexpr int LS = kWave / 2;
+    static_assert(MA <= LS, "nested pair larger than a half-wave");
+    const int slot = lane / LS;
+    const int base = slot * LS;
+    int me = lane - base;
+    asm volatile("" : "+v"(me));
+    const int li = me < MA ? me : 0;
+
+    double ar[MA];
+#pragma unroll
+    for (int k = 0; k < MA; ++k) ar[k] = mom[abase + grid_index<GW>(F.ex[k][0], F.ex[k][1], F.ex[k][2])];
+    double dfin = 0.0, dfin_s = 0.0, dinv = 0.0, dc = 0.0;
+    double rb = b1v, yb = 0.0, rb2 = b2v, yb2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < M1; ++k) {
+        // alternate buffers between steps; operator 1's right-hand side
+        // column goes through the solution buffers, free until the end
+        double2* col2 = lbuf + (k & 1) * kWave;
+        double* colb = (k & 1) ? sx : sx2;
+        col2[lane] = make_double2(ar[k], rb);
+        if (k < S) colb[lane] = rb2;
+        wave_lds_sync();
+        const double2 pk = col2[base + k];
+        const double dkk = pk.x, rk = pk.y;
+        const double rk2 = k < S ? colb[base + k] : 0.0;
+        double cv[MA];
+#pragma unroll
+        for (int j = k + 1; j < MA; ++j) cv[j] = col2[base + j].x;
+        const double inv = pivot_rsqrt(dkk);
+        dfin = fma(inv, 0.0, dfin);  // s exactly when inv is finite
+        dinv = (me == k) ? inv : dinv;
+        ar[k] *= inv;
+        const double t = ar[k] * inv;
+        const double yk = rk * inv;
+        yb = (me == k) ? yk : yb;
+        rb = fma(-ar[k], yk, rb);
+        if (k < S) {
+            const double yk2 = rk2 * inv;
+            yb2 = (me == k) ? yk2 : yb2;
+            rb2 = fma(-ar[k], yk2, rb2);
+        }
+#pragma unroll
+        for (int j = k + 1; j < MA; ++j) ar[j] = fma(-t, cv[j], ar[j]);
+        if (k == S - 1) {
+            dfin_s = dfin;
+            if (E) dc = ar[MA - 1];  // the constant row's pivot (its lane only)
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    ok1 = dfin == 0.0;
+    // unknown k of each half-wave to all lanes of the half (v_readlane per half)
+    auto bcast = [&](double v, int k) -> double {
+        const double y0 = readlane_d(v, k), y1 = readlane_d(v, LS + k);
+        return slot ? y1 : y0;
+    };
+    double dinv2 = dinv;
+    if constexpr (E > 0) {
+        const double ic = pivot_rsqrt(bcast(dc, M1));
+        dfin_s = fma(ic, 0.0, dfin_s);
+        dinv2 = (me == M1) ? ic : dinv;
+        yb2 = (me == M1) ? rb2 * ic : yb2;
+    }
+    ok2 = dfin_s == 0.0;
+    // strictly lower L rows to LDS (row stride MS): the substitutions read
+    // column `me`
+#pragma unroll
+    for (int k = 0; k < MA; ++k) ar[k] = k < me ? ar[k] : 0.0;
+    constexpr int MS = MA + (MA & 1);
+    double* lm = lmat + slot * (MA * MS);
+    if (me < MA) {
+#pragma unroll
+        for (int k = 0; k < MA; ++k) lm[me * MS + k] = ar[k];
+    }
+    wave_lds_sync();
+    // backward substitutions L^T x = y: operator 0 over its M0 unknowns,
+    // operator 1 over the constant's (row M0) then the shared ones, which
+    // share their L entries with operator 0's; lane i's sums are final from
+    // its own step on
+    auto back = [&](double& t1, double& t2) {
+#pragma unroll
+        for (int j = M1 - 1; j >= S; --j) t1 = fma(-lm[j * MS + li], bcast(t1 * dinv, j), t1);
+        if constexpr (E > 0) t2 = fma(-lm[M1 * MS + li], bcast(t2 * dinv2, M1), t2);
+#pragma unroll
+        for (int j = S - 1; j >= 0; --j) {
+            const double l = lm[j * MS + li];
+            t1 = fma(-l, bcast(t1 * dinv, j), t1);
+            t2 = fma(-l, bcast(t2 * dinv2, j), t2);
+        }
+        t1 *= dinv;
+        t2 *= dinv2;
+    };
+    double x1 = yb, x2 = yb2;
+    back(x1, x2);
+    if (refine) {
+        // residuals against the moments in solve order (the unknowns through
+        // LDS, one per lane), then L y = r forward (row i of L in ar) and back
+        sx[lane] = x1;
+        sx2[lane] = x2;
+        wave_lds_sync();
+        double r1 = b1v, r2 = b2v;
+#pragma unroll
+        for (int k = 0; k < MA; ++k) {
+            const double aik = mom[abase + grid_index<GW>(F.ex[k][0], F.ex[k][1], F.ex[k][2])];
+            if (k < M1) r1 = fma(-aik, sx[base + k], r1);
+            if (k < S || k == M1) r2 = fma(-aik, sx2[base + k], r2);
+        }
+#pragma unroll
+        for (int k = 0; k < M1; ++k) r1 = fma(-ar[k], bcast(r1 * dinv, k), r1);
+#pragma unroll
+        for (int k = 0; k < S; ++k) r2 = fma(-ar[k], bcast(r2 * dinv2, k), r2);
+        r1 *= dinv;
+        r2 *= dinv2;
+        back(r1, r2);
+        x1 += r1;
+        x2 += r2;
+        wave_lds_sync();
+    }
+    if (p1 >= 0) sx[base + p1] = x1;
+    if (q2 >= 0) sx2[base + q2] = x2;
     wave_lds_sync();
 }
 
@@ -503,7 +702,8 @@ constexpr int asm_waves() {
 // displacements and the moments (the moment set covers both). ε and the
 // moments are computed one particle at a time by the whole wave; when the
 // bases fit in half a wave (m <= 32), two particles' systems are factored and
-// solved together, one per half.
+// solved together, one per half, and a nested pair (FusePlan) factors one
+// system for both operators.
 template <int DIM, int L1, int A1, int L2 = 0, int A2 = 0, int CH = 32, bool PQ = false>
 __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_assemble(const AsmArgs a) {
     constexpr int P1 = 2 * (L1 - 1), P2 = L2 > 0 ? 2 * (L2 - 1) : 0;
@@ -527,7 +727,13 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
     constexpr int ZW = DIM == 3 ? PW : 1;
     constexpr int TROW = 1 + PW + NX + ZW;
     static_assert(CH == 32 || CH == 64, "neighbour chunk of 32 or 64");
-    constexpr int TAB = (CH * TROW > SLOTS * MX * MSX) ? CH * TROW : SLOTS * MX * MSX;
+    // nested pairs (FusePlan) share one factorisation of M1 + E rows
+    using FP = FusePlan<DIM, L1, A1, L2, A2>;
+    constexpr FP FPL{};
+    constexpr bool FUSE = L2 > 0 && SLOTS == 2 && FPL.ok && M1 + FPL.E <= kWave / 2;
+    constexpr int MF = M1 + FPL.E, MSF = MF + (MF & 1);
+    constexpr int LT = FUSE && SLOTS * MF * MSF > SLOTS * MX * MSX ? SLOTS * MF * MSF : SLOTS * MX * MSX;
+    constexpr int TAB = CH * TROW > LT ? CH * TROW : LT;
     // lanes per neighbour (the segments of the moment pass fit them); a pass
     // covers HALVES = 64 / LPN neighbours
     constexpr int NSEG = Seg::N;
@@ -560,6 +766,20 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
     const double b2 = L2 > 0 && lrow < M2 ? a.op[1].b[lrow] : 0.0;
     const int ab1 = lslot * MGP + mom_row_base<DIM, L1, A1, GW>(lrow);
     const int ab2 = L2 > 0 ? lslot * MGP + mom_row_base<DIM, (L2 > 0 ? L2 : 2), A2, GW>(lrow) : 0;
+    // nested pair: solve row lrow's basis indices, right-hand sides and moment row
+    int fp1 = -1, fq2 = -1, fab = 0;
+    double fb1 = 0.0, fb2 = 0.0;
+    if constexpr (FUSE) {
+        if (lrow < MF) {
+            fp1 = kFuse<DIM, L1, A1, L2, A2>.p1[lrow];
+            fq2 = kFuse<DIM, L1, A1, L2, A2>.q2[lrow];
+            const int e2 = DIM == 3 ? kFuse<DIM, L1, A1, L2, A2>.ex[lrow][2] : 0;
+            fab = grid_index<GW>(kFuse<DIM, L1, A1, L2, A2>.ex[lrow][0], kFuse<DIM, L1, A1, L2, A2>.ex[lrow][1], e2);
+        }
+        fb1 = fp1 >= 0 ? a.op[0].b[fp1] : 0.0;
+        fb2 = fq2 >= 0 ? a.op[1].b[fq2] : 0.0;
+        fab += lslot * MGP;
+    }
 
     // Batches of a.ppb particles. Static grid (PQ false): batch = block, its
     // particles b * ppb + i. Persistent grid (PQ true): batches taken from a
@@ -814,9 +1034,23 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
             asm_factor_solve<DIM, LL, AA, GW, SLOTS>(mom, abase, b_lane, a.refine, tab, lbuf, xs, lane, okl);
             return __ballot(okl);
         };
-        const uint64_t okm1 = solve(BasisTag<L1, A1>{}, M1, ab1, b1, sx);
-        uint64_t okm2 = 0ull;
-        if constexpr (L2 > 0) okm2 = solve(BasisTag<(L2 > 0 ? L2 : 2), A2>{}, M2, ab2, b2, sx2);
+        uint64_t okm1 = 0ull, okm2 = 0ull;
+        if constexpr (FUSE) {
+            // one factorisation for both operators
+            bool any = false;
+#pragma unroll
+            for (int s = 0; s < SLOTS; ++s) any |= go_s[s] && n_s[s] >= MN;
+            if (any) {
+                bool o1, o2;
+                asm_factor_solve_fused<DIM, L1, A1, L2, A2, GW>(mom, fab, fb1, fb2, fp1, fq2, a.refine, tab, lbuf, sx,
+                                                                sx2, lane, o1, o2);
+                okm1 = __ballot(o1);
+                okm2 = __ballot(o2);
+            }
+        } else {
+            okm1 = solve(BasisTag<L1, A1>{}, M1, ab1, b1, sx);
+            if constexpr (L2 > 0) okm2 = solve(BasisTag<(L2 > 0 ? L2 : 2), A2>{}, M2, ab2, b2, sx2);
+        }
 #pragma unroll
         for (int s = 0; s < SLOTS; ++s) {
             if (!go_s[s]) continue;

@@ -671,6 +671,63 @@ def test_refinement_step_agrees(name, cuda):
         assert rel <= 1e-12
 
 
+@pytest.mark.parametrize("refine", [False, True])
+def test_nested_pairs_share_one_factorisation(refine, oracle, cuda):
+    """Pairs whose bases nest are factored once for both operators: Laplacian
+    + Dx (Dx's basis is the Laplacian's degree-1-2 monomials and the
+    constant), Dx + Dy and Dxx + Dyy (one basis). On an open 10^3 box at rCut
+    2.3h the corner particles have 10-18 neighbours: enough for Dx (10
+    monomials), too few for the Laplacian (19), so the pair marks only the
+    Laplacian TOO_FEW there. Every pair equals the oracle per row (1e-10, or
+    16 cond(A) u for ill-conditioned boundary rows) with the oracle's status
+    flags, and the single-operator assembly to the same bar, with and without
+    the refinement step."""
+    from _dcpse_tol import check_weights
+    from openfpm_numerics_amd import ParticleSet
+    from openfpm_numerics_amd.dcpse import LOAD, Dcpse, assemble_together
+
+    rng = np.random.default_rng(11)
+    h = 0.1
+    g = (np.arange(10) + 0.5) * h
+    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    pos += rng.uniform(-0.05 * h, 0.05 * h, pos.shape)
+    lo, hi = [0.0, 0.0, 0.0], [1.0, 1.0, 1.0]
+    ps = ParticleSet(pos, lo, hi, bc=[0, 0, 0], device=cuda)
+    rc = 2.3 * h
+    vl = ps.getVerlet(rc)
+    rp, col = vl.row_ptr.cpu().numpy(), vl.col.cpu().numpy()
+    nn = np.diff(rp)
+    few = (nn >= 10) & (nn < 19)
+    assert few.sum() >= 8, nn.min()
+    sig = {"lap": [(2, 0, 0), (0, 2, 0), (0, 0, 2)], "dx": [(1, 0, 0)], "dy": [(0, 1, 0)],
+           "dxx": [(2, 0, 0)], "dyy": [(0, 2, 0)]}
+
+    def make(k):
+        return Dcpse(ps, vl, sig[k][0], 2, rc, LOAD, signatures=sig[k], coeffs=[1.0] * len(sig[k]), refine=refine)
+
+    for pair in (("lap", "dx"), ("dx", "dy"), ("dxx", "dyy")):
+        sep = [make(k) for k in pair]
+        for d in sep:
+            assemble_together([d], ps, ps, check=False)
+        tog = [make(k) for k in pair]
+        assemble_together(tog, ps, ps, check=False)
+        for k, s, t in zip(pair, sep, tog):
+            refs = [oracle.assemble(pos, rp, col, b, 2, lo, hi, 0, dense_ev=False) for b in sig[k]]
+            info = t.info.cpu().numpy()
+            assert np.array_equal(info, refs[0].info), (pair, k)
+            assert np.array_equal(info, s.info.cpu().numpy()), (pair, k)
+            assert torch.equal(s.localEps, t.localEps)
+            Kt = t.calcKernels.cpu().numpy()
+            rel, nover, worst = check_weights(Kt, sum(r.K for r in refs), rp, pos, pos, col,
+                                              oracle.basis(sig[k][0], 2), label=f"{pair}/{k}")
+            sep_rel, _, _ = check_weights(Kt, s.calcKernels.cpu().numpy(), rp, pos, pos, col,
+                                          oracle.basis(sig[k][0], 2), label=f"{pair}/{k} separate")
+            print(f"[nested] refine={refine} {pair}/{k}: oracle {rel:.2e} ({nover} rows by cond, worst {worst:.1e}),"
+                  f" separate {sep_rel:.2e}, TOO_FEW rows {int((info == 1).sum())}", flush=True)
+        if pair[0] == "lap":
+            assert (tog[0].info.cpu().numpy()[few] == 1).all() and (tog[1].info.cpu().numpy()[few] == 0).all()
+
+
 def test_prepared_apply_fp64_runs_the_sliced_ell_copy(cuda):
     """PreparedApply with FP64 fields binds the FP64 sliced-ELL kernel and gives
     the bits of the operators' own FP64 apply."""
