@@ -169,7 +169,9 @@ dcpse_status dcpse_support_fill_subset(dcpse_ctx* ctx, int32_t dim,
  * betas/coeffs: host arrays (n_sig*dim, n_sig). Outputs (device, any may be
  * NULL): eps[n_dom], eps_inv_pow[n_dom] = eps^-|beta|, K64[nnz] raw kernel
  * (the reference's calcKernels), W32[nnz] and W64[nnz] = eps^-|beta| * K,
- * info[n_dom]. pos_dom == NULL means the domain set is the support set. */
+ * info[n_dom] (W64 left NULL costs nothing and is derived on demand by
+ * dcpse_scale_rows_f64). pos_dom == NULL means the domain set is the support
+ * set. */
 dcpse_status dcpse_assemble(dcpse_ctx* ctx, int32_t dim,
                             const double* pos_sup, int64_t n_sup,
                             const double* pos_dom, int64_t n_dom,
@@ -185,8 +187,11 @@ dcpse_status dcpse_assemble(dcpse_ctx* ctx, int32_t dim,
  * Operators whose bases pair up at order 2 (a second derivative or the
  * Laplacian with a first derivative, two operators of one basis) are
  * assembled two per kernel pass, sharing the neighbour gathers, epsilon and
- * the moments - the reference builds each Dcpse separately
- * (DCPSE_op.hpp:649-786, 1350-1487). Results equal separate dcpse_assemble calls. */
+ * the moments, and a pair whose bases nest (the Laplacian and a first
+ * derivative, two operators of one basis) one Cholesky factorisation - the
+ * reference builds each Dcpse separately (DCPSE_op.hpp:649-786, 1350-1487).
+ * Results equal separate dcpse_assemble calls to rounding (eps and the
+ * status flags exactly). */
 dcpse_status dcpse_assemble_multi(dcpse_ctx* ctx, int32_t dim,
                                   const double* pos_sup, int64_t n_sup,
                                   const double* pos_dom, int64_t n_dom,
@@ -416,6 +421,15 @@ dcpse_status dcpse_export_csr_f64(dcpse_ctx* ctx, int64_t n_rows, const int64_t*
 dcpse_status dcpse_export_csr_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int32_t* col,
                                   const float* W, double sign, int64_t* out_row_ptr, int32_t* out_col,
                                   float* out_val);
+
+/* FP64 pre-scaled weights W[j] = row_scale[p] * K[j] for every neighbour j of
+ * row p, one rounding per weight: the W64 output of dcpse_assemble (bit for
+ * bit, with row_scale = its eps_inv_pow) for an operator assembled with
+ * W64 = NULL. Replaces the per-row eps^-|beta| factor the reference applies
+ * to each row sum (src/DCPSE/Dcpse.hpp:467-499). Device pointers; K and W of
+ * row_ptr[n_rows] entries (non-null when n_rows > 0). */
+dcpse_status dcpse_scale_rows_f64(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const double* K,
+                                  const double* row_scale, double* W);
 
 /* ---- explicit time integration (df/dt = alpha * D f) ----------------- */
 /* One stage of an explicit Runge-Kutta step as ONE launch: the gather

@@ -144,7 +144,7 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
                                     f_self, n, O, n, alpha, beta),
                     "dcpse_apply_f32");
         } else {
-            const double* W[1] = {W64_.data()};
+            const double* W[1] = {w64()};
             c.check(dcpse_apply_f64(c.get(), n, verletList_.devRowPtr(), verletList_.devCol(), 1, W, f, n_sup, 1,
                                     f_self, n, O, n, alpha, beta),
                     "dcpse_apply_f64");
@@ -207,7 +207,7 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
                         "dcpse_apply_sell_f64");
                 return;
             }
-            const double* W[1] = {W64_.data()};
+            const double* W[1] = {w64()};
             c.check(dcpse_apply_f64(c.get(), n, verletList_.devRowPtr(), verletList_.devCol(), 1, W, f, n, 1,
                                     nullptr, n, O, n, alpha, beta),
                     "dcpse_apply_f64");
@@ -512,7 +512,7 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
             const auto& L = verletList_.sell(true);
             const int64_t n = (int64_t)verletList_.size();
             sellW64_.resize((size_t)std::max<int64_t>(L.n_slots, 4));
-            if (dcpse_sell_fill_weights_f64(c.get(), n, verletList_.devRowPtr(), W64_.data(), L.slice_ptr.data(),
+            if (dcpse_sell_fill_weights_f64(c.get(), n, verletList_.devRowPtr(), w64(), L.slice_ptr.data(),
                                             L.sell_row.data(), sellW64_.data())) {
                 sellW64_ = openfpm_dcpse::device_array<double>();
                 return;
@@ -524,6 +524,19 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
     }
 
    private:
+    // FP64 pre-scaled weights eps^-|beta| K, formed from the kernels at the
+    // first FP64 use (the assembly writes only K and the FP32 weights)
+    const double* w64() {
+        const size_t nnz = verletList_.nnz();
+        if (W64_.size() != nnz) {
+            auto& c = openfpm_dcpse::context();
+            W64_.resize(nnz);
+            c.check(dcpse_scale_rows_f64(c.get(), (int64_t)verletList_.size(), verletList_.devRowPtr(), K64_.data(),
+                                         eip_.data(), W64_.data()),
+                    "dcpse_scale_rows_f64");
+        }
+        return W64_.data();
+    }
     bool build_bundle16_64() {
         auto& c = openfpm_dcpse::context();
         b64_ = false;
@@ -543,7 +556,7 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
             row16_64_.resize(n_lanes);
             exc_col64_.resize((size_t)std::max<int64_t>(n_exc, 1));
             exc_W64_.resize((size_t)std::max<int64_t>(n_exc, 1));
-            const double* W[1] = {W64_.data()};
+            const double* W[1] = {w64()};
             if (dcpse_sell_fill_bundle16_f64(c.get(), n, verletList_.devRowPtr(), verletList_.devCol(), 1, W,
                                              L.slice_ptr.data(), L.sell_row.data(), exc_ptr64_.data(), n_exc,
                                              bundle64_.data(), row16_64_.data(), exc_col64_.data(),
@@ -593,7 +606,7 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
         if (opt != support_options::LOAD) assemble();
     }
     void check_fresh() {
-        if (W64_.size() == 0 && verletList_.nnz() != 0)
+        if (K64_.size() == 0 && verletList_.nnz() != 0)
             throw openfpm_dcpse::dcpse_error("DC-PSE operator has no kernels: call load() or update()");
         if (ctr_ != particles_.getMapCtr())
             throw openfpm_dcpse::dcpse_error("particles moved since assembly: call update() first");
@@ -605,7 +618,7 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
         eip_.resize(n);
         info_.resize(n);
         K64_.resize(nnz);
-        W64_.resize(nnz);
+        W64_ = openfpm_dcpse::device_array<double>();  // derived at the first FP64 use (w64)
         W32_.resize(nnz);
         if (n == 0) {  // an empty domain: no rows to assemble
             mirrored_ = false;
@@ -624,7 +637,7 @@ class Dcpse_hip : public openfpm_dcpse::field_operator<typename vector_type::pro
                                particles_.periodicMask(), verletList_.devRowPtr(), verletList_.devCol(),
                                (int32_t)coeffs_.size(), sigs_.data(), coeffs_.data(), convergenceOrder_,
                                (double)HOverEpsilon, refine ? DCPSE_ASM_REFINE : 0u, eps_.data(), eip_.data(), K64_.data(),
-                               W32_.data(), W64_.data(), info_.data()),
+                               W32_.data(), nullptr, info_.data()),
                 "dcpse_assemble");
         const std::vector<int32_t> info = info_.to_host();
         for (size_t p = 0; p < n; ++p)

@@ -89,6 +89,7 @@ SIGNATURES = {
                                              _f64, _f64]),
     "dcpse_export_csr_f64": (_i32, [_vp, _i64, _vp, _vp, _vp, _f64, _vp, _vp, _vp]),
     "dcpse_export_csr_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _f64, _vp, _vp, _vp]),
+    "dcpse_scale_rows_f64": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp]),
     "dcpse_stage_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _f32, _vp, _f32, _vp, _vp, _f32, _vp]),
     "dcpse_stage_f64": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _f64, _vp, _f64, _vp, _vp, _f64, _vp]),
     "dcpse_stage_sell_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _f32, _vp, _f32, _vp, _vp, _f32,

@@ -70,6 +70,21 @@ static dcpse_status export_csr(dcpse_ctx* ctx, int64_t n_rows, const int64_t* ro
     return DCPSE_OK;
 }
 
+// W_pq = s_p K_pq (G lanes per row): the pre-scaled FP64 weights of an
+// operator from its kernels and the row factors s_p = eps_p^-|beta| the
+// reference applies per row (Dcpse.hpp:467-499), one rounding per weight as
+// in the assembly's epilogue.
+template <int G>
+__global__ __launch_bounds__(256) void k_scale_rows(int64_t n_rows, const int64_t* __restrict__ row_ptr,
+                                                    const double* __restrict__ K, const double* __restrict__ s,
+                                                    double* __restrict__ W) {
+    const int64_t p = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;
+    if (p >= n_rows) return;
+    const double sp = s[p];
+    const int64_t re = row_ptr[p + 1];
+    for (int64_t j = row_ptr[p] + threadIdx.x % G; j < re; j += G) W[j] = K[j] * sp;
+}
+
 }  // namespace dcpse
 
 using namespace dcpse;
@@ -86,6 +101,20 @@ dcpse_status dcpse_export_csr_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t*
                                   const float* W, double sign, int64_t* out_row_ptr, int32_t* out_col,
                                   float* out_val) {
     return export_csr<float>(ctx, n_rows, row_ptr, col, W, sign, out_row_ptr, out_col, out_val);
+}
+
+dcpse_status dcpse_scale_rows_f64(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const double* K,
+                                  const double* row_scale, double* W) {
+    if (!ctx) return DCPSE_ERR_INVALID;
+    if (n_rows < 0 || (n_rows > 0 && (!row_ptr || !row_scale || !K || !W)))
+        return fail(ctx, DCPSE_ERR_INVALID, "null pointer or negative size");
+    if (n_rows == 0) return DCPSE_OK;
+    DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    const int64_t rows_per_block = 256 / kExportLanes;
+    const int64_t blocks = (n_rows + rows_per_block - 1) / rows_per_block;
+    k_scale_rows<kExportLanes><<<(unsigned)blocks, 256, 0, ctx->stream>>>(n_rows, row_ptr, K, row_scale, W);
+    DCPSE_LAUNCH_CHECK(ctx);
+    return DCPSE_OK;
 }
 
 }  // extern "C"

@@ -149,7 +149,7 @@ class Dcpse:
         return (torch.empty(n_dom, dtype=torch.float64, device=dev), torch.empty(n_dom, dtype=torch.float64, device=dev),
                 torch.empty(max(nnz, 1), dtype=torch.float64, device=dev),
                 torch.empty(max(nnz, 1), dtype=torch.float32, device=dev),
-                torch.empty(max(nnz, 1), dtype=torch.float64, device=dev),
+                None,  # FP64 pre-scaled weights: derived at the first FP64 use (weights)
                 torch.empty(n_dom, dtype=torch.int32, device=dev))
 
     def _adopt(self, out, map_ctr):
@@ -157,7 +157,7 @@ class Dcpse:
         nnz = self.verletList.nnz
         self.localEps, self.localEpsInvPow = eps, eip
         self.calcKernels = K64[:nnz]
-        self._W = {torch.float64: W64[:nnz], torch.float32: W32[:nnz]}
+        self._W = {torch.float32: W32[:nnz]}
         self._T = {}
         self._n32 = {"fwd": 0, "adj": 0, "fwd64": 0}
         self.info = info
@@ -185,6 +185,16 @@ class Dcpse:
 
     # ---- apply -------------------------------------------------------------------
     def weights(self, dtype) -> torch.Tensor:
+        """Pre-scaled weights W = eps_p^-|beta| K_pq in FP32 (written by the
+        assembly) or FP64 (formed from calcKernels and localEpsInvPow at the
+        first FP64 use, the product the assembly would have written)."""
+        if dtype == torch.float64 and dtype not in self._W and torch.float32 in self._W:
+            vl = self.verletList
+            K = self.calcKernels
+            W = torch.empty(max(vl.nnz, 1), dtype=torch.float64, device=K.device)
+            _lib.context(K.device.index or 0).call("dcpse_scale_rows_f64", vl.n_rows, _lib.ptr(vl.row_ptr),
+                                                   _lib.ptr(K), _lib.ptr(self.localEpsInvPow), _lib.ptr(W))
+            self._W[dtype] = W[:vl.nnz]
         if dtype not in self._W:
             raise TypeError(f"no weights for {dtype}; use float32 or float64")
         return self._W[dtype]
@@ -203,7 +213,7 @@ class Dcpse:
             slice_ptr, sell_row, n_slots = vl.sell_slices()
             sw = torch.empty(n_slots, dtype=torch.float64, device=vl.row_ptr.device)
             _lib.context(vl.row_ptr.device.index or 0).call(
-                "dcpse_sell_fill_weights_f64", vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(self._W[torch.float64]),
+                "dcpse_sell_fill_weights_f64", vl.n_rows, _lib.ptr(vl.row_ptr), _lib.ptr(self.weights(torch.float64)),
                 _lib.ptr(slice_ptr), _lib.ptr(sell_row), _lib.ptr(sw))
             self._W["sell64"] = sw
         return self._W["sell64"]

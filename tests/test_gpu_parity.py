@@ -486,6 +486,27 @@ def test_repeated_applies_are_bitwise_equal(name, cuda):
             assert torch.equal(op(f), first), (name, dt, "after reassembly")
 
 
+def test_fp64_weights_formed_on_demand(cuda):
+    """The assembly writes the kernels and the FP32 weights only; the FP64
+    pre-scaled weights eps_p^-|beta| K_pq are formed at the first FP64 use
+    (dcpse_scale_rows_f64), bit for bit the single product the assembly's
+    epilogue would write, and formed again after a reassembly."""
+    from openfpm_numerics_amd import Derivative_x, Laplacian, workloads as W
+
+    w = W.jittered_3d_holes(20, seed=11)
+    ps = _ps(w, cuda)
+    vl = ps.getVerlet(w["rcut"])
+    rows = torch.repeat_interleave(torch.arange(vl.n_rows, device=cuda), torch.diff(vl.row_ptr))
+    for op in (Laplacian(ps, vl, 2, w["rcut"]).dcpse, Derivative_x(ps, vl, 2, w["rcut"]).dcpse):
+        assert torch.float64 not in op._W
+        W64 = op.weights(torch.float64)
+        assert torch.equal(W64, op.calcKernels * op.localEpsInvPow[rows])
+        assert op.weights(torch.float64) is W64
+        op.initializeStaticSize(ps, ps)
+        assert torch.float64 not in op._W
+        assert torch.equal(op.weights(torch.float64), W64)
+
+
 @pytest.mark.parametrize("name", sorted(WORKLOADS))
 def test_sell_fp64_apply(name, oracle, cuda):
     """FP64 single-field applies switch to the FP64 sliced-ELL copy at the
@@ -671,36 +692,43 @@ def test_refinement_step_agrees(name, cuda):
         assert rel <= 1e-12
 
 
+@pytest.mark.parametrize("dim", [2, 3])
 @pytest.mark.parametrize("refine", [False, True])
-def test_nested_pairs_share_one_factorisation(refine, oracle, cuda):
+def test_nested_pairs_share_one_factorisation(dim, refine, oracle, cuda):
     """Pairs whose bases nest are factored once for both operators: Laplacian
     + Dx (Dx's basis is the Laplacian's degree-1-2 monomials and the
-    constant), Dx + Dy and Dxx + Dyy (one basis). On an open 10^3 box at rCut
-    2.3h the corner particles have 10-18 neighbours: enough for Dx (10
-    monomials), too few for the Laplacian (19), so the pair marks only the
-    Laplacian TOO_FEW there. Every pair equals the oracle per row (1e-10, or
-    16 cond(A) u for ill-conditioned boundary rows) with the oracle's status
-    flags, and the single-operator assembly to the same bar, with and without
-    the refinement step."""
+    constant), Dx + Dy and Dxx + Dyy (one basis). On an open box (10^3 in
+    3D, 20^2 in 2D) at rCut 2.3h the corner particles have enough neighbours
+    for Dx (10 monomials in 3D, 6 in 2D) but too few for the Laplacian (19,
+    9), so the pair marks only the Laplacian TOO_FEW there. Every pair equals
+    the oracle per row (1e-10, or 16 cond(A) u for ill-conditioned boundary
+    rows) with the oracle's status flags, and the single-operator assembly to
+    the same bar, with and without the refinement step."""
     from _dcpse_tol import check_weights
     from openfpm_numerics_amd import ParticleSet
     from openfpm_numerics_amd.dcpse import LOAD, Dcpse, assemble_together
 
     rng = np.random.default_rng(11)
-    h = 0.1
-    g = (np.arange(10) + 0.5) * h
-    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    side = 10 if dim == 3 else 20
+    h = 1.0 / side
+    g = (np.arange(side) + 0.5) * h
+    pos = np.stack(np.meshgrid(*([g] * dim), indexing="ij"), -1).reshape(-1, dim)
     pos += rng.uniform(-0.05 * h, 0.05 * h, pos.shape)
-    lo, hi = [0.0, 0.0, 0.0], [1.0, 1.0, 1.0]
-    ps = ParticleSet(pos, lo, hi, bc=[0, 0, 0], device=cuda)
+    lo, hi = [0.0] * dim, [1.0] * dim
+    ps = ParticleSet(pos, lo, hi, bc=[0] * dim, device=cuda)
     rc = 2.3 * h
     vl = ps.getVerlet(rc)
     rp, col = vl.row_ptr.cpu().numpy(), vl.col.cpu().numpy()
     nn = np.diff(rp)
-    few = (nn >= 10) & (nn < 19)
-    assert few.sum() >= 8, nn.min()
-    sig = {"lap": [(2, 0, 0), (0, 2, 0), (0, 0, 2)], "dx": [(1, 0, 0)], "dy": [(0, 1, 0)],
-           "dxx": [(2, 0, 0)], "dyy": [(0, 2, 0)]}
+    m_dx, m_lap = (10, 19) if dim == 3 else (6, 9)
+    few = (nn >= m_dx) & (nn < m_lap)
+    assert few.sum() >= 2 ** dim, nn.min()
+
+    def e(*k):
+        return tuple(k[:dim])
+
+    sig = {"lap": [e(2, 0, 0), e(0, 2, 0), e(0, 0, 2)][:dim], "dx": [e(1, 0, 0)], "dy": [e(0, 1, 0)],
+           "dxx": [e(2, 0, 0)], "dyy": [e(0, 2, 0)]}
 
     def make(k):
         return Dcpse(ps, vl, sig[k][0], 2, rc, LOAD, signatures=sig[k], coeffs=[1.0] * len(sig[k]), refine=refine)
@@ -722,7 +750,7 @@ def test_nested_pairs_share_one_factorisation(refine, oracle, cuda):
                                               oracle.basis(sig[k][0], 2), label=f"{pair}/{k}")
             sep_rel, _, _ = check_weights(Kt, s.calcKernels.cpu().numpy(), rp, pos, pos, col,
                                           oracle.basis(sig[k][0], 2), label=f"{pair}/{k} separate")
-            print(f"[nested] refine={refine} {pair}/{k}: oracle {rel:.2e} ({nover} rows by cond, worst {worst:.1e}),"
+            print(f"[nested] dim={dim} refine={refine} {pair}/{k}: oracle {rel:.2e} ({nover} rows by cond, worst {worst:.1e}),"
                   f" separate {sep_rel:.2e}, TOO_FEW rows {int((info == 1).sum())}", flush=True)
         if pair[0] == "lap":
             assert (tog[0].info.cpu().numpy()[few] == 1).all() and (tog[1].info.cpu().numpy()[few] == 0).all()
