@@ -97,6 +97,8 @@ def main():
                     help="only build the support and run REPS Laplacian + Dx assembly passes with the default "
                          "launch (assembly counter runs)")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-support assembly workload")
+    ap.add_argument("--no-adaptive", action="store_true",
+                    help="skip the adaptive-resolution workload (support, assembly, 100 RK4 steps)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -213,6 +215,61 @@ def main():
         log(f"[bench] uneven supports N={len(lu)} n {lu.min()}..{lu.max()}: Lap+Dx assembly {uneven_ms:.3f} ms "
             f"(tuned {ucfg}), static grid {uneven_static_ms:.3f} ms")
         del lapu, dxu, vlu, psu
+
+    # ---- adaptive resolution: 1M particles, h(x) = h0 (1 + 3x), per-particle
+    # rCut = 2.4 h(x_p); 100 RK4 steps of df/dt = 0.01 Lap f with FP32 state
+    # on the device against the same integration in FP64 ------------------------
+    adaptive = None
+    if not args.no_adaptive:
+        from openfpm_numerics_amd import rk4
+
+        wa = W.adaptive_3d(1_000_000, seed=5)
+        na = wa["pos"].shape[0]
+        psa = ParticleSet(wa["pos"], wa["box_lo"], wa["box_hi"], bc=[0, 1, 1], device=dev)
+        psa.getVerlet(wa["rcut"])  # warm
+        sup_a_ms, _ = events_ms(lambda: psa.getVerlet(wa["rcut"]), 1, torch)
+        vla = psa.getVerlet(wa["rcut"])
+        la = vla.row_lengths()
+        lapa = Laplacian(psa, vla, 2, wa["rcut"])
+        asm_a_ms, _ = events_ms(lambda: assemble_together([lapa.dcpse], psa, psa, check=False), 3, torch)
+        lapa.dcpse.check_info()
+        # x is not periodic: rows whose support crosses an x face are one-sided
+        # and grow under explicit integration; they are held fixed (zero rows)
+        xa, rca = wa["pos"][:, 0], wa["rcut"]
+        held = torch.as_tensor((xa - rca < 0.0) | (xa + rca > 1.0), device=dev)
+        nz_held = torch.repeat_interleave(held, la)
+        for dt_ in (torch.float64, torch.float32):
+            lapa.dcpse.weights(dt_)[nz_held] = 0
+        rows_a = torch.repeat_interleave(torch.arange(na, device=dev), la)
+        gersh = 2.0 * float(torch.zeros(na, dtype=torch.float64, device=dev)
+                            .index_add_(0, rows_a, lapa.dcpse.weights(torch.float64).abs()).max())
+        alpha, steps = 0.01, 100
+        dt = min(1e-4, 0.5 * 2.78 / (alpha * gersh))
+        fa0 = np.sin(2 * np.pi * wa["pos"][:, 1]) * np.sin(2 * np.pi * wa["pos"][:, 2])
+        fa64_0 = torch.as_tensor(fa0, device=dev)
+        fa32_0 = fa64_0.to(torch.float32)
+        rk4(lapa, fa32_0.clone(), dt, 1, alpha=alpha)  # builds the apply layouts
+        rk4(lapa, fa64_0.clone(), dt, 1, alpha=alpha)
+        out = {}
+
+        def run(f0, key):
+            f = f0.clone()
+            rk4(lapa, f, dt, steps, alpha=alpha)
+            out[key] = f
+
+        rk32_ms, _ = events_ms(lambda: run(fa32_0, "f32"), 1, torch)
+        rk64_ms, _ = events_ms(lambda: run(fa64_0, "f64"), 1, torch)
+        rel_a = float(torch.linalg.vector_norm(out["f32"].double() - out["f64"]) / torch.linalg.vector_norm(out["f64"]))
+        adaptive = {"workload": "3d_adaptive_h0(1+3x)_x_open_yz_periodic_rcut2.4h(x)_order2_lap_rk4",
+                    "particles": int(na), "nnz": int(vla.nnz),
+                    "neighbours_min_max": [int(la.min()), int(la.max())], "support_ms": sup_a_ms,
+                    "assembly_ms": asm_a_ms, "rk4_steps": steps, "dt": dt, "held_rows": int(held.sum()),
+                    "rk4_fp32_ms": rk32_ms, "rk4_fp64_ms": rk64_ms,
+                    "fp32_stage_us": rk32_ms / (4 * steps) * 1e3,
+                    "fp32_vs_fp64_rel_l2": rel_a}
+        log(f"[bench] adaptive N={na}: support {sup_a_ms:.2f} ms, Laplacian assembly {asm_a_ms:.2f} ms, "
+            f"{steps} RK4 steps FP32 {rk32_ms:.2f} ms / FP64 {rk64_ms:.2f} ms, FP32 vs FP64 {rel_a:.2e}")
+        del lapa, vla, psa, out
 
     # ---- apply --------------------------------------------------------------------
     f64, dx_exact, _, lap_exact = W.sin_field(w["pos"])
@@ -453,6 +510,7 @@ def main():
         "assembly_ms_static_grid": assembly_default_ms,
         "assembly_ms_by_operator": {"laplacian": asm_lap_ms, "dx": asm_dx_ms},
         "assembly_uneven": uneven,
+        "adaptive_rk4": adaptive,
         "assembly_ms_with_refinement": assembly_refined_ms,
         "support_ms": support_ms,
         "fp64_apply_us": fp64_apply_us,
