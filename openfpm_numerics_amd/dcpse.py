@@ -11,7 +11,8 @@ convergenceOrder, rCut, opt) and the two-set form (:111-130), the bulk apply
 
 Assembly runs the FP64 HIP kernel (dcpse_assemble) and keeps in HBM the
 reference's four arrays (localEps, localEpsInvPow, kerOffsets, calcKernels) plus
-the eps^-|beta| pre-scaled weights in FP64 and FP32 used by the apply kernels.
+the eps^-|beta| pre-scaled FP32 weights the apply kernels stream; the FP64 ones
+are formed from calcKernels at the first FP64 use.
 A ``Dcpse`` can carry a linear combination of signatures that share one basis
 (``signatures``/``coeffs``); the Laplacian uses that to assemble
 sum_d D^{2 e_d} as one operator.
