@@ -630,8 +630,8 @@ __device__ __forceinline__ void asm_weights(const AsmArgs& a, int64_t p, int64_t
     constexpr int LP = L1 > L2 ? L1 : L2;
     const AsmOp& o1 = a.op[0];
     const AsmOp& o2 = a.op[1];
-    const double eip1 = 1.0 / ipow_rt(eps, o1.border);
-    const double eip2 = L2 > 0 ? 1.0 / ipow_rt(eps, o2.border) : 0.0;
+    const double eip1 = ipow_rt(ieps, o1.border);
+    const double eip2 = L2 > 0 ? ipow_rt(ieps, o2.border) : 0.0;
     bool bad1 = false, bad2 = false;
     for (int j = lane; j < n; j += kWave) {
         double dv[3] = {0.0, 0.0, 0.0};
@@ -908,7 +908,7 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
                 l1 += sa;
             }
             l1 = wave_sum(l1);
-            const double eps = n > 0 ? (l1 / n) / a.h_over_eps : 0.0;
+            const double eps = n > 0 ? l1 / ((double)n * a.h_over_eps) : 0.0;
             const bool eps_bad = !(eps > 0.0) || !isfinite(eps);
 #pragma unroll
             for (int d = 0; d < DIM; ++d) dsl[(s * 3 + d) * kWave + lane] = cdv[d];
@@ -1123,7 +1123,7 @@ __global__ __launch_bounds__(64) void k_assemble_generic(const GenericArgs a) {
             for (int d = 0; d < DIM; ++d)
                 x[d] = min_image_r(xp[d] - a.pos_sup[d * a.n_sup + q], a.len[d], a.ilen[d], (a.periodic >> d) & 1u);
         };
-        // eps = mean |x_pq|_1 / h_over_eps (Vandermonde.hpp:63-81)
+        // eps = mean |x_pq|_1 / h_over_eps (Vandermonde.hpp:63-81), one division
         double l1 = 0.0;
         for (int j = lane; j < n; j += kWave) {
             double x[3];
@@ -1132,7 +1132,7 @@ __global__ __launch_bounds__(64) void k_assemble_generic(const GenericArgs a) {
             for (int d = 0; d < DIM; ++d) l1 += fabs(x[d]);
         }
         l1 = wave_sum(l1);
-        const double eps = n > 0 ? (l1 / n) / a.h_over_eps : 0.0;
+        const double eps = n > 0 ? l1 / ((double)n * a.h_over_eps) : 0.0;
         if (n < M || !(eps > 0.0) || !isfinite(eps)) {
             asm_fail(a.op, p, rs, n, n < M ? DCPSE_INFO_TOO_FEW : DCPSE_INFO_EPS_ZERO, lane);
             continue;
@@ -1218,7 +1218,7 @@ __global__ __launch_bounds__(64) void k_assemble_generic(const GenericArgs a) {
         vq[lane] = xa;
         wave_lds_sync();
         // kernel weights K_q = w_q sum_i a_i v_i(z_q) (Dcpse.hpp:798-809)
-        const double eip = 1.0 / ipow_rt(eps, a.op.border);
+        const double eip = ipow_rt(ieps, a.op.border);
         bool bad = false;
         for (int j = lane; j < n; j += kWave) {
             double x[3] = {0.0, 0.0, 0.0}, z[3] = {0.0, 0.0, 0.0}, z2 = 0.0;
