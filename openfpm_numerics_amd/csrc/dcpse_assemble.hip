@@ -982,10 +982,7 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
                     wave_lds_sync();
                     // a pass covers two neighbours per group; lanes of an
                     // invalid pair accumulate pair 0 and are not stored
-                    for (int j = 0; j < cnt; j += 2 * HALVES) {
-                        const double2* rw = reinterpret_cast<const double2*>(tab) + ((j >> 1) + half) * TROW;
-                        const double2 x = rw[0];
-                        const double2 yb = rw[pys], wc = rw[pzs];
+                    auto pass = [&](double2 x, double2 yb, double2 wc) {
                         double t0 = yb.x * wc.x, t1 = yb.y * wc.y;
 #pragma unroll
                         for (int k = 0; k < ST; ++k) {
@@ -993,6 +990,26 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
                             t0 *= x.x;
                             t1 *= x.y;
                         }
+                    };
+                    // four passes per iteration, all rows read before any is
+                    // accumulated (the later ones at immediate offsets)
+                    const double2* rw0 = reinterpret_cast<const double2*>(tab) + half * TROW;
+                    int j = 0;
+                    for (; j + 8 * HALVES <= cnt; j += 8 * HALVES) {
+                        const double2* ra = rw0 + (j >> 1) * TROW;
+                        double2 xs[4], ys[4], ws[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            xs[u] = ra[u * HALVES * TROW];
+                            ys[u] = ra[u * HALVES * TROW + pys];
+                            ws[u] = ra[u * HALVES * TROW + pzs];
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) pass(xs[u], ys[u], ws[u]);
+                    }
+                    for (; j < cnt; j += 2 * HALVES) {  // at most three passes left
+                        const double2* ra = rw0 + (j >> 1) * TROW;
+                        pass(ra[0], ra[pys], ra[pzs]);
                     }
                     wave_lds_sync();
                 }
