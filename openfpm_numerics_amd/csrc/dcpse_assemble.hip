@@ -936,7 +936,8 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
                     const int jl = lane & (CH - 1);
                     double dv[3];
 #pragma unroll
-                    for (int d = 0; d < DIM; ++d) dv[d] = __shfl(cdv[d], (c0 + jl) & (kWave - 1), kWave);
+                    for (int d = 0; d < DIM; ++d)
+                        dv[d] = CH == kWave ? cdv[d] : __shfl(cdv[d], (c0 + jl) & (kWave - 1), kWave);
                     // rows past cnt are written as zero neighbours (x = 0, w = 0):
                     // the passes then read every row of the chunk unconditionally
                     const bool vj = jl < cnt;
@@ -1007,16 +1008,29 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
 #pragma unroll
                         for (int u = 0; u < 4; ++u) pass(xs[u], ys[u], ws[u]);
                     }
-                    for (; j < cnt; j += 2 * HALVES) {  // at most three passes left
+                    // at most four passes left: two at a time while two are
+                    // needed (rows past cnt are zero neighbours of the chunk)
+                    while (j < cnt) {
                         const double2* ra = rw0 + (j >> 1) * TROW;
-                        pass(ra[0], ra[pys], ra[pzs]);
+                        if (j + 2 * HALVES < cnt) {
+                            const double2* rb = ra + HALVES * TROW;
+                            const double2 xa = ra[0], yba = ra[pys], wca = ra[pzs];
+                            const double2 xb = rb[0], ybb = rb[pys], wcb = rb[pzs];
+                            pass(xa, yba, wca);
+                            pass(xb, ybb, wcb);
+                            j += 4 * HALVES;
+                        } else {
+                            pass(ra[0], ra[pys], ra[pzs]);
+                            j += 2 * HALVES;
+                        }
                     }
                     wave_lds_sync();
                 }
 #pragma unroll
                 for (int off = kWave / 2; off >= LPN; off /= 2) {
 #pragma unroll
-                    for (int k = 0; k < ST; ++k) acc[k] += __shfl_down(acc[k], off, kWave);
+                    for (int k = 0; k < ST; ++k)
+                        acc[k] = off == kWave / 2 ? half_sum(acc[k]) : acc[k] + __shfl_down(acc[k], off, kWave);
                 }
                 if (pv && half == 0) {
                     double* ms = mom + s * MGP;

@@ -32,6 +32,17 @@ __device__ __forceinline__ double dpp_d(double v) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// v of lane l plus v of its partner l ^ 32 in the other half, in every lane
+// (lower half's value first): two v_permlane32_swap per 32-bit word, no LDS.
+__device__ __forceinline__ double half_sum(double v) {
+    const long long x = __double_as_longlong(v);
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)x, (unsigned)x, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(x >> 32), (unsigned)(x >> 32), false, false);
+    const double a = __longlong_as_double(((long long)hi[0] << 32) | (unsigned)lo[0]);
+    const double b = __longlong_as_double(((long long)hi[1] << 32) | (unsigned)lo[1]);
+    return a + b;
+}
+
 // Sum over the 64 lanes (all active), wave-uniform: within each 16-lane row
 // by DPP exchanges (lane ^ 1, lane ^ 2, mirrored half-rows, mirrored rows;
 // every lane of a row ends with the same bits), then the four row sums by
