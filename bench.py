@@ -521,8 +521,8 @@ def main():
         "fp64_error_vs_analytic": ana,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": ("k_apply_sell%s<2,%d,0%s>%s" % ("", (sell_pipe or (2, 2))[0],
-                                                          ",true" if step.layout == "sell_bundle16" else "",
+                     "kernel": ("k_apply_sell%s<2,%d,0,%s,false>%s" % ("", (sell_pipe or (2, 2))[0],
+                                                                "true" if step.layout == "sell_bundle16" else "false",
                                                           {"sell_bundle": " (interleaved bundle)",
                                                            "sell_bundle16": " (interleaved bundle, 16-bit columns)"}
                                                           .get(step.layout, "")))

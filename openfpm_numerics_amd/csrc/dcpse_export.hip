@@ -110,9 +110,10 @@ dcpse_status dcpse_scale_rows_f64(dcpse_ctx* ctx, int64_t n_rows, const int64_t*
         return fail(ctx, DCPSE_ERR_INVALID, "null pointer or negative size");
     if (n_rows == 0) return DCPSE_OK;
     DCPSE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
-    const int64_t rows_per_block = 256 / kExportLanes;
+    constexpr int G = 32;  // lanes per row: two rows per wavefront
+    const int64_t rows_per_block = 256 / G;
     const int64_t blocks = (n_rows + rows_per_block - 1) / rows_per_block;
-    k_scale_rows<kExportLanes><<<(unsigned)blocks, 256, 0, ctx->stream>>>(n_rows, row_ptr, K, row_scale, W);
+    k_scale_rows<G><<<(unsigned)blocks, 256, 0, ctx->stream>>>(n_rows, row_ptr, K, row_scale, W);
     DCPSE_LAUNCH_CHECK(ctx);
     return DCPSE_OK;
 }

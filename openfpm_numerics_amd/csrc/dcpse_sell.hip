@@ -298,45 +298,75 @@ __global__ void k_bundle16_exc(int64_t n_rows, const int64_t* __restrict__ row_p
 // The FP64 16-bit bundle: slot group g of slice s is one block of
 // 128 + 512 * n_ops words (the 64 lanes' four 16-bit offsets, then each
 // operator's four FP64 weights per lane, 32 bytes) at word offset
-// (128 + 512 n_ops) * slice_ptr[s] / 256 + g (128 + 512 n_ops). Thread
-// (lane l, t) of a 256-thread block writes lane l's slot groups t, t + 4, ...
-// of the slice directly from its CSR row (32 contiguous bytes per weight
-// stream; a once-per-assembly build).
+// (128 + 512 n_ops) * slice_ptr[s] / 256 + g (128 + 512 n_ops). One block per
+// slice, through LDS tiles of 32 slots x 64 lanes as in k_sell_fill_bundle16:
+// the column offsets, then each operator's weights in turn, read coalesced
+// along the rows and written 32 contiguous bytes per lane.
 __global__ __launch_bounds__(256) void k_sell_fill_bundle16_f64(
     int64_t n_rows, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ col, int n_ops,
     const double* W0, const double* W1, const double* W2, const double* W3, const int64_t* __restrict__ slice_ptr,
     const int32_t* __restrict__ sell_row, const int64_t* __restrict__ exc_ptr, int32_t* __restrict__ bundle,
     int32_t* __restrict__ sell_row16) {
+    __shared__ int32_t tc[kB16Chunk * kB16Pitch];  // column offsets
+    __shared__ double tw[kB16Chunk * kB16Pitch];   // one operator's weights
+    __shared__ int32_t srow[64];
+    __shared__ int64_t srs[64];
+    __shared__ int32_t slen[64];
     const int64_t n_slices = (n_rows + 63) / 64;
     const double* W[4] = {W0, W1, W2, W3};
     const int64_t blk = 128 + 512 * (int64_t)n_ops;
     int16_t* cols16 = reinterpret_cast<int16_t*>(bundle);
-    const int l = threadIdx.x & 63, gsub = threadIdx.x >> 6;
+    const int tid = threadIdx.x;
+    const int l = tid & 63, gsub = tid >> 6, e = tid & 31;
     for (int64_t s = blockIdx.x; s < n_slices; s += gridDim.x) {
-        const int64_t i = s * 64 + l;
-        const SliceLane r = slice_lane(row_ptr, sell_row, i);
-        if (gsub == 0)
+        __syncthreads();  // the previous slice's tiles and lane table are consumed
+        if (tid < 64) {
+            const int64_t i = s * 64 + tid;
+            const SliceLane r = slice_lane(row_ptr, sell_row, i);
+            srow[tid] = r.row;
+            srs[tid] = r.rs;
+            slen[tid] = r.len;
             sell_row16[i] = r.row < 0 ? r.row : (exc_ptr[i + 1] > exc_ptr[i] ? (r.row | kExcFlag) : r.row);
+        }
+        __syncthreads();
         const int64_t base = slice_ptr[s];
         const int ng = (int)((slice_ptr[s + 1] - base) >> 8);
-        for (int g = gsub; g < ng; g += 4) {
-            const int64_t gb = (base / 256 + g) * blk;  // word offset of the slot group's block
-            vs4a dv;
-#pragma unroll
-            for (int jj = 0; jj < 4; ++jj) {
-                const int j = 4 * g + jj;
-                int32_t d = 0;
-                if (j < r.len) {
-                    const int64_t dj = col_offset(col[r.rs + j], r.row, n_rows);
-                    d = far16(dj) ? 0 : (int32_t)dj;
+        for (int g0 = 0; g0 < ng; g0 += kB16Chunk / 4) {
+            const int j = 4 * g0 + e;
+            const int gn = min(kB16Chunk / 4, ng - g0);
+            for (int o = -1; o < n_ops; ++o) {  // -1: the column offsets
+                // load: wave w takes lanes 16 w .. 16 w + 15, two rows at a time
+                for (int it = 0; it < 8; ++it) {
+                    const int ll = 16 * gsub + 2 * it + ((tid >> 5) & 1);
+                    const int len = slen[ll];
+                    const int64_t rs = srs[ll];
+                    if (o < 0) {
+                        int32_t d = 0;
+                        if (j < len) {
+                            const int64_t dj = col_offset(col[rs + j], srow[ll], n_rows);
+                            d = far16(dj) ? 0 : (int32_t)dj;
+                        }
+                        tc[e * kB16Pitch + ll] = d;
+                    } else {
+                        tw[e * kB16Pitch + ll] = j < len ? W[o][rs + j] : 0.0;
+                    }
                 }
-                dv[jj] = (int16_t)d;
-            }
-            *reinterpret_cast<vs4a*>(cols16 + 2 * gb + 4 * l) = dv;
-            for (int o = 0; o < n_ops; ++o) {
-                double* wb = reinterpret_cast<double*>(bundle + gb + 128 + o * 512) + 4 * l;
+                __syncthreads();
+                // write: thread (lane l, gsub) writes the tile's groups gsub, gsub + 4
+                for (int gl = gsub; gl < gn; gl += 4) {
+                    const int64_t gb = (base / 256 + g0 + gl) * blk;  // word offset of the slot group's block
+                    if (o < 0) {
+                        vs4a dv;
 #pragma unroll
-                for (int jj = 0; jj < 4; ++jj) wb[jj] = 4 * g + jj < r.len ? W[o][r.rs + 4 * g + jj] : 0.0;
+                        for (int jj = 0; jj < 4; ++jj) dv[jj] = (int16_t)tc[(4 * gl + jj) * kB16Pitch + l];
+                        *reinterpret_cast<vs4a*>(cols16 + 2 * gb + 4 * l) = dv;
+                    } else {
+                        double2* wb = reinterpret_cast<double2*>(bundle + gb + 128 + o * 512) + 2 * l;
+                        wb[0] = make_double2(tw[(4 * gl) * kB16Pitch + l], tw[(4 * gl + 1) * kB16Pitch + l]);
+                        wb[1] = make_double2(tw[(4 * gl + 2) * kB16Pitch + l], tw[(4 * gl + 3) * kB16Pitch + l]);
+                    }
+                }
+                __syncthreads();
             }
         }
     }
