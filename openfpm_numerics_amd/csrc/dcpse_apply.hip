@@ -242,6 +242,134 @@ __global__ __launch_bounds__(kApplyMaxBlock) void k_apply_v4(const ApplyArgs<flo
     }
 }
 
+// Batches of FP32 fields on one set: the NF fields are first interleaved
+// particle-major (fil[q * NF + b]), so a neighbour's NF values are one
+// contiguous 4 NF-byte read instead of NF separate gathers, and the col/W
+// streams are read once for all of them. Each field's sum runs over the
+// nonzeros in the order of k_apply_v4 (same lanes, same chunks, same
+// reduction), so the results equal the field-major kernel's bit for bit.
+template <int NF>
+__global__ __launch_bounds__(256) void k_interleave(int64_t n, const float* __restrict__ f, int64_t ld,
+                                                    float* __restrict__ fil) {
+    __shared__ float t[NF][257];
+    const int64_t q0 = (int64_t)blockIdx.x * 256;
+    const int64_t q = q0 + threadIdx.x;
+#pragma unroll
+    for (int b = 0; b < NF; ++b) t[b][threadIdx.x] = q < n ? f[b * ld + q] : 0.0f;
+    __syncthreads();
+    for (int k = threadIdx.x; k < 256 * NF; k += 256) {
+        const int qq = k / NF, b = k % NF;
+        if (q0 + qq < n) fil[(q0 + qq) * NF + b] = t[b][qq];
+    }
+}
+
+typedef float vf4a __attribute__((ext_vector_type(4)));
+
+template <int G, int NCH, int NF, int MODE>
+__global__ __launch_bounds__(kApplyMaxBlock) void k_apply_il(const ApplyArgs<float> a) {
+    static_assert(NF % 4 == 0, "fields in groups of 4");
+    const int64_t hb = blockIdx.x;
+    const int64_t lb = (hb & 7) * a.per_xcd + (hb >> 3);
+    if (lb >= a.blocks_needed) return;
+    const int rows_per_block = blockDim.x / G;
+    const int gl = threadIdx.x % G;
+    const int64_t row = lb * rows_per_block + threadIdx.x / G;
+    if (row >= a.n_rows) return;
+    const int64_t rs = a.row_ptr[row];
+    const int64_t re = a.row_ptr[row + 1];
+    const int64_t nnz_total = a.row_ptr[a.n_rows];
+    auto gather = [&](int64_t q, float* v) {
+        const vf4a* p = reinterpret_cast<const vf4a*>(a.f + q * NF);
+#pragma unroll
+        for (int c = 0; c < NF / 4; ++c) {
+            const vf4a x = p[c];
+            v[4 * c] = x.x;
+            v[4 * c + 1] = x.y;
+            v[4 * c + 2] = x.z;
+            v[4 * c + 3] = x.w;
+        }
+    };
+    float fp[NF];
+    gather(row, fp);
+    double acc[NF];
+#pragma unroll
+    for (int b = 0; b < NF; ++b) acc[b] = 0.0;
+    int64_t jtail = rs + gl;
+    if (re > rs && re + 3 <= nnz_total) {
+        vi4u q[NCH];
+        vf4u w[NCH];
+        int64_t e0[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            e0[c] = rs + 4 * (gl + c * G);
+            const int64_t ee = e0[c] < re ? e0[c] : re - 1;
+            q[c] = DCPSE_STREAM_LOAD((const vi4u*)(a.col + ee));
+            w[c] = DCPSE_STREAM_LOAD((const vf4u*)(a.W[0] + ee));
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const bool v = e0[c] + i < re;
+                float fq[NF];
+                gather(q[c][i], fq);
+#pragma unroll
+                for (int b = 0; b < NF; ++b) {
+                    const double d = MODE == kDiff ? (double)fq[b] - (double)fp[b] : (double)fq[b];
+                    acc[b] = v ? fma((double)w[c][i], d, acc[b]) : acc[b];
+                }
+            }
+        jtail = rs + 4 * G * NCH + gl;
+    }
+    for (int64_t j = jtail; j < re; j += G) {
+        const int32_t qq = DCPSE_STREAM_LOAD(a.col + j);
+        const float ww = DCPSE_STREAM_LOAD(a.W[0] + j);
+        float fq[NF];
+        gather(qq, fq);
+#pragma unroll
+        for (int b = 0; b < NF; ++b) {
+            const double d = MODE == kDiff ? (double)fq[b] - (double)fp[b] : (double)fq[b];
+            acc[b] = fma((double)ww, d, acc[b]);
+        }
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int b = 0; b < NF; ++b) acc[b] += __shfl_xor(acc[b], off, kWave);
+    if (gl == 0) {
+#pragma unroll
+        for (int b = 0; b < NF; ++b) {
+            double val = acc[b];
+            if (MODE == kExplicitDiag) val = fma((double)a.diag[row], (double)fp[b], val);
+            float* dst = a.out[0] + b * a.ld_out + row;
+            val *= (double)a.alpha;
+            if (a.beta != 0.0f) val = fma((double)a.beta, (double)*dst, val);
+            *dst = (float)val;
+        }
+    }
+}
+
+template <int G, int NCH, int NF, int MODE>
+static void launch_apply_il(ApplyArgs<float> a, int block, hipStream_t s) {
+    const int rows_per_block = block / G;
+    a.blocks_needed = (a.n_rows + rows_per_block - 1) / rows_per_block;
+    a.per_xcd = (a.blocks_needed + 7) / 8;
+    k_apply_il<G, NCH, NF, MODE><<<(unsigned)(a.per_xcd * 8), block, 0, s>>>(a);
+}
+
+// The interleaved batch kernel for the shapes of the 16-byte-load kernel.
+template <int MODE, int NF>
+static bool dispatch_il(const ApplyArgs<float>& a, int G, int epl, int block, hipStream_t s) {
+    if (G == 4 && epl == 8) return launch_apply_il<4, 2, NF, MODE>(a, block, s), true;
+    if (G == 8 && epl == 4) return launch_apply_il<8, 1, NF, MODE>(a, block, s), true;
+    if (G == 8 && epl == 8) return launch_apply_il<8, 2, NF, MODE>(a, block, s), true;
+    if (G == 16 && epl == 4) return launch_apply_il<16, 1, NF, MODE>(a, block, s), true;
+    if (G == 16 && epl == 8) return launch_apply_il<16, 2, NF, MODE>(a, block, s), true;
+    if (G == 32 && epl == 4) return launch_apply_il<32, 1, NF, MODE>(a, block, s), true;
+    return false;
+}
+
 template <typename T, int G, int EPL, int NOPS, int NF, int MODE>
 static void launch_apply(ApplyArgs<T> a, int block, hipStream_t s) {
     const int rows_per_block = block / G;
@@ -384,6 +512,37 @@ static dcpse_status apply_impl(dcpse_ctx* ctx, int64_t n_rows, const int64_t* ro
         a.out[o] = nullptr;
     }
     int b0 = 0;
+    if constexpr (std::is_same<T, float>::value) {
+        // batches of 4 or more fields on one set (columns and rows index the
+        // same n_rows particles): interleaved 16 (8, 4) at a time through the
+        // workspace (the col/W streams read once per group)
+        if (nops == 1 && nfields >= 4 && sh.vec == 4 && (!f_self || (f_self == f && ld_self == ld_f))) {
+            Workspace ws;
+            dcpse_status st = reserve_workspace(ctx, (size_t)n_rows * 16 * sizeof(float) + 256, &ws);
+            if (st) return st;
+            float* fil = ws.take<float>((size_t)n_rows * 16);
+            if (!fil) return fail(ctx, DCPSE_ERR_INVALID, "apply workspace sizing error");
+            while (nfields - b0 >= 4) {
+                const int nf = nfields - b0 >= 16 ? 16 : (nfields - b0 >= 8 ? 8 : 4);
+                const unsigned ib = (unsigned)((n_rows + 255) / 256);
+                const float* fb = f + (int64_t)b0 * ld_f;
+                if (nf == 16) k_interleave<16><<<ib, 256, 0, ctx->stream>>>(n_rows, fb, ld_f, fil);
+                else if (nf == 8) k_interleave<8><<<ib, 256, 0, ctx->stream>>>(n_rows, fb, ld_f, fil);
+                else k_interleave<4><<<ib, 256, 0, ctx->stream>>>(n_rows, fb, ld_f, fil);
+                DCPSE_LAUNCH_CHECK(ctx);
+                a.f = fil;
+                a.f_self = fil;
+                a.out[0] = out[0] + (int64_t)b0 * ld_out;
+                const bool ok = nf == 16 ? dispatch_il<MODE, 16>(a, sh.G, sh.epl, block, ctx->stream)
+                                : nf == 8 ? dispatch_il<MODE, 8>(a, sh.G, sh.epl, block, ctx->stream)
+                                          : dispatch_il<MODE, 4>(a, sh.G, sh.epl, block, ctx->stream);
+                if (!ok) return fail(ctx, DCPSE_ERR_UNSUPPORTED, "no batched apply kernel for group=%d x %d", sh.G,
+                                     sh.epl);
+                DCPSE_LAUNCH_CHECK(ctx);
+                b0 += nf;
+            }
+        }
+    }
     while (b0 < nfields) {
         const int nf = field_chunk(nfields - b0, nops, sh.epl);
         a.f = f + (int64_t)b0 * ld_f;
