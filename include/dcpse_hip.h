@@ -208,7 +208,12 @@ dcpse_status dcpse_assemble_multi(dcpse_ctx* ctx, int32_t dim,
  *   out[o][b*ld_out + p] = alpha * sum_j W[o][j] * (f[b*ld_f + col_j] - f_self[b*ld_self + p])
  *                          + beta * out[o][b*ld_out + p]
  * for rows p < n_rows and fields b < n_fields. f_self == NULL means f (the
- * domain set is the support set). beta == 0 never reads out. */
+ * domain set is the support set). beta == 0 never reads out. FP32 batches of
+ * 4 or more fields of one operator on one set are copied particle-major into
+ * the context's workspace (16 x n_rows floats, grown on first use) and
+ * applied 16, 8 or 4 fields per pass; each field's result equals its
+ * single-field apply bit for bit. The same holds for
+ * dcpse_apply_transpose_f32. */
 dcpse_status dcpse_apply_f32(dcpse_ctx* ctx, int64_t n_rows, const int64_t* row_ptr, const int32_t* col,
                              int32_t n_ops, const float* const* W,
                              const float* f, int64_t ld_f, int32_t n_fields,
