@@ -358,12 +358,18 @@ static void launch_apply_il(ApplyArgs<float> a, int block, hipStream_t s) {
     k_apply_il<G, NCH, NF, MODE><<<(unsigned)(a.per_xcd * 8), block, 0, s>>>(a);
 }
 
-// The interleaved batch kernel for the shapes of the 16-byte-load kernel.
+// The interleaved batch kernel for the shapes of the 16-byte-load kernel
+// (il_shape: whether one exists).
+static bool il_shape(int G, int epl) {
+    return (G == 4 && epl == 8) || (G == 8 && (epl == 4 || epl == 8 || epl == 12)) ||
+           (G == 16 && (epl == 4 || epl == 8)) || (G == 32 && epl == 4);
+}
 template <int MODE, int NF>
 static bool dispatch_il(const ApplyArgs<float>& a, int G, int epl, int block, hipStream_t s) {
     if (G == 4 && epl == 8) return launch_apply_il<4, 2, NF, MODE>(a, block, s), true;
     if (G == 8 && epl == 4) return launch_apply_il<8, 1, NF, MODE>(a, block, s), true;
     if (G == 8 && epl == 8) return launch_apply_il<8, 2, NF, MODE>(a, block, s), true;
+    if (G == 8 && epl == 12) return launch_apply_il<8, 3, NF, MODE>(a, block, s), true;
     if (G == 16 && epl == 4) return launch_apply_il<16, 1, NF, MODE>(a, block, s), true;
     if (G == 16 && epl == 8) return launch_apply_il<16, 2, NF, MODE>(a, block, s), true;
     if (G == 32 && epl == 4) return launch_apply_il<32, 1, NF, MODE>(a, block, s), true;
@@ -516,7 +522,8 @@ static dcpse_status apply_impl(dcpse_ctx* ctx, int64_t n_rows, const int64_t* ro
         // batches of 4 or more fields on one set (columns and rows index the
         // same n_rows particles): interleaved 16 (8, 4) at a time through the
         // workspace (the col/W streams read once per group)
-        if (nops == 1 && nfields >= 4 && sh.vec == 4 && (!f_self || (f_self == f && ld_self == ld_f))) {
+        if (nops == 1 && nfields >= 4 && sh.vec == 4 && il_shape(sh.G, sh.epl) &&
+            (!f_self || (f_self == f && ld_self == ld_f))) {
             Workspace ws;
             dcpse_status st = reserve_workspace(ctx, (size_t)n_rows * 16 * sizeof(float) + 256, &ws);
             if (st) return st;

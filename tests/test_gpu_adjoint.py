@@ -147,16 +147,31 @@ def test_batched_fp32_applies_equal_single_field(nb, cuda):
     """Batches of 4 or more FP32 fields on one set run on a particle-major
     copy of the fields (16, 8 or 4 per pass; the rest field-major): every
     field's forward and transposed apply equals the single-field CSR
-    kernel's bit for bit (same lanes, same chunks, same order per field)."""
+    kernel's bit for bit (same lanes, same chunks, same order per field), for
+    the default launch shape and, with 6 fields, every 16-byte-load shape."""
     from openfpm_numerics_amd import _lib
+    from openfpm_numerics_amd.autotune import APPLY_SHAPES
 
     w, ps, vl, lap = _setup(cuda)
     d = lap.dcpse
     n = ps.size_local()
     f = torch.as_tensor(np.random.default_rng(nb).standard_normal((nb, n)), dtype=torch.float32, device=cuda)
+    ctx = _lib.context(cuda.index or 0)
+    shapes = [None] + ([s for s in APPLY_SHAPES if s[2] == 4] if nb == 6 else [])
+    try:
+        for shape in shapes:
+            if shape is not None:
+                ctx.check(ctx.lib.dcpse_set_apply_config(ctx.h, shape[0], shape[1], shape[2], 256), "config")
+            _check_batched(ctx, d, vl, f, n, nb, cuda)
+    finally:
+        ctx.check(ctx.lib.dcpse_set_apply_config(ctx.h, 0, 4, 4, 256), "config reset")
+
+
+def _check_batched(ctx, d, vl, f, n, nb, cuda):
+    from openfpm_numerics_amd import _lib
+
     fwd = d.computeDifferentialOperator(f)
     adj = d.applyTranspose(f)
-    ctx = _lib.context(cuda.index or 0)
     W = d.weights(torch.float32)
     rowT, colT, WT, diag = d.transposed(torch.float32)
     for b in range(nb):
