@@ -253,10 +253,10 @@ __device__ __forceinline__ void asm_factor_solve(const double* mom, int abase, d
     // the unscaled column (with the lane's right-hand side entry), each lane
     // reads its group's pivot and entries from it, and updates with
     // (a_ik / d_kk) a_jk = l_ik l_jk. Entries right of
-    // the diagonal are updated too and never read as part of L. The pivot
-    // check accumulates in a vector register (no per-step scalar masks); a
-    // failed factorisation writes zero weights whatever it computed.
-    double dfin = 0.0;  // NaN once a pivot is not positive and finite
+    // the diagonal are updated too and never read as part of L. The pivots
+    // are checked once after the loop, from each lane's own 1 / L_kk (no
+    // per-step masks); a failed factorisation writes zero weights whatever
+    // it computed.
     double dinv = 0.0;  // lane k: 1 / L_kk
     // two groups: the right-hand side rides along as column M of A, so the
     // factorisation also performs the forward substitution L y = b (lane k
@@ -281,10 +281,6 @@ __device__ __forceinline__ void asm_factor_solve(const double* mom, int abase, d
             for (int j = k + 1; j < M; ++j) cv[j] = col2[base + j].x;
         }
         const double inv = pivot_rsqrt(dkk);
-        // fma(inv, 0, s) is s exactly when inv is finite (an explicit fma:
-        // inv - inv may be contracted with the product that forms inv into
-        // that product's rounding error)
-        dfin = fma(inv, 0.0, dfin);
         dinv = (me == k) ? inv : dinv;
         // lane k's own entry becomes dkk * inv = L_kk
         ar[k] *= inv;
@@ -304,9 +300,19 @@ __device__ __forceinline__ void asm_factor_solve(const double* mom, int abase, d
 #pragma unroll
             for (int j = k + 1; j < M; ++j) ar[j] = fma(-t, cv[j], ar[j]);
         }
+        // the selects stay in their step (not sunk to the end of the loop
+        // with every step's values live)
+        asm volatile("" : "+v"(dinv), "+v"(yb));
         __builtin_amdgcn_sched_barrier(0);
     }
-    okl = dfin == 0.0;
+    // lane k holds 1 / sqrt(pivot k), finite exactly when the pivot is
+    // positive and finite; fma(v, 0, 0) is 0 exactly when v is finite (an
+    // explicit fma: v - v may be contracted with the product that forms v)
+    {
+        const uint64_t bad = __ballot(me < M && fma(dinv, 0.0, 0.0) != 0.0);
+        const uint64_t grp = LS == kWave ? ~0ull : ((1ull << LS) - 1ull) << base;
+        okl = (bad & grp) == 0ull;
+    }
     // keep the strictly lower part of L (the diagonal is in dinv): a lane's
     // partial sums then stop changing once its unknown is determined, so the
     // substitutions need no per-lane capture of their results
@@ -445,7 +451,7 @@ __device__ __forceinline__ void asm_factor_solve_fused(const double* mom, int ab
     double ar[MA];
 #pragma unroll
     for (int k = 0; k < MA; ++k) ar[k] = mom[abase + grid_index<GW>(F.ex[k][0], F.ex[k][1], F.ex[k][2])];
-    double dfin = 0.0, dfin_s = 0.0, dinv = 0.0, dc = 0.0;
+    double dinv = 0.0, dc = 0.0;
     double rb = b1v, yb = 0.0, rb2 = b2v, yb2 = 0.0;
 #pragma unroll
     for (int k = 0; k < M1; ++k) {
@@ -463,7 +469,6 @@ __device__ __forceinline__ void asm_factor_solve_fused(const double* mom, int ab
 #pragma unroll
         for (int j = k + 1; j < MA; ++j) cv[j] = col2[base + j].x;
         const double inv = pivot_rsqrt(dkk);
-        dfin = fma(inv, 0.0, dfin);  // s exactly when inv is finite
         dinv = (me == k) ? inv : dinv;
         ar[k] *= inv;
         const double t = ar[k] * inv;
@@ -475,15 +480,15 @@ __device__ __forceinline__ void asm_factor_solve_fused(const double* mom, int ab
             yb2 = (me == k) ? yk2 : yb2;
             rb2 = fma(-ar[k], yk2, rb2);
         }
+        // the selects stay in their step: sunk to the end of the loop they
+        // would keep every step's inverse and unknowns live (245 instead of
+        // 200 VGPRs)
+        asm volatile("" : "+v"(dinv), "+v"(yb), "+v"(yb2));
 #pragma unroll
         for (int j = k + 1; j < MA; ++j) ar[j] = fma(-t, cv[j], ar[j]);
-        if (k == S - 1) {
-            dfin_s = dfin;
-            if (E) dc = ar[MA - 1];  // the constant row's pivot (its lane only)
-        }
+        if (E && k == S - 1) dc = ar[MA - 1];  // the constant row's pivot (its lane only)
         __builtin_amdgcn_sched_barrier(0);
     }
-    ok1 = dfin == 0.0;
     // unknown k of each half-wave to all lanes of the half (v_readlane per half)
     auto bcast = [&](double v, int k) -> double {
         const double y0 = readlane_d(v, k), y1 = readlane_d(v, LS + k);
@@ -492,11 +497,16 @@ __device__ __forceinline__ void asm_factor_solve_fused(const double* mom, int ab
     double dinv2 = dinv;
     if constexpr (E > 0) {
         const double ic = pivot_rsqrt(bcast(dc, M1));
-        dfin_s = fma(ic, 0.0, dfin_s);
         dinv2 = (me == M1) ? ic : dinv;
         yb2 = (me == M1) ? rb2 * ic : yb2;
     }
-    ok2 = dfin_s == 0.0;
+    // pivot checks: lane k holds step k's 1 / sqrt(pivot) (and lane M0 the
+    // constant's), which is finite exactly when the pivot is positive and
+    // finite; fma(v, 0, 0) is 0 exactly when v is finite
+    const uint64_t bad1 = __ballot(me < M1 && fma(dinv, 0.0, 0.0) != 0.0);
+    const uint64_t bad2 = __ballot((me < S || (E > 0 && me == M1)) && fma(dinv2, 0.0, 0.0) != 0.0);
+    ok1 = ((bad1 >> base) & 0xffffffffull) == 0ull;
+    ok2 = ((bad2 >> base) & 0xffffffffull) == 0ull;
     // strictly lower L rows to LDS (row stride MS): the substitutions read
     // column `me`
 #pragma unroll
