@@ -89,15 +89,17 @@ dcpse_status dcpse_sync(dcpse_ctx* ctx);
 dcpse_status dcpse_set_apply_config(dcpse_ctx* ctx, int32_t group_lanes, int32_t elems_per_lane,
                                     int32_t vector_width, int32_t block_threads);
 /* Assembly launch shape: wavefronts per block (must be 1) and particles each
- * wavefront assembles in sequence (0 = automatic; values above 64 act as 64). */
+ * wavefront assembles in sequence (0 = automatic, 18; values above 64 act as
+ * 64, and on the static grid as 63). */
 dcpse_status dcpse_set_assembly_config(dcpse_ctx* ctx, int32_t waves_per_block, int32_t particles_per_wave);
 /* Assembly launch (kernels of src/DCPSE/Dcpse.hpp:690-809; replaces the
  * grid-stride assembleLocalMatrices_gpu<<<numSMs, 256>>> of
  * src/DCPSE/Dcpse.cuh:668-669). A wavefront assembles batches of
- * particles_per_batch particles (1..64; 0 = 16), one particle's ε and moments
- * at a time over all 64 lanes, staging `chunk` (32 or 64) neighbours per LDS
- * round; two particles share the factorisation when the basis fits half a
- * wavefront. persistent != 0: one launch of as many wavefronts as the GPU
+ * particles_per_batch particles (1..64, at most 63 on the static grid; 0 = 18),
+ * one particle's ε and moments at a time over all 64 lanes, staging `chunk`
+ * (32 or 64) neighbours per LDS round; two particles share the factorisation
+ * when the basis fits half a wavefront, three when the shared system of a
+ * nested pair fits 21 lanes. persistent != 0: one launch of as many wavefronts as the GPU
  * holds, each taking batches from a counter until none is left (0: one block
  * per batch). bucket_width > 0 (persistent grid only): batches ordered by row
  * length, longest first, in buckets of that width (particles keep their

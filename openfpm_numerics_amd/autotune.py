@@ -73,16 +73,17 @@ def tune_sell(fn, pipelines=SELL_PIPELINES, reps=10, device=0):
     return best, results
 
 
-#: the default assembly launch: static grid, 16 particles per batch, chunks of
-#: 64 neighbours, index order
-ASSEMBLY_DEFAULT = (0, 16, 64, 0)
+#: the default assembly launch: static grid, 18 particles per batch (the
+#: kernels factor two or three particles together), chunks of 64 neighbours,
+#: index order
+ASSEMBLY_DEFAULT = (0, 18, 64, 0)
 
 #: (persistent, particles per batch, chunk, bucket width) tried by tune_assembly
-ASSEMBLY_SPACE = tuple((pers, ppb, ch, bw) for pers in (0, 1) for ppb in (4, 8, 16) for ch in (32, 64)
+ASSEMBLY_SPACE = tuple((pers, ppb, ch, bw) for pers in (0, 1) for ppb in (6, 12, 18) for ch in (32, 64)
                        for bw in ((0, 16) if pers else (0,)))
 
 
-def set_assembly_launch(persistent=0, particles_per_batch=8, chunk=32, bucket_width=0, device=0):
+def set_assembly_launch(persistent=0, particles_per_batch=18, chunk=64, bucket_width=0, device=0):
     """Launch shape of the assembly kernels (dcpse_set_assembly_launch); the
     results do not depend on it."""
     ctx = _lib.context(device)

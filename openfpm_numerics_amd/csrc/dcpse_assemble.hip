@@ -421,8 +421,10 @@ struct FusePlan {
 template <int DIM, int L1, int A1, int L2, int A2>
 __constant__ FusePlan<DIM, L1, A1, L2, A2> kFuse = FusePlan<DIM, L1, A1, L2, A2>();
 
-// The factorisation and both solves of a nested pair (FusePlan) for two
-// particles at once, one per half-wave: lane slot * 32 + i holds solve row i
+// The factorisation and both solves of a nested pair (FusePlan) for NS
+// particles at once, one per lane group of LS = 64 / NS lanes (the last group
+// also takes the lanes past NS * LS, as rows beyond the system): lane
+// slot * LS + i holds solve row i
 // (M0 + E rows; abase: its moment row in its slot's grid; b1v, b2v: its
 // right-hand side entries of both operators). Right-looking Cholesky over the
 // M0 pivots of operator 0 as in asm_factor_solve, with both right-hand sides
@@ -433,16 +435,17 @@ __constant__ FusePlan<DIM, L1, A1, L2, A2> kFuse = FusePlan<DIM, L1, A1, L2, A2>
 // for both operators. Solutions go to sx (operator 0) and sx2 (operator 1)
 // in basis order; ok1 / ok2 are false when a pivot of that operator's system
 // is not positive and finite.
-template <int DIM, int L1, int A1, int L2, int A2, int GW>
+template <int DIM, int L1, int A1, int L2, int A2, int GW, int NS>
 __device__ __forceinline__ void asm_factor_solve_fused(const double* mom, int abase, double b1v, double b2v,
                                                        int p1, int q2, int refine, double* lmat, double2* lbuf,
                                                        double* sx, double* sx2, int lane, bool& ok1, bool& ok2) {
     using FP = FusePlan<DIM, L1, A1, L2, A2>;
     constexpr FP F{};
     constexpr int M1 = FP::M1, S = F.S, E = F.E, MA = M1 + E;
-    constexpr int LS = kWave / 2;
-    static_assert(MA <= LS, "nested pair larger than a half-wave");
-    const int slot = lane / LS;
+    constexpr int LS = kWave / NS;
+    static_assert(NS == 2 || NS == 3, "two or three particles per wave");
+    static_assert(MA <= LS, "nested pair larger than a lane group");
+    const int slot = min(lane / LS, NS - 1);
     const int base = slot * LS;
     int me = lane - base;
     asm volatile("" : "+v"(me));
@@ -489,10 +492,15 @@ __device__ __forceinline__ void asm_factor_solve_fused(const double* mom, int ab
         if (E && k == S - 1) dc = ar[MA - 1];  // the constant row's pivot (its lane only)
         __builtin_amdgcn_sched_barrier(0);
     }
-    // unknown k of each half-wave to all lanes of the half (v_readlane per half)
+    // unknown k of each group to all lanes of the group (v_readlane per group)
     auto bcast = [&](double v, int k) -> double {
         const double y0 = readlane_d(v, k), y1 = readlane_d(v, LS + k);
-        return slot ? y1 : y0;
+        if constexpr (NS == 2) {
+            return slot ? y1 : y0;
+        } else {
+            const double y2 = readlane_d(v, 2 * LS + k);
+            return slot == 2 ? y2 : (slot ? y1 : y0);
+        }
     };
     double dinv2 = dinv;
     if constexpr (E > 0) {
@@ -505,8 +513,9 @@ __device__ __forceinline__ void asm_factor_solve_fused(const double* mom, int ab
     // finite; fma(v, 0, 0) is 0 exactly when v is finite
     const uint64_t bad1 = __ballot(me < M1 && fma(dinv, 0.0, 0.0) != 0.0);
     const uint64_t bad2 = __ballot((me < S || (E > 0 && me == M1)) && fma(dinv2, 0.0, 0.0) != 0.0);
-    ok1 = ((bad1 >> base) & 0xffffffffull) == 0ull;
-    ok2 = ((bad2 >> base) & 0xffffffffull) == 0ull;
+    const uint64_t grp = ((1ull << LS) - 1ull) << base;
+    ok1 = (bad1 & grp) == 0ull;
+    ok2 = (bad2 & grp) == 0ull;
     // strictly lower L rows to LDS (row stride MS): the substitutions read
     // column `me`
 #pragma unroll
@@ -627,12 +636,13 @@ __device__ __forceinline__ double basis_sum(const double* xs, const double* px, 
 // lanes own neighbours, K_pq = w_q * sum_i a_i z_q^alpha_i with each
 // operator's solution a in xs1 / xs2 (wave-uniform), written with the
 // eps^-|beta| pre-scaled copies. The displacement, weight w and powers of a
-// neighbour are formed once for both operators; those of the first 64
+// neighbour are formed once for both operators; with DC those of the first 64
 // neighbours come from dsl[d * 64 + lane] and wsl[lane] (as the moment pass
-// computed them). Operator k is assembled when use_k (enough neighbours;
+// computed them), otherwise every neighbour's are formed again by the moment
+// pass's operations. Operator k is assembled when use_k (enough neighbours;
 // otherwise it is marked DCPSE_INFO_TOO_FEW with eps = 0) and solved when
 // ok_k (otherwise DCPSE_INFO_NOT_SPD and zero weights).
-template <int DIM, int L1, int A1, int L2, int A2, typename Disp>
+template <int DIM, int L1, int A1, int L2, int A2, bool DC, typename Disp>
 __device__ __forceinline__ void asm_weights(const AsmArgs& a, int64_t p, int64_t rs, int n, double eps,
                                             double ieps, bool use1, bool ok1, bool use2, bool ok2,
                                             const double* xs1, const double* xs2, const double* dsl,
@@ -645,9 +655,13 @@ __device__ __forceinline__ void asm_weights(const AsmArgs& a, int64_t p, int64_t
     bool bad1 = false, bad2 = false;
     for (int j = lane; j < n; j += kWave) {
         double dv[3] = {0.0, 0.0, 0.0};
+        if constexpr (DC) {
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) dv[d] = dsl[d * kWave + lane];
-        if (j >= kWave) disp(j, dv);
+            for (int d = 0; d < DIM; ++d) dv[d] = dsl[d * kWave + lane];
+            if (j >= kWave) disp(j, dv);
+        } else {
+            disp(j, dv);
+        }
         double z[3] = {0.0, 0.0, 0.0};
         double z2 = 0.0;
 #pragma unroll
@@ -655,7 +669,7 @@ __device__ __forceinline__ void asm_weights(const AsmArgs& a, int64_t p, int64_t
             z[d] = dv[d] * ieps;
             z2 = fma(z[d], z[d], z2);  // as the moment pass forms it
         }
-        const double w = j < kWave ? wsl[j] : exp(-z2);
+        const double w = DC && j < kWave ? wsl[j] : exp(-z2);
         double px[LP], py[LP], pz[LP];
         px[0] = py[0] = pz[0] = 1.0;
 #pragma unroll
@@ -725,8 +739,21 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
     constexpr int M2 = L2 > 0 ? BasisShape<DIM, (L2 > 0 ? L2 : 2), A2>::M : 0;
     constexpr int MX = M1 > M2 ? M1 : M2;
     constexpr int MN = L2 > 0 ? (M1 < M2 ? M1 : M2) : M1;
-    constexpr int SLOTS = MX <= kWave / 2 ? 2 : 1;  // particles factored together
+    // nested pairs (FusePlan) share one factorisation of M1 + E rows; three
+    // particles are factored together when that system has 16 to 21 rows (the
+    // factorisation then outweighs recomputing the epilogue's displacements
+    // and weights: Laplacian + Dx in 3D 5.77 -> 5.65 ms, while Dx + Dy, 9
+    // rows, went from 4.14 to 4.41 ms), two when a basis fits half a wave
+    using FP = FusePlan<DIM, L1, A1, L2, A2>;
+    constexpr FP FPL{};
+    constexpr bool FUSE = L2 > 0 && FPL.ok && M1 + FPL.E <= kWave / 2;
+    constexpr int MF = M1 + FPL.E, MSF = MF + (MF & 1);
+    constexpr int SLOTS = FUSE && MF >= 16 && MF <= kWave / 3 ? 3 : (MX <= kWave / 2 ? 2 : 1);  // particles factored together
     constexpr int LS = kWave / SLOTS;
+    // the first 64 displacements and weights of each slot's particle are kept
+    // in LDS for the epilogue, except with three slots (recomputed there, by
+    // the same operations, to fit the LDS of two blocks per SIMD)
+    constexpr bool DCACHE = SLOTS < 3;
     constexpr int MSX = MX + (MX & 1);
     constexpr int PW = P + 1;
     using Seg = SegTable<DIM, P>;
@@ -737,11 +764,6 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
     constexpr int ZW = DIM == 3 ? PW : 1;
     constexpr int TROW = 1 + PW + NX + ZW;
     static_assert(CH == 32 || CH == 64, "neighbour chunk of 32 or 64");
-    // nested pairs (FusePlan) share one factorisation of M1 + E rows
-    using FP = FusePlan<DIM, L1, A1, L2, A2>;
-    constexpr FP FPL{};
-    constexpr bool FUSE = L2 > 0 && SLOTS == 2 && FPL.ok && M1 + FPL.E <= kWave / 2;
-    constexpr int MF = M1 + FPL.E, MSF = MF + (MF & 1);
     constexpr int LT = FUSE && SLOTS * MF * MSF > SLOTS * MX * MSX ? SLOTS * MF * MSF : SLOTS * MX * MSX;
     constexpr int TAB = CH * TROW > LT ? CH * TROW : LT;
     // lanes per neighbour (the segments of the moment pass fit them); a pass
@@ -756,8 +778,8 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
     __shared__ double2 lbuf[2 * kWave];           // broadcast buffers of the factorisation
     __shared__ double sx[kWave];                  // solutions of operator 0, one entry per lane
     __shared__ double sx2[L2 > 0 ? kWave : 1];    // and of operator 1
-    __shared__ double dsl[SLOTS * 3 * kWave];     // first 64 displacements per slot
-    __shared__ double wsl[SLOTS * kWave];         // and their weights exp(-|z|^2)
+    __shared__ double dsl[DCACHE ? SLOTS * 3 * kWave : 1];  // first 64 displacements per slot
+    __shared__ double wsl[DCACHE ? SLOTS * kWave : 1];      // and their weights exp(-|z|^2)
     const int lane = threadIdx.x;
 
     // the moment segment this lane accumulates, and which neighbour of a pass
@@ -770,8 +792,9 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
     const int pa0 = kSeg<DIM, P>.a0[sgi], plen = kSeg<DIM, P>.len[sgi];
 
     // lane slot * LS + i: row i of each operator's system (right-hand side
-    // entry and offset of A row i in its slot's moment grid)
-    const int lslot = lane / LS, lrow = lane % LS;
+    // entry and offset of A row i in its slot's moment grid); with three
+    // slots lane 63 is row 21 of the last
+    const int lslot = min(lane / LS, SLOTS - 1), lrow = lane - lslot * LS;
     const double b1 = lrow < M1 ? a.op[0].b[lrow] : 0.0;
     const double b2 = L2 > 0 && lrow < M2 ? a.op[1].b[lrow] : 0.0;
     const int ab1 = lslot * MGP + mom_row_base<DIM, L1, A1, GW>(lrow);
@@ -920,8 +943,10 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
             l1 = wave_sum(l1);
             const double eps = n > 0 ? l1 / ((double)n * a.h_over_eps) : 0.0;
             const bool eps_bad = !(eps > 0.0) || !isfinite(eps);
+            if constexpr (DCACHE) {
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) dsl[(s * 3 + d) * kWave + lane] = cdv[d];
+                for (int d = 0; d < DIM; ++d) dsl[(s * 3 + d) * kWave + lane] = cdv[d];
+            }
             if (n < MN || eps_bad) {  // no operator of the pass can be assembled
                 const int st0 = n < M1 ? DCPSE_INFO_TOO_FEW : DCPSE_INFO_EPS_ZERO;
                 asm_fail(a.op[0], p, rs, n, st0, lane);
@@ -983,7 +1008,7 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
 #pragma unroll
                         for (int d = 0; d < DIM; ++d) z2 = fma(z[d], z[d], z2);  // explicit: no contraction choice
                         double pw = vj ? exp(-z2) : 0.0;
-                        if (c0 + jl < kWave) wsl[s * kWave + c0 + jl] = pw;
+                        if (DCACHE && c0 + jl < kWave) wsl[s * kWave + c0 + jl] = pw;
 #pragma unroll
                         for (int k = 0; k < ZW; ++k) {
                             row[2 * (1 + PW + NX + k)] = pw;
@@ -1083,8 +1108,8 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
             for (int s = 0; s < SLOTS; ++s) any |= go_s[s] && n_s[s] >= MN;
             if (any) {
                 bool o1, o2;
-                asm_factor_solve_fused<DIM, L1, A1, L2, A2, GW>(mom, fab, fb1, fb2, fp1, fq2, a.refine, tab, lbuf, sx,
-                                                                sx2, lane, o1, o2);
+                asm_factor_solve_fused<DIM, L1, A1, L2, A2, GW, SLOTS>(mom, fab, fb1, fb2, fp1, fq2, a.refine, tab,
+                                                                       lbuf, sx, sx2, lane, o1, o2);
                 okm1 = __ballot(o1);
                 okm2 = __ballot(o2);
             }
@@ -1098,9 +1123,10 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
             const bool use1 = n_s[s] >= M1, use2 = L2 > 0 && n_s[s] >= M2;
             const bool ok1 = use1 && ((okm1 >> (s * LS)) & 1ull), ok2 = use2 && ((okm2 >> (s * LS)) & 1ull);
             auto disp = [&](int j, double* dv) { load_disp(rs_s[s], n_s[s], xp_s[s], j, dv); };
-            asm_weights<DIM, L1, A1, L2, A2>(a, pid_s[s], rs_s[s], n_s[s], eps_s[s], ieps_s[s], use1, ok1, use2, ok2,
-                                             sx + s * LS, sx2 + s * LS, dsl + s * 3 * kWave, wsl + s * kWave, disp,
-                                             lane);
+            asm_weights<DIM, L1, A1, L2, A2, DCACHE>(a, pid_s[s], rs_s[s], n_s[s], eps_s[s], ieps_s[s], use1, ok1,
+                                                     use2, ok2, sx + s * LS, sx2 + s * LS,
+                                                     dsl + (DCACHE ? s * 3 * kWave : 0),
+                                                     wsl + (DCACHE ? s * kWave : 0), disp, lane);
         }
         wave_lds_sync();
     }
@@ -1564,7 +1590,9 @@ static dcpse_status assemble_ops(dcpse_ctx* ctx, int32_t dim, const double* pos_
     args.periodic = periodic_mask;
     args.h_over_eps = h_over_eps;
     args.refine = (flags & DCPSE_ASM_REFINE) ? 1 : 0;
-    args.ppb = std::min(64, std::max(1, ctx->asm_ppb));  // one lane per particle of a batch
+    // one lane per particle of a batch; the static grid also reads the row
+    // start of the particle after the last from lane ppb, so at most 63
+    args.ppb = std::min(ctx->asm_persistent ? 64 : 63, std::max(1, ctx->asm_ppb));
     if (n_dom == 0) return DCPSE_OK;
     args.n_batches = (n_dom + args.ppb - 1) / args.ppb;
     const int ch = ctx->asm_chunk;

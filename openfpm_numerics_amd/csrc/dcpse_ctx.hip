@@ -93,7 +93,7 @@ dcpse_status dcpse_set_assembly_config(dcpse_ctx* ctx, int32_t waves_per_block, 
         return fail(ctx, DCPSE_ERR_INVALID, "particles_per_wave must be in [0, 4096]");
     particles_per_wave = particles_per_wave > 64 ? 64 : particles_per_wave;
     ctx->asm_wpb = waves_per_block;
-    ctx->asm_ppb = particles_per_wave > 0 ? particles_per_wave : 16;
+    ctx->asm_ppb = particles_per_wave > 0 ? particles_per_wave : 18;
     return DCPSE_OK;
 }
 
@@ -106,7 +106,7 @@ dcpse_status dcpse_set_assembly_launch(dcpse_ctx* ctx, int32_t persistent, int32
     if (bucket_width < 0 || bucket_width > DCPSE_MAX_ROW)
         return fail(ctx, DCPSE_ERR_INVALID, "bucket_width must be in [0, DCPSE_MAX_ROW]");
     ctx->asm_persistent = persistent ? 1 : 0;
-    ctx->asm_ppb = particles_per_batch > 0 ? particles_per_batch : 16;
+    ctx->asm_ppb = particles_per_batch > 0 ? particles_per_batch : 18;
     ctx->asm_chunk = chunk;
     ctx->asm_bucket = bucket_width;
     return DCPSE_OK;

@@ -43,7 +43,7 @@ struct dcpse_ctx {
     int grp_epl = 8;
     int grp_vec = 4;
     int asm_wpb = 1;       // waves per block in the assembly kernel
-    int asm_ppb = 16;      // particles per batch (one wavefront walks a batch)
+    int asm_ppb = 18;      // particles per batch (one wavefront walks a batch)
     int asm_chunk = 64;    // neighbours staged per LDS round (32 or 64)
     int asm_persistent = 0;  // 1: persistent grid taking batches from a counter
     int asm_bucket = 0;    // > 0: batches bucketed by row length, this bucket width

@@ -53,7 +53,11 @@ def test_launch_shapes_agree_and_match_oracle(oracle, cuda):
     ref_W = [lap.dcpse.weights(torch.float32).clone(), dx.dcpse.weights(torch.float32).clone()]
     ref_eps = lap.dcpse.localEps.clone()
     try:
-        for cfg in ASSEMBLY_SPACE + ((1, 1, 32, 4), (1, 64, 64, 1), (0, 3, 32, 32)):
+        # beyond the tuning space: one particle per batch, the largest batches
+        # (a static batch of 64 runs as 63), and batch sizes that leave the
+        # last group of a three-particle factorisation part empty
+        extra = ((1, 1, 32, 4), (1, 64, 64, 1), (0, 64, 64, 0), (0, 16, 64, 0), (0, 3, 32, 32), (0, 1, 64, 0))
+        for cfg in ASSEMBLY_SPACE + extra:
             print(f"[uneven] launch {cfg}", flush=True)
             set_assembly_launch(*cfg)
             assemble_together([lap.dcpse, dx.dcpse], ps, ps)
