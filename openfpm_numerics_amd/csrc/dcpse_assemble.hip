@@ -217,6 +217,15 @@ __device__ __forceinline__ double pivot_rsqrt(double d) {
     return fma(r * e, fma(e, 0.375, 0.5), r);
 }
 
+// LDS doubles between the L factors of two slots: rows x
+// even row stride, padded so that slot s starts 8 s banks on (the slots' lanes
+// read the same row and column at once).
+__host__ __device__ constexpr int lm_slot_stride(int rows) {
+    int s = rows * (rows + (rows & 1));
+    while ((2 * s) % 32 != 8) ++s;
+    return s;
+}
+
 // Moment-system factorisation and solve of one operator for the particles of
 // SLOTS lane groups at once (LS = 64 / SLOTS lanes each, one particle per
 // group): lane slot * LS + i holds row i of its particle's A = (M(alpha_i +
@@ -234,7 +243,7 @@ __device__ __forceinline__ void asm_factor_solve(const double* mom, int abase, d
     constexpr int LS = kWave / SLOTS;
     static_assert(M <= LS, "basis larger than a lane group");
     constexpr BasisTables<DIM, L, AMIN> E{};
-    const int slot = lane / LS;
+    const int slot = min(lane / LS, SLOTS - 1);  // three groups: lane 63 is row 21 of the last
     const int base = slot * LS;
     // row index, opaque so the lane == k tests are not hoisted out of the
     // particle loop as spilled masks
@@ -321,7 +330,7 @@ __device__ __forceinline__ void asm_factor_solve(const double* mom, int abase, d
     // L rows to LDS (row stride MS, one block per group): the backward solve
     // reads column `me`
     constexpr int MS = M + (M & 1);
-    double* lm = lmat + slot * (M * MS);
+    double* lm = lmat + slot * lm_slot_stride(M);
     if (me < M) {
 #pragma unroll
         for (int k = 0; k < M; ++k) lm[me * MS + k] = ar[k];
@@ -332,9 +341,12 @@ __device__ __forceinline__ void asm_factor_solve(const double* mom, int abase, d
     auto bcast = [&](double v, int k) -> double {
         if constexpr (SLOTS == 1) {
             return readlane_d(v, k);
-        } else {
+        } else if constexpr (SLOTS == 2) {
             const double y0 = readlane_d(v, k), y1 = readlane_d(v, LS + k);
             return slot ? y1 : y0;
+        } else {
+            const double y0 = readlane_d(v, k), y1 = readlane_d(v, LS + k), y2 = readlane_d(v, 2 * LS + k);
+            return slot == 2 ? y2 : (slot ? y1 : y0);
         }
     };
     // forward (L y = r) and backward (L^T x = y) substitution: step k
@@ -521,7 +533,7 @@ __device__ __forceinline__ void asm_factor_solve_fused(const double* mom, int ab
 #pragma unroll
     for (int k = 0; k < MA; ++k) ar[k] = k < me ? ar[k] : 0.0;
     constexpr int MS = MA + (MA & 1);
-    double* lm = lmat + slot * (MA * MS);
+    double* lm = lmat + slot * lm_slot_stride(MA);
     if (me < MA) {
 #pragma unroll
         for (int k = 0; k < MA; ++k) lm[me * MS + k] = ar[k];
@@ -740,21 +752,22 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
     constexpr int MX = M1 > M2 ? M1 : M2;
     constexpr int MN = L2 > 0 ? (M1 < M2 ? M1 : M2) : M1;
     // nested pairs (FusePlan) share one factorisation of M1 + E rows; three
-    // particles are factored together when that system has 16 to 21 rows (the
-    // factorisation then outweighs recomputing the epilogue's displacements
-    // and weights: Laplacian + Dx in 3D 5.77 -> 5.65 ms, while Dx + Dy, 9
-    // rows, went from 4.14 to 4.41 ms), two when a basis fits half a wave
+    // particles are factored together when that system (or a single
+    // operator's) has 16 to 21 rows (the factorisation then outweighs
+    // recomputing the epilogue's displacements and weights: Laplacian + Dx in
+    // 3D 5.77 -> 5.65 ms, while Dx + Dy, 9 rows, went from 4.14 to 4.41 ms),
+    // two when a basis fits half a wave
     using FP = FusePlan<DIM, L1, A1, L2, A2>;
     constexpr FP FPL{};
     constexpr bool FUSE = L2 > 0 && FPL.ok && M1 + FPL.E <= kWave / 2;
-    constexpr int MF = M1 + FPL.E, MSF = MF + (MF & 1);
-    constexpr int SLOTS = FUSE && MF >= 16 && MF <= kWave / 3 ? 3 : (MX <= kWave / 2 ? 2 : 1);  // particles factored together
+    constexpr int MF = M1 + FPL.E;
+    constexpr bool S3 = FUSE ? (MF >= 16 && MF <= kWave / 3) : (L2 == 0 && M1 >= 16 && M1 <= kWave / 3);
+    constexpr int SLOTS = S3 ? 3 : (MX <= kWave / 2 ? 2 : 1);  // particles factored together
     constexpr int LS = kWave / SLOTS;
     // the first 64 displacements and weights of each slot's particle are kept
     // in LDS for the epilogue, except with three slots (recomputed there, by
     // the same operations, to fit the LDS of two blocks per SIMD)
     constexpr bool DCACHE = SLOTS < 3;
-    constexpr int MSX = MX + (MX & 1);
     constexpr int PW = P + 1;
     using Seg = SegTable<DIM, P>;
     constexpr int ST = Seg::T;    // x powers per lane segment
@@ -764,7 +777,8 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
     constexpr int ZW = DIM == 3 ? PW : 1;
     constexpr int TROW = 1 + PW + NX + ZW;
     static_assert(CH == 32 || CH == 64, "neighbour chunk of 32 or 64");
-    constexpr int LT = FUSE && SLOTS * MF * MSF > SLOTS * MX * MSX ? SLOTS * MF * MSF : SLOTS * MX * MSX;
+    constexpr int LT = FUSE && lm_slot_stride(MF) > lm_slot_stride(MX) ? SLOTS * lm_slot_stride(MF)
+                                                                        : SLOTS * lm_slot_stride(MX);
     constexpr int TAB = CH * TROW > LT ? CH * TROW : LT;
     // lanes per neighbour (the segments of the moment pass fit them); a pass
     // covers HALVES = 64 / LPN neighbours
