@@ -648,17 +648,20 @@ __device__ __forceinline__ double basis_sum(const double* xs, const double* px, 
 // lanes own neighbours, K_pq = w_q * sum_i a_i z_q^alpha_i with each
 // operator's solution a in xs1 / xs2 (wave-uniform), written with the
 // eps^-|beta| pre-scaled copies. The displacement, weight w and powers of a
-// neighbour are formed once for both operators; with DC those of the first 64
-// neighbours come from dsl[d * 64 + lane] and wsl[lane] (as the moment pass
-// computed them), otherwise every neighbour's are formed again by the moment
-// pass's operations. Operator k is assembled when use_k (enough neighbours;
+// neighbour are formed once for both operators; those of the first 64
+// neighbours are the moment pass's: with DC from dsl[d * 64 + lane] and
+// wsl[lane], otherwise the displacement from rdv (the lane's own neighbour,
+// kept in registers) and the weight from rw when WR (else formed again by
+// the moment pass's operations); later neighbours' are formed again. Operator
+// k is assembled when use_k (enough neighbours;
 // otherwise it is marked DCPSE_INFO_TOO_FEW with eps = 0) and solved when
 // ok_k (otherwise DCPSE_INFO_NOT_SPD and zero weights).
-template <int DIM, int L1, int A1, int L2, int A2, bool DC, typename Disp>
+template <int DIM, int L1, int A1, int L2, int A2, bool DC, bool WR, typename Disp>
 __device__ __forceinline__ void asm_weights(const AsmArgs& a, int64_t p, int64_t rs, int n, double eps,
                                             double ieps, bool use1, bool ok1, bool use2, bool ok2,
                                             const double* xs1, const double* xs2, const double* dsl,
-                                            const double* wsl, Disp disp, int lane) {
+                                            const double* wsl, const double* rdv, double rw, Disp disp,
+                                            int lane) {
     constexpr int LP = L1 > L2 ? L1 : L2;
     const AsmOp& o1 = a.op[0];
     const AsmOp& o2 = a.op[1];
@@ -667,13 +670,9 @@ __device__ __forceinline__ void asm_weights(const AsmArgs& a, int64_t p, int64_t
     bool bad1 = false, bad2 = false;
     for (int j = lane; j < n; j += kWave) {
         double dv[3] = {0.0, 0.0, 0.0};
-        if constexpr (DC) {
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) dv[d] = dsl[d * kWave + lane];
-            if (j >= kWave) disp(j, dv);
-        } else {
-            disp(j, dv);
-        }
+        for (int d = 0; d < DIM; ++d) dv[d] = DC ? dsl[d * kWave + lane] : rdv[d];
+        if (j >= kWave) disp(j, dv);
         double z[3] = {0.0, 0.0, 0.0};
         double z2 = 0.0;
 #pragma unroll
@@ -681,7 +680,7 @@ __device__ __forceinline__ void asm_weights(const AsmArgs& a, int64_t p, int64_t
             z[d] = dv[d] * ieps;
             z2 = fma(z[d], z[d], z2);  // as the moment pass forms it
         }
-        const double w = DC && j < kWave ? wsl[j] : exp(-z2);
+        const double w = j < kWave && (DC || WR) ? (DC ? wsl[j] : rw) : exp(-z2);
         double px[LP], py[LP], pz[LP];
         px[0] = py[0] = pz[0] = 1.0;
 #pragma unroll
@@ -914,6 +913,9 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
         double xp_s[SLOTS][3];
         double eps_s[SLOTS], ieps_s[SLOTS];
         bool go_s[SLOTS];
+        // three slots: the lane's own neighbour's displacement and weight, in
+        // registers instead of the LDS copies
+        double rdv_s[SLOTS][3], rw_s[SLOTS];
 #pragma unroll
         for (int s = 0; s < SLOTS; ++s) {
             rs_s[s] = rs;
@@ -922,6 +924,9 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
             for (int d = 0; d < 3; ++d) xp_s[s][d] = xp[d];
             eps_s[s] = ieps_s[s] = 0.0;
             go_s[s] = false;
+            rw_s[s] = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) rdv_s[s][d] = cdv[d];
             pid_s[s] = 0;
             const int ii = i + s;
             if (ii >= pcnt) {
@@ -1023,6 +1028,7 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
                         for (int d = 0; d < DIM; ++d) z2 = fma(z[d], z[d], z2);  // explicit: no contraction choice
                         double pw = vj ? exp(-z2) : 0.0;
                         if (DCACHE && c0 + jl < kWave) wsl[s * kWave + c0 + jl] = pw;
+                        if (!DCACHE && CH == kWave && c0 == 0) rw_s[s] = pw;
 #pragma unroll
                         for (int k = 0; k < ZW; ++k) {
                             row[2 * (1 + PW + NX + k)] = pw;
@@ -1137,10 +1143,10 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
             const bool use1 = n_s[s] >= M1, use2 = L2 > 0 && n_s[s] >= M2;
             const bool ok1 = use1 && ((okm1 >> (s * LS)) & 1ull), ok2 = use2 && ((okm2 >> (s * LS)) & 1ull);
             auto disp = [&](int j, double* dv) { load_disp(rs_s[s], n_s[s], xp_s[s], j, dv); };
-            asm_weights<DIM, L1, A1, L2, A2, DCACHE>(a, pid_s[s], rs_s[s], n_s[s], eps_s[s], ieps_s[s], use1, ok1,
-                                                     use2, ok2, sx + s * LS, sx2 + s * LS,
-                                                     dsl + (DCACHE ? s * 3 * kWave : 0),
-                                                     wsl + (DCACHE ? s * kWave : 0), disp, lane);
+            asm_weights<DIM, L1, A1, L2, A2, DCACHE, CH == kWave>(
+                a, pid_s[s], rs_s[s], n_s[s], eps_s[s], ieps_s[s], use1, ok1, use2, ok2, sx + s * LS,
+                sx2 + s * LS, dsl + (DCACHE ? s * 3 * kWave : 0), wsl + (DCACHE ? s * kWave : 0), rdv_s[s],
+                rw_s[s], disp, lane);
         }
         wave_lds_sync();
     }
