@@ -960,7 +960,11 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
                 l1 += sa;
             }
             l1 = wave_sum(l1);
-            const double eps = n > 0 ? l1 / ((double)n * a.h_over_eps) : 0.0;
+            // ε and 1/ε as two independent divisions (1/ε within an ulp of
+            // 1 / ε, and the same value wherever this kernel uses it)
+            const double dn = (double)n * a.h_over_eps;
+            const double eps = n > 0 ? l1 / dn : 0.0;
+            const double ieps_n = dn / l1;
             const bool eps_bad = !(eps > 0.0) || !isfinite(eps);
             if constexpr (DCACHE) {
 #pragma unroll
@@ -971,7 +975,7 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
                 asm_fail(a.op[0], p, rs, n, st0, lane);
                 if (L2 > 0) asm_fail(a.op[1], p, rs, n, n < M2 ? DCPSE_INFO_TOO_FEW : DCPSE_INFO_EPS_ZERO, lane);
             } else {
-                const double ieps = 1.0 / eps;
+                const double ieps = ieps_n;
                 eps_s[s] = eps;
                 ieps_s[s] = ieps;
                 go_s[s] = true;
