@@ -67,6 +67,35 @@ struct AsmArgs {
 template <int DIM, int L, int AMIN>
 __constant__ BasisTables<DIM, L, AMIN> kTab = BasisTables<DIM, L, AMIN>();
 
+// The output fields of operator k, read again from the kernel-argument
+// segment where they are needed (a scalar load through a pointer the
+// compiler cannot match with the kernel's first read): kept from the start,
+// they would hold scalar registers across the whole particle loop. The
+// kernel's one argument, the AsmArgs, starts the segment.
+struct AsmOut {
+    unsigned border;
+    double* eps;
+    double* eps_inv_pow;
+    double* K64;
+    float* W32;
+    double* W64;
+    int32_t* info;
+};
+__device__ __forceinline__ AsmOut kernarg_out(int k) {
+    typedef __attribute__((address_space(4))) const AsmArgs KArgs;
+    KArgs* kp = (KArgs*)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(kp));
+    AsmOut o;
+    o.border = kp->op[k].border;
+    o.eps = kp->op[k].eps;
+    o.eps_inv_pow = kp->op[k].eps_inv_pow;
+    o.K64 = kp->op[k].K64;
+    o.W32 = kp->op[k].W32;
+    o.W64 = kp->op[k].W64;
+    o.info = kp->op[k].info;
+    return o;
+}
+
 // The (y, z) exponent pairs (b, c) with b + c <= P (c = 0 in 2D): the moment
 // pass accumulates M(a, b, c) for every a <= P - b - c of each.
 template <int DIM, int P>
@@ -192,7 +221,8 @@ struct BasisTag {
 };
 
 // Writes the "no kernel" result of one operator for particle p.
-__device__ __forceinline__ void asm_fail(const AsmOp& op, int64_t p, int64_t rs, int n, int status, int lane) {
+template <class Op>
+__device__ __forceinline__ void asm_fail(const Op& op, int64_t p, int64_t rs, int n, int status, int lane) {
     for (int j = lane; j < n; j += kWave) {
         if (op.K64) op.K64[rs + j] = 0.0;
         if (op.W64) op.W64[rs + j] = 0.0;
@@ -663,8 +693,8 @@ __device__ __forceinline__ void asm_weights(const AsmArgs& a, int64_t p, int64_t
                                             const double* wsl, const double* rdv, double rw, Disp disp,
                                             int lane) {
     constexpr int LP = L1 > L2 ? L1 : L2;
-    const AsmOp& o1 = a.op[0];
-    const AsmOp& o2 = a.op[1];
+    const AsmOut o1 = kernarg_out(0);
+    const AsmOut o2 = L2 > 0 ? kernarg_out(1) : o1;
     const double eip1 = ipow_rt(ieps, o1.border);
     const double eip2 = L2 > 0 ? ipow_rt(ieps, o2.border) : 0.0;
     bool bad1 = false, bad2 = false;
@@ -707,7 +737,7 @@ __device__ __forceinline__ void asm_weights(const AsmArgs& a, int64_t p, int64_t
     const bool anybad1 = __ballot(bad1) != 0ull;
     const bool anybad2 = __ballot(bad2) != 0ull;
     if (lane == 0) {
-        auto mark = [&](const AsmOp& op, bool use, bool ok, bool anybad, double eip) {
+        auto mark = [&](const AsmOut& op, bool use, bool ok, bool anybad, double eip) {
             if (op.eps) op.eps[p] = use ? eps : 0.0;
             if (op.eps_inv_pow) op.eps_inv_pow[p] = use ? eip : 0.0;
             if (op.info)
@@ -972,8 +1002,8 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
             }
             if (n < MN || eps_bad) {  // no operator of the pass can be assembled
                 const int st0 = n < M1 ? DCPSE_INFO_TOO_FEW : DCPSE_INFO_EPS_ZERO;
-                asm_fail(a.op[0], p, rs, n, st0, lane);
-                if (L2 > 0) asm_fail(a.op[1], p, rs, n, n < M2 ? DCPSE_INFO_TOO_FEW : DCPSE_INFO_EPS_ZERO, lane);
+                asm_fail(kernarg_out(0), p, rs, n, st0, lane);
+                if (L2 > 0) asm_fail(kernarg_out(1), p, rs, n, n < M2 ? DCPSE_INFO_TOO_FEW : DCPSE_INFO_EPS_ZERO, lane);
             } else {
                 const double ieps = ieps_n;
                 eps_s[s] = eps;
