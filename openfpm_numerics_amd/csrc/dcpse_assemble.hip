@@ -782,10 +782,11 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
     constexpr int MN = L2 > 0 ? (M1 < M2 ? M1 : M2) : M1;
     // nested pairs (FusePlan) share one factorisation of M1 + E rows; three
     // particles are factored together when that system (or a single
-    // operator's) has 16 to 21 rows (the factorisation then outweighs
-    // recomputing the epilogue's displacements and weights: Laplacian + Dx in
-    // 3D 5.77 -> 5.65 ms, while Dx + Dy, 9 rows, went from 4.14 to 4.41 ms),
-    // two when a basis fits half a wave
+    // operator's) has 16 to 21 rows (the factorisation then outweighs the
+    // epilogue inputs the three-slot kernels give up keeping in LDS:
+    // Laplacian + Dx in 3D 5.77 -> 5.65 ms even with those recomputed, while
+    // Dx + Dy, 9 rows, went from 4.14 to 4.41 ms), two when a basis fits half
+    // a wave
     using FP = FusePlan<DIM, L1, A1, L2, A2>;
     constexpr FP FPL{};
     constexpr bool FUSE = L2 > 0 && FPL.ok && M1 + FPL.E <= kWave / 2;
@@ -794,8 +795,11 @@ __global__ __launch_bounds__(64, (asm_waves<DIM, L1, A1, L2, A2>())) void k_asse
     constexpr int SLOTS = S3 ? 3 : (MX <= kWave / 2 ? 2 : 1);  // particles factored together
     constexpr int LS = kWave / SLOTS;
     // the first 64 displacements and weights of each slot's particle are kept
-    // in LDS for the epilogue, except with three slots (recomputed there, by
-    // the same operations, to fit the LDS of two blocks per SIMD)
+    // in LDS for the epilogue, except with three slots (three copies would not
+    // fit the LDS of two blocks per SIMD): there each lane keeps its own
+    // neighbour's in registers (the weight with chunks of 64; with chunks of
+    // 32 another lane formed it, so the epilogue forms it again by the same
+    // operations)
     constexpr bool DCACHE = SLOTS < 3;
     constexpr int PW = P + 1;
     using Seg = SegTable<DIM, P>;
